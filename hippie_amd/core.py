@@ -1,0 +1,380 @@
+"""Host-side mirror of hippie's builder / BVH / tracer interface over libhx.
+
+Class and method names follow the reference (src/hippie/):
+  Scene                   rt/scene/Scene.h (vertices, triangles, scene box)
+  BVHBuilder, LBVHBuilder rt/bvh/BVHBuilder.h:16-68, rt/bvh/LBVHBuilder.cpp
+  HipBVH                  rt/bvh/HipBVH.h:19-106 (nodes, woop triangles,
+                          triangle indices, getCost, update)
+  HipTracer               rt/tracer/HipTracer.h:16-44 (setBVH, trace,
+                          traceStats, traceSort)
+  RayBuffer               rt/ray/RayBuffer.h:16-81 (rays, results, stats, indices)
+Invalid parameters raise ValueError instead of the reference's log-and-ignore.
+PyTorch only provides device memory and the stream; all work runs in libhx.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import HxBuildParams, HxBvhView, HxBvhDebugView, call, lib
+
+METHODS = {
+    "lbvh": _lib.HX_METHOD_LBVH,
+    "hlbvh": _lib.HX_METHOD_HLBVH,
+    "ploc": _lib.HX_METHOD_PLOC,
+    "atr": _lib.HX_METHOD_ATRBVH,
+    "insertion": _lib.HX_METHOD_INSERTION,
+}
+RAY_KEY_METHODS = {"twopoint": _lib.HX_RAYKEY_TWOPOINT, "aila": _lib.HX_RAYKEY_AILA}
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+class Context:
+    """One libhx context per device (HipModule::staticInit)."""
+
+    _default = {}
+
+    def __init__(self, device=0):
+        if not torch.cuda.is_available():
+            raise RuntimeError("hippie_amd needs a ROCm GPU: torch.cuda.is_available() is False")
+        self.device = int(device)
+        h = ctypes.c_void_p()
+        call("hx_context_create", self.device, ctypes.byref(h))
+        self.handle = h
+        with torch.cuda.device(self.device):
+            stream = torch.cuda.current_stream().cuda_stream
+        call("hx_context_set_stream", self.handle, ctypes.c_void_p(stream))
+
+    @classmethod
+    def default(cls, device=0):
+        if device not in cls._default:
+            cls._default[device] = Context(device)
+        return cls._default[device]
+
+    def set_stream(self, stream):
+        call("hx_context_set_stream", self.handle, ctypes.c_void_p(stream.cuda_stream))
+
+    def synchronize(self):
+        call("hx_context_synchronize", self.handle)
+
+    def set_profiling(self, on=True):
+        call("hx_context_set_profiling", self.handle, int(bool(on)))
+
+    def profile(self):
+        """{kernel name: (launches, total ms)} since the last reset."""
+        cnt = ctypes.c_int32()
+        call("hx_context_profile_count", self.handle, ctypes.byref(cnt))
+        out = {}
+        buf = ctypes.create_string_buffer(64)
+        for i in range(cnt.value):
+            n = ctypes.c_int64()
+            ms = ctypes.c_double()
+            call("hx_context_profile_entry", self.handle, i, buf, 64, ctypes.byref(n), ctypes.byref(ms))
+            out[buf.value.decode()] = (n.value, ms.value)
+        return out
+
+    def profile_reset(self):
+        call("hx_context_profile_reset", self.handle)
+
+    def memcpy(self, dst_ptr, src_ptr, nbytes):
+        call("hx_memcpy", self.handle, ctypes.c_void_p(dst_ptr), ctypes.c_void_p(src_ptr), int(nbytes))
+
+    def overflow_count(self):
+        v = ctypes.c_int64()
+        call("hx_trace_overflow_count", self.handle, ctypes.byref(v))
+        return v.value
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            try:
+                lib().hx_context_destroy(h)
+            except Exception:
+                pass
+            self.handle = None
+
+
+class Scene:
+    """Triangle geometry resident on the device (float32 Vec3f, int32 Vec3i)."""
+
+    def __init__(self, vertices, triangles, device=0):
+        dev = torch.device("cuda", device)
+        v = torch.as_tensor(vertices)
+        t = torch.as_tensor(triangles)
+        if v.ndim != 2 or v.shape[1] != 3 or t.ndim != 2 or t.shape[1] != 3:
+            raise ValueError("vertices must be (V,3) and triangles (F,3)")
+        self.vertices = v.to(device=dev, dtype=torch.float32).contiguous()
+        self.triangles = t.to(device=dev, dtype=torch.int32).contiguous()
+
+    @property
+    def n_triangles(self):
+        return int(self.triangles.shape[0])
+
+    @property
+    def n_vertices(self):
+        return int(self.vertices.shape[0])
+
+
+class BuildParams:
+    """hx_build_params with the reference's Bvh.* defaults (AppEnvironment.cpp:52-75)."""
+
+    ENV_KEYS = {
+        "Bvh.method": "method", "Bvh.maxLeafSize": "max_leaf_size",
+        "Bvh.adaptiveLeafSize": "adaptive_leaf_size", "Bvh.ct": "ct", "Bvh.ci": "ci",
+        "Bvh.presplitting": "presplitting", "Bvh.presplitterBeta": "presplitter_beta",
+        "Bvh.plocRadius": "ploc_radius", "Bvh.plocMaxRadius": "ploc_max_radius",
+        "Bvh.plocAdaptive": "ploc_adaptive", "Bvh.atrIterations": "atr_iterations",
+        "Bvh.hlbvhMortonCodeSAHBits": "hlbvh_sah_bits", "Bvh.insertionMod": "insertion_mod",
+    }
+    BITS_KEYS = {"lbvh": "Bvh.lbvhMortonCodeBits", "ploc": "Bvh.plocMortonCodeBits",
+                 "atr": "Bvh.atrMortonCodeBits", "hlbvh": "Bvh.hlbvhMortonCodeBits",
+                 "insertion": "Bvh.insertionMortonCodeBits"}
+
+    def __init__(self, **kw):
+        self.c = HxBuildParams()
+        call("hx_build_params_default", ctypes.byref(self.c))
+        for k, v in kw.items():
+            setattr(self, k, v)
+
+    def __setattr__(self, name, value):
+        if name == "c":
+            object.__setattr__(self, name, value)
+            return
+        if name == "method" and isinstance(value, str):
+            value = METHODS[value]
+        if name == "scene_box":
+            arr = (ctypes.c_float * 6)(*[float(x) for x in value])
+            self.c.scene_box = arr
+            self.c.use_scene_box = 1
+            return
+        if not hasattr(self.c, name):
+            raise AttributeError(f"unknown build parameter {name}")
+        setattr(self.c, name, value)
+
+    def __getattr__(self, name):
+        return getattr(object.__getattribute__(self, "c"), name)
+
+    @classmethod
+    def from_env(cls, env):
+        """Map Bvh.* key/value pairs of a reference .env block onto parameters."""
+        p = cls()
+        method = str(env.get("Bvh.method", "lbvh"))
+        p.method = method
+        if method in cls.BITS_KEYS and cls.BITS_KEYS[method] in env:
+            p.morton_bits = int(env[cls.BITS_KEYS[method]])
+        for key, field in cls.ENV_KEYS.items():
+            if key in env and key != "Bvh.method":
+                cur = getattr(p.c, field)
+                setattr(p, field, type(cur)(env[key]))
+        return p
+
+
+class HipBVH:
+    """A built Bin BVH owned by libhx (HipBVH)."""
+
+    def __init__(self, ctx, handle, scene, params):
+        self.ctx = ctx
+        self.handle = handle
+        self.scene = scene
+        self.params = params
+
+    def view(self):
+        v = HxBvhView()
+        call("hx_bvh_get_view", self.handle, ctypes.byref(v))
+        return v
+
+    @property
+    def n_interior(self):
+        return self.view().n_interior
+
+    @property
+    def n_leaves(self):
+        return self.view().n_leaves
+
+    @property
+    def n_nodes(self):
+        v = self.view()
+        return v.n_interior + v.n_leaves
+
+    def _copy(self, ptr, shape, dtype):
+        out = np.empty(shape, dtype)
+        if out.nbytes:
+            self.ctx.memcpy(out.ctypes.data, ptr, out.nbytes)
+        return out
+
+    def nodes_host(self):
+        v = self.view()
+        return self._copy(v.nodes, (v.n_interior + v.n_leaves, 16), np.int32)
+
+    def woop_host(self):
+        v = self.view()
+        return self._copy(v.woop, (v.n_references, 12), np.float32)
+
+    def tri_indices_host(self):
+        v = self.view()
+        return self._copy(v.tri_indices, (v.n_references,), np.int32)
+
+    def scene_box(self):
+        return np.array(self.view().scene_box[:], np.float32)
+
+    def debug_host(self):
+        """Intermediate arrays of the build (sorted keys, Karras arrays, boxes)."""
+        d = HxBvhDebugView()
+        call("hx_bvh_debug", self.handle, ctypes.byref(d))
+        n = self.view().n_references
+        N = max(2 * n - 1, 1)
+        kd = np.uint64 if d.keys_are_64bit else np.uint32
+        out = dict(
+            sorted_keys=self._copy(d.sorted_keys, (n,), kd).astype(np.uint64),
+            sorted_tri=self._copy(d.sorted_tri, (n,), np.int32),
+            left=self._copy(d.left, (N,), np.int32),
+            right=self._copy(d.right, (N,), np.int32),
+            parent=self._copy(d.parent, (N,), np.int32),
+            node_min=self._copy(d.node_min, (N, 4), np.float32),
+            node_max=self._copy(d.node_max, (N, 4), np.float32),
+            state=self._copy(d.state, (max(n - 1, 0),), np.int32),
+        )
+        return out
+
+    def get_cost(self):
+        c = ctypes.c_double()
+        call("hx_bvh_cost", self.handle, ctypes.byref(c))
+        return c.value
+
+    def update(self, scene=None):
+        """HipBVH::update -- refit + re-woopify with unchanged topology."""
+        s = scene or self.scene
+        call("hx_refit", self.ctx.handle, self.handle, _ptr(s.vertices), _ptr(s.triangles))
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            try:
+                lib().hx_bvh_destroy(h)
+            except Exception:
+                pass
+            self.handle = None
+
+
+class BVHBuilder:
+    """BVHBuilder::build / rebuild over libhx (rt/bvh/BVHBuilder.h:37-39)."""
+
+    method = "lbvh"
+
+    def __init__(self, ctx=None, params=None, **kw):
+        self.ctx = ctx or Context.default()
+        self.params = params or BuildParams(method=self.method)
+        for k, v in kw.items():
+            setattr(self.params, k, v)
+
+    def set_max_leaf_size(self, n):
+        if not (0 < n <= 1024):
+            raise ValueError("maximum leaf size must be in range (0,1024]")
+        self.params.max_leaf_size = n
+
+    def set_morton_code_bits(self, bits):
+        if not (6 <= bits <= 60):
+            raise ValueError("Morton code bits must be in range [6,60]")
+        self.params.morton_bits = bits
+
+    def build(self, scene):
+        h = ctypes.c_void_p()
+        call("hx_build", self.ctx.handle, ctypes.byref(self.params.c), _ptr(scene.vertices), scene.n_vertices,
+             _ptr(scene.triangles), scene.n_triangles, ctypes.byref(h))
+        return HipBVH(self.ctx, h, scene, self.params)
+
+    def rebuild(self, bvh, scene=None):
+        s = scene or bvh.scene
+        call("hx_rebuild", self.ctx.handle, bvh.handle, _ptr(s.vertices), _ptr(s.triangles))
+        bvh.scene = s
+
+
+class LBVHBuilder(BVHBuilder):
+    method = "lbvh"
+
+
+class RayBuffer:
+    """Rays (float4 origin/tmin, float4 direction/tmax), results int4, stats int2."""
+
+    def __init__(self, rays, device=0, closest_hit=True):
+        r = torch.as_tensor(rays)
+        if r.ndim != 2 or r.shape[1] != 8:
+            raise ValueError("rays must be (R, 8) float32")
+        self.rays = r.to(device=torch.device("cuda", device), dtype=torch.float32).contiguous()
+        n = self.rays.shape[0]
+        self.results = torch.zeros((n, 4), dtype=torch.int32, device=self.rays.device)
+        self.stats = None
+        self.indices = None
+        self.closest_hit = closest_hit
+
+    @property
+    def size(self):
+        return int(self.rays.shape[0])
+
+
+class HipTracer:
+    """HipTracer::setBVH / trace / traceStats / traceSort (rt/tracer/HipTracer.h:16-44)."""
+
+    def __init__(self, ctx=None, bvh=None):
+        self.ctx = ctx or Context.default()
+        self.bvh = bvh
+
+    def set_bvh(self, bvh):
+        self.bvh = bvh
+
+    def _need_bvh(self):
+        if self.bvh is None:
+            raise RuntimeError("HipTracer: no BVH set")
+
+    def trace(self, rays):
+        self._need_bvh()
+        call("hx_trace", self.ctx.handle, self.bvh.handle, _ptr(rays.rays), rays.size, int(not rays.closest_hit),
+             _ptr(rays.results), None)
+        return rays.results
+
+    def trace_stats(self, rays):
+        self._need_bvh()
+        if rays.stats is None:
+            rays.stats = torch.zeros((rays.size, 2), dtype=torch.int32, device=rays.rays.device)
+        call("hx_trace", self.ctx.handle, self.bvh.handle, _ptr(rays.rays), rays.size, int(not rays.closest_hit),
+             _ptr(rays.results), _ptr(rays.stats))
+        return rays.results, rays.stats
+
+    def trace_sort(self, rays, method="aila", bits=60, ray_length=0.25):
+        self._need_bvh()
+        if rays.indices is None:
+            rays.indices = torch.zeros(rays.size, dtype=torch.int32, device=rays.rays.device)
+        call("hx_trace_sorted", self.ctx.handle, self.bvh.handle, _ptr(rays.rays), rays.size,
+             int(not rays.closest_hit), RAY_KEY_METHODS[method], int(bits), float(ray_length),
+             _ptr(rays.results), None, _ptr(rays.indices))
+        return rays.results
+
+
+def scene_box(ctx, scene):
+    out = torch.zeros(6, dtype=torch.float32, device=scene.vertices.device)
+    call("hx_scene_box", ctx.handle, _ptr(scene.vertices), _ptr(scene.triangles), scene.n_triangles, _ptr(out))
+    return out
+
+
+def morton_codes(ctx, scene, box, bits):
+    box = torch.as_tensor(box, dtype=torch.float32).to(scene.vertices.device).contiguous()
+    keys = torch.zeros(scene.n_triangles, dtype=torch.int64, device=scene.vertices.device)
+    call("hx_morton_codes", ctx.handle, _ptr(scene.vertices), _ptr(scene.triangles), scene.n_triangles, _ptr(box),
+         int(bits), _ptr(keys))
+    return keys
+
+
+def sort_pairs(ctx, keys, values, end_bit):
+    """In-place stable radix sort of (int64-stored uint64 key, int32 value)."""
+    call("hx_sort_pairs_u64", ctx.handle, _ptr(keys), _ptr(values), int(keys.shape[0]), int(end_bit))
+
+
+def ray_keys(ctx, rays, method="aila", bits=60, ray_length=0.25):
+    keys = torch.zeros(rays.shape[0], dtype=torch.int64, device=rays.device)
+    call("hx_ray_keys", ctx.handle, _ptr(rays), int(rays.shape[0]), RAY_KEY_METHODS[method], int(bits),
+         float(ray_length), _ptr(keys))
+    return keys

@@ -1,0 +1,617 @@
+// hx_api.cpp -- the extern "C" boundary of libhx (declared in include/hx.h).
+//
+// Host orchestration of the build and trace stages.  Everything is enqueued
+// on the context's stream; nothing synchronises except the calls that return
+// host data (node counts, cost, profile timings, overflow count).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+#include "hx_internal.h"
+
+namespace hx {
+
+namespace {
+thread_local char g_error[512] = "";
+}
+
+void set_error(const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_error, sizeof(g_error), fmt, ap);
+    va_end(ap);
+}
+
+int hip_fail(hipError_t e, const char* what, const char* file, int line) {
+    set_error("%s failed: %s (%s:%d)", what, hipGetErrorString(e), file, line);
+    return e == hipErrorOutOfMemory ? HX_ERR_OUT_OF_MEMORY : HX_ERR_HIP;
+}
+
+int check_launch(const char* name) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error("kernel %s: launch failed: %s", name, hipGetErrorString(e));
+        return HX_ERR_HIP;
+    }
+    return HX_OK;
+}
+
+int DevBuf::reserve(size_t want) {
+    if (want <= bytes && ptr) return HX_OK;
+    release();
+    size_t alloc = std::max<size_t>(want, 256);
+    hipError_t e = hipMalloc(&ptr, alloc);
+    if (e != hipSuccess) {
+        ptr = nullptr;
+        bytes = 0;
+        (void)hipGetLastError();
+        set_error("hipMalloc(%zu bytes) failed: %s", alloc, hipGetErrorString(e));
+        return HX_ERR_OUT_OF_MEMORY;
+    }
+    bytes = alloc;
+    return HX_OK;
+}
+
+void DevBuf::release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+}
+
+int Profiler::begin(hipStream_t s, const char* name, hipEvent_t* a, hipEvent_t* b) {
+    int idx = -1;
+    for (size_t i = 0; i < stats.size(); ++i)
+        if (stats[i].name == name) { idx = (int)i; break; }
+    if (idx < 0) {
+        stats.push_back(KernelStat{name, 0, 0.0});
+        idx = (int)stats.size() - 1;
+    }
+    for (hipEvent_t* e : {a, b}) {
+        if (!pool.empty()) {
+            *e = pool.back();
+            pool.pop_back();
+        } else {
+            HX_HIP(hipEventCreate(e));
+        }
+    }
+    HX_HIP(hipEventRecord(*a, s));
+    pending.push_back(Pending{idx, *a, *b});
+    return HX_OK;
+}
+
+void Profiler::end(hipStream_t s, hipEvent_t b) { (void)hipEventRecord(b, s); }
+
+int Profiler::flush() {
+    for (const Pending& p : pending) {
+        HX_HIP(hipEventSynchronize(p.b));
+        float ms = 0.0f;
+        HX_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+        stats[p.stat].count += 1;
+        stats[p.stat].total_ms += ms;
+        pool.push_back(p.a);
+        pool.push_back(p.b);
+    }
+    pending.clear();
+    return HX_OK;
+}
+
+void Profiler::reset() {
+    (void)flush();
+    stats.clear();
+}
+
+Profiler::~Profiler() {
+    for (const Pending& p : pending) {
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+    }
+    for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+}
+
+}  // namespace hx
+
+using hx::Context;
+using hx::DevBuf;
+
+struct hx_context {
+    Context c;
+};
+
+struct hx_bvh {
+    hx_context* ctx = nullptr;
+    hx_build_params params{};
+    int64_t n_refs = 0;
+    const float* verts = nullptr;
+    const int32_t* tris = nullptr;
+    int64_t n_verts = 0;
+    bool keys64 = true;
+    int keys_index = 0;
+    // Output (owned).
+    DevBuf nodes, woop, tri_idx;
+    // Build workspace.
+    DevBuf keys[2], vals_tmp, left, right, parent, other, counters, nmin, nmax, cost, state, cover, leaf_num,
+        int_slot, counts, scene_box, cost_out;
+    bool counts_known = false;
+    int64_t n_interior = 0, n_leaves = 0;
+    float host_box[6] = {0, 0, 0, 0, 0, 0};
+    bool box_known = false;
+};
+
+namespace {
+
+#define HX_CHECK_ARG(cond, ...)                   \
+    do {                                          \
+        if (!(cond)) {                            \
+            hx::set_error(__VA_ARGS__);           \
+            return HX_ERR_INVALID_ARG;            \
+        }                                         \
+    } while (0)
+
+int validate_params(const hx_build_params* p) {
+    HX_CHECK_ARG(p != nullptr, "params is NULL");
+    HX_CHECK_ARG(p->morton_bits >= 6 && p->morton_bits <= 60, "morton_bits must be in [6,60] (got %d)",
+                 p->morton_bits);
+    HX_CHECK_ARG(p->max_leaf_size > 0 && p->max_leaf_size <= 1024, "max_leaf_size must be in (0,1024] (got %d)",
+                 p->max_leaf_size);
+    HX_CHECK_ARG(p->ct > 0.0f && p->ci > 0.0f, "ct and ci must be positive");
+    HX_CHECK_ARG(p->layout == HX_LAYOUT_BIN, "only the Bin layout is supported (got %d)", p->layout);
+    if (p->method != HX_METHOD_LBVH) {
+        hx::set_error("builder method %d is not available in this build", p->method);
+        return HX_ERR_UNSUPPORTED;
+    }
+    if (p->presplitting) {
+        hx::set_error("presplitting is not available in this build");
+        return HX_ERR_UNSUPPORTED;
+    }
+    return HX_OK;
+}
+
+int reserve_bvh(hx_bvh* b, int64_t n) {
+    const int64_t N = std::max<int64_t>(2 * n - 1, 2);
+    const size_t ks = b->keys64 ? 8 : 4;
+    HX_TRY(b->nodes.reserve(64 * (size_t)N));
+    HX_TRY(b->woop.reserve(48 * (size_t)n));
+    HX_TRY(b->tri_idx.reserve(4 * (size_t)n));
+    HX_TRY(b->keys[0].reserve(ks * (size_t)n));
+    HX_TRY(b->keys[1].reserve(ks * (size_t)n));
+    HX_TRY(b->vals_tmp.reserve(4 * (size_t)n));
+    HX_TRY(b->left.reserve(4 * (size_t)N));
+    HX_TRY(b->right.reserve(4 * (size_t)N));
+    HX_TRY(b->parent.reserve(4 * (size_t)N));
+    HX_TRY(b->other.reserve(4 * (size_t)n));
+    HX_TRY(b->counters.reserve(4 * (size_t)n));
+    HX_TRY(b->nmin.reserve(16 * (size_t)N));
+    HX_TRY(b->nmax.reserve(16 * (size_t)N));
+    HX_TRY(b->cost.reserve(4 * (size_t)N));
+    HX_TRY(b->state.reserve(4 * (size_t)n));
+    HX_TRY(b->cover.reserve(4 * (size_t)n));
+    HX_TRY(b->leaf_num.reserve(4 * (size_t)n));
+    HX_TRY(b->int_slot.reserve(4 * (size_t)n));
+    HX_TRY(b->counts.reserve(16));
+    HX_TRY(b->scene_box.reserve(32));
+    HX_TRY(b->cost_out.reserve(16));
+    return HX_OK;
+}
+
+int run_lbvh(hx_bvh* b) {
+    Context* c = &b->ctx->c;
+    const hx_build_params& p = b->params;
+    const int64_t n = b->n_refs;
+    HX_HIP(hipSetDevice(c->device));
+    float* box = b->scene_box.as<float>();
+    if (p.use_scene_box) {
+        HX_HIP(hipMemcpyAsync(box, p.scene_box, 6 * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    } else {
+        HX_TRY(hx::launch_scene_box(c, b->verts, b->tris, n, box));
+    }
+    b->box_known = false;
+    HX_TRY(hx::launch_morton(c, b->verts, b->tris, n, box, p.morton_bits, b->keys[0].ptr, b->keys64));
+    // The sorted triangle ids must end in tri_idx: pass p reads vals[p&1].
+    const int passes = (p.morton_bits + 7) / 8;
+    int32_t* vals[2];
+    vals[passes & 1] = b->tri_idx.as<int32_t>();
+    vals[(passes & 1) ^ 1] = b->vals_tmp.as<int32_t>();
+    void* keys[2] = {b->keys[0].ptr, b->keys[1].ptr};
+    int ri = 0;
+    HX_TRY(hx::launch_sort_pairs(c, keys, vals, n, p.morton_bits, b->keys64, true, &ri));
+    if (vals[ri] != b->tri_idx.as<int32_t>()) {
+        HX_HIP(hipMemcpyAsync(b->tri_idx.ptr, vals[ri], 4 * (size_t)n, hipMemcpyDeviceToDevice, c->stream));
+    }
+    b->keys_index = ri;
+    const int32_t* sorted_tri = b->tri_idx.as<int32_t>();
+    if (n == 1) {
+        HX_TRY(hx::launch_single_ref(c, b->verts, b->tris, sorted_tri, b->nmin.as<float4>(), b->nmax.as<float4>(),
+                                     b->woop.as<float4>(), b->nodes.as<int4>(), b->counts.as<int32_t>()));
+    } else {
+        HX_TRY(hx::launch_construct(c, keys[ri], b->keys64, n, b->left.as<int32_t>(), b->right.as<int32_t>(),
+                                    b->parent.as<int32_t>(), b->other.as<int32_t>(), b->counters.as<int32_t>(),
+                                    b->cover.as<int32_t>(), b->counts.as<int32_t>() + 2));
+        hx::ClimbArgs ca{};
+        ca.n = n;
+        ca.sorted_tri = sorted_tri;
+        ca.verts = b->verts;
+        ca.tris = b->tris;
+        ca.left = b->left.as<int32_t>();
+        ca.right = b->right.as<int32_t>();
+        ca.parent = b->parent.as<int32_t>();
+        ca.other = b->other.as<int32_t>();
+        ca.counters = b->counters.as<int32_t>();
+        ca.nmin = b->nmin.as<float4>();
+        ca.nmax = b->nmax.as<float4>();
+        ca.cost = b->cost.as<float>();
+        ca.state = b->state.as<int32_t>();
+        ca.end_at = b->cover.as<int32_t>();
+        ca.woop = b->woop.as<float4>();
+        ca.frontier = b->leaf_num.as<int32_t>();  // free until the collapse scans
+        ca.frontier_count = b->counts.as<int32_t>() + 2;
+        ca.adaptive = p.adaptive_leaf_size ? 1 : 0;
+        ca.max_leaf_size = p.max_leaf_size;
+        ca.ct = p.ct;
+        ca.ci = p.ci;
+        HX_TRY(hx::launch_climb(c, ca));
+        hx::EmitArgs ea{};
+        ea.n = n;
+        ea.left = ca.left;
+        ea.right = ca.right;
+        ea.parent = ca.parent;
+        ea.other = ca.other;
+        ea.state = ca.state;
+        ea.nmin = ca.nmin;
+        ea.nmax = ca.nmax;
+        ea.cover = b->cover.as<int32_t>();
+        ea.leaf_num = b->leaf_num.as<int32_t>();
+        ea.int_slot = b->int_slot.as<int32_t>();
+        ea.counts = b->counts.as<int32_t>();
+        ea.nodes = b->nodes.as<int4>();
+        HX_TRY(hx::launch_collapse_emit(c, ea));
+    }
+    b->counts_known = false;
+    return HX_OK;
+}
+
+int read_counts(hx_bvh* b) {
+    if (b->counts_known) return HX_OK;
+    Context* c = &b->ctx->c;
+    int32_t cnt[2] = {0, 0};
+    HX_HIP(hipMemcpyAsync(cnt, b->counts.ptr, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
+    HX_HIP(hipStreamSynchronize(c->stream));
+    b->n_interior = cnt[0];
+    b->n_leaves = cnt[1];
+    b->counts_known = true;
+    return HX_OK;
+}
+
+int read_box(hx_bvh* b) {
+    if (b->box_known) return HX_OK;
+    Context* c = &b->ctx->c;
+    HX_HIP(hipMemcpyAsync(b->host_box, b->scene_box.ptr, sizeof(b->host_box), hipMemcpyDeviceToHost, c->stream));
+    HX_HIP(hipStreamSynchronize(c->stream));
+    b->box_known = true;
+    return HX_OK;
+}
+
+int trace_common(hx_context* ctx, hx_bvh* bvh, const float* d_rays, int64_t n_rays, int32_t any_hit,
+                 int32_t* d_results, int32_t* d_stats, const int32_t* d_order) {
+    HX_CHECK_ARG(ctx && bvh, "context or bvh is NULL");
+    HX_CHECK_ARG(n_rays >= 0, "n_rays must be >= 0");
+    if (n_rays == 0) return HX_OK;
+    HX_CHECK_ARG(d_rays && d_results, "rays and results must be device pointers");
+    Context* c = &ctx->c;
+    HX_TRY(c->control.reserve(64));
+    hx::TraceArgs a{};
+    a.n_rays = n_rays;
+    a.any_hit = any_hit ? 1 : 0;
+    a.rays = reinterpret_cast<const float4*>(d_rays);
+    a.results = reinterpret_cast<int4*>(d_results);
+    a.nodes = bvh->nodes.as<int4>();
+    a.woop = bvh->woop.as<float4>();
+    a.tri_idx = bvh->tri_idx.as<int32_t>();
+    a.stats = reinterpret_cast<int2*>(d_stats);
+    a.order = d_order;
+    a.control = c->control.as<int32_t>();
+    return hx::launch_trace(c, a);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* hx_status_string(int status) {
+    switch (status) {
+        case HX_OK: return "ok";
+        case HX_ERR_INVALID_ARG: return "invalid argument";
+        case HX_ERR_HIP: return "HIP runtime error";
+        case HX_ERR_OUT_OF_MEMORY: return "out of device memory";
+        case HX_ERR_UNSUPPORTED: return "unsupported";
+        case HX_ERR_STACK_OVERFLOW: return "traversal stack overflow";
+        default: return "unknown status";
+    }
+}
+
+const char* hx_last_error(void) { return hx::g_error; }
+
+int hx_context_create(int device, hx_context** out) {
+    HX_CHECK_ARG(out != nullptr, "out is NULL");
+    *out = nullptr;
+    int count = 0;
+    HX_HIP(hipGetDeviceCount(&count));
+    HX_CHECK_ARG(device >= 0 && device < count, "device %d out of range (%d devices)", device, count);
+    HX_HIP(hipSetDevice(device));
+    hx_context* ctx = new (std::nothrow) hx_context();
+    if (!ctx) return HX_ERR_OUT_OF_MEMORY;
+    ctx->c.device = device;
+    hipDeviceProp_t prop;
+    hipError_t e = hipGetDeviceProperties(&prop, device);
+    if (e != hipSuccess) {
+        delete ctx;
+        return hx::hip_fail(e, "hipGetDeviceProperties", __FILE__, __LINE__);
+    }
+    ctx->c.cu_count = prop.multiProcessorCount;
+    *out = ctx;
+    return HX_OK;
+}
+
+int hx_context_destroy(hx_context* ctx) {
+    if (!ctx) return HX_OK;
+    (void)hipSetDevice(ctx->c.device);
+    (void)hipStreamSynchronize(ctx->c.stream);
+    delete ctx;
+    return HX_OK;
+}
+
+int hx_context_set_stream(hx_context* ctx, void* stream) {
+    HX_CHECK_ARG(ctx != nullptr, "context is NULL");
+    ctx->c.stream = static_cast<hipStream_t>(stream);
+    return HX_OK;
+}
+
+int hx_context_synchronize(hx_context* ctx) {
+    HX_CHECK_ARG(ctx != nullptr, "context is NULL");
+    HX_HIP(hipStreamSynchronize(ctx->c.stream));
+    return HX_OK;
+}
+
+int hx_context_set_profiling(hx_context* ctx, int enabled) {
+    HX_CHECK_ARG(ctx != nullptr, "context is NULL");
+    if (!enabled) HX_TRY(ctx->c.prof.flush());
+    ctx->c.prof.enabled = enabled != 0;
+    return HX_OK;
+}
+
+int hx_context_profile_count(hx_context* ctx, int32_t* count) {
+    HX_CHECK_ARG(ctx && count, "NULL argument");
+    HX_TRY(ctx->c.prof.flush());
+    *count = (int32_t)ctx->c.prof.stats.size();
+    return HX_OK;
+}
+
+int hx_context_profile_entry(hx_context* ctx, int32_t index, char* name, int32_t name_len, int64_t* launches,
+                             double* total_ms) {
+    HX_CHECK_ARG(ctx != nullptr, "context is NULL");
+    HX_TRY(ctx->c.prof.flush());
+    HX_CHECK_ARG(index >= 0 && index < (int32_t)ctx->c.prof.stats.size(), "profile index out of range");
+    const hx::KernelStat& s = ctx->c.prof.stats[index];
+    if (name && name_len > 0) {
+        std::strncpy(name, s.name.c_str(), (size_t)name_len - 1);
+        name[name_len - 1] = '\0';
+    }
+    if (launches) *launches = s.count;
+    if (total_ms) *total_ms = s.total_ms;
+    return HX_OK;
+}
+
+int hx_context_profile_reset(hx_context* ctx) {
+    HX_CHECK_ARG(ctx != nullptr, "context is NULL");
+    ctx->c.prof.reset();
+    return HX_OK;
+}
+
+int hx_build_params_default(hx_build_params* p) {
+    HX_CHECK_ARG(p != nullptr, "params is NULL");
+    std::memset(p, 0, sizeof(*p));
+    p->method = HX_METHOD_LBVH;
+    p->morton_bits = 60;
+    p->max_leaf_size = 8;
+    p->adaptive_leaf_size = 1;
+    p->ct = 3.0f;
+    p->ci = 2.0f;
+    p->presplitting = 0;
+    p->presplitter_beta = 0.1f;
+    p->ploc_radius = 1;
+    p->ploc_max_radius = 512;
+    p->ploc_adaptive = 0;
+    p->atr_iterations = 2;
+    p->atr_treelet_size = 20;
+    p->hlbvh_sah_bits = 15;
+    p->insertion_mod = 1;
+    p->layout = HX_LAYOUT_BIN;
+    return HX_OK;
+}
+
+int hx_build(hx_context* ctx, const hx_build_params* params, const float* d_vertices, int64_t n_vertices,
+             const int32_t* d_triangles, int64_t n_triangles, hx_bvh** out) {
+    HX_CHECK_ARG(ctx && out, "context or out is NULL");
+    *out = nullptr;
+    HX_TRY(validate_params(params));
+    HX_CHECK_ARG(n_triangles > 0, "number of triangles must be positive (got %lld)", (long long)n_triangles);
+    HX_CHECK_ARG(n_triangles < ((int64_t)1 << 30), "number of triangles must be < 2^30");
+    HX_CHECK_ARG(d_vertices && d_triangles && n_vertices > 0, "geometry pointers/counts invalid");
+    hx_bvh* b = new (std::nothrow) hx_bvh();
+    if (!b) return HX_ERR_OUT_OF_MEMORY;
+    b->ctx = ctx;
+    b->params = *params;
+    b->n_refs = n_triangles;
+    b->verts = d_vertices;
+    b->tris = d_triangles;
+    b->n_verts = n_vertices;
+    b->keys64 = params->morton_bits > 32;
+    int st = reserve_bvh(b, n_triangles);
+    if (st == HX_OK) st = run_lbvh(b);
+    if (st != HX_OK) {
+        delete b;
+        return st;
+    }
+    *out = b;
+    return HX_OK;
+}
+
+int hx_rebuild(hx_context* ctx, hx_bvh* bvh, const float* d_vertices, const int32_t* d_triangles) {
+    HX_CHECK_ARG(ctx && bvh, "context or bvh is NULL");
+    HX_CHECK_ARG(d_vertices && d_triangles, "geometry pointers invalid");
+    bvh->ctx = ctx;
+    bvh->verts = d_vertices;
+    bvh->tris = d_triangles;
+    return run_lbvh(bvh);
+}
+
+int hx_refit(hx_context* ctx, hx_bvh* bvh, const float* d_vertices, const int32_t* d_triangles) {
+    HX_CHECK_ARG(ctx && bvh && d_vertices && d_triangles, "NULL argument");
+    hx::set_error("hx_refit is not available in this build");
+    return HX_ERR_UNSUPPORTED;
+}
+
+int hx_bvh_destroy(hx_bvh* bvh) {
+    if (!bvh) return HX_OK;
+    if (bvh->ctx) {
+        (void)hipSetDevice(bvh->ctx->c.device);
+        (void)hipStreamSynchronize(bvh->ctx->c.stream);
+    }
+    delete bvh;
+    return HX_OK;
+}
+
+int hx_bvh_get_view(hx_bvh* bvh, hx_bvh_view* out) {
+    HX_CHECK_ARG(bvh && out, "NULL argument");
+    HX_TRY(read_counts(bvh));
+    HX_TRY(read_box(bvh));
+    out->nodes = bvh->nodes.ptr;
+    out->woop = bvh->woop.ptr;
+    out->tri_indices = bvh->tri_idx.as<int32_t>();
+    out->n_interior = bvh->n_interior;
+    out->n_leaves = bvh->n_leaves;
+    out->n_references = bvh->n_refs;
+    std::memcpy(out->scene_box, bvh->host_box, sizeof(out->scene_box));
+    return HX_OK;
+}
+
+int hx_bvh_debug(hx_bvh* bvh, hx_bvh_debug_view* out) {
+    HX_CHECK_ARG(bvh && out, "NULL argument");
+    out->sorted_keys = bvh->keys[bvh->keys_index].ptr;
+    out->keys_are_64bit = bvh->keys64 ? 1 : 0;
+    out->sorted_tri = bvh->tri_idx.as<int32_t>();
+    out->left = bvh->left.as<int32_t>();
+    out->right = bvh->right.as<int32_t>();
+    out->parent = bvh->parent.as<int32_t>();
+    out->node_min = bvh->nmin.as<float>();
+    out->node_max = bvh->nmax.as<float>();
+    out->state = bvh->state.as<int32_t>();
+    return HX_OK;
+}
+
+int hx_bvh_cost(hx_bvh* bvh, double* out) {
+    HX_CHECK_ARG(bvh && out, "NULL argument");
+    Context* c = &bvh->ctx->c;
+    const int64_t max_nodes = std::max<int64_t>(2 * bvh->n_refs - 1, 2);
+    HX_TRY(hx::launch_cost(c, bvh->nodes.as<int4>(), bvh->counts.as<int32_t>(), max_nodes,
+                           bvh->scene_box.as<float>(), bvh->params.ct, bvh->params.ci, bvh->cost_out.as<double>()));
+    HX_HIP(hipMemcpyAsync(out, bvh->cost_out.ptr, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HX_HIP(hipStreamSynchronize(c->stream));
+    return HX_OK;
+}
+
+int hx_scene_box(hx_context* ctx, const float* d_vertices, const int32_t* d_triangles, int64_t n, float* d_box6) {
+    HX_CHECK_ARG(ctx && d_vertices && d_triangles && d_box6 && n > 0, "invalid argument");
+    return hx::launch_scene_box(&ctx->c, d_vertices, d_triangles, n, d_box6);
+}
+
+int hx_morton_codes(hx_context* ctx, const float* d_vertices, const int32_t* d_triangles, int64_t n,
+                    const float* d_box6, int32_t bits, uint64_t* d_keys) {
+    HX_CHECK_ARG(ctx && d_vertices && d_triangles && d_box6 && d_keys && n > 0, "invalid argument");
+    HX_CHECK_ARG(bits >= 6 && bits <= 60, "bits must be in [6,60]");
+    return hx::launch_morton(&ctx->c, d_vertices, d_triangles, n, d_box6, bits, d_keys, true);
+}
+
+int hx_sort_pairs_u64(hx_context* ctx, uint64_t* d_keys, int32_t* d_values, int64_t n, int32_t end_bit) {
+    HX_CHECK_ARG(ctx && d_keys && d_values && n >= 0, "invalid argument");
+    HX_CHECK_ARG(end_bit >= 0 && end_bit <= 64, "end_bit must be in [0,64]");
+    if (n <= 1) return HX_OK;
+    Context* c = &ctx->c;
+    HX_TRY(c->tmp_keys.reserve(8 * (size_t)n));
+    HX_TRY(c->tmp_vals.reserve(4 * (size_t)n));
+    void* keys[2] = {d_keys, c->tmp_keys.ptr};
+    int32_t* vals[2] = {d_values, c->tmp_vals.as<int32_t>()};
+    int ri = 0;
+    HX_TRY(hx::launch_sort_pairs(c, keys, vals, n, end_bit, true, false, &ri));
+    if (ri == 1) {
+        HX_HIP(hipMemcpyAsync(d_keys, keys[1], 8 * (size_t)n, hipMemcpyDeviceToDevice, c->stream));
+        HX_HIP(hipMemcpyAsync(d_values, vals[1], 4 * (size_t)n, hipMemcpyDeviceToDevice, c->stream));
+    }
+    return HX_OK;
+}
+
+int hx_trace(hx_context* ctx, hx_bvh* bvh, const float* d_rays, int64_t n_rays, int32_t any_hit, int32_t* d_results,
+             int32_t* d_stats) {
+    return trace_common(ctx, bvh, d_rays, n_rays, any_hit, d_results, d_stats, nullptr);
+}
+
+int hx_ray_keys(hx_context* ctx, const float* d_rays, int64_t n_rays, int32_t key_method, int32_t key_bits,
+                float ray_length, uint64_t* d_keys) {
+    HX_CHECK_ARG(ctx && d_rays && d_keys && n_rays >= 0, "invalid argument");
+    HX_CHECK_ARG(key_method == HX_RAYKEY_AILA || key_method == HX_RAYKEY_TWOPOINT, "unknown ray key method %d",
+                 key_method);
+    HX_CHECK_ARG(key_bits >= 1 && key_bits <= 64, "key_bits must be in [1,64]");
+    return hx::launch_ray_keys(&ctx->c, reinterpret_cast<const float4*>(d_rays), n_rays, key_method, key_bits,
+                               ray_length, d_keys);
+}
+
+int hx_trace_sorted(hx_context* ctx, hx_bvh* bvh, const float* d_rays, int64_t n_rays, int32_t any_hit,
+                    int32_t key_method, int32_t key_bits, float ray_length, int32_t* d_results, int32_t* d_stats,
+                    int32_t* d_order) {
+    HX_CHECK_ARG(ctx && bvh, "context or bvh is NULL");
+    HX_CHECK_ARG(n_rays >= 0 && n_rays < ((int64_t)1 << 30), "n_rays must be in [0, 2^30)");
+    if (n_rays == 0) return HX_OK;
+    Context* c = &ctx->c;
+    for (int k = 0; k < 2; ++k) {
+        HX_TRY(c->ray_keys[k].reserve(8 * (size_t)n_rays));
+        HX_TRY(c->ray_order[k].reserve(4 * (size_t)n_rays));
+    }
+    HX_TRY(hx_ray_keys(ctx, d_rays, n_rays, key_method, key_bits, ray_length, c->ray_keys[0].as<uint64_t>()));
+    void* keys[2] = {c->ray_keys[0].ptr, c->ray_keys[1].ptr};
+    int32_t* vals[2] = {c->ray_order[0].as<int32_t>(), c->ray_order[1].as<int32_t>()};
+    int ri = 0;
+    HX_TRY(hx::launch_sort_pairs(c, keys, vals, n_rays, key_bits, true, true, &ri));
+    const int32_t* order = vals[ri];
+    HX_TRY(trace_common(ctx, bvh, d_rays, n_rays, any_hit, d_results, d_stats, order));
+    if (d_order)
+        HX_HIP(hipMemcpyAsync(d_order, order, 4 * (size_t)n_rays, hipMemcpyDeviceToDevice, c->stream));
+    return HX_OK;
+}
+
+int hx_memcpy(hx_context* ctx, void* dst, const void* src, int64_t bytes) {
+    HX_CHECK_ARG(ctx && bytes >= 0, "invalid argument");
+    if (bytes == 0) return HX_OK;
+    HX_CHECK_ARG(dst && src, "NULL pointer");
+    HX_HIP(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDefault, ctx->c.stream));
+    HX_HIP(hipStreamSynchronize(ctx->c.stream));
+    return HX_OK;
+}
+
+int hx_trace_overflow_count(hx_context* ctx, int64_t* out) {
+    HX_CHECK_ARG(ctx && out, "NULL argument");
+    Context* c = &ctx->c;
+    if (!c->control.ptr) {
+        *out = 0;
+        return HX_OK;
+    }
+    int32_t v[2] = {0, 0};
+    HX_HIP(hipMemcpyAsync(v, c->control.ptr, sizeof(v), hipMemcpyDeviceToHost, c->stream));
+    HX_HIP(hipStreamSynchronize(c->stream));
+    *out = v[1];
+    return HX_OK;
+}
+
+}  // extern "C"

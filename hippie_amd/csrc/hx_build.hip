@@ -1,0 +1,806 @@
+// hx_build.hip -- gfx950 kernels of the LBVH build: scene box, Morton codes,
+// Karras hierarchy, one fused bottom-up pass (refit + Woop + SAH collapse
+// decision) and a scan-based deterministic collapse to the 64-byte Bin layout.
+//
+// Reference (src/hippie/): rt/bvh/LBVHBuilderKernels.cu, BVHCollapserKernels.cu,
+// HipBVHKernels.cu, rt/HipUtil.h, rt/bvh/HipBVHUtil.h.
+#include "hx_device.h"
+#include "hx_internal.h"
+
+namespace hx {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kWavesPerBlock = kBlock / kWave;
+
+inline unsigned grid_for(int64_t n, int block = kBlock) { return (unsigned)((n + block - 1) / block); }
+
+HX_DEV float3 ld3(const float* p) { return make_float3(p[0], p[1], p[2]); }
+
+HX_DEV void tri_vertices(const float* verts, const int32_t* tris, int64_t t, float3& a, float3& b, float3& c) {
+    const int32_t* tr = tris + 3 * t;
+    int i0 = tr[0], i1 = tr[1], i2 = tr[2];
+    a = ld3(verts + 3 * (int64_t)i0);
+    b = ld3(verts + 3 * (int64_t)i1);
+    c = ld3(verts + 3 * (int64_t)i2);
+}
+
+HX_DEV Box3 tri_box(float3 a, float3 b, float3 c) {
+    Box3 bx = box_empty();
+    box_grow(bx, a);
+    box_grow(bx, b);
+    box_grow(bx, c);
+    return bx;
+}
+
+// ---------------------------------------------------------------------------
+// Scene box: exact min/max reduction (order-independent up to the sign of a
+// zero, which does not change any Morton code or area).
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(kBlock) void k_scene_box_partial(const float* __restrict__ verts,
+                                                               const int32_t* __restrict__ tris, int64_t n,
+                                                               float* __restrict__ partial) {
+    float r[6] = {kMaxFloat, kMaxFloat, kMaxFloat, -kMaxFloat, -kMaxFloat, -kMaxFloat};
+    for (int64_t t = blockIdx.x * (int64_t)kBlock + threadIdx.x; t < n; t += (int64_t)gridDim.x * kBlock) {
+        float3 a, b, c;
+        tri_vertices(verts, tris, t, a, b, c);
+        r[0] = fminf(r[0], fminf(fminf(a.x, b.x), c.x));
+        r[1] = fminf(r[1], fminf(fminf(a.y, b.y), c.y));
+        r[2] = fminf(r[2], fminf(fminf(a.z, b.z), c.z));
+        r[3] = fmaxf(r[3], fmaxf(fmaxf(a.x, b.x), c.x));
+        r[4] = fmaxf(r[4], fmaxf(fmaxf(a.y, b.y), c.y));
+        r[5] = fmaxf(r[5], fmaxf(fmaxf(a.z, b.z), c.z));
+    }
+#pragma unroll
+    for (int m = 1; m < kWave; m <<= 1) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) r[k] = fminf(r[k], __shfl_xor(r[k], m, kWave));
+#pragma unroll
+        for (int k = 3; k < 6; ++k) r[k] = fmaxf(r[k], __shfl_xor(r[k], m, kWave));
+    }
+    __shared__ float s[kWavesPerBlock][6];
+    const int w = threadIdx.x / kWave;
+    if (lane_id() == 0)
+        for (int k = 0; k < 6; ++k) s[w][k] = r[k];
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int k = threadIdx.x;
+        float v = s[0][k];
+        for (int i = 1; i < kWavesPerBlock; ++i) v = (k < 3) ? fminf(v, s[i][k]) : fmaxf(v, s[i][k]);
+        partial[blockIdx.x * 6 + k] = v;
+    }
+}
+
+__global__ void k_scene_box_final(const float* __restrict__ partial, int nblocks, float* __restrict__ box) {
+    const int k = threadIdx.x;
+    if (k < 6) {
+        float v = (k < 3) ? kMaxFloat : -kMaxFloat;
+        for (int i = 0; i < nblocks; ++i) v = (k < 3) ? fminf(v, partial[i * 6 + k]) : fmaxf(v, partial[i * 6 + k]);
+        box[k] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Morton codes: setupBoxes + computeMortonCodes60 fused (LBVHBuilderKernels.cu
+// :65-98,131-160).  The per-triangle box is recomputed, never stored.
+// ---------------------------------------------------------------------------
+
+template <bool K64>
+__global__ __launch_bounds__(kBlock) void k_morton(const float* __restrict__ verts, const int32_t* __restrict__ tris,
+                                                   int64_t n, const float* __restrict__ sb, int shift,
+                                                   void* __restrict__ keys_out) {
+    const int64_t t = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    if (t >= n) return;
+    const float3 smn = make_float3(sb[0], sb[1], sb[2]);
+    const float3 scale = make_float3(1.0f / (sb[3] - sb[0]), 1.0f / (sb[4] - sb[1]), 1.0f / (sb[5] - sb[2]));
+    float3 a, b, c;
+    tri_vertices(verts, tris, t, a, b, c);
+    const Box3 tb = tri_box(a, b, c);
+    // computeMortonCodes60 regrows the stored box from its two corners.
+    Box3 bb = box_empty();
+    box_grow(bb, tb.mn);
+    box_grow(bb, tb.mx);
+    const float cx = (bb.mn.x + bb.mx.x) * 0.5f;
+    const float cy = (bb.mn.y + bb.mx.y) * 0.5f;
+    const float cz = (bb.mn.z + bb.mx.z) * 0.5f;
+    const uint64_t code = morton60((cx - smn.x) * scale.x, (cy - smn.y) * scale.y, (cz - smn.z) * scale.z) >> shift;
+    if (K64)
+        static_cast<uint64_t*>(keys_out)[t] = code;
+    else
+        static_cast<uint32_t*>(keys_out)[t] = (uint32_t)code;
+}
+
+// ---------------------------------------------------------------------------
+// Karras 2012 hierarchy (LBVHBuilderKernels.cu:11-63,226-252; delta from
+// rt/bvh/HipBVHUtil.h:17-33).  32-bit keys use clz32 with the tie value
+// shifted by the same 32 as every other delta, which keeps the tree identical.
+// Also initialises the climb counters and the collapse cover array, and writes
+// the leaf entries setupLeaves gives left/right (:185-187).
+// ---------------------------------------------------------------------------
+
+template <bool K64>
+HX_DEV int kdelta(int i, int j, int n, const void* keys) {
+    if (j < 0 || j >= n) return -1;
+    if (K64) {
+        const uint64_t* k = static_cast<const uint64_t*>(keys);
+        const uint64_t a = k[i], b = k[j];
+        return a != b ? __clzll(a ^ b) : __clzll((uint64_t)(uint32_t)(i ^ j)) + 64;
+    } else {
+        const uint32_t* k = static_cast<const uint32_t*>(keys);
+        const uint32_t a = k[i], b = k[j];
+        return a != b ? __clz(a ^ b) : __clz((uint32_t)(i ^ j)) + 64;
+    }
+}
+
+template <bool K64>
+__global__ __launch_bounds__(kBlock) void k_construct(const void* __restrict__ keys, int n, int32_t* __restrict__ left,
+                                                      int32_t* __restrict__ right, int32_t* __restrict__ parent,
+                                                      int32_t* __restrict__ other, int32_t* __restrict__ counters,
+                                                      int32_t* __restrict__ end_at, int32_t* __restrict__ zero_word) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    end_at[i] = -1;
+    left[n - 1 + i] = i;
+    right[n - 1 + i] = i + 1;
+    if (i == 0) {
+        parent[0] = -1;
+        *zero_word = 0;
+    }
+    if (i >= n - 1) return;
+    counters[i] = 0;
+    const int dn = kdelta<K64>(i, i + 1, n, keys);
+    const int dp = kdelta<K64>(i, i - 1, n, keys);
+    const int d = (dn > dp) - (dn < dp);
+    const int dmin = d > 0 ? dp : dn;
+    int lmax = 2;
+    while (kdelta<K64>(i, i + lmax * d, n, keys) > dmin) lmax <<= 1;
+    int l = 0;
+    for (int t = lmax >> 1; t >= 1; t >>= 1)
+        if (kdelta<K64>(i, i + (l + t) * d, n, keys) > dmin) l += t;
+    const int j = i + l * d;
+    const int dnode = kdelta<K64>(i, j, n, keys);
+    int s = 0, k = 2, t;
+    do {
+        t = (l + k - 1) / k;
+        k <<= 1;
+        if (kdelta<K64>(i, i + (s + t) * d, n, keys) > dnode) s += t;
+    } while (t > 1);
+    const int gamma = i + s * d + min(d, 0);
+    int lc = gamma, rc = gamma + 1;
+    if (min(i, j) == gamma) lc += n - 1;
+    if (max(i, j) == gamma + 1) rc += n - 1;
+    left[i] = lc;
+    right[i] = rc;
+    other[i] = j;
+    parent[lc] = i;
+    parent[rc] = i;
+}
+
+// ---------------------------------------------------------------------------
+// Woop transform: rt/bvh/HipBVHKernels.cu:12-33 with util/Math.h:318-369
+// (cofactor inverse, d summing all 16 cofactor products, r * rcp(d) * 4).
+// ---------------------------------------------------------------------------
+
+HX_DEV float det3(float a00, float a01, float a02, float a10, float a11, float a12, float a20, float a21,
+                  float a22) {
+    return a00 * a11 * a22 - a00 * a12 * a21 + a10 * a21 * a02 - a10 * a22 * a01 + a20 * a01 * a12 -
+           a20 * a02 * a11;
+}
+
+HX_DEV void woop_rows(float3 v0, float3 v1, float3 v2, float4& w0, float4& w1, float4& w2) {
+    const float ax = v0.x - v2.x, ay = v0.y - v2.y, az = v0.z - v2.z;
+    const float bx = v1.x - v2.x, by = v1.y - v2.y, bz = v1.z - v2.z;
+    const float nx = ay * bz - az * by, ny = az * bx - ax * bz, nz = ax * by - ay * bx;
+    // M[row][col]: columns (v0-v2, v1-v2, n, (v2, 1)).
+    const float M[4][4] = {{ax, bx, nx, v2.x}, {ay, by, ny, v2.y}, {az, bz, nz, v2.z}, {0.0f, 0.0f, 0.0f, 1.0f}};
+    float R[4][4];
+    float d = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            float s[3][3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+#pragma unroll
+                for (int l = 0; l < 3; ++l) s[k][l] = M[(k < j) ? k : k + 1][(l < i) ? l : l + 1];
+            const float sgn = ((i + j) & 1) ? -1.0f : 1.0f;
+            const float dd = det3(s[0][0], s[0][1], s[0][2], s[1][0], s[1][1], s[1][2], s[2][0], s[2][1], s[2][2]) * sgn;
+            R[i][j] = dd;
+            d += dd * M[j][i];
+        }
+    }
+    const float rd = (d != 0.0f) ? 1.0f / d : 0.0f;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) R[a][b] = (R[a][b] * rd) * 4.0f;
+    w0 = make_float4(R[2][0], R[2][1], R[2][2], -R[2][3]);
+    w1 = make_float4(R[0][0], R[0][1], R[0][2], R[0][3]);
+    w2 = make_float4(R[1][0], R[1][1], R[1][2], R[1][3]);
+}
+
+// ---------------------------------------------------------------------------
+// Fused bottom-up pass: leaf box (setupLeaves), Woop rows (woopifyTriangles),
+// leaf SAH cost, then the termination-counter climb in which the second
+// arriver at each internal node unions the child boxes (refit,
+// LBVHBuilderKernels.cu:254-305) and takes the collapse decision
+// (computeNodeStatesAdaptive / computeNodeStates, BVHCollapserKernels.cu
+// :58-184).  A state-0 node other than the root records its range end at its
+// range start (atomicMax), the input of the scan-based collapse below.
+//
+// Two levels, because each XCD of gfx950 has its own L2: an agent-scope
+// acquire/release per climb step writes back and invalidates that L2.
+//   level 1 (k_climb_local): one workgroup per 512 consecutive sorted leaves
+//     climbs every node whose range lies inside its leaves, with boxes, costs
+//     and counters in LDS and workgroup-scope atomics.  A node whose parent
+//     straddles the block boundary is appended to the frontier list.
+//   level 2 (k_climb_top): the ~2n/512 straddling nodes, one thread per
+//     frontier entry; the exchanged boxes and costs are written and read with
+//     agent-coherent (sc1) relaxed atomic accesses, and each writer waits for
+//     its stores to complete before it bumps the parent's counter, so no L2
+//     writeback or invalidate is needed.
+// ---------------------------------------------------------------------------
+
+constexpr int kClimbThreads = 256;
+constexpr int kClimbLeaves = 512;
+
+struct Combined {
+    float3 mn, mx;
+    float cost;
+    int state;
+};
+
+// Union of the two child boxes in the reference's grow order, and the
+// collapse decision for node [first, last].
+HX_DEV Combined combine(const ClimbArgs& a, int first, int last, float3 lmn, float3 lmx, float lcost, float3 rmn,
+                        float3 rmx, float rcost) {
+    Combined c;
+    Box3 bx = box_empty();
+    box_grow(bx, lmn);
+    box_grow(bx, rmn);
+    c.mn = bx.mn;
+    box_grow(bx, lmx);
+    box_grow(bx, rmx);
+    c.mx = bx.mx;
+    const int size = last - first + 1;
+    c.cost = 0.0f;
+    if (a.adaptive) {
+        Box3 bb = box_empty();
+        box_grow(bb, c.mn);
+        box_grow(bb, c.mx);
+        const float area = box_area(bb.mn, bb.mx);
+        const float cost = a.ct * area + lcost + rcost;
+        const float leafCost = a.ci * area * (float)size;
+        c.state = (leafCost < cost) ? 0 : 1;
+        c.cost = c.state == 0 ? leafCost : cost;
+    } else {
+        c.state = size <= a.max_leaf_size ? 0 : 1;
+    }
+    return c;
+}
+
+HX_DEV float3 xyz(float4 v) { return make_float3(v.x, v.y, v.z); }
+
+__global__ __launch_bounds__(kClimbThreads) void k_climb_local(ClimbArgs a, int32_t* __restrict__ frontier,
+                                                              int32_t* __restrict__ frontier_count) {
+    constexpr int T = kClimbLeaves;
+    // Slots: internal node i -> i - base (in [0, T-1)), leaf p -> T + p - base.
+    __shared__ float s_box[6][2 * T];
+    __shared__ float s_cost[2 * T];
+    __shared__ int s_cnt[T];
+    const int n = (int)a.n;
+    const int base = blockIdx.x * T;
+    const int last_leaf = min(base + T, n) - 1;
+    for (int i = threadIdx.x; i < T; i += kClimbThreads) s_cnt[i] = 0;
+
+    for (int k = 0; k < T / kClimbThreads; ++k) {
+        const int p = base + k * kClimbThreads + threadIdx.x;
+        if (p > last_leaf) break;
+        const int tri = a.sorted_tri[p];
+        float3 v0, v1, v2;
+        tri_vertices(a.verts, a.tris, tri, v0, v1, v2);
+        const Box3 tb = tri_box(v0, v1, v2);
+        const int leaf = n - 1 + p;
+        a.nmin[leaf] = make_float4(tb.mn.x, tb.mn.y, tb.mn.z, 0.0f);
+        a.nmax[leaf] = make_float4(tb.mx.x, tb.mx.y, tb.mx.z, 0.0f);
+        float4 w0, w1, w2;
+        woop_rows(v0, v1, v2, w0, w1, w2);
+        a.woop[3 * (int64_t)p + 0] = w0;
+        a.woop[3 * (int64_t)p + 1] = w1;
+        a.woop[3 * (int64_t)p + 2] = w2;
+        float cost = 0.0f;
+        if (a.adaptive) {
+            Box3 bb = box_empty();
+            box_grow(bb, tb.mn);
+            box_grow(bb, tb.mx);
+            cost = a.ci * box_area(bb.mn, bb.mx);
+            a.cost[leaf] = cost;
+        }
+        const int sl = T + p - base;
+        s_box[0][sl] = tb.mn.x; s_box[1][sl] = tb.mn.y; s_box[2][sl] = tb.mn.z;
+        s_box[3][sl] = tb.mx.x; s_box[4][sl] = tb.mx.y; s_box[5][sl] = tb.mx.z;
+        s_cost[sl] = cost;
+    }
+    __syncthreads();
+
+    for (int k = 0; k < T / kClimbThreads; ++k) {
+        const int p = base + k * kClimbThreads + threadIdx.x;
+        if (p > last_leaf) break;
+        int child = n - 1 + p;
+        int node = a.parent[child];
+        while (node >= 0) {
+            const int j = a.other[node];
+            const int first = min(node, j), last = max(node, j);
+            if (first < base || last > last_leaf) {
+                frontier[atomicAdd(frontier_count, 1)] = child;
+                break;
+            }
+            if (__hip_atomic_fetch_add(&s_cnt[node - base], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) == 0)
+                break;
+            const int l = a.left[node], r = a.right[node];
+            const int ls = l >= n - 1 ? T + (l - (n - 1) - base) : l - base;
+            const int rs = r >= n - 1 ? T + (r - (n - 1) - base) : r - base;
+            const Combined c = combine(
+                a, first, last, make_float3(s_box[0][ls], s_box[1][ls], s_box[2][ls]),
+                make_float3(s_box[3][ls], s_box[4][ls], s_box[5][ls]), s_cost[ls],
+                make_float3(s_box[0][rs], s_box[1][rs], s_box[2][rs]),
+                make_float3(s_box[3][rs], s_box[4][rs], s_box[5][rs]), s_cost[rs]);
+            const int sn = node - base;
+            s_box[0][sn] = c.mn.x; s_box[1][sn] = c.mn.y; s_box[2][sn] = c.mn.z;
+            s_box[3][sn] = c.mx.x; s_box[4][sn] = c.mx.y; s_box[5][sn] = c.mx.z;
+            s_cost[sn] = c.cost;
+            a.nmin[node] = make_float4(c.mn.x, c.mn.y, c.mn.z, 0.0f);
+            a.nmax[node] = make_float4(c.mx.x, c.mx.y, c.mx.z, 0.0f);
+            if (a.adaptive) a.cost[node] = c.cost;
+            a.state[node] = c.state;
+            if (c.state == 0 && node != 0) atomicMax(&a.end_at[first], last);
+            child = node;
+            node = a.parent[node];
+        }
+    }
+}
+
+// Agent-coherent (sc1) relaxed accesses of a float4 as two 64-bit words.
+HX_DEV float4 ld_agent(float4* p) {
+    uint64_t* q = reinterpret_cast<uint64_t*>(p);
+    const uint64_t lo = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint64_t hi = __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return make_float4(__uint_as_float((uint32_t)lo), __uint_as_float((uint32_t)(lo >> 32)),
+                       __uint_as_float((uint32_t)hi), __uint_as_float((uint32_t)(hi >> 32)));
+}
+HX_DEV void st_agent(float4* p, float4 v) {
+    uint64_t* q = reinterpret_cast<uint64_t*>(p);
+    const uint64_t lo = (uint64_t)__float_as_uint(v.x) | ((uint64_t)__float_as_uint(v.y) << 32);
+    const uint64_t hi = (uint64_t)__float_as_uint(v.z) | ((uint64_t)__float_as_uint(v.w) << 32);
+    __hip_atomic_store(q, lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(q + 1, hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+HX_DEV float ld_agent(float* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+HX_DEV void st_agent(float* p, float v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+
+__global__ __launch_bounds__(kBlock) void k_climb_top(ClimbArgs a, const int32_t* __restrict__ frontier,
+                                                     const int32_t* __restrict__ frontier_count) {
+    const int count = *frontier_count;
+    for (int e = blockIdx.x * kBlock + threadIdx.x; e < count; e += gridDim.x * kBlock) {
+        int node = a.parent[frontier[e]];
+        while (node >= 0) {
+            if (__hip_atomic_fetch_add(&a.counters[node], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) break;
+            const int j = a.other[node];
+            const int first = min(node, j), last = max(node, j);
+            const int l = a.left[node], r = a.right[node];
+            const float4 lmn = ld_agent(a.nmin + l), lmx = ld_agent(a.nmax + l);
+            const float4 rmn = ld_agent(a.nmin + r), rmx = ld_agent(a.nmax + r);
+            const float lc = a.adaptive ? ld_agent(a.cost + l) : 0.0f;
+            const float rc = a.adaptive ? ld_agent(a.cost + r) : 0.0f;
+            const Combined c = combine(a, first, last, xyz(lmn), xyz(lmx), lc, xyz(rmn), xyz(rmx), rc);
+            st_agent(a.nmin + node, make_float4(c.mn.x, c.mn.y, c.mn.z, 0.0f));
+            st_agent(a.nmax + node, make_float4(c.mx.x, c.mx.y, c.mx.z, 0.0f));
+            if (a.adaptive) st_agent(a.cost + node, c.cost);
+            a.state[node] = c.state;
+            if (c.state == 0 && node != 0) atomicMax(&a.end_at[first], last);
+            // Complete this node's stores before the parent's counter can be
+            // observed (the compiler fence keeps the stores above the wait).
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            __builtin_amdgcn_s_waitcnt(0);
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            node = a.parent[node];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Deterministic collapse.
+//
+// The reference marks the topmost SAH-leaf below the root per reference
+// (computeLeafIndices), invalidates everything under it, lays out nodes by a
+// BFS with warp atomics and reorders triangles with atomics.  Here the
+// collapsed leaves are found with scans instead: the ranges of state-0 nodes
+// form a laminar family, so with cover[p] = max over a <= p of end_at[a]
+//   - a final leaf starts at p iff p == 0 or cover[p-1] < p;
+//   - internal node i is a final interior iff i is the root or
+//     (state[i] == 1 and cover[first_i] < last_i).
+// Interior slots number the final interiors in Karras index order (root = 0);
+// leaf slots follow in start order, and a leaf's triangles are the contiguous
+// sorted range [first, last], so the triangle index array is the sorted
+// triangle ids themselves.
+// ---------------------------------------------------------------------------
+
+constexpr int kScanItems = 16;
+constexpr int kScanTile = kBlock * kScanItems;
+
+struct MaxOp {
+    HX_DEV int operator()(int a, int b) const { return a > b ? a : b; }
+};
+struct Sum2Op {
+    HX_DEV int2 operator()(int2 a, int2 b) const { return make_int2(a.x + b.x, a.y + b.y); }
+};
+
+__global__ __launch_bounds__(kBlock) void k_tile_max(const int32_t* __restrict__ in, int64_t n, int32_t* __restrict__ partial) {
+    const int64_t base = blockIdx.x * (int64_t)kScanTile;
+    int m = -1;
+    for (int k = 0; k < kScanItems; ++k) {
+        const int64_t p = base + k * kBlock + threadIdx.x;
+        if (p < n) m = max(m, in[p]);
+    }
+#pragma unroll
+    for (int s = 1; s < kWave; s <<= 1) m = max(m, __shfl_xor(m, s, kWave));
+    __shared__ int wm[kWavesPerBlock];
+    if (lane_id() == 0) wm[threadIdx.x / kWave] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int r = wm[0];
+        for (int i = 1; i < kWavesPerBlock; ++i) r = max(r, wm[i]);
+        partial[blockIdx.x] = r;
+    }
+}
+
+// Single block: exclusive prefix max of the tile partials, in place.
+__global__ __launch_bounds__(kBlock) void k_scan_partials_max(int32_t* __restrict__ partial, int64_t ntiles) {
+    __shared__ int wt[kWavesPerBlock];
+    __shared__ int incs[kBlock];
+    int carry = -1;
+    for (int64_t base = 0; base < ntiles; base += kBlock) {
+        const int64_t t = base + threadIdx.x;
+        const int v = t < ntiles ? partial[t] : -1;
+        int tot;
+        const int inc = block_inclusive_scan<kWavesPerBlock>(v, MaxOp(), wt, tot);
+        incs[threadIdx.x] = inc;
+        __syncthreads();
+        const int e = threadIdx.x == 0 ? carry : max(carry, incs[threadIdx.x - 1]);
+        if (t < ntiles) partial[t] = e;
+        carry = max(carry, tot);
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_tile_cover(int32_t* __restrict__ cover, int64_t n,
+                                                       const int32_t* __restrict__ partial) {
+    __shared__ int wt[kWavesPerBlock];
+    const int64_t base = blockIdx.x * (int64_t)kScanTile;
+    int carry = partial[blockIdx.x];
+    for (int k = 0; k < kScanItems; ++k) {
+        const int64_t p = base + k * kBlock + threadIdx.x;
+        const int v = p < n ? cover[p] : -1;
+        int tot;
+        const int inc = block_inclusive_scan<kWavesPerBlock>(v, MaxOp(), wt, tot);
+        if (p < n) cover[p] = max(carry, inc);
+        carry = max(carry, tot);
+    }
+}
+
+struct FlagCtx {
+    int n;
+    const int32_t* cover;
+    const int32_t* other;
+    const int32_t* state;
+};
+
+HX_DEV int leaf_start(const FlagCtx& f, int p) {
+    return (p < f.n) && (p == 0 || f.cover[p - 1] < p);
+}
+
+HX_DEV int interior_flag(const FlagCtx& f, int i) {
+    if (i >= f.n - 1) return 0;
+    if (i == 0) return 1;
+    if (f.state[i] != 1) return 0;
+    const int j = f.other[i];
+    const int first = min(i, j), last = max(i, j);
+    return f.cover[first] < last;
+}
+
+__global__ __launch_bounds__(kBlock) void k_tile_flag_counts(FlagCtx f, int2* __restrict__ partial) {
+    const int64_t base = blockIdx.x * (int64_t)kScanTile;
+    int cl = 0, ci = 0;
+    for (int k = 0; k < kScanItems; ++k) {
+        const int64_t p = base + k * kBlock + threadIdx.x;
+        if (p < f.n) {
+            ci += interior_flag(f, (int)p);
+            cl += leaf_start(f, (int)p);
+        }
+    }
+    ci = wave_sum_i32(ci);
+    cl = wave_sum_i32(cl);
+    __shared__ int2 ws[kWavesPerBlock];
+    if (lane_id() == 0) ws[threadIdx.x / kWave] = make_int2(ci, cl);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int2 r = ws[0];
+        for (int i = 1; i < kWavesPerBlock; ++i) r = make_int2(r.x + ws[i].x, r.y + ws[i].y);
+        partial[blockIdx.x] = r;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_partials_sum2(int2* __restrict__ partial, int64_t ntiles,
+                                                               int32_t* __restrict__ counts) {
+    __shared__ int2 wt[kWavesPerBlock];
+    int2 carry = make_int2(0, 0);
+    for (int64_t base = 0; base < ntiles; base += kBlock) {
+        const int64_t t = base + threadIdx.x;
+        const int2 v = t < ntiles ? partial[t] : make_int2(0, 0);
+        int2 tot;
+        const int2 inc = block_inclusive_scan<kWavesPerBlock>(v, Sum2Op(), wt, tot);
+        if (t < ntiles) partial[t] = make_int2(carry.x + inc.x - v.x, carry.y + inc.y - v.y);
+        carry = make_int2(carry.x + tot.x, carry.y + tot.y);
+    }
+    if (threadIdx.x == 0) {
+        counts[0] = carry.x;  // interiors
+        counts[1] = carry.y;  // leaves
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_tile_slots(FlagCtx f, const int2* __restrict__ partial,
+                                                       int32_t* __restrict__ int_slot, int32_t* __restrict__ leaf_num) {
+    __shared__ int2 wt[kWavesPerBlock];
+    const int64_t base = blockIdx.x * (int64_t)kScanTile;
+    int2 carry = partial[blockIdx.x];
+    for (int k = 0; k < kScanItems; ++k) {
+        const int64_t p = base + k * kBlock + threadIdx.x;
+        int2 v = make_int2(0, 0);
+        if (p < f.n) v = make_int2(interior_flag(f, (int)p), leaf_start(f, (int)p));
+        int2 tot;
+        const int2 inc = block_inclusive_scan<kWavesPerBlock>(v, Sum2Op(), wt, tot);
+        if (p < f.n) {
+            if (p < f.n - 1) int_slot[p] = carry.x + inc.x - v.x;
+            leaf_num[p] = carry.y + inc.y - v.y;
+        }
+        carry = make_int2(carry.x + tot.x, carry.y + tot.y);
+    }
+}
+
+HX_DEV void write_node(int4* nodes, int slot, float4 amn, float4 amx, float4 bmn, float4 bmx, int c0, int c1, int size,
+                       int parent) {
+    int4* o = nodes + 4 * (int64_t)slot;
+    o[0] = make_int4(__float_as_int(amn.x), __float_as_int(amx.x), __float_as_int(amn.y), __float_as_int(amx.y));
+    o[1] = make_int4(__float_as_int(bmn.x), __float_as_int(bmx.x), __float_as_int(bmn.y), __float_as_int(bmx.y));
+    o[2] = make_int4(__float_as_int(amn.z), __float_as_int(amx.z), __float_as_int(bmn.z), __float_as_int(bmx.z));
+    o[3] = make_int4(c0, c1, size, parent);
+}
+
+__global__ __launch_bounds__(kBlock) void k_emit(EmitArgs a, FlagCtx f) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const int n = (int)a.n;
+    if (i >= n - 1) return;
+    if (!interior_flag(f, i)) return;
+    const int nInt = a.counts[0];
+    const int s = a.int_slot[i];
+    const int j = a.other[i];
+    const int first = min(i, j), last = max(i, j);
+    const int ch[2] = {a.left[i], a.right[i]};
+    int cidx[2];
+    float4 cmn[2], cmx[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int c = ch[k];
+        cmn[k] = a.nmin[c];
+        cmx[k] = a.nmax[c];
+        if (c < n - 1 && a.state[c] == 1) {
+            cidx[k] = a.int_slot[c];
+        } else {
+            int cf, cl;
+            if (c >= n - 1) {
+                cf = cl = c - (n - 1);
+            } else {
+                const int jj = a.other[c];
+                cf = min(c, jj);
+                cl = max(c, jj);
+            }
+            const int L = nInt + a.leaf_num[cf];
+            cidx[k] = ~L;
+            write_node(a.nodes, L, cmn[k], cmx[k], cmn[k], cmx[k], cf, cl + 1, ~(cl - cf + 1), s);
+        }
+    }
+    const int par = (i == 0) ? -1 : a.int_slot[a.parent[i]];
+    write_node(a.nodes, s, cmn[0], cmx[0], cmn[1], cmx[1], cidx[0], cidx[1], last - first + 1, par);
+}
+
+// n == 1: interior root whose two children are the single leaf (the
+// reference indexes termCounters[-1] here, BVHCollapserKernels.cu:29-32).
+__global__ void k_single_ref(const float* __restrict__ verts, const int32_t* __restrict__ tris,
+                             const int32_t* __restrict__ sorted_tri, float4* nmin, float4* nmax, float4* woop,
+                             int4* nodes, int32_t* counts) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    float3 v0, v1, v2;
+    tri_vertices(verts, tris, sorted_tri[0], v0, v1, v2);
+    const Box3 tb = tri_box(v0, v1, v2);
+    const float4 mn = make_float4(tb.mn.x, tb.mn.y, tb.mn.z, 0.0f);
+    const float4 mx = make_float4(tb.mx.x, tb.mx.y, tb.mx.z, 0.0f);
+    nmin[0] = mn;
+    nmax[0] = mx;
+    float4 w0, w1, w2;
+    woop_rows(v0, v1, v2, w0, w1, w2);
+    woop[0] = w0;
+    woop[1] = w1;
+    woop[2] = w2;
+    write_node(nodes, 0, mn, mx, mn, mx, ~1, ~1, 1, -1);
+    write_node(nodes, 1, mn, mx, mn, mx, 0, 1, ~1, 0);
+    counts[0] = 1;
+    counts[1] = 1;
+}
+
+// ---------------------------------------------------------------------------
+// SAH cost (HipBVHKernels.cu:184-238): FP32 per-node terms, FP64 sums in a
+// fixed tree order (the reference adds with a float atomicAdd).
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(kBlock) void k_cost_partial(const int4* __restrict__ nodes, const int32_t* __restrict__ counts,
+                                                         const float* __restrict__ sb, float ct, float ci,
+                                                         double* __restrict__ partial) {
+    const int total = counts[0] + counts[1];
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const float sceneArea = box_area(make_float3(sb[0], sb[1], sb[2]), make_float3(sb[3], sb[4], sb[5]));
+    double term = 0.0;
+    if (i < total) {
+        const int4* nd = nodes + 4 * (int64_t)i;
+        const int4 q0 = nd[0], q1 = nd[1], q2 = nd[2], q3 = nd[3];
+        Box3 b = box_empty();
+        box_grow(b, make_float3(__int_as_float(q0.x), __int_as_float(q0.z), __int_as_float(q2.x)));
+        box_grow(b, make_float3(__int_as_float(q0.y), __int_as_float(q0.w), __int_as_float(q2.y)));
+        box_grow(b, make_float3(__int_as_float(q1.x), __int_as_float(q1.z), __int_as_float(q2.z)));
+        box_grow(b, make_float3(__int_as_float(q1.y), __int_as_float(q1.w), __int_as_float(q2.w)));
+        const float P = box_area(b.mn, b.mx) / sceneArea;
+        const float t = q3.z < 0 ? ci * P * (float)(~q3.z) : ct * P;
+        term = (double)t;
+    }
+    __shared__ double s[kBlock];
+    s[threadIdx.x] = term;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = s[0];
+}
+
+__global__ void k_cost_final(const double* __restrict__ partial, int nblocks, double* __restrict__ out) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double acc = 0.0;
+        for (int i = 0; i < nblocks; ++i) acc += partial[i];
+        out[0] = acc;
+    }
+}
+
+__global__ void k_fill_i32(int32_t* p, int32_t v, int64_t n) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Launchers.
+// ---------------------------------------------------------------------------
+
+int launch_fill_i32(Context* c, int32_t* p, int32_t v, int64_t n) {
+    if (n <= 0) return HX_OK;
+    LaunchScope ls(c, "fill_i32");
+    hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, p, v, n);
+    return check_launch("fill_i32");
+}
+
+int launch_scene_box(Context* c, const float* verts, const int32_t* tris, int64_t n, float* d_box6) {
+    const int nblocks = (int)std::min<int64_t>(std::max<int64_t>((n + kBlock * 8 - 1) / (kBlock * 8), 1), 2048);
+    HX_TRY(c->scan_partials.reserve(sizeof(float) * 6 * nblocks));
+    {
+        LaunchScope ls(c, "scene_box");
+        hipLaunchKernelGGL(k_scene_box_partial, dim3(nblocks), dim3(kBlock), 0, c->stream, verts, tris, n,
+                           c->scan_partials.as<float>());
+        HX_TRY(check_launch("scene_box_partial"));
+        hipLaunchKernelGGL(k_scene_box_final, dim3(1), dim3(64), 0, c->stream, c->scan_partials.as<float>(), nblocks,
+                           d_box6);
+    }
+    return check_launch("scene_box_final");
+}
+
+int launch_morton(Context* c, const float* verts, const int32_t* tris, int64_t n, const float* d_box6, int bits,
+                  void* d_keys, bool keys64) {
+    LaunchScope ls(c, "morton");
+    const int shift = 60 - bits;
+    if (keys64)
+        hipLaunchKernelGGL(k_morton<true>, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, verts, tris, n, d_box6, shift,
+                           d_keys);
+    else
+        hipLaunchKernelGGL(k_morton<false>, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, verts, tris, n, d_box6,
+                           shift, d_keys);
+    return check_launch("morton");
+}
+
+int launch_construct(Context* c, const void* d_keys, bool keys64, int64_t n, int32_t* left, int32_t* right,
+                     int32_t* parent, int32_t* other, int32_t* counters, int32_t* end_at, int32_t* zero_word) {
+    LaunchScope ls(c, "construct");
+    if (keys64)
+        hipLaunchKernelGGL(k_construct<true>, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, d_keys, (int)n, left, right,
+                           parent, other, counters, end_at, zero_word);
+    else
+        hipLaunchKernelGGL(k_construct<false>, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, d_keys, (int)n, left,
+                           right, parent, other, counters, end_at, zero_word);
+    return check_launch("construct");
+}
+
+int launch_climb(Context* c, const ClimbArgs& a) {
+    LaunchScope ls(c, "climb");
+    const unsigned blocks = (unsigned)((a.n + kClimbLeaves - 1) / kClimbLeaves);
+    hipLaunchKernelGGL(k_climb_local, dim3(blocks), dim3(kClimbThreads), 0, c->stream, a, a.frontier,
+                       a.frontier_count);
+    HX_TRY(check_launch("climb_local"));
+    const unsigned top = (unsigned)std::max(1, std::min<int>(c->cu_count * 4, (int)((a.n + kBlock - 1) / kBlock)));
+    hipLaunchKernelGGL(k_climb_top, dim3(top), dim3(kBlock), 0, c->stream, a, a.frontier, a.frontier_count);
+    return check_launch("climb_top");
+}
+
+int launch_collapse_emit(Context* c, const EmitArgs& a) {
+    const int64_t n = a.n;
+    const int64_t ntiles = (n + kScanTile - 1) / kScanTile;
+    HX_TRY(c->scan_partials.reserve(sizeof(int2) * (size_t)ntiles + 64));
+    FlagCtx f{(int)n, a.cover, a.other, a.state};
+    {
+        LaunchScope ls(c, "collapse_cover");
+        int32_t* pm = c->scan_partials.as<int32_t>();
+        hipLaunchKernelGGL(k_tile_max, dim3((unsigned)ntiles), dim3(kBlock), 0, c->stream, a.cover, n, pm);
+        HX_TRY(check_launch("tile_max"));
+        hipLaunchKernelGGL(k_scan_partials_max, dim3(1), dim3(kBlock), 0, c->stream, pm, ntiles);
+        HX_TRY(check_launch("scan_partials_max"));
+        hipLaunchKernelGGL(k_tile_cover, dim3((unsigned)ntiles), dim3(kBlock), 0, c->stream, a.cover, n, pm);
+        HX_TRY(check_launch("tile_cover"));
+    }
+    {
+        LaunchScope ls(c, "collapse_slots");
+        int2* p2 = c->scan_partials.as<int2>();
+        hipLaunchKernelGGL(k_tile_flag_counts, dim3((unsigned)ntiles), dim3(kBlock), 0, c->stream, f, p2);
+        HX_TRY(check_launch("tile_flag_counts"));
+        hipLaunchKernelGGL(k_scan_partials_sum2, dim3(1), dim3(kBlock), 0, c->stream, p2, ntiles, a.counts);
+        HX_TRY(check_launch("scan_partials_sum2"));
+        hipLaunchKernelGGL(k_tile_slots, dim3((unsigned)ntiles), dim3(kBlock), 0, c->stream, f, p2, a.int_slot,
+                           a.leaf_num);
+        HX_TRY(check_launch("tile_slots"));
+    }
+    {
+        LaunchScope ls(c, "collapse_emit");
+        hipLaunchKernelGGL(k_emit, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, a, f);
+    }
+    return check_launch("emit");
+}
+
+int launch_single_ref(Context* c, const float* verts, const int32_t* tris, const int32_t* sorted_tri, float4* nmin,
+                      float4* nmax, float4* woop, int4* nodes, int32_t* counts) {
+    LaunchScope ls(c, "single_ref");
+    hipLaunchKernelGGL(k_single_ref, dim3(1), dim3(64), 0, c->stream, verts, tris, sorted_tri, nmin, nmax, woop, nodes,
+                       counts);
+    return check_launch("single_ref");
+}
+
+int launch_cost(Context* c, const int4* nodes, const int32_t* d_counts, int64_t max_nodes, const float* d_box6,
+                float ct, float ci, double* d_out) {
+    const int nblocks = (int)grid_for(max_nodes);
+    HX_TRY(c->scan_partials.reserve(sizeof(double) * nblocks));
+    LaunchScope ls(c, "cost");
+    hipLaunchKernelGGL(k_cost_partial, dim3(nblocks), dim3(kBlock), 0, c->stream, nodes, d_counts, d_box6, ct, ci,
+                       c->scan_partials.as<double>());
+    HX_TRY(check_launch("cost_partial"));
+    hipLaunchKernelGGL(k_cost_final, dim3(1), dim3(64), 0, c->stream, c->scan_partials.as<double>(), nblocks, d_out);
+    return check_launch("cost_final");
+}
+
+}  // namespace hx

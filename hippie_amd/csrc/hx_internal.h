@@ -1,0 +1,173 @@
+// hx_internal.h -- host-side internals shared by the libhx translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/hx.h"
+
+namespace hx {
+
+// Status propagation: every internal call returns an hx status code.
+#define HX_TRY(expr)                      \
+    do {                                  \
+        int _st = (expr);                 \
+        if (_st != HX_OK) return _st;     \
+    } while (0)
+
+#define HX_HIP(expr)                                   \
+    do {                                               \
+        hipError_t _e = (expr);                        \
+        if (_e != hipSuccess) return ::hx::hip_fail(_e, #expr, __FILE__, __LINE__); \
+    } while (0)
+
+int hip_fail(hipError_t e, const char* what, const char* file, int line);
+void set_error(const char* fmt, ...);
+
+// Device allocation owned by the library (grow-only).
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    int reserve(size_t want);
+    void release();
+    template <class T>
+    T* as() const { return static_cast<T*>(ptr); }
+};
+
+// Per-launch event timing, enabled by hx_context_set_profiling.
+struct KernelStat {
+    std::string name;
+    int64_t count = 0;
+    double total_ms = 0.0;
+};
+
+struct Profiler {
+    bool enabled = false;
+    struct Pending {
+        int stat;
+        hipEvent_t a, b;
+    };
+    std::vector<KernelStat> stats;
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> pool;
+    int begin(hipStream_t s, const char* name, hipEvent_t* a, hipEvent_t* b);
+    void end(hipStream_t s, hipEvent_t b);
+    int flush();
+    void reset();
+    ~Profiler();
+};
+
+struct Context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int cu_count = 0;
+    Profiler prof;
+    // Scratch reused across calls (grow-only).
+    DevBuf sort_status, sort_hist, scan_partials, control;
+    DevBuf ray_keys[2], ray_order[2];  // sorted trace
+    DevBuf tmp_keys, tmp_vals;         // hx_sort_pairs_u64 ping-pong
+};
+
+// RAII-free launch scope: records events around one launch when profiling.
+struct LaunchScope {
+    Context* ctx;
+    hipEvent_t a = nullptr, b = nullptr;
+    bool on = false;
+    LaunchScope(Context* c, const char* name) : ctx(c) {
+        if (ctx->prof.enabled) on = ctx->prof.begin(ctx->stream, name, &a, &b) == HX_OK;
+    }
+    ~LaunchScope() {
+        if (on) ctx->prof.end(ctx->stream, b);
+    }
+};
+
+int check_launch(const char* name);
+
+// ---------------------------------------------------------------------------
+// Kernel launchers (one per .hip translation unit).
+// ---------------------------------------------------------------------------
+
+// hx_build.hip
+int launch_scene_box(Context* c, const float* verts, const int32_t* tris, int64_t n, float* d_box6);
+int launch_morton(Context* c, const float* verts, const int32_t* tris, int64_t n, const float* d_box6,
+                  int bits, void* d_keys, bool keys64);
+int launch_construct(Context* c, const void* d_keys, bool keys64, int64_t n, int32_t* left, int32_t* right,
+                     int32_t* parent, int32_t* other, int32_t* counters, int32_t* end_at, int32_t* zero_word);
+struct ClimbArgs {
+    int64_t n;
+    const int32_t* sorted_tri;
+    const float* verts;
+    const int32_t* tris;
+    const int32_t* left;
+    const int32_t* right;
+    const int32_t* parent;
+    const int32_t* other;
+    int32_t* counters;
+    float4* nmin;
+    float4* nmax;
+    float* cost;
+    int32_t* state;
+    int32_t* end_at;
+    float4* woop;
+    int32_t* frontier;        // n entries: nodes whose parent straddles a climb block
+    int32_t* frontier_count;  // zeroed by launch_construct
+    int adaptive;
+    int max_leaf_size;
+    float ct, ci;
+};
+int launch_climb(Context* c, const ClimbArgs& a);
+struct EmitArgs {
+    int64_t n;
+    const int32_t* left;
+    const int32_t* right;
+    const int32_t* parent;
+    const int32_t* other;
+    const int32_t* state;
+    const float4* nmin;
+    const float4* nmax;
+    int32_t* cover;      // in: end_at, out: prefix max (in place)
+    int32_t* leaf_num;   // out
+    int32_t* int_slot;   // out
+    int32_t* counts;     // out: [n_interior, n_leaves]
+    int4* nodes;         // out: Bin nodes
+};
+int launch_collapse_emit(Context* c, const EmitArgs& a);
+int launch_single_ref(Context* c, const float* verts, const int32_t* tris, const int32_t* sorted_tri,
+                      float4* nmin, float4* nmax, float4* woop, int4* nodes, int32_t* counts);
+int launch_cost(Context* c, const int4* nodes, const int32_t* d_counts, int64_t max_nodes,
+                const float* d_box6, float ct, float ci, double* d_out);
+int launch_fill_i32(Context* c, int32_t* p, int32_t v, int64_t n);
+
+// hx_sort.hip
+size_t sort_status_bytes(int64_t n, int end_bit, bool keys64);
+// Stable LSD sort of (key, value) over bits [0, end_bit).  keys/vals ping-pong
+// between [0] and [1]; returns in *result_index which buffer holds the output.
+// vals_identity: the input values are 0..n-1 and are not read.
+int launch_sort_pairs(Context* c, void* keys[2], int32_t* vals[2], int64_t n, int end_bit, bool keys64,
+                      bool vals_identity, int* result_index);
+
+// hx_trace.hip
+struct TraceArgs {
+    int64_t n_rays;
+    int any_hit;
+    const float4* rays;
+    int4* results;
+    const int4* nodes;
+    const float4* woop;
+    const int32_t* tri_idx;
+    int2* stats;            // optional
+    const int32_t* order;   // optional: ray slot -> ray index (sorted trace)
+    int32_t* control;       // [0] ray counter, [1] overflow counter
+};
+int launch_trace(Context* c, const TraceArgs& a);
+int launch_ray_keys(Context* c, const float4* rays, int64_t n, int method, int bits, float ray_length,
+                    uint64_t* keys);
+
+}  // namespace hx

@@ -1,0 +1,287 @@
+// hx_sort.hip -- stable LSD radix sort of (key, int32 value) pairs for gfx950.
+//
+// Replaces src/radix_sort/ (RadixSort.cpp:181-238: 5-bit digits, three
+// launches and a host sync per pass).  Here: 8-bit digits, one histogram pass
+// for all digit positions, then one single-pass "onesweep" kernel per digit:
+//   - tiles of 4096 keys, 4 waves of 64 lanes, 16 keys per lane in
+//     wave-striped order (item i of lane l = i*64 + l within the wave slice);
+//   - ranking by 64-bit ballot match: lanes with the same digit form a peer
+//     mask from nbits ballots; rank = running wave count + peers below;
+//   - decoupled look-back over per-tile digit counts gives each tile its
+//     global digit offsets; tiles take their index from an atomic counter so a
+//     tile only waits on tiles that are already running;
+//   - keys and values are staged in LDS in digit order and written out in
+//     contiguous runs.
+// The order is stable, so with identity input values the result equals a
+// stable sort by key -- the semantics the reference's callers rely on.
+#include "hx_device.h"
+#include "hx_internal.h"
+
+namespace hx {
+
+namespace {
+
+constexpr int kSortBlock = 256;
+constexpr int kSortWaves = kSortBlock / kWave;
+constexpr int kSortItems = 16;
+constexpr int kSortTile = kSortBlock * kSortItems;
+constexpr int kRadix = 256;
+constexpr int kMaxPasses = 8;
+
+constexpr uint32_t kFlagAggregate = 1u << 30;
+constexpr uint32_t kFlagPrefix = 2u << 30;
+constexpr uint32_t kValueMask = (1u << 30) - 1u;
+
+inline int sort_passes(int end_bit) { return (end_bit + 7) / 8; }
+inline int64_t sort_tiles(int64_t n) { return (n + kSortTile - 1) / kSortTile; }
+
+__global__ void k_zero_u32(uint32_t* p, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
+__global__ void k_iota(int32_t* p, int64_t n) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) p[i] = (int32_t)i;
+}
+
+template <class K>
+HX_DEV uint32_t digit_of(K key, int shift, uint32_t mask) {
+    return (uint32_t)(key >> shift) & mask;
+}
+
+// Histograms of every digit position in one read of the keys.
+template <class K>
+__global__ __launch_bounds__(kSortBlock) void k_sort_hist(const K* __restrict__ keys, int64_t n, int passes,
+                                                          int end_bit, uint32_t* __restrict__ hist) {
+    __shared__ uint32_t h[kMaxPasses][kRadix];
+    for (int i = threadIdx.x; i < kMaxPasses * kRadix; i += kSortBlock) (&h[0][0])[i] = 0u;
+    __syncthreads();
+    for (int64_t i = blockIdx.x * (int64_t)kSortBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kSortBlock) {
+        const K key = keys[i];
+        for (int p = 0; p < passes; ++p) {
+            const int shift = 8 * p;
+            const int nb = min(8, end_bit - shift);
+            atomicAdd(&h[p][digit_of(key, shift, (1u << nb) - 1u)], 1u);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < passes * kRadix; i += kSortBlock) {
+        const uint32_t c = (&h[0][0])[i];
+        if (c) atomicAdd(&hist[i], c);
+    }
+}
+
+// Exclusive scan of each pass's 256 digit counts, in place.
+__global__ __launch_bounds__(kRadix) void k_sort_scan_hist(uint32_t* __restrict__ hist) {
+    __shared__ uint32_t wt[kRadix / kWave];
+    uint32_t* h = hist + blockIdx.x * kRadix;
+    const uint32_t v = h[threadIdx.x];
+    uint32_t tot;
+    const uint32_t inc = block_inclusive_scan<kRadix / kWave>(v, [](uint32_t a, uint32_t b) { return a + b; }, wt, tot);
+    h[threadIdx.x] = inc - v;
+}
+
+template <class K, bool IDENT>
+__global__ __launch_bounds__(kSortBlock) void k_onesweep(const K* __restrict__ keys_in, K* __restrict__ keys_out,
+                                                         const int32_t* __restrict__ vals_in,
+                                                         int32_t* __restrict__ vals_out, int64_t n, int shift,
+                                                         int nbits, const uint32_t* __restrict__ digit_offsets,
+                                                         uint32_t* __restrict__ status,
+                                                         uint32_t* __restrict__ tile_counter) {
+    __shared__ uint32_t s_tile;
+    __shared__ uint32_t s_wave_hist[kSortWaves][kRadix];
+    __shared__ uint32_t s_digit_start[kRadix];
+    __shared__ uint32_t s_global_base[kRadix];
+    __shared__ uint32_t s_scan[kSortWaves];
+    __shared__ union {
+        K keys[kSortTile];
+        int32_t vals[kSortTile];
+    } s_stage;
+
+    const int tid = threadIdx.x;
+    const int w = tid / kWave;
+    const int lane = lane_id();
+    const uint32_t mask = (1u << nbits) - 1u;
+
+    if (tid == 0) s_tile = atomicAdd(tile_counter, 1u);
+    for (int i = tid; i < kSortWaves * kRadix; i += kSortBlock) (&s_wave_hist[0][0])[i] = 0u;
+    __syncthreads();
+    const int64_t tile = s_tile;
+    const int64_t tile_base = tile * kSortTile;
+    const int64_t wave_base = tile_base + (int64_t)w * kWave * kSortItems;
+
+    K key[kSortItems];
+    uint32_t pos[kSortItems];
+#pragma unroll
+    for (int i = 0; i < kSortItems; ++i) {
+        const int64_t idx = wave_base + i * kWave + lane;
+        key[i] = idx < n ? keys_in[idx] : (K)0;
+    }
+
+    // Rank within the wave, items in order, lanes in order: stable.
+#pragma unroll
+    for (int i = 0; i < kSortItems; ++i) {
+        const int64_t idx = wave_base + i * kWave + lane;
+        const bool valid = idx < n;
+        const uint32_t d = digit_of(key[i], shift, mask);
+        uint64_t peers = ballot64(valid);
+        for (int b = 0; b < nbits; ++b) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t bb = ballot64(bit);
+            peers &= bit ? bb : ~bb;
+        }
+        if (valid) {
+            const int below = count_below(peers);
+            const uint32_t prev = s_wave_hist[w][d];
+            pos[i] = prev + (uint32_t)below;
+            if (below == 0) s_wave_hist[w][d] = prev + (uint32_t)__popcll(peers);
+        }
+    }
+    __syncthreads();
+
+    // Per digit: exclusive offsets across waves, tile total, tile-local start.
+    const int d = tid;  // kSortBlock == kRadix
+    uint32_t total = 0;
+#pragma unroll
+    for (int ww = 0; ww < kSortWaves; ++ww) {
+        const uint32_t c = s_wave_hist[ww][d];
+        s_wave_hist[ww][d] = total;
+        total += c;
+    }
+    uint32_t tile_tot;
+    const uint32_t inc = block_inclusive_scan<kSortWaves>(total, [](uint32_t a, uint32_t b) { return a + b; }, s_scan,
+                                                          tile_tot);
+    s_digit_start[d] = inc - total;
+
+    // Decoupled look-back for digit d.
+    uint32_t* my = status + tile * kRadix + d;
+    uint32_t excl = 0;
+    if (tile == 0) {
+        atomic_store_relaxed(my, kFlagPrefix | total);
+    } else {
+        atomic_store_relaxed(my, kFlagAggregate | total);
+        int64_t j = tile - 1;
+        while (true) {
+            const uint32_t s = atomic_load_relaxed(status + j * kRadix + d);
+            const uint32_t flag = s & ~kValueMask;
+            if (flag == 0u) {
+                __builtin_amdgcn_s_sleep(1);
+                continue;
+            }
+            excl += s & kValueMask;
+            if (flag == kFlagPrefix) break;
+            --j;
+        }
+        atomic_store_relaxed(my, kFlagPrefix | (excl + total));
+    }
+    s_global_base[d] = digit_offsets[d] + excl;
+    __syncthreads();
+
+    // Stage keys in digit order.
+#pragma unroll
+    for (int i = 0; i < kSortItems; ++i) {
+        const int64_t idx = wave_base + i * kWave + lane;
+        if (idx < n) {
+            const uint32_t dd = digit_of(key[i], shift, mask);
+            pos[i] += s_digit_start[dd] + s_wave_hist[w][dd];
+            s_stage.keys[pos[i]] = key[i];
+        }
+    }
+    __syncthreads();
+
+    const int64_t tile_count = min((int64_t)kSortTile, n - tile_base);
+    int64_t dst[kSortItems];
+#pragma unroll
+    for (int i = 0; i < kSortItems; ++i) {
+        const int slot = i * kSortBlock + tid;
+        if (slot < tile_count) {
+            const K k = s_stage.keys[slot];
+            const uint32_t dd = digit_of(k, shift, mask);
+            dst[i] = (int64_t)s_global_base[dd] + (slot - (int64_t)s_digit_start[dd]);
+            keys_out[dst[i]] = k;
+        }
+    }
+    __syncthreads();
+
+#pragma unroll
+    for (int i = 0; i < kSortItems; ++i) {
+        const int64_t idx = wave_base + i * kWave + lane;
+        if (idx < n) s_stage.vals[pos[i]] = IDENT ? (int32_t)idx : vals_in[idx];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kSortItems; ++i) {
+        const int slot = i * kSortBlock + tid;
+        if (slot < tile_count) vals_out[dst[i]] = s_stage.vals[slot];
+    }
+}
+
+template <class K>
+int sort_impl(Context* c, void* keys[2], int32_t* vals[2], int64_t n, int end_bit, bool vals_identity, int* result) {
+    const int passes = sort_passes(end_bit);
+    *result = 0;
+    if (n <= 1 || passes == 0) {
+        if (vals_identity && n > 0) {
+            hipLaunchKernelGGL(k_iota, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, vals[0], n);
+            HX_TRY(check_launch("iota"));
+        }
+        return HX_OK;
+    }
+    const int64_t ntiles = sort_tiles(n);
+    const int64_t status_words = (int64_t)passes * ntiles * kRadix;
+    const int64_t words = status_words + (int64_t)passes * kRadix + kMaxPasses;
+    HX_TRY(c->sort_status.reserve(sizeof(uint32_t) * (size_t)words));
+    uint32_t* status = c->sort_status.as<uint32_t>();
+    uint32_t* hist = status + status_words;
+    uint32_t* counters = hist + passes * kRadix;
+    {
+        LaunchScope ls(c, "sort_hist");
+        const unsigned zb = (unsigned)std::min<int64_t>((words + 255) / 256, 4096);
+        hipLaunchKernelGGL(k_zero_u32, dim3(zb), dim3(256), 0, c->stream, status, words);
+        HX_TRY(check_launch("sort_zero"));
+        const unsigned hb = (unsigned)std::min<int64_t>((n + kSortBlock * 16 - 1) / (kSortBlock * 16), 2048);
+        hipLaunchKernelGGL(k_sort_hist<K>, dim3(hb), dim3(kSortBlock), 0, c->stream, static_cast<const K*>(keys[0]), n,
+                           passes, end_bit, hist);
+        HX_TRY(check_launch("sort_hist"));
+        hipLaunchKernelGGL(k_sort_scan_hist, dim3(passes), dim3(kRadix), 0, c->stream, hist);
+        HX_TRY(check_launch("sort_scan_hist"));
+    }
+    for (int p = 0; p < passes; ++p) {
+        const int in = p & 1, out = in ^ 1;
+        const int shift = 8 * p;
+        const int nb = std::min(8, end_bit - shift);
+        LaunchScope ls(c, "sort_onesweep");
+        if (p == 0 && vals_identity)
+            hipLaunchKernelGGL((k_onesweep<K, true>), dim3((unsigned)ntiles), dim3(kSortBlock), 0, c->stream,
+                               static_cast<const K*>(keys[in]), static_cast<K*>(keys[out]), (const int32_t*)nullptr,
+                               vals[out], n, shift, nb, hist + p * kRadix, status + (int64_t)p * ntiles * kRadix,
+                               counters + p);
+        else
+            hipLaunchKernelGGL((k_onesweep<K, false>), dim3((unsigned)ntiles), dim3(kSortBlock), 0, c->stream,
+                               static_cast<const K*>(keys[in]), static_cast<K*>(keys[out]), vals[in], vals[out], n,
+                               shift, nb, hist + p * kRadix, status + (int64_t)p * ntiles * kRadix, counters + p);
+        HX_TRY(check_launch("sort_onesweep"));
+    }
+    *result = passes & 1;
+    return HX_OK;
+}
+
+}  // namespace
+
+size_t sort_status_bytes(int64_t n, int end_bit, bool) {
+    const int passes = sort_passes(end_bit);
+    return sizeof(uint32_t) * (size_t)((int64_t)passes * sort_tiles(n) * kRadix + passes * kRadix + kMaxPasses);
+}
+
+int launch_sort_pairs(Context* c, void* keys[2], int32_t* vals[2], int64_t n, int end_bit, bool keys64,
+                      bool vals_identity, int* result_index) {
+    if (n >= (int64_t)1 << 30) {
+        set_error("radix sort: n = %lld exceeds 2^30 - 1", (long long)n);
+        return HX_ERR_INVALID_ARG;
+    }
+    if (keys64) return sort_impl<uint64_t>(c, keys, vals, n, end_bit, vals_identity, result_index);
+    return sort_impl<uint32_t>(c, keys, vals, n, end_bit, vals_identity, result_index);
+}
+
+}  // namespace hx

@@ -1,0 +1,324 @@
+// hx_trace.hip -- persistent while-while traversal of the Bin BVH for gfx950,
+// plus the Morton ray keys used by the sorted trace.
+//
+// Reference (src/hippie/): rt/tracer/HipTracerBinKernels.cu:19-286 (trace),
+// :572-842 (traceSort), rt/tracer/HipTracerKernels.h:29-199,
+// rt/ray/RayBufferKernels.cu:12-56,492-614.
+//
+// Wave64 design: each wave fetches rays for its terminated lanes with one
+// atomicAdd (64-bit ballot + mbcnt), traverses with the Aila-Laine
+// while-while loop, postpones one leaf per lane, and refills when fewer than
+// kDynamicFetch of its 64 lanes are still busy.
+#include "hx_device.h"
+#include "hx_internal.h"
+
+namespace hx {
+
+namespace {
+
+constexpr int kTraceBlock = 256;
+constexpr int kStackSize = 64;                 // HipTracerBinKernels.cu:5
+constexpr int kDynamicFetch = 40;              // of 64 lanes (reference: 20 of 32)
+constexpr int32_t kSentinel = 0x76543210;      // HipTracerKernels.h:11
+
+// fminf/fmaxf written as selects, and the integer min/max on float bits of
+// HipTracerKernels.h:144-195 (spanBeginKepler / spanEndKepler).
+HX_DEV float fmin_s(float a, float b) { return (b < a) ? b : a; }
+HX_DEV float fmax_s(float a, float b) { return (b > a) ? b : a; }
+HX_DEV float span_begin(float a0, float a1, float b0, float b1, float c0, float c1, float d) {
+    const int c = max(min(__float_as_int(c0), __float_as_int(c1)), __float_as_int(d));
+    return __int_as_float(max(max(__float_as_int(fmin_s(a0, a1)), __float_as_int(fmin_s(b0, b1))), c));
+}
+HX_DEV float span_end(float a0, float a1, float b0, float b1, float c0, float c1, float d) {
+    const int c = min(max(__float_as_int(c0), __float_as_int(c1)), __float_as_int(d));
+    return __int_as_float(min(min(__float_as_int(fmax_s(a0, a1)), __float_as_int(fmax_s(b0, b1))), c));
+}
+
+template <bool STATS, bool ORDERED>
+__global__ __launch_bounds__(kTraceBlock) void k_trace(TraceArgs a) {
+    int32_t stack[kStackSize];
+    int sp = 0;
+    float ox = 0.f, oy = 0.f, oz = 0.f, tmin = 0.f, dx = 0.f, dy = 0.f, dz = 0.f;
+    float idx = 0.f, idy = 0.f, idz = 0.f, oodx = 0.f, oody = 0.f, oodz = 0.f;
+    float hitT = 0.f, hitU = 0.f, hitV = 0.f;
+    int32_t nodeAddr = kSentinel, leafAddr = 0, hitIndex = -1, hitId = -1;
+    int64_t rayidx = 0;
+    int32_t nNodes = 0, nTris = 0;
+    int32_t* rayCounter = a.control;
+
+    while (true) {
+        // Fetch new rays for the terminated lanes of this wave.
+        const bool terminated = nodeAddr == kSentinel;
+        const uint64_t tmask = ballot64(terminated);
+        const int ntm = __popcll(tmask);
+        const int below = count_below(tmask);
+        const int leader = __ffsll((unsigned long long)tmask) - 1;
+        int base = 0;
+        if (terminated && below == 0) base = atomicAdd(rayCounter, ntm);
+        base = __shfl(base, leader, kWave);
+        if (terminated) {
+            rayidx = (int64_t)base + below;
+            if (rayidx >= a.n_rays) break;
+            if (ORDERED) rayidx = a.order[rayidx];
+            const float4 o = a.rays[2 * rayidx + 0];
+            const float4 d = a.rays[2 * rayidx + 1];
+            ox = o.x; oy = o.y; oz = o.z; tmin = o.w;
+            dx = d.x; dy = d.y; dz = d.z; hitT = d.w;
+            const float ooeps = 8.2718061255302767e-25f;  // exp2f(-80)
+            idx = 1.0f / (fabsf(dx) > ooeps ? dx : copysignf(ooeps, dx));
+            idy = 1.0f / (fabsf(dy) > ooeps ? dy : copysignf(ooeps, dy));
+            idz = 1.0f / (fabsf(dz) > ooeps ? dz : copysignf(ooeps, dz));
+            oodx = ox * idx;
+            oody = oy * idy;
+            oodz = oz * idz;
+            stack[0] = kSentinel;
+            sp = 0;
+            leafAddr = 0;
+            nodeAddr = 0;
+            hitIndex = -1;
+            hitId = -1;
+            if (STATS) { nNodes = 0; nTris = 0; }
+        }
+
+        while (nodeAddr != kSentinel) {
+            while ((uint32_t)nodeAddr < (uint32_t)kSentinel) {
+                const float4* np = reinterpret_cast<const float4*>(a.nodes + 4 * (int64_t)nodeAddr);
+                const float4 n0xy = np[0];
+                const float4 n1xy = np[1];
+                const float4 nz = np[2];
+                const int4 cn = a.nodes[4 * (int64_t)nodeAddr + 3];
+                if (STATS) ++nNodes;
+                const float c0lox = fmaf(n0xy.x, idx, -oodx);
+                const float c0hix = fmaf(n0xy.y, idx, -oodx);
+                const float c0loy = fmaf(n0xy.z, idy, -oody);
+                const float c0hiy = fmaf(n0xy.w, idy, -oody);
+                const float c0loz = fmaf(nz.x, idz, -oodz);
+                const float c0hiz = fmaf(nz.y, idz, -oodz);
+                const float c1loz = fmaf(nz.z, idz, -oodz);
+                const float c1hiz = fmaf(nz.w, idz, -oodz);
+                const float c0min = span_begin(c0lox, c0hix, c0loy, c0hiy, c0loz, c0hiz, tmin);
+                const float c0max = span_end(c0lox, c0hix, c0loy, c0hiy, c0loz, c0hiz, hitT);
+                const float c1lox = fmaf(n1xy.x, idx, -oodx);
+                const float c1hix = fmaf(n1xy.y, idx, -oodx);
+                const float c1loy = fmaf(n1xy.z, idy, -oody);
+                const float c1hiy = fmaf(n1xy.w, idy, -oody);
+                const float c1min = span_begin(c1lox, c1hix, c1loy, c1hiy, c1loz, c1hiz, tmin);
+                const float c1max = span_end(c1lox, c1hix, c1loy, c1hiy, c1loz, c1hiz, hitT);
+                const bool swp = c1min < c0min;
+                const bool t0 = c0max >= c0min;
+                const bool t1 = c1max >= c1min;
+                int32_t cy = cn.y;
+                if (!t0 && !t1) {
+                    nodeAddr = stack[sp--];
+                } else {
+                    nodeAddr = t0 ? cn.x : cy;
+                    if (t0 && t1) {
+                        if (swp) {
+                            const int32_t tmp = nodeAddr;
+                            nodeAddr = cy;
+                            cy = tmp;
+                        }
+                        if (sp + 1 >= kStackSize) {
+                            atomicAdd(a.control + 1, 1);
+                            nodeAddr = kSentinel;
+                            leafAddr = 0;
+                            break;
+                        }
+                        stack[++sp] = cy;
+                    }
+                }
+                if (nodeAddr < 0 && leafAddr >= 0) {
+                    leafAddr = nodeAddr;
+                    nodeAddr = stack[sp--];
+                }
+                if (!__any(leafAddr >= 0)) break;
+            }
+
+            while (leafAddr < 0) {
+                const int4 lb = a.nodes[4 * (int64_t)(~leafAddr) + 3];
+                for (int tri = lb.x; tri < lb.y; ++tri) {
+                    if (STATS) ++nTris;
+                    const float4 v00 = a.woop[3 * (int64_t)tri + 0];
+                    const float Oz = fmaf(-oz, v00.z, fmaf(-oy, v00.y, fmaf(-ox, v00.x, v00.w)));
+                    const float den = fmaf(dz, v00.z, fmaf(dy, v00.y, dx * v00.x));
+                    const float t = Oz * (1.0f / den);
+                    if (t > tmin && t <= hitT) {
+                        const float4 v11 = a.woop[3 * (int64_t)tri + 1];
+                        const float Ox = fmaf(oz, v11.z, fmaf(oy, v11.y, fmaf(ox, v11.x, v11.w)));
+                        const float Dx = fmaf(dz, v11.z, fmaf(dy, v11.y, dx * v11.x));
+                        const float u = fmaf(t, Dx, Ox);
+                        if (u >= 0.0f) {
+                            const float4 v22 = a.woop[3 * (int64_t)tri + 2];
+                            const float Oy = fmaf(oz, v22.z, fmaf(oy, v22.y, fmaf(ox, v22.x, v22.w)));
+                            const float Dy = fmaf(dz, v22.z, fmaf(dy, v22.y, dx * v22.x));
+                            const float v = fmaf(t, Dy, Oy);
+                            if (v >= 0.0f && u + v <= 1.0f) {
+                                const int32_t id = a.tri_idx[tri];
+                                // Closest hit; equal t resolves to the smaller triangle id.
+                                if (t < hitT || id < hitId) {
+                                    hitT = t;
+                                    hitIndex = tri;
+                                    hitId = id;
+                                    hitU = u;
+                                    hitV = v;
+                                    if (a.any_hit) {
+                                        nodeAddr = kSentinel;
+                                        break;
+                                    }
+                                }
+                            }
+                        }
+                    }
+                }
+                leafAddr = nodeAddr;
+                if (nodeAddr < 0) nodeAddr = stack[sp--];
+            }
+
+            if (__popcll(ballot64(true)) < kDynamicFetch) break;
+        }
+
+        if (nodeAddr == kSentinel) {
+            if (hitIndex == -1)
+                reinterpret_cast<int2*>(a.results)[2 * rayidx] = make_int2(-1, __float_as_int(hitT));
+            else
+                a.results[rayidx] =
+                    make_int4(hitId, __float_as_int(hitT), __float_as_int(hitU), __float_as_int(hitV));
+            if (STATS) a.stats[rayidx] = make_int2(nNodes, nTris);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Ray Morton keys for the [0,1]^3 ray box (RayBuffer.cpp:63-66).
+// ---------------------------------------------------------------------------
+
+HX_DEV float fmin_n(float a, float b) { return (a != a) ? b : (b != b) ? a : ((b < a) ? b : a); }
+HX_DEV float fmax_n(float a, float b) { return (a != a) ? b : (b != b) ? a : ((b > a) ? b : a); }
+
+__global__ void k_ray_keys_aila(const float4* __restrict__ rays, int64_t n, int bits, uint64_t* __restrict__ keys) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 o = rays[2 * i], d = rays[2 * i + 1];
+    const float ax = (o.x - 0.0f) / (1.0f - 0.0f), ay = (o.y - 0.0f) / (1.0f - 0.0f), az = (o.z - 0.0f) / (1.0f - 0.0f);
+    float ls = 0.0f;
+    ls += d.x * d.x;
+    ls += d.y * d.y;
+    ls += d.z * d.z;
+    const float len = sqrtf(ls);
+    const float rl = (len != 0.0f) ? 1.0f / len : 0.0f;
+    const float bx = (d.x * rl + 1.0f) * 0.5f, by = (d.y * rl + 1.0f) * 0.5f, bz = (d.z * rl + 1.0f) * 0.5f;
+    const int iax = f2i_sat(ax * 8191.0f), iay = f2i_sat(ay * 8191.0f), iaz = f2i_sat(az * 8191.0f);
+    const int ibx = f2i_sat(bx * 8191.0f), iby = f2i_sat(by * 8191.0f), ibz = f2i_sat(bz * 8191.0f);
+    uint64_t code = 0;
+    code |= (uint64_t)((iax >> 12) & 1) << 63;
+    code |= (uint64_t)((iay >> 12) & 1) << 62;
+    code |= (uint64_t)((iaz >> 12) & 1) << 61;
+    code |= (uint64_t)((iax >> 11) & 1) << 60;
+    code |= (uint64_t)((iay >> 11) & 1) << 59;
+    code |= (uint64_t)((iaz >> 11) & 1) << 58;
+    code |= (uint64_t)((iax >> 10) & 1) << 57;
+    code |= (uint64_t)((iay >> 10) & 1) << 56;
+    code |= (uint64_t)((iaz >> 10) & 1) << 55;
+#pragma unroll
+    for (int k = 9; k >= 1; --k) {
+        code |= (uint64_t)((iax >> k) & 1) << (6 * k + 0);
+        code |= (uint64_t)((iay >> k) & 1) << (6 * k - 1);
+        code |= (uint64_t)((iaz >> k) & 1) << (6 * k - 2);
+    }
+#pragma unroll
+    for (int k = 12; k >= 4; --k) {
+        code |= (uint64_t)((ibx >> k) & 1) << (6 * k - 21);
+        code |= (uint64_t)((iby >> k) & 1) << (6 * k - 22);
+        code |= (uint64_t)((ibz >> k) & 1) << (6 * k - 23);
+    }
+    keys[i] = code >> (64 - bits);
+}
+
+__global__ void k_ray_keys_twopoint(const float4* __restrict__ rays, int64_t n, int bits, float ray_length,
+                                    uint64_t* __restrict__ keys) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 ro = rays[2 * i], rd = rays[2 * i + 1];
+    const float o[3] = {ro.x, ro.y, ro.z}, d[3] = {rd.x, rd.y, rd.z};
+    float tmn[3], tmx[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (d[k] >= 0.0f) {
+            tmn[k] = (0.0f - o[k]) / d[k];
+            tmx[k] = (1.0f - o[k]) / d[k];
+        } else {
+            tmn[k] = (1.0f - o[k]) / d[k];
+            tmx[k] = (0.0f - o[k]) / d[k];
+        }
+    }
+    const float t0 = fmax_n(tmn[2], fmax_n(tmn[0], tmn[1]));
+    const float t1 = fmin_n(tmx[2], fmin_n(tmx[0], tmx[1]));
+    const float boxT = (t0 > t1) ? -1.0f : (1.0f - 1.0e-3f) * (t0 >= 0.0f ? t0 : t1);
+    float len = ray_length;
+    len = fmin_n(len, fmax_n(rd.w, 0.0f));
+    len = fmin_n(len, boxT);
+    int ia[3], ib[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float av = (o[k] - 0.0f) * (1.0f / (1.0f - 0.0f));
+        const float bv = ((o[k] + len * d[k]) - 0.0f) * (1.0f / (1.0f - 0.0f));
+        ia[k] = f2i_sat(av * 8191.0f);
+        ib[k] = f2i_sat(bv * 8191.0f);
+    }
+    uint64_t code = 0;
+#pragma unroll
+    for (int k = 12; k >= 2; --k) {
+        code |= (uint64_t)((ia[0] >> k) & 1) << (6 * k - 9);
+        code |= (uint64_t)((ia[1] >> k) & 1) << (6 * k - 10);
+        code |= (uint64_t)((ia[2] >> k) & 1) << (6 * k - 11);
+    }
+#pragma unroll
+    for (int k = 12; k >= 3; --k) {
+        code |= (uint64_t)((ib[0] >> k) & 1) << (6 * k - 12);
+        code |= (uint64_t)((ib[1] >> k) & 1) << (6 * k - 13);
+        code |= (uint64_t)((ib[2] >> k) & 1) << (6 * k - 14);
+    }
+    code |= (uint64_t)(ib[0] & 1);
+    keys[i] = code >> (64 - bits);
+}
+
+__global__ void k_zero_control(int32_t* control) {
+    if (threadIdx.x < 2) control[threadIdx.x] = 0;
+}
+
+template <bool STATS, bool ORDERED>
+int launch_trace_t(Context* c, const TraceArgs& a) {
+    int per_cu = 0;
+    HX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<STATS, ORDERED>, kTraceBlock, 0));
+    per_cu = std::max(per_cu, 1);
+    const int64_t want = (a.n_rays + kTraceBlock - 1) / kTraceBlock;
+    const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>((int64_t)c->cu_count * per_cu, want));
+    hipLaunchKernelGGL((k_trace<STATS, ORDERED>), dim3(blocks), dim3(kTraceBlock), 0, c->stream, a);
+    return check_launch("trace");
+}
+
+}  // namespace
+
+int launch_trace(Context* c, const TraceArgs& a) {
+    if (a.n_rays <= 0) return HX_OK;
+    hipLaunchKernelGGL(k_zero_control, dim3(1), dim3(64), 0, c->stream, a.control);
+    HX_TRY(check_launch("zero_control"));
+    LaunchScope ls(c, a.order ? "trace_sorted" : "trace");
+    if (a.stats)
+        return a.order ? launch_trace_t<true, true>(c, a) : launch_trace_t<true, false>(c, a);
+    return a.order ? launch_trace_t<false, true>(c, a) : launch_trace_t<false, false>(c, a);
+}
+
+int launch_ray_keys(Context* c, const float4* rays, int64_t n, int method, int bits, float ray_length,
+                    uint64_t* keys) {
+    if (n <= 0) return HX_OK;
+    LaunchScope ls(c, "ray_keys");
+    const unsigned g = (unsigned)((n + 255) / 256);
+    if (method == HX_RAYKEY_AILA)
+        hipLaunchKernelGGL(k_ray_keys_aila, dim3(g), dim3(256), 0, c->stream, rays, n, bits, keys);
+    else
+        hipLaunchKernelGGL(k_ray_keys_twopoint, dim3(g), dim3(256), 0, c->stream, rays, n, bits, ray_length, keys);
+    return check_launch("ray_keys");
+}
+
+}  // namespace hx

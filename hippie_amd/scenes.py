@@ -1,0 +1,177 @@
+"""Seeded synthetic scenes and rays for the benchmark configurations.
+
+Harness counterpart of the reference's scene loading and ray generation
+(src/hippie/rt/scene/SceneLoader.cpp, rt/ray/RayGenKernels.cu:106-153,232-349):
+geometry and rays are produced once on the host, in float32, from numpy's
+PCG64 generator, and fed identically to the GPU path and to the oracle.
+"""
+import math
+
+import numpy as np
+
+
+def _u(rng, shape, lo=0.0, hi=1.0):
+    return (np.float32(lo) + np.float32(hi - lo) * rng.random(shape, dtype=np.float32)).astype(np.float32)
+
+
+def soup_triangles(centroids, rng, smin=0.001, smax=0.02):
+    """Triangle soup around centroids: v_k = c + s (u_k - 0.5), s ~ U[smin, smax]."""
+    n = centroids.shape[0]
+    s = _u(rng, (n, 1, 1), smin, smax)
+    u = rng.random((n, 3, 3), dtype=np.float32)
+    v = centroids[:, None, :].astype(np.float32) + s * (u - np.float32(0.5))
+    verts = v.reshape(-1, 3).astype(np.float32)
+    tris = np.arange(3 * n, dtype=np.int32).reshape(n, 3)
+    return verts, tris
+
+
+def uniform_soup(n, seed=1):
+    """Config 1 geometry: n triangles, centroids U[0,1]^3, edge scale U[0.001,0.02]."""
+    rng = np.random.default_rng(seed)
+    c = rng.random((n, 3), dtype=np.float32)
+    return soup_triangles(c, rng)
+
+
+def gaussian_mixture_soup(n, components=64, seed=2):
+    """Triangles whose centroids come from a Gaussian mixture (mu ~ U[0,1]^3, sigma ~ U[0.01,0.1])."""
+    rng = np.random.default_rng(seed)
+    mu = rng.random((components, 3), dtype=np.float32)
+    sigma = _u(rng, (components,), 0.01, 0.1)
+    comp = rng.integers(0, components, n)
+    c = mu[comp] + sigma[comp][:, None] * rng.standard_normal((n, 3), dtype=np.float32)
+    return soup_triangles(c.astype(np.float32), rng)
+
+
+def ground_grid(cells):
+    """cells x cells quads (2 triangles each) at y = 0 over [0,1]^2 in x, z; shared vertices."""
+    g = np.linspace(0.0, 1.0, cells + 1, dtype=np.float32)
+    xx, zz = np.meshgrid(g, g, indexing="xy")
+    verts = np.stack([xx.ravel(), np.zeros(xx.size, np.float32), zz.ravel()], axis=1).astype(np.float32)
+    i = np.arange(cells, dtype=np.int32)
+    ix, iz = np.meshgrid(i, i, indexing="xy")
+    v00 = (iz * (cells + 1) + ix).ravel()
+    v10 = v00 + 1
+    v01 = v00 + (cells + 1)
+    v11 = v01 + 1
+    t0 = np.stack([v00, v10, v11], axis=1)
+    t1 = np.stack([v00, v11, v01], axis=1)
+    tris = np.stack([t0, t1], axis=1).reshape(-1, 3).astype(np.int32)
+    return verts, tris
+
+
+def merge(*meshes):
+    verts, tris, off = [], [], 0
+    for v, t in meshes:
+        verts.append(v)
+        tris.append(t + off)
+        off += v.shape[0]
+    return np.concatenate(verts).astype(np.float32), np.concatenate(tris).astype(np.int32)
+
+
+def config2_scene(n_mixture=8388608, grid_cells=2048, seed=2):
+    """Config 2: Gaussian-mixture soup plus a ground-plane grid (n = n_mixture + 2 cells^2)."""
+    return merge(gaussian_mixture_soup(n_mixture, seed=seed), ground_grid(grid_cells))
+
+
+def _camera_basis(forward, up=(0.0, 1.0, 0.0)):
+    f = np.asarray(forward, np.float64)
+    f = f / np.linalg.norm(f)
+    r = np.cross(np.asarray(up, np.float64), f)
+    if np.linalg.norm(r) < 1e-8:
+        r = np.cross((1.0, 0.0, 0.0), f)
+    r = r / np.linalg.norm(r)
+    u = np.cross(f, r)
+    return r, u, f
+
+
+def pinhole_rays(width, height, origin, forward, fov_y_deg=60.0, tmin=0.0, tmax=3.0):
+    """Primary rays through pixel corners (Halton sample 0, RayGenKernels.cu:128-132).
+
+    Returns float32 (width*height, 8): origin, tmin, normalised direction, tmax;
+    ray index = y * width + x.
+    """
+    r, u, f = _camera_basis(forward)
+    tan = math.tan(math.radians(fov_y_deg) * 0.5)
+    aspect = width / height
+    x = np.arange(width, dtype=np.float64)
+    y = np.arange(height, dtype=np.float64)
+    xx, yy = np.meshgrid(x, y, indexing="xy")
+    nx = (2.0 * xx / width - 1.0) * tan * aspect
+    ny = (2.0 * yy / height - 1.0) * tan
+    d = nx[..., None] * r + ny[..., None] * u + f
+    d = d / np.linalg.norm(d, axis=-1, keepdims=True)
+    n = width * height
+    rays = np.empty((n, 8), np.float32)
+    rays[:, 0:3] = np.asarray(origin, np.float32)
+    rays[:, 3] = tmin
+    rays[:, 4:7] = d.reshape(n, 3).astype(np.float32)
+    rays[:, 7] = tmax
+    return rays
+
+
+def config1_rays(width=1024, height=1024):
+    """Config 1: pinhole at (0.5,0.5,-1.5) looking +z, 60 deg vertical FOV, tmax 3.0."""
+    return pinhole_rays(width, height, (0.5, 0.5, -1.5), (0.0, 0.0, 1.0), 60.0, 0.0, 3.0)
+
+
+def config2_primary_rays(width=4096, height=4096):
+    """Config 2 primaries: camera inside the unit box (the Aila key assumes [0,1]^3 origins)."""
+    return pinhole_rays(width, height, (0.5, 0.6, 0.02), (0.0, -0.35, 1.0), 60.0, 0.0, 3.0)
+
+
+def secondary_rays(verts, tris, rays, results, seed=3, offset=1e-4, tmax=3.0):
+    """Cosine-hemisphere diffuse rays from primary hits (A20 conventions).
+
+    origin = hit - offset * direction; the geometric normal is flipped to face
+    the incoming ray; misses become dead rays (tmax = -1).
+    """
+    rng = np.random.default_rng(seed)
+    n = rays.shape[0]
+    ids = results[:, 0]
+    hit = ids >= 0
+    t = results[:, 1].view(np.float32)
+    o = rays[:, 0:3].astype(np.float64)
+    d = rays[:, 4:7].astype(np.float64)
+    out = np.empty((n, 8), np.float32)
+    out[:] = rays
+    out[~hit, 7] = -1.0
+    idx = np.nonzero(hit)[0]
+    tri = tris[ids[idx]]
+    v0, v1, v2 = verts[tri[:, 0]].astype(np.float64), verts[tri[:, 1]].astype(np.float64), verts[tri[:, 2]].astype(
+        np.float64)
+    nrm = np.cross(v1 - v0, v2 - v0)
+    nrm /= np.maximum(np.linalg.norm(nrm, axis=1, keepdims=True), 1e-30)
+    flip = np.sum(nrm * d[idx], axis=1) > 0
+    nrm[flip] *= -1.0
+    p = o[idx] + (t[idx].astype(np.float64) - offset)[:, None] * d[idx]
+    r1 = rng.random(idx.size)
+    r2 = rng.random(idx.size)
+    phi = 2.0 * np.pi * r1
+    sr = np.sqrt(r2)
+    lx, ly, lz = sr * np.cos(phi), sr * np.sin(phi), np.sqrt(np.maximum(0.0, 1.0 - r2))
+    a = np.where(np.abs(nrm[:, 0:1]) > 0.9, np.array([[0.0, 1.0, 0.0]]), np.array([[1.0, 0.0, 0.0]]))
+    tx = np.cross(a, nrm)
+    tx /= np.linalg.norm(tx, axis=1, keepdims=True)
+    ty = np.cross(nrm, tx)
+    dirs = lx[:, None] * tx + ly[:, None] * ty + lz[:, None] * nrm
+    out[idx, 0:3] = p.astype(np.float32)
+    out[idx, 3] = 0.0
+    out[idx, 4:7] = dirs.astype(np.float32)
+    out[idx, 7] = tmax
+    return out
+
+
+def random_rays(n, seed=5, box=(0.0, 1.0), tmax=3.0):
+    """Rays from uniform origins around the unit box aimed at uniform targets inside it."""
+    rng = np.random.default_rng(seed)
+    lo, hi = box
+    o = rng.uniform(lo - 0.5, hi + 0.5, (n, 3))
+    tgt = rng.uniform(lo, hi, (n, 3))
+    d = tgt - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.empty((n, 8), np.float32)
+    rays[:, 0:3] = o
+    rays[:, 3] = 0.0
+    rays[:, 4:7] = d
+    rays[:, 7] = tmax
+    return rays

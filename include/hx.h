@@ -1,0 +1,197 @@
+/*
+ * hx.h -- C ABI of libhx, the MI355X (gfx950) BVH construction and ray
+ * traversal core.  Plain C: no HIP or torch types in any signature; device
+ * pointers are passed as plain pointers, sizes as int64_t.
+ *
+ * Every entry point returns an int status (HX_OK == 0) and never exits the
+ * process.  hx_last_error() gives a message for the calling thread.
+ *
+ * Reference interfaces replaced (paths relative to src/hippie/ of hippie):
+ *   BVHBuilder::build / rebuild            rt/bvh/BVHBuilder.h:37-39
+ *   LBVHBuilder::build / rebuild           rt/bvh/LBVHBuilder.cpp:183-263
+ *   HipBVH (nodes, Woop triangles, triangle indices, getCost, update)
+ *                                          rt/bvh/HipBVH.h:19-106
+ *   HipTracer::setBVH / trace / traceStats / traceSort
+ *                                          rt/tracer/HipTracer.h:16-44
+ *   TRACE_FUNC kernel ABI                  rt/tracer/HipTracerKernels.h:29-39
+ *   RadixSort::sort                        radix_sort/RadixSort.h:64-87
+ *   RayBuffer::mortonSort / computeMortonCodes
+ *                                          rt/ray/RayBuffer.cpp:31-131
+ *
+ * Conventions (differences from the reference are deliberate):
+ *   - one context per host thread; calls are asynchronous on the context's
+ *     stream unless they return host data (counts, cost, timings);
+ *   - geometry, rays and results are caller-owned device memory; BVH storage
+ *     (nodes, Woop triangles, triangle indices) is owned by the hx_bvh;
+ *   - sizes are 64-bit; the number of references per build is < 2^30.
+ *
+ * Memory layouts (identical to the reference):
+ *   vertices   float[3*V]   (Vec3f)
+ *   triangles  int32[3*F]   (Vec3i)
+ *   Bin node   64 bytes     int32[16]: words 0-11 child boxes
+ *              (c0.lo.x,c0.hi.x,c0.lo.y,c0.hi.y, c1.lo.x,c1.hi.x,c1.lo.y,c1.hi.y,
+ *               c0.lo.z,c0.hi.z,c1.lo.z,c1.hi.z), 12/13 child indices (leaf
+ *              child = ~leafNodeIndex) or, in a leaf, begin/end, 14 size (leaf:
+ *              ~size), 15 parent.  Interior nodes first (root = 0), then leaves.
+ *              (rt/bvh/HipBVHNode.h:117-216)
+ *   Woop       48 bytes per reference slot: 3 x float4 (HipBVHKernels.cu:59-61)
+ *   ray        32 bytes: float4 (origin, tmin), float4 (direction, tmax)
+ *   result     16 bytes: int4 (triangle id, t bits, u bits, v bits); a miss
+ *              writes only (-1, tmax) to the first 8 bytes (HipTracerKernels.h:109)
+ *   stats      8 bytes: int2 (traversed nodes, tested triangles)
+ */
+#ifndef HX_H
+#define HX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------- */
+#define HX_OK 0
+#define HX_ERR_INVALID_ARG 1
+#define HX_ERR_HIP 2
+#define HX_ERR_OUT_OF_MEMORY 3
+#define HX_ERR_UNSUPPORTED 4
+#define HX_ERR_STACK_OVERFLOW 5
+
+/* ---- builder methods: the Bvh.method strings of Benchmark.cpp:567-589 ---- */
+#define HX_METHOD_LBVH 0
+#define HX_METHOD_HLBVH 1
+#define HX_METHOD_PLOC 2
+#define HX_METHOD_ATRBVH 3
+#define HX_METHOD_INSERTION 4
+
+#define HX_LAYOUT_BIN 2
+
+/* Ray key methods: RayBuffer::MortonCodeMethod (rt/ray/RayBuffer.h). */
+#define HX_RAYKEY_TWOPOINT 0
+#define HX_RAYKEY_AILA 1
+
+typedef struct hx_context hx_context;
+typedef struct hx_bvh hx_bvh;
+
+/* Builder parameters.  Field names follow the Bvh.* env keys of
+ * environment/AppEnvironment.cpp:52-75; defaults from hx_build_params_default
+ * equal the registered defaults there. */
+typedef struct hx_build_params {
+    int32_t method;              /* Bvh.method                                */
+    int32_t morton_bits;         /* Bvh.{lbvh,ploc,atr,...}MortonCodeBits [6,60] */
+    int32_t max_leaf_size;       /* Bvh.maxLeafSize (0,1024]                  */
+    int32_t adaptive_leaf_size;  /* Bvh.adaptiveLeafSize                      */
+    float ct;                    /* Bvh.ct                                    */
+    float ci;                    /* Bvh.ci                                    */
+    int32_t presplitting;        /* Bvh.presplitting                          */
+    float presplitter_beta;      /* Bvh.presplitterBeta                       */
+    int32_t ploc_radius;         /* Bvh.plocRadius                            */
+    int32_t ploc_max_radius;     /* Bvh.plocMaxRadius                         */
+    int32_t ploc_adaptive;       /* Bvh.plocAdaptive                          */
+    int32_t atr_iterations;      /* Bvh.atrIterations [0,25]                  */
+    int32_t atr_treelet_size;    /* ATRBuilder::setTreeletSize [4,30]         */
+    int32_t hlbvh_sah_bits;      /* Bvh.hlbvhMortonCodeSAHBits                */
+    int32_t insertion_mod;       /* Bvh.insertionMod                          */
+    int32_t layout;              /* Bvh.layout (HX_LAYOUT_BIN)                */
+    int32_t use_scene_box;       /* 1: scene_box below is the Morton frame    */
+    float scene_box[6];          /* mn.xyz, mx.xyz                            */
+    int32_t keep_debug;          /* 1: keep keys, binary arrays for hx_bvh_debug */
+} hx_build_params;
+
+/* Device pointers and counts of a built BVH (HipBVH accessors). */
+typedef struct hx_bvh_view {
+    void* nodes;             /* Bin nodes, 64 B each                          */
+    void* woop;              /* 48 B per reference slot                       */
+    int32_t* tri_indices;    /* reference slot -> triangle id                 */
+    int64_t n_interior;
+    int64_t n_leaves;
+    int64_t n_references;
+    float scene_box[6];
+} hx_bvh_view;
+
+/* Intermediate arrays of the last build (requires keep_debug), for parity
+ * checks against the reference's own buffers (LBVHBuilder.h members). */
+typedef struct hx_bvh_debug_view {
+    void* sorted_keys;       /* uint64 or uint32 per sorted reference          */
+    int32_t keys_are_64bit;
+    int32_t* sorted_tri;     /* triangle id per sorted position                */
+    int32_t* left;           /* 2n-1, Karras indexing (leaves n-1..2n-2)        */
+    int32_t* right;
+    int32_t* parent;
+    float* node_min;         /* float4[2n-1]                                   */
+    float* node_max;
+    int32_t* state;          /* n-1 collapse states of internal nodes (0 leaf) */
+} hx_bvh_debug_view;
+
+/* ---- errors --------------------------------------------------------------- */
+const char* hx_status_string(int status);
+const char* hx_last_error(void);
+
+/* ---- context (HipModule::staticInit, HipModule.cpp:121-149) ------------------ */
+int hx_context_create(int device, hx_context** out);
+int hx_context_destroy(hx_context* ctx);
+/* Use an existing hipStream_t (passed as void*); NULL = the null stream. */
+int hx_context_set_stream(hx_context* ctx, void* stream);
+int hx_context_synchronize(hx_context* ctx);
+/* Per-launch event timing (HipKernel::launchTimed, HipKernel.cpp:296-325, but
+ * with no synchronisation between launches). */
+int hx_context_set_profiling(hx_context* ctx, int enabled);
+int hx_context_profile_count(hx_context* ctx, int32_t* count);
+int hx_context_profile_entry(hx_context* ctx, int32_t index, char* name, int32_t name_len,
+                             int64_t* launches, double* total_ms);
+int hx_context_profile_reset(hx_context* ctx);
+
+/* ---- builders (BVHBuilder.h:37-39) ---------------------------------------- */
+int hx_build_params_default(hx_build_params* p);
+/* BVHBuilder::build: allocates a new BVH owned by the caller (hx_bvh_destroy). */
+int hx_build(hx_context* ctx, const hx_build_params* params, const float* d_vertices, int64_t n_vertices,
+             const int32_t* d_triangles, int64_t n_triangles, hx_bvh** out);
+/* BVHBuilder::rebuild: rebuild in place from (possibly moved) geometry. */
+int hx_rebuild(hx_context* ctx, hx_bvh* bvh, const float* d_vertices, const int32_t* d_triangles);
+/* HipBVH::update (HipBVH.cpp:472-477): refit the final layout and re-woopify,
+ * topology unchanged. */
+int hx_refit(hx_context* ctx, hx_bvh* bvh, const float* d_vertices, const int32_t* d_triangles);
+int hx_bvh_destroy(hx_bvh* bvh);
+/* Synchronises on first call after a build to read the node counts. */
+int hx_bvh_get_view(hx_bvh* bvh, hx_bvh_view* out);
+int hx_bvh_debug(hx_bvh* bvh, hx_bvh_debug_view* out);
+/* HipBVH::getCost (HipBVH.cpp:346-370): SAH cost, FP64 deterministic sum. */
+int hx_bvh_cost(hx_bvh* bvh, double* out);
+
+/* ---- stages (the reference's individually launched kernels) --------------- */
+/* Scene box: exact min/max over the triangles' vertices (SceneLoader.cpp:143-153).
+ * d_box6 receives mn.xyz, mx.xyz on the device. */
+int hx_scene_box(hx_context* ctx, const float* d_vertices, const int32_t* d_triangles, int64_t n,
+                 float* d_box6);
+/* computeMortonCodes60 (LBVHBuilderKernels.cu:131-160): code >> (60 - bits),
+ * uint64 per triangle in triangle order.  d_box6: scene box on the device. */
+int hx_morton_codes(hx_context* ctx, const float* d_vertices, const int32_t* d_triangles, int64_t n,
+                    const float* d_box6, int32_t bits, uint64_t* d_keys);
+/* RadixSort::sort (RadixSort.cpp:181-238): stable sort of (uint64 key, int32
+ * value) over bits [0, end_bit), in place. */
+int hx_sort_pairs_u64(hx_context* ctx, uint64_t* d_keys, int32_t* d_values, int64_t n, int32_t end_bit);
+
+/* ---- trace (HipTracer.h:16-44, TRACE_FUNC) ---------------------------------- */
+int hx_trace(hx_context* ctx, hx_bvh* bvh, const float* d_rays, int64_t n_rays, int32_t any_hit,
+             int32_t* d_results, int32_t* d_stats);
+/* HipTracer::traceSort (HipTracer.cpp:114-121): ray keys (method, bits),
+ * radix sort, then trace in sorted order writing results at the original
+ * index.  d_order (optional, int32[n_rays]) receives the sorted ray order. */
+int hx_trace_sorted(hx_context* ctx, hx_bvh* bvh, const float* d_rays, int64_t n_rays, int32_t any_hit,
+                    int32_t key_method, int32_t key_bits, float ray_length, int32_t* d_results,
+                    int32_t* d_stats, int32_t* d_order);
+/* RayBuffer::computeMortonCodes (RayBuffer.cpp:31-85) for the [0,1]^3 ray box. */
+int hx_ray_keys(hx_context* ctx, const float* d_rays, int64_t n_rays, int32_t key_method, int32_t key_bits,
+                float ray_length, uint64_t* d_keys);
+/* Synchronous copy between any two host/device addresses on the context's
+ * stream (Buffer migration, gpu/Buffer.cpp:437-498). */
+int hx_memcpy(hx_context* ctx, void* dst, const void* src, int64_t bytes);
+
+/* Number of rays whose traversal stack overflowed in the last trace call. */
+int hx_trace_overflow_count(hx_context* ctx, int64_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HX_H */

@@ -1,0 +1,953 @@
+/*
+ * hx_oracle.cpp -- CPU restatement of hippie's LBVH path (TEST INFRASTRUCTURE).
+ *
+ * Loaded only by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+ * leg; the product never links it.  Built with -O3 -ffp-contract=off -fopenmp
+ * (oracle/Makefile).  Reference citations are relative to src/hippie/.
+ *
+ * How each function is pinned (tests/test_oracle_*.py):
+ *   scene box, setup boxes, refit  -- exact min/max vs brute-force unions.
+ *   Morton codes                   -- bit-by-bit interleave closed form over
+ *                                     all 2^10 values per half, end points.
+ *   sort                           -- numpy stable argsort (library routine).
+ *   construct                      -- brute-force top-down split at the highest
+ *                                     differing bit (middle index for ties).
+ *   collapse                       -- brute-force recursive SAH decision on the
+ *                                     binary tree (same formula) + validate().
+ *   woopify                        -- FP64 check that the transform maps
+ *                                     v0,v1,v2 to (u,v) = (1,0),(0,1),(0,0).
+ *   cost                           -- FP64 closed form over the canonical tree.
+ *   trace                          -- brute force over all triangles; FP64
+ *                                     Moller-Trumbore for t.
+ *   ray keys                       -- bit loops re-derived in numpy.
+ */
+#include "hx_oracle.h"
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+#include <omp.h>
+
+namespace {
+
+/* Globals.h:70-74 -- MIN/MAX are selects, not fminf/fmaxf (signed zeros and
+ * NaN follow the select). */
+constexpr float kMaxFloat = 3.402823466e+38f;
+inline float MINs(float a, float b) { return (b < a) ? b : a; }
+inline float MAXs(float a, float b) { return (b > a) ? b : a; }
+
+struct V3 {
+    float x, y, z;
+};
+
+/* util/AABB.h:14-49 */
+struct Box {
+    V3 mn{kMaxFloat, kMaxFloat, kMaxFloat};
+    V3 mx{-kMaxFloat, -kMaxFloat, -kMaxFloat};
+    void grow(const V3& p) {
+        mn.x = MINs(mn.x, p.x); mn.y = MINs(mn.y, p.y); mn.z = MINs(mn.z, p.z);
+        mx.x = MAXs(mx.x, p.x); mx.y = MAXs(mx.y, p.y); mx.z = MAXs(mx.z, p.z);
+    }
+    bool valid() const { return mn.x <= mx.x && mn.y <= mx.y && mn.z <= mx.z; }
+    float area() const {
+        if (!valid()) return 0.0f;
+        float dx = mx.x - mn.x, dy = mx.y - mn.y, dz = mx.z - mn.z;
+        return 2.0f * (dx * dy + dy * dz + dz * dx);
+    }
+};
+
+inline V3 load3(const float* p) { return V3{p[0], p[1], p[2]}; }
+inline V3 load4(const float* p) { return V3{p[0], p[1], p[2]}; }
+inline void store4(float* p, const V3& v) { p[0] = v.x; p[1] = v.y; p[2] = v.z; p[3] = 0.0f; }
+
+/* Float -> unsigned / int conversions with the saturating semantics of the
+ * GPU convert instructions (NaN -> 0).  Inputs on the path are in range; the
+ * helpers only make the out-of-range cases defined and equal on both sides. */
+inline uint32_t f2u(float x) {
+    if (!(x > 0.0f)) return 0u;
+    if (x >= 4294967296.0f) return 0xFFFFFFFFu;
+    return (uint32_t)x;
+}
+inline int32_t f2i(float x) {
+    if (x != x) return 0;
+    if (x >= 2147483648.0f) return 0x7FFFFFFF;
+    if (x <= -2147483648.0f) return (int32_t)0x80000000;
+    return (int32_t)x;
+}
+
+/* rt/HipUtil.h:35-49 */
+inline uint32_t morton10(uint32_t x, uint32_t y, uint32_t z) {
+    x = (x | (x << 16)) & 0x030000FFu;
+    x = (x | (x << 8)) & 0x0300F00Fu;
+    x = (x | (x << 4)) & 0x030C30C3u;
+    x = (x | (x << 2)) & 0x09249249u;
+    y = (y | (y << 16)) & 0x030000FFu;
+    y = (y | (y << 8)) & 0x0300F00Fu;
+    y = (y | (y << 4)) & 0x030C30C3u;
+    y = (y | (y << 2)) & 0x09249249u;
+    z = (z | (z << 16)) & 0x030000FFu;
+    z = (z | (z << 8)) & 0x0300F00Fu;
+    z = (z | (z << 4)) & 0x030C30C3u;
+    z = (z | (z << 2)) & 0x09249249u;
+    return x | (y << 1) | (z << 2);
+}
+
+/* rt/HipUtil.h:72-82,100-106 */
+inline uint64_t morton60(const V3& c) {
+    const float scale = (float)((1u << 20) - 1);
+    uint32_t x = f2u(c.x * scale);
+    uint32_t y = f2u(c.y * scale);
+    uint32_t z = f2u(c.z * scale);
+    uint64_t lo = morton10(x & 1023u, y & 1023u, z & 1023u);
+    uint64_t hi = morton10(x >> 10u, y >> 10u, z >> 10u);
+    return (hi << 30) | lo;
+}
+
+/* rt/bvh/HipBVHUtil.h:17-24 */
+inline int delta(int64_t i, int64_t j, int64_t n, const uint64_t* k) {
+    if (j < 0 || j >= n) return -1;
+    uint64_t a = k[i], b = k[j];
+    if (a != b) return __builtin_clzll(a ^ b);
+    return __builtin_clzll((uint64_t)(i ^ j)) + 64;
+}
+
+inline int sgn(int v) { return (0 < v) - (v < 0); }
+
+template <class F>
+void parallel_for(int64_t n, F f) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) f(i);
+}
+
+/* util/Math.h:318-346 (det of a 3x3 built from MatBase (r,c) accessors). */
+inline float det3(const float m[3][3]) {
+    /* v(r,c) = m[r][c] */
+    return m[0][0] * m[1][1] * m[2][2] - m[0][0] * m[1][2] * m[2][1] +
+           m[1][0] * m[2][1] * m[0][2] - m[1][0] * m[2][2] * m[0][1] +
+           m[2][0] * m[0][1] * m[1][2] - m[2][0] * m[0][2] * m[1][1];
+}
+
+/* util/Math.h:349-369 inverse() of a 4x4 via cofactors; d sums all 16
+ * cofactor products (= 4 det), result = r * rcp(d) * 4. m[r][c]. */
+void inverse4(const float v[4][4], float r[4][4]) {
+    float d = 0.0f;
+    float si = 1.0f;
+    for (int i = 0; i < 4; ++i) {
+        float sj = si;
+        for (int j = 0; j < 4; ++j) {
+            float sub[3][3];
+            for (int k = 0; k < 3; ++k)
+                for (int l = 0; l < 3; ++l)
+                    sub[k][l] = v[(k < j) ? k : k + 1][(l < i) ? l : l + 1];
+            float dd = det3(sub) * sj;
+            r[i][j] = dd;
+            d += dd * v[j][i];
+            sj = -sj;
+        }
+        si = -si;
+    }
+    float rd = (d != 0.0f) ? 1.0f / d : 0.0f;
+    for (int a = 0; a < 4; ++a)
+        for (int b = 0; b < 4; ++b) r[a][b] = (r[a][b] * rd) * 4.0f;
+}
+
+/* rt/bvh/HipBVHKernels.cu:12-33,59-61 */
+void woop_rows(const V3& v0, const V3& v1, const V3& v2, float out[12]) {
+    V3 v02{v0.x - v2.x, v0.y - v2.y, v0.z - v2.z};
+    V3 v12{v1.x - v2.x, v1.y - v2.y, v1.z - v2.z};
+    /* util/Math.h cross() */
+    V3 nrm{v02.y * v12.z - v02.z * v12.y, v02.z * v12.x - v02.x * v12.z,
+           v02.x * v12.y - v02.y * v12.x};
+    /* m[col][row] in the reference; here M[row][col]. */
+    float M[4][4];
+    M[0][0] = v02.x; M[1][0] = v02.y; M[2][0] = v02.z; M[3][0] = 0.0f;
+    M[0][1] = v12.x; M[1][1] = v12.y; M[2][1] = v12.z; M[3][1] = 0.0f;
+    M[0][2] = nrm.x; M[1][2] = nrm.y; M[2][2] = nrm.z; M[3][2] = 0.0f;
+    M[0][3] = v2.x;  M[1][3] = v2.y;  M[2][3] = v2.z;  M[3][3] = 1.0f;
+    float I[4][4];
+    inverse4(M, I);
+    out[0] = I[2][0]; out[1] = I[2][1]; out[2] = I[2][2]; out[3] = -I[2][3];
+    out[4] = I[0][0]; out[5] = I[0][1]; out[6] = I[0][2]; out[7] = I[0][3];
+    out[8] = I[1][0]; out[9] = I[1][1]; out[10] = I[1][2]; out[11] = I[1][3];
+}
+
+inline void tri_verts(const float* verts, const int32_t* tris, int64_t t, V3& a, V3& b, V3& c) {
+    const int32_t* tr = tris + 3 * t;
+    a = load3(verts + 3 * (int64_t)tr[0]);
+    b = load3(verts + 3 * (int64_t)tr[1]);
+    c = load3(verts + 3 * (int64_t)tr[2]);
+}
+
+/* Bin node accessors: rt/bvh/HipBVHNode.h:117-216. */
+inline float w2f(int32_t w) { float f; std::memcpy(&f, &w, 4); return f; }
+inline int32_t f2w(float f) { int32_t w; std::memcpy(&w, &f, 4); return w; }
+
+void set_child_box(int32_t* node, int i, const V3& mn, const V3& mx) {
+    node[4 * i + 0] = f2w(mn.x);
+    node[4 * i + 2] = f2w(mn.y);
+    node[2 * i + 8] = f2w(mn.z);
+    node[4 * i + 1] = f2w(mx.x);
+    node[4 * i + 3] = f2w(mx.y);
+    node[2 * i + 9] = f2w(mx.z);
+}
+
+/* Trace helpers: rt/tracer/HipTracerKernels.h:124-195 with the float
+ * fminf/fmaxf written as selects and the integer min/max on float bits. */
+inline float fmin_s(float a, float b) { return (b < a) ? b : a; }
+inline float fmax_s(float a, float b) { return (b > a) ? b : a; }
+inline int32_t fb(float f) { return f2w(f); }
+inline float bf(int32_t i) { return w2f(i); }
+inline float i_min_max(float a, float b, float c) {  /* max(min(a,b),c) on bits */
+    int32_t x = std::min(fb(a), fb(b));
+    return bf(std::max(x, fb(c)));
+}
+inline float i_max_min(float a, float b, float c) {  /* min(max(a,b),c) on bits */
+    int32_t x = std::max(fb(a), fb(b));
+    return bf(std::min(x, fb(c)));
+}
+inline float i_max_max(float a, float b, float c) { return bf(std::max(std::max(fb(a), fb(b)), fb(c))); }
+inline float i_min_min(float a, float b, float c) { return bf(std::min(std::min(fb(a), fb(b)), fb(c))); }
+inline float span_begin(float a0, float a1, float b0, float b1, float c0, float c1, float d) {
+    return i_max_max(fmin_s(a0, a1), fmin_s(b0, b1), i_min_max(c0, c1, d));
+}
+inline float span_end(float a0, float a1, float b0, float b1, float c0, float c1, float d) {
+    return i_min_min(fmax_s(a0, a1), fmax_s(b0, b1), i_max_min(c0, c1, d));
+}
+
+constexpr int32_t kSentinel = 0x76543210;  /* HipTracerKernels.h:11 */
+constexpr int kStackSize = 64;             /* HipTracerBinKernels.cu:5 */
+
+struct RayState {
+    float ox, oy, oz, tmin, dx, dy, dz, tmax;
+    float idx, idy, idz, oodx, oody, oodz;
+};
+
+inline void setup_ray(const float* r, RayState& s) {
+    s.ox = r[0]; s.oy = r[1]; s.oz = r[2]; s.tmin = r[3];
+    s.dx = r[4]; s.dy = r[5]; s.dz = r[6]; s.tmax = r[7];
+    const float ooeps = 8.2718061255302767e-25f; /* exp2f(-80.0f) */
+    s.idx = 1.0f / (std::fabs(s.dx) > ooeps ? s.dx : std::copysign(ooeps, s.dx));
+    s.idy = 1.0f / (std::fabs(s.dy) > ooeps ? s.dy : std::copysign(ooeps, s.dy));
+    s.idz = 1.0f / (std::fabs(s.dz) > ooeps ? s.dz : std::copysign(ooeps, s.dz));
+    s.oodx = s.ox * s.idx;
+    s.oody = s.oy * s.idy;
+    s.oodz = s.oz * s.idz;
+}
+
+/* Woop test of HipTracerBinKernels.cu:215-268 written with the fused
+ * multiply-adds an FMA-contracting compiler forms from it. Returns true and
+ * writes t,u,v on a hit within (tmin, hitT]. */
+inline bool woop_test(const RayState& s, const float* w, float hitT, float& t, float& u, float& v) {
+    float Oz = std::fmaf(-s.oz, w[2], std::fmaf(-s.oy, w[1], std::fmaf(-s.ox, w[0], w[3])));
+    float den = std::fmaf(s.dz, w[2], std::fmaf(s.dy, w[1], s.dx * w[0]));
+    float invDz = 1.0f / den;
+    t = Oz * invDz;
+    if (!(t > s.tmin && t <= hitT)) return false;
+    float Ox = std::fmaf(s.oz, w[6], std::fmaf(s.oy, w[5], std::fmaf(s.ox, w[4], w[7])));
+    float Dx = std::fmaf(s.dz, w[6], std::fmaf(s.dy, w[5], s.dx * w[4]));
+    u = std::fmaf(t, Dx, Ox);
+    if (!(u >= 0.0f)) return false;
+    float Oy = std::fmaf(s.oz, w[10], std::fmaf(s.oy, w[9], std::fmaf(s.ox, w[8], w[11])));
+    float Dy = std::fmaf(s.dz, w[10], std::fmaf(s.dy, w[9], s.dx * w[8]));
+    v = std::fmaf(t, Dy, Oy);
+    return v >= 0.0f && u + v <= 1.0f;
+}
+
+}  // namespace
+
+extern "C" {
+
+void hxo_set_threads(int threads) {
+    if (threads > 0) omp_set_num_threads(threads);
+}
+
+int hxo_get_threads(void) { return omp_get_max_threads(); }
+
+void hxo_scene_box(const float* verts, const int32_t* tris, int64_t n, float out[6]) {
+    Box scene;
+    for (int64_t t = 0; t < n; ++t) {
+        V3 a, b, c;
+        tri_verts(verts, tris, t, a, b, c);
+        Box box;
+        box.grow(a); box.grow(b); box.grow(c);
+        scene.grow(box.mn);
+        scene.grow(box.mx);
+    }
+    out[0] = scene.mn.x; out[1] = scene.mn.y; out[2] = scene.mn.z;
+    out[3] = scene.mx.x; out[4] = scene.mx.y; out[5] = scene.mx.z;
+}
+
+void hxo_setup_boxes(const float* verts, const int32_t* tris, int64_t n, float* bmin, float* bmax) {
+    parallel_for(n, [&](int64_t t) {
+        V3 a, b, c;
+        tri_verts(verts, tris, t, a, b, c);
+        Box box;
+        box.grow(a); box.grow(b); box.grow(c);
+        store4(bmin + 4 * t, box.mn);
+        store4(bmax + 4 * t, box.mx);
+    });
+}
+
+void hxo_morton_codes(const float* bmin, const float* bmax, int64_t n, const float sb[6], int bits,
+                      uint64_t* keys) {
+    V3 smn{sb[0], sb[1], sb[2]};
+    V3 diag{sb[3] - sb[0], sb[4] - sb[1], sb[5] - sb[2]};
+    V3 scale{1.0f / diag.x, 1.0f / diag.y, 1.0f / diag.z};
+    const int shift = 60 - bits;
+    parallel_for(n, [&](int64_t r) {
+        Box box;
+        box.grow(load4(bmin + 4 * r));
+        box.grow(load4(bmax + 4 * r));
+        V3 c{(box.mn.x + box.mx.x) * 0.5f, (box.mn.y + box.mx.y) * 0.5f, (box.mn.z + box.mx.z) * 0.5f};
+        V3 q{(c.x - smn.x) * scale.x, (c.y - smn.y) * scale.y, (c.z - smn.z) * scale.z};
+        keys[r] = morton60(q) >> shift;
+    });
+}
+
+void hxo_sort_pairs(uint64_t* keys, int32_t* vals, int64_t n, int end_bit) {
+    if (n <= 1 || end_bit <= 0) return;
+    std::vector<uint64_t> k2(n);
+    std::vector<int32_t> v2(n);
+    uint64_t* ks = keys; int32_t* vs = vals;
+    uint64_t* kd = k2.data(); int32_t* vd = v2.data();
+    const int T = omp_get_max_threads();
+    std::vector<int64_t> hist((size_t)T * 256);
+    for (int bit = 0; bit < end_bit; bit += 8) {
+        const int nb = std::min(8, end_bit - bit);
+        const uint64_t mask = (1ull << nb) - 1;
+        std::fill(hist.begin(), hist.end(), 0);
+#pragma omp parallel num_threads(T)
+        {
+            int tid = omp_get_thread_num();
+            int64_t lo = n * tid / T, hi = n * (tid + 1) / T;
+            int64_t* h = hist.data() + (size_t)tid * 256;
+            for (int64_t i = lo; i < hi; ++i) h[(ks[i] >> bit) & mask]++;
+#pragma omp barrier
+#pragma omp single
+            {
+                int64_t sum = 0;
+                for (int d = 0; d < 256; ++d)
+                    for (int t = 0; t < T; ++t) {
+                        int64_t c = hist[(size_t)t * 256 + d];
+                        hist[(size_t)t * 256 + d] = sum;
+                        sum += c;
+                    }
+            }
+            for (int64_t i = lo; i < hi; ++i) {
+                int64_t dst = h[(ks[i] >> bit) & mask]++;
+                kd[dst] = ks[i];
+                vd[dst] = vs[i];
+            }
+        }
+        std::swap(ks, kd);
+        std::swap(vs, vd);
+    }
+    if (ks != keys) {
+        std::memcpy(keys, ks, sizeof(uint64_t) * n);
+        std::memcpy(vals, vs, sizeof(int32_t) * n);
+    }
+}
+
+void hxo_construct(const uint64_t* k, int64_t n, int32_t* left, int32_t* right, int32_t* parent) {
+    /* setupLeaves (LBVHBuilderKernels.cu:185-187): leaf node n-1+i has
+     * left = i, right = i + 1. */
+    parallel_for(n, [&](int64_t i) {
+        left[n - 1 + i] = (int32_t)i;
+        right[n - 1 + i] = (int32_t)(i + 1);
+    });
+    parent[0] = -1;
+    parallel_for(n - 1, [&](int64_t i) {
+        const int d = sgn(delta(i, i + 1, n, k) - delta(i, i - 1, n, k));
+        const int dmin = delta(i, i - d, n, k);
+        int64_t lmax = 2;
+        while (delta(i, i + lmax * d, n, k) > dmin) lmax <<= 1;
+        int64_t l = 0;
+        for (int64_t t = lmax >> 1; t >= 1; t >>= 1)
+            if (delta(i, i + (l + t) * d, n, k) > dmin) l += t;
+        const int64_t j = i + l * d;
+        const int dnode = delta(i, j, n, k);
+        int64_t s = 0, kk = 2, t;
+        do {
+            t = (l + kk - 1) / kk;
+            kk <<= 1;
+            if (delta(i, i + (s + t) * d, n, k) > dnode) s += t;
+        } while (t > 1);
+        const int64_t gamma = i + s * d + std::min(d, 0);
+        int64_t lc = gamma, rc = gamma + 1;
+        if (std::min(i, j) == gamma) lc += n - 1;
+        if (std::max(i, j) == gamma + 1) rc += n - 1;
+        left[i] = (int32_t)lc;
+        right[i] = (int32_t)rc;
+        parent[lc] = (int32_t)i;
+        parent[rc] = (int32_t)i;
+    });
+}
+
+void hxo_refit(const int32_t* sv, const float* rmin, const float* rmax, int64_t n, const int32_t* left,
+               const int32_t* right, const int32_t* parent, float* nmin, float* nmax) {
+    const int64_t N = 2 * n - 1;
+    /* setupLeaves: leaf box = reference box of the sorted reference. */
+    parallel_for(n, [&](int64_t i) {
+        int64_t r = sv[i];
+        std::memcpy(nmin + 4 * (n - 1 + i), rmin + 4 * r, 16);
+        std::memcpy(nmax + 4 * (n - 1 + i), rmax + 4 * r, 16);
+    });
+    if (n < 2) return;
+    /* refit (LBVHBuilderKernels.cu:254-305): second arriver unions children. */
+    std::vector<std::atomic<int>> counters(n - 1);
+    for (auto& c : counters) c.store(0, std::memory_order_relaxed);
+    parallel_for(n, [&](int64_t t) {
+        int64_t node = parent[t + (N >> 1)];
+        while (node >= 0 && counters[node].fetch_add(1, std::memory_order_acq_rel) > 0) {
+            int64_t l = left[node], r = right[node];
+            Box box;
+            box.grow(load4(nmin + 4 * l));
+            box.grow(load4(nmin + 4 * r));
+            V3 mn = box.mn;
+            box.grow(load4(nmax + 4 * l));
+            box.grow(load4(nmax + 4 * r));
+            store4(nmin + 4 * node, mn);
+            store4(nmax + 4 * node, box.mx);
+            std::atomic_thread_fence(std::memory_order_release);
+            node = parent[node];
+        }
+    });
+}
+
+int64_t hxo_collapse(int64_t n, const int32_t* sv, const int32_t* left, const int32_t* right,
+                     const int32_t* parent, const float* nmin, const float* nmax, int adaptive,
+                     int max_leaf_size, float ct, float ci, int32_t* nodes, int32_t* tri_out,
+                     int64_t* n_interior, int64_t* n_leaves) {
+    if (n == 1) {
+        /* A single reference: the reference indexes termCounters[-1]
+         * (BVHCollapserKernels.cu:29-32).  Emit an interior root whose two
+         * children are the one leaf. */
+        V3 mn = load4(nmin), mx = load4(nmax);
+        int32_t* root = nodes;
+        int32_t* leaf = nodes + 16;
+        std::memset(nodes, 0, 32 * sizeof(int32_t));
+        set_child_box(root, 0, mn, mx);
+        set_child_box(root, 1, mn, mx);
+        root[12] = ~1; root[13] = ~1; root[14] = 1; root[15] = -1;
+        set_child_box(leaf, 0, mn, mx);
+        set_child_box(leaf, 1, mn, mx);
+        leaf[12] = 0; leaf[13] = 1; leaf[14] = ~1; leaf[15] = 0;
+        tri_out[0] = sv[0];
+        *n_interior = 1;
+        *n_leaves = 1;
+        return 2;
+    }
+    const int64_t N = 2 * n - 1;
+    std::vector<int32_t> sizes(N), states(N), leafIdx(n), offsets(N, 0), refOffsets(N, 0);
+    std::vector<float> costs(N);
+    std::vector<std::atomic<int>> counters(n - 1);
+    auto clear = [&]() { for (auto& c : counters) c.store(0, std::memory_order_relaxed); };
+
+    /* computeSizes (BVHCollapserKernels.cu:11-56). */
+    clear();
+    parallel_for(n, [&](int64_t t) {
+        int64_t leafNode = t + (N >> 1);
+        sizes[leafNode] = 1;
+        std::atomic_thread_fence(std::memory_order_release);
+        int64_t node = parent[leafNode];
+        while (counters[node].fetch_add(1, std::memory_order_acq_rel) > 0) {
+            sizes[node] = sizes[left[node]] + sizes[right[node]];
+            std::atomic_thread_fence(std::memory_order_release);
+            if (node == 0) break;
+            node = parent[node];
+        }
+    });
+
+    /* computeNodeStatesAdaptive (:58-136) / computeNodeStates (:138-184). */
+    clear();
+    parallel_for(n, [&](int64_t t) {
+        int64_t node = t + n - 1;
+        if (adaptive) {
+            Box box;
+            box.grow(load4(nmin + 4 * node));
+            box.grow(load4(nmax + 4 * node));
+            costs[node] = ci * box.area();
+        }
+        states[node] = 0;
+        std::atomic_thread_fence(std::memory_order_release);
+        node = parent[node];
+        while (counters[node].fetch_add(1, std::memory_order_acq_rel) > 0) {
+            if (adaptive) {
+                Box box;
+                box.grow(load4(nmin + 4 * node));
+                box.grow(load4(nmax + 4 * node));
+                float area = box.area();
+                float cost = ct * area + costs[left[node]] + costs[right[node]];
+                float costAsLeaf = ci * area * (float)sizes[node];
+                if (costAsLeaf < cost) { costs[node] = costAsLeaf; states[node] = 0; }
+                else { costs[node] = cost; states[node] = 1; }
+            } else {
+                states[node] = sizes[node] <= max_leaf_size ? 0 : 1;
+            }
+            std::atomic_thread_fence(std::memory_order_release);
+            if (node == 0) break;
+            node = parent[node];
+        }
+    });
+    /* The root is always interior: computeLeafIndices (:205) never stops at the
+     * root, and compact (:463-464) places a state-0 root at an unwritten slot. */
+    states[0] = 1;
+
+    /* computeLeafIndices (:186-217). */
+    parallel_for(n, [&](int64_t t) {
+        int64_t node = t + n - 1;
+        int64_t li = node;
+        int64_t p = parent[li];
+        int st = states[p];
+        while (p > 0) {
+            if (st == 0) li = p;
+            node = p;
+            p = parent[node];
+            st = states[p];
+        }
+        leafIdx[t] = (int32_t)li;
+    });
+
+    /* invalidateCollapsedNodes (:219-266). */
+    clear();
+    parallel_for(n, [&](int64_t t) {
+        int64_t li = leafIdx[t];
+        int64_t node = t + n - 1;
+        if (node == li) return;
+        states[node] = -1;
+        node = parent[node];
+        while (counters[node].fetch_add(1, std::memory_order_acq_rel) > 0) {
+            if (node == li) break;
+            states[node] = -1;
+            if (node == 0) break;
+            node = parent[node];
+        }
+    });
+
+    /* computeNodeOffsets (:268-355, host loop BVHCollapser.cpp:211-231): BFS by
+     * levels; within a level, tasks in order, left child before right.  The
+     * reference orders warps by atomic arrival; here the order is sequential. */
+    std::vector<int32_t> taskNodes(n);
+    int64_t interiorCount = 1, leafCount = 0;
+    taskNodes[0] = 0;
+    offsets[0] = 0; /* the reference never writes the root slot */
+    int64_t taskOffset = 0, numTasks = 1;
+    while (numTasks > 0) {
+        for (int64_t t = taskOffset; t < taskOffset + numTasks; ++t) {
+            int64_t node = taskNodes[t];
+            int32_t c[2] = {left[node], right[node]};
+            for (int k = 0; k < 2; ++k)
+                if (states[c[k]] > 0) {
+                    offsets[c[k]] = (int32_t)interiorCount;
+                    taskNodes[interiorCount++] = c[k];
+                }
+            for (int k = 0; k < 2; ++k)
+                if (states[c[k]] == 0) offsets[c[k]] = (int32_t)(leafCount++);
+        }
+        taskOffset += numTasks;
+        numTasks = interiorCount - taskOffset;
+    }
+
+    /* computeReferenceOffsets (:357-404): leaf triangle ranges in node order. */
+    {
+        int64_t sum = 0;
+        for (int64_t node = 0; node < N; ++node)
+            if (states[node] == 0) { refOffsets[node] = (int32_t)sum; sum += sizes[node]; }
+    }
+
+    /* compactBin (:435-526). */
+    const int64_t nInt = interiorCount;
+    parallel_for(N, [&](int64_t node) {
+        int st = states[node];
+        if (st == -1) return;
+        int64_t newIdx = st > 0 ? offsets[node] : offsets[node] + nInt;
+        int32_t par = parent[node];
+        int32_t sz = sizes[node];
+        int32_t l = left[node], r = right[node];
+        if (par >= 0) par = offsets[par];
+        V3 lmn, lmx, rmn, rmx;
+        int32_t li, ri;
+        if (st > 0) {
+            int sl = states[l], sr = states[r];
+            lmn = load4(nmin + 4 * (int64_t)l); lmx = load4(nmax + 4 * (int64_t)l);
+            rmn = load4(nmin + 4 * (int64_t)r); rmx = load4(nmax + 4 * (int64_t)r);
+            li = sl > 0 ? offsets[l] : ~(int32_t)(offsets[l] + nInt);
+            ri = sr > 0 ? offsets[r] : ~(int32_t)(offsets[r] + nInt);
+        } else {
+            lmn = rmn = load4(nmin + 4 * node);
+            lmx = rmx = load4(nmax + 4 * node);
+            li = refOffsets[node];
+            ri = li + sz;
+            sz = ~sz;
+        }
+        int32_t* out = nodes + 16 * newIdx;
+        set_child_box(out, 0, lmn, lmx);
+        set_child_box(out, 1, rmn, rmx);
+        out[12] = li; out[13] = ri; out[14] = sz; out[15] = par;
+    });
+
+    /* reorderTriangleIndices (:406-433): within a leaf, sorted order. */
+    for (int64_t t = 0; t < n; ++t) {
+        int64_t refIdx = left[t + n - 1];
+        int64_t off = refOffsets[leafIdx[t]]++;
+        tri_out[off] = sv[refIdx];
+    }
+    *n_interior = nInt;
+    *n_leaves = leafCount;
+    return nInt + leafCount;
+}
+
+void hxo_woopify(const float* verts, const int32_t* tris, const int32_t* tri_idx, int64_t n, float* woop) {
+    parallel_for(n, [&](int64_t r) {
+        V3 a, b, c;
+        tri_verts(verts, tris, tri_idx[r], a, b, c);
+        woop_rows(a, b, c, woop + 12 * r);
+    });
+}
+
+double hxo_cost(const int32_t* nodes, int64_t n_nodes, float scene_area, float ct, float ci) {
+    double sum = 0.0;
+    for (int64_t i = 0; i < n_nodes; ++i) {
+        const int32_t* nd = nodes + 16 * i;
+        Box box;
+        for (int c = 0; c < 2; ++c) {
+            V3 mn{w2f(nd[4 * c + 0]), w2f(nd[4 * c + 2]), w2f(nd[2 * c + 8])};
+            V3 mx{w2f(nd[4 * c + 1]), w2f(nd[4 * c + 3]), w2f(nd[2 * c + 9])};
+            box.grow(mn);
+            box.grow(mx);
+        }
+        float P = box.area() / scene_area;
+        float term;
+        if (nd[14] < 0) term = ci * P * (float)(~nd[14]);
+        else term = ct * P;
+        sum += (double)term;
+    }
+    return sum;
+}
+
+int64_t hxo_trace(const int32_t* nodes, const float* woop, const int32_t* tri_idx, const float* rays,
+                  int64_t n_rays, int any_hit, int32_t* results, int32_t* stats) {
+    int64_t overflows = 0;
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : overflows)
+    for (int64_t ri = 0; ri < n_rays; ++ri) {
+        RayState s;
+        setup_ray(rays + 8 * ri, s);
+        int32_t stack[kStackSize];
+        stack[0] = kSentinel;
+        int sp = 0;
+        bool overflow = false;
+        int32_t leafAddr = 0, nodeAddr = 0, hitIndex = -1, hitId = -1;
+        float hitT = s.tmax, hitU = 0.0f, hitV = 0.0f;
+        int32_t nNodes = 0, nTris = 0;
+        while (nodeAddr != kSentinel) {
+            while ((uint32_t)nodeAddr < (uint32_t)kSentinel) {
+                const int32_t* nd = nodes + 16 * (int64_t)nodeAddr;
+                const float* f = (const float*)nd;
+                nNodes++;
+                float c0lox = std::fmaf(f[0], s.idx, -s.oodx);
+                float c0hix = std::fmaf(f[1], s.idx, -s.oodx);
+                float c0loy = std::fmaf(f[2], s.idy, -s.oody);
+                float c0hiy = std::fmaf(f[3], s.idy, -s.oody);
+                float c0loz = std::fmaf(f[8], s.idz, -s.oodz);
+                float c0hiz = std::fmaf(f[9], s.idz, -s.oodz);
+                float c1loz = std::fmaf(f[10], s.idz, -s.oodz);
+                float c1hiz = std::fmaf(f[11], s.idz, -s.oodz);
+                float c0min = span_begin(c0lox, c0hix, c0loy, c0hiy, c0loz, c0hiz, s.tmin);
+                float c0max = span_end(c0lox, c0hix, c0loy, c0hiy, c0loz, c0hiz, hitT);
+                float c1lox = std::fmaf(f[4], s.idx, -s.oodx);
+                float c1hix = std::fmaf(f[5], s.idx, -s.oodx);
+                float c1loy = std::fmaf(f[6], s.idy, -s.oody);
+                float c1hiy = std::fmaf(f[7], s.idy, -s.oody);
+                float c1min = span_begin(c1lox, c1hix, c1loy, c1hiy, c1loz, c1hiz, s.tmin);
+                float c1max = span_end(c1lox, c1hix, c1loy, c1hiy, c1loz, c1hiz, hitT);
+                bool swp = c1min < c0min;
+                bool t0 = c0max >= c0min;
+                bool t1 = c1max >= c1min;
+                int32_t cx = nd[12], cy = nd[13];
+                if (!t0 && !t1) {
+                    nodeAddr = stack[sp--];
+                } else {
+                    nodeAddr = t0 ? cx : cy;
+                    if (t0 && t1) {
+                        if (swp) std::swap(nodeAddr, cy);
+                        if (sp + 1 >= kStackSize) { overflow = true; nodeAddr = kSentinel; leafAddr = 0; break; }
+                        stack[++sp] = cy;
+                    }
+                }
+                if (nodeAddr < 0 && leafAddr >= 0) {
+                    leafAddr = nodeAddr;
+                    nodeAddr = stack[sp--];
+                }
+                if (!(leafAddr >= 0)) break; /* one lane: __any() over itself */
+            }
+            while (leafAddr < 0) {
+                const int32_t* leaf = nodes + 16 * (int64_t)(~leafAddr);
+                for (int32_t tri = leaf[12]; tri < leaf[13]; ++tri) {
+                    nTris++;
+                    float t, u, v;
+                    if (woop_test(s, woop + 12 * (int64_t)tri, hitT, t, u, v)) {
+                        int32_t id = tri_idx[tri];
+                        if (t < hitT || id < hitId) {
+                            hitT = t; hitIndex = tri; hitId = id; hitU = u; hitV = v;
+                            if (any_hit) { nodeAddr = kSentinel; break; }
+                        }
+                    }
+                }
+                leafAddr = nodeAddr;
+                if (nodeAddr < 0) nodeAddr = stack[sp--];
+            }
+        }
+        int32_t* res = results + 4 * ri;
+        if (overflow) overflows++;
+        if (hitIndex == -1) {
+            res[0] = -1;
+            res[1] = f2w(hitT);
+        } else {
+            res[0] = hitId; res[1] = f2w(hitT); res[2] = f2w(hitU); res[3] = f2w(hitV);
+        }
+        if (stats) { stats[2 * ri] = nNodes; stats[2 * ri + 1] = nTris; }
+    }
+    return overflows;
+}
+
+void hxo_trace_brute(const float* verts, const int32_t* tris, int64_t n, const float* rays, int64_t n_rays,
+                     int32_t* results) {
+    std::vector<float> w(12 * n);
+    std::vector<int32_t> ident(n);
+    for (int64_t i = 0; i < n; ++i) ident[i] = (int32_t)i;
+    hxo_woopify(verts, tris, ident.data(), n, w.data());
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t ri = 0; ri < n_rays; ++ri) {
+        RayState s;
+        setup_ray(rays + 8 * ri, s);
+        float hitT = s.tmax, hitU = 0.0f, hitV = 0.0f;
+        int32_t hitId = -1;
+        for (int64_t t = 0; t < n; ++t) {
+            float tt, u, v;
+            if (woop_test(s, w.data() + 12 * t, hitT, tt, u, v) && (tt < hitT || t < hitId)) {
+                hitT = tt; hitId = (int32_t)t; hitU = u; hitV = v;
+            }
+        }
+        int32_t* res = results + 4 * ri;
+        res[0] = hitId; res[1] = f2w(hitT);
+        if (hitId >= 0) { res[2] = f2w(hitU); res[3] = f2w(hitV); }
+    }
+}
+
+int64_t hxo_bin_canonical(const int32_t* nodes, int64_t n_interior, int64_t n_leaves, const int32_t* tri_idx,
+                          uint32_t* out, int64_t capacity) {
+    std::vector<uint32_t> buf;
+    buf.reserve((size_t)(n_interior * 13 + n_leaves * 8));
+    std::vector<int32_t> stack;
+    stack.push_back(0);
+    std::vector<int32_t> ids;
+    while (!stack.empty()) {
+        int32_t idx = stack.back();
+        stack.pop_back();
+        const int32_t* nd = nodes + 16 * (int64_t)idx;
+        if (nd[14] < 0) {
+            buf.push_back(1u);
+            for (int w : {0, 1, 2, 3, 8, 9}) buf.push_back((uint32_t)nd[w]);
+            int32_t b = nd[12], e = nd[13];
+            buf.push_back((uint32_t)(e - b));
+            ids.assign(tri_idx + b, tri_idx + e);
+            std::sort(ids.begin(), ids.end());
+            for (int32_t id : ids) buf.push_back((uint32_t)id);
+        } else {
+            buf.push_back(0u);
+            for (int w = 0; w < 12; ++w) buf.push_back((uint32_t)nd[w]);
+            int32_t c0 = nd[12] < 0 ? ~nd[12] : nd[12];
+            int32_t c1 = nd[13] < 0 ? ~nd[13] : nd[13];
+            stack.push_back(c1);
+            stack.push_back(c0);
+        }
+    }
+    if (out) {
+        int64_t m = std::min<int64_t>(capacity, (int64_t)buf.size());
+        std::memcpy(out, buf.data(), sizeof(uint32_t) * m);
+    }
+    return (int64_t)buf.size();
+}
+
+int hxo_validate_bin(const int32_t* nodes, int64_t n_interior, int64_t n_leaves, const int32_t* tri_idx,
+                     int64_t n_refs, const float* verts, const int32_t* tris) {
+    const int64_t nn = n_interior + n_leaves;
+    if (n_interior < 1) return 1;
+    std::vector<int> seen(nn, 0), refSeen(n_refs, 0);
+    for (int64_t i = 0; i < n_interior; ++i) {
+        const int32_t* nd = nodes + 16 * i;
+        if (nd[14] < 0) return 2;                       /* interior first */
+        if (i == 0 && nd[15] != -1) return 3;           /* root parent */
+        int64_t sum = 0;
+        for (int c = 0; c < 2; ++c) {
+            int32_t ci = nd[12 + c];
+            int64_t child = ci < 0 ? ~(int64_t)ci : ci;
+            if (child <= 0 || child >= nn) return 4;
+            if ((ci < 0) != (child >= n_interior)) return 5;  /* leaf flag */
+            const int32_t* ch = nodes + 16 * child;
+            if (ch[15] != (int32_t)i) return 6;        /* parent pointer */
+            bool dup = (c == 1 && nd[12] == nd[13] && n_refs == 1);
+            if (!dup) { if (seen[child]) return 7; seen[child] = 1; }
+            int32_t sz = ch[14] < 0 ? ~ch[14] : ch[14];
+            if (!dup) sum += sz;
+            /* child box stored in the parent equals the child's own box */
+            for (int w = 0; w < 6; ++w) {
+                static const int wi[2][6] = {{0, 1, 2, 3, 8, 9}, {4, 5, 6, 7, 10, 11}};
+                float pv = w2f(nd[wi[c][w]]);
+                const int32_t* cb = ch;
+                float lo = w2f(cb[wi[0][w]]), lo2 = w2f(cb[wi[1][w]]);
+                float v = (w % 2 == 0) ? std::min(lo, lo2) : std::max(lo, lo2);
+                if (!(pv == v)) return 8;
+            }
+        }
+        if (n_refs > 1 && sum != nd[14]) return 9;      /* sizes */
+    }
+    for (int64_t i = 1; i < nn; ++i)
+        if (!seen[i]) return 10;                        /* every node referenced once */
+    for (int64_t i = n_interior; i < nn; ++i) {
+        const int32_t* nd = nodes + 16 * i;
+        if (nd[14] >= 0) return 11;
+        int32_t b = nd[12], e = nd[13];
+        if (b < 0 || e > n_refs || e - b != ~nd[14]) return 12;
+        V3 mn{w2f(nd[0]), w2f(nd[2]), w2f(nd[8])}, mx{w2f(nd[1]), w2f(nd[3]), w2f(nd[9])};
+        for (int32_t r = b; r < e; ++r) {
+            if (refSeen[r]) return 13;
+            refSeen[r] = 1;
+            V3 a, bb, c;
+            tri_verts(verts, tris, tri_idx[r], a, bb, c);
+            for (const V3* p : {&a, &bb, &c})
+                if (p->x < mn.x || p->y < mn.y || p->z < mn.z || p->x > mx.x || p->y > mx.y || p->z > mx.z)
+                    return 14;                          /* leaf box contains its triangles */
+        }
+    }
+    for (int64_t r = 0; r < n_refs; ++r)
+        if (!refSeen[r]) return 15;
+    return 0;
+}
+
+int64_t hxo_build_lbvh(const float* verts, const int32_t* tris, int64_t n, const float* scene_box, int bits,
+                       int adaptive, int max_leaf_size, float ct, float ci, int32_t* nodes, int32_t* tri_out,
+                       float* woop, int64_t* n_interior, int64_t* n_leaves, double sec[8]) {
+    using clk = std::chrono::steady_clock;
+    auto t0 = clk::now();
+    auto lap = [&](int i) {
+        auto t1 = clk::now();
+        sec[i] = std::chrono::duration<double>(t1 - t0).count();
+        t0 = t1;
+    };
+    for (int i = 0; i < 8; ++i) sec[i] = 0.0;
+    float sb[6];
+    if (scene_box) std::memcpy(sb, scene_box, sizeof(sb));
+    else hxo_scene_box(verts, tris, n, sb);
+    lap(0);
+    std::vector<float> rmin(4 * n), rmax(4 * n);
+    hxo_setup_boxes(verts, tris, n, rmin.data(), rmax.data());
+    std::vector<uint64_t> keys(n);
+    std::vector<int32_t> vals(n);
+    hxo_morton_codes(rmin.data(), rmax.data(), n, sb, bits, keys.data());
+    for (int64_t i = 0; i < n; ++i) vals[i] = (int32_t)i;
+    lap(1);
+    hxo_sort_pairs(keys.data(), vals.data(), n, bits);
+    lap(2);
+    const int64_t N = 2 * n - 1;
+    std::vector<int32_t> left(N), right(N), parent(N);
+    hxo_construct(keys.data(), n, left.data(), right.data(), parent.data());
+    lap(3);
+    std::vector<float> nmin(4 * N), nmax(4 * N);
+    hxo_refit(vals.data(), rmin.data(), rmax.data(), n, left.data(), right.data(), parent.data(), nmin.data(),
+              nmax.data());
+    lap(4);
+    int64_t nn = hxo_collapse(n, vals.data(), left.data(), right.data(), parent.data(), nmin.data(), nmax.data(),
+                              adaptive, max_leaf_size, ct, ci, nodes, tri_out, n_interior, n_leaves);
+    lap(5);
+    hxo_woopify(verts, tris, tri_out, n, woop);
+    lap(6);
+    return nn;
+}
+
+/* rt/ray/RayBufferKernels.cu:556-614 */
+void hxo_ray_keys_aila64(const float* rays, int64_t n_rays, int bits, uint64_t* keys) {
+    parallel_for(n_rays, [&](int64_t i) {
+        const float* r = rays + 8 * i;
+        /* ray box [0,1]^3: (o - mn) / (mx - mn) */
+        V3 a{(r[0] - 0.0f) / (1.0f - 0.0f), (r[1] - 0.0f) / (1.0f - 0.0f), (r[2] - 0.0f) / (1.0f - 0.0f)};
+        float ls = 0.0f;
+        ls += r[4] * r[4];
+        ls += r[5] * r[5];
+        ls += r[6] * r[6];
+        float len = std::sqrt(ls);
+        float rl = (len != 0.0f) ? 1.0f / len : 0.0f;
+        V3 b{(r[4] * rl + 1.0f) * 0.5f, (r[5] * rl + 1.0f) * 0.5f, (r[6] * rl + 1.0f) * 0.5f};
+        int32_t iax = f2i(a.x * 8191.0f), iay = f2i(a.y * 8191.0f), iaz = f2i(a.z * 8191.0f);
+        int32_t ibx = f2i(b.x * 8191.0f), iby = f2i(b.y * 8191.0f), ibz = f2i(b.z * 8191.0f);
+        uint64_t code = 0;
+        code |= (uint64_t)((iax >> 12) & 1) << 63;
+        code |= (uint64_t)((iay >> 12) & 1) << 62;
+        code |= (uint64_t)((iaz >> 12) & 1) << 61;
+        code |= (uint64_t)((iax >> 11) & 1) << 60;
+        code |= (uint64_t)((iay >> 11) & 1) << 59;
+        code |= (uint64_t)((iaz >> 11) & 1) << 58;
+        code |= (uint64_t)((iax >> 10) & 1) << 57;
+        code |= (uint64_t)((iay >> 10) & 1) << 56;
+        code |= (uint64_t)((iaz >> 10) & 1) << 55;
+        for (int k = 9; k >= 1; --k) {
+            code |= (uint64_t)((iax >> k) & 1) << (6 * k + 0);
+            code |= (uint64_t)((iay >> k) & 1) << (6 * k - 1);
+            code |= (uint64_t)((iaz >> k) & 1) << (6 * k - 2);
+        }
+        for (int k = 12; k >= 4; --k) {
+            code |= (uint64_t)((ibx >> k) & 1) << (6 * k - 21);
+            code |= (uint64_t)((iby >> k) & 1) << (6 * k - 22);
+            code |= (uint64_t)((ibz >> k) & 1) << (6 * k - 23);
+        }
+        keys[i] = code >> (64 - bits);
+    });
+}
+
+/* rt/ray/RayBufferKernels.cu:12-56 (intersect) and :492-554 (TwoPoint64). */
+void hxo_ray_keys_twopoint64(const float* rays, int64_t n_rays, int bits, float ray_length, uint64_t* keys) {
+    auto fmin_n = [](float a, float b) { return (a != a) ? b : (b != b) ? a : ((b < a) ? b : a); };
+    auto fmax_n = [](float a, float b) { return (a != a) ? b : (b != b) ? a : ((b > a) ? b : a); };
+    parallel_for(n_rays, [&](int64_t i) {
+        const float* r = rays + 8 * i;
+        float o[3] = {r[0], r[1], r[2]}, d[3] = {r[4], r[5], r[6]};
+        float tmn[3], tmx[3];
+        for (int k = 0; k < 3; ++k) {
+            if (d[k] >= 0.0f) { tmn[k] = (0.0f - o[k]) / d[k]; tmx[k] = (1.0f - o[k]) / d[k]; }
+            else { tmn[k] = (1.0f - o[k]) / d[k]; tmx[k] = (0.0f - o[k]) / d[k]; }
+        }
+        float t0 = fmax_n(tmn[2], fmax_n(tmn[0], tmn[1]));
+        float t1 = fmin_n(tmx[2], fmin_n(tmx[0], tmx[1]));
+        float boxT = (t0 > t1) ? -1.0f : (1.0f - 1.0e-3f) * (t0 >= 0.0f ? t0 : t1);
+        float len = ray_length;
+        len = fmin_n(len, fmax_n(r[7], 0.0f));
+        len = fmin_n(len, boxT);
+        float a[3], b[3];
+        for (int k = 0; k < 3; ++k) {
+            a[k] = o[k];
+            b[k] = o[k] + len * d[k];
+            a[k] = (a[k] - 0.0f) * (1.0f / (1.0f - 0.0f));
+            b[k] = (b[k] - 0.0f) * (1.0f / (1.0f - 0.0f));
+        }
+        int32_t ia[3], ib[3];
+        for (int k = 0; k < 3; ++k) { ia[k] = f2i(a[k] * 8191.0f); ib[k] = f2i(b[k] * 8191.0f); }
+        uint64_t code = 0;
+        for (int k = 12; k >= 2; --k) {
+            code |= (uint64_t)((ia[0] >> k) & 1) << (6 * k - 9);
+            code |= (uint64_t)((ia[1] >> k) & 1) << (6 * k - 10);
+            code |= (uint64_t)((ia[2] >> k) & 1) << (6 * k - 11);
+        }
+        for (int k = 12; k >= 3; --k) {
+            code |= (uint64_t)((ib[0] >> k) & 1) << (6 * k - 12);
+            code |= (uint64_t)((ib[1] >> k) & 1) << (6 * k - 13);
+            code |= (uint64_t)((ib[2] >> k) & 1) << (6 * k - 14);
+        }
+        code |= (uint64_t)(ib[0] & 1);
+        keys[i] = code >> (64 - bits);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,221 @@
+"""ctypes binding of oracle/liboracle.so -- the CPU restatement of hippie's LBVH path.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg.  The product (hippie_amd/) never imports it.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_p = ctypes.c_void_p
+_i64 = ctypes.c_int64
+_i32 = ctypes.c_int
+_f32 = ctypes.c_float
+
+
+def build():
+    """Compile liboracle.so with oracle/Makefile."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        sig = {
+            "hxo_set_threads": (None, [_i32]),
+            "hxo_get_threads": (_i32, []),
+            "hxo_scene_box": (None, [_p, _p, _i64, _p]),
+            "hxo_setup_boxes": (None, [_p, _p, _i64, _p, _p]),
+            "hxo_morton_codes": (None, [_p, _p, _i64, _p, _i32, _p]),
+            "hxo_sort_pairs": (None, [_p, _p, _i64, _i32]),
+            "hxo_construct": (None, [_p, _i64, _p, _p, _p]),
+            "hxo_refit": (None, [_p, _p, _p, _i64, _p, _p, _p, _p, _p]),
+            "hxo_collapse": (_i64, [_i64, _p, _p, _p, _p, _p, _p, _i32, _i32, _f32, _f32,
+                                    _p, _p, _p, _p]),
+            "hxo_woopify": (None, [_p, _p, _p, _i64, _p]),
+            "hxo_cost": (ctypes.c_double, [_p, _i64, _f32, _f32, _f32]),
+            "hxo_trace": (_i64, [_p, _p, _p, _p, _i64, _i32, _p, _p]),
+            "hxo_trace_brute": (None, [_p, _p, _i64, _p, _i64, _p]),
+            "hxo_bin_canonical": (_i64, [_p, _i64, _i64, _p, _p, _i64]),
+            "hxo_validate_bin": (_i32, [_p, _i64, _i64, _p, _i64, _p, _p]),
+            "hxo_build_lbvh": (_i64, [_p, _p, _i64, _p, _i32, _i32, _i32, _f32, _f32,
+                                      _p, _p, _p, _p, _p, _p]),
+            "hxo_ray_keys_aila64": (None, [_p, _i64, _i32, _p]),
+            "hxo_ray_keys_twopoint64": (None, [_p, _i64, _i32, _f32, _p]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _c(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+def set_threads(n):
+    lib().hxo_set_threads(int(n))
+
+
+def get_threads():
+    return lib().hxo_get_threads()
+
+
+def scene_box(verts, tris):
+    v, t = _c(verts, np.float32), _c(tris, np.int32)
+    out = np.zeros(6, np.float32)
+    lib().hxo_scene_box(_ptr(v), _ptr(t), t.shape[0], _ptr(out))
+    return out
+
+
+def setup_boxes(verts, tris):
+    v, t = _c(verts, np.float32), _c(tris, np.int32)
+    n = t.shape[0]
+    bmin = np.zeros((n, 4), np.float32)
+    bmax = np.zeros((n, 4), np.float32)
+    lib().hxo_setup_boxes(_ptr(v), _ptr(t), n, _ptr(bmin), _ptr(bmax))
+    return bmin, bmax
+
+
+def morton_codes(bmin, bmax, box, bits):
+    bmin, bmax = _c(bmin, np.float32), _c(bmax, np.float32)
+    box = _c(box, np.float32)
+    keys = np.zeros(bmin.shape[0], np.uint64)
+    lib().hxo_morton_codes(_ptr(bmin), _ptr(bmax), bmin.shape[0], _ptr(box), int(bits), _ptr(keys))
+    return keys
+
+
+def sort_pairs(keys, vals, end_bit):
+    k, v = _c(keys, np.uint64).copy(), _c(vals, np.int32).copy()
+    lib().hxo_sort_pairs(_ptr(k), _ptr(v), k.shape[0], int(end_bit))
+    return k, v
+
+
+def construct(sorted_keys):
+    k = _c(sorted_keys, np.uint64)
+    n = k.shape[0]
+    N = 2 * n - 1
+    left, right, parent = (np.zeros(N, np.int32) for _ in range(3))
+    lib().hxo_construct(_ptr(k), n, _ptr(left), _ptr(right), _ptr(parent))
+    return left, right, parent
+
+
+def refit(sorted_vals, ref_min, ref_max, left, right, parent):
+    sv = _c(sorted_vals, np.int32)
+    n = sv.shape[0]
+    N = 2 * n - 1
+    nmin = np.zeros((N, 4), np.float32)
+    nmax = np.zeros((N, 4), np.float32)
+    lib().hxo_refit(_ptr(sv), _ptr(_c(ref_min, np.float32)), _ptr(_c(ref_max, np.float32)), n,
+                    _ptr(_c(left, np.int32)), _ptr(_c(right, np.int32)), _ptr(_c(parent, np.int32)),
+                    _ptr(nmin), _ptr(nmax))
+    return nmin, nmax
+
+
+def collapse(sorted_vals, left, right, parent, nmin, nmax, adaptive=True, max_leaf_size=8,
+             ct=3.0, ci=2.0):
+    sv = _c(sorted_vals, np.int32)
+    n = sv.shape[0]
+    N = max(2 * n - 1, 2)
+    nodes = np.zeros((N, 16), np.int32)
+    tri = np.zeros(n, np.int32)
+    ni, nl = ctypes.c_int64(0), ctypes.c_int64(0)
+    nn = lib().hxo_collapse(n, _ptr(sv), _ptr(_c(left, np.int32)), _ptr(_c(right, np.int32)),
+                            _ptr(_c(parent, np.int32)), _ptr(_c(nmin, np.float32)),
+                            _ptr(_c(nmax, np.float32)), int(bool(adaptive)), int(max_leaf_size),
+                            float(ct), float(ci), _ptr(nodes), _ptr(tri), ctypes.byref(ni),
+                            ctypes.byref(nl))
+    return nodes[:nn].copy(), tri, ni.value, nl.value
+
+
+def woopify(verts, tris, tri_idx):
+    v, t, ti = _c(verts, np.float32), _c(tris, np.int32), _c(tri_idx, np.int32)
+    woop = np.zeros((ti.shape[0], 12), np.float32)
+    lib().hxo_woopify(_ptr(v), _ptr(t), _ptr(ti), ti.shape[0], _ptr(woop))
+    return woop
+
+
+def cost(nodes, scene_area, ct=3.0, ci=2.0):
+    nd = _c(nodes, np.int32)
+    return lib().hxo_cost(_ptr(nd), nd.shape[0], float(scene_area), float(ct), float(ci))
+
+
+def trace(nodes, woop, tri_idx, rays, any_hit=False, want_stats=False):
+    nd, w, ti = _c(nodes, np.int32), _c(woop, np.float32), _c(tri_idx, np.int32)
+    r = _c(rays, np.float32).reshape(-1, 8)
+    R = r.shape[0]
+    res = np.zeros((R, 4), np.int32)
+    stats = np.zeros((R, 2), np.int32) if want_stats else None
+    ov = lib().hxo_trace(_ptr(nd), _ptr(w), _ptr(ti), _ptr(r), R, int(bool(any_hit)), _ptr(res),
+                         _ptr(stats))
+    return (res, stats, ov) if want_stats else (res, ov)
+
+
+def trace_brute(verts, tris, rays):
+    v, t = _c(verts, np.float32), _c(tris, np.int32)
+    r = _c(rays, np.float32).reshape(-1, 8)
+    res = np.zeros((r.shape[0], 4), np.int32)
+    lib().hxo_trace_brute(_ptr(v), _ptr(t), t.shape[0], _ptr(r), r.shape[0], _ptr(res))
+    return res
+
+
+def bin_canonical(nodes, n_interior, n_leaves, tri_idx):
+    nd, ti = _c(nodes, np.int32), _c(tri_idx, np.int32)
+    m = lib().hxo_bin_canonical(_ptr(nd), int(n_interior), int(n_leaves), _ptr(ti), None, 0)
+    out = np.zeros(m, np.uint32)
+    lib().hxo_bin_canonical(_ptr(nd), int(n_interior), int(n_leaves), _ptr(ti), _ptr(out), m)
+    return out
+
+
+def validate_bin(nodes, n_interior, n_leaves, tri_idx, verts, tris):
+    nd, ti = _c(nodes, np.int32), _c(tri_idx, np.int32)
+    v, t = _c(verts, np.float32), _c(tris, np.int32)
+    return lib().hxo_validate_bin(_ptr(nd), int(n_interior), int(n_leaves), _ptr(ti), ti.shape[0],
+                                  _ptr(v), _ptr(t))
+
+
+def build_lbvh(verts, tris, bits=60, scene_box=None, adaptive=True, max_leaf_size=8, ct=3.0, ci=2.0):
+    """Whole LBVH build; returns dict with nodes, tri_idx, woop, counts and stage seconds."""
+    v, t = _c(verts, np.float32), _c(tris, np.int32)
+    n = t.shape[0]
+    N = max(2 * n - 1, 2)
+    nodes = np.zeros((N, 16), np.int32)
+    tri = np.zeros(n, np.int32)
+    woop = np.zeros((n, 12), np.float32)
+    ni, nl = ctypes.c_int64(0), ctypes.c_int64(0)
+    sec = np.zeros(8, np.float64)
+    sb = None if scene_box is None else _c(scene_box, np.float32)
+    nn = lib().hxo_build_lbvh(_ptr(v), _ptr(t), n, _ptr(sb), int(bits), int(bool(adaptive)),
+                              int(max_leaf_size), float(ct), float(ci), _ptr(nodes), _ptr(tri),
+                              _ptr(woop), ctypes.byref(ni), ctypes.byref(nl), _ptr(sec))
+    return dict(nodes=nodes[:nn].copy(), tri_idx=tri, woop=woop, n_interior=ni.value,
+                n_leaves=nl.value, seconds=sec)
+
+
+def ray_keys_aila64(rays, bits=60):
+    r = _c(rays, np.float32).reshape(-1, 8)
+    keys = np.zeros(r.shape[0], np.uint64)
+    lib().hxo_ray_keys_aila64(_ptr(r), r.shape[0], int(bits), _ptr(keys))
+    return keys
+
+
+def ray_keys_twopoint64(rays, bits=60, ray_length=0.25):
+    r = _c(rays, np.float32).reshape(-1, 8)
+    keys = np.zeros(r.shape[0], np.uint64)
+    lib().hxo_ray_keys_twopoint64(_ptr(r), r.shape[0], int(bits), float(ray_length), _ptr(keys))
+    return keys

@@ -1,0 +1,187 @@
+"""GPU parity of the LBVH build (libhx through the C ABI) against the CPU oracle.
+
+Bit-exact: Morton keys, sorted order, Karras left/right/parent, binary boxes,
+the collapsed Bin tree in canonical DFS form, Woop rows.  FP64-summed SAH
+cost within 1e-9 relative.
+"""
+import numpy as np
+import pytest
+import torch
+
+import hippie_amd as hx
+from hippie_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _area(b):
+    d = b[3:6] - b[0:3]
+    return float(np.float32(2.0) * (d[0] * d[1] + d[1] * d[2] + d[2] * d[0]))
+
+
+def _oracle_build(oracle, v, t, bits, adaptive=True, max_leaf=8):
+    sb = oracle.scene_box(v, t)
+    bmin, bmax = oracle.setup_boxes(v, t)
+    keys = oracle.morton_codes(bmin, bmax, sb, bits)
+    sk, sv = oracle.sort_pairs(keys, np.arange(t.shape[0], dtype=np.int32), bits)
+    left, right, parent = oracle.construct(sk)
+    nmin, nmax = oracle.refit(sv, bmin, bmax, left, right, parent)
+    nodes, tri, ni, nl = oracle.collapse(sv, left, right, parent, nmin, nmax, adaptive, max_leaf)
+    woop = oracle.woopify(v, t, tri)
+    return dict(sb=sb, keys=keys, sorted_keys=sk, sorted_tri=sv, left=left, right=right, parent=parent,
+                nmin=nmin, nmax=nmax, nodes=nodes, tri=tri, n_interior=ni, n_leaves=nl, woop=woop)
+
+
+def _gpu_build(ctx, v, t, bits, adaptive=True, max_leaf=8):
+    scene = hx.Scene(v, t)
+    b = hx.LBVHBuilder(ctx, morton_bits=bits, adaptive_leaf_size=int(adaptive), max_leaf_size=max_leaf,
+                       keep_debug=1)
+    bvh = b.build(scene)
+    return scene, bvh
+
+
+def _check_build(oracle, ctx, v, t, bits, adaptive=True, max_leaf=8, check_binary=True):
+    ref = _oracle_build(oracle, v, t, bits, adaptive, max_leaf)
+    scene, bvh = _gpu_build(ctx, v, t, bits, adaptive, max_leaf)
+    n = t.shape[0]
+    assert np.array_equal(bvh.scene_box(), ref["sb"]), "scene box"
+    dbg = bvh.debug_host()
+    assert np.array_equal(dbg["sorted_keys"], ref["sorted_keys"]), "sorted Morton keys"
+    assert np.array_equal(dbg["sorted_tri"], ref["sorted_tri"]), "sorted order"
+    if check_binary and n > 1:
+        assert np.array_equal(dbg["left"], ref["left"]), "Karras left"
+        assert np.array_equal(dbg["right"], ref["right"]), "Karras right"
+        assert np.array_equal(dbg["parent"], ref["parent"]), "Karras parent"
+        assert np.array_equal(dbg["node_min"][:, :3].view(np.uint32), ref["nmin"][:, :3].view(np.uint32)), "box min"
+        assert np.array_equal(dbg["node_max"][:, :3].view(np.uint32), ref["nmax"][:, :3].view(np.uint32)), "box max"
+    nodes = bvh.nodes_host()
+    tri = bvh.tri_indices_host()
+    assert bvh.n_interior == ref["n_interior"] and bvh.n_leaves == ref["n_leaves"], "node counts"
+    assert oracle.validate_bin(nodes, bvh.n_interior, bvh.n_leaves, tri, v, t) == 0, "validate"
+    cg = oracle.bin_canonical(nodes, bvh.n_interior, bvh.n_leaves, tri)
+    co = oracle.bin_canonical(ref["nodes"], ref["n_interior"], ref["n_leaves"], ref["tri"])
+    assert np.array_equal(cg, co), "canonical Bin tree"
+    woop = bvh.woop_host()
+    assert np.array_equal(woop.view(np.uint32), oracle.woopify(v, t, tri).view(np.uint32)), "Woop rows"
+    return ref, scene, bvh, nodes, tri, woop
+
+
+@pytest.mark.parametrize("bits", [30, 60])
+def test_lbvh_config1_shape(oracle, ctx, bits):
+    """65,536-triangle uniform soup (config 1 geometry) at 30 and 60 bits."""
+    v, t = scenes.uniform_soup(65536, seed=1)
+    ref, scene, bvh, nodes, tri, woop = _check_build(oracle, ctx, v, t, bits)
+    sa = _area(ref["sb"])
+    c_gpu = bvh.get_cost()
+    c_ref = oracle.cost(ref["nodes"], sa)
+    assert abs(c_gpu - c_ref) <= 1e-9 * abs(c_ref)
+
+
+@pytest.mark.parametrize("n", [2, 3, 5, 17, 64, 65, 1000, 4097])
+def test_lbvh_small_sizes(oracle, ctx, n):
+    v, t = scenes.uniform_soup(n, seed=10 + n)
+    _check_build(oracle, ctx, v, t, 60)
+
+
+def test_lbvh_single_triangle(oracle, ctx):
+    v, t = scenes.uniform_soup(1, seed=3)
+    ref, scene, bvh, nodes, tri, woop = _check_build(oracle, ctx, v, t, 60, check_binary=False)
+    assert bvh.n_interior == 1 and bvh.n_leaves == 1
+
+
+def test_lbvh_duplicate_keys(oracle, ctx):
+    """All centroids equal: every Morton key ties; Karras ties break on index."""
+    rng = np.random.default_rng(7)
+    n = 3000
+    base = np.array([[0.2, 0.3, 0.4], [0.25, 0.3, 0.4], [0.2, 0.35, 0.45]], np.float32)
+    v = np.tile(base, (n, 1)).astype(np.float32)
+    t = np.arange(3 * n, dtype=np.int32).reshape(n, 3)
+    # one outlier so the scene box is not degenerate
+    v[-3:] += np.float32(0.5)
+    _check_build(oracle, ctx, v, t, 60)
+    _check_build(oracle, ctx, v, t, 30)
+    del rng
+
+
+def test_lbvh_indexed_grid_and_degenerate(oracle, ctx):
+    """Shared-vertex grid (flat in y) plus zero-area triangles."""
+    gv, gt = scenes.ground_grid(40)
+    dv = np.repeat(np.array([[0.5, 0.5, 0.5]], np.float32), 30, axis=0)
+    dt = np.arange(30, dtype=np.int32).reshape(10, 3)
+    v, t = scenes.merge((gv, gt), (dv, dt))
+    _check_build(oracle, ctx, v, t, 60)
+
+
+@pytest.mark.parametrize("max_leaf", [1, 4, 8])
+def test_lbvh_fixed_leaf_size(oracle, ctx, max_leaf):
+    v, t = scenes.uniform_soup(20000, seed=4)
+    _check_build(oracle, ctx, v, t, 60, adaptive=False, max_leaf=max_leaf)
+
+
+def test_lbvh_gaussian_mixture(oracle, ctx):
+    v, t = scenes.config2_scene(n_mixture=50000, grid_cells=100, seed=2)
+    _check_build(oracle, ctx, v, t, 60)
+
+
+def test_rebuild_matches_build(oracle, ctx):
+    v, t = scenes.uniform_soup(5000, seed=8)
+    scene = hx.Scene(v, t)
+    builder = hx.LBVHBuilder(ctx, morton_bits=60)
+    bvh = builder.build(scene)
+    a = oracle.bin_canonical(bvh.nodes_host(), bvh.n_interior, bvh.n_leaves, bvh.tri_indices_host())
+    v2, t2 = scenes.uniform_soup(5000, seed=9)
+    scene2 = hx.Scene(v2, t2)
+    builder.rebuild(bvh, scene2)
+    builder.rebuild(bvh, scene)
+    b = oracle.bin_canonical(bvh.nodes_host(), bvh.n_interior, bvh.n_leaves, bvh.tri_indices_host())
+    assert np.array_equal(a, b)
+
+
+def test_morton_stage_entry(oracle, ctx):
+    v, t = scenes.uniform_soup(10000, seed=12)
+    scene = hx.Scene(v, t)
+    box = hx.scene_box(ctx, scene)
+    sb = oracle.scene_box(v, t)
+    assert np.array_equal(box.cpu().numpy(), sb)
+    bmin, bmax = oracle.setup_boxes(v, t)
+    for bits in (6, 30, 45, 60):
+        keys = hx.morton_codes(ctx, scene, box, bits).cpu().numpy().view(np.uint64)
+        assert np.array_equal(keys, oracle.morton_codes(bmin, bmax, sb, bits)), bits
+
+
+@pytest.mark.parametrize("n", [1, 2, 100, 4095, 4096, 4097, 100000, 1 << 20])
+@pytest.mark.parametrize("kind", ["random", "dups", "sorted", "reversed", "equal"])
+def test_sort_pairs(ctx, n, kind):
+    rng = np.random.default_rng(n)
+    bits = 60
+    if kind == "random":
+        k = rng.integers(0, 1 << bits, n, dtype=np.uint64)
+    elif kind == "dups":
+        k = rng.integers(0, 50, n, dtype=np.uint64) << np.uint64(20)
+    elif kind == "sorted":
+        k = np.sort(rng.integers(0, 1 << bits, n, dtype=np.uint64))
+    elif kind == "reversed":
+        k = np.sort(rng.integers(0, 1 << bits, n, dtype=np.uint64))[::-1].copy()
+    else:
+        k = np.full(n, 12345, np.uint64)
+    vals = rng.permutation(n).astype(np.int32)
+    kt = torch.from_numpy(k.view(np.int64).copy()).cuda()
+    vt = torch.from_numpy(vals.copy()).cuda()
+    hx.sort_pairs(ctx, kt, vt, bits)
+    order = np.argsort(k, kind="stable")
+    assert np.array_equal(kt.cpu().numpy().view(np.uint64), k[order])
+    assert np.array_equal(vt.cpu().numpy(), vals[order])
+
+
+@pytest.mark.parametrize("end_bit", [4, 13, 30, 33, 64])
+def test_sort_pairs_bit_ranges(ctx, end_bit):
+    n = 50000
+    rng = np.random.default_rng(end_bit)
+    hi = np.uint64((1 << end_bit) - 1) if end_bit < 64 else np.uint64(0xFFFFFFFFFFFFFFFF)
+    k = rng.integers(0, 1 << 63, n, dtype=np.uint64) & hi
+    vals = np.arange(n, dtype=np.int32)
+    kt = torch.from_numpy(k.view(np.int64).copy()).cuda()
+    vt = torch.from_numpy(vals.copy()).cuda()
+    hx.sort_pairs(ctx, kt, vt, end_bit)
+    order = np.argsort(k, kind="stable")
+    assert np.array_equal(vt.cpu().numpy(), vals[order])
