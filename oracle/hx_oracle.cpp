@@ -825,6 +825,14 @@ int hxo_validate_bin(const int32_t* nodes, int64_t n_interior, int64_t n_leaves,
     }
     for (int64_t r = 0; r < n_refs; ++r)
         if (!refSeen[r]) return 15;
+    /* Every triangle is referenced (HipBVH.cpp:290-303); without presplitting
+     * the references are a permutation of the triangle ids. */
+    std::vector<int> triSeen(n_refs, 0);
+    for (int64_t r = 0; r < n_refs; ++r) {
+        int32_t id = tri_idx[r];
+        if (id < 0 || id >= n_refs || triSeen[id]) return 16;
+        triSeen[id] = 1;
+    }
     return 0;
 }
 
