@@ -37,6 +37,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=2097152, help="triangles in the CPU baseline sample")
     ap.add_argument("--cpu-rays", type=int, default=262144, help="rays in the CPU trace sample")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="run one plain device copy of this many MiB first (calibrates PMC byte counters)")
     return ap.parse_args()
 
 
@@ -79,6 +81,15 @@ def main():
 
     import hippie_amd as hx
     from hippie_amd import scenes
+
+    if args.copy_probe_mib > 0:
+        nelem = args.copy_probe_mib * (1 << 20) // 4
+        src = torch.ones(nelem, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy probe: {nelem * 4} bytes read, {nelem * 4} bytes written")
+        del src, dst
 
     t_setup = time.perf_counter()
     log("generating scene")
