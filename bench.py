@@ -37,6 +37,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=2097152, help="triangles in the CPU baseline sample")
     ap.add_argument("--cpu-rays", type=int, default=262144, help="rays in the CPU trace sample")
+    ap.add_argument("--tune", action="append", default=[], metavar="NAME=VALUE",
+                    help="libhx launch tunable (hx_context_set_tuning), e.g. trace_lds_stack=16")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run one plain device copy of this many MiB first (calibrates PMC byte counters)")
     return ap.parse_args()
@@ -98,6 +100,9 @@ def main():
     log(f"{n} triangles; generating {args.res}x{args.res} primary rays")
     prim = scenes.config2_primary_rays(args.res, args.res)
     ctx = hx.Context.default(0)
+    for kv in args.tune:
+        name, value = kv.split("=")
+        ctx.set_tuning(name, int(value))
     scene = hx.Scene(verts, tris)
     builder = hx.LBVHBuilder(ctx, morton_bits=60)
     bvh = builder.build(scene)

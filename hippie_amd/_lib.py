@@ -95,6 +95,8 @@ SIGNATURES = {
     "hx_context_profile_entry": [_p, _i32, ctypes.c_char_p, _i32, ctypes.POINTER(_i64),
                                  ctypes.POINTER(ctypes.c_double)],
     "hx_context_profile_reset": [_p],
+    "hx_context_set_tuning": [_p, ctypes.c_char_p, _i32],
+    "hx_context_get_tuning": [_p, ctypes.c_char_p, ctypes.POINTER(_i32)],
     "hx_build_params_default": [ctypes.POINTER(HxBuildParams)],
     "hx_build": [_p, ctypes.POINTER(HxBuildParams), _p, _i64, _p, _i64, _pp],
     "hx_rebuild": [_p, _p, _p, _p],

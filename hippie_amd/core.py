@@ -80,6 +80,14 @@ class Context:
     def profile_reset(self):
         call("hx_context_profile_reset", self.handle)
 
+    def set_tuning(self, name, value):
+        call("hx_context_set_tuning", self.handle, name.encode(), int(value))
+
+    def get_tuning(self, name):
+        v = ctypes.c_int32()
+        call("hx_context_get_tuning", self.handle, name.encode(), ctypes.byref(v))
+        return v.value
+
     def memcpy(self, dst_ptr, src_ptr, nbytes):
         call("hx_memcpy", self.handle, ctypes.c_void_p(dst_ptr), ctypes.c_void_p(src_ptr), int(nbytes))
 

@@ -313,6 +313,7 @@ int trace_common(hx_context* ctx, hx_bvh* bvh, const float* d_rays, int64_t n_ra
     a.stats = reinterpret_cast<int2*>(d_stats);
     a.order = d_order;
     a.control = c->control.as<int32_t>();
+    a.fetch_threshold = c->tune.trace_fetch;
     return hx::launch_trace(c, a);
 }
 
@@ -401,6 +402,35 @@ int hx_context_profile_entry(hx_context* ctx, int32_t index, char* name, int32_t
     }
     if (launches) *launches = s.count;
     if (total_ms) *total_ms = s.total_ms;
+    return HX_OK;
+}
+
+int hx_context_set_tuning(hx_context* ctx, const char* name, int32_t value) {
+    HX_CHECK_ARG(ctx && name, "NULL argument");
+    hx::Tuning& t = ctx->c.tune;
+    if (!std::strcmp(name, "trace_lds_stack")) {
+        HX_CHECK_ARG(value == 0 || value == 16 || value == 24 || value == 32 || value == 64,
+                     "trace_lds_stack must be 0, 16, 24, 32 or 64");
+        t.trace_lds_stack = value;
+    } else if (!std::strcmp(name, "trace_fetch")) {
+        HX_CHECK_ARG(value >= 1 && value <= 65, "trace_fetch must be in [1, 65]");
+        t.trace_fetch = value;
+    } else {
+        hx::set_error("unknown tuning parameter '%s'", name);
+        return HX_ERR_INVALID_ARG;
+    }
+    return HX_OK;
+}
+
+int hx_context_get_tuning(hx_context* ctx, const char* name, int32_t* value) {
+    HX_CHECK_ARG(ctx && name && value, "NULL argument");
+    const hx::Tuning& t = ctx->c.tune;
+    if (!std::strcmp(name, "trace_lds_stack")) *value = t.trace_lds_stack;
+    else if (!std::strcmp(name, "trace_fetch")) *value = t.trace_fetch;
+    else {
+        hx::set_error("unknown tuning parameter '%s'", name);
+        return HX_ERR_INVALID_ARG;
+    }
     return HX_OK;
 }
 

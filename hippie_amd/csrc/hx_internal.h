@@ -64,10 +64,17 @@ struct Profiler {
     ~Profiler();
 };
 
+// Launch-shape choices the autotuner may change (hx_context_set_tuning).
+struct Tuning {
+    int trace_lds_stack = 16;   // traversal-stack entries kept in LDS: 0, 16, 24, 32, 64
+    int trace_fetch = 40;       // refill a wave when fewer lanes than this are still busy
+};
+
 struct Context {
     int device = 0;
     hipStream_t stream = nullptr;
     int cu_count = 0;
+    Tuning tune;
     Profiler prof;
     // Scratch reused across calls (grow-only).
     DevBuf sort_status, sort_hist, scan_partials, control;
@@ -165,6 +172,7 @@ struct TraceArgs {
     int2* stats;            // optional
     const int32_t* order;   // optional: ray slot -> ray index (sorted trace)
     int32_t* control;       // [0] ray counter, [1] overflow counter
+    int fetch_threshold;    // dynamic fetch: busy lanes below which a wave refills
 };
 int launch_trace(Context* c, const TraceArgs& a);
 int launch_ray_keys(Context* c, const float4* rays, int64_t n, int method, int bits, float ray_length,

@@ -18,7 +18,6 @@ namespace {
 
 constexpr int kTraceBlock = 256;
 constexpr int kStackSize = 64;                 // HipTracerBinKernels.cu:5
-constexpr int kDynamicFetch = 40;              // of 64 lanes (reference: 20 of 32)
 constexpr int32_t kSentinel = 0x76543210;      // HipTracerKernels.h:11
 
 // fminf/fmaxf written as selects, and the integer min/max on float bits of
@@ -34,9 +33,26 @@ HX_DEV float span_end(float a0, float a1, float b0, float b1, float c0, float c1
     return __int_as_float(min(min(__float_as_int(fmax_s(a0, a1)), __float_as_int(fmax_s(b0, b1))), c));
 }
 
-template <bool STATS, bool ORDERED>
+// Traversal stack: entries [0, LDS_DEPTH) in LDS, one column per lane (entry e
+// of lane t at e * kTraceBlock + t, so a wave's accesses hit distinct banks);
+// deeper entries in private (scratch) memory, touched only by deep traversals.
+template <int LDS_DEPTH>
+struct TraversalStack {
+    static constexpr int kDeep = kStackSize - LDS_DEPTH > 0 ? kStackSize - LDS_DEPTH : 1;
+    int32_t* col;
+    int32_t deep[kDeep];
+    HX_DEV void put(int e, int32_t v) {
+        if (e < LDS_DEPTH) col[e * kTraceBlock] = v;
+        else deep[e - LDS_DEPTH] = v;
+    }
+    HX_DEV int32_t get(int e) const { return e < LDS_DEPTH ? col[e * kTraceBlock] : deep[e - LDS_DEPTH]; }
+};
+
+template <bool STATS, bool ORDERED, int LDS_DEPTH>
 __global__ __launch_bounds__(kTraceBlock) void k_trace(TraceArgs a) {
-    int32_t stack[kStackSize];
+    __shared__ int32_t s_stack[LDS_DEPTH > 0 ? LDS_DEPTH * kTraceBlock : 1];
+    TraversalStack<LDS_DEPTH> stack;
+    stack.col = s_stack + threadIdx.x;
     int sp = 0;
     float ox = 0.f, oy = 0.f, oz = 0.f, tmin = 0.f, dx = 0.f, dy = 0.f, dz = 0.f;
     float idx = 0.f, idy = 0.f, idz = 0.f, oodx = 0.f, oody = 0.f, oodz = 0.f;
@@ -71,7 +87,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(TraceArgs a) {
             oodx = ox * idx;
             oody = oy * idy;
             oodz = oz * idz;
-            stack[0] = kSentinel;
+            stack.put(0, kSentinel);
             sp = 0;
             leafAddr = 0;
             nodeAddr = 0;
@@ -109,7 +125,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(TraceArgs a) {
                 const bool t1 = c1max >= c1min;
                 int32_t cy = cn.y;
                 if (!t0 && !t1) {
-                    nodeAddr = stack[sp--];
+                    nodeAddr = stack.get(sp--);
                 } else {
                     nodeAddr = t0 ? cn.x : cy;
                     if (t0 && t1) {
@@ -124,12 +140,12 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(TraceArgs a) {
                             leafAddr = 0;
                             break;
                         }
-                        stack[++sp] = cy;
+                        stack.put(++sp, cy);
                     }
                 }
                 if (nodeAddr < 0 && leafAddr >= 0) {
                     leafAddr = nodeAddr;
-                    nodeAddr = stack[sp--];
+                    nodeAddr = stack.get(sp--);
                 }
                 if (!__any(leafAddr >= 0)) break;
             }
@@ -171,10 +187,10 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(TraceArgs a) {
                     }
                 }
                 leafAddr = nodeAddr;
-                if (nodeAddr < 0) nodeAddr = stack[sp--];
+                if (nodeAddr < 0) nodeAddr = stack.get(sp--);
             }
 
-            if (__popcll(ballot64(true)) < kDynamicFetch) break;
+            if (__popcll(ballot64(true)) < a.fetch_threshold) break;
         }
 
         if (nodeAddr == kSentinel) {
@@ -286,15 +302,26 @@ __global__ void k_zero_control(int32_t* control) {
     if (threadIdx.x < 2) control[threadIdx.x] = 0;
 }
 
-template <bool STATS, bool ORDERED>
+template <bool STATS, bool ORDERED, int LDS_DEPTH>
 int launch_trace_t(Context* c, const TraceArgs& a) {
     int per_cu = 0;
-    HX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<STATS, ORDERED>, kTraceBlock, 0));
+    HX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<STATS, ORDERED, LDS_DEPTH>, kTraceBlock, 0));
     per_cu = std::max(per_cu, 1);
     const int64_t want = (a.n_rays + kTraceBlock - 1) / kTraceBlock;
     const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>((int64_t)c->cu_count * per_cu, want));
-    hipLaunchKernelGGL((k_trace<STATS, ORDERED>), dim3(blocks), dim3(kTraceBlock), 0, c->stream, a);
+    hipLaunchKernelGGL((k_trace<STATS, ORDERED, LDS_DEPTH>), dim3(blocks), dim3(kTraceBlock), 0, c->stream, a);
     return check_launch("trace");
+}
+
+template <bool STATS, bool ORDERED>
+int launch_trace_depth(Context* c, const TraceArgs& a) {
+    switch (c->tune.trace_lds_stack) {
+        case 0: return launch_trace_t<STATS, ORDERED, 0>(c, a);
+        case 16: return launch_trace_t<STATS, ORDERED, 16>(c, a);
+        case 24: return launch_trace_t<STATS, ORDERED, 24>(c, a);
+        case 32: return launch_trace_t<STATS, ORDERED, 32>(c, a);
+        default: return launch_trace_t<STATS, ORDERED, 64>(c, a);
+    }
 }
 
 }  // namespace
@@ -305,8 +332,8 @@ int launch_trace(Context* c, const TraceArgs& a) {
     HX_TRY(check_launch("zero_control"));
     LaunchScope ls(c, a.order ? "trace_sorted" : "trace");
     if (a.stats)
-        return a.order ? launch_trace_t<true, true>(c, a) : launch_trace_t<true, false>(c, a);
-    return a.order ? launch_trace_t<false, true>(c, a) : launch_trace_t<false, false>(c, a);
+        return a.order ? launch_trace_depth<true, true>(c, a) : launch_trace_depth<true, false>(c, a);
+    return a.order ? launch_trace_depth<false, true>(c, a) : launch_trace_depth<false, false>(c, a);
 }
 
 int launch_ray_keys(Context* c, const float4* rays, int64_t n, int method, int bits, float ray_length,

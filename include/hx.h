@@ -140,6 +140,12 @@ int hx_context_profile_count(hx_context* ctx, int32_t* count);
 int hx_context_profile_entry(hx_context* ctx, int32_t index, char* name, int32_t name_len,
                              int64_t* launches, double* total_ms);
 int hx_context_profile_reset(hx_context* ctx);
+/* Launch-shape tunables chosen by the autotuner (no reference counterpart;
+ * the reference hard-codes them, HipTracerBinKernels.cu:5-6,10-15):
+ *   "trace_lds_stack"  traversal-stack entries held in LDS (0,16,24,32,64)
+ *   "trace_fetch"      refill a wave when fewer of its 64 lanes are busy   */
+int hx_context_set_tuning(hx_context* ctx, const char* name, int32_t value);
+int hx_context_get_tuning(hx_context* ctx, const char* name, int32_t* value);
 
 /* ---- builders (BVHBuilder.h:37-39) ---------------------------------------- */
 int hx_build_params_default(hx_build_params* p);
