@@ -112,6 +112,8 @@ SIGNATURES = {
     "hx_trace_sorted": [_p, _p, _p, _i64, _i32, _i32, _i32, _f32, _p, _p, _p],
     "hx_ray_keys": [_p, _p, _i64, _i32, _i32, _f32, _p],
     "hx_memcpy": [_p, _p, _p, _i64],
+    "hx_backward": [_p, _p, _p, _p, _p, _i64, _p, _p, _p, _p],
+    "hx_recompute_hits": [_p, _p, _p, _p, _p, _i64, _p, _p, _p],
     "hx_trace_overflow_count": [_p, ctypes.POINTER(_i64)],
 }
 _RESTYPE = {"hx_status_string": ctypes.c_char_p, "hx_last_error": ctypes.c_char_p}

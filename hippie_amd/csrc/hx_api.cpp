@@ -134,7 +134,8 @@ struct hx_bvh {
     DevBuf nodes, woop, tri_idx;
     // Build workspace.
     DevBuf keys[2], vals_tmp, left, right, parent, other, counters, nmin, nmax, cost, state, cover, leaf_num,
-        int_slot, counts, scene_box, cost_out;
+        int_slot, counts, scene_box, cost_out, frontier;
+    bool built = false;
     bool counts_known = false;
     int64_t n_interior = 0, n_leaves = 0;
     float host_box[6] = {0, 0, 0, 0, 0, 0};
@@ -191,10 +192,57 @@ int reserve_bvh(hx_bvh* b, int64_t n) {
     HX_TRY(b->cover.reserve(4 * (size_t)n));
     HX_TRY(b->leaf_num.reserve(4 * (size_t)n));
     HX_TRY(b->int_slot.reserve(4 * (size_t)n));
+    HX_TRY(b->frontier.reserve(4 * (size_t)n));
     HX_TRY(b->counts.reserve(16));
     HX_TRY(b->scene_box.reserve(32));
     HX_TRY(b->cost_out.reserve(16));
     return HX_OK;
+}
+
+hx::ClimbArgs climb_args(hx_bvh* b, bool refit_only) {
+    const hx_build_params& p = b->params;
+    hx::ClimbArgs ca{};
+    ca.n = b->n_refs;
+    ca.sorted_tri = b->tri_idx.as<int32_t>();
+    ca.verts = b->verts;
+    ca.tris = b->tris;
+    ca.left = b->left.as<int32_t>();
+    ca.right = b->right.as<int32_t>();
+    ca.parent = b->parent.as<int32_t>();
+    ca.other = b->other.as<int32_t>();
+    ca.counters = b->counters.as<int32_t>();
+    ca.nmin = b->nmin.as<float4>();
+    ca.nmax = b->nmax.as<float4>();
+    ca.cost = b->cost.as<float>();
+    ca.state = b->state.as<int32_t>();
+    ca.end_at = b->cover.as<int32_t>();
+    ca.woop = b->woop.as<float4>();
+    ca.frontier = b->frontier.as<int32_t>();
+    ca.frontier_count = b->counts.as<int32_t>() + 2;
+    ca.refit_only = refit_only ? 1 : 0;
+    ca.adaptive = (p.adaptive_leaf_size && !refit_only) ? 1 : 0;
+    ca.max_leaf_size = p.max_leaf_size;
+    ca.ct = p.ct;
+    ca.ci = p.ci;
+    return ca;
+}
+
+hx::EmitArgs emit_args(hx_bvh* b) {
+    hx::EmitArgs ea{};
+    ea.n = b->n_refs;
+    ea.left = b->left.as<int32_t>();
+    ea.right = b->right.as<int32_t>();
+    ea.parent = b->parent.as<int32_t>();
+    ea.other = b->other.as<int32_t>();
+    ea.state = b->state.as<int32_t>();
+    ea.nmin = b->nmin.as<float4>();
+    ea.nmax = b->nmax.as<float4>();
+    ea.cover = b->cover.as<int32_t>();
+    ea.leaf_num = b->leaf_num.as<int32_t>();
+    ea.int_slot = b->int_slot.as<int32_t>();
+    ea.counts = b->counts.as<int32_t>();
+    ea.nodes = b->nodes.as<int4>();
+    return ea;
 }
 
 int run_lbvh(hx_bvh* b) {
@@ -230,47 +278,30 @@ int run_lbvh(hx_bvh* b) {
         HX_TRY(hx::launch_construct(c, keys[ri], b->keys64, n, b->left.as<int32_t>(), b->right.as<int32_t>(),
                                     b->parent.as<int32_t>(), b->other.as<int32_t>(), b->counters.as<int32_t>(),
                                     b->cover.as<int32_t>(), b->counts.as<int32_t>() + 2));
-        hx::ClimbArgs ca{};
-        ca.n = n;
-        ca.sorted_tri = sorted_tri;
-        ca.verts = b->verts;
-        ca.tris = b->tris;
-        ca.left = b->left.as<int32_t>();
-        ca.right = b->right.as<int32_t>();
-        ca.parent = b->parent.as<int32_t>();
-        ca.other = b->other.as<int32_t>();
-        ca.counters = b->counters.as<int32_t>();
-        ca.nmin = b->nmin.as<float4>();
-        ca.nmax = b->nmax.as<float4>();
-        ca.cost = b->cost.as<float>();
-        ca.state = b->state.as<int32_t>();
-        ca.end_at = b->cover.as<int32_t>();
-        ca.woop = b->woop.as<float4>();
-        ca.frontier = b->leaf_num.as<int32_t>();  // free until the collapse scans
-        ca.frontier_count = b->counts.as<int32_t>() + 2;
-        ca.adaptive = p.adaptive_leaf_size ? 1 : 0;
-        ca.max_leaf_size = p.max_leaf_size;
-        ca.ct = p.ct;
-        ca.ci = p.ci;
+        const hx::ClimbArgs ca = climb_args(b, false);
         HX_TRY(hx::launch_climb(c, ca));
-        hx::EmitArgs ea{};
-        ea.n = n;
-        ea.left = ca.left;
-        ea.right = ca.right;
-        ea.parent = ca.parent;
-        ea.other = ca.other;
-        ea.state = ca.state;
-        ea.nmin = ca.nmin;
-        ea.nmax = ca.nmax;
-        ea.cover = b->cover.as<int32_t>();
-        ea.leaf_num = b->leaf_num.as<int32_t>();
-        ea.int_slot = b->int_slot.as<int32_t>();
-        ea.counts = b->counts.as<int32_t>();
-        ea.nodes = b->nodes.as<int4>();
+        const hx::EmitArgs ea = emit_args(b);
         HX_TRY(hx::launch_collapse_emit(c, ea));
     }
     b->counts_known = false;
+    b->built = true;
     return HX_OK;
+}
+
+// HipBVH::update (HipBVH.cpp:472-477): same topology, boxes and Woop rows
+// from the new vertices.  The binary climb reruns in boxes-only mode and the
+// emit kernel rewrites every node with the build's slots and collapse states.
+int run_refit(hx_bvh* b) {
+    Context* c = &b->ctx->c;
+    const int64_t n = b->n_refs;
+    HX_HIP(hipSetDevice(c->device));
+    const int32_t* sorted_tri = b->tri_idx.as<int32_t>();
+    if (n == 1)
+        return hx::launch_single_ref(c, b->verts, b->tris, sorted_tri, b->nmin.as<float4>(), b->nmax.as<float4>(),
+                                     b->woop.as<float4>(), b->nodes.as<int4>(), b->counts.as<int32_t>());
+    HX_TRY(hx::launch_refit_reset(c, b->counters.as<int32_t>(), n - 1, b->counts.as<int32_t>() + 2));
+    HX_TRY(hx::launch_climb(c, climb_args(b, true)));
+    return hx::launch_emit_only(c, emit_args(b));
 }
 
 int read_counts(hx_bvh* b) {
@@ -500,8 +531,46 @@ int hx_rebuild(hx_context* ctx, hx_bvh* bvh, const float* d_vertices, const int3
 
 int hx_refit(hx_context* ctx, hx_bvh* bvh, const float* d_vertices, const int32_t* d_triangles) {
     HX_CHECK_ARG(ctx && bvh && d_vertices && d_triangles, "NULL argument");
-    hx::set_error("hx_refit is not available in this build");
-    return HX_ERR_UNSUPPORTED;
+    HX_CHECK_ARG(bvh->built, "the BVH has not been built");
+    bvh->ctx = ctx;
+    bvh->verts = d_vertices;
+    bvh->tris = d_triangles;
+    return run_refit(bvh);
+}
+
+int hx_backward(hx_context* ctx, const float* d_vertices, const int32_t* d_triangles, const float* d_rays,
+                const int32_t* d_results, int64_t n_rays, const float* d_dl_dt, const float* d_dl_du,
+                const float* d_dl_dv, float* d_dl_dvertices) {
+    HX_CHECK_ARG(ctx && d_vertices && d_triangles && d_dl_dvertices, "NULL argument");
+    HX_CHECK_ARG(n_rays >= 0, "n_rays must be >= 0");
+    if (n_rays == 0) return HX_OK;
+    HX_CHECK_ARG(d_rays && d_results, "rays and results must be device pointers");
+    hx::BackwardArgs a{};
+    a.n_rays = n_rays;
+    a.verts = d_vertices;
+    a.tris = d_triangles;
+    a.rays = reinterpret_cast<const float4*>(d_rays);
+    a.results = reinterpret_cast<const int4*>(d_results);
+    a.dl_dt = d_dl_dt;
+    a.dl_du = d_dl_du;
+    a.dl_dv = d_dl_dv;
+    a.dl_dverts = d_dl_dvertices;
+    return hx::launch_backward(&ctx->c, a);
+}
+
+int hx_recompute_hits(hx_context* ctx, const float* d_vertices, const int32_t* d_triangles, const float* d_rays,
+                      const int32_t* d_results, int64_t n_rays, float* d_t, float* d_u, float* d_v) {
+    HX_CHECK_ARG(ctx && d_vertices && d_triangles && d_t && d_u && d_v, "NULL argument");
+    HX_CHECK_ARG(n_rays >= 0, "n_rays must be >= 0");
+    if (n_rays == 0) return HX_OK;
+    HX_CHECK_ARG(d_rays && d_results, "rays and results must be device pointers");
+    hx::BackwardArgs a{};
+    a.n_rays = n_rays;
+    a.verts = d_vertices;
+    a.tris = d_triangles;
+    a.rays = reinterpret_cast<const float4*>(d_rays);
+    a.results = reinterpret_cast<const int4*>(d_results);
+    return hx::launch_recompute_hits(&ctx->c, a, d_t, d_u, d_v);
 }
 
 int hx_bvh_destroy(hx_bvh* bvh) {

@@ -354,9 +354,11 @@ __global__ __launch_bounds__(kClimbThreads) void k_climb_local(ClimbArgs a, int3
             s_cost[sn] = c.cost;
             a.nmin[node] = make_float4(c.mn.x, c.mn.y, c.mn.z, 0.0f);
             a.nmax[node] = make_float4(c.mx.x, c.mx.y, c.mx.z, 0.0f);
-            if (a.adaptive) a.cost[node] = c.cost;
-            a.state[node] = c.state;
-            if (c.state == 0 && node != 0) atomicMax(&a.end_at[first], last);
+            if (!a.refit_only) {
+                if (a.adaptive) a.cost[node] = c.cost;
+                a.state[node] = c.state;
+                if (c.state == 0 && node != 0) atomicMax(&a.end_at[first], last);
+            }
             child = node;
             node = a.parent[node];
         }
@@ -398,9 +400,11 @@ __global__ __launch_bounds__(kBlock) void k_climb_top(ClimbArgs a, const int32_t
             const Combined c = combine(a, first, last, xyz(lmn), xyz(lmx), lc, xyz(rmn), xyz(rmx), rc);
             st_agent(a.nmin + node, make_float4(c.mn.x, c.mn.y, c.mn.z, 0.0f));
             st_agent(a.nmax + node, make_float4(c.mx.x, c.mx.y, c.mx.z, 0.0f));
-            if (a.adaptive) st_agent(a.cost + node, c.cost);
-            a.state[node] = c.state;
-            if (c.state == 0 && node != 0) atomicMax(&a.end_at[first], last);
+            if (!a.refit_only) {
+                if (a.adaptive) st_agent(a.cost + node, c.cost);
+                a.state[node] = c.state;
+                if (c.state == 0 && node != 0) atomicMax(&a.end_at[first], last);
+            }
             // Complete this node's stores before the parent's counter can be
             // observed (the compiler fence keeps the stores above the wait).
             __atomic_signal_fence(__ATOMIC_SEQ_CST);
@@ -682,6 +686,12 @@ __global__ void k_cost_final(const double* __restrict__ partial, int nblocks, do
     }
 }
 
+__global__ void k_refit_reset(int32_t* __restrict__ counters, int64_t n_internal, int32_t* __restrict__ frontier_count) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n_internal) counters[i] = 0;
+    if (i == 0) *frontier_count = 0;
+}
+
 __global__ void k_fill_i32(int32_t* p, int32_t v, int64_t n) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -780,6 +790,20 @@ int launch_collapse_emit(Context* c, const EmitArgs& a) {
         LaunchScope ls(c, "collapse_emit");
         hipLaunchKernelGGL(k_emit, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, a, f);
     }
+    return check_launch("emit");
+}
+
+int launch_refit_reset(Context* c, int32_t* counters, int64_t n_internal, int32_t* frontier_count) {
+    LaunchScope ls(c, "refit_reset");
+    hipLaunchKernelGGL(k_refit_reset, dim3(grid_for(std::max<int64_t>(n_internal, 1))), dim3(kBlock), 0, c->stream,
+                       counters, n_internal, frontier_count);
+    return check_launch("refit_reset");
+}
+
+int launch_emit_only(Context* c, const EmitArgs& a) {
+    FlagCtx f{(int)a.n, a.cover, a.other, a.state};
+    LaunchScope ls(c, "collapse_emit");
+    hipLaunchKernelGGL(k_emit, dim3(grid_for(a.n)), dim3(kBlock), 0, c->stream, a, f);
     return check_launch("emit");
 }
 

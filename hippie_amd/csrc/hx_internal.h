@@ -124,7 +124,8 @@ struct ClimbArgs {
     int32_t* end_at;
     float4* woop;
     int32_t* frontier;        // n entries: nodes whose parent straddles a climb block
-    int32_t* frontier_count;  // zeroed by launch_construct
+    int32_t* frontier_count;  // zeroed by launch_construct (or launch_refit_reset)
+    int refit_only;           // boxes and Woop only: keep states, costs and cover
     int adaptive;
     int max_leaf_size;
     float ct, ci;
@@ -146,6 +147,24 @@ struct EmitArgs {
     int4* nodes;         // out: Bin nodes
 };
 int launch_collapse_emit(Context* c, const EmitArgs& a);
+// Refit: re-zero the climb counters and frontier count, then (after the
+// climb) rewrite the Bin nodes from the binary boxes with the build's slots.
+int launch_refit_reset(Context* c, int32_t* counters, int64_t n_internal, int32_t* frontier_count);
+int launch_emit_only(Context* c, const EmitArgs& a);
+// Adjoint of closest-hit t/u/v with respect to the triangle vertices.
+struct BackwardArgs {
+    int64_t n_rays;
+    const float* verts;
+    const int32_t* tris;
+    const float4* rays;
+    const int4* results;
+    const float* dl_dt;   // per ray, may be NULL
+    const float* dl_du;   // per ray, may be NULL
+    const float* dl_dv;   // per ray, may be NULL
+    float* dl_dverts;     // float[3V], accumulated
+};
+int launch_backward(Context* c, const BackwardArgs& a);
+int launch_recompute_hits(Context* c, const BackwardArgs& a, float* t_out, float* u_out, float* v_out);
 int launch_single_ref(Context* c, const float* verts, const int32_t* tris, const int32_t* sorted_tri,
                       float4* nmin, float4* nmax, float4* woop, int4* nodes, int32_t* counts);
 int launch_cost(Context* c, const int4* nodes, const int32_t* d_counts, int64_t max_nodes,

@@ -161,6 +161,85 @@ def secondary_rays(verts, tris, rays, results, seed=3, offset=1e-4, tmax=3.0):
     return out
 
 
+def icosphere(level):
+    """Unit icosphere: 20 * 4^level triangles, 10 * 4^level + 2 shared vertices."""
+    p = (1.0 + 5 ** 0.5) / 2.0
+    v = np.array([[-1, p, 0], [1, p, 0], [-1, -p, 0], [1, -p, 0], [0, -1, p], [0, 1, p], [0, -1, -p], [0, 1, -p],
+                  [p, 0, -1], [p, 0, 1], [-p, 0, -1], [-p, 0, 1]], np.float64)
+    f = np.array([[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4], [11, 10, 2],
+                  [10, 7, 6], [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9], [4, 9, 5],
+                  [2, 4, 11], [6, 2, 10], [8, 6, 7], [9, 8, 1]], np.int64)
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    for _ in range(level):
+        e = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]])
+        e.sort(axis=1)
+        uniq, inv = np.unique(e, axis=0, return_inverse=True)
+        inv = inv.reshape(-1)
+        mid = v[uniq[:, 0]] + v[uniq[:, 1]]
+        mid /= np.linalg.norm(mid, axis=1, keepdims=True)
+        m = inv + v.shape[0]
+        nf = f.shape[0]
+        a, b, c = m[:nf], m[nf:2 * nf], m[2 * nf:]
+        v = np.concatenate([v, mid])
+        f = np.concatenate([np.stack([f[:, 0], a, c], 1), np.stack([f[:, 1], b, a], 1),
+                            np.stack([f[:, 2], c, b], 1), np.stack([a, b, c], 1)])
+    return v, f.astype(np.int32)
+
+
+def value_noise3(p, seed, freq=4.0):
+    """Seeded 3D value noise in [-1, 1]: lattice values, smoothstep trilinear blend."""
+    rng = np.random.default_rng(seed)
+    size = 64
+    lattice = rng.uniform(-1.0, 1.0, (size, size, size))
+    q = (p + 2.0) * freq
+    i0 = np.floor(q).astype(np.int64)
+    fr = q - i0
+    s = fr * fr * (3.0 - 2.0 * fr)
+    out = np.zeros(p.shape[0])
+    for dx in (0, 1):
+        for dy in (0, 1):
+            for dz in (0, 1):
+                w = ((s[:, 0] if dx else 1 - s[:, 0]) * (s[:, 1] if dy else 1 - s[:, 1]) *
+                     (s[:, 2] if dz else 1 - s[:, 2]))
+                out += w * lattice[(i0[:, 0] + dx) % size, (i0[:, 1] + dy) % size, (i0[:, 2] + dz) % size]
+    return out
+
+
+def displaced_icosphere(level=8, seed=1, amplitude=0.05):
+    """Config 4 mesh: icosphere with radial displacement amplitude * noise(seed)."""
+    v, f = icosphere(level)
+    r = 1.0 + amplitude * value_noise3(v, seed)
+    return (v * r[:, None]).astype(np.float32), f
+
+
+def fibonacci_cameras(count=16, radius=3.0):
+    """Camera positions on a Fibonacci sphere, all looking at the origin."""
+    i = np.arange(count) + 0.5
+    phi = np.arccos(1.0 - 2.0 * i / count)
+    theta = np.pi * (1.0 + 5 ** 0.5) * i
+    return radius * np.stack([np.cos(theta) * np.sin(phi), np.sin(theta) * np.sin(phi), np.cos(phi)], axis=1)
+
+
+def orbit_rays(count=16, res=512, radius=3.0, fov_y_deg=30.0, tmax=10.0):
+    """Config 4 rays: res x res pinhole rays from each of `count` cameras."""
+    out = []
+    for c in fibonacci_cameras(count, radius):
+        up = (0.0, 1.0, 0.0) if abs(c[1]) < 0.99 * radius else (1.0, 0.0, 0.0)
+        r, u, f = _camera_basis(-c, up)
+        tan = math.tan(math.radians(fov_y_deg) * 0.5)
+        x = (2.0 * (np.arange(res) + 0.5) / res - 1.0) * tan
+        xx, yy = np.meshgrid(x, x, indexing="xy")
+        d = xx[..., None] * r + yy[..., None] * u + f
+        d = (d / np.linalg.norm(d, axis=-1, keepdims=True)).reshape(-1, 3)
+        rays = np.empty((res * res, 8), np.float32)
+        rays[:, 0:3] = c
+        rays[:, 3] = 0.0
+        rays[:, 4:7] = d
+        rays[:, 7] = tmax
+        out.append(rays)
+    return np.concatenate(out)
+
+
 def random_rays(n, seed=5, box=(0.0, 1.0), tmax=3.0):
     """Rays from uniform origins around the unit box aimed at uniform targets inside it."""
     rng = np.random.default_rng(seed)

@@ -154,8 +154,9 @@ int hx_build(hx_context* ctx, const hx_build_params* params, const float* d_vert
              const int32_t* d_triangles, int64_t n_triangles, hx_bvh** out);
 /* BVHBuilder::rebuild: rebuild in place from (possibly moved) geometry. */
 int hx_rebuild(hx_context* ctx, hx_bvh* bvh, const float* d_vertices, const int32_t* d_triangles);
-/* HipBVH::update (HipBVH.cpp:472-477): refit the final layout and re-woopify,
- * topology unchanged. */
+/* HipBVH::update (HipBVH.cpp:472-477; refitLeaves/refitInteriors,
+ * HipBVHKernels.cu:240-326): refit the final layout and re-woopify from new
+ * vertex positions, topology unchanged. */
 int hx_refit(hx_context* ctx, hx_bvh* bvh, const float* d_vertices, const int32_t* d_triangles);
 int hx_bvh_destroy(hx_bvh* bvh);
 /* Synchronises on first call after a build to read the node counts. */
@@ -189,6 +190,22 @@ int hx_trace_sorted(hx_context* ctx, hx_bvh* bvh, const float* d_rays, int64_t n
 /* RayBuffer::computeMortonCodes (RayBuffer.cpp:31-85) for the [0,1]^3 ray box. */
 int hx_ray_keys(hx_context* ctx, const float* d_rays, int64_t n_rays, int32_t key_method, int32_t key_bits,
                 float ray_length, uint64_t* d_keys);
+/* ---- differentiable closest hit (no reference counterpart) ------------------
+ * Adjoint of the closest-hit (t, u, v) of each ray with respect to the vertex
+ * positions, for the triangle ids in d_results (as written by hx_trace).  The
+ * hit is recomputed from the id; upstream gradients dL/dt, dL/du, dL/dv are
+ * float[n_rays] each (any may be NULL); the vertex gradients are ADDED into
+ * d_dl_dvertices (float[3V]; the caller zeroes it).  Topology stays discrete:
+ * the hit triangle is fixed, as is the BVH between refits. */
+int hx_backward(hx_context* ctx, const float* d_vertices, const int32_t* d_triangles, const float* d_rays,
+                const int32_t* d_results, int64_t n_rays, const float* d_dl_dt, const float* d_dl_du,
+                const float* d_dl_dv, float* d_dl_dvertices);
+/* Forward companion of hx_backward: (t, u, v) of each hit recomputed from its
+ * triangle id by the function hx_backward differentiates (solved in FP64,
+ * rounded once); misses get t = tmax and u = v = 0.  Outputs float[n_rays]. */
+int hx_recompute_hits(hx_context* ctx, const float* d_vertices, const int32_t* d_triangles, const float* d_rays,
+                      const int32_t* d_results, int64_t n_rays, float* d_t, float* d_u, float* d_v);
+
 /* Synchronous copy between any two host/device addresses on the context's
  * stream (Buffer migration, gpu/Buffer.cpp:437-498). */
 int hx_memcpy(hx_context* ctx, void* dst, const void* src, int64_t bytes);

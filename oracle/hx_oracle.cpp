@@ -608,6 +608,59 @@ void hxo_woopify(const float* verts, const int32_t* tris, const int32_t* tri_idx
     });
 }
 
+/* HipBVH::refit (HipBVH.cpp:36-93) with refitLeaves / refitInteriors
+ * (HipBVHKernels.cu:240-326).  The reference launches refitLeaves with 4 of
+ * its 6 parameters (HipBVH.cpp:43-48); this follows the kernels' intent.
+ * Interiors are processed children-first (post-order), the order a counter
+ * climb also guarantees; the child box is the child's getBoundingBox(), the
+ * grow-union of its two child boxes. */
+void hxo_refit_final(int32_t* nodes, int64_t n_interior, int64_t n_leaves, const int32_t* tri_idx,
+                     const float* verts, const int32_t* tris) {
+    const int64_t nn = n_interior + n_leaves;
+    auto node_box = [&](int64_t i) {
+        const int32_t* nd = nodes + 16 * i;
+        Box b;
+        for (int c = 0; c < 2; ++c) {
+            Box cb;
+            cb.mn = V3{w2f(nd[4 * c + 0]), w2f(nd[4 * c + 2]), w2f(nd[2 * c + 8])};
+            cb.mx = V3{w2f(nd[4 * c + 1]), w2f(nd[4 * c + 3]), w2f(nd[2 * c + 9])};
+            b.grow(cb.mn);
+            b.grow(cb.mx);
+        }
+        return b;
+    };
+    parallel_for(n_leaves, [&](int64_t k) {
+        int32_t* nd = nodes + 16 * (n_interior + k);
+        Box b;
+        for (int32_t r = nd[12]; r < nd[13]; ++r) {
+            V3 a, bb, c;
+            tri_verts(verts, tris, tri_idx[r], a, bb, c);
+            b.grow(a); b.grow(bb); b.grow(c);
+        }
+        set_child_box(nd, 0, b.mn, b.mx);
+        set_child_box(nd, 1, b.mn, b.mx);
+    });
+    std::vector<int64_t> order;
+    std::vector<int64_t> stack{0};
+    while (!stack.empty()) {
+        int64_t i = stack.back();
+        stack.pop_back();
+        order.push_back(i);
+        const int32_t* nd = nodes + 16 * i;
+        for (int c = 0; c < 2; ++c)
+            if (nd[12 + c] >= 0) stack.push_back(nd[12 + c]);
+    }
+    for (auto it = order.rbegin(); it != order.rend(); ++it) {
+        int32_t* nd = nodes + 16 * (*it);
+        for (int c = 0; c < 2; ++c) {
+            int64_t child = nd[12 + c] < 0 ? ~(int64_t)nd[12 + c] : nd[12 + c];
+            Box cb = node_box(child);
+            set_child_box(nd, c, cb.mn, cb.mx);
+        }
+    }
+    (void)nn;
+}
+
 double hxo_cost(const int32_t* nodes, int64_t n_nodes, float scene_area, float ct, float ci) {
     double sum = 0.0;
     for (int64_t i = 0; i < n_nodes; ++i) {

@@ -71,6 +71,11 @@ int64_t hxo_collapse(int64_t n, const int32_t* sorted_vals,
 void hxo_woopify(const float* verts, const int32_t* tris, const int32_t* tri_idx,
                  int64_t n, float* woop);
 
+/* Refit of the final Bin layout in place (HipBVH::refit, HipBVH.cpp:36-93;
+ * refitLeaves / refitInteriors, HipBVHKernels.cu:240-326). */
+void hxo_refit_final(int32_t* nodes, int64_t n_interior, int64_t n_leaves, const int32_t* tri_idx,
+                     const float* verts, const int32_t* tris);
+
 /* SAH cost: HipBVHKernels.cu:184-238 per-node terms in FP32, summed in FP64
  * in node order (the reference sums with a float atomicAdd). */
 double hxo_cost(const int32_t* nodes, int64_t n_nodes, float scene_area, float ct, float ci);

@@ -42,6 +42,7 @@ def lib():
             "hxo_collapse": (_i64, [_i64, _p, _p, _p, _p, _p, _p, _i32, _i32, _f32, _f32,
                                     _p, _p, _p, _p]),
             "hxo_woopify": (None, [_p, _p, _p, _i64, _p]),
+            "hxo_refit_final": (None, [_p, _i64, _i64, _p, _p, _p]),
             "hxo_cost": (ctypes.c_double, [_p, _i64, _f32, _f32, _f32]),
             "hxo_trace": (_i64, [_p, _p, _p, _p, _i64, _i32, _p, _p]),
             "hxo_trace_brute": (None, [_p, _p, _i64, _p, _i64, _p]),
@@ -148,6 +149,14 @@ def woopify(verts, tris, tri_idx):
     woop = np.zeros((ti.shape[0], 12), np.float32)
     lib().hxo_woopify(_ptr(v), _ptr(t), _ptr(ti), ti.shape[0], _ptr(woop))
     return woop
+
+
+def refit_final(nodes, n_interior, n_leaves, tri_idx, verts, tris):
+    """Refit a copy of the Bin nodes to new vertex positions (HipBVH::refit)."""
+    nd = _c(nodes, np.int32).copy()
+    lib().hxo_refit_final(_ptr(nd), int(n_interior), int(n_leaves), _ptr(_c(tri_idx, np.int32)),
+                          _ptr(_c(verts, np.float32)), _ptr(_c(tris, np.int32)))
+    return nd
 
 
 def cost(nodes, scene_area, ct=3.0, ci=2.0):
