@@ -305,6 +305,27 @@ class LBVHBuilder(BVHBuilder):
     method = "lbvh"
 
 
+class PLOCBuilder(BVHBuilder):
+    """PLOCBuilder (rt/bvh/PLOCBuilder.h, PLOCBuilder.cpp:210-273).  The
+    reference warns and keeps the old value on an out-of-range radius; here the
+    setters raise."""
+
+    method = "ploc"
+
+    def set_radius(self, r):
+        if not (1 <= r <= self.params.ploc_max_radius):
+            raise ValueError("radius must be in range [1,%d]" % self.params.ploc_max_radius)
+        self.params.ploc_radius = r
+
+    def set_max_radius(self, r):
+        if r < 1:
+            raise ValueError("max. radius must be positive")
+        self.params.ploc_max_radius = r
+
+    def set_adaptive(self, on):
+        self.params.ploc_adaptive = int(bool(on))
+
+
 class RayBuffer:
     """Rays (float4 origin/tmin, float4 direction/tmax), results int4, stats int2."""
 

@@ -133,8 +133,12 @@ struct hx_bvh {
     // Output (owned).
     DevBuf nodes, woop, tri_idx;
     // Build workspace.
-    DevBuf keys[2], vals_tmp, left, right, parent, other, counters, nmin, nmax, cost, state, cover, leaf_num,
+    DevBuf keys[2], vals_tmp, left, right, parent, first, last, counters, nmin, nmax, cost, state, cover, leaf_num,
         int_slot, counts, scene_box, cost_out, frontier;
+    // PLOC: split position per internal node (kept for refits) and the
+    // clustering workspace (cluster ids, boxes and neighbours, ping-pong).
+    DevBuf split, ploc_work;
+    int ploc_rounds = 0;
     bool built = false;
     bool counts_known = false;
     int64_t n_interior = 0, n_leaves = 0;
@@ -160,9 +164,15 @@ int validate_params(const hx_build_params* p) {
                  p->max_leaf_size);
     HX_CHECK_ARG(p->ct > 0.0f && p->ci > 0.0f, "ct and ci must be positive");
     HX_CHECK_ARG(p->layout == HX_LAYOUT_BIN, "only the Bin layout is supported (got %d)", p->layout);
-    if (p->method != HX_METHOD_LBVH) {
+    if (p->method != HX_METHOD_LBVH && p->method != HX_METHOD_PLOC) {
         hx::set_error("builder method %d is not available in this build", p->method);
         return HX_ERR_UNSUPPORTED;
+    }
+    if (p->method == HX_METHOD_PLOC) {
+        // PLOCBuilder::setMaxRadius / setRadius (PLOCBuilder.cpp:261-273).
+        HX_CHECK_ARG(p->ploc_max_radius >= 1, "ploc_max_radius must be positive (got %d)", p->ploc_max_radius);
+        HX_CHECK_ARG(p->ploc_radius >= 1 && p->ploc_radius <= p->ploc_max_radius,
+                     "ploc_radius must be in [1, %d] (got %d)", p->ploc_max_radius, p->ploc_radius);
     }
     if (p->presplitting) {
         hx::set_error("presplitting is not available in this build");
@@ -170,6 +180,9 @@ int validate_params(const hx_build_params* p) {
     }
     return HX_OK;
 }
+
+// PLOC workspace: cl[2] (int), ca[2] (float4), cb[2] (float2), nb (int).
+size_t ploc_work_bytes(int64_t n) { return (size_t)n * (2 * 4 + 2 * 16 + 2 * 8 + 4) + 1024; }
 
 int reserve_bvh(hx_bvh* b, int64_t n) {
     const int64_t N = std::max<int64_t>(2 * n - 1, 2);
@@ -183,7 +196,8 @@ int reserve_bvh(hx_bvh* b, int64_t n) {
     HX_TRY(b->left.reserve(4 * (size_t)N));
     HX_TRY(b->right.reserve(4 * (size_t)N));
     HX_TRY(b->parent.reserve(4 * (size_t)N));
-    HX_TRY(b->other.reserve(4 * (size_t)n));
+    HX_TRY(b->first.reserve(4 * (size_t)N));
+    HX_TRY(b->last.reserve(4 * (size_t)N));
     HX_TRY(b->counters.reserve(4 * (size_t)n));
     HX_TRY(b->nmin.reserve(16 * (size_t)N));
     HX_TRY(b->nmax.reserve(16 * (size_t)N));
@@ -196,6 +210,10 @@ int reserve_bvh(hx_bvh* b, int64_t n) {
     HX_TRY(b->counts.reserve(16));
     HX_TRY(b->scene_box.reserve(32));
     HX_TRY(b->cost_out.reserve(16));
+    if (b->params.method == HX_METHOD_PLOC) {
+        HX_TRY(b->split.reserve(4 * (size_t)n));
+        HX_TRY(b->ploc_work.reserve(ploc_work_bytes(n)));
+    }
     return HX_OK;
 }
 
@@ -209,7 +227,9 @@ hx::ClimbArgs climb_args(hx_bvh* b, bool refit_only) {
     ca.left = b->left.as<int32_t>();
     ca.right = b->right.as<int32_t>();
     ca.parent = b->parent.as<int32_t>();
-    ca.other = b->other.as<int32_t>();
+    ca.first = b->first.as<int32_t>();
+    ca.last = b->last.as<int32_t>();
+    ca.split = b->params.method == HX_METHOD_PLOC ? b->split.as<int32_t>() : nullptr;
     ca.counters = b->counters.as<int32_t>();
     ca.nmin = b->nmin.as<float4>();
     ca.nmax = b->nmax.as<float4>();
@@ -233,7 +253,8 @@ hx::EmitArgs emit_args(hx_bvh* b) {
     ea.left = b->left.as<int32_t>();
     ea.right = b->right.as<int32_t>();
     ea.parent = b->parent.as<int32_t>();
-    ea.other = b->other.as<int32_t>();
+    ea.first = b->first.as<int32_t>();
+    ea.last = b->last.as<int32_t>();
     ea.state = b->state.as<int32_t>();
     ea.nmin = b->nmin.as<float4>();
     ea.nmax = b->nmax.as<float4>();
@@ -245,7 +266,10 @@ hx::EmitArgs emit_args(hx_bvh* b) {
     return ea;
 }
 
-int run_lbvh(hx_bvh* b) {
+// LBVHBuilder::build front half (LBVHBuilder.cpp:187-213), shared by PLOC
+// (PLOCBuilder.cpp:161-187): scene box, Morton codes, sort; the sorted
+// triangle ids end in tri_idx.
+int run_front(hx_bvh* b) {
     Context* c = &b->ctx->c;
     const hx_build_params& p = b->params;
     const int64_t n = b->n_refs;
@@ -270,13 +294,64 @@ int run_lbvh(hx_bvh* b) {
         HX_HIP(hipMemcpyAsync(b->tri_idx.ptr, vals[ri], 4 * (size_t)n, hipMemcpyDeviceToDevice, c->stream));
     }
     b->keys_index = ri;
+    return HX_OK;
+}
+
+int run_ploc(hx_bvh* b) {
+    Context* c = &b->ctx->c;
+    const hx_build_params& p = b->params;
+    const int64_t n = b->n_refs;
+    hx::PlocArgs a{};
+    a.n = n;
+    a.verts = b->verts;
+    a.tris = b->tris;
+    a.sorted_tri = b->tri_idx.as<int32_t>();
+    a.tri_out = b->vals_tmp.as<int32_t>();
+    char* w = b->ploc_work.as<char>();
+    auto carve = [&](size_t bytes) {
+        char* q = w;
+        w += (bytes + 255) & ~(size_t)255;
+        return q;
+    };
+    for (int k = 0; k < 2; ++k) a.ca[k] = reinterpret_cast<float4*>(carve(16 * (size_t)n));
+    for (int k = 0; k < 2; ++k) a.cb[k] = reinterpret_cast<float2*>(carve(8 * (size_t)n));
+    for (int k = 0; k < 2; ++k) a.cl[k] = reinterpret_cast<int32_t*>(carve(4 * (size_t)n));
+    a.nb = reinterpret_cast<int32_t*>(carve(4 * (size_t)n));
+    a.left = b->left.as<int32_t>();
+    a.right = b->right.as<int32_t>();
+    a.parent = b->parent.as<int32_t>();
+    a.first = b->first.as<int32_t>();
+    a.last = b->last.as<int32_t>();
+    a.split = b->split.as<int32_t>();
+    a.counters = b->counters.as<int32_t>();
+    a.end_at = b->cover.as<int32_t>();
+    a.frontier_count = b->counts.as<int32_t>() + 2;
+    a.radius = p.ploc_radius;
+    a.max_radius = p.ploc_max_radius;
+    a.adaptive = p.ploc_adaptive ? 1 : 0;
+    HX_TRY(hx::launch_ploc(c, a, &b->ploc_rounds));
+    // The triangle ids in depth-first leaf order become the output array.
+    std::swap(b->tri_idx.ptr, b->vals_tmp.ptr);
+    std::swap(b->tri_idx.bytes, b->vals_tmp.bytes);
+    HX_TRY(hx::launch_climb(c, climb_args(b, false)));
+    return hx::launch_collapse_emit(c, emit_args(b));
+}
+
+int run_build(hx_bvh* b) {
+    Context* c = &b->ctx->c;
+    const hx_build_params& p = b->params;
+    const int64_t n = b->n_refs;
+    HX_TRY(run_front(b));
     const int32_t* sorted_tri = b->tri_idx.as<int32_t>();
     if (n == 1) {
         HX_TRY(hx::launch_single_ref(c, b->verts, b->tris, sorted_tri, b->nmin.as<float4>(), b->nmax.as<float4>(),
                                      b->woop.as<float4>(), b->nodes.as<int4>(), b->counts.as<int32_t>()));
+    } else if (p.method == HX_METHOD_PLOC) {
+        HX_TRY(run_ploc(b));
     } else {
-        HX_TRY(hx::launch_construct(c, keys[ri], b->keys64, n, b->left.as<int32_t>(), b->right.as<int32_t>(),
-                                    b->parent.as<int32_t>(), b->other.as<int32_t>(), b->counters.as<int32_t>(),
+        HX_TRY(hx::launch_construct(c, b->keys[b->keys_index].ptr, b->keys64, n, b->left.as<int32_t>(), b->right.as<int32_t>(),
+                                    b->parent.as<int32_t>(), b->first.as<int32_t>(), b->last.as<int32_t>(),
+                                    b->counters.as<int32_t>(),
                                     b->cover.as<int32_t>(), b->counts.as<int32_t>() + 2));
         const hx::ClimbArgs ca = climb_args(b, false);
         HX_TRY(hx::launch_climb(c, ca));
@@ -511,7 +586,7 @@ int hx_build(hx_context* ctx, const hx_build_params* params, const float* d_vert
     b->n_verts = n_vertices;
     b->keys64 = params->morton_bits > 32;
     int st = reserve_bvh(b, n_triangles);
-    if (st == HX_OK) st = run_lbvh(b);
+    if (st == HX_OK) st = run_build(b);
     if (st != HX_OK) {
         delete b;
         return st;
@@ -526,7 +601,7 @@ int hx_rebuild(hx_context* ctx, hx_bvh* bvh, const float* d_vertices, const int3
     bvh->ctx = ctx;
     bvh->verts = d_vertices;
     bvh->tris = d_triangles;
-    return run_lbvh(bvh);
+    return run_build(bvh);
 }
 
 int hx_refit(hx_context* ctx, hx_bvh* bvh, const float* d_vertices, const int32_t* d_triangles) {

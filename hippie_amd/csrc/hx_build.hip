@@ -137,13 +137,16 @@ HX_DEV int kdelta(int i, int j, int n, const void* keys) {
 template <bool K64>
 __global__ __launch_bounds__(kBlock) void k_construct(const void* __restrict__ keys, int n, int32_t* __restrict__ left,
                                                       int32_t* __restrict__ right, int32_t* __restrict__ parent,
-                                                      int32_t* __restrict__ other, int32_t* __restrict__ counters,
-                                                      int32_t* __restrict__ end_at, int32_t* __restrict__ zero_word) {
+                                                      int32_t* __restrict__ first, int32_t* __restrict__ last,
+                                                      int32_t* __restrict__ counters, int32_t* __restrict__ end_at,
+                                                      int32_t* __restrict__ zero_word) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     end_at[i] = -1;
     left[n - 1 + i] = i;
     right[n - 1 + i] = i + 1;
+    first[n - 1 + i] = i;
+    last[n - 1 + i] = i;
     if (i == 0) {
         parent[0] = -1;
         *zero_word = 0;
@@ -173,7 +176,8 @@ __global__ __launch_bounds__(kBlock) void k_construct(const void* __restrict__ k
     if (max(i, j) == gamma + 1) rc += n - 1;
     left[i] = lc;
     right[i] = rc;
-    other[i] = j;
+    first[i] = min(i, j);
+    last[i] = max(i, j);
     parent[lc] = i;
     parent[rc] = i;
 }
@@ -284,10 +288,14 @@ HX_DEV Combined combine(const ClimbArgs& a, int first, int last, float3 lmn, flo
 
 HX_DEV float3 xyz(float4 v) { return make_float3(v.x, v.y, v.z); }
 
+// GENERAL: the tree is not a Karras tree (PLOC), so an internal node's LDS
+// slot is its split position a.split[node] (the last leaf of its left child),
+// which lies in [first, last-1]; a Karras node's index already does.
+template <bool GENERAL>
 __global__ __launch_bounds__(kClimbThreads) void k_climb_local(ClimbArgs a, int32_t* __restrict__ frontier,
                                                               int32_t* __restrict__ frontier_count) {
     constexpr int T = kClimbLeaves;
-    // Slots: internal node i -> i - base (in [0, T-1)), leaf p -> T + p - base.
+    // Slots: internal node i -> key(i) - base (in [0, T-1)), leaf p -> T + p - base.
     __shared__ float s_box[6][2 * T];
     __shared__ float s_cost[2 * T];
     __shared__ int s_cnt[T];
@@ -332,23 +340,22 @@ __global__ __launch_bounds__(kClimbThreads) void k_climb_local(ClimbArgs a, int3
         int child = n - 1 + p;
         int node = a.parent[child];
         while (node >= 0) {
-            const int j = a.other[node];
-            const int first = min(node, j), last = max(node, j);
+            const int first = a.first[node], last = a.last[node];
             if (first < base || last > last_leaf) {
                 frontier[atomicAdd(frontier_count, 1)] = child;
                 break;
             }
-            if (__hip_atomic_fetch_add(&s_cnt[node - base], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) == 0)
+            const int sn = (GENERAL ? a.split[node] : node) - base;
+            if (__hip_atomic_fetch_add(&s_cnt[sn], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) == 0)
                 break;
             const int l = a.left[node], r = a.right[node];
-            const int ls = l >= n - 1 ? T + (l - (n - 1) - base) : l - base;
-            const int rs = r >= n - 1 ? T + (r - (n - 1) - base) : r - base;
+            const int ls = l >= n - 1 ? T + (l - (n - 1) - base) : (GENERAL ? a.split[l] : l) - base;
+            const int rs = r >= n - 1 ? T + (r - (n - 1) - base) : (GENERAL ? a.split[r] : r) - base;
             const Combined c = combine(
                 a, first, last, make_float3(s_box[0][ls], s_box[1][ls], s_box[2][ls]),
                 make_float3(s_box[3][ls], s_box[4][ls], s_box[5][ls]), s_cost[ls],
                 make_float3(s_box[0][rs], s_box[1][rs], s_box[2][rs]),
                 make_float3(s_box[3][rs], s_box[4][rs], s_box[5][rs]), s_cost[rs]);
-            const int sn = node - base;
             s_box[0][sn] = c.mn.x; s_box[1][sn] = c.mn.y; s_box[2][sn] = c.mn.z;
             s_box[3][sn] = c.mx.x; s_box[4][sn] = c.mx.y; s_box[5][sn] = c.mx.z;
             s_cost[sn] = c.cost;
@@ -390,8 +397,7 @@ __global__ __launch_bounds__(kBlock) void k_climb_top(ClimbArgs a, const int32_t
         int node = a.parent[frontier[e]];
         while (node >= 0) {
             if (__hip_atomic_fetch_add(&a.counters[node], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) break;
-            const int j = a.other[node];
-            const int first = min(node, j), last = max(node, j);
+            const int first = a.first[node], last = a.last[node];
             const int l = a.left[node], r = a.right[node];
             const float4 lmn = ld_agent(a.nmin + l), lmx = ld_agent(a.nmax + l);
             const float4 rmn = ld_agent(a.nmin + r), rmx = ld_agent(a.nmax + r);
@@ -498,7 +504,8 @@ __global__ __launch_bounds__(kBlock) void k_tile_cover(int32_t* __restrict__ cov
 struct FlagCtx {
     int n;
     const int32_t* cover;
-    const int32_t* other;
+    const int32_t* first;
+    const int32_t* last;
     const int32_t* state;
 };
 
@@ -510,9 +517,7 @@ HX_DEV int interior_flag(const FlagCtx& f, int i) {
     if (i >= f.n - 1) return 0;
     if (i == 0) return 1;
     if (f.state[i] != 1) return 0;
-    const int j = f.other[i];
-    const int first = min(i, j), last = max(i, j);
-    return f.cover[first] < last;
+    return f.cover[f.first[i]] < f.last[i];
 }
 
 __global__ __launch_bounds__(kBlock) void k_tile_flag_counts(FlagCtx f, int2* __restrict__ partial) {
@@ -590,8 +595,7 @@ __global__ __launch_bounds__(kBlock) void k_emit(EmitArgs a, FlagCtx f) {
     if (!interior_flag(f, i)) return;
     const int nInt = a.counts[0];
     const int s = a.int_slot[i];
-    const int j = a.other[i];
-    const int first = min(i, j), last = max(i, j);
+    const int first = a.first[i], last = a.last[i];
     const int ch[2] = {a.left[i], a.right[i]};
     int cidx[2];
     float4 cmn[2], cmx[2];
@@ -603,14 +607,7 @@ __global__ __launch_bounds__(kBlock) void k_emit(EmitArgs a, FlagCtx f) {
         if (c < n - 1 && a.state[c] == 1) {
             cidx[k] = a.int_slot[c];
         } else {
-            int cf, cl;
-            if (c >= n - 1) {
-                cf = cl = c - (n - 1);
-            } else {
-                const int jj = a.other[c];
-                cf = min(c, jj);
-                cl = max(c, jj);
-            }
+            const int cf = a.first[c], cl = a.last[c];
             const int L = nInt + a.leaf_num[cf];
             cidx[k] = ~L;
             write_node(a.nodes, L, cmn[k], cmx[k], cmn[k], cmx[k], cf, cl + 1, ~(cl - cf + 1), s);
@@ -738,22 +735,27 @@ int launch_morton(Context* c, const float* verts, const int32_t* tris, int64_t n
 }
 
 int launch_construct(Context* c, const void* d_keys, bool keys64, int64_t n, int32_t* left, int32_t* right,
-                     int32_t* parent, int32_t* other, int32_t* counters, int32_t* end_at, int32_t* zero_word) {
+                     int32_t* parent, int32_t* first, int32_t* last, int32_t* counters, int32_t* end_at,
+                     int32_t* zero_word) {
     LaunchScope ls(c, "construct");
     if (keys64)
         hipLaunchKernelGGL(k_construct<true>, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, d_keys, (int)n, left, right,
-                           parent, other, counters, end_at, zero_word);
+                           parent, first, last, counters, end_at, zero_word);
     else
         hipLaunchKernelGGL(k_construct<false>, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, d_keys, (int)n, left,
-                           right, parent, other, counters, end_at, zero_word);
+                           right, parent, first, last, counters, end_at, zero_word);
     return check_launch("construct");
 }
 
 int launch_climb(Context* c, const ClimbArgs& a) {
     LaunchScope ls(c, "climb");
     const unsigned blocks = (unsigned)((a.n + kClimbLeaves - 1) / kClimbLeaves);
-    hipLaunchKernelGGL(k_climb_local, dim3(blocks), dim3(kClimbThreads), 0, c->stream, a, a.frontier,
-                       a.frontier_count);
+    if (a.split)
+        hipLaunchKernelGGL(k_climb_local<true>, dim3(blocks), dim3(kClimbThreads), 0, c->stream, a, a.frontier,
+                           a.frontier_count);
+    else
+        hipLaunchKernelGGL(k_climb_local<false>, dim3(blocks), dim3(kClimbThreads), 0, c->stream, a, a.frontier,
+                           a.frontier_count);
     HX_TRY(check_launch("climb_local"));
     const unsigned top = (unsigned)std::max(1, std::min<int>(c->cu_count * 4, (int)((a.n + kBlock - 1) / kBlock)));
     hipLaunchKernelGGL(k_climb_top, dim3(top), dim3(kBlock), 0, c->stream, a, a.frontier, a.frontier_count);
@@ -764,7 +766,7 @@ int launch_collapse_emit(Context* c, const EmitArgs& a) {
     const int64_t n = a.n;
     const int64_t ntiles = (n + kScanTile - 1) / kScanTile;
     HX_TRY(c->scan_partials.reserve(sizeof(int2) * (size_t)ntiles + 64));
-    FlagCtx f{(int)n, a.cover, a.other, a.state};
+    FlagCtx f{(int)n, a.cover, a.first, a.last, a.state};
     {
         LaunchScope ls(c, "collapse_cover");
         int32_t* pm = c->scan_partials.as<int32_t>();
@@ -801,7 +803,7 @@ int launch_refit_reset(Context* c, int32_t* counters, int64_t n_internal, int32_
 }
 
 int launch_emit_only(Context* c, const EmitArgs& a) {
-    FlagCtx f{(int)a.n, a.cover, a.other, a.state};
+    FlagCtx f{(int)a.n, a.cover, a.first, a.last, a.state};
     LaunchScope ls(c, "collapse_emit");
     hipLaunchKernelGGL(k_emit, dim3(grid_for(a.n)), dim3(kBlock), 0, c->stream, a, f);
     return check_launch("emit");

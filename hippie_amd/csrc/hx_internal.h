@@ -106,7 +106,8 @@ int launch_scene_box(Context* c, const float* verts, const int32_t* tris, int64_
 int launch_morton(Context* c, const float* verts, const int32_t* tris, int64_t n, const float* d_box6,
                   int bits, void* d_keys, bool keys64);
 int launch_construct(Context* c, const void* d_keys, bool keys64, int64_t n, int32_t* left, int32_t* right,
-                     int32_t* parent, int32_t* other, int32_t* counters, int32_t* end_at, int32_t* zero_word);
+                     int32_t* parent, int32_t* first, int32_t* last, int32_t* counters, int32_t* end_at,
+                     int32_t* zero_word);
 struct ClimbArgs {
     int64_t n;
     const int32_t* sorted_tri;
@@ -115,7 +116,9 @@ struct ClimbArgs {
     const int32_t* left;
     const int32_t* right;
     const int32_t* parent;
-    const int32_t* other;
+    const int32_t* first;     // 2n-1: range of sorted (or DFS-ordered) leaves under each node
+    const int32_t* last;
+    const int32_t* split;     // n-1, or NULL for a Karras tree: last leaf of each node's left child
     int32_t* counters;
     float4* nmin;
     float4* nmax;
@@ -136,7 +139,8 @@ struct EmitArgs {
     const int32_t* left;
     const int32_t* right;
     const int32_t* parent;
-    const int32_t* other;
+    const int32_t* first;
+    const int32_t* last;
     const int32_t* state;
     const float4* nmin;
     const float4* nmax;
@@ -170,6 +174,32 @@ int launch_single_ref(Context* c, const float* verts, const int32_t* tris, const
 int launch_cost(Context* c, const int4* nodes, const int32_t* d_counts, int64_t max_nodes,
                 const float* d_box6, float ct, float ci, double* d_out);
 int launch_fill_i32(Context* c, int32_t* p, int32_t v, int64_t n);
+
+// hx_ploc.hip: PLOC clustering over the sorted leaves, then depth-first
+// relabelling (leaf ranks, first/last/split, tri_out in leaf order) and the
+// clears the climb expects.  n >= 2.
+struct PlocArgs {
+    int64_t n;
+    const float* verts;
+    const int32_t* tris;
+    const int32_t* sorted_tri;   // Morton order
+    int32_t* tri_out;            // depth-first leaf order (n)
+    int32_t* cl[2];              // cluster node ids, ping-pong (n each)
+    float4* ca[2];               // cluster boxes: (mn.x, mn.y, mn.z, mx.x)
+    float2* cb[2];               //                (mx.y, mx.z)
+    int32_t* nb;                 // nearest neighbour per cluster (n)
+    int32_t* left;               // 2n-1
+    int32_t* right;
+    int32_t* parent;
+    int32_t* first;
+    int32_t* last;
+    int32_t* split;              // n-1
+    int32_t* counters;           // n-1, cleared for the climb
+    int32_t* end_at;             // n, set to -1 for the climb
+    int32_t* frontier_count;     // cleared for the climb
+    int radius, max_radius, adaptive;
+};
+int launch_ploc(Context* c, const PlocArgs& a, int* rounds_out);
 
 // hx_sort.hip
 size_t sort_status_bytes(int64_t n, int end_bit, bool keys64);

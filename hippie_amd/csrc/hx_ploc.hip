@@ -1,0 +1,356 @@
+// hx_ploc.hip -- PLOC clustering on gfx950 (parallel locally-ordered
+// clustering over the Morton-sorted leaves), and the relabelling that turns
+// its tree into one whose leaves are numbered in depth-first order, so the
+// shared climb and scan-based collapse of hx_build.hip apply unchanged.
+//
+// Reference (src/hippie/rt/bvh/): PLOCBuilder.cpp:21-208 (clustering loop and
+// build), PLOCBuilderKernels.cu:11-113 (generateNeighboursCached), :116-205
+// (generateNeighbours), :207-286 (merge), :288-389 (prefix scan + compact).
+//
+// Per round with m clusters (cluster order = Morton order of the leaves):
+//   k_ploc_neighbours  nearest neighbour of every cluster in [i-r, i+r] by the
+//                      surface area of the union box; the leftmost minimum
+//                      wins (strict '>' over a left-to-right scan);
+//   k_ploc_tile_counts merges (mutual pair, lower index) and survivors
+//                      (everything but the higher index of a mutual pair) per
+//                      4096-cluster tile;
+//   k_ploc_scan        exclusive scan of the tile counts, totals for the host;
+//   k_ploc_merge       new node = (m-2) - (merge rank in cluster order): the
+//                      reference takes this rank from a warp atomicAdd, here it
+//                      is a scan so the labels are deterministic; survivors are
+//                      written compacted, with their boxes, in cluster order.
+// The cluster boxes travel with the clusters (coalesced arrays in cluster
+// order), so no round gathers boxes through node indices.
+//
+// Relabelling (k_ploc_down, one launch per round in reverse round order, so
+// every parent is placed before its children): offsets of the two children
+// from the subtree sizes the merges recorded; a leaf child is renumbered to
+// n-1+(its depth-first rank) and its triangle id is written at that rank.
+#include <algorithm>
+#include <vector>
+
+#include "hx_device.h"
+#include "hx_internal.h"
+
+namespace hx {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kWavesPerBlock = kBlock / kWave;
+constexpr int kItems = 16;
+constexpr int kTile = kBlock * kItems;
+constexpr int kMaxLdsRadius = 1024;   // window of 256 + 2r boxes, 6 floats each, in LDS
+
+inline unsigned grid_for(int64_t n, int block = kBlock) { return (unsigned)((n + block - 1) / block); }
+
+struct Sum2 {
+    HX_DEV int2 operator()(int2 a, int2 b) const { return make_int2(a.x + b.x, a.y + b.y); }
+};
+
+HX_DEV float3 ld3(const float* p) { return make_float3(p[0], p[1], p[2]); }
+
+// Leaves: box of the sorted triangle p (setupLeaves, LBVHBuilderKernels.cu
+// :162-195) as cluster p, node n-1+p.  Also clears what the climb expects
+// cleared (counters, cover, frontier count).
+__global__ __launch_bounds__(kBlock) void k_ploc_leaves(PlocArgs a) {
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    const int n = (int)a.n;
+    if (p >= n) return;
+    const int32_t* tr = a.tris + 3 * (int64_t)a.sorted_tri[p];
+    const float3 v0 = ld3(a.verts + 3 * (int64_t)tr[0]);
+    const float3 v1 = ld3(a.verts + 3 * (int64_t)tr[1]);
+    const float3 v2 = ld3(a.verts + 3 * (int64_t)tr[2]);
+    Box3 b = box_empty();
+    box_grow(b, v0);
+    box_grow(b, v1);
+    box_grow(b, v2);
+    a.cl[0][p] = n - 1 + p;
+    a.ca[0][p] = make_float4(b.mn.x, b.mn.y, b.mn.z, b.mx.x);
+    a.cb[0][p] = make_float2(b.mx.y, b.mx.z);
+    a.end_at[p] = -1;
+    if (p < n - 1) a.counters[p] = 0;
+    if (p == 0) *a.frontier_count = 0;
+}
+
+// area(union) in the reference's order: the neighbour's box grown by the
+// cluster's min corner, then by its max corner (AABB::grow, util/AABB.h:26-31).
+HX_DEV float union_area(float jx0, float jy0, float jz0, float jx1, float jy1, float jz1, const Box3& own) {
+    float mnx = mins(mins(jx0, own.mn.x), own.mx.x), mny = mins(mins(jy0, own.mn.y), own.mx.y);
+    float mnz = mins(mins(jz0, own.mn.z), own.mx.z);
+    float mxx = maxs(maxs(jx1, own.mn.x), own.mx.x), mxy = maxs(maxs(jy1, own.mn.y), own.mx.y);
+    float mxz = maxs(maxs(jz1, own.mn.z), own.mx.z);
+    return box_area(make_float3(mnx, mny, mnz), make_float3(mxx, mxy, mxz));
+}
+
+HX_DEV Box3 cluster_box(float4 a, float2 b) {
+    Box3 r;
+    r.mn = make_float3(a.x, a.y, a.z);
+    r.mx = make_float3(a.w, b.x, b.y);
+    return r;
+}
+
+// LDS window: boxes of clusters [lo, lo + W) as six rows of stride `stride`.
+template <bool LDS>
+__global__ __launch_bounds__(kBlock) void k_ploc_neighbours(const float4* __restrict__ ca,
+                                                            const float2* __restrict__ cb, int m, int r,
+                                                            int32_t* __restrict__ nb) {
+    extern __shared__ float s_win[];
+    const int b0 = blockIdx.x * kBlock;
+    const int i = b0 + threadIdx.x;
+    const int lo = max(0, b0 - r);
+    if (LDS) {
+        const int hi = min(m, b0 + kBlock + r);
+        const int stride = kBlock + 2 * r;
+        for (int k = threadIdx.x; k < hi - lo; k += kBlock) {
+            const float4 x = ca[lo + k];
+            const float2 y = cb[lo + k];
+            s_win[0 * stride + k] = x.x;
+            s_win[1 * stride + k] = x.y;
+            s_win[2 * stride + k] = x.z;
+            s_win[3 * stride + k] = x.w;
+            s_win[4 * stride + k] = y.x;
+            s_win[5 * stride + k] = y.y;
+        }
+        __syncthreads();
+        if (i >= m) return;
+        const int me = i - lo;
+        Box3 own;
+        own.mn = make_float3(s_win[me], s_win[stride + me], s_win[2 * stride + me]);
+        own.mx = make_float3(s_win[3 * stride + me], s_win[4 * stride + me], s_win[5 * stride + me]);
+        int best = -1;
+        float bestD = kMaxFloat;
+        const int jlo = max(0, i - r), jhi = min(m - 1, i + r);
+        for (int j = jlo; j <= jhi; ++j) {
+            if (j == i) continue;
+            const int q = j - lo;
+            const float d = union_area(s_win[q], s_win[stride + q], s_win[2 * stride + q], s_win[3 * stride + q],
+                                       s_win[4 * stride + q], s_win[5 * stride + q], own);
+            if (bestD > d) {
+                bestD = d;
+                best = j;
+            }
+        }
+        nb[i] = best;
+    } else {
+        if (i >= m) return;
+        const Box3 own = cluster_box(ca[i], cb[i]);
+        int best = -1;
+        float bestD = kMaxFloat;
+        const int jlo = max(0, i - r), jhi = min(m - 1, i + r);
+        for (int j = jlo; j <= jhi; ++j) {
+            if (j == i) continue;
+            const float4 x = ca[j];
+            const float2 y = cb[j];
+            const float d = union_area(x.x, x.y, x.z, x.w, y.x, y.y, own);
+            if (bestD > d) {
+                bestD = d;
+                best = j;
+            }
+        }
+        nb[i] = best;
+    }
+}
+
+// (merging, surviving) of cluster i (PLOCBuilderKernels.cu:232-240): -- a mutual pair merges
+// into the lower index; the higher index drops out of the compacted list.
+HX_DEV int2 ploc_flags(const int32_t* __restrict__ nb, int m, int64_t i) {
+    if (i >= m) return make_int2(0, 0);
+    const int j = nb[i];
+    const bool mutual = j >= 0 && nb[j] == (int)i;
+    return make_int2(mutual && i < j ? 1 : 0, (mutual && i > j) ? 0 : 1);
+}
+
+__global__ __launch_bounds__(kBlock) void k_ploc_tile_counts(const int32_t* __restrict__ nb, int m,
+                                                             int2* __restrict__ partial) {
+    const int64_t base = blockIdx.x * (int64_t)kTile;
+    int2 c = make_int2(0, 0);
+    for (int k = 0; k < kItems; ++k) {
+        const int2 f = ploc_flags(nb, m, base + k * kBlock + threadIdx.x);
+        c.x += f.x;
+        c.y += f.y;
+    }
+    c.x = wave_sum_i32(c.x);
+    c.y = wave_sum_i32(c.y);
+    __shared__ int2 ws[kWavesPerBlock];
+    if (lane_id() == 0) ws[threadIdx.x / kWave] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int2 s = ws[0];
+        for (int w = 1; w < kWavesPerBlock; ++w) s = make_int2(s.x + ws[w].x, s.y + ws[w].y);
+        partial[blockIdx.x] = s;
+    }
+}
+
+// Single block: exclusive scan of the tile counts in place; totals to *total.
+__global__ __launch_bounds__(kBlock) void k_ploc_scan(int2* __restrict__ partial, int ntiles,
+                                                      int2* __restrict__ total) {
+    __shared__ int2 wt[kWavesPerBlock];
+    int2 carry = make_int2(0, 0);
+    for (int base = 0; base < ntiles; base += kBlock) {
+        const int t = base + threadIdx.x;
+        const int2 v = t < ntiles ? partial[t] : make_int2(0, 0);
+        int2 tot;
+        const int2 inc = block_inclusive_scan<kWavesPerBlock>(v, Sum2(), wt, tot);
+        if (t < ntiles) partial[t] = make_int2(carry.x + inc.x - v.x, carry.y + inc.y - v.y);
+        carry = make_int2(carry.x + tot.x, carry.y + tot.y);
+    }
+    if (threadIdx.x == 0) *total = carry;
+}
+
+// merge (PLOCBuilderKernels.cu:207-286) + compact (:288-389) in one pass.
+// The new node's box is the select-based union of the two cluster boxes
+// (the reference uses fminf/fmaxf, which differ from selects only in the sign
+// of a zero); its subtree size goes to last[node] for the relabelling.
+__global__ __launch_bounds__(kBlock) void k_ploc_merge(PlocArgs a, int src, int m, int node_offset,
+                                                       const int2* __restrict__ partial) {
+    __shared__ int2 wt[kWavesPerBlock];
+    const int n = (int)a.n;
+    const int32_t* __restrict__ cl = a.cl[src];
+    const float4* __restrict__ ca = a.ca[src];
+    const float2* __restrict__ cb = a.cb[src];
+    int32_t* __restrict__ cl_out = a.cl[src ^ 1];
+    float4* __restrict__ ca_out = a.ca[src ^ 1];
+    float2* __restrict__ cb_out = a.cb[src ^ 1];
+    const int64_t base = blockIdx.x * (int64_t)kTile;
+    int2 carry = partial[blockIdx.x];
+    for (int k = 0; k < kItems; ++k) {
+        const int64_t i = base + k * kBlock + threadIdx.x;
+        const int2 f = ploc_flags(a.nb, m, i);
+        int2 tot;
+        const int2 inc = block_inclusive_scan<kWavesPerBlock>(f, Sum2(), wt, tot);
+        if (i < m) {
+            const int mrank = carry.x + inc.x - f.x;
+            const int srank = carry.y + inc.y - f.y;
+            if (f.x) {
+                const int j = a.nb[i];
+                const int ln = cl[i], rn = cl[j];
+                const int node = node_offset - mrank;
+                const Box3 lb = cluster_box(ca[i], cb[i]);
+                const Box3 rb = cluster_box(ca[j], cb[j]);
+                a.parent[ln] = node;
+                a.parent[rn] = node;
+                a.left[node] = ln;
+                a.right[node] = rn;
+                const int sl = ln >= n - 1 ? 1 : a.last[ln];
+                const int sr = rn >= n - 1 ? 1 : a.last[rn];
+                a.last[node] = sl + sr;
+                cl_out[srank] = node;
+                ca_out[srank] = make_float4(mins(lb.mn.x, rb.mn.x), mins(lb.mn.y, rb.mn.y), mins(lb.mn.z, rb.mn.z),
+                                            maxs(lb.mx.x, rb.mx.x));
+                cb_out[srank] = make_float2(maxs(lb.mx.y, rb.mx.y), maxs(lb.mx.z, rb.mx.z));
+            } else if (f.y) {
+                cl_out[srank] = cl[i];
+                ca_out[srank] = ca[i];
+                cb_out[srank] = cb[i];
+            }
+        }
+        carry = make_int2(carry.x + tot.x, carry.y + tot.y);
+    }
+}
+
+// Depth-first placement of the children of the nodes [lo, hi] (all created in
+// one round).  last[v] holds v's subtree size on entry and its last leaf rank
+// on exit; first[v] was written by v's parent (the root starts at 0).
+__global__ __launch_bounds__(kBlock) void k_ploc_down(PlocArgs a, int lo, int hi) {
+    const int v = lo + blockIdx.x * kBlock + threadIdx.x;
+    if (v > hi) return;
+    const int n = (int)a.n;
+    const int off = v == 0 ? 0 : a.first[v];
+    const int size = a.last[v];
+    const int l = a.left[v], r = a.right[v];
+    const int sl = l >= n - 1 ? 1 : a.last[l];
+    if (v == 0) {
+        a.first[0] = 0;
+        a.parent[0] = -1;
+    }
+    a.last[v] = off + size - 1;
+    a.split[v] = off + sl - 1;
+    const int ch[2] = {l, r};
+    const int co[2] = {off, off + sl};
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int c = ch[k], o = co[k];
+        if (c >= n - 1) {
+            const int leaf = n - 1 + o;
+            a.tri_out[o] = a.sorted_tri[c - (n - 1)];
+            a.parent[leaf] = v;
+            a.first[leaf] = o;
+            a.last[leaf] = o;
+            a.left[leaf] = o;
+            a.right[leaf] = o + 1;
+            if (k == 0)
+                a.left[v] = leaf;
+            else
+                a.right[v] = leaf;
+        } else {
+            a.first[c] = o;
+        }
+    }
+}
+
+}  // namespace
+
+int launch_ploc(Context* c, const PlocArgs& a, int* rounds_out) {
+    const int n = (int)a.n;
+    {
+        LaunchScope ls(c, "ploc_leaves");
+        hipLaunchKernelGGL(k_ploc_leaves, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, a);
+        HX_TRY(check_launch("ploc_leaves"));
+    }
+    const int64_t max_tiles = (n + kTile - 1) / kTile;
+    HX_TRY(c->scan_partials.reserve(sizeof(int2) * (size_t)(max_tiles + 1) + 64));
+    int2* partial = c->scan_partials.as<int2>();
+    int2* total = partial + max_tiles;
+    std::vector<int2> ranges;   // [lo, hi] of the nodes each round created
+    int m = n, src = 0, r = a.radius;
+    while (m > 1) {
+        if (a.adaptive) r = std::min(r + 1, a.max_radius);
+        const int ntiles = (m + kTile - 1) / kTile;
+        {
+            LaunchScope ls(c, "ploc_neighbours");
+            if (r <= kMaxLdsRadius) {
+                const size_t lds = sizeof(float) * 6 * (size_t)(kBlock + 2 * r);
+                hipLaunchKernelGGL(k_ploc_neighbours<true>, dim3(grid_for(m)), dim3(kBlock), lds, c->stream, a.ca[src],
+                                   a.cb[src], m, r, a.nb);
+            } else {
+                hipLaunchKernelGGL(k_ploc_neighbours<false>, dim3(grid_for(m)), dim3(kBlock), 0, c->stream,
+                                   a.ca[src], a.cb[src], m, r, a.nb);
+            }
+            HX_TRY(check_launch("ploc_neighbours"));
+        }
+        const int node_offset = m - 2;
+        {
+            LaunchScope ls(c, "ploc_merge");
+            hipLaunchKernelGGL(k_ploc_tile_counts, dim3(ntiles), dim3(kBlock), 0, c->stream, a.nb, m, partial);
+            HX_TRY(check_launch("ploc_tile_counts"));
+            hipLaunchKernelGGL(k_ploc_scan, dim3(1), dim3(kBlock), 0, c->stream, partial, ntiles, total);
+            HX_TRY(check_launch("ploc_scan"));
+            hipLaunchKernelGGL(k_ploc_merge, dim3(ntiles), dim3(kBlock), 0, c->stream, a, src, m, node_offset,
+                               partial);
+            HX_TRY(check_launch("ploc_merge"));
+        }
+        int2 t = make_int2(0, 0);
+        HX_HIP(hipMemcpyAsync(&t, total, sizeof(t), hipMemcpyDeviceToHost, c->stream));
+        HX_HIP(hipStreamSynchronize(c->stream));
+        if (t.x <= 0 || t.x + t.y != m) {
+            set_error("PLOC round with %d clusters merged %d pairs (non-finite boxes?)", m, t.x);
+            return HX_ERR_INVALID_ARG;
+        }
+        ranges.push_back(make_int2(node_offset - t.x + 1, node_offset));
+        m = t.y;
+        src ^= 1;
+    }
+    {
+        LaunchScope ls(c, "ploc_relabel");
+        for (size_t k = ranges.size(); k-- > 0;) {
+            const int lo = ranges[k].x, hi = ranges[k].y;
+            hipLaunchKernelGGL(k_ploc_down, dim3(grid_for(hi - lo + 1)), dim3(kBlock), 0, c->stream, a, lo, hi);
+            HX_TRY(check_launch("ploc_down"));
+        }
+    }
+    if (rounds_out) *rounds_out = (int)ranges.size();
+    return HX_OK;
+}
+
+}  // namespace hx

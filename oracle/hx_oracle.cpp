@@ -417,6 +417,75 @@ void hxo_refit(const int32_t* sv, const float* rmin, const float* rmax, int64_t 
     });
 }
 
+/* PLOC clustering: PLOCBuilder::clustering (PLOCBuilder.cpp:21-159) with
+ * generateNeighbours / merge / compact (PLOCBuilderKernels.cu:116-389).
+ * Input: leaf boxes of the sorted references in nmin/nmax[n-1+i] (as written
+ * by hxo_refit's leaf setup).  Output: left/right/parent and boxes of the
+ * internal nodes 0..n-2.  The reference numbers the nodes a round creates
+ * with warp atomics; here they are numbered in cluster order. */
+int hxo_ploc(int64_t n, int radius, int max_radius, int adaptive, int32_t* left, int32_t* right, int32_t* parent,
+             float* nmin, float* nmax) {
+    std::vector<int32_t> cl(n), next(n), nb(n);
+    for (int64_t i = 0; i < n; ++i) {
+        cl[i] = (int32_t)(n - 1 + i);
+        left[n - 1 + i] = (int32_t)i;
+        right[n - 1 + i] = (int32_t)(i + 1);
+    }
+    parent[n - 1] = -1;
+    int64_t m = n;
+    int rounds = 0;
+    int r = radius;
+    while (m > 1) {
+        ++rounds;
+        /* Adaptive radius (PLOCBuilder.cpp:57-58). */
+        if (adaptive) r = std::min(r + 1, max_radius);
+        /* generateNeighbours: leftmost minimum of area(union) in [i-r, i+r]. */
+        parallel_for(m, [&](int64_t i) {
+            const int64_t ni = cl[i];
+            V3 bmn = load4(nmin + 4 * ni), bmx = load4(nmax + 4 * ni);
+            int32_t best = -1;
+            float bestD = kMaxFloat;
+            const int64_t lo = std::max<int64_t>(0, i - r), hi = std::min<int64_t>(m, i + r + 1);
+            for (int64_t j = lo; j < hi; ++j) {
+                if (j == i) continue;
+                const int64_t nj = cl[j];
+                Box nbx;
+                nbx.mn = load4(nmin + 4 * nj);
+                nbx.mx = load4(nmax + 4 * nj);
+                nbx.grow(bmn);
+                nbx.grow(bmx);
+                const float d = nbx.area();
+                if (bestD > d) { bestD = d; best = (int32_t)j; }
+            }
+            nb[i] = best;
+        });
+        /* merge + compact, merges numbered in cluster order from m-2 down. */
+        int64_t nodeOffset = m - 2, merges = 0, out = 0;
+        for (int64_t i = 0; i < m; ++i) {
+            const int64_t j = nb[i];
+            const bool mutual = j >= 0 && nb[j] == i;
+            if (mutual && i > j) continue;          /* absorbed by its partner */
+            if (!mutual) { next[out++] = cl[i]; continue; }
+            const int64_t node = nodeOffset - merges++;
+            const int32_t ln = cl[i], rn = cl[j];
+            V3 lmn = load4(nmin + 4 * (int64_t)ln), rmn = load4(nmin + 4 * (int64_t)rn);
+            V3 lmx = load4(nmax + 4 * (int64_t)ln), rmx = load4(nmax + 4 * (int64_t)rn);
+            store4(nmin + 4 * node, V3{MINs(lmn.x, rmn.x), MINs(lmn.y, rmn.y), MINs(lmn.z, rmn.z)});
+            store4(nmax + 4 * node, V3{MAXs(lmx.x, rmx.x), MAXs(lmx.y, rmx.y), MAXs(lmx.z, rmx.z)});
+            parent[ln] = (int32_t)node;
+            parent[rn] = (int32_t)node;
+            left[node] = ln;
+            right[node] = rn;
+            next[out++] = (int32_t)node;
+        }
+        std::swap(cl, next);
+        if (merges == 0) return -1; /* no finite union area: the reference reads nb[-1] */
+        m = out;
+    }
+    parent[cl[0]] = -1;
+    return rounds;
+}
+
 int64_t hxo_collapse(int64_t n, const int32_t* sv, const int32_t* left, const int32_t* right,
                      const int32_t* parent, const float* nmin, const float* nmax, int adaptive,
                      int max_leaf_size, float ct, float ci, int32_t* nodes, int32_t* tri_out,
@@ -921,6 +990,52 @@ int64_t hxo_build_lbvh(const float* verts, const int32_t* tris, int64_t n, const
     hxo_refit(vals.data(), rmin.data(), rmax.data(), n, left.data(), right.data(), parent.data(), nmin.data(),
               nmax.data());
     lap(4);
+    int64_t nn = hxo_collapse(n, vals.data(), left.data(), right.data(), parent.data(), nmin.data(), nmax.data(),
+                              adaptive, max_leaf_size, ct, ci, nodes, tri_out, n_interior, n_leaves);
+    lap(5);
+    hxo_woopify(verts, tris, tri_out, n, woop);
+    lap(6);
+    return nn;
+}
+
+/* PLOCBuilder::build (PLOCBuilder.cpp:161-208): LBVH front half, clustering,
+ * collapse, woopify.  Returns the node count; sec[0..6] as hxo_build_lbvh. */
+int64_t hxo_build_ploc(const float* verts, const int32_t* tris, int64_t n, const float* scene_box, int bits,
+                       int radius, int max_radius, int ploc_adaptive, int adaptive, int max_leaf_size, float ct,
+                       float ci, int32_t* nodes, int32_t* tri_out, float* woop, int64_t* n_interior,
+                       int64_t* n_leaves, double sec[8]) {
+    using clk = std::chrono::steady_clock;
+    auto t0 = clk::now();
+    auto lap = [&](int i) {
+        auto t1 = clk::now();
+        sec[i] = std::chrono::duration<double>(t1 - t0).count();
+        t0 = t1;
+    };
+    for (int i = 0; i < 8; ++i) sec[i] = 0.0;
+    float sb[6];
+    if (scene_box) std::memcpy(sb, scene_box, sizeof(sb));
+    else hxo_scene_box(verts, tris, n, sb);
+    lap(0);
+    std::vector<float> rmin(4 * n), rmax(4 * n);
+    hxo_setup_boxes(verts, tris, n, rmin.data(), rmax.data());
+    std::vector<uint64_t> keys(n);
+    std::vector<int32_t> vals(n);
+    hxo_morton_codes(rmin.data(), rmax.data(), n, sb, bits, keys.data());
+    for (int64_t i = 0; i < n; ++i) vals[i] = (int32_t)i;
+    lap(1);
+    hxo_sort_pairs(keys.data(), vals.data(), n, bits);
+    lap(2);
+    const int64_t N = 2 * n - 1;
+    std::vector<int32_t> left(N), right(N), parent(N);
+    std::vector<float> nmin(4 * N), nmax(4 * N);
+    parallel_for(n, [&](int64_t i) {  /* setupLeaves */
+        std::memcpy(nmin.data() + 4 * (n - 1 + i), rmin.data() + 4 * (int64_t)vals[i], 16);
+        std::memcpy(nmax.data() + 4 * (n - 1 + i), rmax.data() + 4 * (int64_t)vals[i], 16);
+    });
+    if (hxo_ploc(n, radius, max_radius, ploc_adaptive, left.data(), right.data(), parent.data(), nmin.data(),
+                 nmax.data()) < 0)
+        return -1;
+    lap(3);
     int64_t nn = hxo_collapse(n, vals.data(), left.data(), right.data(), parent.data(), nmin.data(), nmax.data(),
                               adaptive, max_leaf_size, ct, ci, nodes, tri_out, n_interior, n_leaves);
     lap(5);

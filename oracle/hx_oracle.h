@@ -115,6 +115,21 @@ int64_t hxo_build_lbvh(const float* verts, const int32_t* tris, int64_t n,
                        int32_t* nodes, int32_t* tri_out, float* woop,
                        int64_t* n_interior, int64_t* n_leaves, double stage_seconds[8]);
 
+/* PLOC clustering (PLOCBuilder.cpp:21-159, PLOCBuilderKernels.cu:116-389) over
+ * the sorted leaves whose boxes are in nmin/nmax[n-1+i]; fills left/right/
+ * parent (2n-1) and the internal boxes.  Nodes a round creates are numbered
+ * in cluster order from (clusters - 2) down.  adaptive: the radius grows by
+ * one before every round up to max_radius.  Returns the number of rounds, or
+ * -1 if a round merged nothing (no finite union area). */
+int hxo_ploc(int64_t n, int radius, int max_radius, int adaptive, int32_t* left, int32_t* right, int32_t* parent,
+             float* nmin, float* nmax);
+
+/* PLOCBuilder::build (PLOCBuilder.cpp:161-208). */
+int64_t hxo_build_ploc(const float* verts, const int32_t* tris, int64_t n, const float* scene_box, int bits,
+                       int radius, int max_radius, int ploc_adaptive, int adaptive, int max_leaf_size, float ct,
+                       float ci, int32_t* nodes, int32_t* tri_out, float* woop, int64_t* n_interior,
+                       int64_t* n_leaves, double stage_seconds[8]);
+
 /* Ray Morton keys: rt/ray/RayBufferKernels.cu:556-614 (Aila64) and
  * :492-554 (TwoPoint64) with the ray box fixed to [0,1]^3
  * (rt/ray/RayBuffer.cpp:63-66). */

@@ -50,6 +50,9 @@ def lib():
             "hxo_validate_bin": (_i32, [_p, _i64, _i64, _p, _i64, _p, _p]),
             "hxo_build_lbvh": (_i64, [_p, _p, _i64, _p, _i32, _i32, _i32, _f32, _f32,
                                       _p, _p, _p, _p, _p, _p]),
+            "hxo_ploc": (_i32, [_i64, _i32, _i32, _i32, _p, _p, _p, _p, _p]),
+            "hxo_build_ploc": (_i64, [_p, _p, _i64, _p, _i32, _i32, _i32, _i32, _i32, _i32, _f32, _f32,
+                                      _p, _p, _p, _p, _p, _p]),
             "hxo_ray_keys_aila64": (None, [_p, _i64, _i32, _p]),
             "hxo_ray_keys_twopoint64": (None, [_p, _i64, _i32, _f32, _p]),
         }
@@ -214,6 +217,39 @@ def build_lbvh(verts, tris, bits=60, scene_box=None, adaptive=True, max_leaf_siz
                               _ptr(woop), ctypes.byref(ni), ctypes.byref(nl), _ptr(sec))
     return dict(nodes=nodes[:nn].copy(), tri_idx=tri, woop=woop, n_interior=ni.value,
                 n_leaves=nl.value, seconds=sec)
+
+
+def ploc(leaf_min, leaf_max, radius, max_radius=512, adaptive=False):
+    """PLOC over sorted leaf boxes (n x 4 each); returns left, right, parent, nmin, nmax (2n-1), rounds."""
+    lmin, lmax = _c(leaf_min, np.float32), _c(leaf_max, np.float32)
+    n = lmin.shape[0]
+    N = 2 * n - 1
+    left, right, parent = (np.zeros(N, np.int32) for _ in range(3))
+    nmin = np.zeros((N, 4), np.float32)
+    nmax = np.zeros((N, 4), np.float32)
+    nmin[n - 1:] = lmin
+    nmax[n - 1:] = lmax
+    rounds = lib().hxo_ploc(n, int(radius), int(max_radius), int(bool(adaptive)), _ptr(left), _ptr(right), _ptr(parent), _ptr(nmin), _ptr(nmax))
+    return left, right, parent, nmin, nmax, rounds
+
+
+def build_ploc(verts, tris, bits=60, radius=25, scene_box=None, adaptive=True, max_leaf_size=8, ct=3.0, ci=2.0,
+               max_radius=512, ploc_adaptive=False):
+    v, t = _c(verts, np.float32), _c(tris, np.int32)
+    n = t.shape[0]
+    N = max(2 * n - 1, 2)
+    nodes = np.zeros((N, 16), np.int32)
+    tri = np.zeros(n, np.int32)
+    woop = np.zeros((n, 12), np.float32)
+    ni, nl = ctypes.c_int64(0), ctypes.c_int64(0)
+    sec = np.zeros(8, np.float64)
+    sb = None if scene_box is None else _c(scene_box, np.float32)
+    nn = lib().hxo_build_ploc(_ptr(v), _ptr(t), n, _ptr(sb), int(bits), int(radius), int(max_radius),
+                              int(bool(ploc_adaptive)), int(bool(adaptive)),
+                              int(max_leaf_size), float(ct), float(ci), _ptr(nodes), _ptr(tri), _ptr(woop),
+                              ctypes.byref(ni), ctypes.byref(nl), _ptr(sec))
+    return dict(nodes=nodes[:nn].copy(), tri_idx=tri, woop=woop, n_interior=ni.value, n_leaves=nl.value,
+                seconds=sec)
 
 
 def ray_keys_aila64(rays, bits=60):
