@@ -326,6 +326,24 @@ class PLOCBuilder(BVHBuilder):
         self.params.ploc_adaptive = int(bool(on))
 
 
+class ATRBuilder(BVHBuilder):
+    """ATRBuilder (rt/bvh/ATRBuilder.h, ATRBuilder.cpp:238-290): LBVH plus
+    agglomerative treelet restructuring.  Out-of-range values raise instead of
+    being ignored with a warning."""
+
+    method = "atr"
+
+    def set_treelet_size(self, t):
+        if not (4 <= t <= 30):
+            raise ValueError("treelet size must be in range of [4,30]")
+        self.params.atr_treelet_size = t
+
+    def set_iterations(self, k):
+        if not (0 <= k <= 25):
+            raise ValueError("iterations must be in range of [0,25]")
+        self.params.atr_iterations = k
+
+
 class RayBuffer:
     """Rays (float4 origin/tmin, float4 direction/tmax), results int4, stats int2."""
 

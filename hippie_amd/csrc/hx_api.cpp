@@ -137,8 +137,9 @@ struct hx_bvh {
         int_slot, counts, scene_box, cost_out, frontier;
     // PLOC: split position per internal node (kept for refits) and the
     // clustering workspace (cluster ids, boxes and neighbours, ping-pong).
-    DevBuf split, ploc_work;
+    DevBuf split, ploc_work, atr_work;
     int ploc_rounds = 0;
+    int64_t atr_accepted = 0;
     bool built = false;
     bool counts_known = false;
     int64_t n_interior = 0, n_leaves = 0;
@@ -164,7 +165,7 @@ int validate_params(const hx_build_params* p) {
                  p->max_leaf_size);
     HX_CHECK_ARG(p->ct > 0.0f && p->ci > 0.0f, "ct and ci must be positive");
     HX_CHECK_ARG(p->layout == HX_LAYOUT_BIN, "only the Bin layout is supported (got %d)", p->layout);
-    if (p->method != HX_METHOD_LBVH && p->method != HX_METHOD_PLOC) {
+    if (p->method != HX_METHOD_LBVH && p->method != HX_METHOD_PLOC && p->method != HX_METHOD_ATRBVH) {
         hx::set_error("builder method %d is not available in this build", p->method);
         return HX_ERR_UNSUPPORTED;
     }
@@ -173,6 +174,13 @@ int validate_params(const hx_build_params* p) {
         HX_CHECK_ARG(p->ploc_max_radius >= 1, "ploc_max_radius must be positive (got %d)", p->ploc_max_radius);
         HX_CHECK_ARG(p->ploc_radius >= 1 && p->ploc_radius <= p->ploc_max_radius,
                      "ploc_radius must be in [1, %d] (got %d)", p->ploc_max_radius, p->ploc_radius);
+    }
+    if (p->method == HX_METHOD_ATRBVH) {
+        // ATRBuilder::setTreeletSize / setIterations (ATRBuilder.cpp:273-290).
+        HX_CHECK_ARG(p->atr_treelet_size >= 4 && p->atr_treelet_size <= 30,
+                     "atr_treelet_size must be in [4,30] (got %d)", p->atr_treelet_size);
+        HX_CHECK_ARG(p->atr_iterations >= 0 && p->atr_iterations <= 25, "atr_iterations must be in [0,25] (got %d)",
+                     p->atr_iterations);
     }
     if (p->presplitting) {
         hx::set_error("presplitting is not available in this build");
@@ -183,6 +191,12 @@ int validate_params(const hx_build_params* p) {
 
 // PLOC workspace: cl[2] (int), ca[2] (float4), cb[2] (float2), nb (int).
 size_t ploc_work_bytes(int64_t n) { return (size_t)n * (2 * 4 + 2 * 16 + 2 * 8 + 4) + 1024; }
+
+// ATR workspace: area, cost (2n-1 floats), refs, height, list (n ints),
+// hist, cursor (levels + 1 ints), accepted.
+size_t atr_work_bytes(int64_t n) {
+    return (size_t)(2 * n) * 8 + (size_t)n * 12 + 2 * sizeof(int32_t) * (hx::kAtrMaxLevels + 1) + 4096;
+}
 
 int reserve_bvh(hx_bvh* b, int64_t n) {
     const int64_t N = std::max<int64_t>(2 * n - 1, 2);
@@ -214,6 +228,10 @@ int reserve_bvh(hx_bvh* b, int64_t n) {
         HX_TRY(b->split.reserve(4 * (size_t)n));
         HX_TRY(b->ploc_work.reserve(ploc_work_bytes(n)));
     }
+    if (b->params.method == HX_METHOD_ATRBVH) {
+        HX_TRY(b->split.reserve(4 * (size_t)n));
+        HX_TRY(b->atr_work.reserve(atr_work_bytes(n)));
+    }
     return HX_OK;
 }
 
@@ -229,7 +247,8 @@ hx::ClimbArgs climb_args(hx_bvh* b, bool refit_only) {
     ca.parent = b->parent.as<int32_t>();
     ca.first = b->first.as<int32_t>();
     ca.last = b->last.as<int32_t>();
-    ca.split = b->params.method == HX_METHOD_PLOC ? b->split.as<int32_t>() : nullptr;
+    ca.split = (b->params.method == HX_METHOD_PLOC || b->params.method == HX_METHOD_ATRBVH) ? b->split.as<int32_t>()
+                                                                                             : nullptr;
     ca.counters = b->counters.as<int32_t>();
     ca.nmin = b->nmin.as<float4>();
     ca.nmax = b->nmax.as<float4>();
@@ -337,6 +356,79 @@ int run_ploc(hx_bvh* b) {
     return hx::launch_collapse_emit(c, emit_args(b));
 }
 
+hx::PlocArgs relabel_args(hx_bvh* b) {
+    hx::PlocArgs a{};
+    a.n = b->n_refs;
+    a.verts = b->verts;
+    a.tris = b->tris;
+    a.sorted_tri = b->tri_idx.as<int32_t>();
+    a.tri_out = b->vals_tmp.as<int32_t>();
+    a.left = b->left.as<int32_t>();
+    a.right = b->right.as<int32_t>();
+    a.parent = b->parent.as<int32_t>();
+    a.first = b->first.as<int32_t>();
+    a.last = b->last.as<int32_t>();
+    a.split = b->split.as<int32_t>();
+    a.counters = b->counters.as<int32_t>();
+    a.end_at = b->cover.as<int32_t>();
+    a.frontier_count = b->counts.as<int32_t>() + 2;
+    return a;
+}
+
+// ATRBuilder::build (ATRBuilder.cpp:179-236): Karras hierarchy, refit with
+// the ATR inputs, treelet optimisation, then (as for PLOC) depth-first
+// relabelling, the general climb and the collapse.
+int run_atr(hx_bvh* b) {
+    Context* c = &b->ctx->c;
+    const hx_build_params& p = b->params;
+    const int64_t n = b->n_refs;
+    HX_TRY(hx::launch_construct(c, b->keys[b->keys_index].ptr, b->keys64, n, b->left.as<int32_t>(),
+                                b->right.as<int32_t>(), b->parent.as<int32_t>(), b->first.as<int32_t>(),
+                                b->last.as<int32_t>(), b->counters.as<int32_t>(), b->cover.as<int32_t>(),
+                                b->counts.as<int32_t>() + 2));
+    char* w = b->atr_work.as<char>();
+    auto carve = [&](size_t bytes) {
+        char* q = w;
+        w += (bytes + 255) & ~(size_t)255;
+        return q;
+    };
+    hx::AtrArgs a{};
+    a.n = n;
+    a.treelet_size = p.atr_treelet_size;
+    a.iterations = p.atr_iterations;
+    a.ct = p.ct;
+    a.ci = p.ci;
+    a.left = b->left.as<int32_t>();
+    a.right = b->right.as<int32_t>();
+    a.parent = b->parent.as<int32_t>();
+    a.nmin = b->nmin.as<float4>();
+    a.nmax = b->nmax.as<float4>();
+    a.area = reinterpret_cast<float*>(carve(4 * (size_t)(2 * n)));
+    a.cost = reinterpret_cast<float*>(carve(4 * (size_t)(2 * n)));
+    a.refs = reinterpret_cast<int32_t*>(carve(4 * (size_t)n));
+    a.height = reinterpret_cast<int32_t*>(carve(4 * (size_t)n));
+    a.list = reinterpret_cast<int32_t*>(carve(4 * (size_t)n));
+    a.hist = reinterpret_cast<int32_t*>(carve(4 * (size_t)(hx::kAtrMaxLevels + 1)));
+    a.cursor = reinterpret_cast<int32_t*>(carve(4 * (size_t)(hx::kAtrMaxLevels + 1)));
+    a.accepted = reinterpret_cast<int32_t*>(carve(16));
+    hx::ClimbArgs ca = climb_args(b, false);
+    ca.split = nullptr;  // still a Karras tree
+    ca.atr_area = a.area;
+    ca.atr_cost = a.cost;
+    ca.atr_refs = a.refs;
+    ca.atr_height = a.height;
+    HX_TRY(hx::launch_climb(c, ca));
+    const hx::PlocArgs rl = relabel_args(b);
+    HX_TRY(hx::launch_atr(c, a, rl, &b->atr_accepted));
+    std::swap(b->tri_idx.ptr, b->vals_tmp.ptr);
+    std::swap(b->tri_idx.bytes, b->vals_tmp.bytes);
+    // Fresh climb state for the general tree.
+    HX_TRY(hx::launch_refit_reset(c, b->counters.as<int32_t>(), n - 1, b->counts.as<int32_t>() + 2));
+    HX_TRY(hx::launch_fill_i32(c, b->cover.as<int32_t>(), -1, n));
+    HX_TRY(hx::launch_climb(c, climb_args(b, false)));
+    return hx::launch_collapse_emit(c, emit_args(b));
+}
+
 int run_build(hx_bvh* b) {
     Context* c = &b->ctx->c;
     const hx_build_params& p = b->params;
@@ -348,6 +440,8 @@ int run_build(hx_bvh* b) {
                                      b->woop.as<float4>(), b->nodes.as<int4>(), b->counts.as<int32_t>()));
     } else if (p.method == HX_METHOD_PLOC) {
         HX_TRY(run_ploc(b));
+    } else if (p.method == HX_METHOD_ATRBVH) {
+        HX_TRY(run_atr(b));
     } else {
         HX_TRY(hx::launch_construct(c, b->keys[b->keys_index].ptr, b->keys64, n, b->left.as<int32_t>(), b->right.as<int32_t>(),
                                     b->parent.as<int32_t>(), b->first.as<int32_t>(), b->last.as<int32_t>(),

@@ -1,0 +1,399 @@
+// hx_atr.hip -- ATRBVH (agglomerative treelet restructuring) on gfx950.
+//
+// Reference (src/hippie/rt/bvh/): ATRBuilder.cpp:23-236 (schedule, optimise
+// loop, build), ATRBuilderKernels.cu:11-577 (optimize / optimizeSmall and
+// their helpers), TRBuilderUtil.h:11-289 (formTreelet and the area helpers).
+//
+// The reference climbs from every leaf with atomic counters and lets a 32-lane
+// warp restructure each node whose subtree holds at least gamma references.
+// Here the climb order is replaced by a level order: before each iteration the
+// internal nodes with >= gamma references are bucketed by subtree height, and
+// one launch per height processes that bucket.  Nodes of one height are never
+// ancestors of each other and each treelet lies inside its root's subtree, so
+// the buckets are independent, and every parent is processed after its
+// children as in the reference.  Heights are kept current: a restructured
+// treelet's interior nodes get heights (and sizes) from the merge, and every
+// processed root recomputes its own from its final children.
+//
+// Each treelet is handled by 32 lanes (half a wave64), so the reference's
+// warp-32 reductions -- whose tie-breaking depends on the shuffle tree -- are
+// reproduced exactly: findLeafToExpand over 2+i lanes and findMinimumDistance
+// (per-lane strided minimum over 32 lanes, then a 32-lane tree).  The
+// distance matrix lives in LDS; the treelet leaves live in the lanes'
+// registers and move by cross-lane permutes.
+#include <algorithm>
+#include <vector>
+
+#include "hx_device.h"
+#include "hx_internal.h"
+
+namespace hx {
+
+namespace {
+
+constexpr int kG = 32;                         // lanes per treelet
+constexpr int kAtrBlock = 256;                 // 8 treelets per workgroup
+constexpr int kGroups = kAtrBlock / kG;
+constexpr int kMaxT = 30;
+constexpr int kTrm = kMaxT * (kMaxT - 1) / 2;  // 435 pairs
+
+constexpr int kBlock = 256;
+constexpr int kItems = 16;
+constexpr int kTile = kBlock * kItems;
+
+inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
+
+// Lower-triangular matrix without its diagonal (TRBuilderUtil.h:44-49,
+// ATRBuilderKernels.cu:11-20; the row comes from a float square root, as in
+// the reference).
+HX_DEV int trm_index(int row, int col) { return ((row - 1) * row) / 2 + col; }
+HX_DEV int trm_row(int index) { return (int)((-1.0f + sqrtf(8.0f * (float)index + 1.0f)) / 2.0f) + 1; }
+HX_DEV int trm_col(int index) {
+    const int y = trm_row(index);
+    return index - (y * (y - 1)) / 2;
+}
+
+// TRBuilderUtil.h:100-130 (x*y + x*z + y*z, no validity test).
+HX_DEV float atr_area3(float3 mn, float3 mx) {
+    const float x = mx.x - mn.x, y = mx.y - mn.y, z = mx.z - mn.z;
+    return 2.0f * (x * y + x * z + y * z);
+}
+HX_DEV float atr_union_area(float3 amn, float3 amx, float3 bmn, float3 bmx) {
+    const float x = fmaxf(amx.x, bmx.x) - fminf(amn.x, bmn.x);
+    const float y = fmaxf(amx.y, bmx.y) - fminf(amn.y, bmn.y);
+    const float z = fmaxf(amx.z, bmx.z) - fminf(amn.z, bmn.z);
+    return 2.0f * (x * y + x * z + y * z);
+}
+
+template <class T>
+HX_DEV T gshfl(T v, int src) { return __shfl(v, src, kG); }
+template <class T>
+HX_DEV T gshfl_down(T v, int d) { return __shfl_down(v, d, kG); }
+HX_DEV float3 gshfl3(float3 v, int src) { return make_float3(gshfl(v.x, src), gshfl(v.y, src), gshfl(v.z, src)); }
+
+// Orders this wave's LDS accesses (no instruction: the wave runs in lockstep).
+HX_DEV void wave_lds_fence() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
+
+// One treelet per 32-lane group; task = index into the height bucket.
+__global__ __launch_bounds__(kAtrBlock) void k_atr_level(AtrArgs a, const int32_t* __restrict__ list, int count,
+                                                         int gamma) {
+    __shared__ float s_d[kGroups][kTrm];
+    const int grp = threadIdx.x / kG;
+    const int lane = threadIdx.x & (kG - 1);
+    const int task = blockIdx.x * kGroups + grp;
+    if (task >= count) return;  // whole 32-lane groups leave together
+    float* D = s_d[grp];
+    const int n = (int)a.n;
+    const int t = a.treelet_size;
+    const int X = list[task];
+
+    // The climb step at X (ATRBuilderKernels.cu:293-310): children are final.
+    const int xl = a.left[X], xr = a.right[X];
+    const float costX = a.ct * a.area[X] + a.cost[xl] + a.cost[xr];
+    const int hl = xl >= n - 1 ? 0 : a.height[xl];
+    const int hr = xr >= n - 1 ? 0 : a.height[xr];
+    const int heightX = 1 + max(hl, hr);
+    (void)gamma;
+
+    // formTreelet (TRBuilderUtil.h:171-226): expand the treelet leaf of
+    // largest area that is an internal node, t-2 times.
+    int tl = -1, inode = -1;
+    float tl_area = 0.0f;
+    if (lane == 0) {
+        tl = xl;
+        tl_area = a.area[xl];
+        inode = X;
+    }
+    if (lane == 1) {
+        tl = xr;
+        tl_area = a.area[xr];
+    }
+    for (int it = 0; it < t - 2; ++it) {
+        int idx = lane;
+        float val = (tl >= n - 1) ? -kMaxFloat : tl_area;
+        for (int k = 2 + it; k > 1; k = (k + 1) / 2) {  // findLeafToExpand
+            const int s = k / 2;
+            const int oi = gshfl_down(idx, s);
+            const float ov = gshfl_down(val, s);
+            if (ov > val) {
+                val = ov;
+                idx = oi;
+            }
+        }
+        const int e = gshfl(idx, 0);
+        const int rep = gshfl(tl, e);
+        const int cl = a.left[rep], cr = a.right[rep];
+        const float al = a.area[cl], ar = a.area[cr];
+        if (lane == it + 1) inode = rep;
+        if (lane == e) {
+            tl = cl;
+            tl_area = al;
+        }
+        if (lane == it + 2) {
+            tl = cr;
+            tl_area = ar;
+        }
+    }
+
+    // Treelet leaves: node, box, cost, size, height per lane.
+    int node = tl, rnode = -1, refs = 1, hgt = 0, mrefs = 0, mh = 0;
+    float sah = 0.0f, mcost = 0.0f;
+    float3 mn = make_float3(0.0f, 0.0f, 0.0f), mx = mn;
+    if (lane < t) {
+        const float4 bmn = a.nmin[tl], bmx = a.nmax[tl];
+        mn = make_float3(bmn.x, bmn.y, bmn.z);
+        mx = make_float3(bmx.x, bmx.y, bmx.z);
+        sah = a.cost[tl];
+        if (tl < n - 1) {
+            refs = a.refs[tl];
+            hgt = a.height[tl];
+        }
+    }
+
+    // calculateDistancesMatrix (ATRBuilderKernels.cu:22-64): every pair.
+    const int pairs = t * (t - 1) / 2;
+    for (int base = 0; base < pairs; base += kG) {
+        const int e = base + lane;
+        const int row = e < pairs ? trm_row(e) : 0;
+        const int col = e < pairs ? e - (row * (row - 1)) / 2 : 0;
+        const float3 amn = gshfl3(mn, row), amx = gshfl3(mx, row);
+        const float3 bmn = gshfl3(mn, col), bmx = gshfl3(mx, col);
+        if (e < pairs) D[e] = atr_union_area(amn, amx, bmn, bmx);
+    }
+    wave_lds_fence();
+
+    for (int lastRow = t - 1; lastRow > 0; --lastRow) {
+        // findMinimumDistance (:81-94): strided minimum, then a 32-lane tree.
+        const int msize = lastRow * (lastRow + 1) / 2;
+        int mi = 0;
+        float md = kMaxFloat;
+        for (int j = lane; j < msize; j += kG) {
+            const float d = D[j];
+            if (d < md) {
+                md = d;
+                mi = j;
+            }
+        }
+        for (int s = kG / 2; s >= 1; s >>= 1) {
+            const int oi = gshfl_down(mi, s);
+            const float od = gshfl_down(md, s);
+            if (od < md) {
+                md = od;
+                mi = oi;
+            }
+        }
+        const int minIndex = gshfl(mi, 0);
+        const int joinCol = trm_col(minIndex), joinRow = trm_row(minIndex);
+
+        // updateState (:96-158).
+        const float ssum = gshfl(sah, joinRow) + gshfl(sah, joinCol);
+        const int leftIndex = gshfl(node, joinRow), rightIndex = gshfl(node, joinCol);
+        float3 amn = gshfl3(mn, joinRow), amx = gshfl3(mx, joinRow);
+        const float3 bmn = gshfl3(mn, joinCol), bmx = gshfl3(mx, joinCol);
+        amn = make_float3(fminf(bmn.x, amn.x), fminf(bmn.y, amn.y), fminf(bmn.z, amn.z));
+        amx = make_float3(fmaxf(bmx.x, amx.x), fmaxf(bmx.y, amx.y), fmaxf(bmx.z, amx.z));
+        const int refsNew = gshfl(refs, joinRow) + gshfl(refs, joinCol);
+        const int hNew = 1 + max(gshfl(hgt, joinRow), gshfl(hgt, joinCol));
+        const int newId = gshfl(inode, lastRow - 1);
+        const float c = ssum + a.ct * atr_area3(amn, amx);
+        if (lane == joinCol) {
+            node = newId;
+            mn = amn;
+            mx = amx;
+            sah = c;
+            refs = refsNew;
+            hgt = hNew;
+        }
+        const int lastIndex = gshfl(node, lastRow);
+        const float sahLast = gshfl(sah, lastRow);
+        const float3 lmn = gshfl3(mn, lastRow), lmx = gshfl3(mx, lastRow);
+        const int refsLast = gshfl(refs, lastRow), hLast = gshfl(hgt, lastRow);
+        if (lane == joinRow) {
+            node = lastIndex;
+            sah = sahLast;
+            mn = lmn;
+            mx = lmx;
+            refs = refsLast;
+            hgt = hLast;
+        }
+        if (lane == lastRow) {
+            node = leftIndex;
+            rnode = rightIndex;
+            mn = amn;
+            mx = amx;
+            mcost = c;
+            mrefs = refsNew;
+            mh = hNew;
+        }
+
+        if (lastRow > 1) {
+            // updateDistancesMatrix (:160-196): row lastRow moves to joinRow,
+            // then row/column joinCol is recomputed from the merged box.
+            if (lane < lastRow && lane != joinRow) {
+                const float d = D[trm_index(lastRow, lane)];
+                D[trm_index(max(joinRow, lane), min(joinRow, lane))] = d;
+            }
+            wave_lds_fence();
+            const bool upd = lane < lastRow && lane != joinCol;
+            const int dr = upd ? max(joinCol, lane) : lane;
+            const int dc = upd ? min(joinCol, lane) : lane;
+            const float3 pmn = gshfl3(mn, dr), pmx = gshfl3(mx, dr);
+            const float3 qmn = gshfl3(mn, dc), qmx = gshfl3(mx, dc);
+            const float d = atr_union_area(pmn, pmx, qmn, qmx);
+            if (upd) D[trm_index(dr, dc)] = d;
+            wave_lds_fence();
+        }
+    }
+
+    // updateTreelet (:198-228): keep the new treelet if its cost is lower.
+    const float newCost = gshfl(mcost, 1);
+    const int vid = gshfl(inode, lane >= 1 ? lane - 1 : 0);
+    if (newCost < costX) {
+        if (lane >= 1 && lane < t) {
+            a.left[vid] = node;
+            a.right[vid] = rnode;
+            a.parent[node] = vid;
+            a.parent[rnode] = vid;
+            a.cost[vid] = mcost;
+            a.nmin[vid] = make_float4(mn.x, mn.y, mn.z, 0.0f);
+            a.nmax[vid] = make_float4(mx.x, mx.y, mx.z, 0.0f);
+            a.area[vid] = atr_area3(mn, mx);
+            a.refs[vid] = mrefs;
+            a.height[vid] = mh;
+        }
+        if (lane == 0) atomicAdd(a.accepted, 1);
+    } else if (lane == 0) {
+        a.cost[X] = costX;
+        a.height[X] = heightX;
+    }
+}
+
+// Height buckets of the internal nodes with at least `gamma` references:
+// per-tile LDS counts, then one global add per (tile, height).
+__global__ __launch_bounds__(kBlock) void k_atr_count(const int32_t* __restrict__ refs,
+                                                      const int32_t* __restrict__ height, int m, int gamma,
+                                                      int32_t* __restrict__ hist) {
+    __shared__ int s_cnt[kAtrMaxLevels + 1];
+    for (int i = threadIdx.x; i <= kAtrMaxLevels; i += kBlock) s_cnt[i] = 0;
+    __syncthreads();
+    const int64_t base = blockIdx.x * (int64_t)kTile;
+    for (int k = 0; k < kItems; ++k) {
+        const int64_t v = base + k * kBlock + threadIdx.x;
+        if (v < m && refs[v] >= gamma) atomicAdd(&s_cnt[min(height[v], kAtrMaxLevels)], 1);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i <= kAtrMaxLevels; i += kBlock)
+        if (s_cnt[i]) atomicAdd(&hist[i], s_cnt[i]);
+}
+
+// Scatter into the buckets (order inside a bucket is free: its nodes are
+// independent).  cursor[h] starts at the bucket's offset.
+__global__ __launch_bounds__(kBlock) void k_atr_scatter(const int32_t* __restrict__ refs,
+                                                        const int32_t* __restrict__ height, int m, int gamma,
+                                                        int32_t* __restrict__ cursor, int32_t* __restrict__ list) {
+    __shared__ int s_cnt[kAtrMaxLevels];
+    __shared__ int s_base[kAtrMaxLevels];
+    for (int i = threadIdx.x; i < kAtrMaxLevels; i += kBlock) s_cnt[i] = 0;
+    __syncthreads();
+    const int64_t base = blockIdx.x * (int64_t)kTile;
+    int lvl[kItems], loc[kItems];
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+        const int64_t v = base + k * kBlock + threadIdx.x;
+        lvl[k] = -1;
+        if (v < m && refs[v] >= gamma) {
+            lvl[k] = height[v];
+            loc[k] = atomicAdd(&s_cnt[lvl[k]], 1);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kAtrMaxLevels; i += kBlock)
+        if (s_cnt[i]) s_base[i] = atomicAdd(&cursor[i], s_cnt[i]);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kItems; ++k)
+        if (lvl[k] >= 0) list[s_base[lvl[k]] + loc[k]] = (int)(base + k * kBlock + threadIdx.x);
+}
+
+__global__ void k_atr_zero(int32_t* __restrict__ p, int count) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) p[i] = 0;
+}
+
+// Buckets the internal nodes with >= gamma references by height; returns the
+// per-height counts on the host and leaves the nodes in a.list.
+int bucket_by_height(Context* c, const AtrArgs& a, int gamma, std::vector<int>& counts) {
+    const int m = (int)a.n - 1;
+    hipLaunchKernelGGL(k_atr_zero, dim3(4), dim3(kBlock), 0, c->stream, a.hist, kAtrMaxLevels + 1);
+    HX_TRY(check_launch("atr_zero"));
+    const unsigned tiles = grid_for(m, kTile);
+    hipLaunchKernelGGL(k_atr_count, dim3(tiles), dim3(kBlock), 0, c->stream, a.refs, a.height, m, gamma, a.hist);
+    HX_TRY(check_launch("atr_count"));
+    counts.assign(kAtrMaxLevels + 1, 0);
+    HX_HIP(hipMemcpyAsync(counts.data(), a.hist, sizeof(int) * (kAtrMaxLevels + 1), hipMemcpyDeviceToHost,
+                          c->stream));
+    HX_HIP(hipStreamSynchronize(c->stream));
+    if (counts[kAtrMaxLevels] != 0) {
+        set_error("ATRBVH: a subtree is deeper than %d levels", kAtrMaxLevels - 1);
+        return HX_ERR_UNSUPPORTED;
+    }
+    std::vector<int> offs(kAtrMaxLevels, 0);
+    int run = 0;
+    for (int h = 0; h < kAtrMaxLevels; ++h) {
+        offs[h] = run;
+        run += counts[h];
+    }
+    HX_HIP(hipMemcpyAsync(a.cursor, offs.data(), sizeof(int) * kAtrMaxLevels, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_atr_scatter, dim3(tiles), dim3(kBlock), 0, c->stream, a.refs, a.height, m, gamma,
+                       a.cursor, a.list);
+    HX_TRY(check_launch("atr_scatter"));
+    // The host vector must outlive the asynchronous upload.
+    HX_HIP(hipStreamSynchronize(c->stream));
+    return HX_OK;
+}
+
+}  // namespace
+
+int launch_atr(Context* c, const AtrArgs& a, const PlocArgs& relabel, int64_t* accepted_out) {
+    const int64_t n = a.n;
+    hipLaunchKernelGGL(k_atr_zero, dim3(1), dim3(64), 0, c->stream, a.accepted, 1);
+    HX_TRY(check_launch("atr_zero"));
+    std::vector<int> counts;
+    int64_t gamma = a.treelet_size;
+    {
+        LaunchScope ls(c, "atr_optimize");
+        for (int i = 0; i < a.iterations; ++i) {
+            HX_TRY(bucket_by_height(c, a, (int)std::min<int64_t>(gamma, INT32_MAX), counts));
+            int off = 0;
+            for (int h = 0; h < kAtrMaxLevels; ++h) {
+                if (counts[h] > 0) {
+                    hipLaunchKernelGGL(k_atr_level, dim3(grid_for(counts[h], kGroups)), dim3(kAtrBlock), 0,
+                                       c->stream, a, (const int32_t*)(a.list + off), counts[h], (int)gamma);
+                    HX_TRY(check_launch("atr_level"));
+                }
+                off += counts[h];
+            }
+            gamma *= 2;  // ATRBuilder.cpp:163-171
+            if (gamma > n) break;
+        }
+    }
+    {
+        // Depth-first relabelling, parents first: buckets by height, top down.
+        LaunchScope ls(c, "atr_relabel");
+        HX_TRY(bucket_by_height(c, a, 0, counts));
+        HX_HIP(hipMemcpyAsync(relabel.last, a.refs, sizeof(int32_t) * (size_t)(n - 1), hipMemcpyDeviceToDevice,
+                              c->stream));
+        std::vector<int> offs(kAtrMaxLevels + 1, 0);
+        for (int h = 0; h < kAtrMaxLevels; ++h) offs[h + 1] = offs[h] + counts[h];
+        for (int h = kAtrMaxLevels - 1; h >= 0; --h)
+            if (counts[h] > 0) HX_TRY(launch_tree_down(c, relabel, a.list + offs[h], counts[h]));
+    }
+    if (accepted_out) {
+        int32_t acc = 0;
+        HX_HIP(hipMemcpyAsync(&acc, a.accepted, sizeof(acc), hipMemcpyDeviceToHost, c->stream));
+        HX_HIP(hipStreamSynchronize(c->stream));
+        *accepted_out = acc;
+    }
+    return HX_OK;
+}
+
+}  // namespace hx

@@ -288,10 +288,20 @@ HX_DEV Combined combine(const ClimbArgs& a, int first, int last, float3 lmn, flo
 
 HX_DEV float3 xyz(float4 v) { return make_float3(v.x, v.y, v.z); }
 
-// GENERAL: the tree is not a Karras tree (PLOC), so an internal node's LDS
-// slot is its split position a.split[node] (the last leaf of its left child),
-// which lies in [first, last-1]; a Karras node's index already does.
-template <bool GENERAL>
+// Surface area in the treelet kernels' order (TRBuilderUtil.h:112-130:
+// x*y + x*z + y*z, no validity test), used by the ATR preparation.
+HX_DEV float atr_area(float3 mn, float3 mx) {
+    const float x = mx.x - mn.x, y = mx.y - mn.y, z = mx.z - mn.z;
+    return 2.0f * (x * y + x * z + y * z);
+}
+
+// GENERAL: the tree is not a Karras tree (PLOC, ATRBVH), so an internal
+// node's LDS slot is its split position a.split[node] (the last leaf of its
+// left child), which lies in [first, last-1]; a Karras node's index already
+// does.  ATR: also produce the ATRBVH inputs -- per node the treelet-order
+// area, the binary SAH cost (leaf ci*A, interior ct*A + children;
+// ATRBuilderKernels.cu:271-309), the subtree size and the height.
+template <bool GENERAL, bool ATR>
 __global__ __launch_bounds__(kClimbThreads) void k_climb_local(ClimbArgs a, int32_t* __restrict__ frontier,
                                                               int32_t* __restrict__ frontier_count) {
     constexpr int T = kClimbLeaves;
@@ -299,6 +309,8 @@ __global__ __launch_bounds__(kClimbThreads) void k_climb_local(ClimbArgs a, int3
     __shared__ float s_box[6][2 * T];
     __shared__ float s_cost[2 * T];
     __shared__ int s_cnt[T];
+    __shared__ float s_ac[ATR ? 2 * T : 1];
+    __shared__ int s_h[ATR ? 2 * T : 1];
     const int n = (int)a.n;
     const int base = blockIdx.x * T;
     const int last_leaf = min(base + T, n) - 1;
@@ -331,6 +343,13 @@ __global__ __launch_bounds__(kClimbThreads) void k_climb_local(ClimbArgs a, int3
         s_box[0][sl] = tb.mn.x; s_box[1][sl] = tb.mn.y; s_box[2][sl] = tb.mn.z;
         s_box[3][sl] = tb.mx.x; s_box[4][sl] = tb.mx.y; s_box[5][sl] = tb.mx.z;
         s_cost[sl] = cost;
+        if (ATR) {
+            const float ar = atr_area(tb.mn, tb.mx);
+            a.atr_area[leaf] = ar;
+            a.atr_cost[leaf] = a.ci * ar;
+            s_ac[sl] = a.ci * ar;
+            s_h[sl] = 0;
+        }
     }
     __syncthreads();
 
@@ -361,6 +380,17 @@ __global__ __launch_bounds__(kClimbThreads) void k_climb_local(ClimbArgs a, int3
             s_cost[sn] = c.cost;
             a.nmin[node] = make_float4(c.mn.x, c.mn.y, c.mn.z, 0.0f);
             a.nmax[node] = make_float4(c.mx.x, c.mx.y, c.mx.z, 0.0f);
+            if (ATR) {
+                const float ar = atr_area(c.mn, c.mx);
+                const float ac = a.ct * ar + s_ac[ls] + s_ac[rs];
+                const int h = 1 + max(s_h[ls], s_h[rs]);
+                s_ac[sn] = ac;
+                s_h[sn] = h;
+                a.atr_area[node] = ar;
+                a.atr_cost[node] = ac;
+                a.atr_height[node] = h;
+                a.atr_refs[node] = last - first + 1;
+            }
             if (!a.refit_only) {
                 if (a.adaptive) a.cost[node] = c.cost;
                 a.state[node] = c.state;
@@ -390,6 +420,7 @@ HX_DEV void st_agent(float4* p, float4 v) {
 HX_DEV float ld_agent(float* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 HX_DEV void st_agent(float* p, float v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 
+template <bool ATR>
 __global__ __launch_bounds__(kBlock) void k_climb_top(ClimbArgs a, const int32_t* __restrict__ frontier,
                                                      const int32_t* __restrict__ frontier_count) {
     const int count = *frontier_count;
@@ -406,6 +437,19 @@ __global__ __launch_bounds__(kBlock) void k_climb_top(ClimbArgs a, const int32_t
             const Combined c = combine(a, first, last, xyz(lmn), xyz(lmx), lc, xyz(rmn), xyz(rmx), rc);
             st_agent(a.nmin + node, make_float4(c.mn.x, c.mn.y, c.mn.z, 0.0f));
             st_agent(a.nmax + node, make_float4(c.mx.x, c.mx.y, c.mx.z, 0.0f));
+            if (ATR) {
+                const int n = (int)a.n;
+                const float ar = atr_area(c.mn, c.mx);
+                const float ac = a.ct * ar + ld_agent(a.atr_cost + l) + ld_agent(a.atr_cost + r);
+                const int hl = l >= n - 1 ? 0 : __hip_atomic_load(a.atr_height + l, __ATOMIC_RELAXED,
+                                                                  __HIP_MEMORY_SCOPE_AGENT);
+                const int hr = r >= n - 1 ? 0 : __hip_atomic_load(a.atr_height + r, __ATOMIC_RELAXED,
+                                                                  __HIP_MEMORY_SCOPE_AGENT);
+                a.atr_area[node] = ar;
+                st_agent(a.atr_cost + node, ac);
+                __hip_atomic_store(a.atr_height + node, 1 + max(hl, hr), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                a.atr_refs[node] = last - first + 1;
+            }
             if (!a.refit_only) {
                 if (a.adaptive) st_agent(a.cost + node, c.cost);
                 a.state[node] = c.state;
@@ -750,15 +794,23 @@ int launch_construct(Context* c, const void* d_keys, bool keys64, int64_t n, int
 int launch_climb(Context* c, const ClimbArgs& a) {
     LaunchScope ls(c, "climb");
     const unsigned blocks = (unsigned)((a.n + kClimbLeaves - 1) / kClimbLeaves);
+    const bool atr = a.atr_cost != nullptr;
     if (a.split)
-        hipLaunchKernelGGL(k_climb_local<true>, dim3(blocks), dim3(kClimbThreads), 0, c->stream, a, a.frontier,
-                           a.frontier_count);
+        hipLaunchKernelGGL((k_climb_local<true, false>), dim3(blocks), dim3(kClimbThreads), 0, c->stream, a,
+                           a.frontier, a.frontier_count);
+    else if (atr)
+        hipLaunchKernelGGL((k_climb_local<false, true>), dim3(blocks), dim3(kClimbThreads), 0, c->stream, a,
+                           a.frontier, a.frontier_count);
     else
-        hipLaunchKernelGGL(k_climb_local<false>, dim3(blocks), dim3(kClimbThreads), 0, c->stream, a, a.frontier,
-                           a.frontier_count);
+        hipLaunchKernelGGL((k_climb_local<false, false>), dim3(blocks), dim3(kClimbThreads), 0, c->stream, a,
+                           a.frontier, a.frontier_count);
     HX_TRY(check_launch("climb_local"));
     const unsigned top = (unsigned)std::max(1, std::min<int>(c->cu_count * 4, (int)((a.n + kBlock - 1) / kBlock)));
-    hipLaunchKernelGGL(k_climb_top, dim3(top), dim3(kBlock), 0, c->stream, a, a.frontier, a.frontier_count);
+    if (atr)
+        hipLaunchKernelGGL(k_climb_top<true>, dim3(top), dim3(kBlock), 0, c->stream, a, a.frontier, a.frontier_count);
+    else
+        hipLaunchKernelGGL(k_climb_top<false>, dim3(top), dim3(kBlock), 0, c->stream, a, a.frontier,
+                           a.frontier_count);
     return check_launch("climb_top");
 }
 

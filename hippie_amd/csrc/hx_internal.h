@@ -128,6 +128,12 @@ struct ClimbArgs {
     float4* woop;
     int32_t* frontier;        // n entries: nodes whose parent straddles a climb block
     int32_t* frontier_count;  // zeroed by launch_construct (or launch_refit_reset)
+    // ATRBVH preparation outputs (Karras trees only; NULL otherwise): 2n-1
+    // areas and binary SAH costs, n-1 subtree sizes and heights.
+    float* atr_area;
+    float* atr_cost;
+    int32_t* atr_refs;
+    int32_t* atr_height;
     int refit_only;           // boxes and Woop only: keep states, costs and cover
     int adaptive;
     int max_leaf_size;
@@ -200,6 +206,35 @@ struct PlocArgs {
     int radius, max_radius, adaptive;
 };
 int launch_ploc(Context* c, const PlocArgs& a, int* rounds_out);
+// Depth-first placement of the children of list[0, count) (uses n, left,
+// right, parent, first, last (sizes in, last ranks out), split, sorted_tri and
+// tri_out of a); the parents of every listed node must be placed already.
+int launch_tree_down(Context* c, const PlocArgs& a, const int32_t* list, int count);
+
+// hx_atr.hip: ATRBVH treelet restructuring of a Karras tree prepared by the
+// climb (ClimbArgs::atr_*), then the depth-first relabelling.  n >= 2.
+struct AtrArgs {
+    int64_t n;
+    int treelet_size;
+    int iterations;
+    float ct, ci;
+    int32_t* left;
+    int32_t* right;
+    int32_t* parent;
+    float4* nmin;
+    float4* nmax;
+    float* area;         // 2n-1
+    float* cost;         // 2n-1
+    int32_t* refs;       // n-1
+    int32_t* height;     // n-1
+    int32_t* list;       // n-1: internal nodes grouped by height
+    int32_t* hist;       // level counts / cursors (kAtrMaxLevels + 1 each)
+    int32_t* cursor;
+    int32_t* accepted;   // 1: treelets whose restructuring was kept
+};
+constexpr int kAtrMaxLevels = 1024;
+// relabel: fields of a PLOC argument block used by launch_tree_down.
+int launch_atr(Context* c, const AtrArgs& a, const PlocArgs& relabel, int64_t* accepted_out);
 
 // hx_sort.hip
 size_t sort_status_bytes(int64_t n, int end_bit, bool keys64);

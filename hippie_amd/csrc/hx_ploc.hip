@@ -22,7 +22,7 @@
 // The cluster boxes travel with the clusters (coalesced arrays in cluster
 // order), so no round gathers boxes through node indices.
 //
-// Relabelling (k_ploc_down, one launch per round in reverse round order, so
+// Relabelling (k_tree_down, one launch per round in reverse round order, so
 // every parent is placed before its children): offsets of the two children
 // from the subtree sizes the merges recorded; a leaf child is renumbered to
 // n-1+(its depth-first rank) and its triangle id is written at that rank.
@@ -249,12 +249,15 @@ __global__ __launch_bounds__(kBlock) void k_ploc_merge(PlocArgs a, int src, int 
     }
 }
 
-// Depth-first placement of the children of the nodes [lo, hi] (all created in
-// one round).  last[v] holds v's subtree size on entry and its last leaf rank
-// on exit; first[v] was written by v's parent (the root starts at 0).
-__global__ __launch_bounds__(kBlock) void k_ploc_down(PlocArgs a, int lo, int hi) {
-    const int v = lo + blockIdx.x * kBlock + threadIdx.x;
-    if (v > hi) return;
+// Depth-first placement of the children of the nodes list[0, count) (or, if
+// list is NULL, of lo + [0, count)), all independent: every parent was placed
+// by an earlier launch.  last[v] holds v's subtree size on entry and its last
+// leaf rank on exit; first[v] was written by v's parent (the root starts at 0).
+__global__ __launch_bounds__(kBlock) void k_tree_down(PlocArgs a, const int32_t* __restrict__ list, int lo,
+                                                      int count) {
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= count) return;
+    const int v = list ? list[k] : lo + k;
     const int n = (int)a.n;
     const int off = v == 0 ? 0 : a.first[v];
     const int size = a.last[v];
@@ -271,6 +274,9 @@ __global__ __launch_bounds__(kBlock) void k_ploc_down(PlocArgs a, int lo, int hi
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
         const int c = ch[k], o = co[k];
+        // A consistent tree keeps every index in range; a broken one must not
+        // write outside the arrays.
+        if ((unsigned)c >= (unsigned)(2 * n - 1) || (unsigned)o >= (unsigned)n) continue;
         if (c >= n - 1) {
             const int leaf = n - 1 + o;
             a.tri_out[o] = a.sorted_tri[c - (n - 1)];
@@ -290,6 +296,12 @@ __global__ __launch_bounds__(kBlock) void k_ploc_down(PlocArgs a, int lo, int hi
 }
 
 }  // namespace
+
+int launch_tree_down(Context* c, const PlocArgs& a, const int32_t* list, int count) {
+    if (count <= 0) return HX_OK;
+    hipLaunchKernelGGL(k_tree_down, dim3(grid_for(count)), dim3(kBlock), 0, c->stream, a, list, 0, count);
+    return check_launch("tree_down");
+}
 
 int launch_ploc(Context* c, const PlocArgs& a, int* rounds_out) {
     const int n = (int)a.n;
@@ -345,8 +357,9 @@ int launch_ploc(Context* c, const PlocArgs& a, int* rounds_out) {
         LaunchScope ls(c, "ploc_relabel");
         for (size_t k = ranges.size(); k-- > 0;) {
             const int lo = ranges[k].x, hi = ranges[k].y;
-            hipLaunchKernelGGL(k_ploc_down, dim3(grid_for(hi - lo + 1)), dim3(kBlock), 0, c->stream, a, lo, hi);
-            HX_TRY(check_launch("ploc_down"));
+            hipLaunchKernelGGL(k_tree_down, dim3(grid_for(hi - lo + 1)), dim3(kBlock), 0, c->stream, a,
+                               (const int32_t*)nullptr, lo, hi - lo + 1);
+            HX_TRY(check_launch("tree_down"));
         }
     }
     if (rounds_out) *rounds_out = (int)ranges.size();

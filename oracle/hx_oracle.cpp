@@ -486,6 +486,227 @@ int hxo_ploc(int64_t n, int radius, int max_radius, int adaptive, int32_t* left,
     return rounds;
 }
 
+/* ------------------------------------------------------------------------ */
+/* ATRBVH (ATRBuilder.cpp:23-177, ATRBuilderKernels.cu:11-577,
+ * TRBuilderUtil.h:11-289).  The reference restructures one treelet per
+ * 32-lane warp; its two warp reductions (findLeafToExpand over 2+i lanes,
+ * findMinimum over 32 lanes of strided minima) decide ties by the shape of
+ * the shuffle tree, so both are emulated lane for lane here.  Nodes are
+ * processed in post-order, which is every order the reference's climb can
+ * take (a node is processed once both children are final, and its treelet
+ * lies inside its own subtree). */
+}  // extern "C"
+
+namespace {
+
+/* TRBuilderUtil.h:112-121 / 123-144: note the x*y + x*z + y*z order. */
+inline float atr_area(const V3& mn, const V3& mx) {
+    const float x = mx.x - mn.x, y = mx.y - mn.y, z = mx.z - mn.z;
+    return 2.0f * (x * y + x * z + y * z);
+}
+inline float atr_union_area(const V3& amn, const V3& amx, const V3& bmn, const V3& bmx) {
+    const float x = std::fmax(amx.x, bmx.x) - std::fmin(amn.x, bmn.x);
+    const float y = std::fmax(amx.y, bmx.y) - std::fmin(amn.y, bmn.y);
+    const float z = std::fmax(amx.z, bmx.z) - std::fmin(amn.z, bmn.z);
+    return 2.0f * (x * y + x * z + y * z);
+}
+inline int trm_index(int row, int col) { return ((row - 1) * row) / 2 + col; }
+/* ATRBuilderKernels.cu:11-20 (float sqrt, as the kernel computes it). */
+inline int trm_row(int index) { return (int)((-1.0f + std::sqrt(8.0f * (float)index + 1.0f)) / 2.0f) + 1; }
+inline int trm_col(int index) {
+    const int y = trm_row(index);
+    return index - (y * (y - 1)) / 2;
+}
+
+/* Shuffle-down tree over k lanes (TRBuilderUtil.h:157-169, ATRBuilderKernels.cu
+ * :67-79): every lane reads its partner's value from before the step; a lane
+ * keeps its own pair unless the partner's is strictly better. */
+template <class Better>
+void lane_tree(int k, int* idx, float* val, Better better) {
+    int oi[64];
+    float ov[64];
+    while (k > 1) {
+        const int s = k / 2;
+        const int live = (k + 1) / 2;
+        for (int i = 0; i < live; ++i) {
+            oi[i] = idx[i + s];
+            ov[i] = val[i + s];
+        }
+        for (int i = 0; i < live; ++i)
+            if (better(ov[i], val[i])) {
+                val[i] = ov[i];
+                idx[i] = oi[i];
+            }
+        k = live;
+    }
+}
+
+struct AtrTree {
+    int64_t n;
+    int32_t* left;
+    int32_t* right;
+    int32_t* parent;
+    float* nmin;
+    float* nmax;
+    std::vector<float> area, cost;
+    std::vector<int32_t> refs;
+    bool is_leaf(int32_t v) const { return v >= n - 1; }
+    V3 bmin(int32_t v) const { return load4(nmin + 4 * (int64_t)v); }
+    V3 bmax(int32_t v) const { return load4(nmax + 4 * (int64_t)v); }
+};
+
+/* One treelet at root X (formTreelet, calculateDistancesMatrix, the
+ * lastRow loop with findMinimumDistance / updateState / updateDistancesMatrix,
+ * updateTreelet).  Returns true if the restructured treelet was kept. */
+bool atr_treelet(AtrTree& T, int32_t X, int t, float ct) {
+    int internal[32], leaves[32];
+    float areas[32];
+    internal[0] = X;
+    leaves[0] = T.left[X];
+    leaves[1] = T.right[X];
+    areas[0] = T.area[leaves[0]];
+    areas[1] = T.area[leaves[1]];
+    for (int it = 0; it < t - 2; ++it) {
+        int idx[64];
+        float val[64];
+        const int k = 2 + it;
+        for (int i = 0; i < k; ++i) {
+            idx[i] = i;
+            val[i] = T.is_leaf(leaves[i]) ? -kMaxFloat : areas[i];
+        }
+        lane_tree(k, idx, val, [](float o, float m) { return o > m; });
+        const int e = idx[0];
+        const int32_t rep = leaves[e];
+        internal[it + 1] = rep;
+        leaves[e] = T.left[rep];
+        leaves[it + 2] = T.right[rep];
+        areas[e] = T.area[leaves[e]];
+        areas[it + 2] = T.area[leaves[it + 2]];
+    }
+    int node[32], rnode[32];
+    float sah[32], mcost[32];
+    V3 mn[32], mx[32];
+    for (int i = 0; i < t; ++i) {
+        node[i] = leaves[i];
+        sah[i] = T.cost[leaves[i]];
+        mn[i] = T.bmin(leaves[i]);
+        mx[i] = T.bmax(leaves[i]);
+    }
+    float D[32 * 31 / 2];
+    for (int a = 1; a < t; ++a)
+        for (int b = 0; b < a; ++b) D[trm_index(a, b)] = atr_union_area(mn[a], mx[a], mn[b], mx[b]);
+    for (int lastRow = t - 1; lastRow > 0; --lastRow) {
+        /* findMinimumDistance: 32 lanes, strided, strict '<'. */
+        int idx[64];
+        float val[64];
+        const int msize = lastRow * (lastRow + 1) / 2;
+        for (int l = 0; l < 32; ++l) {
+            idx[l] = 0;
+            val[l] = kMaxFloat;
+            for (int j = l; j < msize; j += 32)
+                if (D[j] < val[l]) {
+                    val[l] = D[j];
+                    idx[l] = j;
+                }
+        }
+        lane_tree(32, idx, val, [](float o, float m) { return o < m; });
+        const int minIndex = idx[0];
+        const int joinCol = trm_col(minIndex), joinRow = trm_row(minIndex);
+        /* updateState (ATRBuilderKernels.cu:96-158). */
+        const float s = sah[joinRow] + sah[joinCol];
+        const int leftIndex = node[joinRow], rightIndex = node[joinCol];
+        V3 amn = mn[joinRow], amx = mx[joinRow];
+        const V3 bmn = mn[joinCol], bmx = mx[joinCol];
+        amn = V3{std::fmin(bmn.x, amn.x), std::fmin(bmn.y, amn.y), std::fmin(bmn.z, amn.z)};
+        amx = V3{std::fmax(bmx.x, amx.x), std::fmax(bmx.y, amx.y), std::fmax(bmx.z, amx.z)};
+        const float c = s + ct * atr_area(amn, amx);
+        node[joinCol] = internal[lastRow - 1];
+        mn[joinCol] = amn;
+        mx[joinCol] = amx;
+        mcost[lastRow] = c;
+        sah[joinCol] = c;
+        const int lastIndex = node[lastRow];
+        const float sahLast = sah[lastRow];
+        const V3 lmn = mn[lastRow], lmx = mx[lastRow];
+        node[joinRow] = lastIndex;
+        sah[joinRow] = sahLast;
+        mn[joinRow] = lmn;
+        mx[joinRow] = lmx;
+        node[lastRow] = leftIndex;
+        rnode[lastRow] = rightIndex;
+        mn[lastRow] = amn;
+        mx[lastRow] = amx;
+        if (lastRow > 1) {
+            /* updateDistancesMatrix (:160-196). */
+            for (int l = 0; l < lastRow; ++l)
+                if (l != joinRow)
+                    D[trm_index(std::max(joinRow, l), std::min(joinRow, l))] = D[trm_index(lastRow, l)];
+            for (int l = 0; l < lastRow; ++l)
+                if (l != joinCol) {
+                    const int r = std::max(joinCol, l), cc = std::min(joinCol, l);
+                    D[trm_index(r, cc)] = atr_union_area(mn[r], mx[r], mn[cc], mx[cc]);
+                }
+        }
+    }
+    /* updateTreelet (:198-228). */
+    if (!(mcost[1] < T.cost[X])) return false;
+    for (int l = 1; l < t; ++l) {
+        const int32_t v = internal[l - 1];
+        T.left[v] = node[l];
+        T.right[v] = rnode[l];
+        T.parent[node[l]] = v;
+        T.parent[rnode[l]] = v;
+        T.cost[v] = mcost[l];
+        store4(T.nmin + 4 * (int64_t)v, mn[l]);
+        store4(T.nmax + 4 * (int64_t)v, mx[l]);
+        T.area[v] = atr_area(mn[l], mx[l]);
+    }
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t hxo_atr(int64_t n, int treelet_size, int iterations, float ct, float ci, int32_t* left,
+                           int32_t* right, int32_t* parent, float* nmin, float* nmax) {
+    if (n < 2 || treelet_size < 4 || treelet_size > 30) return 0;
+    const int64_t N = 2 * n - 1;
+    AtrTree T{n, left, right, parent, nmin, nmax, std::vector<float>(N), std::vector<float>(N),
+              std::vector<int32_t>(N)};
+    /* computeSurfaceAreas (:560-577), once per build. */
+    for (int64_t v = 0; v < N; ++v) T.area[v] = atr_area(T.bmin((int32_t)v), T.bmax((int32_t)v));
+    int64_t accepted = 0;
+    int64_t gamma = treelet_size;
+    std::vector<int32_t> stack;
+    for (int i = 0; i < iterations; ++i) {
+        for (int64_t p = 0; p < n; ++p) {
+            T.refs[n - 1 + p] = 1;
+            T.cost[n - 1 + p] = ci * T.area[n - 1 + p];
+        }
+        /* Post-order over the current tree (explicit stack: (node, visited)). */
+        stack.assign(1, 0);
+        std::vector<uint8_t> seen(n - 1, 0);
+        while (!stack.empty()) {
+            const int32_t v = stack.back();
+            if (!seen[v]) {
+                seen[v] = 1;
+                if (!T.is_leaf(T.right[v])) stack.push_back(T.right[v]);
+                if (!T.is_leaf(T.left[v])) stack.push_back(T.left[v]);
+                continue;
+            }
+            stack.pop_back();
+            const int32_t l = T.left[v], r = T.right[v];
+            T.refs[v] = T.refs[l] + T.refs[r];
+            T.cost[v] = ct * T.area[v] + T.cost[l] + T.cost[r];
+            if (T.refs[v] >= gamma && atr_treelet(T, v, treelet_size, ct)) ++accepted;
+        }
+        gamma *= 2;
+        if (gamma > n) break;
+    }
+    return accepted;
+}
+
 int64_t hxo_collapse(int64_t n, const int32_t* sv, const int32_t* left, const int32_t* right,
                      const int32_t* parent, const float* nmin, const float* nmax, int adaptive,
                      int max_leaf_size, float ct, float ci, int32_t* nodes, int32_t* tri_out,
@@ -1041,6 +1262,37 @@ int64_t hxo_build_ploc(const float* verts, const int32_t* tris, int64_t n, const
     lap(5);
     hxo_woopify(verts, tris, tri_out, n, woop);
     lap(6);
+    return nn;
+}
+
+/* ATRBuilder::build (ATRBuilder.cpp:179-236): LBVH front half and hierarchy,
+ * refit, ATR optimisation, collapse, woopify. */
+int64_t hxo_build_atr(const float* verts, const int32_t* tris, int64_t n, const float* scene_box, int bits,
+                      int treelet_size, int iterations, int adaptive, int max_leaf_size, float ct, float ci,
+                      int32_t* nodes, int32_t* tri_out, float* woop, int64_t* n_interior, int64_t* n_leaves,
+                      int64_t* accepted) {
+    float sb[6];
+    if (scene_box) std::memcpy(sb, scene_box, sizeof(sb));
+    else hxo_scene_box(verts, tris, n, sb);
+    std::vector<float> rmin(4 * n), rmax(4 * n);
+    hxo_setup_boxes(verts, tris, n, rmin.data(), rmax.data());
+    std::vector<uint64_t> keys(n);
+    std::vector<int32_t> vals(n);
+    hxo_morton_codes(rmin.data(), rmax.data(), n, sb, bits, keys.data());
+    for (int64_t i = 0; i < n; ++i) vals[i] = (int32_t)i;
+    hxo_sort_pairs(keys.data(), vals.data(), n, bits);
+    const int64_t N = std::max<int64_t>(2 * n - 1, 1);
+    std::vector<int32_t> left(N), right(N), parent(N);
+    std::vector<float> nmin(4 * N), nmax(4 * N);
+    if (n > 1) hxo_construct(keys.data(), n, left.data(), right.data(), parent.data());
+    hxo_refit(vals.data(), rmin.data(), rmax.data(), n, left.data(), right.data(), parent.data(), nmin.data(),
+              nmax.data());
+    const int64_t acc = hxo_atr(n, treelet_size, iterations, ct, ci, left.data(), right.data(), parent.data(),
+                                nmin.data(), nmax.data());
+    if (accepted) *accepted = acc;
+    int64_t nn = hxo_collapse(n, vals.data(), left.data(), right.data(), parent.data(), nmin.data(), nmax.data(),
+                              adaptive, max_leaf_size, ct, ci, nodes, tri_out, n_interior, n_leaves);
+    hxo_woopify(verts, tris, tri_out, n, woop);
     return nn;
 }
 

@@ -130,6 +130,19 @@ int64_t hxo_build_ploc(const float* verts, const int32_t* tris, int64_t n, const
                        float ci, int32_t* nodes, int32_t* tri_out, float* woop, int64_t* n_interior,
                        int64_t* n_leaves, double stage_seconds[8]);
 
+/* ATRBVH optimisation (ATRBuilder.cpp:23-177, ATRBuilderKernels.cu:11-577) of
+ * the binary tree in place: left/right/parent and the boxes of restructured
+ * nodes change.  The reference's warp-32 tie rules are emulated.  Returns the
+ * number of treelets whose restructuring was kept. */
+int64_t hxo_atr(int64_t n, int treelet_size, int iterations, float ct, float ci, int32_t* left, int32_t* right,
+                int32_t* parent, float* nmin, float* nmax);
+
+/* ATRBuilder::build (ATRBuilder.cpp:179-236). */
+int64_t hxo_build_atr(const float* verts, const int32_t* tris, int64_t n, const float* scene_box, int bits,
+                      int treelet_size, int iterations, int adaptive, int max_leaf_size, float ct, float ci,
+                      int32_t* nodes, int32_t* tri_out, float* woop, int64_t* n_interior, int64_t* n_leaves,
+                      int64_t* accepted);
+
 /* Ray Morton keys: rt/ray/RayBufferKernels.cu:556-614 (Aila64) and
  * :492-554 (TwoPoint64) with the ray box fixed to [0,1]^3
  * (rt/ray/RayBuffer.cpp:63-66). */
