@@ -253,6 +253,12 @@ class HipBVH:
         call("hx_bvh_cost", self.handle, ctypes.byref(c))
         return c.value
 
+    def info(self, key):
+        """Build statistic: "ploc_rounds", "atr_accepted" or "method"."""
+        v = ctypes.c_int64()
+        call("hx_bvh_info", self.handle, key.encode(), ctypes.byref(v))
+        return v.value
+
     def update(self, scene=None):
         """HipBVH::update -- refit + re-woopify with unchanged topology."""
         s = scene or self.scene

@@ -195,7 +195,7 @@ size_t ploc_work_bytes(int64_t n) { return (size_t)n * (2 * 4 + 2 * 16 + 2 * 8 +
 // ATR workspace: area, cost (2n-1 floats), refs, height, list (n ints),
 // hist, cursor (levels + 1 ints), accepted.
 size_t atr_work_bytes(int64_t n) {
-    return (size_t)(2 * n) * 8 + (size_t)n * 12 + 2 * sizeof(int32_t) * (hx::kAtrMaxLevels + 1) + 4096;
+    return (size_t)(2 * n) * 8 + (size_t)n * 12 + 2 * sizeof(int32_t) * (2 * hx::kAtrMaxLevels + 1) + 4096;
 }
 
 int reserve_bvh(hx_bvh* b, int64_t n) {
@@ -408,8 +408,8 @@ int run_atr(hx_bvh* b) {
     a.refs = reinterpret_cast<int32_t*>(carve(4 * (size_t)n));
     a.height = reinterpret_cast<int32_t*>(carve(4 * (size_t)n));
     a.list = reinterpret_cast<int32_t*>(carve(4 * (size_t)n));
-    a.hist = reinterpret_cast<int32_t*>(carve(4 * (size_t)(hx::kAtrMaxLevels + 1)));
-    a.cursor = reinterpret_cast<int32_t*>(carve(4 * (size_t)(hx::kAtrMaxLevels + 1)));
+    a.hist = reinterpret_cast<int32_t*>(carve(4 * (size_t)(2 * hx::kAtrMaxLevels + 1)));
+    a.cursor = reinterpret_cast<int32_t*>(carve(4 * (size_t)(2 * hx::kAtrMaxLevels + 1)));
     a.accepted = reinterpret_cast<int32_t*>(carve(16));
     hx::ClimbArgs ca = climb_args(b, false);
     ca.split = nullptr;  // still a Karras tree
@@ -777,6 +777,21 @@ int hx_bvh_debug(hx_bvh* bvh, hx_bvh_debug_view* out) {
     out->node_min = bvh->nmin.as<float>();
     out->node_max = bvh->nmax.as<float>();
     out->state = bvh->state.as<int32_t>();
+    return HX_OK;
+}
+
+int hx_bvh_info(hx_bvh* bvh, const char* key, int64_t* value) {
+    HX_CHECK_ARG(bvh && key && value, "NULL argument");
+    if (!std::strcmp(key, "ploc_rounds")) {
+        *value = bvh->ploc_rounds;
+    } else if (!std::strcmp(key, "atr_accepted")) {
+        *value = bvh->atr_accepted;
+    } else if (!std::strcmp(key, "method")) {
+        *value = bvh->params.method;
+    } else {
+        hx::set_error("unknown build statistic '%s'", key);
+        return HX_ERR_INVALID_ARG;
+    }
     return HX_OK;
 }
 

@@ -4,16 +4,18 @@
 // loop, build), ATRBuilderKernels.cu:11-577 (optimize / optimizeSmall and
 // their helpers), TRBuilderUtil.h:11-289 (formTreelet and the area helpers).
 //
-// The reference climbs from every leaf with atomic counters and lets a 32-lane
-// warp restructure each node whose subtree holds at least gamma references.
-// Here the climb order is replaced by a level order: before each iteration the
-// internal nodes with >= gamma references are bucketed by subtree height, and
-// one launch per height processes that bucket.  Nodes of one height are never
-// ancestors of each other and each treelet lies inside its root's subtree, so
-// the buckets are independent, and every parent is processed after its
-// children as in the reference.  Heights are kept current: a restructured
-// treelet's interior nodes get heights (and sizes) from the merge, and every
-// processed root recomputes its own from its final children.
+// The reference climbs from every leaf with atomic counters, recomputes every
+// node's cost on the way, and lets a 32-lane warp restructure each node whose
+// subtree holds at least gamma references.  Here the climb order is replaced by
+// a level order: before each iteration the internal nodes are bucketed by
+// (subtree height, >= gamma references), and per height one launch recomputes
+// the costs of the small nodes (a thread each) and one restructures the large
+// ones (32 lanes each).  Nodes of one height are never ancestors of each other
+// and each treelet lies inside its root's subtree, so a height's nodes are
+// independent, and every parent is processed after its children as in the
+// reference.  Heights are kept current: a restructured treelet's interior
+// nodes get heights (and sizes) from the merge, and every processed node
+// recomputes its own from its final children.
 //
 // Each treelet is handled by 32 lanes (half a wave64), so the reference's
 // warp-32 reductions -- whose tie-breaking depends on the shuffle tree -- are
@@ -75,8 +77,7 @@ HX_DEV float3 gshfl3(float3 v, int src) { return make_float3(gshfl(v.x, src), gs
 HX_DEV void wave_lds_fence() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
 
 // One treelet per 32-lane group; task = index into the height bucket.
-__global__ __launch_bounds__(kAtrBlock) void k_atr_level(AtrArgs a, const int32_t* __restrict__ list, int count,
-                                                         int gamma) {
+__global__ __launch_bounds__(kAtrBlock) void k_atr_level(AtrArgs a, const int32_t* __restrict__ list, int count) {
     __shared__ float s_d[kGroups][kTrm];
     const int grp = threadIdx.x / kG;
     const int lane = threadIdx.x & (kG - 1);
@@ -93,7 +94,6 @@ __global__ __launch_bounds__(kAtrBlock) void k_atr_level(AtrArgs a, const int32_
     const int hl = xl >= n - 1 ? 0 : a.height[xl];
     const int hr = xr >= n - 1 ? 0 : a.height[xr];
     const int heightX = 1 + max(hl, hr);
-    (void)gamma;
 
     // formTreelet (TRBuilderUtil.h:171-226): expand the treelet leaf of
     // largest area that is an internal node, t-2 times.
@@ -268,21 +268,37 @@ __global__ __launch_bounds__(kAtrBlock) void k_atr_level(AtrArgs a, const int32_
     }
 }
 
-// Height buckets of the internal nodes with at least `gamma` references:
-// per-tile LDS counts, then one global add per (tile, height).
+// The cost climb step for a node that is not restructured (ATRBuilderKernels.cu
+// :293-310): every iteration recomputes it, which matters for interior nodes a
+// kept treelet gave a cost in merge order.
+__global__ __launch_bounds__(kBlock) void k_atr_small(AtrArgs a, const int32_t* __restrict__ list, int count) {
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= count) return;
+    const int v = list[k];
+    a.cost[v] = a.ct * a.area[v] + a.cost[a.left[v]] + a.cost[a.right[v]];
+}
+
+// Bucket of an internal node: 2 * height + (references >= gamma).
+constexpr int kBuckets = 2 * kAtrMaxLevels;
+
+HX_DEV int bucket_of(const int32_t* refs, const int32_t* height, int v, int gamma) {
+    return min(2 * height[v] + (refs[v] >= gamma ? 1 : 0), kBuckets);
+}
+
+// Bucket counts: per-tile LDS counts, then one global add per (tile, bucket).
 __global__ __launch_bounds__(kBlock) void k_atr_count(const int32_t* __restrict__ refs,
                                                       const int32_t* __restrict__ height, int m, int gamma,
                                                       int32_t* __restrict__ hist) {
-    __shared__ int s_cnt[kAtrMaxLevels + 1];
-    for (int i = threadIdx.x; i <= kAtrMaxLevels; i += kBlock) s_cnt[i] = 0;
+    __shared__ int s_cnt[kBuckets + 1];
+    for (int i = threadIdx.x; i <= kBuckets; i += kBlock) s_cnt[i] = 0;
     __syncthreads();
     const int64_t base = blockIdx.x * (int64_t)kTile;
     for (int k = 0; k < kItems; ++k) {
         const int64_t v = base + k * kBlock + threadIdx.x;
-        if (v < m && refs[v] >= gamma) atomicAdd(&s_cnt[min(height[v], kAtrMaxLevels)], 1);
+        if (v < m) atomicAdd(&s_cnt[bucket_of(refs, height, (int)v, gamma)], 1);
     }
     __syncthreads();
-    for (int i = threadIdx.x; i <= kAtrMaxLevels; i += kBlock)
+    for (int i = threadIdx.x; i <= kBuckets; i += kBlock)
         if (s_cnt[i]) atomicAdd(&hist[i], s_cnt[i]);
 }
 
@@ -291,9 +307,9 @@ __global__ __launch_bounds__(kBlock) void k_atr_count(const int32_t* __restrict_
 __global__ __launch_bounds__(kBlock) void k_atr_scatter(const int32_t* __restrict__ refs,
                                                         const int32_t* __restrict__ height, int m, int gamma,
                                                         int32_t* __restrict__ cursor, int32_t* __restrict__ list) {
-    __shared__ int s_cnt[kAtrMaxLevels];
-    __shared__ int s_base[kAtrMaxLevels];
-    for (int i = threadIdx.x; i < kAtrMaxLevels; i += kBlock) s_cnt[i] = 0;
+    __shared__ int s_cnt[kBuckets];
+    __shared__ int s_base[kBuckets];
+    for (int i = threadIdx.x; i < kBuckets; i += kBlock) s_cnt[i] = 0;
     __syncthreads();
     const int64_t base = blockIdx.x * (int64_t)kTile;
     int lvl[kItems], loc[kItems];
@@ -301,13 +317,13 @@ __global__ __launch_bounds__(kBlock) void k_atr_scatter(const int32_t* __restric
     for (int k = 0; k < kItems; ++k) {
         const int64_t v = base + k * kBlock + threadIdx.x;
         lvl[k] = -1;
-        if (v < m && refs[v] >= gamma) {
-            lvl[k] = height[v];
+        if (v < m) {
+            lvl[k] = bucket_of(refs, height, (int)v, gamma);
             loc[k] = atomicAdd(&s_cnt[lvl[k]], 1);
         }
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < kAtrMaxLevels; i += kBlock)
+    for (int i = threadIdx.x; i < kBuckets; i += kBlock)
         if (s_cnt[i]) s_base[i] = atomicAdd(&cursor[i], s_cnt[i]);
     __syncthreads();
 #pragma unroll
@@ -319,30 +335,29 @@ __global__ void k_atr_zero(int32_t* __restrict__ p, int count) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) p[i] = 0;
 }
 
-// Buckets the internal nodes with >= gamma references by height; returns the
-// per-height counts on the host and leaves the nodes in a.list.
+// Buckets the internal nodes by (height, >= gamma references); returns the
+// per-bucket counts on the host and leaves the nodes in a.list.
 int bucket_by_height(Context* c, const AtrArgs& a, int gamma, std::vector<int>& counts) {
     const int m = (int)a.n - 1;
-    hipLaunchKernelGGL(k_atr_zero, dim3(4), dim3(kBlock), 0, c->stream, a.hist, kAtrMaxLevels + 1);
+    hipLaunchKernelGGL(k_atr_zero, dim3(4), dim3(kBlock), 0, c->stream, a.hist, kBuckets + 1);
     HX_TRY(check_launch("atr_zero"));
     const unsigned tiles = grid_for(m, kTile);
     hipLaunchKernelGGL(k_atr_count, dim3(tiles), dim3(kBlock), 0, c->stream, a.refs, a.height, m, gamma, a.hist);
     HX_TRY(check_launch("atr_count"));
-    counts.assign(kAtrMaxLevels + 1, 0);
-    HX_HIP(hipMemcpyAsync(counts.data(), a.hist, sizeof(int) * (kAtrMaxLevels + 1), hipMemcpyDeviceToHost,
-                          c->stream));
+    counts.assign(kBuckets + 1, 0);
+    HX_HIP(hipMemcpyAsync(counts.data(), a.hist, sizeof(int) * (kBuckets + 1), hipMemcpyDeviceToHost, c->stream));
     HX_HIP(hipStreamSynchronize(c->stream));
-    if (counts[kAtrMaxLevels] != 0) {
+    if (counts[kBuckets] != 0) {
         set_error("ATRBVH: a subtree is deeper than %d levels", kAtrMaxLevels - 1);
         return HX_ERR_UNSUPPORTED;
     }
-    std::vector<int> offs(kAtrMaxLevels, 0);
+    std::vector<int> offs(kBuckets, 0);
     int run = 0;
-    for (int h = 0; h < kAtrMaxLevels; ++h) {
-        offs[h] = run;
-        run += counts[h];
+    for (int b = 0; b < kBuckets; ++b) {
+        offs[b] = run;
+        run += counts[b];
     }
-    HX_HIP(hipMemcpyAsync(a.cursor, offs.data(), sizeof(int) * kAtrMaxLevels, hipMemcpyHostToDevice, c->stream));
+    HX_HIP(hipMemcpyAsync(a.cursor, offs.data(), sizeof(int) * kBuckets, hipMemcpyHostToDevice, c->stream));
     hipLaunchKernelGGL(k_atr_scatter, dim3(tiles), dim3(kBlock), 0, c->stream, a.refs, a.height, m, gamma,
                        a.cursor, a.list);
     HX_TRY(check_launch("atr_scatter"));
@@ -364,13 +379,19 @@ int launch_atr(Context* c, const AtrArgs& a, const PlocArgs& relabel, int64_t* a
         for (int i = 0; i < a.iterations; ++i) {
             HX_TRY(bucket_by_height(c, a, (int)std::min<int64_t>(gamma, INT32_MAX), counts));
             int off = 0;
-            for (int h = 0; h < kAtrMaxLevels; ++h) {
-                if (counts[h] > 0) {
-                    hipLaunchKernelGGL(k_atr_level, dim3(grid_for(counts[h], kGroups)), dim3(kAtrBlock), 0,
-                                       c->stream, a, (const int32_t*)(a.list + off), counts[h], (int)gamma);
+            for (int b = 0; b < kBuckets; ++b) {
+                const int cnt = counts[b];
+                if (cnt > 0 && (b & 1)) {
+                    hipLaunchKernelGGL(k_atr_level, dim3(grid_for(cnt, kGroups)), dim3(kAtrBlock), 0, c->stream,
+                                       a, (const int32_t*)(a.list + off), cnt);
                     HX_TRY(check_launch("atr_level"));
+                } else if (cnt > 0 && i > 0) {
+                    // The first iteration's small-node costs are the prepared ones.
+                    hipLaunchKernelGGL(k_atr_small, dim3(grid_for(cnt, kBlock)), dim3(kBlock), 0, c->stream, a,
+                                       (const int32_t*)(a.list + off), cnt);
+                    HX_TRY(check_launch("atr_small"));
                 }
-                off += counts[h];
+                off += cnt;
             }
             gamma *= 2;  // ATRBuilder.cpp:163-171
             if (gamma > n) break;
@@ -382,10 +403,13 @@ int launch_atr(Context* c, const AtrArgs& a, const PlocArgs& relabel, int64_t* a
         HX_TRY(bucket_by_height(c, a, 0, counts));
         HX_HIP(hipMemcpyAsync(relabel.last, a.refs, sizeof(int32_t) * (size_t)(n - 1), hipMemcpyDeviceToDevice,
                               c->stream));
-        std::vector<int> offs(kAtrMaxLevels + 1, 0);
-        for (int h = 0; h < kAtrMaxLevels; ++h) offs[h + 1] = offs[h] + counts[h];
-        for (int h = kAtrMaxLevels - 1; h >= 0; --h)
-            if (counts[h] > 0) HX_TRY(launch_tree_down(c, relabel, a.list + offs[h], counts[h]));
+        std::vector<int> offs(kBuckets + 1, 0);
+        for (int b = 0; b < kBuckets; ++b) offs[b + 1] = offs[b] + counts[b];
+        // Both buckets of a height are contiguous: one launch per height.
+        for (int h = kAtrMaxLevels - 1; h >= 0; --h) {
+            const int cnt = counts[2 * h] + counts[2 * h + 1];
+            if (cnt > 0) HX_TRY(launch_tree_down(c, relabel, a.list + offs[2 * h], cnt));
+        }
     }
     if (accepted_out) {
         int32_t acc = 0;

@@ -228,7 +228,7 @@ struct AtrArgs {
     int32_t* refs;       // n-1
     int32_t* height;     // n-1
     int32_t* list;       // n-1: internal nodes grouped by height
-    int32_t* hist;       // level counts / cursors (kAtrMaxLevels + 1 each)
+    int32_t* hist;       // bucket counts / cursors (2 * kAtrMaxLevels + 1 each)
     int32_t* cursor;
     int32_t* accepted;   // 1: treelets whose restructuring was kept
 };

@@ -162,6 +162,10 @@ int hx_bvh_destroy(hx_bvh* bvh);
 /* Synchronises on first call after a build to read the node counts. */
 int hx_bvh_get_view(hx_bvh* bvh, hx_bvh_view* out);
 int hx_bvh_debug(hx_bvh* bvh, hx_bvh_debug_view* out);
+/* Statistics of the last build (no reference counterpart; the reference
+ * logs them, PLOCBuilder.cpp:148, ATRBuilder.cpp:166): "ploc_rounds",
+ * "atr_accepted" (treelets kept), "method". */
+int hx_bvh_info(hx_bvh* bvh, const char* key, int64_t* value);
 /* HipBVH::getCost (HipBVH.cpp:346-370): SAH cost, FP64 deterministic sum. */
 int hx_bvh_cost(hx_bvh* bvh, double* out);
 

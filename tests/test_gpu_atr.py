@@ -23,6 +23,8 @@ def _check(oracle, ctx, v, t, ts=9, iters=2, adaptive=True, max_leaf=8, bits=60)
                       adaptive_leaf_size=int(adaptive), max_leaf_size=max_leaf)
     bvh = b.build(scene)
     nodes, tri = bvh.nodes_host(), bvh.tri_indices_host()
+    if t.shape[0] > 1:
+        assert bvh.info("atr_accepted") == ref["accepted"], "treelets kept"
     assert (bvh.n_interior, bvh.n_leaves) == (ref["n_interior"], ref["n_leaves"]), "node counts"
     assert oracle.validate_bin(nodes, bvh.n_interior, bvh.n_leaves, tri, v, t) == 0, "validate"
     cg = oracle.bin_canonical(nodes, bvh.n_interior, bvh.n_leaves, tri)
