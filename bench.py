@@ -8,6 +8,16 @@ sorted by the 60-bit Aila ray key before tracing.  Geometry and rays are
 seeded synthetic data resident in HBM before timing starts.
 
 value = build throughput in Mtris/s; trace throughput is reported beside it.
+
+Also measured in the same run (reported beside `value`):
+  builders  PLOC (radius 25) and ATRBVH (treelet 9, 2 iterations) rebuilds of
+            the same scene, with their SAH cost against the LBVH's;
+  batched   4096 meshes of round(exp(U[ln 64, ln 262144])) triangles (half
+            displaced icospheres, half soups, each in its own unit box): one
+            batched build against a loop of per-mesh rebuilds, and one
+            differentiable step (refit -> trace of 256 rays per 1k triangles ->
+            loss -> adjoint scatter) batched and graph-captured against the
+            per-mesh loop.
 """
 import argparse
 import json
@@ -35,8 +45,14 @@ def parse():
     ap.add_argument("--grid", type=int, default=2048, help="ground grid cells per side")
     ap.add_argument("--res", type=int, default=4096, help="primary ray resolution (res x res)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=2097152, help="triangles in the CPU baseline sample")
-    ap.add_argument("--cpu-rays", type=int, default=262144, help="rays in the CPU trace sample")
+    ap.add_argument("--cpu-sample", type=int, default=0, help="triangles in the CPU baseline sample (0 = all)")
+    ap.add_argument("--cpu-rays", type=int, default=4194304, help="rays in the CPU trace sample")
+    ap.add_argument("--builders", default="ploc,atr", help="other builders to time on the same scene ('' = none)")
+    ap.add_argument("--batch-meshes", type=int, default=4096, help="meshes in the batched workload (0 = skip)")
+    ap.add_argument("--batch-lo", type=int, default=64)
+    ap.add_argument("--batch-hi", type=int, default=262144)
+    ap.add_argument("--batch-rays-per-1k", type=int, default=256)
+    ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--tune", action="append", default=[], metavar="NAME=VALUE",
                     help="libhx launch tunable (hx_context_set_tuning), e.g. trace_lds_stack=16")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -50,7 +66,7 @@ def cpu_baseline(verts, tris, rays, args):
     from oracle import hx_oracle as O
     rng = np.random.default_rng(11)
     n = tris.shape[0]
-    m = min(args.cpu_sample, n)
+    m = min(args.cpu_sample, n) if args.cpu_sample > 0 else n
     sel = np.sort(rng.choice(n, m, replace=False)) if m < n else np.arange(n)
     t_s = tris[sel]
     O.lib()
@@ -61,15 +77,232 @@ def cpu_baseline(verts, tris, rays, args):
         b = O.build_lbvh(verts, t_s, bits=60)
         times.append(time.perf_counter() - t0)
     build_s = min(times)
+    what = "all" if m == n else f"{m} of the"
     r = rays[: min(args.cpu_rays, rays.shape[0])]
     t0 = time.perf_counter()
     O.trace(b["nodes"], b["woop"], b["tri_idx"], r)
     trace_s = time.perf_counter() - t0
     return dict(value=round(m / build_s / 1e6, 3), unit="Mtris/s", cores=threads, kind="port",
-                sample=f"LBVH-60 build of {m} of the {n} triangles (random subset, best of 2): "
+                sample=f"LBVH-60 build of {what} {n} triangles (best of 2): "
                        f"{build_s:.2f} s; closest-hit trace of {r.shape[0]} primary rays on that BVH: "
                        f"{r.shape[0] / trace_s / 1e6:.3f} Mrays/s",
                 trace_mrays_s=round(r.shape[0] / trace_s / 1e6, 3))
+
+
+def timed(fn, steps, warmup):
+    """Average ms of fn() over `steps` calls after `warmup`, by HIP events on torch's stream."""
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(steps):
+        fn()
+    b.record()
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / steps
+
+
+def bench_builders(ctx, scene, n, args):
+    """PLOC r=25 and ATRBVH t=9 x2 rebuilds of the benchmark scene, with SAH costs."""
+    import hippie_amd as hx
+    out = {}
+    kinds = [k for k in args.builders.split(",") if k]
+    specs = {"ploc": ("ploc_r25", hx.PLOCBuilder, dict(ploc_radius=25)),
+             "atr": ("atr_t9_i2", hx.ATRBuilder, dict(atr_treelet_size=9, atr_iterations=2))}
+    for k in kinds:
+        name, cls, kw = specs[k]
+        b = cls(ctx, morton_bits=60, **kw)
+        bvh = b.build(scene)
+        ms = timed(lambda: b.rebuild(bvh, scene), max(1, args.steps // 2), 1)
+        out[name] = {"ms": round(ms, 3), "mtris_s": round(n / (ms * 1e-3) / 1e6, 2), "sah": round(bvh.get_cost(), 4)}
+        if k == "atr":
+            out[name]["treelets_kept"] = bvh.info("atr_accepted")
+        if k == "ploc":
+            out[name]["rounds"] = bvh.info("ploc_rounds")
+        log(f"{name}: {ms:.3f} ms ({out[name]['mtris_s']} Mtris/s), SAH {out[name]['sah']}")
+        del bvh
+    return out
+
+
+def batch_workload(args, device):
+    """Config 3 geometry and rays generated on the device: meshes of
+    round(exp(U[ln lo, ln hi])) triangles, half displaced icospheres (largest
+    level that fits, topped up with soup) and half soups, each in the unit box;
+    256 rays per 1k triangles from a sphere around the box aimed inside it."""
+    import math
+    import numpy as np
+    import torch
+    from hippie_amd import scenes
+    rng = np.random.default_rng(3)
+    M = args.batch_meshes
+    counts = np.rint(np.exp(rng.uniform(math.log(args.batch_lo), math.log(args.batch_hi), M))).astype(np.int64)
+    sphere = rng.random(M) < 0.5
+    g = torch.Generator(device=device)
+    g.manual_seed(3)
+    ico = {}
+    verts, tris_local, offs, vbase = [], [], [0], [0]
+
+    def soup(c):
+        cen = torch.rand((c, 1, 3), generator=g, device=device)
+        sc = 0.001 + 0.019 * torch.rand((c, 1, 1), generator=g, device=device)
+        v = cen + sc * (torch.rand((c, 3, 3), generator=g, device=device) - 0.5)
+        return v.reshape(-1, 3), torch.arange(3 * c, device=device, dtype=torch.int32).reshape(-1, 3)
+
+    for m in range(M):
+        c = int(counts[m])
+        if sphere[m] and c >= 20:
+            level = 0
+            while 20 * 4 ** (level + 1) <= c and level < 7:
+                level += 1
+            if level not in ico:
+                v0, f0 = scenes.icosphere(level)
+                ico[level] = (torch.as_tensor(v0, dtype=torch.float32, device=device),
+                              torch.as_tensor(f0, dtype=torch.int32, device=device))
+            v0, f0 = ico[level]
+            k = torch.randint(1, 6, (3,), generator=g, device=device).float()
+            ph = 6.283 * torch.rand(3, generator=g, device=device)
+            r = 1.0 + 0.05 * torch.prod(torch.sin(k * v0 + ph), dim=1)
+            v = 0.5 + 0.45 * v0 * r[:, None]
+            f = f0
+            extra = c - f0.shape[0]
+            if extra > 0:
+                sv, sf = soup(extra)
+                f = torch.cat([f0, sf + v.shape[0]])
+                v = torch.cat([v, sv])
+        else:
+            v, f = soup(c)
+        verts.append(v)
+        tris_local.append(f)
+        offs.append(offs[-1] + c)
+        vbase.append(vbase[-1] + v.shape[0])
+    V = torch.cat(verts).contiguous()
+    T = torch.cat([f + vb for f, vb in zip(tris_local, vbase[:-1])]).contiguous()
+    # Rays: per mesh round(per_1k * c / 1000) (at least 1), grouped by mesh.
+    rc = np.maximum(1, np.rint(args.batch_rays_per_1k * counts / 1000.0)).astype(np.int64)
+    owner = torch.as_tensor(np.repeat(np.arange(M, dtype=np.int32), rc), device=device)
+    R = int(rc.sum())
+    u = torch.randn((R, 3), generator=g, device=device)
+    u = u / u.norm(dim=1, keepdim=True)
+    rad = 0.75 * math.sqrt(3.0) + 1e-3
+    o = 0.5 + rad * u
+    tgt = torch.rand((R, 3), generator=g, device=device)
+    d = tgt - o
+    d = d / d.norm(dim=1, keepdim=True)
+    rays = torch.empty((R, 8), dtype=torch.float32, device=device)
+    rays[:, 0:3] = o
+    rays[:, 3] = 0.0
+    rays[:, 4:7] = d
+    rays[:, 7] = 3.0 * rad
+    roffs = np.concatenate([[0], np.cumsum(rc)])
+    return dict(V=V, T=T, offs=np.array(offs, np.int64), vbase=np.array(vbase, np.int64), counts=counts,
+                tris_local=tris_local, rays=rays, owner=owner, roffs=roffs)
+
+
+def bench_batch(ctx, args):
+    """Batched build and differentiable step against the per-mesh loop."""
+    import numpy as np
+    import torch
+    import hippie_amd as hx
+    from hippie_amd.diff import backward as single_backward
+    dev = torch.device("cuda", 0)
+    t0 = time.perf_counter()
+    w = batch_workload(args, dev)
+    M, N, R = args.batch_meshes, int(w["offs"][-1]), int(w["rays"].shape[0])
+    torch.cuda.synchronize()
+    log(f"batch workload: {M} meshes, {N} triangles, {R} rays ({time.perf_counter() - t0:.1f} s)")
+    mb = hx.MeshBatch(w["V"], w["T"], w["offs"])
+    hb = hx.HipBatch(ctx, mb, morton_bits=60)
+    steps = args.batch_steps
+    b_ms = timed(hb.rebuild, steps, 1)
+
+    # Plain path: one BVH per mesh (built once), then a timed loop of rebuilds.
+    scenes_m, bvhs = [], []
+    builder = hx.LBVHBuilder(ctx, morton_bits=60)
+    for m in range(M):
+        vb0, vb1 = int(w["vbase"][m]), int(w["vbase"][m + 1])
+        sc = hx.Scene.__new__(hx.Scene)
+        sc.vertices = w["V"][vb0:vb1]
+        sc.triangles = w["tris_local"][m]
+        scenes_m.append(sc)
+        bvhs.append(builder.build(sc))
+    torch.cuda.synchronize()
+
+    def loop_build():
+        for sc, bv in zip(scenes_m, bvhs):
+            builder.rebuild(bv, sc)
+    l_ms = timed(loop_build, steps, 1)
+    log(f"batched build {b_ms:.3f} ms vs per-mesh loop {l_ms:.3f} ms ({l_ms / b_ms:.1f}x)")
+
+    # Differentiable step: refit -> trace -> loss (t - t_ref)^2 over hits -> adjoint.
+    rays, owner = w["rays"], w["owner"]
+    res = hb.trace(rays, owner)
+    t_ref = res[:, 1].view(torch.float32).clone() * 0.999
+    grad = torch.zeros_like(mb.scene.vertices)
+    dl_dt = torch.zeros(R, dtype=torch.float32, device=dev)
+
+    def batched_step():
+        hb.refit()
+        hb.trace(rays, owner, results=res)
+        hit = res[:, 0] >= 0
+        t = res[:, 1].view(torch.float32)
+        torch.where(hit, 2.0 * (t - t_ref), torch.zeros_like(t), out=dl_dt)
+        grad.zero_()
+        hb.backward(rays, owner, res, dl_dt=dl_dt, out=grad)
+
+    e_ms = timed(batched_step, steps, 1)
+    graph_ms = None
+    try:
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            ctx.set_stream(s.cuda_stream)
+            batched_step()
+        torch.cuda.current_stream().wait_stream(s)
+        torch.cuda.synchronize()
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+            batched_step()
+        ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+        graph_ms = timed(graph.replay, steps, 1)
+    except Exception as e:  # capture is optional; the eager step is the measured fallback
+        log(f"graph capture failed: {e}")
+        ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+
+    roffs = w["roffs"]
+    res_m = [res[int(roffs[m]):int(roffs[m + 1])] for m in range(M)]
+    rays_m = [rays[int(roffs[m]):int(roffs[m + 1])] for m in range(M)]
+    dl_m = [dl_dt[int(roffs[m]):int(roffs[m + 1])] for m in range(M)]
+    gm = [grad[int(w["vbase"][m]):int(w["vbase"][m + 1])] for m in range(M)]
+    tracer = hx.HipTracer(ctx)
+
+    def loop_step():
+        for m in range(M):
+            bv, sc = bvhs[m], scenes_m[m]
+            bv.update(sc)
+            tracer.set_bvh(bv)
+            rb = hx.RayBuffer.__new__(hx.RayBuffer)
+            rb.rays, rb.results, rb.closest_hit = rays_m[m], res_m[m], True
+            tracer.trace(rb)
+            single_backward(ctx, sc.vertices, sc.triangles, rays_m[m], res_m[m], dl_dt=dl_m[m], out=gm[m])
+    ld_ms = timed(loop_step, steps, 1)
+    best = graph_ms if graph_ms is not None else e_ms
+    log(f"diff step: batched {e_ms:.3f} ms, graph {graph_ms} ms, per-mesh loop {ld_ms:.3f} ms")
+    out = {
+        "workload": f"{M} meshes, {N} triangles (log-uniform {args.batch_lo}..{args.batch_hi}, half displaced "
+                    f"icospheres), {R} rays",
+        "meshes": M, "triangles": N, "rays": R,
+        "build_ms": round(b_ms, 3), "build_mtris_s": round(N / (b_ms * 1e-3) / 1e6, 2),
+        "loop_build_ms": round(l_ms, 3), "build_speedup": round(l_ms / b_ms, 2),
+        "diff_step_ms": round(e_ms, 3),
+        "diff_step_graph_ms": None if graph_ms is None else round(graph_ms, 3),
+        "loop_diff_step_ms": round(ld_ms, 3), "diff_step_speedup": round(ld_ms / best, 2),
+    }
+    for bv in bvhs:
+        del bv
+    return out
 
 
 def main():
@@ -163,6 +396,11 @@ def main():
     bytes_per_launch = sort_bytes_per_step / max(launches_per_step, 1)
     achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9 if sort_cnt else 0.0
 
+    builders = bench_builders(ctx, scene, n, args) if args.builders else {}
+    if builders:
+        builders["lbvh"] = {"ms": round(build_ms, 3), "sah": round(bvh.get_cost(), 4)}
+    batched = bench_batch(ctx, args) if args.batch_meshes > 0 else None
+
     cpu = None
     if not args.no_cpu_baseline:
         log("CPU baseline (oracle)")
@@ -207,6 +445,8 @@ def main():
             "avg_launch_ms": round(avg_launch_ms, 5),
         },
         "cpu_baseline": cpu,
+        "builders": builders,
+        "batched": batched,
     }
     print(json.dumps(out), flush=True)
 

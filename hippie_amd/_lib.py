@@ -114,6 +114,14 @@ SIGNATURES = {
     "hx_ray_keys": [_p, _p, _i64, _i32, _i32, _f32, _p],
     "hx_memcpy": [_p, _p, _p, _i64],
     "hx_backward": [_p, _p, _p, _p, _p, _i64, _p, _p, _p, _p],
+    "hx_build_batch": [_p, _p, _p, _i64, _p, _p, _i32, _p],
+    "hx_batch_refit": [_p, _p, _p, _p],
+    "hx_batch_rebuild": [_p, _p, _p, _p],
+    "hx_batch_destroy": [_p],
+    "hx_batch_mesh": [_p, _i32, _p],
+    "hx_batch_trace": [_p, _p, _p, _p, _i64, _i32, _p, _p],
+    "hx_batch_backward": [_p, _p, _p, _p, _p, _p, _p, _i64, _p, _p, _p, _p],
+    "hx_batch_recompute_hits": [_p, _p, _p, _p, _p, _p, _p, _i64, _p, _p, _p],
     "hx_recompute_hits": [_p, _p, _p, _p, _p, _i64, _p, _p, _p],
     "hx_trace_overflow_count": [_p, ctypes.POINTER(_i64)],
 }

@@ -431,3 +431,115 @@ def ray_keys(ctx, rays, method="aila", bits=60, ray_length=0.25):
     call("hx_ray_keys", ctx.handle, _ptr(rays), int(rays.shape[0]), RAY_KEY_METHODS[method], int(bits),
          float(ray_length), _ptr(keys))
     return keys
+
+
+class HxBatchMeshView(ctypes.Structure):
+    _fields_ = [("nodes", ctypes.c_void_p), ("woop", ctypes.c_void_p), ("tri_indices", ctypes.c_void_p),
+                ("n_interior", ctypes.c_int64), ("n_leaves", ctypes.c_int64), ("n_references", ctypes.c_int64),
+                ("tri_offset", ctypes.c_int64)]
+
+
+class MeshBatch:
+    """Many meshes concatenated: shared vertices, triangles with global vertex
+    indices, mesh m = triangles [offsets[m], offsets[m+1])."""
+
+    def __init__(self, vertices, triangles, offsets, device=0):
+        self.scene = Scene(vertices, triangles, device)
+        self.offsets = np.ascontiguousarray(np.asarray(offsets, dtype=np.int64))
+        if self.offsets.ndim != 1 or self.offsets[0] != 0 or self.offsets[-1] != self.scene.n_triangles:
+            raise ValueError("offsets must run from 0 to the number of triangles")
+
+    @classmethod
+    def from_meshes(cls, meshes, device=0):
+        """meshes: list of (vertices (V,3), triangles (F,3) local indices)."""
+        vs, ts, offs, vbase = [], [], [0], 0
+        for v, t in meshes:
+            vs.append(np.asarray(v, np.float32))
+            ts.append(np.asarray(t, np.int32) + vbase)
+            vbase += len(v)
+            offs.append(offs[-1] + len(t))
+        return cls(np.concatenate(vs), np.concatenate(ts), offs, device)
+
+    @property
+    def n_meshes(self):
+        return int(self.offsets.shape[0] - 1)
+
+
+class HipBatch:
+    """Batched LBVH (hx_build_batch): one Bin BVH per mesh, built, refit and
+    traced together."""
+
+    def __init__(self, ctx, batch, params=None, **kw):
+        self.ctx = ctx or Context.default()
+        self.batch = batch
+        self.params = params or BuildParams(method="lbvh")
+        for k, v in kw.items():
+            setattr(self.params, k, v)
+        h = ctypes.c_void_p()
+        s = batch.scene
+        call("hx_build_batch", self.ctx.handle, ctypes.byref(self.params.c), _ptr(s.vertices), s.n_vertices,
+             _ptr(s.triangles), batch.offsets.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), batch.n_meshes,
+             ctypes.byref(h))
+        self.handle = h
+
+    def mesh_view(self, m):
+        v = HxBatchMeshView()
+        call("hx_batch_mesh", self.handle, int(m), ctypes.byref(v))
+        return v
+
+    def mesh_host(self, m):
+        """Host copies of mesh m's Bin nodes, Woop rows and mesh-local triangle ids."""
+        v = self.mesh_view(m)
+        nn = v.n_interior + v.n_leaves
+        nodes = np.empty((nn, 16), np.int32)
+        woop = np.empty((v.n_references, 12), np.float32)
+        tri = np.empty(v.n_references, np.int32)
+        for arr, ptr in ((nodes, v.nodes), (woop, v.woop), (tri, v.tri_indices)):
+            self.ctx.memcpy(arr.ctypes.data, ptr, arr.nbytes)
+        return dict(nodes=nodes, woop=woop, tri_idx=tri, n_interior=v.n_interior, n_leaves=v.n_leaves)
+
+    def rebuild(self):
+        s = self.batch.scene
+        call("hx_batch_rebuild", self.ctx.handle, self.handle, _ptr(s.vertices), _ptr(s.triangles))
+
+    def refit(self, vertices=None):
+        s = self.batch.scene
+        if vertices is not None:
+            s.vertices = torch.as_tensor(vertices).to(device=s.vertices.device, dtype=torch.float32).contiguous()
+        call("hx_batch_refit", self.ctx.handle, self.handle, _ptr(s.vertices), _ptr(s.triangles))
+
+    def trace(self, rays, ray_mesh, any_hit=False, results=None):
+        """rays: RayBuffer or (R,8) tensor on the device; ray_mesh: int32 (R,) on the device."""
+        r = rays.rays if isinstance(rays, RayBuffer) else rays
+        n = int(r.shape[0])
+        if results is None:
+            results = torch.zeros((n, 4), dtype=torch.int32, device=r.device)
+        call("hx_batch_trace", self.ctx.handle, self.handle, _ptr(r), _ptr(ray_mesh), n, int(bool(any_hit)),
+             _ptr(results), None)
+        return results
+
+    def backward(self, rays, ray_mesh, results, dl_dt=None, dl_du=None, dl_dv=None, out=None):
+        s = self.batch.scene
+        n = int(rays.shape[0])
+        if out is None:
+            out = torch.zeros_like(s.vertices)
+        call("hx_batch_backward", self.ctx.handle, self.handle, _ptr(s.vertices), _ptr(s.triangles), _ptr(rays),
+             _ptr(ray_mesh), _ptr(results), n, _ptr(dl_dt), _ptr(dl_du), _ptr(dl_dv), _ptr(out))
+        return out
+
+    def recompute_hits(self, rays, ray_mesh, results):
+        s = self.batch.scene
+        n = int(rays.shape[0])
+        t, u, v = (torch.empty(n, dtype=torch.float32, device=rays.device) for _ in range(3))
+        call("hx_batch_recompute_hits", self.ctx.handle, self.handle, _ptr(s.vertices), _ptr(s.triangles),
+             _ptr(rays), _ptr(ray_mesh), _ptr(results), n, _ptr(t), _ptr(u), _ptr(v))
+        return t, u, v
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            try:
+                call("hx_batch_destroy", h)
+            except Exception:
+                pass
+            self.handle = None

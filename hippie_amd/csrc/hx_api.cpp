@@ -121,6 +121,8 @@ struct hx_context {
     Context c;
 };
 
+struct hx_batch;
+
 struct hx_bvh {
     hx_context* ctx = nullptr;
     hx_build_params params{};
@@ -140,11 +142,24 @@ struct hx_bvh {
     DevBuf split, ploc_work, atr_work;
     int ploc_rounds = 0;
     int64_t atr_accepted = 0;
+    // Per-mesh tables of the emitted layout (one entry unless batched).
+    DevBuf mesh_tab, root_slot;
+    // Batched builds: meshes occupy triangle ranges [off[m], off[m+1]).
+    bool batched = false;
+    int n_meshes = 1;
+    std::vector<int32_t> h_off;
+    std::vector<int4> h_tab;
+    bool tab_known = false;
+    DevBuf mesh_off, mesh_of, mesh_box, mesh_box_bits, keys_orig, tri_local;
     bool built = false;
     bool counts_known = false;
     int64_t n_interior = 0, n_leaves = 0;
     float host_box[6] = {0, 0, 0, 0, 0, 0};
     bool box_known = false;
+};
+
+struct hx_batch {
+    hx_bvh bvh;
 };
 
 namespace {
@@ -224,6 +239,17 @@ int reserve_bvh(hx_bvh* b, int64_t n) {
     HX_TRY(b->counts.reserve(16));
     HX_TRY(b->scene_box.reserve(32));
     HX_TRY(b->cost_out.reserve(16));
+    HX_TRY(b->mesh_tab.reserve(16 * (size_t)b->n_meshes));
+    HX_TRY(b->root_slot.reserve(4 * (size_t)b->n_meshes));
+    if (b->batched) {
+        const size_t M = (size_t)b->n_meshes;
+        HX_TRY(b->mesh_off.reserve(4 * (M + 1)));
+        HX_TRY(b->mesh_of.reserve(4 * (size_t)n));
+        HX_TRY(b->mesh_box.reserve(24 * M));
+        HX_TRY(b->mesh_box_bits.reserve(24 * M));
+        HX_TRY(b->keys_orig.reserve(8 * (size_t)n));
+        HX_TRY(b->tri_local.reserve(4 * (size_t)n));
+    }
     if (b->params.method == HX_METHOD_PLOC) {
         HX_TRY(b->split.reserve(4 * (size_t)n));
         HX_TRY(b->ploc_work.reserve(ploc_work_bytes(n)));
@@ -258,6 +284,8 @@ hx::ClimbArgs climb_args(hx_bvh* b, bool refit_only) {
     ca.woop = b->woop.as<float4>();
     ca.frontier = b->frontier.as<int32_t>();
     ca.frontier_count = b->counts.as<int32_t>() + 2;
+    ca.mesh_of = b->batched ? b->mesh_of.as<int32_t>() : nullptr;
+    ca.mesh_off = b->batched ? b->mesh_off.as<int32_t>() : nullptr;
     ca.refit_only = refit_only ? 1 : 0;
     ca.adaptive = (p.adaptive_leaf_size && !refit_only) ? 1 : 0;
     ca.max_leaf_size = p.max_leaf_size;
@@ -282,6 +310,11 @@ hx::EmitArgs emit_args(hx_bvh* b) {
     ea.int_slot = b->int_slot.as<int32_t>();
     ea.counts = b->counts.as<int32_t>();
     ea.nodes = b->nodes.as<int4>();
+    ea.mesh_of = b->batched ? b->mesh_of.as<int32_t>() : nullptr;
+    ea.mesh_off = b->batched ? b->mesh_off.as<int32_t>() : nullptr;
+    ea.n_meshes = b->n_meshes;
+    ea.mesh_tab = b->mesh_tab.as<int4>();
+    ea.root_slot = b->root_slot.as<int32_t>();
     return ea;
 }
 
@@ -427,6 +460,66 @@ int run_atr(hx_bvh* b) {
     HX_TRY(hx::launch_fill_i32(c, b->cover.as<int32_t>(), -1, n));
     HX_TRY(hx::launch_climb(c, climb_args(b, false)));
     return hx::launch_collapse_emit(c, emit_args(b));
+}
+
+// Batched LBVH: per-mesh boxes and Morton codes, (mesh, Morton) order from
+// two stable sorts, one Karras construction over the composite key, then the
+// shared climb and collapse with per-mesh tables.
+int run_batch(hx_bvh* b) {
+    Context* c = &b->ctx->c;
+    const hx_build_params& p = b->params;
+    const int64_t n = b->n_refs;
+    const int M = b->n_meshes;
+    HX_HIP(hipSetDevice(c->device));
+    int32_t* off = b->mesh_off.as<int32_t>();
+    int32_t* mesh_of = b->mesh_of.as<int32_t>();
+    HX_HIP(hipMemcpyAsync(off, b->h_off.data(), 4 * (size_t)(M + 1), hipMemcpyHostToDevice, c->stream));
+    HX_TRY(hx::launch_mesh_of(c, off, M, n, mesh_of));
+    HX_TRY(hx::launch_mesh_boxes(c, b->verts, b->tris, n, mesh_of, M, b->mesh_box_bits.as<uint32_t>(),
+                                 b->mesh_box.as<float>()));
+    uint64_t* korig = b->keys_orig.as<uint64_t>();
+    HX_TRY(hx::launch_morton(c, b->verts, b->tris, n, b->mesh_box.as<float>(), p.morton_bits, korig, true, mesh_of));
+    HX_HIP(hipMemcpyAsync(b->keys[0].ptr, korig, 8 * (size_t)n, hipMemcpyDeviceToDevice, c->stream));
+    void* keys[2] = {b->keys[0].ptr, b->keys[1].ptr};
+    int32_t* vals[2] = {b->tri_idx.as<int32_t>(), b->vals_tmp.as<int32_t>()};
+    int ri = 0;
+    HX_TRY(hx::launch_sort_pairs(c, keys, vals, n, p.morton_bits, true, true, &ri));
+    int32_t* sorted = vals[ri];
+    if (M > 1) {
+        int mbits = 0;
+        while ((1 << mbits) < M) ++mbits;
+        HX_TRY(hx::launch_gather_mesh_keys(c, sorted, mesh_of, n, static_cast<uint32_t*>(keys[0])));
+        int32_t* v2[2] = {sorted, vals[ri ^ 1]};
+        int r2 = 0;
+        HX_TRY(hx::launch_sort_pairs(c, keys, v2, n, mbits, false, false, &r2));
+        sorted = v2[r2];
+    }
+    if (sorted != b->tri_idx.as<int32_t>())
+        HX_HIP(hipMemcpyAsync(b->tri_idx.ptr, sorted, 4 * (size_t)n, hipMemcpyDeviceToDevice, c->stream));
+    HX_TRY(hx::launch_gather_keys(c, b->tri_idx.as<int32_t>(), korig, n, static_cast<uint64_t*>(keys[0])));
+    b->keys_index = 0;
+    HX_TRY(hx::launch_construct(c, keys[0], true, n, b->left.as<int32_t>(), b->right.as<int32_t>(),
+                                b->parent.as<int32_t>(), b->first.as<int32_t>(), b->last.as<int32_t>(),
+                                b->counters.as<int32_t>(), b->cover.as<int32_t>(), b->counts.as<int32_t>() + 2,
+                                mesh_of, off));
+    HX_TRY(hx::launch_climb(c, climb_args(b, false)));
+    HX_TRY(hx::launch_collapse_emit(c, emit_args(b)));
+    HX_TRY(hx::launch_tri_local(c, b->tri_idx.as<int32_t>(), mesh_of, off, n, b->tri_local.as<int32_t>()));
+    b->counts_known = false;
+    b->tab_known = false;
+    b->built = true;
+    return HX_OK;
+}
+
+int read_tables(hx_bvh* b) {
+    if (b->tab_known) return HX_OK;
+    Context* c = &b->ctx->c;
+    b->h_tab.resize(b->n_meshes);
+    HX_HIP(hipMemcpyAsync(b->h_tab.data(), b->mesh_tab.ptr, sizeof(int4) * (size_t)b->n_meshes, hipMemcpyDeviceToHost,
+                          c->stream));
+    HX_HIP(hipStreamSynchronize(c->stream));
+    b->tab_known = true;
+    return HX_OK;
 }
 
 int run_build(hx_bvh* b) {
@@ -739,6 +832,156 @@ int hx_recompute_hits(hx_context* ctx, const float* d_vertices, const int32_t* d
     a.tris = d_triangles;
     a.rays = reinterpret_cast<const float4*>(d_rays);
     a.results = reinterpret_cast<const int4*>(d_results);
+    return hx::launch_recompute_hits(&ctx->c, a, d_t, d_u, d_v);
+}
+
+int hx_build_batch(hx_context* ctx, const hx_build_params* params, const float* d_vertices, int64_t n_vertices,
+                   const int32_t* d_triangles, const int64_t* h_mesh_offsets, int32_t n_meshes, hx_batch** out) {
+    HX_CHECK_ARG(ctx && out && h_mesh_offsets, "NULL argument");
+    *out = nullptr;
+    HX_TRY(validate_params(params));
+    HX_CHECK_ARG(params->method == HX_METHOD_LBVH, "batched builds use the LBVH method (got %d)", params->method);
+    HX_CHECK_ARG(n_meshes >= 1, "n_meshes must be positive (got %d)", n_meshes);
+    HX_CHECK_ARG(h_mesh_offsets[0] == 0, "mesh offsets must start at 0");
+    const int64_t n = h_mesh_offsets[n_meshes];
+    HX_CHECK_ARG(n < ((int64_t)1 << 30), "total number of triangles must be < 2^30");
+    for (int32_t m = 0; m < n_meshes; ++m)
+        HX_CHECK_ARG(h_mesh_offsets[m + 1] - h_mesh_offsets[m] >= 2, "mesh %d has fewer than 2 triangles", m);
+    HX_CHECK_ARG(d_vertices && d_triangles && n_vertices > 0, "geometry pointers/counts invalid");
+    hx_batch* bt = new (std::nothrow) hx_batch();
+    if (!bt) return HX_ERR_OUT_OF_MEMORY;
+    hx_bvh* b = &bt->bvh;
+    b->ctx = ctx;
+    b->params = *params;
+    b->n_refs = n;
+    b->verts = d_vertices;
+    b->tris = d_triangles;
+    b->n_verts = n_vertices;
+    b->keys64 = true;
+    b->batched = true;
+    b->n_meshes = n_meshes;
+    b->h_off.assign(h_mesh_offsets, h_mesh_offsets + n_meshes + 1);
+    int st = reserve_bvh(b, n);
+    if (st == HX_OK) st = run_batch(b);
+    if (st != HX_OK) {
+        delete bt;
+        return st;
+    }
+    *out = bt;
+    return HX_OK;
+}
+
+int hx_batch_refit(hx_context* ctx, hx_batch* batch, const float* d_vertices, const int32_t* d_triangles) {
+    HX_CHECK_ARG(ctx && batch && d_vertices && d_triangles, "NULL argument");
+    hx_bvh* b = &batch->bvh;
+    b->ctx = ctx;
+    b->verts = d_vertices;
+    b->tris = d_triangles;
+    return run_refit(b);
+}
+
+int hx_batch_rebuild(hx_context* ctx, hx_batch* batch, const float* d_vertices, const int32_t* d_triangles) {
+    HX_CHECK_ARG(ctx && batch && d_vertices && d_triangles, "NULL argument");
+    hx_bvh* b = &batch->bvh;
+    b->ctx = ctx;
+    b->verts = d_vertices;
+    b->tris = d_triangles;
+    return run_batch(b);
+}
+
+int hx_batch_destroy(hx_batch* batch) {
+    if (!batch) return HX_OK;
+    if (batch->bvh.ctx) {
+        (void)hipSetDevice(batch->bvh.ctx->c.device);
+        (void)hipStreamSynchronize(batch->bvh.ctx->c.stream);
+    }
+    delete batch;
+    return HX_OK;
+}
+
+int hx_batch_mesh(hx_batch* batch, int32_t mesh, hx_batch_mesh_view* out) {
+    HX_CHECK_ARG(batch && out, "NULL argument");
+    hx_bvh* b = &batch->bvh;
+    HX_CHECK_ARG(mesh >= 0 && mesh < b->n_meshes, "mesh %d out of range (%d meshes)", mesh, b->n_meshes);
+    HX_TRY(read_counts(b));
+    HX_TRY(read_tables(b));
+    const int4 t = b->h_tab[mesh];
+    const int next_leaf = mesh + 1 < b->n_meshes ? b->h_tab[mesh + 1].z : (int)b->n_leaves;
+    const int64_t off = b->h_off[mesh];
+    out->nodes = b->nodes.as<int32_t>() + 16 * (int64_t)t.x;
+    out->woop = b->woop.as<float>() + 12 * off;
+    out->tri_indices = b->tri_local.as<int32_t>() + off;
+    out->n_interior = t.w;
+    out->n_leaves = next_leaf - t.z;
+    out->n_references = b->h_off[mesh + 1] - off;
+    out->tri_offset = off;
+    return HX_OK;
+}
+
+int hx_batch_trace(hx_context* ctx, hx_batch* batch, const float* d_rays, const int32_t* d_ray_mesh,
+                   int64_t n_rays, int32_t any_hit, int32_t* d_results, int32_t* d_stats) {
+    HX_CHECK_ARG(ctx && batch && d_ray_mesh, "NULL argument");
+    HX_CHECK_ARG(n_rays >= 0, "n_rays must be >= 0");
+    if (n_rays == 0) return HX_OK;
+    HX_CHECK_ARG(d_rays && d_results, "rays and results must be device pointers");
+    hx_bvh* b = &batch->bvh;
+    Context* c = &ctx->c;
+    HX_TRY(c->control.reserve(64));
+    hx::TraceArgs a{};
+    a.n_rays = n_rays;
+    a.any_hit = any_hit ? 1 : 0;
+    a.rays = reinterpret_cast<const float4*>(d_rays);
+    a.results = reinterpret_cast<int4*>(d_results);
+    a.nodes = b->nodes.as<int4>();
+    a.woop = b->woop.as<float4>();
+    a.tri_idx = b->tri_local.as<int32_t>();
+    a.stats = reinterpret_cast<int2*>(d_stats);
+    a.control = c->control.as<int32_t>();
+    a.fetch_threshold = c->tune.trace_fetch;
+    a.ray_mesh = d_ray_mesh;
+    a.mesh_tab = b->mesh_tab.as<int4>();
+    a.mesh_off = b->mesh_off.as<int32_t>();
+    a.n_meshes = b->n_meshes;
+    return hx::launch_trace(c, a);
+}
+
+int hx_batch_backward(hx_context* ctx, hx_batch* batch, const float* d_vertices, const int32_t* d_triangles,
+                      const float* d_rays, const int32_t* d_ray_mesh, const int32_t* d_results, int64_t n_rays,
+                      const float* d_dl_dt, const float* d_dl_du, const float* d_dl_dv, float* d_dl_dvertices) {
+    HX_CHECK_ARG(ctx && batch && d_vertices && d_triangles && d_ray_mesh && d_dl_dvertices, "NULL argument");
+    HX_CHECK_ARG(n_rays >= 0, "n_rays must be >= 0");
+    if (n_rays == 0) return HX_OK;
+    HX_CHECK_ARG(d_rays && d_results, "rays and results must be device pointers");
+    hx::BackwardArgs a{};
+    a.n_rays = n_rays;
+    a.verts = d_vertices;
+    a.tris = d_triangles;
+    a.rays = reinterpret_cast<const float4*>(d_rays);
+    a.results = reinterpret_cast<const int4*>(d_results);
+    a.dl_dt = d_dl_dt;
+    a.dl_du = d_dl_du;
+    a.dl_dv = d_dl_dv;
+    a.dl_dverts = d_dl_dvertices;
+    a.ray_mesh = d_ray_mesh;
+    a.mesh_off = batch->bvh.mesh_off.as<int32_t>();
+    return hx::launch_backward(&ctx->c, a);
+}
+
+int hx_batch_recompute_hits(hx_context* ctx, hx_batch* batch, const float* d_vertices, const int32_t* d_triangles,
+                            const float* d_rays, const int32_t* d_ray_mesh, const int32_t* d_results, int64_t n_rays,
+                            float* d_t, float* d_u, float* d_v) {
+    HX_CHECK_ARG(ctx && batch && d_vertices && d_triangles && d_ray_mesh && d_t && d_u && d_v, "NULL argument");
+    HX_CHECK_ARG(n_rays >= 0, "n_rays must be >= 0");
+    if (n_rays == 0) return HX_OK;
+    HX_CHECK_ARG(d_rays && d_results, "rays and results must be device pointers");
+    hx::BackwardArgs a{};
+    a.n_rays = n_rays;
+    a.verts = d_vertices;
+    a.tris = d_triangles;
+    a.rays = reinterpret_cast<const float4*>(d_rays);
+    a.results = reinterpret_cast<const int4*>(d_results);
+    a.ray_mesh = d_ray_mesh;
+    a.mesh_off = batch->bvh.mesh_off.as<int32_t>();
     return hx::launch_recompute_hits(&ctx->c, a, d_t, d_u, d_v);
 }
 

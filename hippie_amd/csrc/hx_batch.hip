@@ -1,0 +1,155 @@
+// hx_batch.hip -- the pieces a batched build adds to the single-mesh path.
+//
+// Many independent meshes are built as one: triangles of mesh m occupy
+// positions [off[m], off[m+1]) of the input, every triangle is quantised in
+// its own mesh's box, the references are sorted by (mesh, Morton) with two
+// stable radix sorts (Morton digits, then mesh digits), and one Karras
+// construction over the composite key gives every mesh its own subtree (the
+// separator nodes above the mesh roots are never emitted).  No reference
+// counterpart: the reference builds one scene per HipBVH.
+#include "hx_device.h"
+#include "hx_internal.h"
+
+namespace hx {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+inline unsigned grid_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+// Float <-> order-preserving unsigned key, so min/max are integer atomics.
+HX_DEV uint32_t ord(float f) {
+    const uint32_t b = __float_as_uint(f);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+HX_DEV float unord(uint32_t k) { return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k); }
+
+__global__ __launch_bounds__(kBlock) void k_mesh_of(const int32_t* __restrict__ off, int M, int n,
+                                                    int32_t* __restrict__ mesh_of) {
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= n) return;
+    int lo = 0, hi = M - 1;  // last m with off[m] <= t
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    mesh_of[t] = lo;
+}
+
+__global__ __launch_bounds__(kBlock) void k_mesh_box_init(uint32_t* __restrict__ bits, int M) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= 6 * M) return;
+    bits[i] = (i % 6) < 3 ? 0xffffffffu : 0u;
+}
+
+// Exact per-mesh min/max of the vertices of its triangles.  A wave whose 64
+// triangles share a mesh (all but the waves at mesh boundaries) reduces in
+// registers and issues six atomics.
+__global__ __launch_bounds__(kBlock) void k_mesh_box(const float* __restrict__ verts, const int32_t* __restrict__ tris,
+                                                     int n, const int32_t* __restrict__ mesh_of,
+                                                     uint32_t* __restrict__ bits) {
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    const bool live = t < n;
+    int m = -1;
+    float r[6] = {kMaxFloat, kMaxFloat, kMaxFloat, -kMaxFloat, -kMaxFloat, -kMaxFloat};
+    if (live) {
+        m = mesh_of[t];
+        const int32_t* tr = tris + 3 * (int64_t)t;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const float* v = verts + 3 * (int64_t)tr[k];
+            r[0] = fminf(r[0], v[0]);
+            r[1] = fminf(r[1], v[1]);
+            r[2] = fminf(r[2], v[2]);
+            r[3] = fmaxf(r[3], v[0]);
+            r[4] = fmaxf(r[4], v[1]);
+            r[5] = fmaxf(r[5], v[2]);
+        }
+    }
+    const int m0 = __shfl(m, 0, kWave);
+    const bool uniform = __all(m == m0 || !live) && m0 >= 0;
+    if (uniform) {
+#pragma unroll
+        for (int s = 1; s < kWave; s <<= 1) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) r[k] = fminf(r[k], __shfl_xor(r[k], s, kWave));
+#pragma unroll
+            for (int k = 3; k < 6; ++k) r[k] = fmaxf(r[k], __shfl_xor(r[k], s, kWave));
+        }
+        if (lane_id() == 0) {
+            for (int k = 0; k < 3; ++k) atomicMin(bits + 6 * m0 + k, ord(r[k]));
+            for (int k = 3; k < 6; ++k) atomicMax(bits + 6 * m0 + k, ord(r[k]));
+        }
+    } else if (live) {
+        for (int k = 0; k < 3; ++k) atomicMin(bits + 6 * m + k, ord(r[k]));
+        for (int k = 3; k < 6; ++k) atomicMax(bits + 6 * m + k, ord(r[k]));
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_mesh_box_final(const uint32_t* __restrict__ bits, int M,
+                                                           float* __restrict__ boxes) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < 6 * M) boxes[i] = unord(bits[i]);
+}
+
+__global__ __launch_bounds__(kBlock) void k_gather_mesh_key(const int32_t* __restrict__ vals,
+                                                            const int32_t* __restrict__ mesh_of, int n,
+                                                            uint32_t* __restrict__ keys) {
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    if (p < n) keys[p] = (uint32_t)mesh_of[vals[p]];
+}
+
+__global__ __launch_bounds__(kBlock) void k_gather_key(const int32_t* __restrict__ vals,
+                                                       const uint64_t* __restrict__ src, int n,
+                                                       uint64_t* __restrict__ dst) {
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    if (p < n) dst[p] = src[vals[p]];
+}
+
+__global__ __launch_bounds__(kBlock) void k_tri_local(const int32_t* __restrict__ sorted_tri,
+                                                      const int32_t* __restrict__ mesh_of,
+                                                      const int32_t* __restrict__ off, int n,
+                                                      int32_t* __restrict__ tri_local) {
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    if (p < n) tri_local[p] = sorted_tri[p] - off[mesh_of[p]];
+}
+
+}  // namespace
+
+int launch_mesh_of(Context* c, const int32_t* off, int M, int64_t n, int32_t* mesh_of) {
+    LaunchScope ls(c, "batch_mesh_of");
+    hipLaunchKernelGGL(k_mesh_of, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, off, M, (int)n, mesh_of);
+    return check_launch("mesh_of");
+}
+
+int launch_mesh_boxes(Context* c, const float* verts, const int32_t* tris, int64_t n, const int32_t* mesh_of, int M,
+                      uint32_t* bits, float* boxes) {
+    LaunchScope ls(c, "batch_mesh_boxes");
+    hipLaunchKernelGGL(k_mesh_box_init, dim3(grid_for(6 * (int64_t)M)), dim3(kBlock), 0, c->stream, bits, M);
+    HX_TRY(check_launch("mesh_box_init"));
+    hipLaunchKernelGGL(k_mesh_box, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, verts, tris, (int)n, mesh_of, bits);
+    HX_TRY(check_launch("mesh_box"));
+    hipLaunchKernelGGL(k_mesh_box_final, dim3(grid_for(6 * (int64_t)M)), dim3(kBlock), 0, c->stream, bits, M, boxes);
+    return check_launch("mesh_box_final");
+}
+
+int launch_gather_mesh_keys(Context* c, const int32_t* vals, const int32_t* mesh_of, int64_t n, uint32_t* keys) {
+    hipLaunchKernelGGL(k_gather_mesh_key, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, vals, mesh_of, (int)n, keys);
+    return check_launch("gather_mesh_key");
+}
+
+int launch_gather_keys(Context* c, const int32_t* vals, const uint64_t* src, int64_t n, uint64_t* dst) {
+    hipLaunchKernelGGL(k_gather_key, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, vals, src, (int)n, dst);
+    return check_launch("gather_key");
+}
+
+int launch_tri_local(Context* c, const int32_t* sorted_tri, const int32_t* mesh_of, const int32_t* off, int64_t n,
+                     int32_t* tri_local) {
+    hipLaunchKernelGGL(k_tri_local, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, sorted_tri, mesh_of, off, (int)n,
+                       tri_local);
+    return check_launch("tri_local");
+}
+
+}  // namespace hx

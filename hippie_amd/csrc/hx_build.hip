@@ -73,11 +73,32 @@ __global__ __launch_bounds__(kBlock) void k_scene_box_partial(const float* __res
     }
 }
 
-__global__ void k_scene_box_final(const float* __restrict__ partial, int nblocks, float* __restrict__ box) {
-    const int k = threadIdx.x;
-    if (k < 6) {
-        float v = (k < 3) ? kMaxFloat : -kMaxFloat;
-        for (int i = 0; i < nblocks; ++i) v = (k < 3) ? fminf(v, partial[i * 6 + k]) : fmaxf(v, partial[i * 6 + k]);
+// One block: every thread folds a strided share of the partials, then the
+// block reduces (min/max are exact, so the order does not matter).
+__global__ __launch_bounds__(kBlock) void k_scene_box_final(const float* __restrict__ partial, int nblocks,
+                                                            float* __restrict__ box) {
+    float r[6] = {kMaxFloat, kMaxFloat, kMaxFloat, -kMaxFloat, -kMaxFloat, -kMaxFloat};
+    for (int i = threadIdx.x; i < nblocks; i += kBlock) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) r[k] = fminf(r[k], partial[i * 6 + k]);
+#pragma unroll
+        for (int k = 3; k < 6; ++k) r[k] = fmaxf(r[k], partial[i * 6 + k]);
+    }
+#pragma unroll
+    for (int m = 1; m < kWave; m <<= 1) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) r[k] = fminf(r[k], __shfl_xor(r[k], m, kWave));
+#pragma unroll
+        for (int k = 3; k < 6; ++k) r[k] = fmaxf(r[k], __shfl_xor(r[k], m, kWave));
+    }
+    __shared__ float s[kWavesPerBlock][6];
+    if (lane_id() == 0)
+        for (int k = 0; k < 6; ++k) s[threadIdx.x / kWave][k] = r[k];
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int k = threadIdx.x;
+        float v = s[0][k];
+        for (int w = 1; w < kWavesPerBlock; ++w) v = (k < 3) ? fminf(v, s[w][k]) : fmaxf(v, s[w][k]);
         box[k] = v;
     }
 }
@@ -87,12 +108,14 @@ __global__ void k_scene_box_final(const float* __restrict__ partial, int nblocks
 // :65-98,131-160).  The per-triangle box is recomputed, never stored.
 // ---------------------------------------------------------------------------
 
+// mesh_of (batched builds): each triangle is quantised in its own mesh's box.
 template <bool K64>
 __global__ __launch_bounds__(kBlock) void k_morton(const float* __restrict__ verts, const int32_t* __restrict__ tris,
-                                                   int64_t n, const float* __restrict__ sb, int shift,
-                                                   void* __restrict__ keys_out) {
+                                                   int64_t n, const float* __restrict__ boxes, int shift,
+                                                   void* __restrict__ keys_out, const int32_t* __restrict__ mesh_of) {
     const int64_t t = blockIdx.x * (int64_t)kBlock + threadIdx.x;
     if (t >= n) return;
+    const float* sb = mesh_of ? boxes + 6 * (int64_t)mesh_of[t] : boxes;
     const float3 smn = make_float3(sb[0], sb[1], sb[2]);
     const float3 scale = make_float3(1.0f / (sb[3] - sb[0]), 1.0f / (sb[4] - sb[1]), 1.0f / (sb[5] - sb[2]));
     float3 a, b, c;
@@ -120,17 +143,29 @@ __global__ __launch_bounds__(kBlock) void k_morton(const float* __restrict__ ver
 // the leaf entries setupLeaves gives left/right (:185-187).
 // ---------------------------------------------------------------------------
 
+// Batched builds (mesh_of != NULL) order by the composite key (mesh, Morton):
+// positions in different meshes get deltas below 32 (the mesh id bits), and
+// inside a mesh every delta is shifted by 32 with the tie broken on the
+// mesh-local index, so each mesh's subtree is exactly its own Karras tree.
 template <bool K64>
-HX_DEV int kdelta(int i, int j, int n, const void* keys) {
+HX_DEV int kdelta(int i, int j, int n, const void* keys, const int32_t* mesh_of, const int32_t* off) {
     if (j < 0 || j >= n) return -1;
+    int ti = i, tj = j, shift = 0;
+    if (mesh_of) {
+        const int mi = mesh_of[i], mj = mesh_of[j];
+        if (mi != mj) return __clz((uint32_t)(mi ^ mj));
+        ti = i - off[mi];
+        tj = j - off[mi];
+        shift = 32;
+    }
     if (K64) {
         const uint64_t* k = static_cast<const uint64_t*>(keys);
         const uint64_t a = k[i], b = k[j];
-        return a != b ? __clzll(a ^ b) : __clzll((uint64_t)(uint32_t)(i ^ j)) + 64;
+        return shift + (a != b ? __clzll(a ^ b) : __clzll((uint64_t)(uint32_t)(ti ^ tj)) + 64);
     } else {
         const uint32_t* k = static_cast<const uint32_t*>(keys);
         const uint32_t a = k[i], b = k[j];
-        return a != b ? __clz(a ^ b) : __clz((uint32_t)(i ^ j)) + 64;
+        return shift + (a != b ? __clz(a ^ b) : __clz((uint32_t)(ti ^ tj)) + 64);
     }
 }
 
@@ -139,7 +174,9 @@ __global__ __launch_bounds__(kBlock) void k_construct(const void* __restrict__ k
                                                       int32_t* __restrict__ right, int32_t* __restrict__ parent,
                                                       int32_t* __restrict__ first, int32_t* __restrict__ last,
                                                       int32_t* __restrict__ counters, int32_t* __restrict__ end_at,
-                                                      int32_t* __restrict__ zero_word) {
+                                                      int32_t* __restrict__ zero_word,
+                                                      const int32_t* __restrict__ mesh_of,
+                                                      const int32_t* __restrict__ off) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     end_at[i] = -1;
@@ -153,22 +190,22 @@ __global__ __launch_bounds__(kBlock) void k_construct(const void* __restrict__ k
     }
     if (i >= n - 1) return;
     counters[i] = 0;
-    const int dn = kdelta<K64>(i, i + 1, n, keys);
-    const int dp = kdelta<K64>(i, i - 1, n, keys);
+    const int dn = kdelta<K64>(i, i + 1, n, keys, mesh_of, off);
+    const int dp = kdelta<K64>(i, i - 1, n, keys, mesh_of, off);
     const int d = (dn > dp) - (dn < dp);
     const int dmin = d > 0 ? dp : dn;
     int lmax = 2;
-    while (kdelta<K64>(i, i + lmax * d, n, keys) > dmin) lmax <<= 1;
+    while (kdelta<K64>(i, i + lmax * d, n, keys, mesh_of, off) > dmin) lmax <<= 1;
     int l = 0;
     for (int t = lmax >> 1; t >= 1; t >>= 1)
-        if (kdelta<K64>(i, i + (l + t) * d, n, keys) > dmin) l += t;
+        if (kdelta<K64>(i, i + (l + t) * d, n, keys, mesh_of, off) > dmin) l += t;
     const int j = i + l * d;
-    const int dnode = kdelta<K64>(i, j, n, keys);
+    const int dnode = kdelta<K64>(i, j, n, keys, mesh_of, off);
     int s = 0, k = 2, t;
     do {
         t = (l + k - 1) / k;
         k <<= 1;
-        if (kdelta<K64>(i, i + (s + t) * d, n, keys) > dnode) s += t;
+        if (kdelta<K64>(i, i + (s + t) * d, n, keys, mesh_of, off) > dnode) s += t;
     } while (t > 1);
     const int gamma = i + s * d + min(d, 0);
     int lc = gamma, rc = gamma + 1;
@@ -288,6 +325,15 @@ HX_DEV Combined combine(const ClimbArgs& a, int first, int last, float3 lmn, flo
 
 HX_DEV float3 xyz(float4 v) { return make_float3(v.x, v.y, v.z); }
 
+// A node that must stay an interior whatever its SAH state: the root (of each
+// mesh, in a batched build) and, in a batched build, the separator nodes
+// above the mesh roots (ranges spanning meshes), which are never emitted.
+HX_DEV bool seg_top(const int32_t* mesh_of, const int32_t* off, int node, int first, int last) {
+    if (!mesh_of) return node == 0;
+    const int m = mesh_of[first];
+    return mesh_of[last] != m || (first == off[m] && last == off[m + 1] - 1);
+}
+
 // Surface area in the treelet kernels' order (TRBuilderUtil.h:112-130:
 // x*y + x*z + y*z, no validity test), used by the ATR preparation.
 HX_DEV float atr_area(float3 mn, float3 mx) {
@@ -311,14 +357,32 @@ __global__ __launch_bounds__(kClimbThreads) void k_climb_local(ClimbArgs a, int3
     __shared__ int s_cnt[T];
     __shared__ float s_ac[ATR ? 2 * T : 1];
     __shared__ int s_h[ATR ? 2 * T : 1];
+    // Karras trees: the topology of the internal nodes [base, base + T) --
+    // every node this block can finish -- staged once, coalesced, so the
+    // climb's dependent reads hit LDS instead of HBM.
+    constexpr int TT = GENERAL ? 1 : T;
+    __shared__ int s_first[TT], s_last[TT], s_left[TT], s_right[TT], s_par[TT];
+    __shared__ int s_state[TT];
     const int n = (int)a.n;
     const int base = blockIdx.x * T;
     const int last_leaf = min(base + T, n) - 1;
-    for (int i = threadIdx.x; i < T; i += kClimbThreads) s_cnt[i] = 0;
+    for (int i = threadIdx.x; i < T; i += kClimbThreads) {
+        s_cnt[i] = 0;
+        if (!GENERAL && base + i < n - 1) {
+            s_first[i] = a.first[base + i];
+            s_last[i] = a.last[base + i];
+            s_left[i] = a.left[base + i];
+            s_right[i] = a.right[base + i];
+            s_par[i] = a.parent[base + i];
+        }
+    }
+    int leaf_parent[T / kClimbThreads];
 
+#pragma unroll
     for (int k = 0; k < T / kClimbThreads; ++k) {
         const int p = base + k * kClimbThreads + threadIdx.x;
         if (p > last_leaf) break;
+        leaf_parent[k] = a.parent[n - 1 + p];
         const int tri = a.sorted_tri[p];
         float3 v0, v1, v2;
         tri_vertices(a.verts, a.tris, tri, v0, v1, v2);
@@ -353,21 +417,30 @@ __global__ __launch_bounds__(kClimbThreads) void k_climb_local(ClimbArgs a, int3
     }
     __syncthreads();
 
+#pragma unroll
     for (int k = 0; k < T / kClimbThreads; ++k) {
         const int p = base + k * kClimbThreads + threadIdx.x;
         if (p > last_leaf) break;
         int child = n - 1 + p;
-        int node = a.parent[child];
+        int node = leaf_parent[k];
         while (node >= 0) {
-            const int first = a.first[node], last = a.last[node];
+            const int li = node - base;
+            // A Karras node finishing in this block has its index in the block.
+            if (!GENERAL && (unsigned)li >= (unsigned)T) {
+                frontier[atomicAdd(frontier_count, 1)] = child;
+                break;
+            }
+            const int first = GENERAL ? a.first[node] : s_first[li];
+            const int last = GENERAL ? a.last[node] : s_last[li];
             if (first < base || last > last_leaf) {
                 frontier[atomicAdd(frontier_count, 1)] = child;
                 break;
             }
-            const int sn = (GENERAL ? a.split[node] : node) - base;
+            const int sn = GENERAL ? a.split[node] - base : li;
             if (__hip_atomic_fetch_add(&s_cnt[sn], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) == 0)
                 break;
-            const int l = a.left[node], r = a.right[node];
+            const int l = GENERAL ? a.left[node] : s_left[li];
+            const int r = GENERAL ? a.right[node] : s_right[li];
             const int ls = l >= n - 1 ? T + (l - (n - 1) - base) : (GENERAL ? a.split[l] : l) - base;
             const int rs = r >= n - 1 ? T + (r - (n - 1) - base) : (GENERAL ? a.split[r] : r) - base;
             const Combined c = combine(
@@ -378,26 +451,46 @@ __global__ __launch_bounds__(kClimbThreads) void k_climb_local(ClimbArgs a, int3
             s_box[0][sn] = c.mn.x; s_box[1][sn] = c.mn.y; s_box[2][sn] = c.mn.z;
             s_box[3][sn] = c.mx.x; s_box[4][sn] = c.mx.y; s_box[5][sn] = c.mx.z;
             s_cost[sn] = c.cost;
-            a.nmin[node] = make_float4(c.mn.x, c.mn.y, c.mn.z, 0.0f);
-            a.nmax[node] = make_float4(c.mx.x, c.mx.y, c.mx.z, 0.0f);
             if (ATR) {
-                const float ar = atr_area(c.mn, c.mx);
-                const float ac = a.ct * ar + s_ac[ls] + s_ac[rs];
-                const int h = 1 + max(s_h[ls], s_h[rs]);
-                s_ac[sn] = ac;
-                s_h[sn] = h;
-                a.atr_area[node] = ar;
-                a.atr_cost[node] = ac;
-                a.atr_height[node] = h;
-                a.atr_refs[node] = last - first + 1;
+                s_ac[sn] = a.ct * atr_area(c.mn, c.mx) + s_ac[ls] + s_ac[rs];
+                s_h[sn] = 1 + max(s_h[ls], s_h[rs]);
             }
-            if (!a.refit_only) {
-                if (a.adaptive) a.cost[node] = c.cost;
-                a.state[node] = c.state;
-                if (c.state == 0 && node != 0) atomicMax(&a.end_at[first], last);
+            if (GENERAL) {
+                a.nmin[node] = make_float4(c.mn.x, c.mn.y, c.mn.z, 0.0f);
+                a.nmax[node] = make_float4(c.mx.x, c.mx.y, c.mx.z, 0.0f);
+                if (!a.refit_only) {
+                    if (a.adaptive) a.cost[node] = c.cost;
+                    a.state[node] = c.state;
+                }
+            } else {
+                s_state[li] = c.state;
             }
+            if (!a.refit_only && c.state == 0 && !seg_top(a.mesh_of, a.mesh_off, node, first, last))
+                atomicMax(&a.end_at[first], last);
             child = node;
-            node = a.parent[node];
+            node = GENERAL ? a.parent[node] : s_par[li];
+        }
+    }
+    if (GENERAL) return;
+    // Karras trees: the nodes this block finished (both children arrived,
+    // s_cnt == 2) are the block's index range; write them out coalesced.
+    __syncthreads();
+    for (int i = threadIdx.x; i < T; i += kClimbThreads) {
+        const int node = base + i;
+        if (node >= n - 1 || s_cnt[i] != 2) continue;
+        const float3 mn = make_float3(s_box[0][i], s_box[1][i], s_box[2][i]);
+        const float3 mx = make_float3(s_box[3][i], s_box[4][i], s_box[5][i]);
+        a.nmin[node] = make_float4(mn.x, mn.y, mn.z, 0.0f);
+        a.nmax[node] = make_float4(mx.x, mx.y, mx.z, 0.0f);
+        if (ATR) {
+            a.atr_area[node] = atr_area(mn, mx);
+            a.atr_cost[node] = s_ac[i];
+            a.atr_height[node] = s_h[i];
+            a.atr_refs[node] = s_last[i] - s_first[i] + 1;
+        }
+        if (!a.refit_only) {
+            if (a.adaptive) a.cost[node] = s_cost[i];
+            a.state[node] = s_state[i];
         }
     }
 }
@@ -453,7 +546,8 @@ __global__ __launch_bounds__(kBlock) void k_climb_top(ClimbArgs a, const int32_t
             if (!a.refit_only) {
                 if (a.adaptive) st_agent(a.cost + node, c.cost);
                 a.state[node] = c.state;
-                if (c.state == 0 && node != 0) atomicMax(&a.end_at[first], last);
+                if (c.state == 0 && !seg_top(a.mesh_of, a.mesh_off, node, first, last))
+                    atomicMax(&a.end_at[first], last);
             }
             // Complete this node's stores before the parent's counter can be
             // observed (the compiler fence keeps the stores above the wait).
@@ -551,6 +645,8 @@ struct FlagCtx {
     const int32_t* first;
     const int32_t* last;
     const int32_t* state;
+    const int32_t* mesh_of;  // batched builds only
+    const int32_t* off;
 };
 
 HX_DEV int leaf_start(const FlagCtx& f, int p) {
@@ -559,9 +655,16 @@ HX_DEV int leaf_start(const FlagCtx& f, int p) {
 
 HX_DEV int interior_flag(const FlagCtx& f, int i) {
     if (i >= f.n - 1) return 0;
-    if (i == 0) return 1;
+    const int fi = f.first[i], la = f.last[i];
+    if (!f.mesh_of) {
+        if (i == 0) return 1;
+    } else {
+        const int m = f.mesh_of[fi];
+        if (f.mesh_of[la] != m) return 0;                       // separator
+        if (fi == f.off[m] && la == f.off[m + 1] - 1) return 1;  // mesh root
+    }
     if (f.state[i] != 1) return 0;
-    return f.cover[f.first[i]] < f.last[i];
+    return f.cover[fi] < la;
 }
 
 __global__ __launch_bounds__(kBlock) void k_tile_flag_counts(FlagCtx f, int2* __restrict__ partial) {
@@ -632,14 +735,36 @@ HX_DEV void write_node(int4* nodes, int slot, float4 amn, float4 amx, float4 bmn
     o[3] = make_int4(c0, c1, size, parent);
 }
 
+// Per-mesh layout (one mesh unless batched): mesh m's nodes start at
+// tab[m].x; its interiors keep the global interior order with the mesh root
+// rotated to local slot 0; its leaves follow (tab[m].w interiors) in start
+// order; triangle ranges and child/parent links are mesh-local.
+struct MeshSlots {
+    int4 tab;   // node base, first interior slot, first leaf number, interiors
+    int root;   // interior slot of the mesh root
+    int toff;   // first triangle position of the mesh
+    HX_DEV int interior(int s) const { return s == root ? 0 : s - tab.y + (s < root ? 1 : 0); }
+    HX_DEV int leaf(int num) const { return tab.w + num - tab.z; }
+};
+
+HX_DEV MeshSlots mesh_slots(const EmitArgs& a, const FlagCtx& f, int first) {
+    const int m = f.mesh_of ? f.mesh_of[first] : 0;
+    MeshSlots ms;
+    ms.tab = a.mesh_tab[m];
+    ms.root = a.root_slot[m];
+    ms.toff = f.mesh_of ? f.off[m] : 0;
+    return ms;
+}
+
 __global__ __launch_bounds__(kBlock) void k_emit(EmitArgs a, FlagCtx f) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     const int n = (int)a.n;
     if (i >= n - 1) return;
     if (!interior_flag(f, i)) return;
-    const int nInt = a.counts[0];
-    const int s = a.int_slot[i];
     const int first = a.first[i], last = a.last[i];
+    const MeshSlots ms = mesh_slots(a, f, first);
+    const int s = a.int_slot[i];
+    const int ls = ms.interior(s);
     const int ch[2] = {a.left[i], a.right[i]};
     int cidx[2];
     float4 cmn[2], cmx[2];
@@ -649,16 +774,43 @@ __global__ __launch_bounds__(kBlock) void k_emit(EmitArgs a, FlagCtx f) {
         cmn[k] = a.nmin[c];
         cmx[k] = a.nmax[c];
         if (c < n - 1 && a.state[c] == 1) {
-            cidx[k] = a.int_slot[c];
+            cidx[k] = ms.interior(a.int_slot[c]);
         } else {
             const int cf = a.first[c], cl = a.last[c];
-            const int L = nInt + a.leaf_num[cf];
+            const int L = ms.leaf(a.leaf_num[cf]);
             cidx[k] = ~L;
-            write_node(a.nodes, L, cmn[k], cmx[k], cmn[k], cmx[k], cf, cl + 1, ~(cl - cf + 1), s);
+            write_node(a.nodes, ms.tab.x + L, cmn[k], cmx[k], cmn[k], cmx[k], cf - ms.toff, cl + 1 - ms.toff,
+                       ~(cl - cf + 1), ls);
         }
     }
-    const int par = (i == 0) ? -1 : a.int_slot[a.parent[i]];
-    write_node(a.nodes, s, cmn[0], cmx[0], cmn[1], cmx[1], cidx[0], cidx[1], last - first + 1, par);
+    const int par = (s == ms.root) ? -1 : ms.interior(a.int_slot[a.parent[i]]);
+    write_node(a.nodes, ms.tab.x + ls, cmn[0], cmx[0], cmn[1], cmx[1], cidx[0], cidx[1], last - first + 1, par);
+}
+
+// Mesh tables from the slot scans: the root's interior slot, then per mesh
+// the node base, first interior slot, first leaf number and interior count.
+__global__ __launch_bounds__(kBlock) void k_mesh_roots(EmitArgs a, FlagCtx f) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= f.n - 1) return;
+    const int fi = f.first[i], la = f.last[i];
+    if (!f.mesh_of) {
+        if (i == 0) a.root_slot[0] = a.int_slot[0];
+        return;
+    }
+    const int m = f.mesh_of[fi];
+    if (f.mesh_of[la] == m && fi == f.off[m] && la == f.off[m + 1] - 1) a.root_slot[m] = a.int_slot[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_mesh_tab(EmitArgs a, FlagCtx f) {
+    const int m = blockIdx.x * kBlock + threadIdx.x;
+    if (m >= a.n_meshes) return;
+    int ib = 0, lb = 0, ie = a.counts[0];
+    if (f.mesh_of) {
+        ib = a.int_slot[f.off[m]];
+        lb = a.leaf_num[f.off[m]];
+        if (m + 1 < a.n_meshes) ie = a.int_slot[f.off[m + 1]];
+    }
+    a.mesh_tab[m] = make_int4(ib + lb, ib, lb, ie - ib);
 }
 
 // n == 1: interior root whose two children are the single leaf (the
@@ -759,35 +911,35 @@ int launch_scene_box(Context* c, const float* verts, const int32_t* tris, int64_
         hipLaunchKernelGGL(k_scene_box_partial, dim3(nblocks), dim3(kBlock), 0, c->stream, verts, tris, n,
                            c->scan_partials.as<float>());
         HX_TRY(check_launch("scene_box_partial"));
-        hipLaunchKernelGGL(k_scene_box_final, dim3(1), dim3(64), 0, c->stream, c->scan_partials.as<float>(), nblocks,
+        hipLaunchKernelGGL(k_scene_box_final, dim3(1), dim3(kBlock), 0, c->stream, c->scan_partials.as<float>(), nblocks,
                            d_box6);
     }
     return check_launch("scene_box_final");
 }
 
 int launch_morton(Context* c, const float* verts, const int32_t* tris, int64_t n, const float* d_box6, int bits,
-                  void* d_keys, bool keys64) {
+                  void* d_keys, bool keys64, const int32_t* mesh_of) {
     LaunchScope ls(c, "morton");
     const int shift = 60 - bits;
     if (keys64)
         hipLaunchKernelGGL(k_morton<true>, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, verts, tris, n, d_box6, shift,
-                           d_keys);
+                           d_keys, mesh_of);
     else
         hipLaunchKernelGGL(k_morton<false>, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, verts, tris, n, d_box6,
-                           shift, d_keys);
+                           shift, d_keys, mesh_of);
     return check_launch("morton");
 }
 
 int launch_construct(Context* c, const void* d_keys, bool keys64, int64_t n, int32_t* left, int32_t* right,
                      int32_t* parent, int32_t* first, int32_t* last, int32_t* counters, int32_t* end_at,
-                     int32_t* zero_word) {
+                     int32_t* zero_word, const int32_t* mesh_of, const int32_t* off) {
     LaunchScope ls(c, "construct");
     if (keys64)
         hipLaunchKernelGGL(k_construct<true>, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, d_keys, (int)n, left, right,
-                           parent, first, last, counters, end_at, zero_word);
+                           parent, first, last, counters, end_at, zero_word, mesh_of, off);
     else
         hipLaunchKernelGGL(k_construct<false>, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, d_keys, (int)n, left,
-                           right, parent, first, last, counters, end_at, zero_word);
+                           right, parent, first, last, counters, end_at, zero_word, mesh_of, off);
     return check_launch("construct");
 }
 
@@ -818,7 +970,7 @@ int launch_collapse_emit(Context* c, const EmitArgs& a) {
     const int64_t n = a.n;
     const int64_t ntiles = (n + kScanTile - 1) / kScanTile;
     HX_TRY(c->scan_partials.reserve(sizeof(int2) * (size_t)ntiles + 64));
-    FlagCtx f{(int)n, a.cover, a.first, a.last, a.state};
+    FlagCtx f{(int)n, a.cover, a.first, a.last, a.state, a.mesh_of, a.mesh_off};
     {
         LaunchScope ls(c, "collapse_cover");
         int32_t* pm = c->scan_partials.as<int32_t>();
@@ -842,6 +994,10 @@ int launch_collapse_emit(Context* c, const EmitArgs& a) {
     }
     {
         LaunchScope ls(c, "collapse_emit");
+        hipLaunchKernelGGL(k_mesh_roots, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, a, f);
+        HX_TRY(check_launch("mesh_roots"));
+        hipLaunchKernelGGL(k_mesh_tab, dim3(grid_for(a.n_meshes)), dim3(kBlock), 0, c->stream, a, f);
+        HX_TRY(check_launch("mesh_tab"));
         hipLaunchKernelGGL(k_emit, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, a, f);
     }
     return check_launch("emit");
@@ -855,7 +1011,7 @@ int launch_refit_reset(Context* c, int32_t* counters, int64_t n_internal, int32_
 }
 
 int launch_emit_only(Context* c, const EmitArgs& a) {
-    FlagCtx f{(int)a.n, a.cover, a.first, a.last, a.state};
+    FlagCtx f{(int)a.n, a.cover, a.first, a.last, a.state, a.mesh_of, a.mesh_off};
     LaunchScope ls(c, "collapse_emit");
     hipLaunchKernelGGL(k_emit, dim3(grid_for(a.n)), dim3(kBlock), 0, c->stream, a, f);
     return check_launch("emit");

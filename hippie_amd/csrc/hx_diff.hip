@@ -64,6 +64,11 @@ HX_DEV bool solve_hit(const BackwardArgs& a, int64_t r, int id, Hit& h) {
     return true;
 }
 
+// Triangle of a hit: results hold mesh-local ids in a batched trace.
+HX_DEV int hit_triangle(const BackwardArgs& a, int64_t r, int id) {
+    return a.ray_mesh ? a.mesh_off[a.ray_mesh[r]] + id : id;
+}
+
 __global__ __launch_bounds__(kDiffBlock) void k_backward(BackwardArgs a) {
     const int64_t r = blockIdx.x * (int64_t)kDiffBlock + threadIdx.x;
     if (r >= a.n_rays) return;
@@ -74,7 +79,7 @@ __global__ __launch_bounds__(kDiffBlock) void k_backward(BackwardArgs a) {
     const double gv = a.dl_dv ? (double)a.dl_dv[r] : 0.0;
     if (gt == 0.0 && gu == 0.0 && gv == 0.0) return;
     Hit h;
-    if (!solve_hit(a, r, id, h)) return;
+    if (!solve_hit(a, r, hit_triangle(a, r, id), h)) return;
     const double w = 1.0 - h.u - h.v;
     const D3 lam{-(h.bc.x * gu + h.ca.x * gv + h.ab.x * gt) * h.inv, -(h.bc.y * gu + h.ca.y * gv + h.ab.y * gt) * h.inv,
                  -(h.bc.z * gu + h.ca.z * gv + h.ab.z * gt) * h.inv};
@@ -99,7 +104,7 @@ __global__ __launch_bounds__(kDiffBlock) void k_recompute_hits(BackwardArgs a, f
     const int4 res = a.results[r];
     float t = __int_as_float(res.y), u = 0.0f, v = 0.0f;
     Hit h;
-    if (res.x >= 0 && solve_hit(a, r, res.x, h)) {
+    if (res.x >= 0 && solve_hit(a, r, hit_triangle(a, r, res.x), h)) {
         t = (float)h.t;
         u = (float)h.u;
         v = (float)h.v;

@@ -103,11 +103,14 @@ int check_launch(const char* name);
 
 // hx_build.hip
 int launch_scene_box(Context* c, const float* verts, const int32_t* tris, int64_t n, float* d_box6);
+// mesh_of / off (batched builds, else NULL): mesh of each triangle (sorted
+// position for construct) and the M+1 triangle offsets; d_box6 then holds
+// one box per mesh.
 int launch_morton(Context* c, const float* verts, const int32_t* tris, int64_t n, const float* d_box6,
-                  int bits, void* d_keys, bool keys64);
+                  int bits, void* d_keys, bool keys64, const int32_t* mesh_of = nullptr);
 int launch_construct(Context* c, const void* d_keys, bool keys64, int64_t n, int32_t* left, int32_t* right,
                      int32_t* parent, int32_t* first, int32_t* last, int32_t* counters, int32_t* end_at,
-                     int32_t* zero_word);
+                     int32_t* zero_word, const int32_t* mesh_of = nullptr, const int32_t* off = nullptr);
 struct ClimbArgs {
     int64_t n;
     const int32_t* sorted_tri;
@@ -134,6 +137,8 @@ struct ClimbArgs {
     float* atr_cost;
     int32_t* atr_refs;
     int32_t* atr_height;
+    const int32_t* mesh_of;   // batched builds: mesh of each sorted position (else NULL)
+    const int32_t* mesh_off;  // M+1 triangle offsets
     int refit_only;           // boxes and Woop only: keep states, costs and cover
     int adaptive;
     int max_leaf_size;
@@ -155,6 +160,11 @@ struct EmitArgs {
     int32_t* int_slot;   // out
     int32_t* counts;     // out: [n_interior, n_leaves]
     int4* nodes;         // out: Bin nodes
+    const int32_t* mesh_of;   // batched builds (else NULL)
+    const int32_t* mesh_off;
+    int n_meshes;             // 1 unless batched
+    int4* mesh_tab;           // out: per mesh {node base, first interior slot, first leaf number, interiors}
+    int32_t* root_slot;       // out: per mesh interior slot of the root
 };
 int launch_collapse_emit(Context* c, const EmitArgs& a);
 // Refit: re-zero the climb counters and frontier count, then (after the
@@ -172,6 +182,8 @@ struct BackwardArgs {
     const float* dl_du;   // per ray, may be NULL
     const float* dl_dv;   // per ray, may be NULL
     float* dl_dverts;     // float[3V], accumulated
+    const int32_t* ray_mesh;  // batched: mesh of each ray (else NULL)
+    const int32_t* mesh_off;  // batched: M+1 triangle offsets
 };
 int launch_backward(Context* c, const BackwardArgs& a);
 int launch_recompute_hits(Context* c, const BackwardArgs& a, float* t_out, float* u_out, float* v_out);
@@ -236,6 +248,16 @@ constexpr int kAtrMaxLevels = 1024;
 // relabel: fields of a PLOC argument block used by launch_tree_down.
 int launch_atr(Context* c, const AtrArgs& a, const PlocArgs& relabel, int64_t* accepted_out);
 
+// hx_batch.hip: batched-build helpers (mesh of each triangle, per-mesh
+// boxes, key gathers, mesh-local triangle ids).
+int launch_mesh_of(Context* c, const int32_t* off, int M, int64_t n, int32_t* mesh_of);
+int launch_mesh_boxes(Context* c, const float* verts, const int32_t* tris, int64_t n, const int32_t* mesh_of, int M,
+                      uint32_t* bits, float* boxes);
+int launch_gather_mesh_keys(Context* c, const int32_t* vals, const int32_t* mesh_of, int64_t n, uint32_t* keys);
+int launch_gather_keys(Context* c, const int32_t* vals, const uint64_t* src, int64_t n, uint64_t* dst);
+int launch_tri_local(Context* c, const int32_t* sorted_tri, const int32_t* mesh_of, const int32_t* off, int64_t n,
+                     int32_t* tri_local);
+
 // hx_sort.hip
 size_t sort_status_bytes(int64_t n, int end_bit, bool keys64);
 // Stable LSD sort of (key, value) over bits [0, end_bit).  keys/vals ping-pong
@@ -257,6 +279,11 @@ struct TraceArgs {
     const int32_t* order;   // optional: ray slot -> ray index (sorted trace)
     int32_t* control;       // [0] ray counter, [1] overflow counter
     int fetch_threshold;    // dynamic fetch: busy lanes below which a wave refills
+    // Batched trace (else NULL): each ray traverses its own mesh's BVH.
+    const int32_t* ray_mesh;
+    const int4* mesh_tab;   // per mesh {node base, ...}
+    const int32_t* mesh_off;
+    int n_meshes;
 };
 int launch_trace(Context* c, const TraceArgs& a);
 int launch_ray_keys(Context* c, const float4* rays, int64_t n, int method, int bits, float ray_length,

@@ -60,6 +60,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(TraceArgs a) {
     int32_t nodeAddr = kSentinel, leafAddr = 0, hitIndex = -1, hitId = -1;
     int64_t rayidx = 0;
     int32_t nNodes = 0, nTris = 0;
+    // Batched trace: this lane's mesh (node base, first triangle position).
+    int64_t nodeBase = 0, triBase = 0;
     int32_t* rayCounter = a.control;
 
     while (true) {
@@ -76,6 +78,17 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(TraceArgs a) {
             rayidx = (int64_t)base + below;
             if (rayidx >= a.n_rays) break;
             if (ORDERED) rayidx = a.order[rayidx];
+            if (a.ray_mesh) {
+                const int m = a.ray_mesh[rayidx];
+                if ((unsigned)m >= (unsigned)a.n_meshes) {  // no such mesh: a miss
+                    reinterpret_cast<int2*>(a.results)[2 * rayidx] =
+                        make_int2(-1, __float_as_int(a.rays[2 * rayidx + 1].w));
+                    if (STATS) a.stats[rayidx] = make_int2(0, 0);
+                    continue;
+                }
+                nodeBase = a.mesh_tab[m].x;
+                triBase = a.mesh_off[m];
+            }
             const float4 o = a.rays[2 * rayidx + 0];
             const float4 d = a.rays[2 * rayidx + 1];
             ox = o.x; oy = o.y; oz = o.z; tmin = o.w;
@@ -98,11 +111,11 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(TraceArgs a) {
 
         while (nodeAddr != kSentinel) {
             while ((uint32_t)nodeAddr < (uint32_t)kSentinel) {
-                const float4* np = reinterpret_cast<const float4*>(a.nodes + 4 * (int64_t)nodeAddr);
+                const float4* np = reinterpret_cast<const float4*>(a.nodes + 4 * (nodeBase + nodeAddr));
                 const float4 n0xy = np[0];
                 const float4 n1xy = np[1];
                 const float4 nz = np[2];
-                const int4 cn = a.nodes[4 * (int64_t)nodeAddr + 3];
+                const int4 cn = a.nodes[4 * (nodeBase + nodeAddr) + 3];
                 if (STATS) ++nNodes;
                 const float c0lox = fmaf(n0xy.x, idx, -oodx);
                 const float c0hix = fmaf(n0xy.y, idx, -oodx);
@@ -151,25 +164,25 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(TraceArgs a) {
             }
 
             while (leafAddr < 0) {
-                const int4 lb = a.nodes[4 * (int64_t)(~leafAddr) + 3];
+                const int4 lb = a.nodes[4 * (nodeBase + ~leafAddr) + 3];
                 for (int tri = lb.x; tri < lb.y; ++tri) {
                     if (STATS) ++nTris;
-                    const float4 v00 = a.woop[3 * (int64_t)tri + 0];
+                    const float4 v00 = a.woop[3 * (triBase + tri) + 0];
                     const float Oz = fmaf(-oz, v00.z, fmaf(-oy, v00.y, fmaf(-ox, v00.x, v00.w)));
                     const float den = fmaf(dz, v00.z, fmaf(dy, v00.y, dx * v00.x));
                     const float t = Oz * (1.0f / den);
                     if (t > tmin && t <= hitT) {
-                        const float4 v11 = a.woop[3 * (int64_t)tri + 1];
+                        const float4 v11 = a.woop[3 * (triBase + tri) + 1];
                         const float Ox = fmaf(oz, v11.z, fmaf(oy, v11.y, fmaf(ox, v11.x, v11.w)));
                         const float Dx = fmaf(dz, v11.z, fmaf(dy, v11.y, dx * v11.x));
                         const float u = fmaf(t, Dx, Ox);
                         if (u >= 0.0f) {
-                            const float4 v22 = a.woop[3 * (int64_t)tri + 2];
+                            const float4 v22 = a.woop[3 * (triBase + tri) + 2];
                             const float Oy = fmaf(oz, v22.z, fmaf(oy, v22.y, fmaf(ox, v22.x, v22.w)));
                             const float Dy = fmaf(dz, v22.z, fmaf(dy, v22.y, dx * v22.x));
                             const float v = fmaf(t, Dy, Oy);
                             if (v >= 0.0f && u + v <= 1.0f) {
-                                const int32_t id = a.tri_idx[tri];
+                                const int32_t id = a.tri_idx[triBase + tri];
                                 // Closest hit; equal t resolves to the smaller triangle id.
                                 if (t < hitT || id < hitId) {
                                     hitT = t;

@@ -254,3 +254,66 @@ def random_rays(n, seed=5, box=(0.0, 1.0), tmax=3.0):
     rays[:, 4:7] = d
     rays[:, 7] = tmax
     return rays
+
+
+def box_rays(bmin, bmax, count, rng, tmax_scale=3.0):
+    """Config 3 rays for one mesh: origins on a sphere around its box, aimed at
+    uniform points inside it."""
+    bmin = np.asarray(bmin, np.float64)
+    bmax = np.asarray(bmax, np.float64)
+    c = 0.5 * (bmin + bmax)
+    r = 0.75 * np.linalg.norm(bmax - bmin) + 1e-3
+    u = rng.standard_normal((count, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    o = c + r * u
+    tgt = bmin + rng.random((count, 3)) * (bmax - bmin)
+    d = tgt - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.empty((count, 8), np.float32)
+    rays[:, 0:3] = o
+    rays[:, 3] = 0.0
+    rays[:, 4:7] = d
+    rays[:, 7] = tmax_scale * r
+    return rays
+
+
+def batch_mesh(count, rng, sphere):
+    """One config 3 mesh of `count` triangles in the unit box: a displaced
+    icosphere (largest level that fits, topped up with soup) or a soup."""
+    if sphere and count >= 20:
+        level = 0
+        while 20 * 4 ** (level + 1) <= count and level < 7:
+            level += 1
+        v, f = icosphere(level)
+        k = rng.integers(1, 6, 3)
+        ph = rng.random(3) * 6.283
+        r = 1.0 + 0.05 * (np.sin(k[0] * v[:, 0] + ph[0]) * np.sin(k[1] * v[:, 1] + ph[1]) *
+                          np.sin(k[2] * v[:, 2] + ph[2]))
+        v = 0.5 + 0.45 * v * r[:, None]
+        extra = count - f.shape[0]
+        if extra > 0:
+            sv, sf = soup_triangles(rng.random((extra, 3), dtype=np.float32), rng)
+            f = np.concatenate([f, sf + v.shape[0]])
+            v = np.concatenate([v, sv])
+        return v.astype(np.float32), f.astype(np.int32)
+    return soup_triangles(rng.random((count, 3), dtype=np.float32), rng)
+
+
+def batch_meshes(n_meshes, seed=3, lo=64, hi=262144):
+    """Config 3 geometry: counts round(exp(U[ln lo, ln hi])), half icospheres."""
+    rng = np.random.default_rng(seed)
+    counts = np.rint(np.exp(rng.uniform(np.log(lo), np.log(hi), n_meshes))).astype(np.int64)
+    return [batch_mesh(int(c), rng, bool(rng.random() < 0.5)) for c in counts]
+
+
+def batch_rays(meshes, per_1k=256, seed=4):
+    """Rays for a list of meshes: per_1k per 1000 triangles (at least 1) per mesh;
+    returns rays (R, 8) and the mesh of each ray (R,)."""
+    rng = np.random.default_rng(seed)
+    rays, owner = [], []
+    for m, (v, f) in enumerate(meshes):
+        cnt = max(1, int(round(per_1k * f.shape[0] / 1000.0)))
+        used = v[f.reshape(-1)]
+        rays.append(box_rays(used.min(0), used.max(0), cnt, rng))
+        owner.append(np.full(cnt, m, np.int32))
+    return np.concatenate(rays), np.concatenate(owner)

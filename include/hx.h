@@ -217,6 +217,45 @@ int hx_memcpy(hx_context* ctx, void* dst, const void* src, int64_t bytes);
 /* Number of rays whose traversal stack overflowed in the last trace call. */
 int hx_trace_overflow_count(hx_context* ctx, int64_t* out);
 
+/* ---- batched meshes (no reference counterpart) -----------------------------
+ * Many independent meshes built, refit, traced and differentiated as one:
+ * the callers are workloads with thousands of small scenes (per-object BVHs,
+ * differentiable rendering of many meshes), which the reference serves one
+ * HipBVH at a time.  Triangles of mesh m are [off[m], off[m+1]) of
+ * d_triangles (vertex indices into the shared d_vertices); each mesh gets the
+ * BVH a separate LBVH build of it would give (same Bin tree, Woop rows and
+ * mesh-local triangle ids).  Every mesh needs >= 2 triangles. */
+typedef struct hx_batch hx_batch;
+typedef struct hx_batch_mesh_view {
+    const int32_t* nodes;        /* device: this mesh's Bin nodes, 16 int32 each  */
+    const float* woop;           /* device: 12 floats per reference              */
+    const int32_t* tri_indices;  /* device: mesh-local triangle ids              */
+    int64_t n_interior;
+    int64_t n_leaves;
+    int64_t n_references;
+    int64_t tri_offset;          /* off[m]                                       */
+} hx_batch_mesh_view;
+int hx_build_batch(hx_context* ctx, const hx_build_params* params, const float* d_vertices, int64_t n_vertices,
+                   const int32_t* d_triangles, const int64_t* h_mesh_offsets, int32_t n_meshes, hx_batch** out);
+/* Rebuild every mesh from new vertices/triangles (same offsets), reusing the
+ * batch's memory. */
+int hx_batch_rebuild(hx_context* ctx, hx_batch* batch, const float* d_vertices, const int32_t* d_triangles);
+int hx_batch_refit(hx_context* ctx, hx_batch* batch, const float* d_vertices, const int32_t* d_triangles);
+int hx_batch_destroy(hx_batch* batch);
+/* Synchronises on first call after a build to read the mesh tables. */
+int hx_batch_mesh(hx_batch* batch, int32_t mesh, hx_batch_mesh_view* out);
+/* Rays as hx_trace; d_ray_mesh[r] names the mesh ray r traverses (a ray with
+ * no such mesh misses).  Hit ids are mesh-local. */
+int hx_batch_trace(hx_context* ctx, hx_batch* batch, const float* d_rays, const int32_t* d_ray_mesh,
+                   int64_t n_rays, int32_t any_hit, int32_t* d_results, int32_t* d_stats);
+/* hx_backward / hx_recompute_hits for a batched trace's results. */
+int hx_batch_backward(hx_context* ctx, hx_batch* batch, const float* d_vertices, const int32_t* d_triangles,
+                      const float* d_rays, const int32_t* d_ray_mesh, const int32_t* d_results, int64_t n_rays,
+                      const float* d_dl_dt, const float* d_dl_du, const float* d_dl_dv, float* d_dl_dvertices);
+int hx_batch_recompute_hits(hx_context* ctx, hx_batch* batch, const float* d_vertices, const int32_t* d_triangles,
+                            const float* d_rays, const int32_t* d_ray_mesh, const int32_t* d_results, int64_t n_rays,
+                            float* d_t, float* d_u, float* d_v);
+
 #ifdef __cplusplus
 }
 #endif

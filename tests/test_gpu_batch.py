@@ -1,0 +1,154 @@
+"""Batched meshes (hx_build_batch and friends) against the looped per-mesh path.
+
+Each mesh of a batch must get exactly the BVH a separate build gives: the
+oracle builds every mesh alone, and the batched Bin tree (canonical form), Woop
+rows and mesh-local triangle ids must match bit for bit.  The batched trace,
+refit and adjoint are checked per mesh against the oracle trace, the oracle
+refit and the single-mesh adjoint on the GPU.
+"""
+import numpy as np
+import pytest
+import torch
+
+import hippie_amd as hx
+from hippie_amd import scenes
+from hippie_amd.diff import backward as single_backward
+
+pytestmark = pytest.mark.gpu
+
+
+def _meshes(n, seed, lo=2, hi=6000):
+    rng = np.random.default_rng(seed)
+    counts = np.rint(np.exp(rng.uniform(np.log(lo), np.log(hi), n))).astype(int)
+    return [scenes.batch_mesh(int(c), rng, bool(rng.random() < 0.5)) for c in counts]
+
+
+def _check_meshes(oracle, hb, meshes, bits):
+    for m, (v, t) in enumerate(meshes):
+        got = hb.mesh_host(m)
+        ref = oracle.build_lbvh(v, t, bits=bits)
+        assert (got["n_interior"], got["n_leaves"]) == (ref["n_interior"], ref["n_leaves"]), f"mesh {m} counts"
+        assert oracle.validate_bin(got["nodes"], got["n_interior"], got["n_leaves"], got["tri_idx"], v, t) == 0
+        cg = oracle.bin_canonical(got["nodes"], got["n_interior"], got["n_leaves"], got["tri_idx"])
+        co = oracle.bin_canonical(ref["nodes"], ref["n_interior"], ref["n_leaves"], ref["tri_idx"])
+        assert np.array_equal(cg, co), f"mesh {m} canonical tree"
+        assert np.array_equal(got["woop"].view(np.uint32), oracle.woopify(v, t, got["tri_idx"]).view(np.uint32))
+
+
+@pytest.mark.parametrize("bits", [30, 60])
+def test_batch_build_matches_each_mesh_alone(oracle, ctx, bits):
+    meshes = _meshes(40, seed=bits)
+    hb = hx.HipBatch(ctx, hx.MeshBatch.from_meshes(meshes), morton_bits=bits)
+    _check_meshes(oracle, hb, meshes, bits)
+
+
+def test_batch_single_mesh_and_duplicates(oracle, ctx):
+    """One mesh through the batch API, and meshes whose triangles coincide."""
+    v, t = scenes.uniform_soup(5000, seed=3)
+    hb = hx.HipBatch(ctx, hx.MeshBatch.from_meshes([(v, t)]))
+    _check_meshes(oracle, hb, [(v, t)], 60)
+    tri = np.array([[0, 1, 2]] * 7, np.int32)
+    verts = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+    meshes = [(verts, tri), (v, t), (verts, tri[:2])]
+    hb = hx.HipBatch(ctx, hx.MeshBatch.from_meshes(meshes))
+    _check_meshes(oracle, hb, meshes, 60)
+
+
+def test_batch_trace_refit_backward(oracle, ctx):
+    meshes = _meshes(24, seed=7, lo=16, hi=4000)
+    mb = hx.MeshBatch.from_meshes(meshes)
+    hb = hx.HipBatch(ctx, mb)
+    rays, owner = scenes.batch_rays(meshes, per_1k=2000, seed=5)
+    d_rays = torch.as_tensor(rays, device="cuda")
+    d_owner = torch.as_tensor(owner, device="cuda")
+    res = hb.trace(d_rays, d_owner).cpu().numpy()
+    for m in range(len(meshes)):
+        sel = owner == m
+        g = hb.mesh_host(m)
+        exp, ov = oracle.trace(g["nodes"], g["woop"], g["tri_idx"], rays[sel])
+        assert ov == 0
+        assert np.array_equal(res[sel, 0], exp[:, 0]) and np.array_equal(res[sel, 1], exp[:, 1]), f"mesh {m}"
+    # Refit to perturbed vertices: every mesh as the oracle refit of its nodes.
+    before = [hb.mesh_host(m) for m in range(len(meshes))]
+    rng = np.random.default_rng(9)
+    v_all = mb.scene.vertices.cpu().numpy()
+    v2 = (v_all + rng.normal(0.0, 0.001, v_all.shape)).astype(np.float32)
+    hb.refit(v2)
+    base = 0
+    for m, (v, t) in enumerate(meshes):
+        vm = v2[base:base + len(v)]
+        base += len(v)
+        g = hb.mesh_host(m)
+        exp = oracle.refit_final(before[m]["nodes"], g["n_interior"], g["n_leaves"], g["tri_idx"], vm, t)
+        assert np.array_equal(g["nodes"], exp), f"mesh {m} refit"
+    # Adjoint: batched scatter equals the single-mesh adjoint mesh by mesh.
+    res_t = hb.trace(d_rays, d_owner)
+    dl_dt = torch.rand(len(rays), device="cuda", dtype=torch.float32)
+    grad = hb.backward(d_rays, d_owner, res_t, dl_dt=dl_dt).cpu().numpy()
+    base = 0
+    for m, (v, t) in enumerate(meshes):
+        sel = torch.as_tensor(owner == m, device="cuda")
+        vm = torch.as_tensor(v2[base:base + len(v)], device="cuda")
+        tm = torch.as_tensor(t, device="cuda")
+        gm = single_backward(ctx, vm, tm, d_rays[sel].contiguous(), res_t[sel].contiguous(),
+                             dl_dt=dl_dt[sel].contiguous()).cpu().numpy()
+        np.testing.assert_allclose(grad[base:base + len(v)], gm, rtol=1e-5, atol=1e-6)
+        base += len(v)
+
+
+def test_batch_rejects_tiny_meshes(ctx):
+    v, t = scenes.uniform_soup(10, seed=1)
+    with pytest.raises(hx._lib.HxError):
+        hx.HipBatch(ctx, hx.MeshBatch.from_meshes([(v, t), (v, t[:1])]))
+
+
+def test_batch_diff_step_graph_replays_match_eager(ctx):
+    """refit -> trace -> loss -> adjoint captured in one graph: every replay
+    equals the eager step (the capture holds no memset node: all clears are
+    kernels)."""
+    meshes = _meshes(16, seed=11, lo=32, hi=3000)
+    mb = hx.MeshBatch.from_meshes(meshes)
+    hb = hx.HipBatch(ctx, mb)
+    rays, owner = scenes.batch_rays(meshes, per_1k=1000, seed=6)
+    d_rays = torch.as_tensor(rays, device="cuda")
+    d_owner = torch.as_tensor(owner, device="cuda")
+    res = hb.trace(d_rays, d_owner)
+    t_ref = res[:, 1].view(torch.float32).clone() * 0.999
+    grad = torch.zeros_like(mb.scene.vertices)
+    dl_dt = torch.zeros(len(rays), dtype=torch.float32, device="cuda")
+
+    def step():
+        hb.refit()
+        hb.trace(d_rays, d_owner, results=res)
+        t = res[:, 1].view(torch.float32)
+        torch.where(res[:, 0] >= 0, 2.0 * (t - t_ref), torch.zeros_like(t), out=dl_dt)
+        grad.zero_()
+        hb.backward(d_rays, d_owner, res, dl_dt=dl_dt, out=grad)
+
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        ctx.set_stream(side.cuda_stream)
+        step()
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    try:
+        with torch.cuda.graph(g):
+            ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+            step()
+    finally:
+        ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    rng = np.random.default_rng(2)
+    for k in range(3):
+        v2 = mb.scene.vertices + torch.as_tensor(rng.normal(0.0, 1e-4, tuple(mb.scene.vertices.shape)),
+                                                 dtype=torch.float32, device="cuda")
+        mb.scene.vertices.copy_(v2)
+        step()                                   # eager
+        torch.cuda.synchronize()
+        res_e, grad_e = res.clone(), grad.clone()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(res, res_e), f"replay {k}: results"
+        torch.testing.assert_close(grad, grad_e, rtol=1e-5, atol=1e-6)
