@@ -53,6 +53,7 @@ def parse():
     ap.add_argument("--batch-hi", type=int, default=262144)
     ap.add_argument("--batch-rays-per-1k", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--no-autotune", action="store_true", help="keep libhx's default launch shapes")
     ap.add_argument("--tune", action="append", default=[], metavar="NAME=VALUE",
                     help="libhx launch tunable (hx_context_set_tuning), e.g. trace_lds_stack=16")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -257,19 +258,19 @@ def bench_batch(ctx, args):
         s = torch.cuda.Stream()
         s.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(s):
-            ctx.set_stream(s.cuda_stream)
+            ctx.set_stream(s)
             batched_step()
         torch.cuda.current_stream().wait_stream(s)
         torch.cuda.synchronize()
         graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(graph):
-            ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+            ctx.set_stream(torch.cuda.current_stream())
             batched_step()
-        ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+        ctx.set_stream(torch.cuda.current_stream())
         graph_ms = timed(graph.replay, steps, 1)
     except Exception as e:  # capture is optional; the eager step is the measured fallback
         log(f"graph capture failed: {e}")
-        ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+        ctx.set_stream(torch.cuda.current_stream())
 
     roffs = w["roffs"]
     res_m = [res[int(roffs[m]):int(roffs[m + 1])] for m in range(M)]
@@ -347,6 +348,13 @@ def main():
     sec = scenes.secondary_rays(verts, tris, prim, res, seed=3)
     sec_rb = hx.RayBuffer(sec)
     n_rays = prim.shape[0] + sec.shape[0]
+    tuning = None
+    if not args.no_autotune:
+        from hippie_amd.autotune import Autotuner
+        at = Autotuner(ctx, repeats=2, log=log)
+        tuning = dict(at.tune_sort(n, bits=60))
+        tuning.update(at.tune_trace(bvh, sec_rb, sort_method="aila"))
+        log(f"autotuned: {tuning}")
     torch.cuda.synchronize()
     log(f"setup {time.perf_counter() - t_setup:.1f} s")
 
@@ -384,10 +392,12 @@ def main():
 
     # Roofline of the dominant build kernel: one onesweep radix pass moves
     # 24 B per key (8 B key + 4 B value read and written); the first pass
-    # generates the values (20 B).  Averaged over the 8 passes of a 60-bit sort
-    # of the n triangle keys plus the 32M-ray key sorts in the same step.
+    # generates the values (20 B).  Averaged over the passes of a 60-bit sort
+    # (8 at 8-bit digits, 6 at 10-bit) of the n triangle keys plus the 32M-ray
+    # key sorts in the same step.
     sort_cnt, sort_ms = prof.get("sort_onesweep", (0, 0.0))
-    passes = 8
+    digit_bits = ctx.get_tuning("sort_digit_bits")
+    passes = (60 + digit_bits - 1) // digit_bits
     bytes_build = n * (20 + 24 * (passes - 1))
     bytes_rays = prim.shape[0] * (20 + 24 * (passes - 1)) + sec.shape[0] * (20 + 24 * (passes - 1))
     sort_bytes_per_step = bytes_build + bytes_rays
@@ -445,6 +455,7 @@ def main():
             "avg_launch_ms": round(avg_launch_ms, 5),
         },
         "cpu_baseline": cpu,
+        "tuning": tuning,
         "builders": builders,
         "batched": batched,
     }

@@ -335,7 +335,7 @@ int run_front(hx_bvh* b) {
     b->box_known = false;
     HX_TRY(hx::launch_morton(c, b->verts, b->tris, n, box, p.morton_bits, b->keys[0].ptr, b->keys64));
     // The sorted triangle ids must end in tri_idx: pass p reads vals[p&1].
-    const int passes = (p.morton_bits + 7) / 8;
+    const int passes = hx::sort_pass_count(c, p.morton_bits);
     int32_t* vals[2];
     vals[passes & 1] = b->tri_idx.as<int32_t>();
     vals[(passes & 1) ^ 1] = b->vals_tmp.as<int32_t>();
@@ -708,6 +708,12 @@ int hx_context_set_tuning(hx_context* ctx, const char* name, int32_t value) {
     } else if (!std::strcmp(name, "trace_fetch")) {
         HX_CHECK_ARG(value >= 1 && value <= 65, "trace_fetch must be in [1, 65]");
         t.trace_fetch = value;
+    } else if (!std::strcmp(name, "trace_blocks_per_cu")) {
+        HX_CHECK_ARG(value >= 0 && value <= 64, "trace_blocks_per_cu must be in [0, 64]");
+        t.trace_blocks_per_cu = value;
+    } else if (!std::strcmp(name, "sort_digit_bits")) {
+        HX_CHECK_ARG(value == 8 || value == 10, "sort_digit_bits must be 8 or 10");
+        t.sort_digit_bits = value;
     } else {
         hx::set_error("unknown tuning parameter '%s'", name);
         return HX_ERR_INVALID_ARG;
@@ -720,6 +726,8 @@ int hx_context_get_tuning(hx_context* ctx, const char* name, int32_t* value) {
     const hx::Tuning& t = ctx->c.tune;
     if (!std::strcmp(name, "trace_lds_stack")) *value = t.trace_lds_stack;
     else if (!std::strcmp(name, "trace_fetch")) *value = t.trace_fetch;
+    else if (!std::strcmp(name, "trace_blocks_per_cu")) *value = t.trace_blocks_per_cu;
+    else if (!std::strcmp(name, "sort_digit_bits")) *value = t.sort_digit_bits;
     else {
         hx::set_error("unknown tuning parameter '%s'", name);
         return HX_ERR_INVALID_ARG;

@@ -66,8 +66,10 @@ struct Profiler {
 
 // Launch-shape choices the autotuner may change (hx_context_set_tuning).
 struct Tuning {
-    int trace_lds_stack = 16;   // traversal-stack entries kept in LDS: 0, 16, 24, 32, 64
-    int trace_fetch = 40;       // refill a wave when fewer lanes than this are still busy
+    int trace_lds_stack = 16;    // traversal-stack entries kept in LDS: 0, 16, 24, 32, 64
+    int trace_fetch = 40;        // refill a wave when fewer lanes than this are still busy
+    int trace_blocks_per_cu = 0; // persistent trace workgroups per CU (0 = occupancy limit)
+    int sort_digit_bits = 8;     // radix digit width: 8 or 10
 };
 
 struct Context {
@@ -259,7 +261,9 @@ int launch_tri_local(Context* c, const int32_t* sorted_tri, const int32_t* mesh_
                      int32_t* tri_local);
 
 // hx_sort.hip
-size_t sort_status_bytes(int64_t n, int end_bit, bool keys64);
+// Number of digit passes a sort over bits [0, end_bit) takes (digit width is
+// the "sort_digit_bits" tunable); the result lands in buffer (passes & 1).
+int sort_pass_count(const Context* c, int end_bit);
 // Stable LSD sort of (key, value) over bits [0, end_bit).  keys/vals ping-pong
 // between [0] and [1]; returns in *result_index which buffer holds the output.
 // vals_identity: the input values are 0..n-1 and are not read.

@@ -1,8 +1,10 @@
 // hx_sort.hip -- stable LSD radix sort of (key, int32 value) pairs for gfx950.
 //
 // Replaces src/radix_sort/ (RadixSort.cpp:181-238: 5-bit digits, three
-// launches and a host sync per pass).  Here: 8-bit digits, one histogram pass
-// for all digit positions, then one single-pass "onesweep" kernel per digit:
+// launches and a host sync per pass).  Here: RB-bit digits (8 or 10, chosen by
+// the "sort_digit_bits" tunable: 10-bit digits sort 60-bit keys in 6 passes
+// instead of 8), one histogram pass for all digit positions, then one
+// single-pass "onesweep" kernel per digit:
 //   - tiles of 4096 keys, 4 waves of 64 lanes, 16 keys per lane in
 //     wave-striped order (item i of lane l = i*64 + l within the wave slice);
 //   - ranking by 64-bit ballot match: lanes with the same digit form a peer
@@ -25,14 +27,13 @@ constexpr int kSortBlock = 256;
 constexpr int kSortWaves = kSortBlock / kWave;
 constexpr int kSortItems = 16;
 constexpr int kSortTile = kSortBlock * kSortItems;
-constexpr int kRadix = 256;
-constexpr int kMaxPasses = 8;
+constexpr int kMaxPasses = 8;    // 64-bit keys at 8-bit digits; 7 at 10-bit
 
 constexpr uint32_t kFlagAggregate = 1u << 30;
 constexpr uint32_t kFlagPrefix = 2u << 30;
 constexpr uint32_t kValueMask = (1u << 30) - 1u;
 
-inline int sort_passes(int end_bit) { return (end_bit + 7) / 8; }
+inline int sort_passes(int end_bit, int rb) { return (end_bit + rb - 1) / rb; }
 inline int64_t sort_tiles(int64_t n) { return (n + kSortTile - 1) / kSortTile; }
 
 __global__ void k_zero_u32(uint32_t* p, int64_t n) {
@@ -51,48 +52,65 @@ HX_DEV uint32_t digit_of(K key, int shift, uint32_t mask) {
 }
 
 // Histograms of every digit position in one read of the keys.
-template <class K>
+template <class K, int RB>
 __global__ __launch_bounds__(kSortBlock) void k_sort_hist(const K* __restrict__ keys, int64_t n, int passes,
                                                           int end_bit, uint32_t* __restrict__ hist) {
-    __shared__ uint32_t h[kMaxPasses][kRadix];
-    for (int i = threadIdx.x; i < kMaxPasses * kRadix; i += kSortBlock) (&h[0][0])[i] = 0u;
+    constexpr int R = 1 << RB;
+    constexpr int P = (8 * (int)sizeof(K) + RB - 1) / RB;
+    __shared__ uint32_t h[P][R];
+    for (int i = threadIdx.x; i < P * R; i += kSortBlock) (&h[0][0])[i] = 0u;
     __syncthreads();
     for (int64_t i = blockIdx.x * (int64_t)kSortBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kSortBlock) {
         const K key = keys[i];
         for (int p = 0; p < passes; ++p) {
-            const int shift = 8 * p;
-            const int nb = min(8, end_bit - shift);
+            const int shift = RB * p;
+            const int nb = min(RB, end_bit - shift);
             atomicAdd(&h[p][digit_of(key, shift, (1u << nb) - 1u)], 1u);
         }
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < passes * kRadix; i += kSortBlock) {
+    for (int i = threadIdx.x; i < passes * R; i += kSortBlock) {
         const uint32_t c = (&h[0][0])[i];
         if (c) atomicAdd(&hist[i], c);
     }
 }
 
-// Exclusive scan of each pass's 256 digit counts, in place.
-__global__ __launch_bounds__(kRadix) void k_sort_scan_hist(uint32_t* __restrict__ hist) {
-    __shared__ uint32_t wt[kRadix / kWave];
-    uint32_t* h = hist + blockIdx.x * kRadix;
-    const uint32_t v = h[threadIdx.x];
+// Exclusive scan of each pass's R digit counts, in place; each thread owns
+// R / 256 consecutive digits.
+template <int RB>
+__global__ __launch_bounds__(kSortBlock) void k_sort_scan_hist(uint32_t* __restrict__ hist) {
+    constexpr int R = 1 << RB;
+    constexpr int DPT = R / kSortBlock;
+    __shared__ uint32_t wt[kSortWaves];
+    uint32_t* h = hist + blockIdx.x * R + threadIdx.x * DPT;
+    uint32_t v[DPT], sum = 0;
+#pragma unroll
+    for (int k = 0; k < DPT; ++k) {
+        v[k] = h[k];
+        sum += v[k];
+    }
     uint32_t tot;
-    const uint32_t inc = block_inclusive_scan<kRadix / kWave>(v, [](uint32_t a, uint32_t b) { return a + b; }, wt, tot);
-    h[threadIdx.x] = inc - v;
+    uint32_t run = block_inclusive_scan<kSortWaves>(sum, [](uint32_t a, uint32_t b) { return a + b; }, wt, tot) - sum;
+#pragma unroll
+    for (int k = 0; k < DPT; ++k) {
+        h[k] = run;
+        run += v[k];
+    }
 }
 
-template <class K, bool IDENT>
+template <class K, bool IDENT, int RB>
 __global__ __launch_bounds__(kSortBlock) void k_onesweep(const K* __restrict__ keys_in, K* __restrict__ keys_out,
                                                          const int32_t* __restrict__ vals_in,
                                                          int32_t* __restrict__ vals_out, int64_t n, int shift,
                                                          int nbits, const uint32_t* __restrict__ digit_offsets,
                                                          uint32_t* __restrict__ status,
                                                          uint32_t* __restrict__ tile_counter) {
+    constexpr int R = 1 << RB;
+    constexpr int DPT = R / kSortBlock;  // digits per thread in the per-digit phase
     __shared__ uint32_t s_tile;
-    __shared__ uint32_t s_wave_hist[kSortWaves][kRadix];
-    __shared__ uint32_t s_digit_start[kRadix];
-    __shared__ uint32_t s_global_base[kRadix];
+    __shared__ uint32_t s_wave_hist[kSortWaves][R];
+    __shared__ uint32_t s_digit_start[R];
+    __shared__ uint32_t s_global_base[R];
     __shared__ uint32_t s_scan[kSortWaves];
     __shared__ union {
         K keys[kSortTile];
@@ -105,7 +123,7 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const K* __restrict__ k
     const uint32_t mask = (1u << nbits) - 1u;
 
     if (tid == 0) s_tile = atomicAdd(tile_counter, 1u);
-    for (int i = tid; i < kSortWaves * kRadix; i += kSortBlock) (&s_wave_hist[0][0])[i] = 0u;
+    for (int i = tid; i < kSortWaves * R; i += kSortBlock) (&s_wave_hist[0][0])[i] = 0u;
     __syncthreads();
     const int64_t tile = s_tile;
     const int64_t tile_base = tile * kSortTile;
@@ -140,42 +158,59 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const K* __restrict__ k
     }
     __syncthreads();
 
-    // Per digit: exclusive offsets across waves, tile total, tile-local start.
-    const int d = tid;  // kSortBlock == kRadix
-    uint32_t total = 0;
+    // Per digit (thread tid owns digits tid*DPT .. tid*DPT+DPT-1): exclusive
+    // offsets across waves, tile totals, tile-local starts.
+    uint32_t total[DPT], tsum = 0;
 #pragma unroll
-    for (int ww = 0; ww < kSortWaves; ++ww) {
-        const uint32_t c = s_wave_hist[ww][d];
-        s_wave_hist[ww][d] = total;
-        total += c;
+    for (int k = 0; k < DPT; ++k) {
+        const int d = tid * DPT + k;
+        uint32_t t = 0;
+#pragma unroll
+        for (int ww = 0; ww < kSortWaves; ++ww) {
+            const uint32_t c = s_wave_hist[ww][d];
+            s_wave_hist[ww][d] = t;
+            t += c;
+        }
+        total[k] = t;
+        tsum += t;
     }
     uint32_t tile_tot;
-    const uint32_t inc = block_inclusive_scan<kSortWaves>(total, [](uint32_t a, uint32_t b) { return a + b; }, s_scan,
-                                                          tile_tot);
-    s_digit_start[d] = inc - total;
-
-    // Decoupled look-back for digit d.
-    uint32_t* my = status + tile * kRadix + d;
-    uint32_t excl = 0;
-    if (tile == 0) {
-        atomic_store_relaxed(my, kFlagPrefix | total);
-    } else {
-        atomic_store_relaxed(my, kFlagAggregate | total);
-        int64_t j = tile - 1;
-        while (true) {
-            const uint32_t s = atomic_load_relaxed(status + j * kRadix + d);
-            const uint32_t flag = s & ~kValueMask;
-            if (flag == 0u) {
-                __builtin_amdgcn_s_sleep(1);
-                continue;
-            }
-            excl += s & kValueMask;
-            if (flag == kFlagPrefix) break;
-            --j;
-        }
-        atomic_store_relaxed(my, kFlagPrefix | (excl + total));
+    uint32_t run = block_inclusive_scan<kSortWaves>(tsum, [](uint32_t a, uint32_t b) { return a + b; }, s_scan,
+                                                    tile_tot) - tsum;
+#pragma unroll
+    for (int k = 0; k < DPT; ++k) {
+        s_digit_start[tid * DPT + k] = run;
+        run += total[k];
     }
-    s_global_base[d] = digit_offsets[d] + excl;
+
+    // Decoupled look-back for each owned digit: publish this tile's count,
+    // then walk back to a tile whose inclusive prefix is known.
+#pragma unroll
+    for (int k = 0; k < DPT; ++k) {
+        const int d = tid * DPT + k;
+        atomic_store_relaxed(status + tile * R + d, (tile == 0 ? kFlagPrefix : kFlagAggregate) | total[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < DPT; ++k) {
+        const int d = tid * DPT + k;
+        uint32_t excl = 0;
+        if (tile != 0) {
+            int64_t j = tile - 1;
+            while (true) {
+                const uint32_t s = atomic_load_relaxed(status + j * R + d);
+                const uint32_t flag = s & ~kValueMask;
+                if (flag == 0u) {
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                excl += s & kValueMask;
+                if (flag == kFlagPrefix) break;
+                --j;
+            }
+            atomic_store_relaxed(status + tile * R + d, kFlagPrefix | (excl + total[k]));
+        }
+        s_global_base[d] = digit_offsets[d] + excl;
+    }
     __syncthreads();
 
     // Stage keys in digit order.
@@ -217,9 +252,10 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const K* __restrict__ k
     }
 }
 
-template <class K>
+template <class K, int RB>
 int sort_impl(Context* c, void* keys[2], int32_t* vals[2], int64_t n, int end_bit, bool vals_identity, int* result) {
-    const int passes = sort_passes(end_bit);
+    constexpr int R = 1 << RB;
+    const int passes = sort_passes(end_bit, RB);
     *result = 0;
     if (n <= 1 || passes == 0) {
         if (vals_identity && n > 0) {
@@ -229,38 +265,37 @@ int sort_impl(Context* c, void* keys[2], int32_t* vals[2], int64_t n, int end_bi
         return HX_OK;
     }
     const int64_t ntiles = sort_tiles(n);
-    const int64_t status_words = (int64_t)passes * ntiles * kRadix;
-    const int64_t words = status_words + (int64_t)passes * kRadix + kMaxPasses;
+    const int64_t status_words = (int64_t)passes * ntiles * R;
+    const int64_t words = status_words + (int64_t)passes * R + kMaxPasses;
     HX_TRY(c->sort_status.reserve(sizeof(uint32_t) * (size_t)words));
     uint32_t* status = c->sort_status.as<uint32_t>();
     uint32_t* hist = status + status_words;
-    uint32_t* counters = hist + passes * kRadix;
+    uint32_t* counters = hist + passes * R;
     {
         LaunchScope ls(c, "sort_hist");
         const unsigned zb = (unsigned)std::min<int64_t>((words + 255) / 256, 4096);
         hipLaunchKernelGGL(k_zero_u32, dim3(zb), dim3(256), 0, c->stream, status, words);
         HX_TRY(check_launch("sort_zero"));
         const unsigned hb = (unsigned)std::min<int64_t>((n + kSortBlock * 16 - 1) / (kSortBlock * 16), 2048);
-        hipLaunchKernelGGL(k_sort_hist<K>, dim3(hb), dim3(kSortBlock), 0, c->stream, static_cast<const K*>(keys[0]), n,
-                           passes, end_bit, hist);
+        hipLaunchKernelGGL((k_sort_hist<K, RB>), dim3(hb), dim3(kSortBlock), 0, c->stream,
+                           static_cast<const K*>(keys[0]), n, passes, end_bit, hist);
         HX_TRY(check_launch("sort_hist"));
-        hipLaunchKernelGGL(k_sort_scan_hist, dim3(passes), dim3(kRadix), 0, c->stream, hist);
+        hipLaunchKernelGGL(k_sort_scan_hist<RB>, dim3(passes), dim3(kSortBlock), 0, c->stream, hist);
         HX_TRY(check_launch("sort_scan_hist"));
     }
     for (int p = 0; p < passes; ++p) {
         const int in = p & 1, out = in ^ 1;
-        const int shift = 8 * p;
-        const int nb = std::min(8, end_bit - shift);
+        const int shift = RB * p;
+        const int nb = std::min(RB, end_bit - shift);
         LaunchScope ls(c, "sort_onesweep");
         if (p == 0 && vals_identity)
-            hipLaunchKernelGGL((k_onesweep<K, true>), dim3((unsigned)ntiles), dim3(kSortBlock), 0, c->stream,
+            hipLaunchKernelGGL((k_onesweep<K, true, RB>), dim3((unsigned)ntiles), dim3(kSortBlock), 0, c->stream,
                                static_cast<const K*>(keys[in]), static_cast<K*>(keys[out]), (const int32_t*)nullptr,
-                               vals[out], n, shift, nb, hist + p * kRadix, status + (int64_t)p * ntiles * kRadix,
-                               counters + p);
+                               vals[out], n, shift, nb, hist + p * R, status + (int64_t)p * ntiles * R, counters + p);
         else
-            hipLaunchKernelGGL((k_onesweep<K, false>), dim3((unsigned)ntiles), dim3(kSortBlock), 0, c->stream,
+            hipLaunchKernelGGL((k_onesweep<K, false, RB>), dim3((unsigned)ntiles), dim3(kSortBlock), 0, c->stream,
                                static_cast<const K*>(keys[in]), static_cast<K*>(keys[out]), vals[in], vals[out], n,
-                               shift, nb, hist + p * kRadix, status + (int64_t)p * ntiles * kRadix, counters + p);
+                               shift, nb, hist + p * R, status + (int64_t)p * ntiles * R, counters + p);
         HX_TRY(check_launch("sort_onesweep"));
     }
     *result = passes & 1;
@@ -269,10 +304,7 @@ int sort_impl(Context* c, void* keys[2], int32_t* vals[2], int64_t n, int end_bi
 
 }  // namespace
 
-size_t sort_status_bytes(int64_t n, int end_bit, bool) {
-    const int passes = sort_passes(end_bit);
-    return sizeof(uint32_t) * (size_t)((int64_t)passes * sort_tiles(n) * kRadix + passes * kRadix + kMaxPasses);
-}
+int sort_pass_count(const Context* c, int end_bit) { return sort_passes(end_bit, c->tune.sort_digit_bits); }
 
 int launch_sort_pairs(Context* c, void* keys[2], int32_t* vals[2], int64_t n, int end_bit, bool keys64,
                       bool vals_identity, int* result_index) {
@@ -280,8 +312,12 @@ int launch_sort_pairs(Context* c, void* keys[2], int32_t* vals[2], int64_t n, in
         set_error("radix sort: n = %lld exceeds 2^30 - 1", (long long)n);
         return HX_ERR_INVALID_ARG;
     }
-    if (keys64) return sort_impl<uint64_t>(c, keys, vals, n, end_bit, vals_identity, result_index);
-    return sort_impl<uint32_t>(c, keys, vals, n, end_bit, vals_identity, result_index);
+    if (c->tune.sort_digit_bits == 10) {
+        if (keys64) return sort_impl<uint64_t, 10>(c, keys, vals, n, end_bit, vals_identity, result_index);
+        return sort_impl<uint32_t, 10>(c, keys, vals, n, end_bit, vals_identity, result_index);
+    }
+    if (keys64) return sort_impl<uint64_t, 8>(c, keys, vals, n, end_bit, vals_identity, result_index);
+    return sort_impl<uint32_t, 8>(c, keys, vals, n, end_bit, vals_identity, result_index);
 }
 
 }  // namespace hx

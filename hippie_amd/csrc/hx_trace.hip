@@ -317,9 +317,14 @@ __global__ void k_zero_control(int32_t* control) {
 
 template <bool STATS, bool ORDERED, int LDS_DEPTH>
 int launch_trace_t(Context* c, const TraceArgs& a) {
-    int per_cu = 0;
-    HX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_trace<STATS, ORDERED, LDS_DEPTH>, kTraceBlock, 0));
-    per_cu = std::max(per_cu, 1);
+    // Occupancy limit of this instantiation, queried once per process.
+    static int occ = 0;
+    if (occ == 0) {
+        int q = 0;
+        HX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&q, k_trace<STATS, ORDERED, LDS_DEPTH>, kTraceBlock, 0));
+        occ = std::max(q, 1);
+    }
+    const int per_cu = c->tune.trace_blocks_per_cu > 0 ? std::min(c->tune.trace_blocks_per_cu, occ) : occ;
     const int64_t want = (a.n_rays + kTraceBlock - 1) / kTraceBlock;
     const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>((int64_t)c->cu_count * per_cu, want));
     hipLaunchKernelGGL((k_trace<STATS, ORDERED, LDS_DEPTH>), dim3(blocks), dim3(kTraceBlock), 0, c->stream, a);

@@ -142,8 +142,10 @@ int hx_context_profile_entry(hx_context* ctx, int32_t index, char* name, int32_t
 int hx_context_profile_reset(hx_context* ctx);
 /* Launch-shape tunables chosen by the autotuner (no reference counterpart;
  * the reference hard-codes them, HipTracerBinKernels.cu:5-6,10-15):
- *   "trace_lds_stack"  traversal-stack entries held in LDS (0,16,24,32,64)
- *   "trace_fetch"      refill a wave when fewer of its 64 lanes are busy   */
+ *   "trace_lds_stack"      traversal-stack entries held in LDS (0,16,24,32,64)
+ *   "trace_fetch"          refill a wave when fewer of its 64 lanes are busy
+ *   "trace_blocks_per_cu"  persistent trace workgroups per CU (0 = occupancy)
+ *   "sort_digit_bits"      radix digit width of every sort (8 or 10)       */
 int hx_context_set_tuning(hx_context* ctx, const char* name, int32_t value);
 int hx_context_get_tuning(hx_context* ctx, const char* name, int32_t* value);
 

@@ -128,17 +128,17 @@ def test_batch_diff_step_graph_replays_match_eager(ctx):
     side = torch.cuda.Stream()
     side.wait_stream(torch.cuda.current_stream())
     with torch.cuda.stream(side):
-        ctx.set_stream(side.cuda_stream)
+        ctx.set_stream(side)
         step()
     torch.cuda.current_stream().wait_stream(side)
     torch.cuda.synchronize()
     g = torch.cuda.CUDAGraph()
     try:
         with torch.cuda.graph(g):
-            ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+            ctx.set_stream(torch.cuda.current_stream())
             step()
     finally:
-        ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+        ctx.set_stream(torch.cuda.current_stream())
     torch.cuda.synchronize()
     rng = np.random.default_rng(2)
     for k in range(3):

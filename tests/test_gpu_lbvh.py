@@ -185,3 +185,35 @@ def test_sort_pairs_bit_ranges(ctx, end_bit):
     hx.sort_pairs(ctx, kt, vt, end_bit)
     order = np.argsort(k, kind="stable")
     assert np.array_equal(vt.cpu().numpy(), vals[order])
+
+
+@pytest.fixture
+def digits10(ctx):
+    """Run a test with 10-bit radix digits, restoring the default after."""
+    old = ctx.get_tuning("sort_digit_bits")
+    ctx.set_tuning("sort_digit_bits", 10)
+    yield ctx
+    ctx.set_tuning("sort_digit_bits", old)
+
+
+@pytest.mark.parametrize("n", [2, 4097, 300000])
+@pytest.mark.parametrize("end_bit", [13, 30, 60, 64])
+def test_sort_pairs_10bit_digits(digits10, n, end_bit):
+    ctx = digits10
+    rng = np.random.default_rng(n + end_bit)
+    hi = np.uint64((1 << end_bit) - 1) if end_bit < 64 else np.uint64(0xFFFFFFFFFFFFFFFF)
+    k = rng.integers(0, 1 << 63, n, dtype=np.uint64) & hi
+    k[: n // 3] = k[0]  # duplicates
+    vals = rng.permutation(n).astype(np.int32)
+    kt = torch.from_numpy(k.view(np.int64).copy()).cuda()
+    vt = torch.from_numpy(vals.copy()).cuda()
+    hx.sort_pairs(ctx, kt, vt, end_bit)
+    order = np.argsort(k, kind="stable")
+    assert np.array_equal(kt.cpu().numpy().view(np.uint64), k[order])
+    assert np.array_equal(vt.cpu().numpy(), vals[order])
+
+
+@pytest.mark.parametrize("bits", [30, 60])
+def test_lbvh_10bit_digits(oracle, digits10, bits):
+    v, t = scenes.gaussian_mixture_soup(100000, seed=bits)
+    _check_build(oracle, digits10, v, t, bits)

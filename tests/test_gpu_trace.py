@@ -166,3 +166,17 @@ def test_trace_single_triangle_and_dead_rays(oracle, ctx):
     rays[:, 7] = [3.0, 3.0, -1.0, 0.5]
     scene, bvh, res, ref = _check_against_oracle(oracle, ctx, v, t, rays)
     assert list(res[:, 0]) == [0, -1, -1, -1]
+
+
+@pytest.mark.parametrize("name,value", [("trace_lds_stack", 0), ("trace_lds_stack", 32), ("trace_lds_stack", 64),
+                                        ("trace_fetch", 1), ("trace_fetch", 65), ("trace_blocks_per_cu", 1),
+                                        ("trace_blocks_per_cu", 3)])
+def test_trace_tunables_keep_results(oracle, ctx, name, value):
+    """Every launch-shape choice the autotuner can make gives the same hits."""
+    old = ctx.get_tuning(name)
+    ctx.set_tuning(name, value)
+    try:
+        v, t = scenes.gaussian_mixture_soup(40000, seed=31)
+        _check_against_oracle(oracle, ctx, v, t, scenes.random_rays(30000, seed=8))
+    finally:
+        ctx.set_tuning(name, old)
