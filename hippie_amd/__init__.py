@@ -4,10 +4,10 @@ The compute path is libhx.so (HIP kernels + extern "C" boundary, include/hx.h);
 this package is the host-side mirror of hippie's builder/tracer interface.
 """
 from . import _lib  # noqa: F401
-from .core import (ATRBuilder, BuildParams, BVHBuilder, Context, HipBatch, HipBVH, HipTracer, LBVHBuilder,  # noqa: F401
-                   MeshBatch, PLOCBuilder, RayBuffer, Scene, morton_codes, ray_keys, scene_box, sort_pairs)
+from .core import (ATRBuilder, BuildParams, BVHBuilder, Context, HipBatch, HipBVH, HipTracer, HLBVHBuilder,  # noqa: F401
+                   LBVHBuilder, MeshBatch, PLOCBuilder, RayBuffer, Scene, morton_codes, ray_keys, scene_box, sort_pairs)
 
 __all__ = [
-    "ATRBuilder", "BuildParams", "BVHBuilder", "Context", "HipBatch", "HipBVH", "HipTracer", "LBVHBuilder",
+    "ATRBuilder", "BuildParams", "BVHBuilder", "Context", "HipBatch", "HipBVH", "HipTracer", "HLBVHBuilder", "LBVHBuilder",
     "MeshBatch", "PLOCBuilder", "RayBuffer", "Scene", "morton_codes", "ray_keys", "scene_box", "sort_pairs",
 ]

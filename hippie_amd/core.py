@@ -332,6 +332,18 @@ class PLOCBuilder(BVHBuilder):
         self.params.ploc_adaptive = int(bool(on))
 
 
+class HLBVHBuilder(BVHBuilder):
+    """HLBVHBuilder (rt/bvh/HLBVHBuilder.h, HLBVHBuilder.cpp:412-426): LBVH below
+    clusters of the top `hlbvh_sah_bits` Morton bits, binned SAH above."""
+
+    method = "hlbvh"
+
+    def set_morton_code_sah_bits(self, bits):
+        if not (6 <= bits <= 15) or bits > self.params.morton_bits:
+            raise ValueError("Morton code SAH bits must be in range [6,15] and not above the Morton bits")
+        self.params.hlbvh_sah_bits = bits
+
+
 class ATRBuilder(BVHBuilder):
     """ATRBuilder (rt/bvh/ATRBuilder.h, ATRBuilder.cpp:238-290): LBVH plus
     agglomerative treelet restructuring.  Out-of-range values raise instead of

@@ -139,9 +139,11 @@ struct hx_bvh {
         int_slot, counts, scene_box, cost_out, frontier;
     // PLOC: split position per internal node (kept for refits) and the
     // clustering workspace (cluster ids, boxes and neighbours, ping-pong).
-    DevBuf split, ploc_work, atr_work;
+    DevBuf split, ploc_work, atr_work, hlbvh_work;
     int ploc_rounds = 0;
     int64_t atr_accepted = 0;
+    int hlbvh_clusters = 0;
+    bool general = false;  // not a Karras tree: the climb uses split[]
     // Per-mesh tables of the emitted layout (one entry unless batched).
     DevBuf mesh_tab, root_slot;
     // Batched builds: meshes occupy triangle ranges [off[m], off[m+1]).
@@ -180,7 +182,8 @@ int validate_params(const hx_build_params* p) {
                  p->max_leaf_size);
     HX_CHECK_ARG(p->ct > 0.0f && p->ci > 0.0f, "ct and ci must be positive");
     HX_CHECK_ARG(p->layout == HX_LAYOUT_BIN, "only the Bin layout is supported (got %d)", p->layout);
-    if (p->method != HX_METHOD_LBVH && p->method != HX_METHOD_PLOC && p->method != HX_METHOD_ATRBVH) {
+    if (p->method != HX_METHOD_LBVH && p->method != HX_METHOD_PLOC && p->method != HX_METHOD_ATRBVH &&
+        p->method != HX_METHOD_HLBVH) {
         hx::set_error("builder method %d is not available in this build", p->method);
         return HX_ERR_UNSUPPORTED;
     }
@@ -189,6 +192,12 @@ int validate_params(const hx_build_params* p) {
         HX_CHECK_ARG(p->ploc_max_radius >= 1, "ploc_max_radius must be positive (got %d)", p->ploc_max_radius);
         HX_CHECK_ARG(p->ploc_radius >= 1 && p->ploc_radius <= p->ploc_max_radius,
                      "ploc_radius must be in [1, %d] (got %d)", p->ploc_max_radius, p->ploc_radius);
+    }
+    if (p->method == HX_METHOD_HLBVH) {
+        // HLBVHBuilder::setMortonCodeSAHBits (HLBVHBuilder.cpp:422-426).
+        HX_CHECK_ARG(p->hlbvh_sah_bits >= 6 && p->hlbvh_sah_bits <= 15, "hlbvh_sah_bits must be in [6,15] (got %d)",
+                     p->hlbvh_sah_bits);
+        HX_CHECK_ARG(p->hlbvh_sah_bits <= p->morton_bits, "hlbvh_sah_bits must not exceed morton_bits");
     }
     if (p->method == HX_METHOD_ATRBVH) {
         // ATRBuilder::setTreeletSize / setIterations (ATRBuilder.cpp:273-290).
@@ -211,6 +220,12 @@ size_t ploc_work_bytes(int64_t n) { return (size_t)n * (2 * 4 + 2 * 16 + 2 * 8 +
 // hist, cursor (levels + 1 ints), accepted.
 size_t atr_work_bytes(int64_t n) {
     return (size_t)(2 * n) * 8 + (size_t)n * 12 + 2 * sizeof(int32_t) * (2 * hx::kAtrMaxLevels + 1) + 4096;
+}
+
+// HLBVH workspace: two n-int arrays, cluster arrays of up to 2^15 entries.
+size_t hlbvh_work_bytes(int64_t n) {
+    const size_t K = (size_t)hx::kHlbvhMaxClusters;
+    return (size_t)n * 8 + K * (4 * 8 + 16 * 4) + 8192;
 }
 
 int reserve_bvh(hx_bvh* b, int64_t n) {
@@ -258,6 +273,10 @@ int reserve_bvh(hx_bvh* b, int64_t n) {
         HX_TRY(b->split.reserve(4 * (size_t)n));
         HX_TRY(b->atr_work.reserve(atr_work_bytes(n)));
     }
+    if (b->params.method == HX_METHOD_HLBVH) {
+        HX_TRY(b->split.reserve(4 * (size_t)n));
+        HX_TRY(b->hlbvh_work.reserve(hlbvh_work_bytes(n)));
+    }
     return HX_OK;
 }
 
@@ -273,8 +292,7 @@ hx::ClimbArgs climb_args(hx_bvh* b, bool refit_only) {
     ca.parent = b->parent.as<int32_t>();
     ca.first = b->first.as<int32_t>();
     ca.last = b->last.as<int32_t>();
-    ca.split = (b->params.method == HX_METHOD_PLOC || b->params.method == HX_METHOD_ATRBVH) ? b->split.as<int32_t>()
-                                                                                             : nullptr;
+    ca.split = b->general ? b->split.as<int32_t>() : nullptr;
     ca.counters = b->counters.as<int32_t>();
     ca.nmin = b->nmin.as<float4>();
     ca.nmax = b->nmax.as<float4>();
@@ -382,6 +400,7 @@ int run_ploc(hx_bvh* b) {
     a.max_radius = p.ploc_max_radius;
     a.adaptive = p.ploc_adaptive ? 1 : 0;
     HX_TRY(hx::launch_ploc(c, a, &b->ploc_rounds));
+    b->general = true;
     // The triangle ids in depth-first leaf order become the output array.
     std::swap(b->tri_idx.ptr, b->vals_tmp.ptr);
     std::swap(b->tri_idx.bytes, b->vals_tmp.bytes);
@@ -453,6 +472,7 @@ int run_atr(hx_bvh* b) {
     HX_TRY(hx::launch_climb(c, ca));
     const hx::PlocArgs rl = relabel_args(b);
     HX_TRY(hx::launch_atr(c, a, rl, &b->atr_accepted));
+    b->general = true;
     std::swap(b->tri_idx.ptr, b->vals_tmp.ptr);
     std::swap(b->tri_idx.bytes, b->vals_tmp.bytes);
     // Fresh climb state for the general tree.
@@ -522,6 +542,67 @@ int read_tables(hx_bvh* b) {
     return HX_OK;
 }
 
+// HLBVHBuilder::build (HLBVHBuilder.cpp:332-410): Karras hierarchy and its
+// boxes, clusters and the binned-SAH upper levels, then (unless everything is
+// one cluster) the general climb and the collapse.
+int run_hlbvh(hx_bvh* b) {
+    Context* c = &b->ctx->c;
+    const hx_build_params& p = b->params;
+    const int64_t n = b->n_refs;
+    HX_TRY(hx::launch_construct(c, b->keys[b->keys_index].ptr, b->keys64, n, b->left.as<int32_t>(),
+                                b->right.as<int32_t>(), b->parent.as<int32_t>(), b->first.as<int32_t>(),
+                                b->last.as<int32_t>(), b->counters.as<int32_t>(), b->cover.as<int32_t>(),
+                                b->counts.as<int32_t>() + 2));
+    HX_TRY(hx::launch_climb(c, climb_args(b, false)));
+    hx::HlbvhArgs a{};
+    a.n = n;
+    a.bits = p.morton_bits;
+    a.sah_bits = p.hlbvh_sah_bits;
+    a.nbins = 1 << ((p.hlbvh_sah_bits + 2) / 3);
+    a.keys = b->keys[b->keys_index].ptr;
+    a.sorted_tri = b->tri_idx.as<int32_t>();
+    a.tri_out = b->vals_tmp.as<int32_t>();
+    a.left = b->left.as<int32_t>();
+    a.right = b->right.as<int32_t>();
+    a.parent = b->parent.as<int32_t>();
+    a.first = b->first.as<int32_t>();
+    a.last = b->last.as<int32_t>();
+    a.split = b->split.as<int32_t>();
+    a.nmin = b->nmin.as<float4>();
+    a.nmax = b->nmax.as<float4>();
+    char* w = b->hlbvh_work.as<char>();
+    auto carve = [&](size_t bytes) {
+        char* q = w;
+        w += (bytes + 255) & ~(size_t)255;
+        return q;
+    };
+    const size_t K = (size_t)hx::kHlbvhMaxClusters;
+    a.cl_of = reinterpret_cast<int32_t*>(carve(4 * (size_t)n));
+    a.tmp = reinterpret_cast<int32_t*>(carve(4 * (size_t)n));
+    int32_t** ints[] = {&a.ca, &a.cb, &a.croot, &a.cl_off, &a.cl_par, &a.upper, &a.task_size, &a.child_counts};
+    for (int32_t** q : ints) *q = reinterpret_cast<int32_t*>(carve(4 * K));
+    a.members[0] = reinterpret_cast<int32_t*>(carve(4 * K));
+    a.members[1] = reinterpret_cast<int32_t*>(carve(4 * K));
+    a.coord = reinterpret_cast<int4*>(carve(16 * K));
+    a.tasks[0] = reinterpret_cast<int4*>(carve(16 * K));
+    a.tasks[1] = reinterpret_cast<int4*>(carve(16 * K));
+    a.decision = reinterpret_cast<int4*>(carve(16 * K));
+    int C = 0;
+    HX_TRY(hx::launch_hlbvh(c, a, b->keys64, &C));
+    b->hlbvh_clusters = C;
+    if (C <= 1) {
+        b->general = false;
+        return hx::launch_collapse_emit(c, emit_args(b));
+    }
+    b->general = true;
+    std::swap(b->tri_idx.ptr, b->vals_tmp.ptr);
+    std::swap(b->tri_idx.bytes, b->vals_tmp.bytes);
+    HX_TRY(hx::launch_refit_reset(c, b->counters.as<int32_t>(), n - 1, b->counts.as<int32_t>() + 2));
+    HX_TRY(hx::launch_fill_i32(c, b->cover.as<int32_t>(), -1, n));
+    HX_TRY(hx::launch_climb(c, climb_args(b, false)));
+    return hx::launch_collapse_emit(c, emit_args(b));
+}
+
 int run_build(hx_bvh* b) {
     Context* c = &b->ctx->c;
     const hx_build_params& p = b->params;
@@ -535,6 +616,8 @@ int run_build(hx_bvh* b) {
         HX_TRY(run_ploc(b));
     } else if (p.method == HX_METHOD_ATRBVH) {
         HX_TRY(run_atr(b));
+    } else if (p.method == HX_METHOD_HLBVH) {
+        HX_TRY(run_hlbvh(b));
     } else {
         HX_TRY(hx::launch_construct(c, b->keys[b->keys_index].ptr, b->keys64, n, b->left.as<int32_t>(), b->right.as<int32_t>(),
                                     b->parent.as<int32_t>(), b->first.as<int32_t>(), b->last.as<int32_t>(),
@@ -1037,6 +1120,8 @@ int hx_bvh_info(hx_bvh* bvh, const char* key, int64_t* value) {
         *value = bvh->ploc_rounds;
     } else if (!std::strcmp(key, "atr_accepted")) {
         *value = bvh->atr_accepted;
+    } else if (!std::strcmp(key, "hlbvh_clusters")) {
+        *value = bvh->hlbvh_clusters;
     } else if (!std::strcmp(key, "method")) {
         *value = bvh->params.method;
     } else {

@@ -250,6 +250,41 @@ constexpr int kAtrMaxLevels = 1024;
 // relabel: fields of a PLOC argument block used by launch_tree_down.
 int launch_atr(Context* c, const AtrArgs& a, const PlocArgs& relabel, int64_t* accepted_out);
 
+// hx_hlbvh.hip: clusters of equal top-sah_bits Morton prefixes (Karras
+// subtrees), binned-SAH upper levels over them, depth-first relabelling.
+// Cluster arrays hold up to 2^sah_bits entries.  n >= 2.
+constexpr int kHlbvhMaxClusters = 1 << 15;
+struct HlbvhArgs {
+    int64_t n;
+    int bits, sah_bits, nbins;
+    const void* keys;            // sorted Morton keys (u64, or u32 when bits <= 32)
+    const int32_t* sorted_tri;
+    int32_t* tri_out;            // depth-first leaf order
+    int32_t* left;
+    int32_t* right;
+    int32_t* parent;
+    int32_t* first;
+    int32_t* last;
+    int32_t* split;
+    float4* nmin;
+    float4* nmax;
+    int32_t* cl_of;              // n: cluster of each sorted position
+    int32_t* tmp;                // n
+    int32_t* ca;                 // clusters: first / last position, root node
+    int32_t* cb;
+    int32_t* croot;
+    int32_t* cl_off;             // depth-first offset of each cluster
+    int32_t* cl_par;             // upper node above each cluster
+    int4* coord;                 // bin coordinates x, y, z and size
+    int32_t* members[2];         // cluster lists of the tasks, ping-pong
+    int32_t* upper;              // ids of the Karras nodes spanning clusters
+    int32_t* task_size;          // by upper index
+    int4* tasks[2];              // {upper index, first member, end member, dfs offset}
+    int4* decision;              // per task {axis, bin, left count, -}
+    int32_t* child_counts;       // per task
+};
+int launch_hlbvh(Context* c, const HlbvhArgs& a, bool keys64, int* clusters_out);
+
 // hx_batch.hip: batched-build helpers (mesh of each triangle, per-mesh
 // boxes, key gathers, mesh-local triangle ids).
 int launch_mesh_of(Context* c, const int32_t* off, int M, int64_t n, int32_t* mesh_of);

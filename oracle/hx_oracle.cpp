@@ -1265,6 +1265,225 @@ int64_t hxo_build_ploc(const float* verts, const int32_t* tris, int64_t n, const
     return nn;
 }
 
+/* HLBVHBuilder::build (HLBVHBuilder.cpp:61-437, HLBVHBuilderKernels.cu:35-704):
+ * the LBVH below clusters of references that share the top sah_bits Morton
+ * bits (a run of equal prefixes is exactly one Karras subtree), and a binned
+ * SAH split over the clusters above them, in breadth-first task order.  Bins
+ * are the fixed grid cells of the Morton prefix (HLBVHBin.h:14-44).  Where the
+ * reference numbers clusters and median-split members with atomics, the order
+ * here is the clusters' Morton order.  Upper nodes reuse the ids of the Karras
+ * nodes that span clusters (sorted), so the root stays 0.  Returns the number
+ * of clusters; with a single cluster the tree is the LBVH. */
+static int64_t hlbvh_upper(int64_t n, int bits, int sah_bits, const uint64_t* keys, int32_t* left, int32_t* right,
+                           int32_t* parent, float* nmin, float* nmax) {
+    const int64_t N = 2 * n - 1;
+    /* Karras ranges. */
+    std::vector<int32_t> first(N), last(N);
+    {
+        std::vector<int32_t> order;
+        order.reserve(N);
+        std::vector<int32_t> st(1, 0);
+        while (!st.empty()) {
+            const int32_t v = st.back();
+            st.pop_back();
+            order.push_back(v);
+            if (v < n - 1) {
+                st.push_back(left[v]);
+                st.push_back(right[v]);
+            }
+        }
+        for (int64_t k = (int64_t)order.size() - 1; k >= 0; --k) {
+            const int32_t v = order[k];
+            if (v >= n - 1) {
+                first[v] = last[v] = (int32_t)(v - (n - 1));
+            } else {
+                first[v] = std::min(first[left[v]], first[right[v]]);
+                last[v] = std::max(last[left[v]], last[right[v]]);
+            }
+        }
+    }
+    const int shift = bits - sah_bits;
+    auto prefix = [&](int64_t p) { return (uint32_t)(keys[p] >> shift); };
+    std::vector<int32_t> cl_of(n);
+    std::vector<int32_t> ca, cb;
+    for (int64_t p = 0; p < n; ++p) {
+        if (p == 0 || prefix(p) != prefix(p - 1)) {
+            ca.push_back((int32_t)p);
+            cb.push_back((int32_t)p);
+        }
+        cb.back() = (int32_t)p;
+        cl_of[p] = (int32_t)ca.size() - 1;
+    }
+    const int64_t C = (int64_t)ca.size();
+    if (C == 1) return 1;
+    /* Cluster roots: the Karras node whose range is the run. */
+    std::vector<int32_t> croot(C, -1);
+    for (int64_t c = 0; c < C; ++c)
+        if (ca[c] == cb[c]) croot[c] = (int32_t)(n - 1 + ca[c]);
+    for (int64_t v = 0; v < n - 1; ++v) {
+        const int32_t c = cl_of[first[v]];
+        if (first[v] == ca[c] && last[v] == cb[c]) croot[c] = (int32_t)v;
+    }
+    /* Upper ids: Karras internal nodes spanning clusters, ascending. */
+    std::vector<int32_t> upper;
+    for (int64_t v = 0; v < n - 1; ++v)
+        if (cl_of[first[v]] != cl_of[last[v]]) upper.push_back((int32_t)v);
+    /* Cluster bin coordinates (mortonCodeToCoords, :10-19). */
+    const int nbins = 1 << ((sah_bits + 2) / 3);
+    std::vector<int32_t> coord(3 * C);
+    std::vector<V3> cmn(C), cmx(C);
+    for (int64_t c = 0; c < C; ++c) {
+        const uint32_t code = prefix(ca[c]);
+        int co[3] = {0, 0, 0};
+        for (int i = 0; i < 10; ++i)
+            for (int a = 0; a < 3; ++a) co[a] += ((code >> (3 * i + a)) & 1) << i;
+        for (int a = 0; a < 3; ++a) coord[3 * c + a] = co[a];
+        cmn[c] = load4(nmin + 4 * (int64_t)croot[c]);
+        cmx[c] = load4(nmax + 4 * (int64_t)croot[c]);
+    }
+    struct Bin {
+        V3 mn{kMaxFloat, kMaxFloat, kMaxFloat}, mx{-kMaxFloat, -kMaxFloat, -kMaxFloat};
+        int count = 0;
+        void include(const Bin& o) {
+            mn = V3{std::min(mn.x, o.mn.x), std::min(mn.y, o.mn.y), std::min(mn.z, o.mn.z)};
+            mx = V3{std::max(mx.x, o.mx.x), std::max(mx.y, o.mx.y), std::max(mx.z, o.mx.z)};
+            count += o.count;
+        }
+        float area() const {
+            const float dx = mx.x - mn.x, dy = mx.y - mn.y, dz = mx.z - mn.z;
+            return 2.0f * (dx * dy + dx * dz + dy * dz);
+        }
+        float cost() const { return (float)count * area(); }
+    };
+    std::vector<int32_t> members(C);
+    for (int64_t c = 0; c < C; ++c) members[c] = (int32_t)c;
+    struct Task {
+        int32_t k, s, e;
+    };
+    std::vector<Task> level(1, Task{0, 0, (int32_t)C}), next;
+    int32_t next_k = 1;
+    std::vector<int32_t> tmp;
+    parent[0] = -1;
+    while (!level.empty()) {
+        next.clear();
+        for (const Task& tk : level) {
+            std::vector<Bin> bins(3 * nbins);
+            for (int32_t i = tk.s; i < tk.e; ++i) {
+                const int32_t c = members[i];
+                for (int a = 0; a < 3; ++a) {
+                    Bin& b = bins[a * nbins + coord[3 * c + a]];
+                    b.mn = V3{std::min(b.mn.x, cmn[c].x), std::min(b.mn.y, cmn[c].y), std::min(b.mn.z, cmn[c].z)};
+                    b.mx = V3{std::max(b.mx.x, cmx[c].x), std::max(b.mx.y, cmx[c].y), std::max(b.mx.z, cmx[c].z)};
+                    b.count += 1;
+                }
+            }
+            /* split (:395-497): first strict minimum over axes, then bins. */
+            float best = kMaxFloat;
+            int best_axis = -1, best_bin = -1;
+            Bin lbest, rbest, all;
+            for (int a = 0; a < 3; ++a) {
+                std::vector<Bin> right_acc(nbins);
+                right_acc[nbins - 1] = bins[a * nbins + nbins - 1];
+                for (int b = nbins - 2; b >= 0; --b) {
+                    right_acc[b] = right_acc[b + 1];
+                    right_acc[b].include(bins[a * nbins + b]);
+                }
+                Bin acc = bins[a * nbins];
+                for (int b = 0; b < nbins - 1; ++b) {
+                    if (acc.count > 0 && right_acc[b + 1].count > 0) {
+                        const float cst = acc.cost() + right_acc[b + 1].cost();
+                        if (cst < best) {
+                            best = cst;
+                            best_axis = a;
+                            best_bin = b;
+                            lbest = acc;
+                            rbest = right_acc[b + 1];
+                        }
+                    }
+                    acc.include(bins[a * nbins + b + 1]);
+                }
+                all = acc;
+            }
+            const int32_t cnt = tk.e - tk.s;
+            int32_t nl;
+            tmp.assign(members.begin() + tk.s, members.begin() + tk.e);
+            if (best == kMaxFloat) {
+                /* every cluster in one bin on every axis: median by order */
+                nl = cnt / 2;
+                lbest = all;
+                rbest = all;
+            } else {
+                int32_t w = tk.s;
+                for (int32_t c : tmp)
+                    if (coord[3 * c + best_axis] <= best_bin) members[w++] = c;
+                nl = w - tk.s;
+                for (int32_t c : tmp)
+                    if (coord[3 * c + best_axis] > best_bin) members[w++] = c;
+            }
+            const int32_t v = upper[tk.k];
+            Box bx;
+            bx.grow(lbest.mn);
+            bx.grow(lbest.mx);
+            bx.grow(rbest.mn);
+            bx.grow(rbest.mx);
+            store4(nmin + 4 * (int64_t)v, bx.mn);
+            store4(nmax + 4 * (int64_t)v, bx.mx);
+            const int32_t ranges[2][2] = {{tk.s, tk.s + nl}, {tk.s + nl, tk.e}};
+            int32_t child[2];
+            for (int side = 0; side < 2; ++side) {
+                const int32_t s = ranges[side][0], e = ranges[side][1];
+                if (e - s == 1) {
+                    child[side] = croot[members[s]];
+                } else {
+                    const int32_t k = next_k++;
+                    child[side] = upper[k];
+                    next.push_back(Task{k, s, e});
+                }
+                parent[child[side]] = v;
+            }
+            left[v] = child[0];
+            right[v] = child[1];
+        }
+        std::swap(level, next);
+    }
+    return C;
+}
+
+int64_t hxo_hlbvh(int64_t n, int bits, int sah_bits, const uint64_t* sorted_keys, int32_t* left, int32_t* right,
+                  int32_t* parent, float* nmin, float* nmax) {
+    if (n < 2) return 1;
+    return hlbvh_upper(n, bits, sah_bits, sorted_keys, left, right, parent, nmin, nmax);
+}
+
+int64_t hxo_build_hlbvh(const float* verts, const int32_t* tris, int64_t n, const float* scene_box, int bits,
+                        int sah_bits, int adaptive, int max_leaf_size, float ct, float ci, int32_t* nodes,
+                        int32_t* tri_out, float* woop, int64_t* n_interior, int64_t* n_leaves, int64_t* clusters) {
+    float sb[6];
+    if (scene_box) std::memcpy(sb, scene_box, sizeof(sb));
+    else hxo_scene_box(verts, tris, n, sb);
+    std::vector<float> rmin(4 * n), rmax(4 * n);
+    hxo_setup_boxes(verts, tris, n, rmin.data(), rmax.data());
+    std::vector<uint64_t> keys(n);
+    std::vector<int32_t> vals(n);
+    hxo_morton_codes(rmin.data(), rmax.data(), n, sb, bits, keys.data());
+    for (int64_t i = 0; i < n; ++i) vals[i] = (int32_t)i;
+    hxo_sort_pairs(keys.data(), vals.data(), n, bits);
+    const int64_t N = std::max<int64_t>(2 * n - 1, 1);
+    std::vector<int32_t> left(N), right(N), parent(N);
+    std::vector<float> nmin(4 * N), nmax(4 * N);
+    if (n > 1) hxo_construct(keys.data(), n, left.data(), right.data(), parent.data());
+    hxo_refit(vals.data(), rmin.data(), rmax.data(), n, left.data(), right.data(), parent.data(), nmin.data(),
+              nmax.data());
+    const int64_t C = n > 1 ? hlbvh_upper(n, bits, sah_bits, keys.data(), left.data(), right.data(), parent.data(),
+                                          nmin.data(), nmax.data())
+                            : 1;
+    if (clusters) *clusters = C;
+    int64_t nn = hxo_collapse(n, vals.data(), left.data(), right.data(), parent.data(), nmin.data(), nmax.data(),
+                              adaptive, max_leaf_size, ct, ci, nodes, tri_out, n_interior, n_leaves);
+    hxo_woopify(verts, tris, tri_out, n, woop);
+    return nn;
+}
+
 /* ATRBuilder::build (ATRBuilder.cpp:179-236): LBVH front half and hierarchy,
  * refit, ATR optimisation, collapse, woopify. */
 int64_t hxo_build_atr(const float* verts, const int32_t* tris, int64_t n, const float* scene_box, int bits,

@@ -143,6 +143,18 @@ int64_t hxo_build_atr(const float* verts, const int32_t* tris, int64_t n, const 
                       int32_t* nodes, int32_t* tri_out, float* woop, int64_t* n_interior, int64_t* n_leaves,
                       int64_t* accepted);
 
+/* HLBVH upper levels over a Karras tree (left/right/parent, boxes 2n-1) of the
+ * sorted keys, in place; returns the number of clusters (1: unchanged). */
+int64_t hxo_hlbvh(int64_t n, int bits, int sah_bits, const uint64_t* sorted_keys, int32_t* left, int32_t* right,
+                  int32_t* parent, float* nmin, float* nmax);
+
+/* HLBVHBuilder::build (HLBVHBuilder.cpp:61-437, HLBVHBuilderKernels.cu:35-704):
+ * LBVH below clusters sharing the top sah_bits Morton bits, binned SAH above
+ * them.  *clusters receives the number of clusters. */
+int64_t hxo_build_hlbvh(const float* verts, const int32_t* tris, int64_t n, const float* scene_box, int bits,
+                        int sah_bits, int adaptive, int max_leaf_size, float ct, float ci, int32_t* nodes,
+                        int32_t* tri_out, float* woop, int64_t* n_interior, int64_t* n_leaves, int64_t* clusters);
+
 /* Ray Morton keys: rt/ray/RayBufferKernels.cu:556-614 (Aila64) and
  * :492-554 (TwoPoint64) with the ray box fixed to [0,1]^3
  * (rt/ray/RayBuffer.cpp:63-66). */

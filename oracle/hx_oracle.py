@@ -54,6 +54,9 @@ def lib():
             "hxo_atr": (_i64, [_i64, _i32, _i32, _f32, _f32, _p, _p, _p, _p, _p]),
             "hxo_build_atr": (_i64, [_p, _p, _i64, _p, _i32, _i32, _i32, _i32, _i32, _f32, _f32,
                                      _p, _p, _p, _p, _p, _p]),
+            "hxo_hlbvh": (_i64, [_i64, _i32, _i32, _p, _p, _p, _p, _p, _p]),
+            "hxo_build_hlbvh": (_i64, [_p, _p, _i64, _p, _i32, _i32, _i32, _i32, _f32, _f32,
+                                       _p, _p, _p, _p, _p, _p]),
             "hxo_build_ploc": (_i64, [_p, _p, _i64, _p, _i32, _i32, _i32, _i32, _i32, _i32, _f32, _f32,
                                       _p, _p, _p, _p, _p, _p]),
             "hxo_ray_keys_aila64": (None, [_p, _i64, _i32, _p]),
@@ -280,6 +283,33 @@ def build_atr(verts, tris, bits=60, treelet_size=9, iterations=2, scene_box=None
                              _ptr(woop), ctypes.byref(ni), ctypes.byref(nl), ctypes.byref(acc))
     return dict(nodes=nodes[:nn].copy(), tri_idx=tri, woop=woop, n_interior=ni.value, n_leaves=nl.value,
                 accepted=acc.value)
+
+
+def hlbvh(sorted_keys, left, right, parent, nmin, nmax, bits=60, sah_bits=15):
+    """HLBVH upper levels over a Karras tree (copies); returns left, right, parent, nmin, nmax, clusters."""
+    k = _c(sorted_keys, np.uint64)
+    l, r, p = (_c(a, np.int32).copy() for a in (left, right, parent))
+    mn, mx = _c(nmin, np.float32).copy(), _c(nmax, np.float32).copy()
+    C = lib().hxo_hlbvh(k.shape[0], int(bits), int(sah_bits), _ptr(k), _ptr(l), _ptr(r), _ptr(p), _ptr(mn),
+                        _ptr(mx))
+    return l, r, p, mn, mx, C
+
+
+def build_hlbvh(verts, tris, bits=60, sah_bits=15, scene_box=None, adaptive=True, max_leaf_size=8, ct=3.0,
+                ci=2.0):
+    v, t = _c(verts, np.float32), _c(tris, np.int32)
+    n = t.shape[0]
+    N = max(2 * n - 1, 2)
+    nodes = np.zeros((N, 16), np.int32)
+    tri = np.zeros(n, np.int32)
+    woop = np.zeros((n, 12), np.float32)
+    ni, nl, cl = ctypes.c_int64(0), ctypes.c_int64(0), ctypes.c_int64(0)
+    sb = None if scene_box is None else _c(scene_box, np.float32)
+    nn = lib().hxo_build_hlbvh(_ptr(v), _ptr(t), n, _ptr(sb), int(bits), int(sah_bits), int(bool(adaptive)),
+                               int(max_leaf_size), float(ct), float(ci), _ptr(nodes), _ptr(tri), _ptr(woop),
+                               ctypes.byref(ni), ctypes.byref(nl), ctypes.byref(cl))
+    return dict(nodes=nodes[:nn].copy(), tri_idx=tri, woop=woop, n_interior=ni.value, n_leaves=nl.value,
+                clusters=cl.value)
 
 
 def ray_keys_aila64(rays, bits=60):
