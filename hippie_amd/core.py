@@ -174,6 +174,8 @@ class BuildParams:
         p.method = method
         if method in cls.BITS_KEYS and cls.BITS_KEYS[method] in env:
             p.morton_bits = int(env[cls.BITS_KEYS[method]])
+        if method == "insertion" and str(env.get("Bvh.insertionSbvh", "0")).strip().lower() in ("1", "true"):
+            raise NotImplementedError("Bvh.insertionSbvh: the SBVH start of the insertion builder is not available")
         for key, field in cls.ENV_KEYS.items():
             if key in env and key != "Bvh.method":
                 cur = getattr(p.c, field)
@@ -360,6 +362,23 @@ class ATRBuilder(BVHBuilder):
         if not (0 <= k <= 25):
             raise ValueError("iterations must be in range of [0,25]")
         self.params.atr_iterations = k
+
+
+class InsertionBuilder(BVHBuilder):
+    """InsertionBuilder (rt/bvh/InsertionBuilder.h, InsertionBuilder.cpp:442-494):
+    LBVH plus parallel insertion-based optimisation.  Only the LBVH start is
+    available (the reference's SBVH start runs a CPU SBVH builder first)."""
+
+    method = "insertion"
+
+    def set_mod(self, mod):
+        if not (1 <= mod <= 64):
+            raise ValueError("mod must be in range [1,64]")
+        self.params.insertion_mod = mod
+
+    def set_sbvh(self, sbvh):
+        if sbvh:
+            raise NotImplementedError("the SBVH start of the insertion builder is not available")
 
 
 class RayBuffer:

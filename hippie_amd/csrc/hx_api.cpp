@@ -139,10 +139,11 @@ struct hx_bvh {
         int_slot, counts, scene_box, cost_out, frontier;
     // PLOC: split position per internal node (kept for refits) and the
     // clustering workspace (cluster ids, boxes and neighbours, ping-pong).
-    DevBuf split, ploc_work, atr_work, hlbvh_work;
+    DevBuf split, ploc_work, atr_work, hlbvh_work, ins_work;
     int ploc_rounds = 0;
     int64_t atr_accepted = 0;
     int hlbvh_clusters = 0;
+    int64_t ins_steps = 0, ins_reinserted = 0;
     bool general = false;  // not a Karras tree: the climb uses split[]
     // Per-mesh tables of the emitted layout (one entry unless batched).
     DevBuf mesh_tab, root_slot;
@@ -183,7 +184,7 @@ int validate_params(const hx_build_params* p) {
     HX_CHECK_ARG(p->ct > 0.0f && p->ci > 0.0f, "ct and ci must be positive");
     HX_CHECK_ARG(p->layout == HX_LAYOUT_BIN, "only the Bin layout is supported (got %d)", p->layout);
     if (p->method != HX_METHOD_LBVH && p->method != HX_METHOD_PLOC && p->method != HX_METHOD_ATRBVH &&
-        p->method != HX_METHOD_HLBVH) {
+        p->method != HX_METHOD_HLBVH && p->method != HX_METHOD_INSERTION) {
         hx::set_error("builder method %d is not available in this build", p->method);
         return HX_ERR_UNSUPPORTED;
     }
@@ -198,6 +199,11 @@ int validate_params(const hx_build_params* p) {
         HX_CHECK_ARG(p->hlbvh_sah_bits >= 6 && p->hlbvh_sah_bits <= 15, "hlbvh_sah_bits must be in [6,15] (got %d)",
                      p->hlbvh_sah_bits);
         HX_CHECK_ARG(p->hlbvh_sah_bits <= p->morton_bits, "hlbvh_sah_bits must not exceed morton_bits");
+    }
+    if (p->method == HX_METHOD_INSERTION) {
+        // InsertionBuilder::setMod (InsertionBuilder.cpp:491-494).
+        HX_CHECK_ARG(p->insertion_mod >= 1 && p->insertion_mod <= 64, "insertion_mod must be in [1,64] (got %d)",
+                     p->insertion_mod);
     }
     if (p->method == HX_METHOD_ATRBVH) {
         // ATRBuilder::setTreeletSize / setIterations (ATRBuilder.cpp:273-290).
@@ -226,6 +232,13 @@ size_t atr_work_bytes(int64_t n) {
 size_t hlbvh_work_bytes(int64_t n) {
     const size_t K = (size_t)hx::kHlbvhMaxClusters;
     return (size_t)n * 8 + K * (4 * 8 + 16 * 4) + 8192;
+}
+
+// Insertion workspace: out, red, locks and the best tree (2n-1 entries
+// each), refs, height and the height list (n each), bucket counts, counters.
+size_t ins_work_bytes(int64_t n) {
+    const size_t N = (size_t)(2 * n);
+    return N * (4 + 4 + 8 + 12) + (size_t)n * 12 + 2 * sizeof(int32_t) * (2 * hx::kAtrMaxLevels + 1) + 8192;
 }
 
 int reserve_bvh(hx_bvh* b, int64_t n) {
@@ -276,6 +289,10 @@ int reserve_bvh(hx_bvh* b, int64_t n) {
     if (b->params.method == HX_METHOD_HLBVH) {
         HX_TRY(b->split.reserve(4 * (size_t)n));
         HX_TRY(b->hlbvh_work.reserve(hlbvh_work_bytes(n)));
+    }
+    if (b->params.method == HX_METHOD_INSERTION) {
+        HX_TRY(b->split.reserve(4 * (size_t)n));
+        HX_TRY(b->ins_work.reserve(ins_work_bytes(n)));
     }
     return HX_OK;
 }
@@ -482,6 +499,65 @@ int run_atr(hx_bvh* b) {
     return hx::launch_collapse_emit(c, emit_args(b));
 }
 
+// InsertionBuilder::build with the LBVH start (InsertionBuilder.cpp:379-440):
+// Karras hierarchy and its boxes, the insertion loop, then the depth-first
+// relabelling (by height), the general climb and the collapse.
+int run_insertion(hx_bvh* b) {
+    Context* c = &b->ctx->c;
+    const hx_build_params& p = b->params;
+    const int64_t n = b->n_refs;
+    HX_TRY(hx::launch_construct(c, b->keys[b->keys_index].ptr, b->keys64, n, b->left.as<int32_t>(),
+                                b->right.as<int32_t>(), b->parent.as<int32_t>(), b->first.as<int32_t>(),
+                                b->last.as<int32_t>(), b->counters.as<int32_t>(), b->cover.as<int32_t>(),
+                                b->counts.as<int32_t>() + 2));
+    hx::ClimbArgs ca = climb_args(b, true);
+    ca.split = nullptr;  // still a Karras tree
+    HX_TRY(hx::launch_climb(c, ca));
+    char* w = b->ins_work.as<char>();
+    auto carve = [&](size_t bytes) {
+        char* q = w;
+        w += (bytes + 255) & ~(size_t)255;
+        return q;
+    };
+    const size_t N = (size_t)(2 * n - 1);
+    hx::InsArgs a{};
+    a.n = n;
+    a.mod = p.insertion_mod;
+    a.ct = p.ct;
+    a.ci = p.ci;
+    a.left = b->left.as<int32_t>();
+    a.right = b->right.as<int32_t>();
+    a.parent = b->parent.as<int32_t>();
+    a.nmin = b->nmin.as<float4>();
+    a.nmax = b->nmax.as<float4>();
+    a.counters = b->counters.as<int32_t>();
+    a.out = reinterpret_cast<int32_t*>(carve(4 * N));
+    a.red = reinterpret_cast<float*>(carve(4 * N));
+    a.locks = reinterpret_cast<unsigned long long*>(carve(8 * N));
+    a.best_left = reinterpret_cast<int32_t*>(carve(4 * N));
+    a.best_right = reinterpret_cast<int32_t*>(carve(4 * N));
+    a.best_parent = reinterpret_cast<int32_t*>(carve(4 * N));
+    a.acc = reinterpret_cast<unsigned long long*>(carve(32));
+    a.refs = reinterpret_cast<int32_t*>(carve(4 * (size_t)n));
+    a.height = reinterpret_cast<int32_t*>(carve(4 * (size_t)n));
+    HX_TRY(hx::launch_insertion(c, a, &b->ins_steps, &b->ins_reinserted));
+    hx::AtrArgs rel{};
+    rel.n = n;
+    rel.refs = a.refs;
+    rel.height = a.height;
+    rel.list = reinterpret_cast<int32_t*>(carve(4 * (size_t)n));
+    rel.hist = reinterpret_cast<int32_t*>(carve(4 * (size_t)(2 * hx::kAtrMaxLevels + 1)));
+    rel.cursor = reinterpret_cast<int32_t*>(carve(4 * (size_t)(2 * hx::kAtrMaxLevels + 1)));
+    HX_TRY(hx::launch_relabel_by_height(c, rel, relabel_args(b)));
+    b->general = true;
+    std::swap(b->tri_idx.ptr, b->vals_tmp.ptr);
+    std::swap(b->tri_idx.bytes, b->vals_tmp.bytes);
+    HX_TRY(hx::launch_refit_reset(c, b->counters.as<int32_t>(), n - 1, b->counts.as<int32_t>() + 2));
+    HX_TRY(hx::launch_fill_i32(c, b->cover.as<int32_t>(), -1, n));
+    HX_TRY(hx::launch_climb(c, climb_args(b, false)));
+    return hx::launch_collapse_emit(c, emit_args(b));
+}
+
 // Batched LBVH: per-mesh boxes and Morton codes, (mesh, Morton) order from
 // two stable sorts, one Karras construction over the composite key, then the
 // shared climb and collapse with per-mesh tables.
@@ -618,6 +694,8 @@ int run_build(hx_bvh* b) {
         HX_TRY(run_atr(b));
     } else if (p.method == HX_METHOD_HLBVH) {
         HX_TRY(run_hlbvh(b));
+    } else if (p.method == HX_METHOD_INSERTION) {
+        HX_TRY(run_insertion(b));
     } else {
         HX_TRY(hx::launch_construct(c, b->keys[b->keys_index].ptr, b->keys64, n, b->left.as<int32_t>(), b->right.as<int32_t>(),
                                     b->parent.as<int32_t>(), b->first.as<int32_t>(), b->last.as<int32_t>(),
@@ -1122,6 +1200,10 @@ int hx_bvh_info(hx_bvh* bvh, const char* key, int64_t* value) {
         *value = bvh->atr_accepted;
     } else if (!std::strcmp(key, "hlbvh_clusters")) {
         *value = bvh->hlbvh_clusters;
+    } else if (!std::strcmp(key, "insertion_steps")) {
+        *value = bvh->ins_steps;
+    } else if (!std::strcmp(key, "insertion_reinserted")) {
+        *value = bvh->ins_reinserted;
     } else if (!std::strcmp(key, "method")) {
         *value = bvh->params.method;
     } else {

@@ -348,7 +348,7 @@ int bucket_by_height(Context* c, const AtrArgs& a, int gamma, std::vector<int>& 
     HX_HIP(hipMemcpyAsync(counts.data(), a.hist, sizeof(int) * (kBuckets + 1), hipMemcpyDeviceToHost, c->stream));
     HX_HIP(hipStreamSynchronize(c->stream));
     if (counts[kBuckets] != 0) {
-        set_error("ATRBVH: a subtree is deeper than %d levels", kAtrMaxLevels - 1);
+        set_error("a subtree is deeper than %d levels", kAtrMaxLevels - 1);
         return HX_ERR_UNSUPPORTED;
     }
     std::vector<int> offs(kBuckets, 0);
@@ -367,6 +367,23 @@ int bucket_by_height(Context* c, const AtrArgs& a, int gamma, std::vector<int>& 
 }
 
 }  // namespace
+
+int launch_relabel_by_height(Context* c, const AtrArgs& a, const PlocArgs& relabel) {
+    // Depth-first relabelling, parents first: buckets by height, top down.
+    LaunchScope ls(c, "relabel");
+    std::vector<int> counts;
+    HX_TRY(bucket_by_height(c, a, 0, counts));
+    HX_HIP(hipMemcpyAsync(relabel.last, a.refs, sizeof(int32_t) * (size_t)(a.n - 1), hipMemcpyDeviceToDevice,
+                          c->stream));
+    std::vector<int> offs(kBuckets + 1, 0);
+    for (int b = 0; b < kBuckets; ++b) offs[b + 1] = offs[b] + counts[b];
+    // Both buckets of a height are contiguous: one launch per height.
+    for (int h = kAtrMaxLevels - 1; h >= 0; --h) {
+        const int cnt = counts[2 * h] + counts[2 * h + 1];
+        if (cnt > 0) HX_TRY(launch_tree_down(c, relabel, a.list + offs[2 * h], cnt));
+    }
+    return HX_OK;
+}
 
 int launch_atr(Context* c, const AtrArgs& a, const PlocArgs& relabel, int64_t* accepted_out) {
     const int64_t n = a.n;
@@ -397,20 +414,7 @@ int launch_atr(Context* c, const AtrArgs& a, const PlocArgs& relabel, int64_t* a
             if (gamma > n) break;
         }
     }
-    {
-        // Depth-first relabelling, parents first: buckets by height, top down.
-        LaunchScope ls(c, "atr_relabel");
-        HX_TRY(bucket_by_height(c, a, 0, counts));
-        HX_HIP(hipMemcpyAsync(relabel.last, a.refs, sizeof(int32_t) * (size_t)(n - 1), hipMemcpyDeviceToDevice,
-                              c->stream));
-        std::vector<int> offs(kBuckets + 1, 0);
-        for (int b = 0; b < kBuckets; ++b) offs[b + 1] = offs[b] + counts[b];
-        // Both buckets of a height are contiguous: one launch per height.
-        for (int h = kAtrMaxLevels - 1; h >= 0; --h) {
-            const int cnt = counts[2 * h] + counts[2 * h + 1];
-            if (cnt > 0) HX_TRY(launch_tree_down(c, relabel, a.list + offs[2 * h], cnt));
-        }
-    }
+    HX_TRY(launch_relabel_by_height(c, a, relabel));
     if (accepted_out) {
         int32_t acc = 0;
         HX_HIP(hipMemcpyAsync(&acc, a.accepted, sizeof(acc), hipMemcpyDeviceToHost, c->stream));

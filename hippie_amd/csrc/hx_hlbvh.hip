@@ -113,7 +113,11 @@ __global__ __launch_bounds__(kBlock) void k_hl_clusters(HlbvhArgs a, int shift, 
 }
 
 // Cluster roots (the Karras node, or leaf, whose range is the run) and the
-// per-tile counts of the upper Karras nodes (ranges spanning clusters).
+// per-tile counts of the upper Karras nodes (ranges spanning clusters).  A
+// lower node records its cluster as split = ~cluster: the upper levels
+// rewrite first/last of the upper nodes, so the final shift cannot tell the
+// two kinds apart by their ranges any more (every upper node gets a split >=
+// 0 from its partition).
 __global__ __launch_bounds__(kBlock) void k_hl_roots(HlbvhArgs a, int32_t* __restrict__ partial) {
     const int n = (int)a.n;
     const int64_t base = blockIdx.x * (int64_t)kTile;
@@ -127,8 +131,12 @@ __global__ __launch_bounds__(kBlock) void k_hl_roots(HlbvhArgs a, int32_t* __res
         if (i < n - 1) {
             const int f = a.first[i], l = a.last[i];
             const int c = a.cl_of[f];
-            if (a.cl_of[l] != c) ++cnt;
-            else if (a.ca[c] == f && a.cb[c] == l) a.croot[c] = (int)i;
+            if (a.cl_of[l] != c) {
+                ++cnt;
+            } else {
+                a.split[i] = ~c;
+                if (a.ca[c] == f && a.cb[c] == l) a.croot[c] = (int)i;
+            }
         }
     }
     cnt = wave_sum_i32(cnt);
@@ -397,9 +405,10 @@ __global__ __launch_bounds__(kBlock) void k_hl_lower(HlbvhArgs a, int pass) {
             a.tmp[i] = (a.ca[c] == i && a.cb[c] == i) ? a.cl_par[c] : a.parent[n - 1 + i];
         }
         if (i < n - 1) {
+            const int sp = a.split[i];
+            if (sp >= 0) return;  // upper node: already placed
+            const int c = ~sp;
             const int f = a.first[i], l = a.last[i];
-            const int c = a.cl_of[f];
-            if (a.cl_of[l] != c) return;  // upper node: already placed
             const int sh = a.cl_off[c] - a.ca[c];
             const int lc = a.left[i], rc = a.right[i];
             const int gamma = lc >= n - 1 ? lc - (n - 1) : lc;

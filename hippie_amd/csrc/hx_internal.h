@@ -249,6 +249,38 @@ struct AtrArgs {
 constexpr int kAtrMaxLevels = 1024;
 // relabel: fields of a PLOC argument block used by launch_tree_down.
 int launch_atr(Context* c, const AtrArgs& a, const PlocArgs& relabel, int64_t* accepted_out);
+// Depth-first relabelling of any binary tree whose internal nodes carry leaf
+// counts (a.refs) and heights (a.height): nodes are bucketed by height and
+// placed top down.  Uses a.n, refs, height, list, hist, cursor.
+int launch_relabel_by_height(Context* c, const AtrArgs& a, const PlocArgs& relabel);
+
+// hx_insertion.hip: insertion optimisation of a binary tree with boxes (root
+// 0, leaves n-1..2n-2), the reference's step loop run from the host.  On
+// return the tree is the best one seen, refitted, with refs and heights of
+// its internal nodes.  n >= 2.
+constexpr int kInsMaxSteps = 1000;
+struct InsArgs {
+    int64_t n;
+    int mod;
+    float ct, ci;
+    int32_t* left;
+    int32_t* right;
+    int32_t* parent;
+    float4* nmin;
+    float4* nmax;
+    int32_t* counters;        // n-1 climb counters
+    int32_t* out;             // 2n-1 best new sibling per node
+    float* red;               // 2n-1 its area reduction
+    unsigned long long* locks;  // 2n-1
+    int32_t* best_left;       // 2n-1 each: the best tree so far
+    int32_t* best_right;
+    int32_t* best_parent;
+    unsigned long long* acc;  // [0] cost (2^-32 units), [1] moves found, [2] moves applied
+    int32_t* refs;            // n-1 leaf counts (filled by the final refit)
+    int32_t* height;          // n-1 heights
+};
+int launch_insertion(Context* c, const InsArgs& a, int64_t* steps_out, int64_t* reinserted_out);
+int launch_ins_refit(Context* c, const InsArgs& a, bool sizes);
 
 // hx_hlbvh.hip: clusters of equal top-sah_bits Morton prefixes (Karras
 // subtrees), binned-SAH upper levels over them, depth-first relabelling.

@@ -1484,6 +1484,335 @@ int64_t hxo_build_hlbvh(const float* verts, const int32_t* tris, int64_t n, cons
     return nn;
 }
 
+/* ------------------------------------------------------------------------ */
+/* Insertion (InsertionBuilder.cpp:22-274, InsertionBuilderKernels.cu:11-515):
+ * parallel reinsertion.  Each step, every node i > 0 with i = mod*k + step
+ * finds its best new sibling by the branch-and-bound search of findBestNode;
+ * a node keeps its move only if it holds the lock (max of area-reduction bits
+ * and index) on every node the move touches; kept moves are disjoint and are
+ * applied; the tree is refit and its SAH measured.  The loop ends when the
+ * cost stops changing (|dC| <= 0.01) at mod 1, and the best tree is kept.
+ * The cost sum is FP64 in node order (the reference adds floats atomically). */
+}  // extern "C"
+
+namespace {
+
+/* Termination guard: the reference loops until convergence. */
+constexpr int kInsMaxSteps = 1000;
+
+struct InsTree {
+    int64_t n, N;
+    std::vector<int32_t> left, right, parent;
+    std::vector<float> nmin, nmax;  /* float4 per node */
+    Box box(int32_t v) const {
+        Box b;
+        b.mn = load4(nmin.data() + 4 * (int64_t)v);
+        b.mx = load4(nmax.data() + 4 * (int64_t)v);
+        return b;
+    }
+};
+
+inline Box box_union(const Box& a, const Box& b) {
+    Box r;
+    r.grow(a.mn);
+    r.grow(a.mx);
+    r.grow(b.mn);
+    r.grow(b.mx);
+    return r;
+}
+
+/* findBestNode (:11-131) for one node. */
+void ins_find_best(const InsTree& T, int32_t in, int32_t& best_out, float& best_red) {
+    const int32_t p = T.parent[in];
+    const Box inBox = T.box(in);
+    const float pArea = T.box(p).area();
+    const float bound = pArea - inBox.area();
+    Box highest;
+    const int32_t sib = T.left[p] == in ? T.right[p] : T.left[p];
+    int32_t out = sib, highestIdx = p;
+    bool down = true;
+    int32_t best = -1;
+    float bestRed = 0.0f, red = 0.0f;
+    while (true) {
+        Box outBox = T.box(out);
+        Box merged = box_union(inBox, outBox);
+        if (down) {
+            if (out != sib) {
+                const float direct = pArea - merged.area();
+                if (bestRed < direct + red) {
+                    bestRed = direct + red;
+                    best = out;
+                }
+            }
+            const float cur = outBox.area() - merged.area();
+            red += cur;
+            if (out >= T.n - 1 || red + bound <= bestRed) down = false;
+            else out = T.left[out];
+        } else {
+            int32_t op = T.parent[out];
+            const float cur = outBox.area() - merged.area();
+            red -= cur;
+            if (op == highestIdx) {
+                highest.grow(outBox.mn);
+                highest.grow(outBox.mx);
+                out = op;
+                if (out != p && out != 0) {
+                    merged = box_union(inBox, highest);
+                    const float direct = pArea - merged.area();
+                    if (bestRed < direct + red) {
+                        bestRed = direct + red;
+                        best = out;
+                    }
+                    outBox = T.box(out);
+                    const float cur2 = outBox.area() - highest.area();
+                    red += cur2;
+                }
+                op = T.parent[out];
+                if (op < 0) break;
+                highestIdx = op;
+                down = true;
+                out = T.left[highestIdx] == out ? T.right[highestIdx] : T.left[highestIdx];
+            } else {
+                if (T.left[op] == out) {
+                    down = true;
+                    out = T.right[op];
+                } else {
+                    out = op;
+                }
+            }
+        }
+    }
+    best_out = best;
+    best_red = bestRed;
+}
+
+/* The nodes a move of `in` to `out` touches (lockNodes, :133-185). */
+int ins_lock_set(const InsTree& T, int32_t in, int32_t out, int32_t set[8]) {
+    int k = 0;
+    const int32_t p = T.parent[in];
+    const int32_t sib = T.left[p] == in ? T.right[p] : T.left[p];
+    set[k++] = in;
+    set[k++] = sib;
+    set[k++] = p;
+    if (p > 0) {
+        set[k++] = T.parent[p];
+    } else if (sib < T.n - 1) {
+        set[k++] = T.left[sib];
+        set[k++] = T.right[sib];
+    }
+    set[k++] = out;
+    if (out > 0) {
+        set[k++] = T.parent[out];
+    } else {
+        set[k++] = T.left[out];
+        set[k++] = T.right[out];
+    }
+    return k;
+}
+
+/* reinsert (:237-300). */
+void ins_reinsert(InsTree& T, int32_t in, int32_t out) {
+    int32_t p = T.parent[in];
+    const int32_t sib = T.left[p] == in ? T.right[p] : T.left[p];
+    if (p != 0) {
+        const int32_t pp = T.parent[p];
+        if (T.left[pp] == p) T.left[pp] = sib;
+        else T.right[pp] = sib;
+        T.parent[sib] = pp;
+    } else {
+        const int32_t sl = T.left[sib], sr = T.right[sib];
+        T.left[0] = sl;
+        T.right[0] = sr;
+        T.parent[sl] = 0;
+        T.parent[sr] = 0;
+        p = sib;
+    }
+    if (out != 0) {
+        const int32_t op = T.parent[out];
+        if (T.left[op] == out) T.left[op] = p;
+        else T.right[op] = p;
+        T.parent[p] = op;
+        T.left[p] = out;
+        T.right[p] = in;
+        T.parent[out] = p;
+        T.parent[in] = p;
+    } else {
+        const int32_t ol = T.left[0], orr = T.right[0];
+        T.left[p] = ol;
+        T.right[p] = orr;
+        T.parent[ol] = p;
+        T.parent[orr] = p;
+        T.parent[p] = 0;
+        T.parent[in] = 0;
+        T.left[0] = in;
+        T.right[0] = p;
+    }
+}
+
+void ins_refit(InsTree& T) {
+    std::vector<int32_t> order;
+    order.reserve(T.N);
+    std::vector<int32_t> st(1, 0);
+    while (!st.empty()) {
+        const int32_t v = st.back();
+        st.pop_back();
+        order.push_back(v);
+        if (v < T.n - 1) {
+            st.push_back(T.left[v]);
+            st.push_back(T.right[v]);
+        }
+    }
+    for (int64_t k = (int64_t)order.size() - 1; k >= 0; --k) {
+        const int32_t v = order[k];
+        if (v >= T.n - 1) continue;
+        const Box l = T.box(T.left[v]), r = T.box(T.right[v]);
+        Box b;
+        b.grow(l.mn);
+        b.grow(r.mn);
+        b.grow(l.mx);
+        b.grow(r.mx);
+        store4(T.nmin.data() + 4 * (int64_t)v, b.mn);
+        store4(T.nmax.data() + 4 * (int64_t)v, b.mx);
+    }
+}
+
+/* computeCost (:457-515) as an exact sum: each node's term c * (area / scene
+ * area) is truncated to a multiple of 2^-32 and the multiples are added as
+ * integers (modulo 2^64), so the total does not depend on the order of the
+ * additions (the reference adds float block sums atomically).  A term that is
+ * not a number in [0, 1e9] counts as 0. */
+uint64_t ins_fixed(float t) {
+    if (!(t >= 0.0f) || !(t <= 1.0e9f)) return 0;
+    return (uint64_t)((double)t * 4294967296.0);
+}
+
+double ins_cost(const InsTree& T, float sceneArea, float ct, float ci) {
+    uint64_t s = 0;
+    for (int64_t v = 0; v < T.N; ++v) {
+        const float P = T.box((int32_t)v).area() / sceneArea;
+        s += ins_fixed((v >= T.n - 1 ? ci : ct) * P);
+    }
+    return (double)s * (1.0 / 4294967296.0);
+}
+
+}  // namespace
+
+extern "C" {
+
+void hxo_ins_find(int64_t n, const int32_t* left, const int32_t* right, const int32_t* parent, const float* nmin,
+                  const float* nmax, int32_t* out, float* red) {
+    const int64_t N = 2 * n - 1;
+    InsTree T{n, N, std::vector<int32_t>(left, left + N), std::vector<int32_t>(right, right + N),
+              std::vector<int32_t>(parent, parent + N), std::vector<float>(nmin, nmin + 4 * N),
+              std::vector<float>(nmax, nmax + 4 * N)};
+    out[0] = -1;
+    red[0] = 0.0f;
+    for (int64_t i = 1; i < N; ++i) ins_find_best(T, (int32_t)i, out[i], red[i]);
+}
+
+int64_t hxo_insertion(int64_t n, int mod, float ct, float ci, int32_t* left, int32_t* right, int32_t* parent,
+                      float* nmin, float* nmax, int64_t* steps_out) {
+    if (n < 2) {
+        if (steps_out) *steps_out = 0;
+        return 0;
+    }
+    const int64_t N = 2 * n - 1;
+    InsTree T{n, N, std::vector<int32_t>(left, left + N), std::vector<int32_t>(right, right + N),
+              std::vector<int32_t>(parent, parent + N), std::vector<float>(nmin, nmin + 4 * N),
+              std::vector<float>(nmax, nmax + 4 * N)};
+    const float sceneArea = T.box(0).area();
+    std::vector<int32_t> bl = T.left, br = T.right, bp = T.parent;
+    int steps = 0, modCur = std::max(mod, 1);
+    double cost = 3.402823466e+38, bestCost = 3.402823466e+38, prevCost;
+    int64_t total = 0;
+    std::vector<int32_t> outv(N, -1);
+    std::vector<float> redv(N, 0.0f);
+    std::vector<uint64_t> locks(N);
+    while (steps < kInsMaxSteps) {
+        prevCost = cost;
+        const int rem = steps % modCur;
+        const int64_t cnt = (N - rem + modCur - 1) / modCur;
+        parallel_for(cnt, [&](int64_t k) {
+            const int64_t i = rem + k * modCur;
+            outv[i] = -1;
+            if (i > 0) ins_find_best(T, (int32_t)i, outv[i], redv[i]);
+        });
+        std::fill(locks.begin(), locks.end(), 0);
+        int32_t set[8];
+        for (int64_t i = rem; i < N; i += modCur) {
+            if (i == 0 || outv[i] < 0) continue;
+            uint32_t rb;
+            std::memcpy(&rb, &redv[i], 4);
+            const uint64_t key = ((uint64_t)rb << 32) | (uint64_t)i;
+            const int k = ins_lock_set(T, (int32_t)i, outv[i], set);
+            for (int j = 0; j < k; ++j) locks[set[j]] = std::max(locks[set[j]], key);
+        }
+        std::vector<int32_t> kept;
+        for (int64_t i = rem; i < N; i += modCur) {
+            if (i == 0 || outv[i] < 0) continue;
+            uint32_t rb;
+            std::memcpy(&rb, &redv[i], 4);
+            const uint64_t key = ((uint64_t)rb << 32) | (uint64_t)i;
+            const int k = ins_lock_set(T, (int32_t)i, outv[i], set);
+            bool ok = true;
+            for (int j = 0; j < k; ++j) ok = ok && locks[set[j]] == key;
+            if (ok) kept.push_back((int32_t)i);
+        }
+        for (int32_t i : kept) ins_reinsert(T, i, outv[i]);
+        total += (int64_t)kept.size();
+        ins_refit(T);
+        cost = ins_cost(T, sceneArea, ct, ci);
+        if (bestCost > cost) {
+            bestCost = cost;
+            bl = T.left;
+            br = T.right;
+            bp = T.parent;
+        }
+        ++steps;
+        const double BETA = 0.01;
+        if ((std::fabs(prevCost - cost) <= BETA || kept.empty()) && modCur == 1) break;
+        if (std::fabs(prevCost - cost) <= BETA) modCur = std::max(1, modCur - 1);
+    }
+    T.left = bl;
+    T.right = br;
+    T.parent = bp;
+    ins_refit(T);
+    std::copy(T.left.begin(), T.left.end(), left);
+    std::copy(T.right.begin(), T.right.end(), right);
+    std::copy(T.parent.begin(), T.parent.end(), parent);
+    std::copy(T.nmin.begin(), T.nmin.end(), nmin);
+    std::copy(T.nmax.begin(), T.nmax.end(), nmax);
+    if (steps_out) *steps_out = steps;
+    return total;
+}
+
+int64_t hxo_build_insertion(const float* verts, const int32_t* tris, int64_t n, const float* scene_box, int bits,
+                            int mod, int adaptive, int max_leaf_size, float ct, float ci, int32_t* nodes,
+                            int32_t* tri_out, float* woop, int64_t* n_interior, int64_t* n_leaves,
+                            int64_t* steps_out) {
+    float sb[6];
+    if (scene_box) std::memcpy(sb, scene_box, sizeof(sb));
+    else hxo_scene_box(verts, tris, n, sb);
+    std::vector<float> rmin(4 * n), rmax(4 * n);
+    hxo_setup_boxes(verts, tris, n, rmin.data(), rmax.data());
+    std::vector<uint64_t> keys(n);
+    std::vector<int32_t> vals(n);
+    hxo_morton_codes(rmin.data(), rmax.data(), n, sb, bits, keys.data());
+    for (int64_t i = 0; i < n; ++i) vals[i] = (int32_t)i;
+    hxo_sort_pairs(keys.data(), vals.data(), n, bits);
+    const int64_t N = std::max<int64_t>(2 * n - 1, 1);
+    std::vector<int32_t> left(N), right(N), parent(N);
+    std::vector<float> nmin(4 * N), nmax(4 * N);
+    if (n > 1) hxo_construct(keys.data(), n, left.data(), right.data(), parent.data());
+    hxo_refit(vals.data(), rmin.data(), rmax.data(), n, left.data(), right.data(), parent.data(), nmin.data(),
+              nmax.data());
+    hxo_insertion(n, mod, ct, ci, left.data(), right.data(), parent.data(), nmin.data(), nmax.data(), steps_out);
+    int64_t nn = hxo_collapse(n, vals.data(), left.data(), right.data(), parent.data(), nmin.data(), nmax.data(),
+                              adaptive, max_leaf_size, ct, ci, nodes, tri_out, n_interior, n_leaves);
+    hxo_woopify(verts, tris, tri_out, n, woop);
+    return nn;
+}
+
 /* ATRBuilder::build (ATRBuilder.cpp:179-236): LBVH front half and hierarchy,
  * refit, ATR optimisation, collapse, woopify. */
 int64_t hxo_build_atr(const float* verts, const int32_t* tris, int64_t n, const float* scene_box, int bits,

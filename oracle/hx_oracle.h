@@ -155,6 +155,24 @@ int64_t hxo_build_hlbvh(const float* verts, const int32_t* tris, int64_t n, cons
                         int sah_bits, int adaptive, int max_leaf_size, float ct, float ci, int32_t* nodes,
                         int32_t* tri_out, float* woop, int64_t* n_interior, int64_t* n_leaves, int64_t* clusters);
 
+/* findBestNode (InsertionBuilderKernels.cu:11-131) for every node 1..2n-2 of a
+ * binary tree: out[i] the best new sibling (-1 if none), red[i] its area
+ * reduction. */
+void hxo_ins_find(int64_t n, const int32_t* left, const int32_t* right, const int32_t* parent, const float* nmin,
+                  const float* nmax, int32_t* out, float* red);
+
+/* Insertion optimisation (InsertionBuilder.cpp:22-274, InsertionBuilderKernels.cu
+ * :11-515) of a binary tree in place; returns the number of reinsertions,
+ * *steps_out the number of steps. */
+int64_t hxo_insertion(int64_t n, int mod, float ct, float ci, int32_t* left, int32_t* right, int32_t* parent,
+                      float* nmin, float* nmax, int64_t* steps_out);
+
+/* InsertionBuilder::build with the LBVH start (InsertionBuilder.cpp:400-440). */
+int64_t hxo_build_insertion(const float* verts, const int32_t* tris, int64_t n, const float* scene_box, int bits,
+                            int mod, int adaptive, int max_leaf_size, float ct, float ci, int32_t* nodes,
+                            int32_t* tri_out, float* woop, int64_t* n_interior, int64_t* n_leaves,
+                            int64_t* steps_out);
+
 /* Ray Morton keys: rt/ray/RayBufferKernels.cu:556-614 (Aila64) and
  * :492-554 (TwoPoint64) with the ray box fixed to [0,1]^3
  * (rt/ray/RayBuffer.cpp:63-66). */

@@ -59,6 +59,10 @@ def lib():
                                        _p, _p, _p, _p, _p, _p]),
             "hxo_build_ploc": (_i64, [_p, _p, _i64, _p, _i32, _i32, _i32, _i32, _i32, _i32, _f32, _f32,
                                       _p, _p, _p, _p, _p, _p]),
+            "hxo_ins_find": (None, [_i64, _p, _p, _p, _p, _p, _p, _p]),
+            "hxo_insertion": (_i64, [_i64, _i32, _f32, _f32, _p, _p, _p, _p, _p, _p]),
+            "hxo_build_insertion": (_i64, [_p, _p, _i64, _p, _i32, _i32, _i32, _i32, _f32, _f32,
+                                           _p, _p, _p, _p, _p, _p]),
             "hxo_ray_keys_aila64": (None, [_p, _i64, _i32, _p]),
             "hxo_ray_keys_twopoint64": (None, [_p, _i64, _i32, _f32, _p]),
         }
@@ -310,6 +314,45 @@ def build_hlbvh(verts, tris, bits=60, sah_bits=15, scene_box=None, adaptive=True
                                ctypes.byref(ni), ctypes.byref(nl), ctypes.byref(cl))
     return dict(nodes=nodes[:nn].copy(), tri_idx=tri, woop=woop, n_interior=ni.value, n_leaves=nl.value,
                 clusters=cl.value)
+
+
+def ins_find(left, right, parent, nmin, nmax):
+    """findBestNode for every node of a binary tree; returns out (int32), reduction (float32)."""
+    l, r, p = (_c(a, np.int32) for a in (left, right, parent))
+    mn, mx = _c(nmin, np.float32), _c(nmax, np.float32)
+    N = l.shape[0]
+    n = (N + 1) // 2
+    out = np.zeros(N, np.int32)
+    red = np.zeros(N, np.float32)
+    lib().hxo_ins_find(n, _ptr(l), _ptr(r), _ptr(p), _ptr(mn), _ptr(mx), _ptr(out), _ptr(red))
+    return out, red
+
+
+def insertion(left, right, parent, nmin, nmax, mod=1, ct=3.0, ci=2.0):
+    """Insertion optimisation of a binary tree (copies); returns left, right, parent, nmin, nmax, reinserted, steps."""
+    l, r, p = (_c(a, np.int32).copy() for a in (left, right, parent))
+    mn, mx = _c(nmin, np.float32).copy(), _c(nmax, np.float32).copy()
+    n = (l.shape[0] + 1) // 2
+    steps = ctypes.c_int64(0)
+    k = lib().hxo_insertion(n, int(mod), float(ct), float(ci), _ptr(l), _ptr(r), _ptr(p), _ptr(mn), _ptr(mx),
+                            ctypes.byref(steps))
+    return l, r, p, mn, mx, k, steps.value
+
+
+def build_insertion(verts, tris, bits=60, mod=1, scene_box=None, adaptive=True, max_leaf_size=8, ct=3.0, ci=2.0):
+    v, t = _c(verts, np.float32), _c(tris, np.int32)
+    n = t.shape[0]
+    N = max(2 * n - 1, 2)
+    nodes = np.zeros((N, 16), np.int32)
+    tri = np.zeros(n, np.int32)
+    woop = np.zeros((n, 12), np.float32)
+    ni, nl, steps = ctypes.c_int64(0), ctypes.c_int64(0), ctypes.c_int64(0)
+    sb = None if scene_box is None else _c(scene_box, np.float32)
+    nn = lib().hxo_build_insertion(_ptr(v), _ptr(t), n, _ptr(sb), int(bits), int(mod), int(bool(adaptive)),
+                                   int(max_leaf_size), float(ct), float(ci), _ptr(nodes), _ptr(tri), _ptr(woop),
+                                   ctypes.byref(ni), ctypes.byref(nl), ctypes.byref(steps))
+    return dict(nodes=nodes[:nn].copy(), tri_idx=tri, woop=woop, n_interior=ni.value, n_leaves=nl.value,
+                steps=steps.value)
 
 
 def ray_keys_aila64(rays, bits=60):
