@@ -10,8 +10,9 @@ seeded synthetic data resident in HBM before timing starts.
 value = build throughput in Mtris/s; trace throughput is reported beside it.
 
 Also measured in the same run (reported beside `value`):
-  builders  PLOC (radius 25) and ATRBVH (treelet 9, 2 iterations) rebuilds of
-            the same scene, with their SAH cost against the LBVH's;
+  builders  PLOC (radius 25), ATRBVH (treelet 9, 2 iterations), HLBVH (15 SAH
+            bits) and insertion (mod 1) rebuilds of the same scene, with their
+            SAH cost against the LBVH's;
   batched   4096 meshes of round(exp(U[ln 64, ln 262144])) triangles (half
             displaced icospheres, half soups, each in its own unit box): one
             batched build against a loop of per-mesh rebuilds, and one
@@ -47,7 +48,7 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=0, help="triangles in the CPU baseline sample (0 = all)")
     ap.add_argument("--cpu-rays", type=int, default=4194304, help="rays in the CPU trace sample")
-    ap.add_argument("--builders", default="ploc,atr", help="other builders to time on the same scene ('' = none)")
+    ap.add_argument("--builders", default="ploc,atr,hlbvh,insertion", help="other builders to time on the same scene ('' = none)")
     ap.add_argument("--batch-meshes", type=int, default=4096, help="meshes in the batched workload (0 = skip)")
     ap.add_argument("--batch-lo", type=int, default=64)
     ap.add_argument("--batch-hi", type=int, default=262144)
@@ -106,22 +107,30 @@ def timed(fn, steps, warmup):
 
 
 def bench_builders(ctx, scene, n, args):
-    """PLOC r=25 and ATRBVH t=9 x2 rebuilds of the benchmark scene, with SAH costs."""
+    """PLOC r=25, ATRBVH t=9 x2, HLBVH and insertion rebuilds of the benchmark scene, with SAH costs."""
     import hippie_amd as hx
     out = {}
     kinds = [k for k in args.builders.split(",") if k]
     specs = {"ploc": ("ploc_r25", hx.PLOCBuilder, dict(ploc_radius=25)),
-             "atr": ("atr_t9_i2", hx.ATRBuilder, dict(atr_treelet_size=9, atr_iterations=2))}
+             "atr": ("atr_t9_i2", hx.ATRBuilder, dict(atr_treelet_size=9, atr_iterations=2)),
+             "hlbvh": ("hlbvh_s15", hx.HLBVHBuilder, dict(hlbvh_sah_bits=15)),
+             "insertion": ("insertion_m1", hx.InsertionBuilder, dict(insertion_mod=1))}
     for k in kinds:
         name, cls, kw = specs[k]
         b = cls(ctx, morton_bits=60, **kw)
         bvh = b.build(scene)
-        ms = timed(lambda: b.rebuild(bvh, scene), max(1, args.steps // 2), 1)
+        reps = 1 if k == "insertion" else max(1, args.steps // 2)
+        ms = timed(lambda: b.rebuild(bvh, scene), reps, 0 if k == "insertion" else 1)
         out[name] = {"ms": round(ms, 3), "mtris_s": round(n / (ms * 1e-3) / 1e6, 2), "sah": round(bvh.get_cost(), 4)}
         if k == "atr":
             out[name]["treelets_kept"] = bvh.info("atr_accepted")
         if k == "ploc":
             out[name]["rounds"] = bvh.info("ploc_rounds")
+        if k == "hlbvh":
+            out[name]["clusters"] = bvh.info("hlbvh_clusters")
+        if k == "insertion":
+            out[name]["steps"] = bvh.info("insertion_steps")
+            out[name]["reinserted"] = bvh.info("insertion_reinserted")
         log(f"{name}: {ms:.3f} ms ({out[name]['mtris_s']} Mtris/s), SAH {out[name]['sah']}")
         del bvh
     return out

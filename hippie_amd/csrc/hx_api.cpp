@@ -126,7 +126,8 @@ struct hx_batch;
 struct hx_bvh {
     hx_context* ctx = nullptr;
     hx_build_params params{};
-    int64_t n_refs = 0;
+    int64_t n_refs = 0;   // references (leaf slots): the triangles, or the presplitter's pieces
+    int64_t n_tri = 0;    // triangles
     const float* verts = nullptr;
     const int32_t* tris = nullptr;
     int64_t n_verts = 0;
@@ -144,6 +145,12 @@ struct hx_bvh {
     int64_t atr_accepted = 0;
     int hlbvh_clusters = 0;
     int64_t ins_steps = 0, ins_reinserted = 0;
+    // Presplitting: priorities and scratch, the references (boxes, triangle
+    // ids) and the split queues.  refs_active: during a build the leaf slots
+    // hold reference ids (mapped to triangle ids at the end).
+    DevBuf ps_prio, ps_work, ref_min, ref_max, ref_tri, ps_queue;
+    bool refs_active = false;
+    float ps_D = 0.0f;
     bool general = false;  // not a Karras tree: the climb uses split[]
     // Per-mesh tables of the emitted layout (one entry unless batched).
     DevBuf mesh_tab, root_slot;
@@ -213,8 +220,9 @@ int validate_params(const hx_build_params* p) {
                      p->atr_iterations);
     }
     if (p->presplitting) {
-        hx::set_error("presplitting is not available in this build");
-        return HX_ERR_UNSUPPORTED;
+        // Presplitter::setBeta (Presplitter.cpp:328-331).
+        HX_CHECK_ARG(p->presplitter_beta >= 0.0f, "presplitter_beta must be non-negative (got %g)",
+                     (double)p->presplitter_beta);
     }
     return HX_OK;
 }
@@ -321,6 +329,11 @@ hx::ClimbArgs climb_args(hx_bvh* b, bool refit_only) {
     ca.frontier_count = b->counts.as<int32_t>() + 2;
     ca.mesh_of = b->batched ? b->mesh_of.as<int32_t>() : nullptr;
     ca.mesh_off = b->batched ? b->mesh_off.as<int32_t>() : nullptr;
+    if (b->refs_active) {
+        ca.ref_min = b->ref_min.as<float4>();
+        ca.ref_max = b->ref_max.as<float4>();
+        ca.ref_tri = b->ref_tri.as<int32_t>();
+    }
     ca.refit_only = refit_only ? 1 : 0;
     ca.adaptive = (p.adaptive_leaf_size && !refit_only) ? 1 : 0;
     ca.max_leaf_size = p.max_leaf_size;
@@ -353,22 +366,81 @@ hx::EmitArgs emit_args(hx_bvh* b) {
     return ea;
 }
 
+// Presplitter::presplit (Presplitter.cpp:304-324) in place of setupBoxes:
+// priorities and D, then the references; the BVH's buffers are sized for
+// the reference count.
+int run_presplit(hx_bvh* b) {
+    Context* c = &b->ctx->c;
+    const int64_t n = b->n_tri;
+    const int64_t ntiles = (n + 4095) / 4096;
+    HX_TRY(b->ps_prio.reserve(4 * (size_t)n));
+    HX_TRY(b->ps_work.reserve(4 * (size_t)ntiles + 1024));
+    hx::PsArgs a{};
+    a.n = n;
+    a.verts = b->verts;
+    a.tris = b->tris;
+    a.box = b->scene_box.as<float>();
+    a.prio = b->ps_prio.as<float>();
+    char* w = b->ps_work.as<char>();
+    a.acc = reinterpret_cast<unsigned long long*>(w);
+    a.qcount = reinterpret_cast<int32_t*>(w + 256);
+    a.partial = reinterpret_cast<int32_t*>(w + 512);
+    int64_t n_refs = 0;
+    float D = 0.0f;
+    HX_TRY(hx::launch_presplit_count(c, a, b->params.presplitter_beta, &n_refs, &D));
+    if (n_refs >= ((int64_t)1 << 30)) {
+        hx::set_error("presplitting gives %lld references (limit 2^30)", (long long)n_refs);
+        return HX_ERR_UNSUPPORTED;
+    }
+    b->n_refs = n_refs;
+    b->ps_D = D;
+    HX_TRY(reserve_bvh(b, n_refs));
+    HX_TRY(b->ref_min.reserve(16 * (size_t)n_refs));
+    HX_TRY(b->ref_max.reserve(16 * (size_t)n_refs));
+    HX_TRY(b->ref_tri.reserve(4 * (size_t)n_refs));
+    a.queue_cap = std::max<int64_t>(1, n_refs - n);
+    const size_t qb = (size_t)a.queue_cap * 16;
+    HX_TRY(b->ps_queue.reserve(6 * ((qb + 255) & ~(size_t)255) + 256));
+    char* q = b->ps_queue.as<char>();
+    auto carve = [&](size_t bytes) {
+        char* r = q;
+        q += (bytes + 255) & ~(size_t)255;
+        return r;
+    };
+    for (int k = 0; k < 2; ++k) {
+        a.task[k] = reinterpret_cast<int4*>(carve(qb));
+        a.tmn[k] = reinterpret_cast<float4*>(carve(qb));
+        a.tmx[k] = reinterpret_cast<float4*>(carve(qb));
+    }
+    a.ref_min = b->ref_min.as<float4>();
+    a.ref_max = b->ref_max.as<float4>();
+    a.ref_tri = b->ref_tri.as<int32_t>();
+    HX_TRY(hx::launch_presplit_refs(c, a, D, n_refs));
+    b->refs_active = true;
+    return HX_OK;
+}
+
 // LBVHBuilder::build front half (LBVHBuilder.cpp:187-213), shared by PLOC
 // (PLOCBuilder.cpp:161-187): scene box, Morton codes, sort; the sorted
 // triangle ids end in tri_idx.
 int run_front(hx_bvh* b) {
     Context* c = &b->ctx->c;
     const hx_build_params& p = b->params;
-    const int64_t n = b->n_refs;
     HX_HIP(hipSetDevice(c->device));
     float* box = b->scene_box.as<float>();
     if (p.use_scene_box) {
         HX_HIP(hipMemcpyAsync(box, p.scene_box, 6 * sizeof(float), hipMemcpyHostToDevice, c->stream));
     } else {
-        HX_TRY(hx::launch_scene_box(c, b->verts, b->tris, n, box));
+        HX_TRY(hx::launch_scene_box(c, b->verts, b->tris, b->n_tri, box));
     }
     b->box_known = false;
-    HX_TRY(hx::launch_morton(c, b->verts, b->tris, n, box, p.morton_bits, b->keys[0].ptr, b->keys64));
+    b->refs_active = false;
+    if (p.presplitting) HX_TRY(run_presplit(b));
+    const int64_t n = b->n_refs;
+    const float4* rmin = b->refs_active ? b->ref_min.as<float4>() : nullptr;
+    const float4* rmax = b->refs_active ? b->ref_max.as<float4>() : nullptr;
+    HX_TRY(hx::launch_morton(c, b->verts, b->tris, n, box, p.morton_bits, b->keys[0].ptr, b->keys64, nullptr, rmin,
+                             rmax));
     // The sorted triangle ids must end in tri_idx: pass p reads vals[p&1].
     const int passes = hx::sort_pass_count(c, p.morton_bits);
     int32_t* vals[2];
@@ -394,6 +466,10 @@ int run_ploc(hx_bvh* b) {
     a.tris = b->tris;
     a.sorted_tri = b->tri_idx.as<int32_t>();
     a.tri_out = b->vals_tmp.as<int32_t>();
+    if (b->refs_active) {
+        a.ref_min = b->ref_min.as<float4>();
+        a.ref_max = b->ref_max.as<float4>();
+    }
     char* w = b->ploc_work.as<char>();
     auto carve = [&](size_t bytes) {
         char* q = w;
@@ -682,8 +758,10 @@ int run_hlbvh(hx_bvh* b) {
 int run_build(hx_bvh* b) {
     Context* c = &b->ctx->c;
     const hx_build_params& p = b->params;
-    const int64_t n = b->n_refs;
+    // Every build starts from a Karras tree; only the general stages set this.
+    b->general = false;
     HX_TRY(run_front(b));
+    const int64_t n = b->n_refs;
     const int32_t* sorted_tri = b->tri_idx.as<int32_t>();
     if (n == 1) {
         HX_TRY(hx::launch_single_ref(c, b->verts, b->tris, sorted_tri, b->nmin.as<float4>(), b->nmax.as<float4>(),
@@ -705,6 +783,10 @@ int run_build(hx_bvh* b) {
         HX_TRY(hx::launch_climb(c, ca));
         const hx::EmitArgs ea = emit_args(b);
         HX_TRY(hx::launch_collapse_emit(c, ea));
+    }
+    if (b->refs_active) {
+        HX_TRY(hx::launch_refs_to_tris(c, b->tri_idx.as<int32_t>(), b->ref_tri.as<int32_t>(), n));
+        b->refs_active = false;
     }
     b->counts_known = false;
     b->built = true;
@@ -937,6 +1019,7 @@ int hx_build(hx_context* ctx, const hx_build_params* params, const float* d_vert
     b->ctx = ctx;
     b->params = *params;
     b->n_refs = n_triangles;
+    b->n_tri = n_triangles;
     b->verts = d_vertices;
     b->tris = d_triangles;
     b->n_verts = n_vertices;
@@ -1010,6 +1093,10 @@ int hx_build_batch(hx_context* ctx, const hx_build_params* params, const float* 
     *out = nullptr;
     HX_TRY(validate_params(params));
     HX_CHECK_ARG(params->method == HX_METHOD_LBVH, "batched builds use the LBVH method (got %d)", params->method);
+    if (params->presplitting) {
+        hx::set_error("batched builds do not presplit");
+        return HX_ERR_UNSUPPORTED;
+    }
     HX_CHECK_ARG(n_meshes >= 1, "n_meshes must be positive (got %d)", n_meshes);
     HX_CHECK_ARG(h_mesh_offsets[0] == 0, "mesh offsets must start at 0");
     const int64_t n = h_mesh_offsets[n_meshes];
@@ -1023,6 +1110,7 @@ int hx_build_batch(hx_context* ctx, const hx_build_params* params, const float* 
     b->ctx = ctx;
     b->params = *params;
     b->n_refs = n;
+    b->n_tri = n;
     b->verts = d_vertices;
     b->tris = d_triangles;
     b->n_verts = n_vertices;
@@ -1200,6 +1288,12 @@ int hx_bvh_info(hx_bvh* bvh, const char* key, int64_t* value) {
         *value = bvh->atr_accepted;
     } else if (!std::strcmp(key, "hlbvh_clusters")) {
         *value = bvh->hlbvh_clusters;
+    } else if (!std::strcmp(key, "references")) {
+        *value = bvh->n_refs;
+    } else if (!std::strcmp(key, "presplit_d_bits")) {
+        uint32_t u;
+        std::memcpy(&u, &bvh->ps_D, 4);
+        *value = u;
     } else if (!std::strcmp(key, "insertion_steps")) {
         *value = bvh->ins_steps;
     } else if (!std::strcmp(key, "insertion_reinserted")) {

@@ -112,15 +112,24 @@ __global__ __launch_bounds__(kBlock) void k_scene_box_final(const float* __restr
 template <bool K64>
 __global__ __launch_bounds__(kBlock) void k_morton(const float* __restrict__ verts, const int32_t* __restrict__ tris,
                                                    int64_t n, const float* __restrict__ boxes, int shift,
-                                                   void* __restrict__ keys_out, const int32_t* __restrict__ mesh_of) {
+                                                   void* __restrict__ keys_out, const int32_t* __restrict__ mesh_of,
+                                                   const float4* __restrict__ ref_min,
+                                                   const float4* __restrict__ ref_max) {
     const int64_t t = blockIdx.x * (int64_t)kBlock + threadIdx.x;
     if (t >= n) return;
     const float* sb = mesh_of ? boxes + 6 * (int64_t)mesh_of[t] : boxes;
     const float3 smn = make_float3(sb[0], sb[1], sb[2]);
     const float3 scale = make_float3(1.0f / (sb[3] - sb[0]), 1.0f / (sb[4] - sb[1]), 1.0f / (sb[5] - sb[2]));
-    float3 a, b, c;
-    tri_vertices(verts, tris, t, a, b, c);
-    const Box3 tb = tri_box(a, b, c);
+    Box3 tb;
+    if (ref_min) {
+        const float4 rmn = ref_min[t], rmx = ref_max[t];
+        tb.mn = make_float3(rmn.x, rmn.y, rmn.z);
+        tb.mx = make_float3(rmx.x, rmx.y, rmx.z);
+    } else {
+        float3 a, b, c;
+        tri_vertices(verts, tris, t, a, b, c);
+        tb = tri_box(a, b, c);
+    }
     // computeMortonCodes60 regrows the stored box from its two corners.
     Box3 bb = box_empty();
     box_grow(bb, tb.mn);
@@ -383,10 +392,18 @@ __global__ __launch_bounds__(kClimbThreads) void k_climb_local(ClimbArgs a, int3
         const int p = base + k * kClimbThreads + threadIdx.x;
         if (p > last_leaf) break;
         leaf_parent[k] = a.parent[n - 1 + p];
-        const int tri = a.sorted_tri[p];
+        const int id = a.sorted_tri[p];
+        const int tri = a.ref_tri ? a.ref_tri[id] : id;
         float3 v0, v1, v2;
         tri_vertices(a.verts, a.tris, tri, v0, v1, v2);
-        const Box3 tb = tri_box(v0, v1, v2);
+        Box3 tb;
+        if (a.ref_min) {
+            const float4 rmn = a.ref_min[id], rmx = a.ref_max[id];
+            tb.mn = make_float3(rmn.x, rmn.y, rmn.z);
+            tb.mx = make_float3(rmx.x, rmx.y, rmx.z);
+        } else {
+            tb = tri_box(v0, v1, v2);
+        }
         const int leaf = n - 1 + p;
         a.nmin[leaf] = make_float4(tb.mn.x, tb.mn.y, tb.mn.z, 0.0f);
         a.nmax[leaf] = make_float4(tb.mx.x, tb.mx.y, tb.mx.z, 0.0f);
@@ -918,15 +935,15 @@ int launch_scene_box(Context* c, const float* verts, const int32_t* tris, int64_
 }
 
 int launch_morton(Context* c, const float* verts, const int32_t* tris, int64_t n, const float* d_box6, int bits,
-                  void* d_keys, bool keys64, const int32_t* mesh_of) {
+                  void* d_keys, bool keys64, const int32_t* mesh_of, const float4* ref_min, const float4* ref_max) {
     LaunchScope ls(c, "morton");
     const int shift = 60 - bits;
     if (keys64)
         hipLaunchKernelGGL(k_morton<true>, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, verts, tris, n, d_box6, shift,
-                           d_keys, mesh_of);
+                           d_keys, mesh_of, ref_min, ref_max);
     else
         hipLaunchKernelGGL(k_morton<false>, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, verts, tris, n, d_box6,
-                           shift, d_keys, mesh_of);
+                           shift, d_keys, mesh_of, ref_min, ref_max);
     return check_launch("morton");
 }
 

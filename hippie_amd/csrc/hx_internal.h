@@ -108,8 +108,10 @@ int launch_scene_box(Context* c, const float* verts, const int32_t* tris, int64_
 // mesh_of / off (batched builds, else NULL): mesh of each triangle (sorted
 // position for construct) and the M+1 triangle offsets; d_box6 then holds
 // one box per mesh.
+// Keys of n triangles, or (ref_min != NULL) of n presplit reference boxes.
 int launch_morton(Context* c, const float* verts, const int32_t* tris, int64_t n, const float* d_box6,
-                  int bits, void* d_keys, bool keys64, const int32_t* mesh_of = nullptr);
+                  int bits, void* d_keys, bool keys64, const int32_t* mesh_of = nullptr,
+                  const float4* ref_min = nullptr, const float4* ref_max = nullptr);
 int launch_construct(Context* c, const void* d_keys, bool keys64, int64_t n, int32_t* left, int32_t* right,
                      int32_t* parent, int32_t* first, int32_t* last, int32_t* counters, int32_t* end_at,
                      int32_t* zero_word, const int32_t* mesh_of = nullptr, const int32_t* off = nullptr);
@@ -141,6 +143,11 @@ struct ClimbArgs {
     int32_t* atr_height;
     const int32_t* mesh_of;   // batched builds: mesh of each sorted position (else NULL)
     const int32_t* mesh_off;  // M+1 triangle offsets
+    // Presplit builds: sorted_tri holds reference ids; a leaf's box is its
+    // reference's box and its Woop row comes from ref_tri[id] (else NULL).
+    const float4* ref_min;
+    const float4* ref_max;
+    const int32_t* ref_tri;
     int refit_only;           // boxes and Woop only: keep states, costs and cover
     int adaptive;
     int max_leaf_size;
@@ -204,6 +211,8 @@ struct PlocArgs {
     const int32_t* tris;
     const int32_t* sorted_tri;   // Morton order
     int32_t* tri_out;            // depth-first leaf order (n)
+    const float4* ref_min;       // presplit builds: leaf boxes of the reference ids (else NULL)
+    const float4* ref_max;
     int32_t* cl[2];              // cluster node ids, ping-pong (n each)
     float4* ca[2];               // cluster boxes: (mn.x, mn.y, mn.z, mx.x)
     float2* cb[2];               //                (mx.y, mx.z)
@@ -253,6 +262,40 @@ int launch_atr(Context* c, const AtrArgs& a, const PlocArgs& relabel, int64_t* a
 // counts (a.refs) and heights (a.height): nodes are bucketed by height and
 // placed top down.  Uses a.n, refs, height, list, hist, cursor.
 int launch_relabel_by_height(Context* c, const AtrArgs& a, const PlocArgs& relabel);
+
+// hx_presplit.hip: budgeted presplitting of n triangles into references.
+// Phase 1 (priorities, D, the reference count) returns n_refs so that the
+// caller can size the reference arrays (n_refs each) and the two task queues
+// (queue_cap = max(1, n_refs - n) entries each, enough since every queued
+// task holds at least one unit of the budget); phase 2 writes them.
+struct PsArgs {
+    int64_t n;
+    const float* verts;
+    const int32_t* tris;
+    const float* box;           // device scene box (mn.xyz, mx.xyz)
+    float* prio;                // n
+    unsigned long long* acc;    // 4
+    int32_t* partial;           // one entry per 4096 triangles
+    float4* ref_min;            // n_refs
+    float4* ref_max;
+    int32_t* ref_tri;
+    int4* task[2];              // (triangle, budget, first slot, 0)
+    float4* tmn[2];
+    float4* tmx[2];
+    int32_t* qcount;            // 2
+    int64_t queue_cap;
+};
+// Host float -> int with the device convert's saturation (NaN -> 0).
+inline int32_t f2i_host(float x) {
+    if (x != x) return 0;
+    if (x >= 2147483648.0f) return 0x7FFFFFFF;
+    if (x <= -2147483648.0f) return (int32_t)0x80000000;
+    return (int32_t)x;
+}
+int launch_presplit_count(Context* c, const PsArgs& a, float beta, int64_t* n_refs_out, float* D_out);
+int launch_presplit_refs(Context* c, const PsArgs& a, float D, int64_t n_refs);
+// ids[p] = ref_tri[ids[p]]: reference ids to triangle ids after a build.
+int launch_refs_to_tris(Context* c, int32_t* ids, const int32_t* ref_tri, int64_t n);
 
 // hx_insertion.hip: insertion optimisation of a binary tree with boxes (root
 // 0, leaves n-1..2n-2), the reference's step loop run from the host.  On

@@ -57,14 +57,17 @@ __global__ __launch_bounds__(kBlock) void k_ploc_leaves(PlocArgs a) {
     const int p = blockIdx.x * kBlock + threadIdx.x;
     const int n = (int)a.n;
     if (p >= n) return;
-    const int32_t* tr = a.tris + 3 * (int64_t)a.sorted_tri[p];
-    const float3 v0 = ld3(a.verts + 3 * (int64_t)tr[0]);
-    const float3 v1 = ld3(a.verts + 3 * (int64_t)tr[1]);
-    const float3 v2 = ld3(a.verts + 3 * (int64_t)tr[2]);
     Box3 b = box_empty();
-    box_grow(b, v0);
-    box_grow(b, v1);
-    box_grow(b, v2);
+    if (a.ref_min) {
+        const float4 rmn = a.ref_min[a.sorted_tri[p]], rmx = a.ref_max[a.sorted_tri[p]];
+        b.mn = make_float3(rmn.x, rmn.y, rmn.z);
+        b.mx = make_float3(rmx.x, rmx.y, rmx.z);
+    } else {
+        const int32_t* tr = a.tris + 3 * (int64_t)a.sorted_tri[p];
+        box_grow(b, ld3(a.verts + 3 * (int64_t)tr[0]));
+        box_grow(b, ld3(a.verts + 3 * (int64_t)tr[1]));
+        box_grow(b, ld3(a.verts + 3 * (int64_t)tr[2]));
+    }
     a.cl[0][p] = n - 1 + p;
     a.ca[0][p] = make_float4(b.mn.x, b.mn.y, b.mn.z, b.mx.x);
     a.cb[0][p] = make_float2(b.mx.y, b.mx.z);

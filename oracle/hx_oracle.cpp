@@ -1115,8 +1115,9 @@ int64_t hxo_bin_canonical(const int32_t* nodes, int64_t n_interior, int64_t n_le
     return (int64_t)buf.size();
 }
 
-int hxo_validate_bin(const int32_t* nodes, int64_t n_interior, int64_t n_leaves, const int32_t* tri_idx,
-                     int64_t n_refs, const float* verts, const int32_t* tris) {
+int hxo_validate_bin_refs(const int32_t* nodes, int64_t n_interior, int64_t n_leaves, const int32_t* tri_idx,
+                          int64_t n_refs, int64_t n_tris, const float* verts, const int32_t* tris) {
+    const bool split = n_tris != n_refs;
     const int64_t nn = n_interior + n_leaves;
     if (n_interior < 1) return 1;
     std::vector<int> seen(nn, 0), refSeen(n_refs, 0);
@@ -1161,6 +1162,17 @@ int hxo_validate_bin(const int32_t* nodes, int64_t n_interior, int64_t n_leaves,
             refSeen[r] = 1;
             V3 a, bb, c;
             tri_verts(verts, tris, tri_idx[r], a, bb, c);
+            if (split) {
+                /* a presplit reference is clipped: its leaf box meets the triangle's box */
+                Box tb;
+                tb.grow(a);
+                tb.grow(bb);
+                tb.grow(c);
+                if (tb.mx.x < mn.x || tb.mx.y < mn.y || tb.mx.z < mn.z || tb.mn.x > mx.x || tb.mn.y > mx.y ||
+                    tb.mn.z > mx.z)
+                    return 14;
+                continue;
+            }
             for (const V3* p : {&a, &bb, &c})
                 if (p->x < mn.x || p->y < mn.y || p->z < mn.z || p->x > mx.x || p->y > mx.y || p->z > mx.z)
                     return 14;                          /* leaf box contains its triangles */
@@ -1170,18 +1182,336 @@ int hxo_validate_bin(const int32_t* nodes, int64_t n_interior, int64_t n_leaves,
         if (!refSeen[r]) return 15;
     /* Every triangle is referenced (HipBVH.cpp:290-303); without presplitting
      * the references are a permutation of the triangle ids. */
-    std::vector<int> triSeen(n_refs, 0);
+    std::vector<int> triSeen(n_tris, 0);
     for (int64_t r = 0; r < n_refs; ++r) {
         int32_t id = tri_idx[r];
-        if (id < 0 || id >= n_refs || triSeen[id]) return 16;
+        if (id < 0 || id >= n_tris || (!split && triSeen[id])) return 16;
         triSeen[id] = 1;
     }
+    for (int64_t t = 0; t < n_tris; ++t)
+        if (!triSeen[t]) return 17;
     return 0;
 }
 
-int64_t hxo_build_lbvh(const float* verts, const int32_t* tris, int64_t n, const float* scene_box, int bits,
+int hxo_validate_bin(const int32_t* nodes, int64_t n_interior, int64_t n_leaves, const int32_t* tri_idx,
+                     int64_t n_refs, const float* verts, const int32_t* tris) {
+    return hxo_validate_bin_refs(nodes, n_interior, n_leaves, tri_idx, n_refs, n_refs, verts, tris);
+}
+
+/* ------------------------------------------------------------------------ */
+/* GPU presplitter (Presplitter.cpp:124-326, PresplitterKernels.cu:11-419).  */
+/* ------------------------------------------------------------------------ */
+}  // extern "C"
+
+namespace {
+
+/* x^y for the priority, in double precision from the textbook series (no
+ * library call, so that the device and the host run the same operations):
+ * ln x = e ln2 + 2 atanh((m-1)/(m+1)) with m in [sqrt(1/2), sqrt(2)); e^y =
+ * 2^k e^r with |r| <= ln2/2 and the Taylor series of e^r. */
+constexpr double kPsLn2 = 0.69314718055994530942;
+constexpr double kPsLn2Hi = 0.693147180369123816490;  /* 21 low bits zero */
+constexpr double kPsLn2Lo = 1.90821492927058770002e-10;
+
+double ps_ln(double x) {
+    int e = 0;
+    double m = std::frexp(x, &e);
+    if (m < 0.70710678118654752440) {
+        m *= 2.0;
+        e -= 1;
+    }
+    const double s = (m - 1.0) / (m + 1.0), s2 = s * s;
+    double t = 1.0 / 25.0;
+    for (int k = 23; k >= 1; k -= 2) t = t * s2 + 1.0 / (double)k;
+    return (double)e * kPsLn2 + 2.0 * s * t;
+}
+
+double ps_exp(double y) {
+    const double k = std::nearbyint(y / kPsLn2);
+    const double r = (y - k * kPsLn2Hi) - k * kPsLn2Lo;
+    double p = 1.0;
+    for (int j = 20; j >= 1; --j) p = 1.0 + p * r / (double)j;
+    return std::ldexp(p, (int)k);
+}
+
+float ps_pow(float x, float y) {
+    if (!(x > 0.0f) || !(x <= kMaxFloat)) return x == 0.0f ? 0.0f : x;
+    return (float)ps_exp((double)y * ps_ln((double)x));
+}
+
+float axis_of(const V3& v, int a) { return a == 0 ? v.x : (a == 1 ? v.y : v.z); }
+void set_axis(V3& v, int a, float f) {
+    if (a == 0) v.x = f;
+    else if (a == 1) v.y = f;
+    else v.z = f;
+}
+
+/* 21-bit per axis interleave, axis 0 in bit 0 (PresplitterKernels.cu:44-55). */
+uint64_t ps_morton63(const int32_t q[3]) {
+    uint64_t c = 0;
+    for (int i = 0; i < 21; ++i)
+        for (int a = 0; a < 3; ++a) c |= (uint64_t)((q[a] >> i) & 1) << (3 * i + a);
+    return c;
+}
+
+int ps_diff_bit(uint64_t a, uint64_t b) {
+    const uint64_t d = a ^ b;
+    return d ? 63 - __builtin_clzll(d) : -1;
+}
+
+struct PsScene {
+    V3 mn, dg, scale;
+};
+
+PsScene ps_scene(const float* sb) {
+    PsScene s;
+    s.mn = V3{sb[0], sb[1], sb[2]};
+    s.dg = V3{sb[3] - sb[0], sb[4] - sb[1], sb[5] - sb[2]};
+    s.scale = V3{1.0f / s.dg.x, 1.0f / s.dg.y, 1.0f / s.dg.z};
+    return s;
+}
+
+/* computePriorities (:11-71): (2^i (A_box - A_ideal))^(1/3), i from the
+ * highest differing bit of the 63-bit codes of the box corners. */
+float ps_priority(const V3& v0, const V3& v1, const V3& v2, const PsScene& S) {
+    Box box;
+    box.grow(v0);
+    box.grow(v1);
+    box.grow(v2);
+    int32_t qmn[3], qmx[3];
+    for (int a = 0; a < 3; ++a) {
+        qmn[a] = f2i(((axis_of(box.mn, a) - axis_of(S.mn, a)) * axis_of(S.scale, a)) * 2097151.9f);
+        qmx[a] = f2i(((axis_of(box.mx, a) - axis_of(S.mn, a)) * axis_of(S.scale, a)) * 2097151.9f);
+    }
+    const int diff = ps_diff_bit(ps_morton63(qmn), ps_morton63(qmx));
+    const int i = diff / 3 - 21;
+    const V3 d1{v0.x - v1.x, v0.y - v1.y, v0.z - v1.z};
+    const V3 d2{v0.x - v2.x, v0.y - v2.y, v0.z - v2.z};
+    const V3 c{d1.y * d2.z - d1.z * d2.y, d1.z * d2.x - d1.x * d2.z, d1.x * d2.y - d1.y * d2.x};
+    const float ideal = std::fabs(c.x) + std::fabs(c.y) + std::fabs(c.z);
+    return ps_pow(std::ldexp(1.0f, i) * (box.area() - ideal), 1.0f / 3.0f);
+}
+
+struct PsTask {
+    int32_t tri, count;
+    int64_t off;
+    Box box;
+};
+
+/* split (:189-419) of one task into two; the budget count-1 is shared by the
+ * children's longest() values, which the reference's AABB::longest returns as
+ * bool (util/AABB.h:38), i.e. 1 unless the child is a point. */
+void ps_split(const PsTask& t, const V3 v[3], const PsScene& S, PsTask& L, PsTask& R) {
+    const Box& box = t.box;
+    int32_t qmn[3], qmx[3];
+    for (int a = 0; a < 3; ++a) {
+        qmn[a] = f2i(((axis_of(box.mn, a) - axis_of(S.mn, a)) / axis_of(S.dg, a)) * 2097151.9f);
+        qmx[a] = f2i(((axis_of(box.mx, a) - axis_of(S.mn, a)) / axis_of(S.dg, a)) * 2097151.9f);
+    }
+    const int diff = ps_diff_bit(ps_morton63(qmn), ps_morton63(qmx));
+    int axis;
+    float pos;
+    if (diff >= 0) {
+        axis = diff % 3;
+        pos = 0.0f;
+        float delta = 0.5f;
+        for (int i = 20; i >= diff / 3; --i) {
+            if ((qmx[axis] >> i) & 1) pos += delta;
+            delta *= 0.5f;
+        }
+        pos = pos * axis_of(S.dg, axis) + axis_of(S.mn, axis);
+        if (axis_of(box.mn, axis) >= pos || axis_of(box.mx, axis) <= pos)
+            pos = (axis_of(box.mn, axis) + axis_of(box.mx, axis)) * 0.5f;
+    } else {
+        const V3 d{box.mx.x - box.mn.x, box.mx.y - box.mn.y, box.mx.z - box.mn.z};
+        axis = (d.x > d.y && d.x > d.z) ? 0 : (d.y > d.z ? 1 : 2);
+        pos = (axis_of(box.mn, axis) + axis_of(box.mx, axis)) * 0.5f;
+    }
+    Box lb, rb;
+    const V3* p1 = &v[2];
+    for (int i = 0; i < 3; ++i) {
+        const V3* p0 = p1;
+        p1 = &v[i];
+        const float a = axis_of(*p0, axis), b = axis_of(*p1, axis);
+        if (a <= pos) lb.grow(*p0);
+        if (a >= pos) rb.grow(*p0);
+        if ((a < pos && b > pos) || (a > pos && b < pos)) {
+            const float tt = MAXs(MINs((pos - a) / (b - a), 1.0f), 0.0f);
+            const float u = 1.0f - tt;
+            const V3 q{p0->x * u + p1->x * tt, p0->y * u + p1->y * tt, p0->z * u + p1->z * tt};
+            lb.grow(q);
+            rb.grow(q);
+        }
+    }
+    set_axis(lb.mx, axis, pos);
+    set_axis(rb.mn, axis, pos);
+    for (Box* x : {&lb, &rb}) {
+        x->mn = V3{MAXs(x->mn.x, box.mn.x), MAXs(x->mn.y, box.mn.y), MAXs(x->mn.z, box.mn.z)};
+        x->mx = V3{MINs(x->mx.x, box.mx.x), MINs(x->mx.y, box.mx.y), MINs(x->mx.z, box.mx.z)};
+    }
+    auto longest = [](const Box& x) {
+        const float m = std::fmax(std::fmax(x.mx.x - x.mn.x, x.mx.y - x.mn.y), x.mx.z - x.mn.z);
+        return m != 0.0f ? 1.0f : 0.0f;
+    };
+    const float ll = longest(lb), rl = longest(rb);
+    const int32_t lc = f2i(((float)(t.count - 1) * ll) / (ll + rl) + 0.5f);
+    L = PsTask{t.tri, lc, t.off, lb};
+    R = PsTask{t.tri, t.count - 1 - lc, t.off + lc + 1, rb};
+}
+
+/* Exact fixed-point sum (2^-32 units, truncated per term) of the priorities,
+ * floored: the reference's float atomic sum truncated to int. */
+int64_t ps_sum_floor(const std::vector<float>& p) {
+    uint64_t s = 0;
+    for (float x : p)
+        if (x >= 0.0f && x <= 1.0e9f) s += (uint64_t)((double)x * 4294967296.0);
+    return (int64_t)(s >> 32);
+}
+
+int64_t ps_sum_round(const std::vector<float>& p, float D) {
+    int64_t s = 0;
+    for (float x : p) s += f2i(D * x);
+    return s;
+}
+
+}  // namespace
+
+extern "C" {
+
+float hxo_ps_pow(float x, float y) { return ps_pow(x, y); }
+
+/* One split of a task (triangle v[9], box mn.xyz mx.xyz, budget count) in the
+ * scene box sb; writes the children's boxes and budgets. */
+void hxo_ps_split(const float* v, const float* box, int32_t count, const float* sb, float* lbox, float* rbox,
+                  int32_t* lcount, int32_t* rcount) {
+    const PsScene S = ps_scene(sb);
+    const V3 tv[3] = {load3(v), load3(v + 3), load3(v + 6)};
+    PsTask t{0, count, 0, Box{}};
+    t.box.mn = load3(box);
+    t.box.mx = load3(box + 3);
+    PsTask l, r;
+    ps_split(t, tv, S, l, r);
+    const float lb[6] = {l.box.mn.x, l.box.mn.y, l.box.mn.z, l.box.mx.x, l.box.mx.y, l.box.mx.z};
+    const float rb[6] = {r.box.mn.x, r.box.mn.y, r.box.mn.z, r.box.mx.x, r.box.mx.y, r.box.mx.z};
+    std::memcpy(lbox, lb, sizeof(lb));
+    std::memcpy(rbox, rb, sizeof(rb));
+    *lcount = l.count;
+    *rcount = r.count;
+}
+
+int64_t hxo_presplit(const float* verts, const int32_t* tris, int64_t n, const float* scene_box, float beta,
+                     float* ref_min, float* ref_max, int32_t* ref_tri, int64_t cap, float* D_out,
+                     float* priorities) {
+    float sb[6];
+    if (scene_box) std::memcpy(sb, scene_box, sizeof(sb));
+    else hxo_scene_box(verts, tris, n, sb);
+    const PsScene S = ps_scene(sb);
+    std::vector<float> prio(n);
+    auto tri_v = [&](int64_t t, V3 v[3]) {
+        for (int k = 0; k < 3; ++k) v[k] = load3(verts + 3 * (int64_t)tris[3 * t + k]);
+    };
+    parallel_for(n, [&](int64_t t) {
+        V3 v[3];
+        tri_v(t, v);
+        prio[t] = ps_priority(v[0], v[1], v[2], S);
+    });
+    if (priorities) std::copy(prio.begin(), prio.end(), priorities);
+    /* computeD (Presplitter.cpp:124-155), with Smax = (int)(beta * n). */
+    const int64_t smax = f2i(beta * (float)n);
+    const int64_t s1 = ps_sum_floor(prio);
+    float D = 0.0f;
+    if (s1 > 0) {
+        float dmin = (float)(smax / s1);
+        int64_t s = ps_sum_round(prio, dmin);
+        float dmax = (dmin * (float)smax) / (float)s;
+        for (int r = 0; r < 6; ++r) {
+            const float davg = 0.5f * (dmin + dmax);
+            s = ps_sum_round(prio, davg);
+            if (s > smax) dmax = davg;
+            else dmin = davg;
+        }
+        D = dmin;
+    }
+    if (D_out) *D_out = D;
+    std::vector<int64_t> off(n);
+    int64_t total = 0;
+    for (int64_t t = 0; t < n; ++t) {
+        off[t] = total;
+        total += (int64_t)f2i(D * prio[t]) + 1;
+    }
+    if (!ref_tri || total > cap) return total;
+    /* Each triangle's references are the leaves of its split tree in
+     * depth-first order, at the triangle's offset. */
+    parallel_for(n, [&](int64_t t) {
+        V3 v[3];
+        tri_v(t, v);
+        Box b;
+        b.grow(v[0]);
+        b.grow(v[1]);
+        b.grow(v[2]);
+        std::vector<PsTask> stack{PsTask{(int32_t)t, f2i(D * prio[t]), off[t], b}};
+        while (!stack.empty()) {
+            const PsTask k = stack.back();
+            stack.pop_back();
+            if (k.count <= 0) {
+                store4(ref_min + 4 * k.off, k.box.mn);
+                store4(ref_max + 4 * k.off, k.box.mx);
+                ref_tri[k.off] = k.tri;
+                continue;
+            }
+            PsTask l, r;
+            ps_split(k, v, S, l, r);
+            stack.push_back(r);
+            stack.push_back(l);
+        }
+    });
+    return total;
+}
+
+}  // extern "C"
+
+namespace {
+
+/* setupReferences (LBVHBuilder.cpp:26-74): the triangles' boxes, or the
+ * presplitter's references (beta >= 0). */
+struct RefSet {
+    int64_t n = 0;
+    bool split = false;
+    std::vector<float> mn, mx;
+    std::vector<int32_t> tri;
+};
+
+RefSet setup_refs(const float* verts, const int32_t* tris, int64_t n_tri, const float sb[6], float beta) {
+    RefSet r;
+    if (!(beta >= 0.0f)) {
+        r.n = n_tri;
+        r.mn.resize(4 * n_tri);
+        r.mx.resize(4 * n_tri);
+        hxo_setup_boxes(verts, tris, n_tri, r.mn.data(), r.mx.data());
+        return r;
+    }
+    r.split = true;
+    r.n = hxo_presplit(verts, tris, n_tri, sb, beta, nullptr, nullptr, nullptr, 0, nullptr, nullptr);
+    r.mn.resize(4 * r.n);
+    r.mx.resize(4 * r.n);
+    r.tri.resize(r.n);
+    hxo_presplit(verts, tris, n_tri, sb, beta, r.mn.data(), r.mx.data(), r.tri.data(), r.n, nullptr, nullptr);
+    return r;
+}
+
+/* Leaf slots hold reference ids until the end; the output holds triangles. */
+void refs_to_tris(const RefSet& r, int32_t* tri_out) {
+    if (!r.split) return;
+    for (int64_t i = 0; i < r.n; ++i) tri_out[i] = r.tri[tri_out[i]];
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t hxo_build_lbvh(const float* verts, const int32_t* tris, int64_t n_tri, const float* scene_box, int bits,
                        int adaptive, int max_leaf_size, float ct, float ci, int32_t* nodes, int32_t* tri_out,
-                       float* woop, int64_t* n_interior, int64_t* n_leaves, double sec[8]) {
+                       float* woop, int64_t* n_interior, int64_t* n_leaves, double sec[8], float presplit_beta) {
     using clk = std::chrono::steady_clock;
     auto t0 = clk::now();
     auto lap = [&](int i) {
@@ -1192,10 +1522,12 @@ int64_t hxo_build_lbvh(const float* verts, const int32_t* tris, int64_t n, const
     for (int i = 0; i < 8; ++i) sec[i] = 0.0;
     float sb[6];
     if (scene_box) std::memcpy(sb, scene_box, sizeof(sb));
-    else hxo_scene_box(verts, tris, n, sb);
+    else hxo_scene_box(verts, tris, n_tri, sb);
     lap(0);
-    std::vector<float> rmin(4 * n), rmax(4 * n);
-    hxo_setup_boxes(verts, tris, n, rmin.data(), rmax.data());
+    RefSet R = setup_refs(verts, tris, n_tri, sb, presplit_beta);
+    const int64_t n = R.n;
+    std::vector<float>& rmin = R.mn;
+    std::vector<float>& rmax = R.mx;
     std::vector<uint64_t> keys(n);
     std::vector<int32_t> vals(n);
     hxo_morton_codes(rmin.data(), rmax.data(), n, sb, bits, keys.data());
@@ -1214,6 +1546,7 @@ int64_t hxo_build_lbvh(const float* verts, const int32_t* tris, int64_t n, const
     int64_t nn = hxo_collapse(n, vals.data(), left.data(), right.data(), parent.data(), nmin.data(), nmax.data(),
                               adaptive, max_leaf_size, ct, ci, nodes, tri_out, n_interior, n_leaves);
     lap(5);
+    refs_to_tris(R, tri_out);
     hxo_woopify(verts, tris, tri_out, n, woop);
     lap(6);
     return nn;
@@ -1221,10 +1554,10 @@ int64_t hxo_build_lbvh(const float* verts, const int32_t* tris, int64_t n, const
 
 /* PLOCBuilder::build (PLOCBuilder.cpp:161-208): LBVH front half, clustering,
  * collapse, woopify.  Returns the node count; sec[0..6] as hxo_build_lbvh. */
-int64_t hxo_build_ploc(const float* verts, const int32_t* tris, int64_t n, const float* scene_box, int bits,
+int64_t hxo_build_ploc(const float* verts, const int32_t* tris, int64_t n_tri, const float* scene_box, int bits,
                        int radius, int max_radius, int ploc_adaptive, int adaptive, int max_leaf_size, float ct,
                        float ci, int32_t* nodes, int32_t* tri_out, float* woop, int64_t* n_interior,
-                       int64_t* n_leaves, double sec[8]) {
+                       int64_t* n_leaves, double sec[8], float presplit_beta) {
     using clk = std::chrono::steady_clock;
     auto t0 = clk::now();
     auto lap = [&](int i) {
@@ -1235,10 +1568,12 @@ int64_t hxo_build_ploc(const float* verts, const int32_t* tris, int64_t n, const
     for (int i = 0; i < 8; ++i) sec[i] = 0.0;
     float sb[6];
     if (scene_box) std::memcpy(sb, scene_box, sizeof(sb));
-    else hxo_scene_box(verts, tris, n, sb);
+    else hxo_scene_box(verts, tris, n_tri, sb);
     lap(0);
-    std::vector<float> rmin(4 * n), rmax(4 * n);
-    hxo_setup_boxes(verts, tris, n, rmin.data(), rmax.data());
+    RefSet R = setup_refs(verts, tris, n_tri, sb, presplit_beta);
+    const int64_t n = R.n;
+    std::vector<float>& rmin = R.mn;
+    std::vector<float>& rmax = R.mx;
     std::vector<uint64_t> keys(n);
     std::vector<int32_t> vals(n);
     hxo_morton_codes(rmin.data(), rmax.data(), n, sb, bits, keys.data());
@@ -1260,6 +1595,7 @@ int64_t hxo_build_ploc(const float* verts, const int32_t* tris, int64_t n, const
     int64_t nn = hxo_collapse(n, vals.data(), left.data(), right.data(), parent.data(), nmin.data(), nmax.data(),
                               adaptive, max_leaf_size, ct, ci, nodes, tri_out, n_interior, n_leaves);
     lap(5);
+    refs_to_tris(R, tri_out);
     hxo_woopify(verts, tris, tri_out, n, woop);
     lap(6);
     return nn;
@@ -1455,14 +1791,16 @@ int64_t hxo_hlbvh(int64_t n, int bits, int sah_bits, const uint64_t* sorted_keys
     return hlbvh_upper(n, bits, sah_bits, sorted_keys, left, right, parent, nmin, nmax);
 }
 
-int64_t hxo_build_hlbvh(const float* verts, const int32_t* tris, int64_t n, const float* scene_box, int bits,
+int64_t hxo_build_hlbvh(const float* verts, const int32_t* tris, int64_t n_tri, const float* scene_box, int bits,
                         int sah_bits, int adaptive, int max_leaf_size, float ct, float ci, int32_t* nodes,
-                        int32_t* tri_out, float* woop, int64_t* n_interior, int64_t* n_leaves, int64_t* clusters) {
+                        int32_t* tri_out, float* woop, int64_t* n_interior, int64_t* n_leaves, int64_t* clusters, float presplit_beta) {
     float sb[6];
     if (scene_box) std::memcpy(sb, scene_box, sizeof(sb));
-    else hxo_scene_box(verts, tris, n, sb);
-    std::vector<float> rmin(4 * n), rmax(4 * n);
-    hxo_setup_boxes(verts, tris, n, rmin.data(), rmax.data());
+    else hxo_scene_box(verts, tris, n_tri, sb);
+    RefSet R = setup_refs(verts, tris, n_tri, sb, presplit_beta);
+    const int64_t n = R.n;
+    std::vector<float>& rmin = R.mn;
+    std::vector<float>& rmax = R.mx;
     std::vector<uint64_t> keys(n);
     std::vector<int32_t> vals(n);
     hxo_morton_codes(rmin.data(), rmax.data(), n, sb, bits, keys.data());
@@ -1480,9 +1818,11 @@ int64_t hxo_build_hlbvh(const float* verts, const int32_t* tris, int64_t n, cons
     if (clusters) *clusters = C;
     int64_t nn = hxo_collapse(n, vals.data(), left.data(), right.data(), parent.data(), nmin.data(), nmax.data(),
                               adaptive, max_leaf_size, ct, ci, nodes, tri_out, n_interior, n_leaves);
+    refs_to_tris(R, tri_out);
     hxo_woopify(verts, tris, tri_out, n, woop);
     return nn;
 }
+
 
 /* ------------------------------------------------------------------------ */
 /* Insertion (InsertionBuilder.cpp:22-274, InsertionBuilderKernels.cu:11-515):
@@ -1786,15 +2126,17 @@ int64_t hxo_insertion(int64_t n, int mod, float ct, float ci, int32_t* left, int
     return total;
 }
 
-int64_t hxo_build_insertion(const float* verts, const int32_t* tris, int64_t n, const float* scene_box, int bits,
+int64_t hxo_build_insertion(const float* verts, const int32_t* tris, int64_t n_tri, const float* scene_box, int bits,
                             int mod, int adaptive, int max_leaf_size, float ct, float ci, int32_t* nodes,
                             int32_t* tri_out, float* woop, int64_t* n_interior, int64_t* n_leaves,
-                            int64_t* steps_out) {
+                            int64_t* steps_out, float presplit_beta) {
     float sb[6];
     if (scene_box) std::memcpy(sb, scene_box, sizeof(sb));
-    else hxo_scene_box(verts, tris, n, sb);
-    std::vector<float> rmin(4 * n), rmax(4 * n);
-    hxo_setup_boxes(verts, tris, n, rmin.data(), rmax.data());
+    else hxo_scene_box(verts, tris, n_tri, sb);
+    RefSet R = setup_refs(verts, tris, n_tri, sb, presplit_beta);
+    const int64_t n = R.n;
+    std::vector<float>& rmin = R.mn;
+    std::vector<float>& rmax = R.mx;
     std::vector<uint64_t> keys(n);
     std::vector<int32_t> vals(n);
     hxo_morton_codes(rmin.data(), rmax.data(), n, sb, bits, keys.data());
@@ -1809,21 +2151,24 @@ int64_t hxo_build_insertion(const float* verts, const int32_t* tris, int64_t n, 
     hxo_insertion(n, mod, ct, ci, left.data(), right.data(), parent.data(), nmin.data(), nmax.data(), steps_out);
     int64_t nn = hxo_collapse(n, vals.data(), left.data(), right.data(), parent.data(), nmin.data(), nmax.data(),
                               adaptive, max_leaf_size, ct, ci, nodes, tri_out, n_interior, n_leaves);
+    refs_to_tris(R, tri_out);
     hxo_woopify(verts, tris, tri_out, n, woop);
     return nn;
 }
 
 /* ATRBuilder::build (ATRBuilder.cpp:179-236): LBVH front half and hierarchy,
  * refit, ATR optimisation, collapse, woopify. */
-int64_t hxo_build_atr(const float* verts, const int32_t* tris, int64_t n, const float* scene_box, int bits,
+int64_t hxo_build_atr(const float* verts, const int32_t* tris, int64_t n_tri, const float* scene_box, int bits,
                       int treelet_size, int iterations, int adaptive, int max_leaf_size, float ct, float ci,
                       int32_t* nodes, int32_t* tri_out, float* woop, int64_t* n_interior, int64_t* n_leaves,
-                      int64_t* accepted) {
+                      int64_t* accepted, float presplit_beta) {
     float sb[6];
     if (scene_box) std::memcpy(sb, scene_box, sizeof(sb));
-    else hxo_scene_box(verts, tris, n, sb);
-    std::vector<float> rmin(4 * n), rmax(4 * n);
-    hxo_setup_boxes(verts, tris, n, rmin.data(), rmax.data());
+    else hxo_scene_box(verts, tris, n_tri, sb);
+    RefSet R = setup_refs(verts, tris, n_tri, sb, presplit_beta);
+    const int64_t n = R.n;
+    std::vector<float>& rmin = R.mn;
+    std::vector<float>& rmax = R.mx;
     std::vector<uint64_t> keys(n);
     std::vector<int32_t> vals(n);
     hxo_morton_codes(rmin.data(), rmax.data(), n, sb, bits, keys.data());
@@ -1840,6 +2185,7 @@ int64_t hxo_build_atr(const float* verts, const int32_t* tris, int64_t n, const 
     if (accepted) *accepted = acc;
     int64_t nn = hxo_collapse(n, vals.data(), left.data(), right.data(), parent.data(), nmin.data(), nmax.data(),
                               adaptive, max_leaf_size, ct, ci, nodes, tri_out, n_interior, n_leaves);
+    refs_to_tris(R, tri_out);
     hxo_woopify(verts, tris, tri_out, n, woop);
     return nn;
 }

@@ -105,15 +105,33 @@ int64_t hxo_bin_canonical(const int32_t* nodes, int64_t n_interior, int64_t n_le
 int hxo_validate_bin(const int32_t* nodes, int64_t n_interior, int64_t n_leaves,
                      const int32_t* tri_idx, int64_t n_refs,
                      const float* verts, const int32_t* tris);
+/* As hxo_validate_bin for presplit references (n_refs >= n_tris): a leaf box
+ * only has to meet its triangles' boxes, and every triangle is referenced at
+ * least once. */
+int hxo_validate_bin_refs(const int32_t* nodes, int64_t n_interior, int64_t n_leaves, const int32_t* tri_idx,
+                          int64_t n_refs, int64_t n_tris, const float* verts, const int32_t* tris);
+
+/* The presplitter's x^y (double-precision series, see hx_oracle.cpp). */
+float hxo_ps_pow(float x, float y);
+/* One presplitter split (PresplitterKernels.cu:189-419). */
+void hxo_ps_split(const float* v, const float* box, int32_t count, const float* sb, float* lbox, float* rbox,
+                  int32_t* lcount, int32_t* rcount);
+
+/* Presplitter (Presplitter.cpp:124-326, PresplitterKernels.cu:11-419): returns
+ * the number of references; writes them (box corners as float4, triangle id)
+ * when ref_tri is given and the count fits cap.  D_out, priorities optional. */
+int64_t hxo_presplit(const float* verts, const int32_t* tris, int64_t n, const float* scene_box, float beta,
+                     float* ref_min, float* ref_max, int32_t* ref_tri, int64_t cap, float* D_out,
+                     float* priorities);
 
 /* Full LBVH build (setup boxes, Morton, sort, construct, refit, collapse,
  * woopify) with per-stage wall-clock seconds in stage_seconds[8].
  * scene_box may be NULL (then computed by hxo_scene_box, timed as stage 0). */
-int64_t hxo_build_lbvh(const float* verts, const int32_t* tris, int64_t n,
+int64_t hxo_build_lbvh(const float* verts, const int32_t* tris, int64_t n_tri,
                        const float* scene_box, int bits, int adaptive, int max_leaf_size,
                        float ct, float ci,
                        int32_t* nodes, int32_t* tri_out, float* woop,
-                       int64_t* n_interior, int64_t* n_leaves, double stage_seconds[8]);
+                       int64_t* n_interior, int64_t* n_leaves, double stage_seconds[8], float presplit_beta);
 
 /* PLOC clustering (PLOCBuilder.cpp:21-159, PLOCBuilderKernels.cu:116-389) over
  * the sorted leaves whose boxes are in nmin/nmax[n-1+i]; fills left/right/
@@ -125,10 +143,10 @@ int hxo_ploc(int64_t n, int radius, int max_radius, int adaptive, int32_t* left,
              float* nmin, float* nmax);
 
 /* PLOCBuilder::build (PLOCBuilder.cpp:161-208). */
-int64_t hxo_build_ploc(const float* verts, const int32_t* tris, int64_t n, const float* scene_box, int bits,
+int64_t hxo_build_ploc(const float* verts, const int32_t* tris, int64_t n_tri, const float* scene_box, int bits,
                        int radius, int max_radius, int ploc_adaptive, int adaptive, int max_leaf_size, float ct,
                        float ci, int32_t* nodes, int32_t* tri_out, float* woop, int64_t* n_interior,
-                       int64_t* n_leaves, double stage_seconds[8]);
+                       int64_t* n_leaves, double stage_seconds[8], float presplit_beta);
 
 /* ATRBVH optimisation (ATRBuilder.cpp:23-177, ATRBuilderKernels.cu:11-577) of
  * the binary tree in place: left/right/parent and the boxes of restructured
@@ -138,10 +156,10 @@ int64_t hxo_atr(int64_t n, int treelet_size, int iterations, float ct, float ci,
                 int32_t* parent, float* nmin, float* nmax);
 
 /* ATRBuilder::build (ATRBuilder.cpp:179-236). */
-int64_t hxo_build_atr(const float* verts, const int32_t* tris, int64_t n, const float* scene_box, int bits,
+int64_t hxo_build_atr(const float* verts, const int32_t* tris, int64_t n_tri, const float* scene_box, int bits,
                       int treelet_size, int iterations, int adaptive, int max_leaf_size, float ct, float ci,
                       int32_t* nodes, int32_t* tri_out, float* woop, int64_t* n_interior, int64_t* n_leaves,
-                      int64_t* accepted);
+                      int64_t* accepted, float presplit_beta);
 
 /* HLBVH upper levels over a Karras tree (left/right/parent, boxes 2n-1) of the
  * sorted keys, in place; returns the number of clusters (1: unchanged). */
@@ -151,9 +169,9 @@ int64_t hxo_hlbvh(int64_t n, int bits, int sah_bits, const uint64_t* sorted_keys
 /* HLBVHBuilder::build (HLBVHBuilder.cpp:61-437, HLBVHBuilderKernels.cu:35-704):
  * LBVH below clusters sharing the top sah_bits Morton bits, binned SAH above
  * them.  *clusters receives the number of clusters. */
-int64_t hxo_build_hlbvh(const float* verts, const int32_t* tris, int64_t n, const float* scene_box, int bits,
+int64_t hxo_build_hlbvh(const float* verts, const int32_t* tris, int64_t n_tri, const float* scene_box, int bits,
                         int sah_bits, int adaptive, int max_leaf_size, float ct, float ci, int32_t* nodes,
-                        int32_t* tri_out, float* woop, int64_t* n_interior, int64_t* n_leaves, int64_t* clusters);
+                        int32_t* tri_out, float* woop, int64_t* n_interior, int64_t* n_leaves, int64_t* clusters, float presplit_beta);
 
 /* findBestNode (InsertionBuilderKernels.cu:11-131) for every node 1..2n-2 of a
  * binary tree: out[i] the best new sibling (-1 if none), red[i] its area
@@ -168,10 +186,10 @@ int64_t hxo_insertion(int64_t n, int mod, float ct, float ci, int32_t* left, int
                       float* nmin, float* nmax, int64_t* steps_out);
 
 /* InsertionBuilder::build with the LBVH start (InsertionBuilder.cpp:400-440). */
-int64_t hxo_build_insertion(const float* verts, const int32_t* tris, int64_t n, const float* scene_box, int bits,
+int64_t hxo_build_insertion(const float* verts, const int32_t* tris, int64_t n_tri, const float* scene_box, int bits,
                             int mod, int adaptive, int max_leaf_size, float ct, float ci, int32_t* nodes,
                             int32_t* tri_out, float* woop, int64_t* n_interior, int64_t* n_leaves,
-                            int64_t* steps_out);
+                            int64_t* steps_out, float presplit_beta);
 
 /* Ray Morton keys: rt/ray/RayBufferKernels.cu:556-614 (Aila64) and
  * :492-554 (TwoPoint64) with the ray box fixed to [0,1]^3

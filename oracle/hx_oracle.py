@@ -48,21 +48,25 @@ def lib():
             "hxo_trace_brute": (None, [_p, _p, _i64, _p, _i64, _p]),
             "hxo_bin_canonical": (_i64, [_p, _i64, _i64, _p, _p, _i64]),
             "hxo_validate_bin": (_i32, [_p, _i64, _i64, _p, _i64, _p, _p]),
+            "hxo_validate_bin_refs": (_i32, [_p, _i64, _i64, _p, _i64, _i64, _p, _p]),
+            "hxo_presplit": (_i64, [_p, _p, _i64, _p, _f32, _p, _p, _p, _i64, _p, _p]),
+            "hxo_ps_pow": (_f32, [_f32, _f32]),
+            "hxo_ps_split": (None, [_p, _p, _i32, _p, _p, _p, _p, _p]),
             "hxo_build_lbvh": (_i64, [_p, _p, _i64, _p, _i32, _i32, _i32, _f32, _f32,
-                                      _p, _p, _p, _p, _p, _p]),
+                                      _p, _p, _p, _p, _p, _p, _f32]),
             "hxo_ploc": (_i32, [_i64, _i32, _i32, _i32, _p, _p, _p, _p, _p]),
             "hxo_atr": (_i64, [_i64, _i32, _i32, _f32, _f32, _p, _p, _p, _p, _p]),
             "hxo_build_atr": (_i64, [_p, _p, _i64, _p, _i32, _i32, _i32, _i32, _i32, _f32, _f32,
-                                     _p, _p, _p, _p, _p, _p]),
+                                     _p, _p, _p, _p, _p, _p, _f32]),
             "hxo_hlbvh": (_i64, [_i64, _i32, _i32, _p, _p, _p, _p, _p, _p]),
             "hxo_build_hlbvh": (_i64, [_p, _p, _i64, _p, _i32, _i32, _i32, _i32, _f32, _f32,
-                                       _p, _p, _p, _p, _p, _p]),
+                                       _p, _p, _p, _p, _p, _p, _f32]),
             "hxo_build_ploc": (_i64, [_p, _p, _i64, _p, _i32, _i32, _i32, _i32, _i32, _i32, _f32, _f32,
-                                      _p, _p, _p, _p, _p, _p]),
+                                      _p, _p, _p, _p, _p, _p, _f32]),
             "hxo_ins_find": (None, [_i64, _p, _p, _p, _p, _p, _p, _p]),
             "hxo_insertion": (_i64, [_i64, _i32, _f32, _f32, _p, _p, _p, _p, _p, _p]),
             "hxo_build_insertion": (_i64, [_p, _p, _i64, _p, _i32, _i32, _i32, _i32, _f32, _f32,
-                                           _p, _p, _p, _p, _p, _p]),
+                                           _p, _p, _p, _p, _p, _p, _f32]),
             "hxo_ray_keys_aila64": (None, [_p, _i64, _i32, _p]),
             "hxo_ray_keys_twopoint64": (None, [_p, _i64, _i32, _f32, _p]),
         }
@@ -211,22 +215,66 @@ def validate_bin(nodes, n_interior, n_leaves, tri_idx, verts, tris):
                                   _ptr(v), _ptr(t))
 
 
-def build_lbvh(verts, tris, bits=60, scene_box=None, adaptive=True, max_leaf_size=8, ct=3.0, ci=2.0):
+def build_lbvh(verts, tris, bits=60, scene_box=None, adaptive=True, max_leaf_size=8, ct=3.0, ci=2.0, presplit_beta=None):
     """Whole LBVH build; returns dict with nodes, tri_idx, woop, counts and stage seconds."""
     v, t = _c(verts, np.float32), _c(tris, np.int32)
-    n = t.shape[0]
+    sb = None if scene_box is None else _c(scene_box, np.float32)
+    beta = -1.0 if presplit_beta is None else float(presplit_beta)
+    n = t.shape[0] if presplit_beta is None else presplit_count(v, t, beta, sb)
     N = max(2 * n - 1, 2)
     nodes = np.zeros((N, 16), np.int32)
     tri = np.zeros(n, np.int32)
     woop = np.zeros((n, 12), np.float32)
     ni, nl = ctypes.c_int64(0), ctypes.c_int64(0)
     sec = np.zeros(8, np.float64)
-    sb = None if scene_box is None else _c(scene_box, np.float32)
-    nn = lib().hxo_build_lbvh(_ptr(v), _ptr(t), n, _ptr(sb), int(bits), int(bool(adaptive)),
+    nn = lib().hxo_build_lbvh(_ptr(v), _ptr(t), t.shape[0], _ptr(sb), int(bits), int(bool(adaptive)),
                               int(max_leaf_size), float(ct), float(ci), _ptr(nodes), _ptr(tri),
-                              _ptr(woop), ctypes.byref(ni), ctypes.byref(nl), _ptr(sec))
+                              _ptr(woop), ctypes.byref(ni), ctypes.byref(nl), _ptr(sec), beta)
     return dict(nodes=nodes[:nn].copy(), tri_idx=tri, woop=woop, n_interior=ni.value,
                 n_leaves=nl.value, seconds=sec)
+
+
+def presplit_count(verts, tris, beta, scene_box=None):
+    v, t = _c(verts, np.float32), _c(tris, np.int32)
+    sb = None if scene_box is None else _c(scene_box, np.float32)
+    return int(lib().hxo_presplit(_ptr(v), _ptr(t), t.shape[0], _ptr(sb), float(beta), None, None, None, 0, None, None))
+
+
+def presplit(verts, tris, beta, scene_box=None):
+    """Presplitter references: dict(ref_min, ref_max (float4 rows), ref_tri, D, priorities)."""
+    v, t = _c(verts, np.float32), _c(tris, np.int32)
+    sb = None if scene_box is None else _c(scene_box, np.float32)
+    m = presplit_count(v, t, beta, sb)
+    rmin = np.zeros((m, 4), np.float32)
+    rmax = np.zeros((m, 4), np.float32)
+    rtri = np.zeros(m, np.int32)
+    D = ctypes.c_float(0.0)
+    prio = np.zeros(t.shape[0], np.float32)
+    lib().hxo_presplit(_ptr(v), _ptr(t), t.shape[0], _ptr(sb), float(beta), _ptr(rmin), _ptr(rmax), _ptr(rtri), m,
+                       ctypes.byref(D), _ptr(prio))
+    return dict(ref_min=rmin, ref_max=rmax, ref_tri=rtri, D=D.value, priorities=prio)
+
+
+def ps_pow(x, y):
+    return float(lib().hxo_ps_pow(float(x), float(y)))
+
+
+def ps_split(tri_verts, box, count, scene_box):
+    """One presplitter split; returns (left box[6], left count, right box[6], right count)."""
+    v = _c(np.asarray(tri_verts, np.float32).reshape(9), np.float32)
+    b = _c(np.asarray(box, np.float32).reshape(6), np.float32)
+    sb = _c(np.asarray(scene_box, np.float32).reshape(6), np.float32)
+    lb, rb = np.zeros(6, np.float32), np.zeros(6, np.float32)
+    lc, rc = ctypes.c_int32(0), ctypes.c_int32(0)
+    lib().hxo_ps_split(_ptr(v), _ptr(b), int(count), _ptr(sb), _ptr(lb), _ptr(rb), ctypes.byref(lc), ctypes.byref(rc))
+    return lb, lc.value, rb, rc.value
+
+
+def validate_bin_refs(nodes, n_interior, n_leaves, tri_idx, verts, tris):
+    nd, ti = _c(nodes, np.int32), _c(tri_idx, np.int32)
+    v, t = _c(verts, np.float32), _c(tris, np.int32)
+    return lib().hxo_validate_bin_refs(_ptr(nd), int(n_interior), int(n_leaves), _ptr(ti), ti.shape[0], t.shape[0],
+                                       _ptr(v), _ptr(t))
 
 
 def ploc(leaf_min, leaf_max, radius, max_radius=512, adaptive=False):
@@ -244,20 +292,21 @@ def ploc(leaf_min, leaf_max, radius, max_radius=512, adaptive=False):
 
 
 def build_ploc(verts, tris, bits=60, radius=25, scene_box=None, adaptive=True, max_leaf_size=8, ct=3.0, ci=2.0,
-               max_radius=512, ploc_adaptive=False):
+               max_radius=512, ploc_adaptive=False, presplit_beta=None):
     v, t = _c(verts, np.float32), _c(tris, np.int32)
-    n = t.shape[0]
+    sb = None if scene_box is None else _c(scene_box, np.float32)
+    beta = -1.0 if presplit_beta is None else float(presplit_beta)
+    n = t.shape[0] if presplit_beta is None else presplit_count(v, t, beta, sb)
     N = max(2 * n - 1, 2)
     nodes = np.zeros((N, 16), np.int32)
     tri = np.zeros(n, np.int32)
     woop = np.zeros((n, 12), np.float32)
     ni, nl = ctypes.c_int64(0), ctypes.c_int64(0)
     sec = np.zeros(8, np.float64)
-    sb = None if scene_box is None else _c(scene_box, np.float32)
-    nn = lib().hxo_build_ploc(_ptr(v), _ptr(t), n, _ptr(sb), int(bits), int(radius), int(max_radius),
+    nn = lib().hxo_build_ploc(_ptr(v), _ptr(t), t.shape[0], _ptr(sb), int(bits), int(radius), int(max_radius),
                               int(bool(ploc_adaptive)), int(bool(adaptive)),
                               int(max_leaf_size), float(ct), float(ci), _ptr(nodes), _ptr(tri), _ptr(woop),
-                              ctypes.byref(ni), ctypes.byref(nl), _ptr(sec))
+                              ctypes.byref(ni), ctypes.byref(nl), _ptr(sec), beta)
     return dict(nodes=nodes[:nn].copy(), tri_idx=tri, woop=woop, n_interior=ni.value, n_leaves=nl.value,
                 seconds=sec)
 
@@ -273,18 +322,19 @@ def atr(left, right, parent, nmin, nmax, treelet_size=9, iterations=2, ct=3.0, c
 
 
 def build_atr(verts, tris, bits=60, treelet_size=9, iterations=2, scene_box=None, adaptive=True, max_leaf_size=8,
-              ct=3.0, ci=2.0):
+              ct=3.0, ci=2.0, presplit_beta=None):
     v, t = _c(verts, np.float32), _c(tris, np.int32)
-    n = t.shape[0]
+    sb = None if scene_box is None else _c(scene_box, np.float32)
+    beta = -1.0 if presplit_beta is None else float(presplit_beta)
+    n = t.shape[0] if presplit_beta is None else presplit_count(v, t, beta, sb)
     N = max(2 * n - 1, 2)
     nodes = np.zeros((N, 16), np.int32)
     tri = np.zeros(n, np.int32)
     woop = np.zeros((n, 12), np.float32)
     ni, nl, acc = ctypes.c_int64(0), ctypes.c_int64(0), ctypes.c_int64(0)
-    sb = None if scene_box is None else _c(scene_box, np.float32)
-    nn = lib().hxo_build_atr(_ptr(v), _ptr(t), n, _ptr(sb), int(bits), int(treelet_size), int(iterations),
+    nn = lib().hxo_build_atr(_ptr(v), _ptr(t), t.shape[0], _ptr(sb), int(bits), int(treelet_size), int(iterations),
                              int(bool(adaptive)), int(max_leaf_size), float(ct), float(ci), _ptr(nodes), _ptr(tri),
-                             _ptr(woop), ctypes.byref(ni), ctypes.byref(nl), ctypes.byref(acc))
+                             _ptr(woop), ctypes.byref(ni), ctypes.byref(nl), ctypes.byref(acc), beta)
     return dict(nodes=nodes[:nn].copy(), tri_idx=tri, woop=woop, n_interior=ni.value, n_leaves=nl.value,
                 accepted=acc.value)
 
@@ -300,18 +350,19 @@ def hlbvh(sorted_keys, left, right, parent, nmin, nmax, bits=60, sah_bits=15):
 
 
 def build_hlbvh(verts, tris, bits=60, sah_bits=15, scene_box=None, adaptive=True, max_leaf_size=8, ct=3.0,
-                ci=2.0):
+                ci=2.0, presplit_beta=None):
     v, t = _c(verts, np.float32), _c(tris, np.int32)
-    n = t.shape[0]
+    sb = None if scene_box is None else _c(scene_box, np.float32)
+    beta = -1.0 if presplit_beta is None else float(presplit_beta)
+    n = t.shape[0] if presplit_beta is None else presplit_count(v, t, beta, sb)
     N = max(2 * n - 1, 2)
     nodes = np.zeros((N, 16), np.int32)
     tri = np.zeros(n, np.int32)
     woop = np.zeros((n, 12), np.float32)
     ni, nl, cl = ctypes.c_int64(0), ctypes.c_int64(0), ctypes.c_int64(0)
-    sb = None if scene_box is None else _c(scene_box, np.float32)
-    nn = lib().hxo_build_hlbvh(_ptr(v), _ptr(t), n, _ptr(sb), int(bits), int(sah_bits), int(bool(adaptive)),
+    nn = lib().hxo_build_hlbvh(_ptr(v), _ptr(t), t.shape[0], _ptr(sb), int(bits), int(sah_bits), int(bool(adaptive)),
                                int(max_leaf_size), float(ct), float(ci), _ptr(nodes), _ptr(tri), _ptr(woop),
-                               ctypes.byref(ni), ctypes.byref(nl), ctypes.byref(cl))
+                               ctypes.byref(ni), ctypes.byref(nl), ctypes.byref(cl), beta)
     return dict(nodes=nodes[:nn].copy(), tri_idx=tri, woop=woop, n_interior=ni.value, n_leaves=nl.value,
                 clusters=cl.value)
 
@@ -339,18 +390,19 @@ def insertion(left, right, parent, nmin, nmax, mod=1, ct=3.0, ci=2.0):
     return l, r, p, mn, mx, k, steps.value
 
 
-def build_insertion(verts, tris, bits=60, mod=1, scene_box=None, adaptive=True, max_leaf_size=8, ct=3.0, ci=2.0):
+def build_insertion(verts, tris, bits=60, mod=1, scene_box=None, adaptive=True, max_leaf_size=8, ct=3.0, ci=2.0, presplit_beta=None):
     v, t = _c(verts, np.float32), _c(tris, np.int32)
-    n = t.shape[0]
+    sb = None if scene_box is None else _c(scene_box, np.float32)
+    beta = -1.0 if presplit_beta is None else float(presplit_beta)
+    n = t.shape[0] if presplit_beta is None else presplit_count(v, t, beta, sb)
     N = max(2 * n - 1, 2)
     nodes = np.zeros((N, 16), np.int32)
     tri = np.zeros(n, np.int32)
     woop = np.zeros((n, 12), np.float32)
     ni, nl, steps = ctypes.c_int64(0), ctypes.c_int64(0), ctypes.c_int64(0)
-    sb = None if scene_box is None else _c(scene_box, np.float32)
-    nn = lib().hxo_build_insertion(_ptr(v), _ptr(t), n, _ptr(sb), int(bits), int(mod), int(bool(adaptive)),
+    nn = lib().hxo_build_insertion(_ptr(v), _ptr(t), t.shape[0], _ptr(sb), int(bits), int(mod), int(bool(adaptive)),
                                    int(max_leaf_size), float(ct), float(ci), _ptr(nodes), _ptr(tri), _ptr(woop),
-                                   ctypes.byref(ni), ctypes.byref(nl), ctypes.byref(steps))
+                                   ctypes.byref(ni), ctypes.byref(nl), ctypes.byref(steps), beta)
     return dict(nodes=nodes[:nn].copy(), tri_idx=tri, woop=woop, n_interior=ni.value, n_leaves=nl.value,
                 steps=steps.value)
 
