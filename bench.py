@@ -11,8 +11,8 @@ value = build throughput in Mtris/s; trace throughput is reported beside it.
 
 Also measured in the same run (reported beside `value`):
   builders  PLOC (radius 25), ATRBVH (treelet 9, 2 iterations), HLBVH (15 SAH
-            bits) and insertion (mod 1) rebuilds of the same scene, with their
-            SAH cost against the LBVH's;
+            bits), insertion (mod 1) and presplit (beta 0.3) LBVH rebuilds of
+            the same scene, with their SAH cost against the LBVH's;
   batched   4096 meshes of round(exp(U[ln 64, ln 262144])) triangles (half
             displaced icospheres, half soups, each in its own unit box): one
             batched build against a loop of per-mesh rebuilds, and one
@@ -48,7 +48,7 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=0, help="triangles in the CPU baseline sample (0 = all)")
     ap.add_argument("--cpu-rays", type=int, default=4194304, help="rays in the CPU trace sample")
-    ap.add_argument("--builders", default="ploc,atr,hlbvh,insertion", help="other builders to time on the same scene ('' = none)")
+    ap.add_argument("--builders", default="ploc,atr,hlbvh,insertion,presplit", help="other builders to time on the same scene ('' = none)")
     ap.add_argument("--batch-meshes", type=int, default=4096, help="meshes in the batched workload (0 = skip)")
     ap.add_argument("--batch-lo", type=int, default=64)
     ap.add_argument("--batch-hi", type=int, default=262144)
@@ -114,7 +114,8 @@ def bench_builders(ctx, scene, n, args):
     specs = {"ploc": ("ploc_r25", hx.PLOCBuilder, dict(ploc_radius=25)),
              "atr": ("atr_t9_i2", hx.ATRBuilder, dict(atr_treelet_size=9, atr_iterations=2)),
              "hlbvh": ("hlbvh_s15", hx.HLBVHBuilder, dict(hlbvh_sah_bits=15)),
-             "insertion": ("insertion_m1", hx.InsertionBuilder, dict(insertion_mod=1))}
+             "insertion": ("insertion_m1", hx.InsertionBuilder, dict(insertion_mod=1)),
+             "presplit": ("presplit_b03_lbvh", hx.LBVHBuilder, dict(presplitting=1, presplitter_beta=0.3))}
     for k in kinds:
         name, cls, kw = specs[k]
         b = cls(ctx, morton_bits=60, **kw)
@@ -128,6 +129,8 @@ def bench_builders(ctx, scene, n, args):
             out[name]["rounds"] = bvh.info("ploc_rounds")
         if k == "hlbvh":
             out[name]["clusters"] = bvh.info("hlbvh_clusters")
+        if k == "presplit":
+            out[name]["references"] = bvh.info("references")
         if k == "insertion":
             out[name]["steps"] = bvh.info("insertion_steps")
             out[name]["reinserted"] = bvh.info("insertion_reinserted")
