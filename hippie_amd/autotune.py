@@ -6,6 +6,7 @@ depth (HipTracerBinKernels.cu:5-15), radix digit width (RadixSortKernels.h
 measurement on the caller's own workload:
 
   sort_digit_bits      radix digit width of every sort       8, 10
+  sort_items           keys per thread in a sort pass        16, 24, 32
   trace_lds_stack      traversal-stack entries kept in LDS   0, 16, 24, 32, 64
   trace_fetch          wave refill threshold (busy lanes)    16 .. 64
   trace_blocks_per_cu  persistent trace workgroups per CU    0 (occupancy), 1, 2, 4
@@ -28,6 +29,7 @@ import torch
 from . import core
 
 SORT_DIGITS = (8, 10)
+SORT_ITEMS = (16, 24, 32)
 LDS_STACK = (0, 16, 24, 32, 64)
 FETCH = (16, 24, 32, 40, 48, 56, 64)
 BLOCKS_PER_CU = (0, 1, 2, 4)
@@ -113,7 +115,7 @@ class Autotuner:
         def run():
             keys.copy_(src)
             core.sort_pairs(self.ctx, keys, vals, bits)
-        best = self._descend([("sort_digit_bits", SORT_DIGITS)], run)
+        best = self._descend([("sort_digit_bits", SORT_DIGITS), ("sort_items", SORT_ITEMS)], run)
         self.cache[key] = best
         self._save()
         return best

@@ -372,11 +372,19 @@ __global__ __launch_bounds__(kClimbThreads) void k_climb_local(ClimbArgs a, int3
     constexpr int TT = GENERAL ? 1 : T;
     __shared__ int s_first[TT], s_last[TT], s_left[TT], s_right[TT], s_par[TT];
     __shared__ int s_state[TT];
+    // Nodes whose parent straddles the block (at most one per leaf) and the
+    // collapse input end_at for the block's leaf positions: gathered in LDS and
+    // written once per block instead of one global atomic per entry.
+    __shared__ int s_front[T];
+    __shared__ int s_end[T];
+    __shared__ int s_nfront, s_fbase;
     const int n = (int)a.n;
     const int base = blockIdx.x * T;
     const int last_leaf = min(base + T, n) - 1;
+    if (threadIdx.x == 0) s_nfront = 0;
     for (int i = threadIdx.x; i < T; i += kClimbThreads) {
         s_cnt[i] = 0;
+        s_end[i] = -1;
         if (!GENERAL && base + i < n - 1) {
             s_first[i] = a.first[base + i];
             s_last[i] = a.last[base + i];
@@ -444,13 +452,13 @@ __global__ __launch_bounds__(kClimbThreads) void k_climb_local(ClimbArgs a, int3
             const int li = node - base;
             // A Karras node finishing in this block has its index in the block.
             if (!GENERAL && (unsigned)li >= (unsigned)T) {
-                frontier[atomicAdd(frontier_count, 1)] = child;
+                s_front[atomicAdd(&s_nfront, 1)] = child;
                 break;
             }
             const int first = GENERAL ? a.first[node] : s_first[li];
             const int last = GENERAL ? a.last[node] : s_last[li];
             if (first < base || last > last_leaf) {
-                frontier[atomicAdd(frontier_count, 1)] = child;
+                s_front[atomicAdd(&s_nfront, 1)] = child;
                 break;
             }
             const int sn = GENERAL ? a.split[node] - base : li;
@@ -483,15 +491,20 @@ __global__ __launch_bounds__(kClimbThreads) void k_climb_local(ClimbArgs a, int3
                 s_state[li] = c.state;
             }
             if (!a.refit_only && c.state == 0 && !seg_top(a.mesh_of, a.mesh_off, node, first, last))
-                atomicMax(&a.end_at[first], last);
+                atomicMax(&s_end[first - base], last);
             child = node;
             node = GENERAL ? a.parent[node] : s_par[li];
         }
     }
+    __syncthreads();
+    if (threadIdx.x == 0) s_fbase = atomicAdd(frontier_count, s_nfront);
+    __syncthreads();
+    for (int i = threadIdx.x; i < s_nfront; i += kClimbThreads) frontier[s_fbase + i] = s_front[i];
+    if (!a.refit_only)
+        for (int i = threadIdx.x; i < T && base + i <= last_leaf; i += kClimbThreads) a.end_at[base + i] = s_end[i];
     if (GENERAL) return;
     // Karras trees: the nodes this block finished (both children arrived,
     // s_cnt == 2) are the block's index range; write them out coalesced.
-    __syncthreads();
     for (int i = threadIdx.x; i < T; i += kClimbThreads) {
         const int node = base + i;
         if (node >= n - 1 || s_cnt[i] != 2) continue;

@@ -70,6 +70,7 @@ struct Tuning {
     int trace_fetch = 40;        // refill a wave when fewer lanes than this are still busy
     int trace_blocks_per_cu = 0; // persistent trace workgroups per CU (0 = occupancy limit)
     int sort_digit_bits = 8;     // radix digit width: 8 or 10
+    int sort_items = 16;         // keys per thread in a sort pass: 16, 24 or 32
 };
 
 struct Context {

@@ -25,16 +25,16 @@ namespace {
 
 constexpr int kSortBlock = 256;
 constexpr int kSortWaves = kSortBlock / kWave;
-constexpr int kSortItems = 16;
-constexpr int kSortTile = kSortBlock * kSortItems;
+constexpr int kSortItems = 16;  // keys per thread in the histogram grid sizing
 constexpr int kMaxPasses = 8;    // 64-bit keys at 8-bit digits; 7 at 10-bit
 
+constexpr int kLookback = 4;    // predecessors read per look-back round trip
 constexpr uint32_t kFlagAggregate = 1u << 30;
 constexpr uint32_t kFlagPrefix = 2u << 30;
 constexpr uint32_t kValueMask = (1u << 30) - 1u;
 
 inline int sort_passes(int end_bit, int rb) { return (end_bit + rb - 1) / rb; }
-inline int64_t sort_tiles(int64_t n) { return (n + kSortTile - 1) / kSortTile; }
+inline int64_t sort_tiles(int64_t n, int items) { return (n + kSortBlock * items - 1) / (kSortBlock * items); }
 
 __global__ void k_zero_u32(uint32_t* p, int64_t n) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -51,7 +51,12 @@ HX_DEV uint32_t digit_of(K key, int shift, uint32_t mask) {
     return (uint32_t)(key >> shift) & mask;
 }
 
-// Histograms of every digit position in one read of the keys.
+// Histograms of every digit position in one read of the keys.  Each thread
+// counts a run of kHistRun consecutive keys and adds a digit's count to LDS
+// only when the digit changes: Morton keys in input order share their high
+// digits over long runs, which would otherwise hammer a few LDS words.
+constexpr int kHistRun = 16;
+
 template <class K, int RB>
 __global__ __launch_bounds__(kSortBlock) void k_sort_hist(const K* __restrict__ keys, int64_t n, int passes,
                                                           int end_bit, uint32_t* __restrict__ hist) {
@@ -60,14 +65,37 @@ __global__ __launch_bounds__(kSortBlock) void k_sort_hist(const K* __restrict__ 
     __shared__ uint32_t h[P][R];
     for (int i = threadIdx.x; i < P * R; i += kSortBlock) (&h[0][0])[i] = 0u;
     __syncthreads();
-    for (int64_t i = blockIdx.x * (int64_t)kSortBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kSortBlock) {
-        const K key = keys[i];
-        for (int p = 0; p < passes; ++p) {
-            const int shift = RB * p;
-            const int nb = min(RB, end_bit - shift);
-            atomicAdd(&h[p][digit_of(key, shift, (1u << nb) - 1u)], 1u);
+    uint32_t cur[P], cnt[P];
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        cur[p] = 0u;
+        cnt[p] = 0u;
+    }
+    const int64_t runs = (n + kHistRun - 1) / kHistRun;
+    for (int64_t r = blockIdx.x * (int64_t)kSortBlock + threadIdx.x; r < runs; r += (int64_t)gridDim.x * kSortBlock) {
+        const int64_t b = r * kHistRun;
+        const int m = (int)min((int64_t)kHistRun, n - b);
+        for (int j = 0; j < m; ++j) {
+            const K key = keys[b + j];
+#pragma unroll
+            for (int p = 0; p < P; ++p) {
+                if (p >= passes) continue;
+                const int shift = RB * p;
+                const int nb = min(RB, end_bit - shift);
+                const uint32_t d = digit_of(key, shift, (1u << nb) - 1u);
+                if (d == cur[p]) {
+                    ++cnt[p];
+                } else {
+                    if (cnt[p]) atomicAdd(&h[p][cur[p]], cnt[p]);
+                    cur[p] = d;
+                    cnt[p] = 1u;
+                }
+            }
         }
     }
+#pragma unroll
+    for (int p = 0; p < P; ++p)
+        if (p < passes && cnt[p]) atomicAdd(&h[p][cur[p]], cnt[p]);
     __syncthreads();
     for (int i = threadIdx.x; i < passes * R; i += kSortBlock) {
         const uint32_t c = (&h[0][0])[i];
@@ -98,7 +126,7 @@ __global__ __launch_bounds__(kSortBlock) void k_sort_scan_hist(uint32_t* __restr
     }
 }
 
-template <class K, bool IDENT, int RB>
+template <class K, bool IDENT, int RB, int IT>
 __global__ __launch_bounds__(kSortBlock) void k_onesweep(const K* __restrict__ keys_in, K* __restrict__ keys_out,
                                                          const int32_t* __restrict__ vals_in,
                                                          int32_t* __restrict__ vals_out, int64_t n, int shift,
@@ -107,14 +135,15 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const K* __restrict__ k
                                                          uint32_t* __restrict__ tile_counter) {
     constexpr int R = 1 << RB;
     constexpr int DPT = R / kSortBlock;  // digits per thread in the per-digit phase
+    constexpr int kTileKeys = kSortBlock * IT;
     __shared__ uint32_t s_tile;
     __shared__ uint32_t s_wave_hist[kSortWaves][R];
     __shared__ uint32_t s_digit_start[R];
     __shared__ uint32_t s_global_base[R];
     __shared__ uint32_t s_scan[kSortWaves];
     __shared__ union {
-        K keys[kSortTile];
-        int32_t vals[kSortTile];
+        K keys[kTileKeys];
+        int32_t vals[kTileKeys];
     } s_stage;
 
     const int tid = threadIdx.x;
@@ -126,20 +155,20 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const K* __restrict__ k
     for (int i = tid; i < kSortWaves * R; i += kSortBlock) (&s_wave_hist[0][0])[i] = 0u;
     __syncthreads();
     const int64_t tile = s_tile;
-    const int64_t tile_base = tile * kSortTile;
-    const int64_t wave_base = tile_base + (int64_t)w * kWave * kSortItems;
+    const int64_t tile_base = tile * kTileKeys;
+    const int64_t wave_base = tile_base + (int64_t)w * kWave * IT;
 
-    K key[kSortItems];
-    uint32_t pos[kSortItems];
+    K key[IT];
+    uint32_t pos[IT];
 #pragma unroll
-    for (int i = 0; i < kSortItems; ++i) {
+    for (int i = 0; i < IT; ++i) {
         const int64_t idx = wave_base + i * kWave + lane;
         key[i] = idx < n ? keys_in[idx] : (K)0;
     }
 
     // Rank within the wave, items in order, lanes in order: stable.
 #pragma unroll
-    for (int i = 0; i < kSortItems; ++i) {
+    for (int i = 0; i < IT; ++i) {
         const int64_t idx = wave_base + i * kWave + lane;
         const bool valid = idx < n;
         const uint32_t d = digit_of(key[i], shift, mask);
@@ -195,17 +224,29 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const K* __restrict__ k
         const int d = tid * DPT + k;
         uint32_t excl = 0;
         if (tile != 0) {
+            // Read kLookback predecessors per round trip, then consume them in
+            // order until a prefix (done) or an unpublished tile (wait, retry).
             int64_t j = tile - 1;
             while (true) {
-                const uint32_t s = atomic_load_relaxed(status + j * R + d);
-                const uint32_t flag = s & ~kValueMask;
-                if (flag == 0u) {
-                    __builtin_amdgcn_s_sleep(1);
-                    continue;
+                uint32_t sv[kLookback];
+#pragma unroll
+                for (int q = 0; q < kLookback; ++q)
+                    sv[q] = j - q >= 0 ? atomic_load_relaxed(status + (j - q) * R + d) : kFlagPrefix;
+                bool done = false, stall = false;
+#pragma unroll
+                for (int q = 0; q < kLookback; ++q) {
+                    if (done || stall) continue;
+                    const uint32_t flag = sv[q] & ~kValueMask;
+                    if (flag == 0u) {
+                        stall = true;
+                        continue;
+                    }
+                    excl += sv[q] & kValueMask;
+                    --j;
+                    done = flag == kFlagPrefix;
                 }
-                excl += s & kValueMask;
-                if (flag == kFlagPrefix) break;
-                --j;
+                if (done) break;
+                if (stall) __builtin_amdgcn_s_sleep(1);
             }
             atomic_store_relaxed(status + tile * R + d, kFlagPrefix | (excl + total[k]));
         }
@@ -215,7 +256,7 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const K* __restrict__ k
 
     // Stage keys in digit order.
 #pragma unroll
-    for (int i = 0; i < kSortItems; ++i) {
+    for (int i = 0; i < IT; ++i) {
         const int64_t idx = wave_base + i * kWave + lane;
         if (idx < n) {
             const uint32_t dd = digit_of(key[i], shift, mask);
@@ -225,10 +266,10 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const K* __restrict__ k
     }
     __syncthreads();
 
-    const int64_t tile_count = min((int64_t)kSortTile, n - tile_base);
-    int64_t dst[kSortItems];
+    const int64_t tile_count = min((int64_t)kTileKeys, n - tile_base);
+    int64_t dst[IT];
 #pragma unroll
-    for (int i = 0; i < kSortItems; ++i) {
+    for (int i = 0; i < IT; ++i) {
         const int slot = i * kSortBlock + tid;
         if (slot < tile_count) {
             const K k = s_stage.keys[slot];
@@ -240,19 +281,19 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const K* __restrict__ k
     __syncthreads();
 
 #pragma unroll
-    for (int i = 0; i < kSortItems; ++i) {
+    for (int i = 0; i < IT; ++i) {
         const int64_t idx = wave_base + i * kWave + lane;
         if (idx < n) s_stage.vals[pos[i]] = IDENT ? (int32_t)idx : vals_in[idx];
     }
     __syncthreads();
 #pragma unroll
-    for (int i = 0; i < kSortItems; ++i) {
+    for (int i = 0; i < IT; ++i) {
         const int slot = i * kSortBlock + tid;
         if (slot < tile_count) vals_out[dst[i]] = s_stage.vals[slot];
     }
 }
 
-template <class K, int RB>
+template <class K, int RB, int IT>
 int sort_impl(Context* c, void* keys[2], int32_t* vals[2], int64_t n, int end_bit, bool vals_identity, int* result) {
     constexpr int R = 1 << RB;
     const int passes = sort_passes(end_bit, RB);
@@ -264,7 +305,7 @@ int sort_impl(Context* c, void* keys[2], int32_t* vals[2], int64_t n, int end_bi
         }
         return HX_OK;
     }
-    const int64_t ntiles = sort_tiles(n);
+    const int64_t ntiles = sort_tiles(n, IT);
     const int64_t status_words = (int64_t)passes * ntiles * R;
     const int64_t words = status_words + (int64_t)passes * R + kMaxPasses;
     HX_TRY(c->sort_status.reserve(sizeof(uint32_t) * (size_t)words));
@@ -276,7 +317,7 @@ int sort_impl(Context* c, void* keys[2], int32_t* vals[2], int64_t n, int end_bi
         const unsigned zb = (unsigned)std::min<int64_t>((words + 255) / 256, 4096);
         hipLaunchKernelGGL(k_zero_u32, dim3(zb), dim3(256), 0, c->stream, status, words);
         HX_TRY(check_launch("sort_zero"));
-        const unsigned hb = (unsigned)std::min<int64_t>((n + kSortBlock * 16 - 1) / (kSortBlock * 16), 2048);
+        const unsigned hb = (unsigned)std::min<int64_t>((n + kSortBlock * kSortItems - 1) / (kSortBlock * kSortItems), 2048);
         hipLaunchKernelGGL((k_sort_hist<K, RB>), dim3(hb), dim3(kSortBlock), 0, c->stream,
                            static_cast<const K*>(keys[0]), n, passes, end_bit, hist);
         HX_TRY(check_launch("sort_hist"));
@@ -289,11 +330,11 @@ int sort_impl(Context* c, void* keys[2], int32_t* vals[2], int64_t n, int end_bi
         const int nb = std::min(RB, end_bit - shift);
         LaunchScope ls(c, "sort_onesweep");
         if (p == 0 && vals_identity)
-            hipLaunchKernelGGL((k_onesweep<K, true, RB>), dim3((unsigned)ntiles), dim3(kSortBlock), 0, c->stream,
+            hipLaunchKernelGGL((k_onesweep<K, true, RB, IT>), dim3((unsigned)ntiles), dim3(kSortBlock), 0, c->stream,
                                static_cast<const K*>(keys[in]), static_cast<K*>(keys[out]), (const int32_t*)nullptr,
                                vals[out], n, shift, nb, hist + p * R, status + (int64_t)p * ntiles * R, counters + p);
         else
-            hipLaunchKernelGGL((k_onesweep<K, false, RB>), dim3((unsigned)ntiles), dim3(kSortBlock), 0, c->stream,
+            hipLaunchKernelGGL((k_onesweep<K, false, RB, IT>), dim3((unsigned)ntiles), dim3(kSortBlock), 0, c->stream,
                                static_cast<const K*>(keys[in]), static_cast<K*>(keys[out]), vals[in], vals[out], n,
                                shift, nb, hist + p * R, status + (int64_t)p * ntiles * R, counters + p);
         HX_TRY(check_launch("sort_onesweep"));
@@ -312,12 +353,21 @@ int launch_sort_pairs(Context* c, void* keys[2], int32_t* vals[2], int64_t n, in
         set_error("radix sort: n = %lld exceeds 2^30 - 1", (long long)n);
         return HX_ERR_INVALID_ARG;
     }
-    if (c->tune.sort_digit_bits == 10) {
-        if (keys64) return sort_impl<uint64_t, 10>(c, keys, vals, n, end_bit, vals_identity, result_index);
-        return sort_impl<uint32_t, 10>(c, keys, vals, n, end_bit, vals_identity, result_index);
+    const int bits = c->tune.sort_digit_bits, items = c->tune.sort_items;
+#define HX_SORT_CASE(RB, IT)                                                                         \
+    if (bits == RB && items == IT) {                                                                \
+        if (keys64) return sort_impl<uint64_t, RB, IT>(c, keys, vals, n, end_bit, vals_identity, result_index); \
+        return sort_impl<uint32_t, RB, IT>(c, keys, vals, n, end_bit, vals_identity, result_index);           \
     }
-    if (keys64) return sort_impl<uint64_t, 8>(c, keys, vals, n, end_bit, vals_identity, result_index);
-    return sort_impl<uint32_t, 8>(c, keys, vals, n, end_bit, vals_identity, result_index);
+    HX_SORT_CASE(8, 16)
+    HX_SORT_CASE(8, 24)
+    HX_SORT_CASE(8, 32)
+    HX_SORT_CASE(10, 16)
+    HX_SORT_CASE(10, 24)
+    HX_SORT_CASE(10, 32)
+#undef HX_SORT_CASE
+    set_error("radix sort: no kernel for %d-bit digits with %d keys per thread", bits, items);
+    return HX_ERR_INVALID_ARG;
 }
 
 }  // namespace hx

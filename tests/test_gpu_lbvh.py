@@ -217,3 +217,38 @@ def test_sort_pairs_10bit_digits(digits10, n, end_bit):
 def test_lbvh_10bit_digits(oracle, digits10, bits):
     v, t = scenes.gaussian_mixture_soup(100000, seed=bits)
     _check_build(oracle, digits10, v, t, bits)
+
+
+@pytest.mark.parametrize("digits", [8, 10])
+@pytest.mark.parametrize("items", [24, 32])
+def test_sort_pairs_keys_per_thread(ctx, digits, items):
+    """Every (digit width, keys per thread) sort kernel gives the stable order."""
+    old = (ctx.get_tuning("sort_digit_bits"), ctx.get_tuning("sort_items"))
+    ctx.set_tuning("sort_digit_bits", digits)
+    ctx.set_tuning("sort_items", items)
+    try:
+        for n, end_bit in ((5000, 30), (1000003, 60), (2, 64)):
+            rng = np.random.default_rng(n + items)
+            hi = np.uint64((1 << end_bit) - 1) if end_bit < 64 else np.uint64(0xFFFFFFFFFFFFFFFF)
+            k = rng.integers(0, 1 << 63, n, dtype=np.uint64) & hi
+            k[: n // 3] = k[0]
+            vals = rng.permutation(n).astype(np.int32)
+            kt = torch.from_numpy(k.view(np.int64).copy()).cuda()
+            vt = torch.from_numpy(vals.copy()).cuda()
+            hx.sort_pairs(ctx, kt, vt, end_bit)
+            order = np.argsort(k, kind="stable")
+            assert np.array_equal(kt.cpu().numpy().view(np.uint64), k[order])
+            assert np.array_equal(vt.cpu().numpy(), vals[order])
+    finally:
+        ctx.set_tuning("sort_digit_bits", old[0])
+        ctx.set_tuning("sort_items", old[1])
+
+
+def test_lbvh_sort_32_keys_per_thread(oracle, ctx):
+    old = ctx.get_tuning("sort_items")
+    ctx.set_tuning("sort_items", 32)
+    try:
+        v, t = scenes.gaussian_mixture_soup(200000, seed=5)
+        _check_build(oracle, ctx, v, t, 60)
+    finally:
+        ctx.set_tuning("sort_items", old)
