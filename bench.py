@@ -418,6 +418,33 @@ def main():
     bytes_per_launch = sort_bytes_per_step / max(launches_per_step, 1)
     achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9 if sort_cnt else 0.0
 
+    # Every LBVH build kernel against the same peak, from its algorithmic
+    # bytes per launch (n triangles; SAH-collapsed node count for the emit):
+    #   scene_box  12 B index + 36 B vertices per triangle           48n
+    #   morton     the same reads + an 8 B key                          56n
+    #   sort_hist  one read of the 8 B keys                              8 per key
+    #   construct  8 B keys read; left/right/parent/first/last, leaf
+    #              entries, counters, cover written                     52n
+    #   climb      48 B triangle + 4 B id + 4 B leaf parent + 20 B of
+    #              topology read; leaf and interior boxes (64 B), Woop
+    #              (48 B), costs (8 B), state, cover written            204n
+    #   emit       ~100 B read per internal node, 64 B per Bin node     100n + 64 nodes
+    nodes_out = bvh.n_interior + bvh.n_leaves
+    alg = {"scene_box": 48 * n, "morton": 56 * n, "sort_hist": 8 * n, "construct": 52 * n, "climb": 204 * n,
+           "collapse_emit": 100 * n + 64 * nodes_out}
+    build_kernels = {}
+    for k, b in alg.items():
+        cnt, tot = prof.get(k, (0, 0.0))
+        if not cnt:
+            continue
+        per_launch = b
+        if k == "sort_hist":
+            # one histogram per sort: the build's keys and the two ray sets
+            per_launch = (8 * n + 8 * (prim.shape[0] + sec.shape[0])) / (cnt / args.steps)
+        gbs = per_launch / (tot / cnt * 1e-3) / 1e9
+        build_kernels[k] = {"avg_ms": round(tot / cnt, 5), "bytes_per_launch": int(per_launch),
+                            "achieved_gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+
     builders = bench_builders(ctx, scene, n, args) if args.builders else {}
     if builders:
         builders["lbvh"] = {"ms": round(build_ms, 3), "sah": round(bvh.get_cost(), 4)}
@@ -466,6 +493,7 @@ def main():
             "bytes_per_launch": int(bytes_per_launch),
             "avg_launch_ms": round(avg_launch_ms, 5),
         },
+        "roofline_build_kernels": build_kernels,
         "cpu_baseline": cpu,
         "tuning": tuning,
         "builders": builders,

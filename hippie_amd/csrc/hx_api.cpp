@@ -957,6 +957,9 @@ int hx_context_set_tuning(hx_context* ctx, const char* name, int32_t value) {
     } else if (!std::strcmp(name, "sort_digit_bits")) {
         HX_CHECK_ARG(value == 8 || value == 10, "sort_digit_bits must be 8 or 10");
         t.sort_digit_bits = value;
+    } else if (!std::strcmp(name, "sort_lookback")) {
+        HX_CHECK_ARG(value >= 1 && value <= 16, "sort_lookback must be in [1,16]");
+        t.sort_lookback = value;
     } else if (!std::strcmp(name, "sort_items")) {
         HX_CHECK_ARG(value == 16 || value == 24 || value == 32, "sort_items must be 16, 24 or 32");
         t.sort_items = value;
@@ -975,6 +978,7 @@ int hx_context_get_tuning(hx_context* ctx, const char* name, int32_t* value) {
     else if (!std::strcmp(name, "trace_blocks_per_cu")) *value = t.trace_blocks_per_cu;
     else if (!std::strcmp(name, "sort_digit_bits")) *value = t.sort_digit_bits;
     else if (!std::strcmp(name, "sort_items")) *value = t.sort_items;
+    else if (!std::strcmp(name, "sort_lookback")) *value = t.sort_lookback;
     else {
         hx::set_error("unknown tuning parameter '%s'", name);
         return HX_ERR_INVALID_ARG;

@@ -901,12 +901,19 @@ __global__ __launch_bounds__(kBlock) void k_cost_partial(const int4* __restrict_
     if (threadIdx.x == 0) partial[blockIdx.x] = s[0];
 }
 
-__global__ void k_cost_final(const double* __restrict__ partial, int nblocks, double* __restrict__ out) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        double acc = 0.0;
-        for (int i = 0; i < nblocks; ++i) acc += partial[i];
-        out[0] = acc;
+// One workgroup: strided per-thread sums, then a fixed-order tree.
+__global__ __launch_bounds__(kBlock) void k_cost_final(const double* __restrict__ partial, int nblocks,
+                                                      double* __restrict__ out) {
+    __shared__ double s[kBlock];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < nblocks; i += kBlock) acc += partial[i];
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
     }
+    if (threadIdx.x == 0) out[0] = s[0];
 }
 
 __global__ void k_refit_reset(int32_t* __restrict__ counters, int64_t n_internal, int32_t* __restrict__ frontier_count) {
@@ -1063,7 +1070,7 @@ int launch_cost(Context* c, const int4* nodes, const int32_t* d_counts, int64_t 
     hipLaunchKernelGGL(k_cost_partial, dim3(nblocks), dim3(kBlock), 0, c->stream, nodes, d_counts, d_box6, ct, ci,
                        c->scan_partials.as<double>());
     HX_TRY(check_launch("cost_partial"));
-    hipLaunchKernelGGL(k_cost_final, dim3(1), dim3(64), 0, c->stream, c->scan_partials.as<double>(), nblocks, d_out);
+    hipLaunchKernelGGL(k_cost_final, dim3(1), dim3(kBlock), 0, c->stream, c->scan_partials.as<double>(), nblocks, d_out);
     return check_launch("cost_final");
 }
 

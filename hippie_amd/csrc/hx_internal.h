@@ -71,6 +71,7 @@ struct Tuning {
     int trace_blocks_per_cu = 0; // persistent trace workgroups per CU (0 = occupancy limit)
     int sort_digit_bits = 8;     // radix digit width: 8 or 10
     int sort_items = 16;         // keys per thread in a sort pass: 16, 24 or 32
+    int sort_lookback = 4;       // predecessor tiles read per look-back round trip: 1..16
 };
 
 struct Context {

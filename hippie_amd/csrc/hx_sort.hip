@@ -28,7 +28,7 @@ constexpr int kSortWaves = kSortBlock / kWave;
 constexpr int kSortItems = 16;  // keys per thread in the histogram grid sizing
 constexpr int kMaxPasses = 8;    // 64-bit keys at 8-bit digits; 7 at 10-bit
 
-constexpr int kLookback = 4;    // predecessors read per look-back round trip
+constexpr int kMaxLookback = 16;  // predecessors read per look-back round trip (tunable up to this)
 constexpr uint32_t kFlagAggregate = 1u << 30;
 constexpr uint32_t kFlagPrefix = 2u << 30;
 constexpr uint32_t kValueMask = (1u << 30) - 1u;
@@ -51,12 +51,7 @@ HX_DEV uint32_t digit_of(K key, int shift, uint32_t mask) {
     return (uint32_t)(key >> shift) & mask;
 }
 
-// Histograms of every digit position in one read of the keys.  Each thread
-// counts a run of kHistRun consecutive keys and adds a digit's count to LDS
-// only when the digit changes: Morton keys in input order share their high
-// digits over long runs, which would otherwise hammer a few LDS words.
-constexpr int kHistRun = 16;
-
+// Histograms of every digit position in one read of the keys.
 template <class K, int RB>
 __global__ __launch_bounds__(kSortBlock) void k_sort_hist(const K* __restrict__ keys, int64_t n, int passes,
                                                           int end_bit, uint32_t* __restrict__ hist) {
@@ -65,37 +60,14 @@ __global__ __launch_bounds__(kSortBlock) void k_sort_hist(const K* __restrict__ 
     __shared__ uint32_t h[P][R];
     for (int i = threadIdx.x; i < P * R; i += kSortBlock) (&h[0][0])[i] = 0u;
     __syncthreads();
-    uint32_t cur[P], cnt[P];
-#pragma unroll
-    for (int p = 0; p < P; ++p) {
-        cur[p] = 0u;
-        cnt[p] = 0u;
-    }
-    const int64_t runs = (n + kHistRun - 1) / kHistRun;
-    for (int64_t r = blockIdx.x * (int64_t)kSortBlock + threadIdx.x; r < runs; r += (int64_t)gridDim.x * kSortBlock) {
-        const int64_t b = r * kHistRun;
-        const int m = (int)min((int64_t)kHistRun, n - b);
-        for (int j = 0; j < m; ++j) {
-            const K key = keys[b + j];
-#pragma unroll
-            for (int p = 0; p < P; ++p) {
-                if (p >= passes) continue;
-                const int shift = RB * p;
-                const int nb = min(RB, end_bit - shift);
-                const uint32_t d = digit_of(key, shift, (1u << nb) - 1u);
-                if (d == cur[p]) {
-                    ++cnt[p];
-                } else {
-                    if (cnt[p]) atomicAdd(&h[p][cur[p]], cnt[p]);
-                    cur[p] = d;
-                    cnt[p] = 1u;
-                }
-            }
+    for (int64_t i = blockIdx.x * (int64_t)kSortBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kSortBlock) {
+        const K key = keys[i];
+        for (int p = 0; p < passes; ++p) {
+            const int shift = RB * p;
+            const int nb = min(RB, end_bit - shift);
+            atomicAdd(&h[p][digit_of(key, shift, (1u << nb) - 1u)], 1u);
         }
     }
-#pragma unroll
-    for (int p = 0; p < P; ++p)
-        if (p < passes && cnt[p]) atomicAdd(&h[p][cur[p]], cnt[p]);
     __syncthreads();
     for (int i = threadIdx.x; i < passes * R; i += kSortBlock) {
         const uint32_t c = (&h[0][0])[i];
@@ -127,12 +99,12 @@ __global__ __launch_bounds__(kSortBlock) void k_sort_scan_hist(uint32_t* __restr
 }
 
 template <class K, bool IDENT, int RB, int IT>
-__global__ __launch_bounds__(kSortBlock) void k_onesweep(const K* __restrict__ keys_in, K* __restrict__ keys_out,
+__global__ __launch_bounds__(kSortBlock) __attribute__((amdgpu_waves_per_eu(IT <= 16 ? 4 : 2))) void k_onesweep(const K* __restrict__ keys_in, K* __restrict__ keys_out,
                                                          const int32_t* __restrict__ vals_in,
                                                          int32_t* __restrict__ vals_out, int64_t n, int shift,
                                                          int nbits, const uint32_t* __restrict__ digit_offsets,
                                                          uint32_t* __restrict__ status,
-                                                         uint32_t* __restrict__ tile_counter) {
+                                                         uint32_t* __restrict__ tile_counter, int lookback) {
     constexpr int R = 1 << RB;
     constexpr int DPT = R / kSortBlock;  // digits per thread in the per-digit phase
     constexpr int kTileKeys = kSortBlock * IT;
@@ -158,15 +130,26 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const K* __restrict__ k
     const int64_t tile_base = tile * kTileKeys;
     const int64_t wave_base = tile_base + (int64_t)w * kWave * IT;
 
+    // Keys and values are loaded up front so that the value loads' latency
+    // hides behind the ranking.
     K key[IT];
+    int32_t val[IT];
     uint32_t pos[IT];
 #pragma unroll
     for (int i = 0; i < IT; ++i) {
         const int64_t idx = wave_base + i * kWave + lane;
         key[i] = idx < n ? keys_in[idx] : (K)0;
+        val[i] = IDENT ? (int32_t)idx : (idx < n ? vals_in[idx] : 0);
     }
 
-    // Rank within the wave, items in order, lanes in order: stable.
+    // Rank within the wave, items in order, lanes in order: stable.  Phase 1
+    // finds each item's peers (lanes with the same digit) by ballots and packs
+    // (rank among peers, first peer lane, peer count); phase 2 lets the first
+    // peer of every item add the count to the wave's digit counter -- all
+    // items' LDS atomics issue back to back and execute in program order, so
+    // each returns the count of the earlier items; phase 3 hands the returned
+    // base to the other peers.
+    uint32_t packed[IT];
 #pragma unroll
     for (int i = 0; i < IT; ++i) {
         const int64_t idx = wave_base + i * kWave + lane;
@@ -178,12 +161,22 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const K* __restrict__ k
             const uint64_t bb = ballot64(bit);
             peers &= bit ? bb : ~bb;
         }
-        if (valid) {
-            const int below = count_below(peers);
-            const uint32_t prev = s_wave_hist[w][d];
-            pos[i] = prev + (uint32_t)below;
-            if (below == 0) s_wave_hist[w][d] = prev + (uint32_t)__popcll(peers);
-        }
+        const uint32_t below = (uint32_t)count_below(peers);
+        const uint32_t leader = peers ? (uint32_t)__builtin_ctzll(peers) : 0u;
+        const uint32_t cnt = (uint32_t)__popcll(peers);
+        packed[i] = valid ? (below | (leader << 8) | (cnt << 16)) : 0xFFFFFFFFu;
+    }
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        uint32_t old = 0u;
+        if (packed[i] != 0xFFFFFFFFu && (packed[i] & 0xFFu) == 0u)
+            old = atomicAdd(&s_wave_hist[w][digit_of(key[i], shift, mask)], packed[i] >> 16);
+        pos[i] = old;
+    }
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        const uint32_t base = __shfl(pos[i], (int)((packed[i] >> 8) & 0xFFu), kWave);
+        pos[i] = base + (packed[i] & 0xFFu);
     }
     __syncthreads();
 
@@ -224,18 +217,22 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const K* __restrict__ k
         const int d = tid * DPT + k;
         uint32_t excl = 0;
         if (tile != 0) {
-            // Read kLookback predecessors per round trip, then consume them in
-            // order until a prefix (done) or an unpublished tile (wait, retry).
+            // Read `lookback` predecessors per round trip, then consume them
+            // in order until a prefix (done) or an unpublished tile (wait, retry).
             int64_t j = tile - 1;
             while (true) {
-                uint32_t sv[kLookback];
+                uint32_t sv[kMaxLookback];
 #pragma unroll
-                for (int q = 0; q < kLookback; ++q)
-                    sv[q] = j - q >= 0 ? atomic_load_relaxed(status + (j - q) * R + d) : kFlagPrefix;
+                for (int q = 0; q < kMaxLookback; ++q)
+                    sv[q] = (q < lookback && j - q >= 0) ? atomic_load_relaxed(status + (j - q) * R + d) : 0u;
                 bool done = false, stall = false;
 #pragma unroll
-                for (int q = 0; q < kLookback; ++q) {
-                    if (done || stall) continue;
+                for (int q = 0; q < kMaxLookback; ++q) {
+                    if (done || stall || q >= lookback) continue;
+                    if (j < 0) {
+                        done = true;
+                        continue;
+                    }
                     const uint32_t flag = sv[q] & ~kValueMask;
                     if (flag == 0u) {
                         stall = true;
@@ -254,36 +251,45 @@ __global__ __launch_bounds__(kSortBlock) void k_onesweep(const K* __restrict__ k
     }
     __syncthreads();
 
-    // Stage keys in digit order.
+    // Stage keys in digit order: all LDS offsets first, then the writes.
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        const uint32_t dd = digit_of(key[i], shift, mask);
+        pos[i] += s_digit_start[dd] + s_wave_hist[w][dd];
+    }
 #pragma unroll
     for (int i = 0; i < IT; ++i) {
         const int64_t idx = wave_base + i * kWave + lane;
-        if (idx < n) {
-            const uint32_t dd = digit_of(key[i], shift, mask);
-            pos[i] += s_digit_start[dd] + s_wave_hist[w][dd];
-            s_stage.keys[pos[i]] = key[i];
-        }
+        if (idx < n) s_stage.keys[pos[i]] = key[i];
     }
     __syncthreads();
 
-    const int64_t tile_count = min((int64_t)kTileKeys, n - tile_base);
-    int64_t dst[IT];
+    // Write out in contiguous digit runs: read the staged keys, then their
+    // global slots, then store.
+    const int tile_count = (int)min((int64_t)kTileKeys, n - tile_base);
+    uint32_t dst[IT];
 #pragma unroll
     for (int i = 0; i < IT; ++i) {
         const int slot = i * kSortBlock + tid;
-        if (slot < tile_count) {
-            const K k = s_stage.keys[slot];
-            const uint32_t dd = digit_of(k, shift, mask);
-            dst[i] = (int64_t)s_global_base[dd] + (slot - (int64_t)s_digit_start[dd]);
-            keys_out[dst[i]] = k;
-        }
+        key[i] = slot < tile_count ? s_stage.keys[slot] : (K)0;
+    }
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        const int slot = i * kSortBlock + tid;
+        const uint32_t dd = digit_of(key[i], shift, mask);
+        dst[i] = s_global_base[dd] + ((uint32_t)slot - s_digit_start[dd]);
+    }
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+        const int slot = i * kSortBlock + tid;
+        if (slot < tile_count) keys_out[dst[i]] = key[i];
     }
     __syncthreads();
 
 #pragma unroll
     for (int i = 0; i < IT; ++i) {
         const int64_t idx = wave_base + i * kWave + lane;
-        if (idx < n) s_stage.vals[pos[i]] = IDENT ? (int32_t)idx : vals_in[idx];
+        if (idx < n) s_stage.vals[pos[i]] = val[i];
     }
     __syncthreads();
 #pragma unroll
@@ -332,11 +338,13 @@ int sort_impl(Context* c, void* keys[2], int32_t* vals[2], int64_t n, int end_bi
         if (p == 0 && vals_identity)
             hipLaunchKernelGGL((k_onesweep<K, true, RB, IT>), dim3((unsigned)ntiles), dim3(kSortBlock), 0, c->stream,
                                static_cast<const K*>(keys[in]), static_cast<K*>(keys[out]), (const int32_t*)nullptr,
-                               vals[out], n, shift, nb, hist + p * R, status + (int64_t)p * ntiles * R, counters + p);
+                               vals[out], n, shift, nb, hist + p * R, status + (int64_t)p * ntiles * R, counters + p,
+                               c->tune.sort_lookback);
         else
             hipLaunchKernelGGL((k_onesweep<K, false, RB, IT>), dim3((unsigned)ntiles), dim3(kSortBlock), 0, c->stream,
                                static_cast<const K*>(keys[in]), static_cast<K*>(keys[out]), vals[in], vals[out], n,
-                               shift, nb, hist + p * R, status + (int64_t)p * ntiles * R, counters + p);
+                               shift, nb, hist + p * R, status + (int64_t)p * ntiles * R, counters + p,
+                               c->tune.sort_lookback);
         HX_TRY(check_launch("sort_onesweep"));
     }
     *result = passes & 1;

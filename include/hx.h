@@ -146,7 +146,8 @@ int hx_context_profile_reset(hx_context* ctx);
  *   "trace_fetch"          refill a wave when fewer of its 64 lanes are busy
  *   "trace_blocks_per_cu"  persistent trace workgroups per CU (0 = occupancy)
  *   "sort_digit_bits"      radix digit width of every sort (8 or 10)
- *   "sort_items"           keys per thread in a sort pass (16, 24 or 32)    */
+ *   "sort_items"           keys per thread in a sort pass (16, 24 or 32)
+ *   "sort_lookback"        tiles read per sort look-back round trip (1-16)  */
 int hx_context_set_tuning(hx_context* ctx, const char* name, int32_t value);
 int hx_context_get_tuning(hx_context* ctx, const char* name, int32_t* value);
 

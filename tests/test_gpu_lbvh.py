@@ -219,13 +219,14 @@ def test_lbvh_10bit_digits(oracle, digits10, bits):
     _check_build(oracle, digits10, v, t, bits)
 
 
-@pytest.mark.parametrize("digits", [8, 10])
-@pytest.mark.parametrize("items", [24, 32])
-def test_sort_pairs_keys_per_thread(ctx, digits, items):
-    """Every (digit width, keys per thread) sort kernel gives the stable order."""
-    old = (ctx.get_tuning("sort_digit_bits"), ctx.get_tuning("sort_items"))
+@pytest.mark.parametrize("digits,items,lookback", [(8, 24, 4), (8, 32, 1), (10, 24, 16), (10, 32, 8), (8, 16, 16),
+                                                   (8, 16, 1)])
+def test_sort_pairs_keys_per_thread(ctx, digits, items, lookback):
+    """Every (digit width, keys per thread, look-back width) gives the stable order."""
+    old = (ctx.get_tuning("sort_digit_bits"), ctx.get_tuning("sort_items"), ctx.get_tuning("sort_lookback"))
     ctx.set_tuning("sort_digit_bits", digits)
     ctx.set_tuning("sort_items", items)
+    ctx.set_tuning("sort_lookback", lookback)
     try:
         for n, end_bit in ((5000, 30), (1000003, 60), (2, 64)):
             rng = np.random.default_rng(n + items)
@@ -242,6 +243,7 @@ def test_sort_pairs_keys_per_thread(ctx, digits, items):
     finally:
         ctx.set_tuning("sort_digit_bits", old[0])
         ctx.set_tuning("sort_items", old[1])
+        ctx.set_tuning("sort_lookback", old[2])
 
 
 def test_lbvh_sort_32_keys_per_thread(oracle, ctx):
