@@ -161,6 +161,11 @@ struct hx_bvh {
     std::vector<int4> h_tab;
     bool tab_known = false;
     DevBuf mesh_off, mesh_of, mesh_box, mesh_box_bits, keys_orig, tri_local;
+    // With presplitting a mesh's references and triangles have different
+    // offsets: mesh_off / mesh_of / h_roff are over reference slots, tri_off /
+    // tri_mesh_of over triangles.
+    DevBuf tri_off, tri_mesh_of;
+    std::vector<int32_t> h_roff;
     bool built = false;
     bool counts_known = false;
     int64_t n_interior = 0, n_leaves = 0;
@@ -368,32 +373,52 @@ hx::EmitArgs emit_args(hx_bvh* b) {
 
 // Presplitter::presplit (Presplitter.cpp:304-324) in place of setupBoxes:
 // priorities and D, then the references; the BVH's buffers are sized for
-// the reference count.
+// the reference count.  In a batch every mesh is presplit alone (its own box
+// and budget) and h_roff receives the per-mesh reference offsets.
 int run_presplit(hx_bvh* b) {
     Context* c = &b->ctx->c;
     const int64_t n = b->n_tri;
+    const int M = b->batched ? b->n_meshes : 1;
     const int64_t ntiles = (n + 4095) / 4096;
     HX_TRY(b->ps_prio.reserve(4 * (size_t)n));
-    HX_TRY(b->ps_work.reserve(4 * (size_t)ntiles + 1024));
+    HX_TRY(b->ps_work.reserve(16 * (size_t)M + 4 * (size_t)M + 4 * (size_t)ntiles + 2048));
     hx::PsArgs a{};
     a.n = n;
     a.verts = b->verts;
     a.tris = b->tris;
     a.box = b->scene_box.as<float>();
+    if (b->batched) {
+        a.mesh_of = b->tri_mesh_of.as<int32_t>();
+        a.mesh_box = b->mesh_box.as<float>();
+    }
     a.prio = b->ps_prio.as<float>();
     char* w = b->ps_work.as<char>();
-    a.acc = reinterpret_cast<unsigned long long*>(w);
-    a.qcount = reinterpret_cast<int32_t*>(w + 256);
-    a.partial = reinterpret_cast<int32_t*>(w + 512);
+    auto carve_w = [&](size_t bytes) {
+        char* r = w;
+        w += (bytes + 255) & ~(size_t)255;
+        return r;
+    };
+    a.macc = reinterpret_cast<unsigned long long*>(carve_w(16 * (size_t)M));
+    a.Dm = reinterpret_cast<float*>(carve_w(4 * (size_t)M));
+    a.qcount = reinterpret_cast<int32_t*>(carve_w(8));
+    a.partial = reinterpret_cast<int32_t*>(carve_w(4 * (size_t)ntiles));
+    std::vector<int64_t> tri_counts(M), ref_counts(M);
+    std::vector<float> D(M);
+    for (int m = 0; m < M; ++m) tri_counts[m] = b->batched ? (int64_t)b->h_off[m + 1] - b->h_off[m] : n;
+    HX_TRY(hx::launch_presplit_count(c, a, b->params.presplitter_beta, M, tri_counts.data(), ref_counts.data(),
+                                     D.data()));
     int64_t n_refs = 0;
-    float D = 0.0f;
-    HX_TRY(hx::launch_presplit_count(c, a, b->params.presplitter_beta, &n_refs, &D));
+    for (int m = 0; m < M; ++m) n_refs += ref_counts[m];
     if (n_refs >= ((int64_t)1 << 30)) {
         hx::set_error("presplitting gives %lld references (limit 2^30)", (long long)n_refs);
         return HX_ERR_UNSUPPORTED;
     }
     b->n_refs = n_refs;
-    b->ps_D = D;
+    b->ps_D = D[0];
+    if (b->batched) {
+        b->h_roff.assign(M + 1, 0);
+        for (int m = 0; m < M; ++m) b->h_roff[m + 1] = b->h_roff[m] + (int32_t)ref_counts[m];
+    }
     HX_TRY(reserve_bvh(b, n_refs));
     HX_TRY(b->ref_min.reserve(16 * (size_t)n_refs));
     HX_TRY(b->ref_max.reserve(16 * (size_t)n_refs));
@@ -415,7 +440,7 @@ int run_presplit(hx_bvh* b) {
     a.ref_min = b->ref_min.as<float4>();
     a.ref_max = b->ref_max.as<float4>();
     a.ref_tri = b->ref_tri.as<int32_t>();
-    HX_TRY(hx::launch_presplit_refs(c, a, D, n_refs));
+    HX_TRY(hx::launch_presplit_refs(c, a, n_refs));
     b->refs_active = true;
     return HX_OK;
 }
@@ -640,17 +665,34 @@ int run_insertion(hx_bvh* b) {
 int run_batch(hx_bvh* b) {
     Context* c = &b->ctx->c;
     const hx_build_params& p = b->params;
-    const int64_t n = b->n_refs;
     const int M = b->n_meshes;
+    const int64_t n_tri = b->n_tri;
     HX_HIP(hipSetDevice(c->device));
+    HX_TRY(b->tri_off.reserve(4 * (size_t)(M + 1)));
+    HX_TRY(b->tri_mesh_of.reserve(4 * (size_t)n_tri));
+    int32_t* toff = b->tri_off.as<int32_t>();
+    int32_t* tmesh = b->tri_mesh_of.as<int32_t>();
+    HX_HIP(hipMemcpyAsync(toff, b->h_off.data(), 4 * (size_t)(M + 1), hipMemcpyHostToDevice, c->stream));
+    HX_TRY(hx::launch_mesh_of(c, toff, M, n_tri, tmesh));
+    HX_TRY(hx::launch_mesh_boxes(c, b->verts, b->tris, n_tri, tmesh, M, b->mesh_box_bits.as<uint32_t>(),
+                                 b->mesh_box.as<float>()));
+    b->refs_active = false;
+    if (p.presplitting) {
+        HX_TRY(run_presplit(b));
+    } else {
+        b->n_refs = n_tri;
+        b->h_roff = b->h_off;
+    }
+    const int64_t n = b->n_refs;
     int32_t* off = b->mesh_off.as<int32_t>();
     int32_t* mesh_of = b->mesh_of.as<int32_t>();
-    HX_HIP(hipMemcpyAsync(off, b->h_off.data(), 4 * (size_t)(M + 1), hipMemcpyHostToDevice, c->stream));
+    HX_HIP(hipMemcpyAsync(off, b->h_roff.data(), 4 * (size_t)(M + 1), hipMemcpyHostToDevice, c->stream));
     HX_TRY(hx::launch_mesh_of(c, off, M, n, mesh_of));
-    HX_TRY(hx::launch_mesh_boxes(c, b->verts, b->tris, n, mesh_of, M, b->mesh_box_bits.as<uint32_t>(),
-                                 b->mesh_box.as<float>()));
+    const float4* rmin = b->refs_active ? b->ref_min.as<float4>() : nullptr;
+    const float4* rmax = b->refs_active ? b->ref_max.as<float4>() : nullptr;
     uint64_t* korig = b->keys_orig.as<uint64_t>();
-    HX_TRY(hx::launch_morton(c, b->verts, b->tris, n, b->mesh_box.as<float>(), p.morton_bits, korig, true, mesh_of));
+    HX_TRY(hx::launch_morton(c, b->verts, b->tris, n, b->mesh_box.as<float>(), p.morton_bits, korig, true, mesh_of,
+                             rmin, rmax));
     HX_HIP(hipMemcpyAsync(b->keys[0].ptr, korig, 8 * (size_t)n, hipMemcpyDeviceToDevice, c->stream));
     void* keys[2] = {b->keys[0].ptr, b->keys[1].ptr};
     int32_t* vals[2] = {b->tri_idx.as<int32_t>(), b->vals_tmp.as<int32_t>()};
@@ -676,7 +718,11 @@ int run_batch(hx_bvh* b) {
                                 mesh_of, off));
     HX_TRY(hx::launch_climb(c, climb_args(b, false)));
     HX_TRY(hx::launch_collapse_emit(c, emit_args(b)));
-    HX_TRY(hx::launch_tri_local(c, b->tri_idx.as<int32_t>(), mesh_of, off, n, b->tri_local.as<int32_t>()));
+    if (b->refs_active) {
+        HX_TRY(hx::launch_refs_to_tris(c, b->tri_idx.as<int32_t>(), b->ref_tri.as<int32_t>(), n));
+        b->refs_active = false;
+    }
+    HX_TRY(hx::launch_tri_local(c, b->tri_idx.as<int32_t>(), mesh_of, toff, n, b->tri_local.as<int32_t>()));
     b->counts_known = false;
     b->tab_known = false;
     b->built = true;
@@ -958,10 +1004,10 @@ int hx_context_set_tuning(hx_context* ctx, const char* name, int32_t value) {
         HX_CHECK_ARG(value == 8 || value == 10, "sort_digit_bits must be 8 or 10");
         t.sort_digit_bits = value;
     } else if (!std::strcmp(name, "sort_lookback")) {
-        HX_CHECK_ARG(value >= 1 && value <= 16, "sort_lookback must be in [1,16]");
+        HX_CHECK_ARG(value == 4 || value == 8, "sort_lookback must be 4 or 8");
         t.sort_lookback = value;
     } else if (!std::strcmp(name, "sort_items")) {
-        HX_CHECK_ARG(value == 16 || value == 24 || value == 32, "sort_items must be 16, 24 or 32");
+        HX_CHECK_ARG(value == 16 || value == 32, "sort_items must be 16 or 32");
         t.sort_items = value;
     } else {
         hx::set_error("unknown tuning parameter '%s'", name);
@@ -1101,10 +1147,7 @@ int hx_build_batch(hx_context* ctx, const hx_build_params* params, const float* 
     *out = nullptr;
     HX_TRY(validate_params(params));
     HX_CHECK_ARG(params->method == HX_METHOD_LBVH, "batched builds use the LBVH method (got %d)", params->method);
-    if (params->presplitting) {
-        hx::set_error("batched builds do not presplit");
-        return HX_ERR_UNSUPPORTED;
-    }
+
     HX_CHECK_ARG(n_meshes >= 1, "n_meshes must be positive (got %d)", n_meshes);
     HX_CHECK_ARG(h_mesh_offsets[0] == 0, "mesh offsets must start at 0");
     const int64_t n = h_mesh_offsets[n_meshes];
@@ -1172,13 +1215,13 @@ int hx_batch_mesh(hx_batch* batch, int32_t mesh, hx_batch_mesh_view* out) {
     HX_TRY(read_tables(b));
     const int4 t = b->h_tab[mesh];
     const int next_leaf = mesh + 1 < b->n_meshes ? b->h_tab[mesh + 1].z : (int)b->n_leaves;
-    const int64_t off = b->h_off[mesh];
+    const int64_t off = b->h_roff[mesh];
     out->nodes = b->nodes.as<int32_t>() + 16 * (int64_t)t.x;
     out->woop = b->woop.as<float>() + 12 * off;
     out->tri_indices = b->tri_local.as<int32_t>() + off;
     out->n_interior = t.w;
     out->n_leaves = next_leaf - t.z;
-    out->n_references = b->h_off[mesh + 1] - off;
+    out->n_references = b->h_roff[mesh + 1] - off;
     out->tri_offset = off;
     return HX_OK;
 }
@@ -1228,7 +1271,7 @@ int hx_batch_backward(hx_context* ctx, hx_batch* batch, const float* d_vertices,
     a.dl_dv = d_dl_dv;
     a.dl_dverts = d_dl_dvertices;
     a.ray_mesh = d_ray_mesh;
-    a.mesh_off = batch->bvh.mesh_off.as<int32_t>();
+    a.mesh_off = batch->bvh.tri_off.as<int32_t>();
     return hx::launch_backward(&ctx->c, a);
 }
 
@@ -1246,7 +1289,7 @@ int hx_batch_recompute_hits(hx_context* ctx, hx_batch* batch, const float* d_ver
     a.rays = reinterpret_cast<const float4*>(d_rays);
     a.results = reinterpret_cast<const int4*>(d_results);
     a.ray_mesh = d_ray_mesh;
-    a.mesh_off = batch->bvh.mesh_off.as<int32_t>();
+    a.mesh_off = batch->bvh.tri_off.as<int32_t>();
     return hx::launch_recompute_hits(&ctx->c, a, d_t, d_u, d_v);
 }
 

@@ -70,8 +70,8 @@ struct Tuning {
     int trace_fetch = 40;        // refill a wave when fewer lanes than this are still busy
     int trace_blocks_per_cu = 0; // persistent trace workgroups per CU (0 = occupancy limit)
     int sort_digit_bits = 8;     // radix digit width: 8 or 10
-    int sort_items = 16;         // keys per thread in a sort pass: 16, 24 or 32
-    int sort_lookback = 4;       // predecessor tiles read per look-back round trip: 1..16
+    int sort_items = 16;         // keys per thread in a sort pass: 16 or 32
+    int sort_lookback = 4;       // predecessor tiles read per look-back round trip: 4 or 8
 };
 
 struct Context {
@@ -266,17 +266,21 @@ int launch_atr(Context* c, const AtrArgs& a, const PlocArgs& relabel, int64_t* a
 int launch_relabel_by_height(Context* c, const AtrArgs& a, const PlocArgs& relabel);
 
 // hx_presplit.hip: budgeted presplitting of n triangles into references.
-// Phase 1 (priorities, D, the reference count) returns n_refs so that the
-// caller can size the reference arrays (n_refs each) and the two task queues
-// (queue_cap = max(1, n_refs - n) entries each, enough since every queued
-// task holds at least one unit of the budget); phase 2 writes them.
+// Phase 1 (priorities, D per mesh, the reference counts) returns the counts
+// so that the caller can size the reference arrays (n_refs each) and the two
+// task queues (queue_cap = max(1, n_refs - n) entries each, enough since every
+// queued task holds at least one unit of the budget); phase 2 writes them.
+// Without mesh_of everything is one mesh framed by box.
 struct PsArgs {
     int64_t n;
     const float* verts;
     const int32_t* tris;
     const float* box;           // device scene box (mn.xyz, mx.xyz)
+    const int32_t* mesh_of;     // batched: mesh of each triangle (else NULL)
+    const float* mesh_box;      // batched: 6 floats per mesh
+    float* Dm;                  // per-mesh multiplier (M floats, device)
     float* prio;                // n
-    unsigned long long* acc;    // 4
+    unsigned long long* macc;   // 2 per mesh: priority fixed-point sum, budget sum
     int32_t* partial;           // one entry per 4096 triangles
     float4* ref_min;            // n_refs
     float4* ref_max;
@@ -294,8 +298,9 @@ inline int32_t f2i_host(float x) {
     if (x <= -2147483648.0f) return (int32_t)0x80000000;
     return (int32_t)x;
 }
-int launch_presplit_count(Context* c, const PsArgs& a, float beta, int64_t* n_refs_out, float* D_out);
-int launch_presplit_refs(Context* c, const PsArgs& a, float D, int64_t n_refs);
+int launch_presplit_count(Context* c, const PsArgs& a, float beta, int M, const int64_t* tri_counts,
+                          int64_t* ref_counts, float* D_out);
+int launch_presplit_refs(Context* c, const PsArgs& a, int64_t n_refs);
 // ids[p] = ref_tri[ids[p]]: reference ids to triangle ids after a build.
 int launch_refs_to_tris(Context* c, int32_t* ids, const int32_t* ref_tri, int64_t n);
 

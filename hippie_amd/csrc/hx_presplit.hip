@@ -19,6 +19,10 @@
 // slot is fixed by the split tree alone, whatever order the tasks run in.
 // The float sum of the priorities (truncated to int by the reference) is an
 // exact 2^-32 fixed-point sum here, and the rounded sums are 64-bit integers.
+//
+// Batches: with a mesh per triangle, every mesh is presplit as if alone -- its
+// own box frames the codes, its own budget beta * n_m and multiplier D_m --
+// and the bisection runs for all meshes at once (per-mesh sums each round).
 #include <algorithm>
 #include <vector>
 
@@ -128,6 +132,27 @@ HX_DEV Box3 tri_bounds(const float3 v[3]) {
     return b;
 }
 
+HX_DEV int mesh_at(const PsArgs& a, int t) { return a.mesh_of ? a.mesh_of[t] : 0; }
+HX_DEV const float* frame_box(const PsArgs& a, int m) { return a.mesh_of ? a.mesh_box + 6 * (int64_t)m : a.box; }
+
+// Adds s to acc[2 * m + k]: once per wave when the wave's live lanes share a
+// mesh (all but the waves at mesh boundaries), else per lane.
+HX_DEV void mesh_add_u64(unsigned long long* acc, int k, bool live, int m, unsigned long long s) {
+    const int m0 = __shfl(live ? m : -1, __builtin_ctzll(ballot64(live) | (1ull << 63)), kWave);
+    const bool uniform = __all(!live || m == m0);
+    if (uniform) {
+#pragma unroll
+        for (int sh = 1; sh < kWave; sh <<= 1) {
+            const uint32_t lo = __shfl_xor((uint32_t)s, sh, kWave);
+            const uint32_t hi = __shfl_xor((uint32_t)(s >> 32), sh, kWave);
+            s += ((unsigned long long)hi << 32) | lo;
+        }
+        if (lane_id() == 0 && m0 >= 0 && s) atomicAdd(acc + 2 * (int64_t)m0 + k, s);
+    } else if (live && s) {
+        atomicAdd(acc + 2 * (int64_t)m + k, s);
+    }
+}
+
 // computePriorities (:11-71).
 HX_DEV float priority(const float3 v[3], const SceneFrame& S) {
     const Box3 box = tri_bounds(v);
@@ -146,47 +171,44 @@ HX_DEV float priority(const float3 v[3], const SceneFrame& S) {
     return series_pow(ldexpf(1.0f, i) * (box_area(box.mn, box.mx) - ideal), 1.0f / 3.0f);
 }
 
-// Sum of a per-lane 64-bit value over the wave, added once to *dst.
-HX_DEV void wave_add_u64(unsigned long long* dst, unsigned long long s) {
-#pragma unroll
-    for (int m = 1; m < kWave; m <<= 1) {
-        const uint32_t lo = __shfl_xor((uint32_t)s, m, kWave);
-        const uint32_t hi = __shfl_xor((uint32_t)(s >> 32), m, kWave);
-        s += ((unsigned long long)hi << 32) | lo;
-    }
-    if (lane_id() == 0 && s) atomicAdd(dst, s);
-}
-
-// Priorities and their exact fixed-point sum (acc[0]).
+// Priorities and their exact fixed-point sums per mesh (macc[2m]).
 __global__ __launch_bounds__(kBlock) void k_ps_priority(PsArgs a) {
     const int64_t t = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    const bool live = t < a.n;
     unsigned long long s = 0;
-    if (t < a.n) {
-        const SceneFrame S = frame_of(a.box);
+    int m = 0;
+    if (live) {
+        m = mesh_at(a, (int)t);
+        const SceneFrame S = frame_of(frame_box(a, m));
         float3 v[3];
         tri_verts(a, (int)t, v);
         const float p = priority(v, S);
         a.prio[t] = p;
         if (p >= 0.0f && p <= 1.0e9f) s = (unsigned long long)((double)p * 4294967296.0);
     }
-    wave_add_u64(a.acc, s);
+    mesh_add_u64(a.macc, 0, live, m, s);
 }
 
-// sumPrioritiesRound (:116-148): sum of floor(D p) (acc[1]).
-__global__ __launch_bounds__(kBlock) void k_ps_round_sum(PsArgs a, float D) {
+// sumPrioritiesRound (:116-148): per-mesh sums of floor(D_m p) (macc[2m+1]).
+__global__ __launch_bounds__(kBlock) void k_ps_round_sum(PsArgs a) {
+    const int64_t t = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    const bool live = t < a.n;
     unsigned long long s = 0;
-    for (int64_t t = blockIdx.x * (int64_t)kBlock + threadIdx.x; t < a.n; t += (int64_t)gridDim.x * kBlock)
-        s += (unsigned long long)(long long)f2i_sat(D * a.prio[t]);
-    wave_add_u64(a.acc + 1, s);
+    int m = 0;
+    if (live) {
+        m = mesh_at(a, (int)t);
+        s = (unsigned long long)(long long)f2i_sat(a.Dm[m] * a.prio[t]);
+    }
+    mesh_add_u64(a.macc, 1, live, m, s);
 }
 
 // Per-tile sums of floor(D p) + 1.
-__global__ __launch_bounds__(kBlock) void k_ps_tile_sums(PsArgs a, float D) {
+__global__ __launch_bounds__(kBlock) void k_ps_tile_sums(PsArgs a) {
     const int64_t base = blockIdx.x * (int64_t)kTile;
     int s = 0;
     for (int k = 0; k < kItems; ++k) {
         const int64_t t = base + k * kBlock + threadIdx.x;
-        if (t < a.n) s += f2i_sat(D * a.prio[t]) + 1;
+        if (t < a.n) s += f2i_sat(a.Dm[mesh_at(a, (int)t)] * a.prio[t]) + 1;
     }
     s = wave_sum_i32(s);
     __shared__ int ws[kWavesPerBlock];
@@ -238,14 +260,14 @@ HX_DEV void push_task(const PsArgs& a, int q, bool live, int tri, int count, int
 
 // initSplitTasks (:150-187): slot offsets from the tile scan; an unsplit
 // triangle is written as its own reference at once, the others are queued.
-__global__ __launch_bounds__(kBlock) void k_ps_init(PsArgs a, float D) {
+__global__ __launch_bounds__(kBlock) void k_ps_init(PsArgs a) {
     __shared__ int wt[kWavesPerBlock];
     const int64_t base = blockIdx.x * (int64_t)kTile;
     int carry = a.partial[blockIdx.x];
     for (int k = 0; k < kItems; ++k) {
         const int64_t t = base + k * kBlock + threadIdx.x;
         const bool in = t < a.n;
-        const int cnt = in ? f2i_sat(D * a.prio[t]) : 0;
+        const int cnt = in ? f2i_sat(a.Dm[mesh_at(a, (int)t)] * a.prio[t]) : 0;
         const int w = in ? cnt + 1 : 0;
         int tot;
         const int inc = block_inclusive_scan<kWavesPerBlock>(w, Sum(), wt, tot);
@@ -276,7 +298,7 @@ __global__ __launch_bounds__(kBlock) void k_ps_split(PsArgs a, int q, int count)
         box.mn = make_float3(bmn4.x, bmn4.y, bmn4.z);
         box.mx = make_float3(bmx4.x, bmx4.y, bmx4.z);
         tri = tk.x;
-        const SceneFrame S = frame_of(a.box);
+        const SceneFrame S = frame_of(frame_box(a, mesh_at(a, tri)));
         const int3 qmn = make_int3(f2i_sat(((box.mn.x - S.mn.x) / S.dg.x) * 2097151.9f),
                                    f2i_sat(((box.mn.y - S.mn.y) / S.dg.y) * 2097151.9f),
                                    f2i_sat(((box.mn.z - S.mn.z) / S.dg.z) * 2097151.9f));
@@ -343,30 +365,33 @@ __global__ __launch_bounds__(kBlock) void k_ps_split(PsArgs a, int q, int count)
     push_task(a, q ^ 1, live && rc > 0, tri, rc, roff, rb);
 }
 
-__global__ void k_ps_zero(unsigned long long* acc, int32_t* qcount) {
-    if (threadIdx.x < 4) acc[threadIdx.x] = 0ull;
-    if (threadIdx.x < 2) qcount[threadIdx.x] = 0;
+__global__ void k_ps_zero_u64(unsigned long long* p, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0ull;
 }
 
 __global__ void k_ps_zero_q(int32_t* qcount, int q) {
     if (threadIdx.x == 0) qcount[q] = 0;
 }
 
-int read_acc(Context* c, const PsArgs& a, unsigned long long h[4]) {
-    HX_HIP(hipMemcpyAsync(h, a.acc, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream));
-    HX_HIP(hipStreamSynchronize(c->stream));
-    return HX_OK;
+__global__ void k_ps_zero_both(int32_t* qcount) {
+    if (threadIdx.x < 2) qcount[threadIdx.x] = 0;
 }
 
-int round_sum(Context* c, const PsArgs& a, float D, int64_t* out) {
-    hipLaunchKernelGGL(k_ps_zero, dim3(1), dim3(64), 0, c->stream, a.acc, a.qcount);
+// Per-mesh round sums of floor(D_m p) for the multipliers `D` (M entries).
+int round_sums(Context* c, const PsArgs& a, int M, const std::vector<float>& D, std::vector<int64_t>& out) {
+    HX_HIP(hipMemcpyAsync(a.Dm, D.data(), sizeof(float) * (size_t)M, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_ps_zero_u64, dim3(std::min<unsigned>(grid_for(2 * (int64_t)M), 1024u)), dim3(kBlock), 0,
+                       c->stream, a.macc, 2 * (int64_t)M);
     HX_TRY(check_launch("ps_zero"));
-    hipLaunchKernelGGL(k_ps_round_sum, dim3(std::min<unsigned>(grid_for(a.n), 2048u)), dim3(kBlock), 0, c->stream, a,
-                       D);
+    hipLaunchKernelGGL(k_ps_round_sum, dim3(grid_for(a.n)), dim3(kBlock), 0, c->stream, a);
     HX_TRY(check_launch("ps_round_sum"));
-    unsigned long long h[4];
-    HX_TRY(read_acc(c, a, h));
-    *out = (int64_t)h[1];
+    std::vector<unsigned long long> h(2 * (size_t)M);
+    HX_HIP(hipMemcpyAsync(h.data(), a.macc, sizeof(unsigned long long) * h.size(), hipMemcpyDeviceToHost,
+                          c->stream));
+    HX_HIP(hipStreamSynchronize(c->stream));
+    out.resize(M);
+    for (int m = 0; m < M; ++m) out[m] = (int64_t)h[2 * (size_t)m + 1];
     return HX_OK;
 }
 
@@ -383,50 +408,64 @@ int launch_refs_to_tris(Context* c, int32_t* ids, const int32_t* ref_tri, int64_
     return check_launch("refs_to_tris");
 }
 
-int launch_presplit_count(Context* c, const PsArgs& a, float beta, int64_t* n_refs_out, float* D_out) {
+int launch_presplit_count(Context* c, const PsArgs& a, float beta, int M, const int64_t* tri_counts,
+                          int64_t* ref_counts, float* D_out) {
     LaunchScope ls(c, "presplit_count");
-    const int64_t n = a.n;
-    hipLaunchKernelGGL(k_ps_zero, dim3(1), dim3(64), 0, c->stream, a.acc, a.qcount);
+    hipLaunchKernelGGL(k_ps_zero_u64, dim3(std::min<unsigned>(grid_for(2 * (int64_t)M), 1024u)), dim3(kBlock), 0,
+                       c->stream, a.macc, 2 * (int64_t)M);
     HX_TRY(check_launch("ps_zero"));
-    hipLaunchKernelGGL(k_ps_priority, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, a);
+    hipLaunchKernelGGL(k_ps_priority, dim3(grid_for(a.n)), dim3(kBlock), 0, c->stream, a);
     HX_TRY(check_launch("ps_priority"));
-    unsigned long long h[4];
-    HX_TRY(read_acc(c, a, h));
-    // computeD (Presplitter.cpp:124-155) with Smax = (int)(beta * n).
-    const int64_t smax = f2i_host(beta * (float)n);
-    const int64_t s1 = (int64_t)(h[0] >> 32);
-    float D = 0.0f;
-    if (s1 > 0) {
-        float dmin = (float)(smax / s1);
-        int64_t s = 0;
-        HX_TRY(round_sum(c, a, dmin, &s));
-        float dmax = (dmin * (float)smax) / (float)s;
-        for (int r = 0; r < 6; ++r) {
-            const float davg = 0.5f * (dmin + dmax);
-            HX_TRY(round_sum(c, a, davg, &s));
-            if (s > smax) dmax = davg;
-            else dmin = davg;
-        }
-        D = dmin;
+    std::vector<unsigned long long> h(2 * (size_t)M);
+    HX_HIP(hipMemcpyAsync(h.data(), a.macc, sizeof(unsigned long long) * h.size(), hipMemcpyDeviceToHost,
+                          c->stream));
+    HX_HIP(hipStreamSynchronize(c->stream));
+    // computeD (Presplitter.cpp:124-155) per mesh, Smax = (int)(beta * n_m);
+    // a mesh whose priority sum floors to 0 gets D = 0 (no splits).
+    std::vector<int64_t> smax(M), s1(M), s;
+    std::vector<float> dmin(M, 0.0f), dmax(M, 0.0f), cur(M, 0.0f);
+    std::vector<char> live(M, 0);
+    bool any = false;
+    for (int m = 0; m < M; ++m) {
+        smax[m] = f2i_host(beta * (float)tri_counts[m]);
+        s1[m] = (int64_t)(h[2 * (size_t)m] >> 32);
+        live[m] = s1[m] > 0;
+        any = any || live[m];
+        if (live[m]) dmin[m] = (float)(smax[m] / s1[m]);
     }
-    int64_t s = 0;
-    HX_TRY(round_sum(c, a, D, &s));
-    *n_refs_out = n + s;
-    *D_out = D;
+    if (any) {
+        HX_TRY(round_sums(c, a, M, dmin, s));
+        for (int m = 0; m < M; ++m)
+            if (live[m]) dmax[m] = (dmin[m] * (float)smax[m]) / (float)s[m];
+        for (int r = 0; r < 6; ++r) {
+            for (int m = 0; m < M; ++m) cur[m] = live[m] ? 0.5f * (dmin[m] + dmax[m]) : 0.0f;
+            HX_TRY(round_sums(c, a, M, cur, s));
+            for (int m = 0; m < M; ++m) {
+                if (!live[m]) continue;
+                if (s[m] > smax[m]) dmax[m] = cur[m];
+                else dmin[m] = cur[m];
+            }
+        }
+    }
+    HX_TRY(round_sums(c, a, M, dmin, s));
+    for (int m = 0; m < M; ++m) {
+        ref_counts[m] = tri_counts[m] + s[m];
+        D_out[m] = dmin[m];
+    }
     return HX_OK;
 }
 
-int launch_presplit_refs(Context* c, const PsArgs& a, float D, int64_t n_refs) {
+int launch_presplit_refs(Context* c, const PsArgs& a, int64_t n_refs) {
     LaunchScope ls(c, "presplit_split");
     const int64_t n = a.n;
     const int ntiles = (int)((n + kTile - 1) / kTile);
-    hipLaunchKernelGGL(k_ps_zero, dim3(1), dim3(64), 0, c->stream, a.acc, a.qcount);
+    hipLaunchKernelGGL(k_ps_zero_both, dim3(1), dim3(64), 0, c->stream, a.qcount);
     HX_TRY(check_launch("ps_zero"));
-    hipLaunchKernelGGL(k_ps_tile_sums, dim3(ntiles), dim3(kBlock), 0, c->stream, a, D);
+    hipLaunchKernelGGL(k_ps_tile_sums, dim3(ntiles), dim3(kBlock), 0, c->stream, a);
     HX_TRY(check_launch("ps_tile_sums"));
     hipLaunchKernelGGL(k_ps_scan, dim3(1), dim3(kBlock), 0, c->stream, a.partial, ntiles);
     HX_TRY(check_launch("ps_scan"));
-    hipLaunchKernelGGL(k_ps_init, dim3(ntiles), dim3(kBlock), 0, c->stream, a, D);
+    hipLaunchKernelGGL(k_ps_init, dim3(ntiles), dim3(kBlock), 0, c->stream, a);
     HX_TRY(check_launch("ps_init"));
     int q = 0;
     int32_t count = 0;

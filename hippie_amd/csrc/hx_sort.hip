@@ -28,7 +28,6 @@ constexpr int kSortWaves = kSortBlock / kWave;
 constexpr int kSortItems = 16;  // keys per thread in the histogram grid sizing
 constexpr int kMaxPasses = 8;    // 64-bit keys at 8-bit digits; 7 at 10-bit
 
-constexpr int kMaxLookback = 16;  // predecessors read per look-back round trip (tunable up to this)
 constexpr uint32_t kFlagAggregate = 1u << 30;
 constexpr uint32_t kFlagPrefix = 2u << 30;
 constexpr uint32_t kValueMask = (1u << 30) - 1u;
@@ -60,12 +59,31 @@ __global__ __launch_bounds__(kSortBlock) void k_sort_hist(const K* __restrict__ 
     __shared__ uint32_t h[P][R];
     for (int i = threadIdx.x; i < P * R; i += kSortBlock) (&h[0][0])[i] = 0u;
     __syncthreads();
-    for (int64_t i = blockIdx.x * (int64_t)kSortBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kSortBlock) {
-        const K key = keys[i];
+    // Morton keys in input order share their high digits across a wave, and
+    // 64 lanes adding to one LDS word serialise: each pass first peels up to
+    // four digit values shared by several lanes (one add of the peer count
+    // each), then the remaining lanes add their own.
+    const int lane = lane_id();
+    const int64_t stride = (int64_t)gridDim.x * kSortBlock;
+    const int64_t n_round = (n + kWave - 1) / kWave * kWave;
+    for (int64_t i = blockIdx.x * (int64_t)kSortBlock + threadIdx.x; i < n_round; i += stride) {
+        const bool valid = i < n;
+        const K key = valid ? keys[i] : (K)0;
         for (int p = 0; p < passes; ++p) {
             const int shift = RB * p;
             const int nb = min(RB, end_bit - shift);
-            atomicAdd(&h[p][digit_of(key, shift, (1u << nb) - 1u)], 1u);
+            const uint32_t d = digit_of(key, shift, (1u << nb) - 1u);
+            uint64_t act = ballot64(valid);
+#pragma unroll
+            for (int it = 0; it < 4; ++it) {
+                if (!act) break;
+                const int src = __builtin_ctzll(act);
+                const uint32_t v = (uint32_t)__builtin_amdgcn_readlane((int)d, src);
+                const uint64_t same = ballot64(d == v) & act;
+                if (lane == src) atomicAdd(&h[p][v], (uint32_t)__popcll(same));
+                act &= ~same;
+            }
+            if ((act >> lane) & 1ull) atomicAdd(&h[p][d], 1u);
         }
     }
     __syncthreads();
@@ -98,13 +116,13 @@ __global__ __launch_bounds__(kSortBlock) void k_sort_scan_hist(uint32_t* __restr
     }
 }
 
-template <class K, bool IDENT, int RB, int IT>
+template <class K, bool IDENT, int RB, int IT, int LB>
 __global__ __launch_bounds__(kSortBlock) __attribute__((amdgpu_waves_per_eu(IT <= 16 ? 4 : 2))) void k_onesweep(const K* __restrict__ keys_in, K* __restrict__ keys_out,
                                                          const int32_t* __restrict__ vals_in,
                                                          int32_t* __restrict__ vals_out, int64_t n, int shift,
                                                          int nbits, const uint32_t* __restrict__ digit_offsets,
                                                          uint32_t* __restrict__ status,
-                                                         uint32_t* __restrict__ tile_counter, int lookback) {
+                                                         uint32_t* __restrict__ tile_counter) {
     constexpr int R = 1 << RB;
     constexpr int DPT = R / kSortBlock;  // digits per thread in the per-digit phase
     constexpr int kTileKeys = kSortBlock * IT;
@@ -217,22 +235,18 @@ __global__ __launch_bounds__(kSortBlock) __attribute__((amdgpu_waves_per_eu(IT <
         const int d = tid * DPT + k;
         uint32_t excl = 0;
         if (tile != 0) {
-            // Read `lookback` predecessors per round trip, then consume them
-            // in order until a prefix (done) or an unpublished tile (wait, retry).
+            // Read LB predecessors per round trip, then consume them in order
+            // until a prefix (done) or an unpublished tile (wait, retry).
             int64_t j = tile - 1;
             while (true) {
-                uint32_t sv[kMaxLookback];
+                uint32_t sv[LB];
 #pragma unroll
-                for (int q = 0; q < kMaxLookback; ++q)
-                    sv[q] = (q < lookback && j - q >= 0) ? atomic_load_relaxed(status + (j - q) * R + d) : 0u;
+                for (int q = 0; q < LB; ++q)
+                    sv[q] = j - q >= 0 ? atomic_load_relaxed(status + (j - q) * R + d) : kFlagPrefix;
                 bool done = false, stall = false;
 #pragma unroll
-                for (int q = 0; q < kMaxLookback; ++q) {
-                    if (done || stall || q >= lookback) continue;
-                    if (j < 0) {
-                        done = true;
-                        continue;
-                    }
+                for (int q = 0; q < LB; ++q) {
+                    if (done || stall) continue;
                     const uint32_t flag = sv[q] & ~kValueMask;
                     if (flag == 0u) {
                         stall = true;
@@ -299,7 +313,7 @@ __global__ __launch_bounds__(kSortBlock) __attribute__((amdgpu_waves_per_eu(IT <
     }
 }
 
-template <class K, int RB, int IT>
+template <class K, int RB, int IT, int LB>
 int sort_impl(Context* c, void* keys[2], int32_t* vals[2], int64_t n, int end_bit, bool vals_identity, int* result) {
     constexpr int R = 1 << RB;
     const int passes = sort_passes(end_bit, RB);
@@ -323,7 +337,10 @@ int sort_impl(Context* c, void* keys[2], int32_t* vals[2], int64_t n, int end_bi
         const unsigned zb = (unsigned)std::min<int64_t>((words + 255) / 256, 4096);
         hipLaunchKernelGGL(k_zero_u32, dim3(zb), dim3(256), 0, c->stream, status, words);
         HX_TRY(check_launch("sort_zero"));
-        const unsigned hb = (unsigned)std::min<int64_t>((n + kSortBlock * kSortItems - 1) / (kSortBlock * kSortItems), 2048);
+        // Two workgroups per CU: each adds its P x R counts to the global
+        // histogram once, so the grid size bounds those contended atomics.
+        const unsigned hb = (unsigned)std::max<int64_t>(
+            1, std::min<int64_t>((n + kSortBlock * kSortItems - 1) / (kSortBlock * kSortItems), 2 * c->cu_count));
         hipLaunchKernelGGL((k_sort_hist<K, RB>), dim3(hb), dim3(kSortBlock), 0, c->stream,
                            static_cast<const K*>(keys[0]), n, passes, end_bit, hist);
         HX_TRY(check_launch("sort_hist"));
@@ -336,15 +353,13 @@ int sort_impl(Context* c, void* keys[2], int32_t* vals[2], int64_t n, int end_bi
         const int nb = std::min(RB, end_bit - shift);
         LaunchScope ls(c, "sort_onesweep");
         if (p == 0 && vals_identity)
-            hipLaunchKernelGGL((k_onesweep<K, true, RB, IT>), dim3((unsigned)ntiles), dim3(kSortBlock), 0, c->stream,
+            hipLaunchKernelGGL((k_onesweep<K, true, RB, IT, LB>), dim3((unsigned)ntiles), dim3(kSortBlock), 0, c->stream,
                                static_cast<const K*>(keys[in]), static_cast<K*>(keys[out]), (const int32_t*)nullptr,
-                               vals[out], n, shift, nb, hist + p * R, status + (int64_t)p * ntiles * R, counters + p,
-                               c->tune.sort_lookback);
+                               vals[out], n, shift, nb, hist + p * R, status + (int64_t)p * ntiles * R, counters + p);
         else
-            hipLaunchKernelGGL((k_onesweep<K, false, RB, IT>), dim3((unsigned)ntiles), dim3(kSortBlock), 0, c->stream,
+            hipLaunchKernelGGL((k_onesweep<K, false, RB, IT, LB>), dim3((unsigned)ntiles), dim3(kSortBlock), 0, c->stream,
                                static_cast<const K*>(keys[in]), static_cast<K*>(keys[out]), vals[in], vals[out], n,
-                               shift, nb, hist + p * R, status + (int64_t)p * ntiles * R, counters + p,
-                               c->tune.sort_lookback);
+                               shift, nb, hist + p * R, status + (int64_t)p * ntiles * R, counters + p);
         HX_TRY(check_launch("sort_onesweep"));
     }
     *result = passes & 1;
@@ -361,20 +376,22 @@ int launch_sort_pairs(Context* c, void* keys[2], int32_t* vals[2], int64_t n, in
         set_error("radix sort: n = %lld exceeds 2^30 - 1", (long long)n);
         return HX_ERR_INVALID_ARG;
     }
-    const int bits = c->tune.sort_digit_bits, items = c->tune.sort_items;
-#define HX_SORT_CASE(RB, IT)                                                                         \
-    if (bits == RB && items == IT) {                                                                \
-        if (keys64) return sort_impl<uint64_t, RB, IT>(c, keys, vals, n, end_bit, vals_identity, result_index); \
-        return sort_impl<uint32_t, RB, IT>(c, keys, vals, n, end_bit, vals_identity, result_index);           \
+    const int bits = c->tune.sort_digit_bits, items = c->tune.sort_items, lb = c->tune.sort_lookback;
+#define HX_SORT_CASE(RB, IT, LB)                                                                              \
+    if (bits == RB && items == IT && lb == LB) {                                                             \
+        if (keys64) return sort_impl<uint64_t, RB, IT, LB>(c, keys, vals, n, end_bit, vals_identity, result_index); \
+        return sort_impl<uint32_t, RB, IT, LB>(c, keys, vals, n, end_bit, vals_identity, result_index);           \
     }
-    HX_SORT_CASE(8, 16)
-    HX_SORT_CASE(8, 24)
-    HX_SORT_CASE(8, 32)
-    HX_SORT_CASE(10, 16)
-    HX_SORT_CASE(10, 24)
-    HX_SORT_CASE(10, 32)
+    HX_SORT_CASE(8, 16, 4)
+    HX_SORT_CASE(8, 16, 8)
+    HX_SORT_CASE(8, 32, 4)
+    HX_SORT_CASE(8, 32, 8)
+    HX_SORT_CASE(10, 16, 4)
+    HX_SORT_CASE(10, 16, 8)
+    HX_SORT_CASE(10, 32, 4)
+    HX_SORT_CASE(10, 32, 8)
 #undef HX_SORT_CASE
-    set_error("radix sort: no kernel for %d-bit digits with %d keys per thread", bits, items);
+    set_error("radix sort: no kernel for %d-bit digits, %d keys per thread, look-back %d", bits, items, lb);
     return HX_ERR_INVALID_ARG;
 }
 

@@ -299,6 +299,24 @@ def batch_mesh(count, rng, sphere):
     return soup_triangles(rng.random((count, 3), dtype=np.float32), rng)
 
 
+def sliver_soup(count, rng, max_aspect=200.0):
+    """Long thin triangles in the unit box (config 5): each spans a random
+    direction with length U[0.05, 0.4] and width length / U[1, max_aspect]."""
+    c = rng.random((count, 3)).astype(np.float64) * 0.6 + 0.2
+    d = rng.normal(size=(count, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    w = rng.normal(size=(count, 3))
+    w -= (w * d).sum(1, keepdims=True) * d
+    w /= np.maximum(np.linalg.norm(w, axis=1, keepdims=True), 1e-12)
+    length = rng.uniform(0.05, 0.4, (count, 1))
+    width = length / rng.uniform(1.0, max_aspect, (count, 1))
+    v0 = c - 0.5 * length * d
+    v1 = c + 0.5 * length * d
+    v2 = c + width * w
+    v = np.stack([v0, v1, v2], 1).reshape(-1, 3).clip(0.0, 1.0).astype(np.float32)
+    return v, np.arange(3 * count, dtype=np.int32).reshape(-1, 3)
+
+
 def batch_meshes(n_meshes, seed=3, lo=64, hi=262144):
     """Config 3 geometry: counts round(exp(U[ln lo, ln hi])), half icospheres."""
     rng = np.random.default_rng(seed)

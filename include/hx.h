@@ -146,8 +146,8 @@ int hx_context_profile_reset(hx_context* ctx);
  *   "trace_fetch"          refill a wave when fewer of its 64 lanes are busy
  *   "trace_blocks_per_cu"  persistent trace workgroups per CU (0 = occupancy)
  *   "sort_digit_bits"      radix digit width of every sort (8 or 10)
- *   "sort_items"           keys per thread in a sort pass (16, 24 or 32)
- *   "sort_lookback"        tiles read per sort look-back round trip (1-16)  */
+ *   "sort_items"           keys per thread in a sort pass (16 or 32)
+ *   "sort_lookback"        tiles read per sort look-back round trip (4, 8) */
 int hx_context_set_tuning(hx_context* ctx, const char* name, int32_t value);
 int hx_context_get_tuning(hx_context* ctx, const char* name, int32_t* value);
 
@@ -228,7 +228,9 @@ int hx_trace_overflow_count(hx_context* ctx, int64_t* out);
  * HipBVH at a time.  Triangles of mesh m are [off[m], off[m+1]) of
  * d_triangles (vertex indices into the shared d_vertices); each mesh gets the
  * BVH a separate LBVH build of it would give (same Bin tree, Woop rows and
- * mesh-local triangle ids).  Every mesh needs >= 2 triangles. */
+ * mesh-local triangle ids).  Every mesh needs >= 2 triangles.  With
+ * params->presplitting each mesh is presplit alone (its own box and budget
+ * beta * its triangle count), so a mesh has more references than triangles. */
 typedef struct hx_batch hx_batch;
 typedef struct hx_batch_mesh_view {
     const int32_t* nodes;        /* device: this mesh's Bin nodes, 16 int32 each  */
@@ -237,7 +239,8 @@ typedef struct hx_batch_mesh_view {
     int64_t n_interior;
     int64_t n_leaves;
     int64_t n_references;
-    int64_t tri_offset;          /* off[m]                                       */
+    int64_t tri_offset;          /* first reference slot of the mesh (off[m]
+                                    unless presplit)                             */
 } hx_batch_mesh_view;
 int hx_build_batch(hx_context* ctx, const hx_build_params* params, const float* d_vertices, int64_t n_vertices,
                    const int32_t* d_triangles, const int64_t* h_mesh_offsets, int32_t n_meshes, hx_batch** out);

@@ -152,3 +152,83 @@ def test_batch_diff_step_graph_replays_match_eager(ctx):
         torch.cuda.synchronize()
         assert torch.equal(res, res_e), f"replay {k}: results"
         torch.testing.assert_close(grad, grad_e, rtol=1e-5, atol=1e-6)
+
+
+def _check_meshes_presplit(oracle, hb, meshes, bits, beta):
+    for m, (v, t) in enumerate(meshes):
+        got = hb.mesh_host(m)
+        ref = oracle.build_lbvh(v, t, bits=bits, presplit_beta=beta)
+        assert got["tri_idx"].shape[0] == ref["tri_idx"].shape[0], f"mesh {m} references"
+        assert (got["n_interior"], got["n_leaves"]) == (ref["n_interior"], ref["n_leaves"]), f"mesh {m} counts"
+        assert oracle.validate_bin_refs(got["nodes"], got["n_interior"], got["n_leaves"], got["tri_idx"], v, t) == 0
+        cg = oracle.bin_canonical(got["nodes"], got["n_interior"], got["n_leaves"], got["tri_idx"])
+        co = oracle.bin_canonical(ref["nodes"], ref["n_interior"], ref["n_leaves"], ref["tri_idx"])
+        assert np.array_equal(cg, co), f"mesh {m} canonical tree"
+        assert np.array_equal(got["woop"].view(np.uint32), oracle.woopify(v, t, got["tri_idx"]).view(np.uint32))
+
+
+def _config5_meshes(n, seed, lo=64, hi=5000):
+    """Uneven meshes, 30% of them long thin slivers (aspect up to 1:200)."""
+    rng = np.random.default_rng(seed)
+    counts = np.rint(np.exp(rng.uniform(np.log(lo), np.log(hi), n))).astype(int)
+    out = []
+    for c in counts:
+        if rng.random() < 0.3:
+            out.append(scenes.sliver_soup(int(c), rng))
+        else:
+            out.append(scenes.batch_mesh(int(c), rng, bool(rng.random() < 0.5)))
+    return out
+
+
+@pytest.mark.parametrize("beta", [0.3, 1.0])
+def test_batch_presplit_matches_each_mesh_alone(oracle, ctx, beta):
+    """Config 5 shape: every mesh presplit alone (own box, own budget beta * n_m)."""
+    meshes = _config5_meshes(30, seed=21)
+    hb = hx.HipBatch(ctx, hx.MeshBatch.from_meshes(meshes), morton_bits=60, presplitting=1, presplitter_beta=beta)
+    _check_meshes_presplit(oracle, hb, meshes, 60, beta)
+    hb.rebuild()
+    _check_meshes_presplit(oracle, hb, meshes, 60, beta)
+
+
+def test_batch_presplit_trace_refit_backward(oracle, ctx):
+    meshes = _config5_meshes(16, seed=22, lo=32, hi=3000)
+    mb = hx.MeshBatch.from_meshes(meshes)
+    hb = hx.HipBatch(ctx, mb, presplitting=1, presplitter_beta=0.3)
+    rays, owner = scenes.batch_rays(meshes, per_1k=2000, seed=5)
+    d_rays = torch.as_tensor(rays, device="cuda")
+    d_owner = torch.as_tensor(owner, device="cuda")
+    res = hb.trace(d_rays, d_owner).cpu().numpy()
+    for m, (v, t) in enumerate(meshes):
+        sel = owner == m
+        g = hb.mesh_host(m)
+        exp, ov = oracle.trace(g["nodes"], g["woop"], g["tri_idx"], rays[sel])
+        brute = oracle.trace_brute(v, t, rays[sel])
+        assert ov == 0
+        assert np.array_equal(res[sel, 0], exp[:, 0]) and np.array_equal(res[sel, 1], exp[:, 1]), f"mesh {m}"
+        assert np.array_equal(res[sel, 0], brute[:, 0]) and np.array_equal(res[sel, 1], brute[:, 1]), f"mesh {m}"
+    # Refit: leaf boxes grow back to whole triangles, as HipBVH::update does.
+    before = [hb.mesh_host(m) for m in range(len(meshes))]
+    rng = np.random.default_rng(9)
+    v_all = mb.scene.vertices.cpu().numpy()
+    v2 = (v_all + rng.normal(0.0, 0.001, v_all.shape)).astype(np.float32)
+    hb.refit(v2)
+    base = 0
+    for m, (v, t) in enumerate(meshes):
+        vm = v2[base:base + len(v)]
+        base += len(v)
+        g = hb.mesh_host(m)
+        exp = oracle.refit_final(before[m]["nodes"], g["n_interior"], g["n_leaves"], g["tri_idx"], vm, t)
+        assert np.array_equal(g["nodes"], exp), f"mesh {m} refit"
+    # Adjoint: batched scatter equals the single-mesh adjoint mesh by mesh.
+    res_t = hb.trace(d_rays, d_owner)
+    dl_dt = torch.rand(len(rays), device="cuda", dtype=torch.float32)
+    grad = hb.backward(d_rays, d_owner, res_t, dl_dt=dl_dt).cpu().numpy()
+    base = 0
+    for m, (v, t) in enumerate(meshes):
+        sel = torch.as_tensor(owner == m, device="cuda")
+        vm = torch.as_tensor(v2[base:base + len(v)], device="cuda")
+        tm = torch.as_tensor(t, device="cuda")
+        gm = single_backward(ctx, vm, tm, d_rays[sel].contiguous(), res_t[sel].contiguous(),
+                             dl_dt=dl_dt[sel].contiguous()).cpu().numpy()
+        np.testing.assert_allclose(grad[base:base + len(v)], gm, rtol=1e-5, atol=1e-6)
+        base += len(v)
