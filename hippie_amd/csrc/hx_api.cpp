@@ -141,6 +141,7 @@ struct hx_bvh {
     // PLOC: split position per internal node (kept for refits) and the
     // clustering workspace (cluster ids, boxes and neighbours, ping-pong).
     DevBuf split, ploc_work, atr_work, hlbvh_work, ins_work;
+    DevBuf node_at;  // general trees: internal node at each split position
     int ploc_rounds = 0;
     int64_t atr_accepted = 0;
     int hlbvh_clusters = 0;
@@ -165,6 +166,10 @@ struct hx_bvh {
     // offsets: mesh_off / mesh_of / h_roff are over reference slots, tri_off /
     // tri_mesh_of over triangles.
     DevBuf tri_off, tri_mesh_of;
+    // Packed vertices per triangle (3 float4), written by the Morton pass of
+    // a build and gathered by its climbs; stale after a refit.
+    DevBuf tri_pack;
+    bool pack_valid = false;
     std::vector<int32_t> h_roff;
     bool built = false;
     bool counts_known = false;
@@ -233,7 +238,9 @@ int validate_params(const hx_build_params* p) {
 }
 
 // PLOC workspace: cl[2] (int), ca[2] (float4), cb[2] (float2), nb (int).
-size_t ploc_work_bytes(int64_t n) { return (size_t)n * (2 * 4 + 2 * 16 + 2 * 8 + 4) + 1024; }
+// PLOC workspace: cl[2] (int), ca[2] (float4), cb[2] (float2), nb (int), and
+// in batched builds cm[2] (int).
+size_t ploc_work_bytes(int64_t n) { return (size_t)n * (2 * 4 + 2 * 16 + 2 * 8 + 4 + 2 * 4) + 2048; }
 
 // ATR workspace: area, cost (2n-1 floats), refs, height, list (n ints),
 // hist, cursor (levels + 1 ints), accepted.
@@ -293,18 +300,22 @@ int reserve_bvh(hx_bvh* b, int64_t n) {
     }
     if (b->params.method == HX_METHOD_PLOC) {
         HX_TRY(b->split.reserve(4 * (size_t)n));
+        HX_TRY(b->node_at.reserve(4 * (size_t)n));
         HX_TRY(b->ploc_work.reserve(ploc_work_bytes(n)));
     }
     if (b->params.method == HX_METHOD_ATRBVH) {
         HX_TRY(b->split.reserve(4 * (size_t)n));
+        HX_TRY(b->node_at.reserve(4 * (size_t)n));
         HX_TRY(b->atr_work.reserve(atr_work_bytes(n)));
     }
     if (b->params.method == HX_METHOD_HLBVH) {
         HX_TRY(b->split.reserve(4 * (size_t)n));
+        HX_TRY(b->node_at.reserve(4 * (size_t)n));
         HX_TRY(b->hlbvh_work.reserve(hlbvh_work_bytes(n)));
     }
     if (b->params.method == HX_METHOD_INSERTION) {
         HX_TRY(b->split.reserve(4 * (size_t)n));
+        HX_TRY(b->node_at.reserve(4 * (size_t)n));
         HX_TRY(b->ins_work.reserve(ins_work_bytes(n)));
     }
     return HX_OK;
@@ -339,6 +350,7 @@ hx::ClimbArgs climb_args(hx_bvh* b, bool refit_only) {
         ca.ref_max = b->ref_max.as<float4>();
         ca.ref_tri = b->ref_tri.as<int32_t>();
     }
+    ca.tri_pack = b->pack_valid ? b->tri_pack.as<float4>() : nullptr;
     ca.refit_only = refit_only ? 1 : 0;
     ca.adaptive = (p.adaptive_leaf_size && !refit_only) ? 1 : 0;
     ca.max_leaf_size = p.max_leaf_size;
@@ -368,6 +380,10 @@ hx::EmitArgs emit_args(hx_bvh* b) {
     ea.n_meshes = b->n_meshes;
     ea.mesh_tab = b->mesh_tab.as<int4>();
     ea.root_slot = b->root_slot.as<int32_t>();
+    if (b->general) {
+        ea.split = b->split.as<int32_t>();
+        ea.node_at = b->node_at.as<int32_t>();
+    }
     return ea;
 }
 
@@ -464,8 +480,14 @@ int run_front(hx_bvh* b) {
     const int64_t n = b->n_refs;
     const float4* rmin = b->refs_active ? b->ref_min.as<float4>() : nullptr;
     const float4* rmax = b->refs_active ? b->ref_max.as<float4>() : nullptr;
+    float4* pack = nullptr;
+    if (!b->refs_active) {
+        HX_TRY(b->tri_pack.reserve(48 * (size_t)n));
+        pack = b->tri_pack.as<float4>();
+    }
     HX_TRY(hx::launch_morton(c, b->verts, b->tris, n, box, p.morton_bits, b->keys[0].ptr, b->keys64, nullptr, rmin,
-                             rmax));
+                             rmax, pack));
+    b->pack_valid = pack != nullptr;
     // The sorted triangle ids must end in tri_idx: pass p reads vals[p&1].
     const int passes = hx::sort_pass_count(c, p.morton_bits);
     int32_t* vals[2];
@@ -505,6 +527,11 @@ int run_ploc(hx_bvh* b) {
     for (int k = 0; k < 2; ++k) a.cb[k] = reinterpret_cast<float2*>(carve(8 * (size_t)n));
     for (int k = 0; k < 2; ++k) a.cl[k] = reinterpret_cast<int32_t*>(carve(4 * (size_t)n));
     a.nb = reinterpret_cast<int32_t*>(carve(4 * (size_t)n));
+    if (b->batched) {
+        for (int k = 0; k < 2; ++k) a.cm[k] = reinterpret_cast<int32_t*>(carve(4 * (size_t)n));
+        a.mesh_of = b->mesh_of.as<int32_t>();
+        a.n_meshes = b->n_meshes;
+    }
     a.left = b->left.as<int32_t>();
     a.right = b->right.as<int32_t>();
     a.parent = b->parent.as<int32_t>();
@@ -691,8 +718,14 @@ int run_batch(hx_bvh* b) {
     const float4* rmin = b->refs_active ? b->ref_min.as<float4>() : nullptr;
     const float4* rmax = b->refs_active ? b->ref_max.as<float4>() : nullptr;
     uint64_t* korig = b->keys_orig.as<uint64_t>();
+    float4* pack = nullptr;
+    if (!b->refs_active) {
+        HX_TRY(b->tri_pack.reserve(48 * (size_t)n));
+        pack = b->tri_pack.as<float4>();
+    }
     HX_TRY(hx::launch_morton(c, b->verts, b->tris, n, b->mesh_box.as<float>(), p.morton_bits, korig, true, mesh_of,
-                             rmin, rmax));
+                             rmin, rmax, pack));
+    b->pack_valid = pack != nullptr;
     HX_HIP(hipMemcpyAsync(b->keys[0].ptr, korig, 8 * (size_t)n, hipMemcpyDeviceToDevice, c->stream));
     void* keys[2] = {b->keys[0].ptr, b->keys[1].ptr};
     int32_t* vals[2] = {b->tri_idx.as<int32_t>(), b->vals_tmp.as<int32_t>()};
@@ -710,14 +743,21 @@ int run_batch(hx_bvh* b) {
     }
     if (sorted != b->tri_idx.as<int32_t>())
         HX_HIP(hipMemcpyAsync(b->tri_idx.ptr, sorted, 4 * (size_t)n, hipMemcpyDeviceToDevice, c->stream));
-    HX_TRY(hx::launch_gather_keys(c, b->tri_idx.as<int32_t>(), korig, n, static_cast<uint64_t*>(keys[0])));
-    b->keys_index = 0;
-    HX_TRY(hx::launch_construct(c, keys[0], true, n, b->left.as<int32_t>(), b->right.as<int32_t>(),
-                                b->parent.as<int32_t>(), b->first.as<int32_t>(), b->last.as<int32_t>(),
-                                b->counters.as<int32_t>(), b->cover.as<int32_t>(), b->counts.as<int32_t>() + 2,
-                                mesh_of, off));
-    HX_TRY(hx::launch_climb(c, climb_args(b, false)));
-    HX_TRY(hx::launch_collapse_emit(c, emit_args(b)));
+    b->general = false;
+    if (p.method == HX_METHOD_PLOC) {
+        // Clusters merge inside their own mesh, so each mesh gets the tree a
+        // PLOC build of it alone gives.
+        HX_TRY(run_ploc(b));
+    } else {
+        HX_TRY(hx::launch_gather_keys(c, b->tri_idx.as<int32_t>(), korig, n, static_cast<uint64_t*>(keys[0])));
+        b->keys_index = 0;
+        HX_TRY(hx::launch_construct(c, keys[0], true, n, b->left.as<int32_t>(), b->right.as<int32_t>(),
+                                    b->parent.as<int32_t>(), b->first.as<int32_t>(), b->last.as<int32_t>(),
+                                    b->counters.as<int32_t>(), b->cover.as<int32_t>(), b->counts.as<int32_t>() + 2,
+                                    mesh_of, off));
+        HX_TRY(hx::launch_climb(c, climb_args(b, false)));
+        HX_TRY(hx::launch_collapse_emit(c, emit_args(b)));
+    }
     if (b->refs_active) {
         HX_TRY(hx::launch_refs_to_tris(c, b->tri_idx.as<int32_t>(), b->ref_tri.as<int32_t>(), n));
         b->refs_active = false;
@@ -845,6 +885,7 @@ int run_build(hx_bvh* b) {
 int run_refit(hx_bvh* b) {
     Context* c = &b->ctx->c;
     const int64_t n = b->n_refs;
+    b->pack_valid = false;  // the packed vertices are the build's
     HX_HIP(hipSetDevice(c->device));
     const int32_t* sorted_tri = b->tri_idx.as<int32_t>();
     if (n == 1)
@@ -1146,7 +1187,8 @@ int hx_build_batch(hx_context* ctx, const hx_build_params* params, const float* 
     HX_CHECK_ARG(ctx && out && h_mesh_offsets, "NULL argument");
     *out = nullptr;
     HX_TRY(validate_params(params));
-    HX_CHECK_ARG(params->method == HX_METHOD_LBVH, "batched builds use the LBVH method (got %d)", params->method);
+    HX_CHECK_ARG(params->method == HX_METHOD_LBVH || params->method == HX_METHOD_PLOC,
+                 "batched builds use the LBVH or PLOC method (got %d)", params->method);
 
     HX_CHECK_ARG(n_meshes >= 1, "n_meshes must be positive (got %d)", n_meshes);
     HX_CHECK_ARG(h_mesh_offsets[0] == 0, "mesh offsets must start at 0");

@@ -114,7 +114,8 @@ __global__ __launch_bounds__(kBlock) void k_morton(const float* __restrict__ ver
                                                    int64_t n, const float* __restrict__ boxes, int shift,
                                                    void* __restrict__ keys_out, const int32_t* __restrict__ mesh_of,
                                                    const float4* __restrict__ ref_min,
-                                                   const float4* __restrict__ ref_max) {
+                                                   const float4* __restrict__ ref_max,
+                                                   float4* __restrict__ pack_out) {
     const int64_t t = blockIdx.x * (int64_t)kBlock + threadIdx.x;
     if (t >= n) return;
     const float* sb = mesh_of ? boxes + 6 * (int64_t)mesh_of[t] : boxes;
@@ -129,6 +130,11 @@ __global__ __launch_bounds__(kBlock) void k_morton(const float* __restrict__ ver
         float3 a, b, c;
         tri_vertices(verts, tris, t, a, b, c);
         tb = tri_box(a, b, c);
+        if (pack_out) {
+            pack_out[3 * t + 0] = make_float4(a.x, a.y, a.z, b.x);
+            pack_out[3 * t + 1] = make_float4(b.y, b.z, c.x, c.y);
+            pack_out[3 * t + 2] = make_float4(c.z, 0.0f, 0.0f, 0.0f);
+        }
     }
     // computeMortonCodes60 regrows the stored box from its two corners.
     Box3 bb = box_empty();
@@ -403,7 +409,15 @@ __global__ __launch_bounds__(kClimbThreads) void k_climb_local(ClimbArgs a, int3
         const int id = a.sorted_tri[p];
         const int tri = a.ref_tri ? a.ref_tri[id] : id;
         float3 v0, v1, v2;
-        tri_vertices(a.verts, a.tris, tri, v0, v1, v2);
+        if (a.tri_pack) {
+            const float4 q0 = a.tri_pack[3 * (int64_t)tri], q1 = a.tri_pack[3 * (int64_t)tri + 1];
+            const float4 q2 = a.tri_pack[3 * (int64_t)tri + 2];
+            v0 = make_float3(q0.x, q0.y, q0.z);
+            v1 = make_float3(q0.w, q1.x, q1.y);
+            v2 = make_float3(q1.z, q1.w, q2.x);
+        } else {
+            tri_vertices(a.verts, a.tris, tri, v0, v1, v2);
+        }
         Box3 tb;
         if (a.ref_min) {
             const float4 rmn = a.ref_min[id], rmx = a.ref_max[id];
@@ -677,7 +691,12 @@ struct FlagCtx {
     const int32_t* state;
     const int32_t* mesh_of;  // batched builds only
     const int32_t* off;
+    const int32_t* node_at;  // general trees: node at each split position (else NULL)
 };
+
+// The internal node whose slot position is p (its split position in a
+// general tree, its id in a Karras tree).
+HX_DEV int node_of_pos(const FlagCtx& f, int p) { return f.node_at ? f.node_at[p] : p; }
 
 HX_DEV int leaf_start(const FlagCtx& f, int p) {
     return (p < f.n) && (p == 0 || f.cover[p - 1] < p);
@@ -703,7 +722,7 @@ __global__ __launch_bounds__(kBlock) void k_tile_flag_counts(FlagCtx f, int2* __
     for (int k = 0; k < kScanItems; ++k) {
         const int64_t p = base + k * kBlock + threadIdx.x;
         if (p < f.n) {
-            ci += interior_flag(f, (int)p);
+            ci += p < f.n - 1 ? interior_flag(f, node_of_pos(f, (int)p)) : 0;
             cl += leaf_start(f, (int)p);
         }
     }
@@ -745,11 +764,12 @@ __global__ __launch_bounds__(kBlock) void k_tile_slots(FlagCtx f, const int2* __
     for (int k = 0; k < kScanItems; ++k) {
         const int64_t p = base + k * kBlock + threadIdx.x;
         int2 v = make_int2(0, 0);
-        if (p < f.n) v = make_int2(interior_flag(f, (int)p), leaf_start(f, (int)p));
+        const int node = p < f.n - 1 ? node_of_pos(f, (int)p) : 0;
+        if (p < f.n) v = make_int2(p < f.n - 1 ? interior_flag(f, node) : 0, leaf_start(f, (int)p));
         int2 tot;
         const int2 inc = block_inclusive_scan<kWavesPerBlock>(v, Sum2Op(), wt, tot);
         if (p < f.n) {
-            if (p < f.n - 1) int_slot[p] = carry.x + inc.x - v.x;
+            if (p < f.n - 1) int_slot[node] = carry.x + inc.x - v.x;
             leaf_num[p] = carry.y + inc.y - v.y;
         }
         carry = make_int2(carry.x + tot.x, carry.y + tot.y);
@@ -834,11 +854,13 @@ __global__ __launch_bounds__(kBlock) void k_mesh_roots(EmitArgs a, FlagCtx f) {
 __global__ __launch_bounds__(kBlock) void k_mesh_tab(EmitArgs a, FlagCtx f) {
     const int m = blockIdx.x * kBlock + threadIdx.x;
     if (m >= a.n_meshes) return;
+    // Interior slots before position p (position n-1 holds no internal node).
+    auto slots_before = [&](int p) { return p < f.n - 1 ? a.int_slot[node_of_pos(f, p)] : a.counts[0]; };
     int ib = 0, lb = 0, ie = a.counts[0];
     if (f.mesh_of) {
-        ib = a.int_slot[f.off[m]];
+        ib = slots_before(f.off[m]);
         lb = a.leaf_num[f.off[m]];
-        if (m + 1 < a.n_meshes) ie = a.int_slot[f.off[m + 1]];
+        if (m + 1 < a.n_meshes) ie = slots_before(f.off[m + 1]);
     }
     a.mesh_tab[m] = make_int4(ib + lb, ib, lb, ie - ib);
 }
@@ -916,6 +938,12 @@ __global__ __launch_bounds__(kBlock) void k_cost_final(const double* __restrict_
     if (threadIdx.x == 0) out[0] = s[0];
 }
 
+__global__ __launch_bounds__(kBlock) void k_node_at(const int32_t* __restrict__ split, int m,
+                                                   int32_t* __restrict__ node_at) {
+    const int v = blockIdx.x * kBlock + threadIdx.x;
+    if (v < m) node_at[split[v]] = v;
+}
+
 __global__ void k_refit_reset(int32_t* __restrict__ counters, int64_t n_internal, int32_t* __restrict__ frontier_count) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i < n_internal) counters[i] = 0;
@@ -955,15 +983,16 @@ int launch_scene_box(Context* c, const float* verts, const int32_t* tris, int64_
 }
 
 int launch_morton(Context* c, const float* verts, const int32_t* tris, int64_t n, const float* d_box6, int bits,
-                  void* d_keys, bool keys64, const int32_t* mesh_of, const float4* ref_min, const float4* ref_max) {
+                  void* d_keys, bool keys64, const int32_t* mesh_of, const float4* ref_min, const float4* ref_max,
+                  float4* pack_out) {
     LaunchScope ls(c, "morton");
     const int shift = 60 - bits;
     if (keys64)
         hipLaunchKernelGGL(k_morton<true>, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, verts, tris, n, d_box6, shift,
-                           d_keys, mesh_of, ref_min, ref_max);
+                           d_keys, mesh_of, ref_min, ref_max, pack_out);
     else
         hipLaunchKernelGGL(k_morton<false>, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, verts, tris, n, d_box6,
-                           shift, d_keys, mesh_of, ref_min, ref_max);
+                           shift, d_keys, mesh_of, ref_min, ref_max, pack_out);
     return check_launch("morton");
 }
 
@@ -1007,7 +1036,13 @@ int launch_collapse_emit(Context* c, const EmitArgs& a) {
     const int64_t n = a.n;
     const int64_t ntiles = (n + kScanTile - 1) / kScanTile;
     HX_TRY(c->scan_partials.reserve(sizeof(int2) * (size_t)ntiles + 64));
-    FlagCtx f{(int)n, a.cover, a.first, a.last, a.state, a.mesh_of, a.mesh_off};
+    const bool general = a.split != nullptr;
+    if (general) {
+        hipLaunchKernelGGL(k_node_at, dim3(grid_for(n - 1)), dim3(kBlock), 0, c->stream, a.split, (int)(n - 1),
+                           a.node_at);
+        HX_TRY(check_launch("node_at"));
+    }
+    FlagCtx f{(int)n, a.cover, a.first, a.last, a.state, a.mesh_of, a.mesh_off, general ? a.node_at : nullptr};
     {
         LaunchScope ls(c, "collapse_cover");
         int32_t* pm = c->scan_partials.as<int32_t>();
@@ -1048,7 +1083,7 @@ int launch_refit_reset(Context* c, int32_t* counters, int64_t n_internal, int32_
 }
 
 int launch_emit_only(Context* c, const EmitArgs& a) {
-    FlagCtx f{(int)a.n, a.cover, a.first, a.last, a.state, a.mesh_of, a.mesh_off};
+    FlagCtx f{(int)a.n, a.cover, a.first, a.last, a.state, a.mesh_of, a.mesh_off, a.split ? a.node_at : nullptr};
     LaunchScope ls(c, "collapse_emit");
     hipLaunchKernelGGL(k_emit, dim3(grid_for(a.n)), dim3(kBlock), 0, c->stream, a, f);
     return check_launch("emit");

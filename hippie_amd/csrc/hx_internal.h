@@ -111,9 +111,12 @@ int launch_scene_box(Context* c, const float* verts, const int32_t* tris, int64_
 // position for construct) and the M+1 triangle offsets; d_box6 then holds
 // one box per mesh.
 // Keys of n triangles, or (ref_min != NULL) of n presplit reference boxes.
+// With pack_out (triangles only) each triangle's vertices are also written as
+// three float4 in input order (v0.xyz v1.x | v1.yz v2.xy | v2.z), the record
+// the climb gathers instead of the index and three vertex reads.
 int launch_morton(Context* c, const float* verts, const int32_t* tris, int64_t n, const float* d_box6,
                   int bits, void* d_keys, bool keys64, const int32_t* mesh_of = nullptr,
-                  const float4* ref_min = nullptr, const float4* ref_max = nullptr);
+                  const float4* ref_min = nullptr, const float4* ref_max = nullptr, float4* pack_out = nullptr);
 int launch_construct(Context* c, const void* d_keys, bool keys64, int64_t n, int32_t* left, int32_t* right,
                      int32_t* parent, int32_t* first, int32_t* last, int32_t* counters, int32_t* end_at,
                      int32_t* zero_word, const int32_t* mesh_of = nullptr, const int32_t* off = nullptr);
@@ -150,6 +153,7 @@ struct ClimbArgs {
     const float4* ref_min;
     const float4* ref_max;
     const int32_t* ref_tri;
+    const float4* tri_pack;   // packed vertices of each triangle (3 float4), or NULL
     int refit_only;           // boxes and Woop only: keep states, costs and cover
     int adaptive;
     int max_leaf_size;
@@ -176,6 +180,12 @@ struct EmitArgs {
     int n_meshes;             // 1 unless batched
     int4* mesh_tab;           // out: per mesh {node base, first interior slot, first leaf number, interiors}
     int32_t* root_slot;       // out: per mesh interior slot of the root
+    // General trees (PLOC, ATRBVH, HLBVH, insertion): interior slots follow
+    // the split positions, so every mesh's interiors stay contiguous;
+    // node_at (n-1) receives the node at each split position.  NULL split:
+    // a Karras tree, whose node ids already lie in their ranges.
+    const int32_t* split;
+    int32_t* node_at;
 };
 int launch_collapse_emit(Context* c, const EmitArgs& a);
 // Refit: re-zero the climb counters and frontier count, then (after the
@@ -216,6 +226,9 @@ struct PlocArgs {
     const float4* ref_min;       // presplit builds: leaf boxes of the reference ids (else NULL)
     const float4* ref_max;
     int32_t* cl[2];              // cluster node ids, ping-pong (n each)
+    int32_t* cm[2];              // batched builds: cluster mesh ids, ping-pong (else NULL)
+    const int32_t* mesh_of;      // batched builds: mesh of each sorted position
+    int n_meshes;
     float4* ca[2];               // cluster boxes: (mn.x, mn.y, mn.z, mx.x)
     float2* cb[2];               //                (mx.y, mx.z)
     int32_t* nb;                 // nearest neighbour per cluster (n)

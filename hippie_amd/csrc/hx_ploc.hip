@@ -69,6 +69,7 @@ __global__ __launch_bounds__(kBlock) void k_ploc_leaves(PlocArgs a) {
         box_grow(b, ld3(a.verts + 3 * (int64_t)tr[2]));
     }
     a.cl[0][p] = n - 1 + p;
+    if (a.cm[0]) a.cm[0][p] = a.mesh_of[p];
     a.ca[0][p] = make_float4(b.mn.x, b.mn.y, b.mn.z, b.mx.x);
     a.cb[0][p] = make_float2(b.mx.y, b.mx.z);
     a.end_at[p] = -1;
@@ -93,11 +94,15 @@ HX_DEV Box3 cluster_box(float4 a, float2 b) {
     return r;
 }
 
-// LDS window: boxes of clusters [lo, lo + W) as six rows of stride `stride`.
+// LDS window: boxes of clusters [lo, lo + W) as six rows of stride `stride`
+// (a seventh row of mesh ids in batched builds).  With cmesh, a cluster only
+// looks at clusters of its own mesh: the meshes' cluster runs are contiguous,
+// so this is the single-mesh window clipped to the mesh.
 template <bool LDS>
 __global__ __launch_bounds__(kBlock) void k_ploc_neighbours(const float4* __restrict__ ca,
                                                             const float2* __restrict__ cb, int m, int r,
-                                                            int32_t* __restrict__ nb) {
+                                                            int32_t* __restrict__ nb,
+                                                            const int32_t* __restrict__ cmesh) {
     extern __shared__ float s_win[];
     const int b0 = blockIdx.x * kBlock;
     const int i = b0 + threadIdx.x;
@@ -114,6 +119,7 @@ __global__ __launch_bounds__(kBlock) void k_ploc_neighbours(const float4* __rest
             s_win[3 * stride + k] = x.w;
             s_win[4 * stride + k] = y.x;
             s_win[5 * stride + k] = y.y;
+            if (cmesh) s_win[6 * stride + k] = __int_as_float(cmesh[lo + k]);
         }
         __syncthreads();
         if (i >= m) return;
@@ -124,9 +130,11 @@ __global__ __launch_bounds__(kBlock) void k_ploc_neighbours(const float4* __rest
         int best = -1;
         float bestD = kMaxFloat;
         const int jlo = max(0, i - r), jhi = min(m - 1, i + r);
+        const int mine = cmesh ? __float_as_int(s_win[6 * stride + me]) : 0;
         for (int j = jlo; j <= jhi; ++j) {
             if (j == i) continue;
             const int q = j - lo;
+            if (cmesh && __float_as_int(s_win[6 * stride + q]) != mine) continue;
             const float d = union_area(s_win[q], s_win[stride + q], s_win[2 * stride + q], s_win[3 * stride + q],
                                        s_win[4 * stride + q], s_win[5 * stride + q], own);
             if (bestD > d) {
@@ -141,8 +149,10 @@ __global__ __launch_bounds__(kBlock) void k_ploc_neighbours(const float4* __rest
         int best = -1;
         float bestD = kMaxFloat;
         const int jlo = max(0, i - r), jhi = min(m - 1, i + r);
+        const int mine = cmesh ? cmesh[i] : 0;
         for (int j = jlo; j <= jhi; ++j) {
             if (j == i) continue;
+            if (cmesh && cmesh[j] != mine) continue;
             const float4 x = ca[j];
             const float2 y = cb[j];
             const float d = union_area(x.x, x.y, x.z, x.w, y.x, y.y, own);
@@ -153,6 +163,14 @@ __global__ __launch_bounds__(kBlock) void k_ploc_neighbours(const float4* __rest
         }
         nb[i] = best;
     }
+}
+
+// Separator rounds of a batched build (every mesh already one cluster): pair
+// clusters (2k, 2k+1) so the joins keep the meshes in order, whatever their
+// boxes; an odd last cluster waits for the next round.
+__global__ __launch_bounds__(kBlock) void k_ploc_pairs(int m, int32_t* __restrict__ nb) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < m) nb[i] = (i ^ 1) < m ? (i ^ 1) : -1;
 }
 
 // (merging, surviving) of cluster i (PLOCBuilderKernels.cu:232-240): -- a mutual pair merges
@@ -239,11 +257,13 @@ __global__ __launch_bounds__(kBlock) void k_ploc_merge(PlocArgs a, int src, int 
                 const int sr = rn >= n - 1 ? 1 : a.last[rn];
                 a.last[node] = sl + sr;
                 cl_out[srank] = node;
+                if (a.cm[0]) a.cm[src ^ 1][srank] = a.cm[src][i];
                 ca_out[srank] = make_float4(mins(lb.mn.x, rb.mn.x), mins(lb.mn.y, rb.mn.y), mins(lb.mn.z, rb.mn.z),
                                             maxs(lb.mx.x, rb.mx.x));
                 cb_out[srank] = make_float2(maxs(lb.mx.y, rb.mx.y), maxs(lb.mx.z, rb.mx.z));
             } else if (f.y) {
                 cl_out[srank] = cl[i];
+                if (a.cm[0]) a.cm[src ^ 1][srank] = a.cm[src][i];
                 ca_out[srank] = ca[i];
                 cb_out[srank] = cb[i];
             }
@@ -319,18 +339,28 @@ int launch_ploc(Context* c, const PlocArgs& a, int* rounds_out) {
     int2* total = partial + max_tiles;
     std::vector<int2> ranges;   // [lo, hi] of the nodes each round created
     int m = n, src = 0, r = a.radius;
+    // Batched builds merge inside each mesh until every mesh is one cluster
+    // (m == n_meshes), then join the mesh roots pairwise in mesh order
+    // (separators, never emitted), so mesh m's leaves keep positions
+    // [off[m], off[m+1]) after the depth-first relabelling.
+    const bool batched = a.cm[0] != nullptr && a.n_meshes > 1;
+    bool per_mesh = batched;
     while (m > 1) {
+        if (per_mesh && m == a.n_meshes) per_mesh = false;
+        const int32_t* cmesh = per_mesh ? a.cm[src] : nullptr;
         if (a.adaptive) r = std::min(r + 1, a.max_radius);
         const int ntiles = (m + kTile - 1) / kTile;
         {
             LaunchScope ls(c, "ploc_neighbours");
-            if (r <= kMaxLdsRadius) {
-                const size_t lds = sizeof(float) * 6 * (size_t)(kBlock + 2 * r);
+            const size_t lds = sizeof(float) * (cmesh ? 7 : 6) * (size_t)(kBlock + 2 * r);
+            if (batched && !per_mesh) {
+                hipLaunchKernelGGL(k_ploc_pairs, dim3(grid_for(m)), dim3(kBlock), 0, c->stream, m, a.nb);
+            } else if (r <= kMaxLdsRadius && lds <= 65536) {
                 hipLaunchKernelGGL(k_ploc_neighbours<true>, dim3(grid_for(m)), dim3(kBlock), lds, c->stream, a.ca[src],
-                                   a.cb[src], m, r, a.nb);
+                                   a.cb[src], m, r, a.nb, cmesh);
             } else {
                 hipLaunchKernelGGL(k_ploc_neighbours<false>, dim3(grid_for(m)), dim3(kBlock), 0, c->stream,
-                                   a.ca[src], a.cb[src], m, r, a.nb);
+                                   a.ca[src], a.cb[src], m, r, a.nb, cmesh);
             }
             HX_TRY(check_launch("ploc_neighbours"));
         }

@@ -60,9 +60,9 @@ __global__ __launch_bounds__(kSortBlock) void k_sort_hist(const K* __restrict__ 
     for (int i = threadIdx.x; i < P * R; i += kSortBlock) (&h[0][0])[i] = 0u;
     __syncthreads();
     // Morton keys in input order share their high digits across a wave, and
-    // 64 lanes adding to one LDS word serialise: each pass first peels up to
-    // four digit values shared by several lanes (one add of the peer count
-    // each), then the remaining lanes add their own.
+    // 64 lanes adding to one LDS word serialise: a pass whose digit is the
+    // same in every live lane adds the lane count once; otherwise every lane
+    // adds its own.
     const int lane = lane_id();
     const int64_t stride = (int64_t)gridDim.x * kSortBlock;
     const int64_t n_round = (n + kWave - 1) / kWave * kWave;
@@ -73,17 +73,16 @@ __global__ __launch_bounds__(kSortBlock) void k_sort_hist(const K* __restrict__ 
             const int shift = RB * p;
             const int nb = min(RB, end_bit - shift);
             const uint32_t d = digit_of(key, shift, (1u << nb) - 1u);
-            uint64_t act = ballot64(valid);
-#pragma unroll
-            for (int it = 0; it < 4; ++it) {
-                if (!act) break;
-                const int src = __builtin_ctzll(act);
-                const uint32_t v = (uint32_t)__builtin_amdgcn_readlane((int)d, src);
-                const uint64_t same = ballot64(d == v) & act;
-                if (lane == src) atomicAdd(&h[p][v], (uint32_t)__popcll(same));
-                act &= ~same;
+            const uint64_t act = ballot64(valid);
+            if (!act) continue;
+            const int src = __builtin_ctzll(act);
+            const uint32_t v = (uint32_t)__builtin_amdgcn_readlane((int)d, src);
+            const uint64_t same = ballot64(d == v) & act;
+            if (same == act) {
+                if (lane == src) atomicAdd(&h[p][v], (uint32_t)__popcll(act));
+            } else if (valid) {
+                atomicAdd(&h[p][d], 1u);
             }
-            if ((act >> lane) & 1ull) atomicAdd(&h[p][d], 1u);
         }
     }
     __syncthreads();
@@ -337,10 +336,7 @@ int sort_impl(Context* c, void* keys[2], int32_t* vals[2], int64_t n, int end_bi
         const unsigned zb = (unsigned)std::min<int64_t>((words + 255) / 256, 4096);
         hipLaunchKernelGGL(k_zero_u32, dim3(zb), dim3(256), 0, c->stream, status, words);
         HX_TRY(check_launch("sort_zero"));
-        // Two workgroups per CU: each adds its P x R counts to the global
-        // histogram once, so the grid size bounds those contended atomics.
-        const unsigned hb = (unsigned)std::max<int64_t>(
-            1, std::min<int64_t>((n + kSortBlock * kSortItems - 1) / (kSortBlock * kSortItems), 2 * c->cu_count));
+        const unsigned hb = (unsigned)std::min<int64_t>((n + kSortBlock * kSortItems - 1) / (kSortBlock * kSortItems), 2048);
         hipLaunchKernelGGL((k_sort_hist<K, RB>), dim3(hb), dim3(kSortBlock), 0, c->stream,
                            static_cast<const K*>(keys[0]), n, passes, end_bit, hist);
         HX_TRY(check_launch("sort_hist"));

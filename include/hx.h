@@ -227,8 +227,8 @@ int hx_trace_overflow_count(hx_context* ctx, int64_t* out);
  * differentiable rendering of many meshes), which the reference serves one
  * HipBVH at a time.  Triangles of mesh m are [off[m], off[m+1]) of
  * d_triangles (vertex indices into the shared d_vertices); each mesh gets the
- * BVH a separate LBVH build of it would give (same Bin tree, Woop rows and
- * mesh-local triangle ids).  Every mesh needs >= 2 triangles.  With
+ * BVH a separate build of it would give (params->method LBVH or PLOC: same
+ * Bin tree, Woop rows and mesh-local triangle ids).  Every mesh needs >= 2 triangles.  With
  * params->presplitting each mesh is presplit alone (its own box and budget
  * beta * its triangle count), so a mesh has more references than triangles. */
 typedef struct hx_batch hx_batch;

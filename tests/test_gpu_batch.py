@@ -232,3 +232,71 @@ def test_batch_presplit_trace_refit_backward(oracle, ctx):
                              dl_dt=dl_dt[sel].contiguous()).cpu().numpy()
         np.testing.assert_allclose(grad[base:base + len(v)], gm, rtol=1e-5, atol=1e-6)
         base += len(v)
+
+
+def _check_meshes_ploc(oracle, hb, meshes, bits, radius, beta=None):
+    for m, (v, t) in enumerate(meshes):
+        got = hb.mesh_host(m)
+        ref = oracle.build_ploc(v, t, bits=bits, radius=radius, presplit_beta=beta)
+        assert (got["n_interior"], got["n_leaves"]) == (ref["n_interior"], ref["n_leaves"]), f"mesh {m} counts"
+        assert oracle.validate_bin_refs(got["nodes"], got["n_interior"], got["n_leaves"], got["tri_idx"], v, t) == 0
+        cg = oracle.bin_canonical(got["nodes"], got["n_interior"], got["n_leaves"], got["tri_idx"])
+        co = oracle.bin_canonical(ref["nodes"], ref["n_interior"], ref["n_leaves"], ref["tri_idx"])
+        assert np.array_equal(cg, co), f"mesh {m} canonical tree"
+        assert np.array_equal(got["woop"].view(np.uint32), oracle.woopify(v, t, got["tri_idx"]).view(np.uint32))
+
+
+@pytest.mark.parametrize("radius", [1, 8, 25])
+def test_batch_ploc_matches_each_mesh_alone(oracle, ctx, radius):
+    """Clusters merge only inside their mesh: every mesh gets its own PLOC tree."""
+    meshes = _meshes(30, seed=30 + radius, lo=2, hi=5000)
+    hb = hx.HipBatch(ctx, hx.MeshBatch.from_meshes(meshes), method="ploc", ploc_radius=radius)
+    _check_meshes_ploc(oracle, hb, meshes, 60, radius)
+
+
+@pytest.mark.parametrize("method", ["lbvh", "ploc"])
+def test_batch_two_triangle_meshes(oracle, ctx, method):
+    """The smallest meshes a batch takes (two triangles) first, between others
+    and last; a one-triangle mesh is rejected."""
+    two = [scenes.batch_mesh(2, np.random.default_rng(s), False) for s in range(4)]
+    meshes = [two[0]] + _meshes(6, seed=40, lo=2, hi=300) + [two[1], two[2]] + _meshes(3, seed=41, lo=2, hi=50)
+    meshes.append(two[3])
+    hb = hx.HipBatch(ctx, hx.MeshBatch.from_meshes(meshes), method=method, ploc_radius=8)
+    if method == "lbvh":
+        _check_meshes(oracle, hb, meshes, 60)
+    else:
+        _check_meshes_ploc(oracle, hb, meshes, 60, 8)
+    one = scenes.batch_mesh(1, np.random.default_rng(5), False)
+    with pytest.raises(hx._lib.HxError, match="fewer than 2"):
+        hx.HipBatch(ctx, hx.MeshBatch.from_meshes([two[0], one]), method=method)
+
+
+def test_batch_ploc_presplit_config5(oracle, ctx):
+    """Config 5 shape: uneven meshes with 30% slivers, presplit (beta 1.0 so the
+    slivers get a budget), PLOC r=25; rebuild, trace and refit per mesh."""
+    meshes = _config5_meshes(24, seed=31, lo=64, hi=4000)
+    mb = hx.MeshBatch.from_meshes(meshes)
+    hb = hx.HipBatch(ctx, mb, method="ploc", ploc_radius=25, presplitting=1, presplitter_beta=1.0)
+    _check_meshes_ploc(oracle, hb, meshes, 60, 25, beta=1.0)
+    hb.rebuild()
+    _check_meshes_ploc(oracle, hb, meshes, 60, 25, beta=1.0)
+    rays, owner = scenes.batch_rays(meshes, per_1k=1500, seed=7)
+    d_rays = torch.as_tensor(rays, device="cuda")
+    d_owner = torch.as_tensor(owner, device="cuda")
+    res = hb.trace(d_rays, d_owner).cpu().numpy()
+    for m, (v, t) in enumerate(meshes):
+        sel = owner == m
+        brute = oracle.trace_brute(v, t, rays[sel])
+        assert np.array_equal(res[sel, 0], brute[:, 0]) and np.array_equal(res[sel, 1], brute[:, 1]), f"mesh {m}"
+    before = [hb.mesh_host(m) for m in range(len(meshes))]
+    rng = np.random.default_rng(3)
+    v_all = mb.scene.vertices.cpu().numpy()
+    v2 = (v_all + rng.normal(0.0, 0.001, v_all.shape)).astype(np.float32)
+    hb.refit(v2)
+    base = 0
+    for m, (v, t) in enumerate(meshes):
+        vm = v2[base:base + len(v)]
+        base += len(v)
+        g = hb.mesh_host(m)
+        exp = oracle.refit_final(before[m]["nodes"], g["n_interior"], g["n_leaves"], g["tri_idx"], vm, t)
+        assert np.array_equal(g["nodes"], exp), f"mesh {m} refit"
