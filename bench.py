@@ -423,14 +423,13 @@ def main():
     #   scene_box  12 B index + 36 B vertices per triangle           48n
     #   morton     the same reads + an 8 B key                          56n
     #   sort_hist  one read of the 8 B keys                              8 per key
-    #   construct  8 B keys read; left/right/parent/first/last, leaf
-    #              entries, counters, cover written                     52n
-    #   climb      48 B triangle + 4 B id + 4 B leaf parent + 20 B of
-    #              topology read; leaf and interior boxes (64 B), Woop
-    #              (48 B), costs (8 B), state, cover written            204n
+    #   climb      hierarchy built during the climb: 8 B key + 4 B id +
+    #              48 B packed triangle read; leaf and interior boxes
+    #              (64 B), Woop (48 B), costs (8 B), first/last/left/right
+    #              (16 B), parents (8 B), state, cover, counter written  216n
     #   emit       ~100 B read per internal node, 64 B per Bin node     100n + 64 nodes
     nodes_out = bvh.n_interior + bvh.n_leaves
-    alg = {"scene_box": 48 * n, "morton": 56 * n, "sort_hist": 8 * n, "construct": 52 * n, "climb": 204 * n,
+    alg = {"scene_box": 48 * n, "morton": 56 * n, "sort_hist": 8 * n, "climb": 216 * n,
            "collapse_emit": 100 * n + 64 * nodes_out}
     build_kernels = {}
     for k, b in alg.items():

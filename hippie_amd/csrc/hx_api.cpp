@@ -751,11 +751,10 @@ int run_batch(hx_bvh* b) {
     } else {
         HX_TRY(hx::launch_gather_keys(c, b->tri_idx.as<int32_t>(), korig, n, static_cast<uint64_t*>(keys[0])));
         b->keys_index = 0;
-        HX_TRY(hx::launch_construct(c, keys[0], true, n, b->left.as<int32_t>(), b->right.as<int32_t>(),
-                                    b->parent.as<int32_t>(), b->first.as<int32_t>(), b->last.as<int32_t>(),
-                                    b->counters.as<int32_t>(), b->cover.as<int32_t>(), b->counts.as<int32_t>() + 2,
-                                    mesh_of, off));
-        HX_TRY(hx::launch_climb(c, climb_args(b, false)));
+        hx::ClimbArgs ca = climb_args(b, false);
+        ca.keys = keys[0];
+        ca.keys64 = 1;
+        HX_TRY(hx::launch_lbvh_climb(c, ca));
         HX_TRY(hx::launch_collapse_emit(c, emit_args(b)));
     }
     if (b->refs_active) {
@@ -861,12 +860,10 @@ int run_build(hx_bvh* b) {
     } else if (p.method == HX_METHOD_INSERTION) {
         HX_TRY(run_insertion(b));
     } else {
-        HX_TRY(hx::launch_construct(c, b->keys[b->keys_index].ptr, b->keys64, n, b->left.as<int32_t>(), b->right.as<int32_t>(),
-                                    b->parent.as<int32_t>(), b->first.as<int32_t>(), b->last.as<int32_t>(),
-                                    b->counters.as<int32_t>(),
-                                    b->cover.as<int32_t>(), b->counts.as<int32_t>() + 2));
-        const hx::ClimbArgs ca = climb_args(b, false);
-        HX_TRY(hx::launch_climb(c, ca));
+        hx::ClimbArgs ca = climb_args(b, false);
+        ca.keys = b->keys[b->keys_index].ptr;
+        ca.keys64 = b->keys64 ? 1 : 0;
+        HX_TRY(hx::launch_lbvh_climb(c, ca));
         const hx::EmitArgs ea = emit_args(b);
         HX_TRY(hx::launch_collapse_emit(c, ea));
     }

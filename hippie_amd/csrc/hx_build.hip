@@ -343,8 +343,8 @@ HX_DEV float3 xyz(float4 v) { return make_float3(v.x, v.y, v.z); }
 // A node that must stay an interior whatever its SAH state: the root (of each
 // mesh, in a batched build) and, in a batched build, the separator nodes
 // above the mesh roots (ranges spanning meshes), which are never emitted.
-HX_DEV bool seg_top(const int32_t* mesh_of, const int32_t* off, int node, int first, int last) {
-    if (!mesh_of) return node == 0;
+HX_DEV bool seg_top(const int32_t* mesh_of, const int32_t* off, int n, int first, int last) {
+    if (!mesh_of) return first == 0 && last == n - 1;
     const int m = mesh_of[first];
     return mesh_of[last] != m || (first == off[m] && last == off[m + 1] - 1);
 }
@@ -354,6 +354,49 @@ HX_DEV bool seg_top(const int32_t* mesh_of, const int32_t* off, int node, int fi
 HX_DEV float atr_area(float3 mn, float3 mx) {
     const float x = mx.x - mn.x, y = mx.y - mn.y, z = mx.z - mn.z;
     return 2.0f * (x * y + x * z + y * z);
+}
+
+// Leaf p of the sorted order: its box (the reference box when presplit),
+// Woop rows and SAH leaf cost, written to the leaf's node entries.
+HX_DEV Box3 climb_leaf(const ClimbArgs& a, int p, float& cost) {
+    const int n = (int)a.n;
+    const int id = a.sorted_tri[p];
+    const int tri = a.ref_tri ? a.ref_tri[id] : id;
+    float3 v0, v1, v2;
+    if (a.tri_pack) {
+        const float4 q0 = a.tri_pack[3 * (int64_t)tri], q1 = a.tri_pack[3 * (int64_t)tri + 1];
+        const float4 q2 = a.tri_pack[3 * (int64_t)tri + 2];
+        v0 = make_float3(q0.x, q0.y, q0.z);
+        v1 = make_float3(q0.w, q1.x, q1.y);
+        v2 = make_float3(q1.z, q1.w, q2.x);
+    } else {
+        tri_vertices(a.verts, a.tris, tri, v0, v1, v2);
+    }
+    Box3 tb;
+    if (a.ref_min) {
+        const float4 rmn = a.ref_min[id], rmx = a.ref_max[id];
+        tb.mn = make_float3(rmn.x, rmn.y, rmn.z);
+        tb.mx = make_float3(rmx.x, rmx.y, rmx.z);
+    } else {
+        tb = tri_box(v0, v1, v2);
+    }
+    const int leaf = n - 1 + p;
+    a.nmin[leaf] = make_float4(tb.mn.x, tb.mn.y, tb.mn.z, 0.0f);
+    a.nmax[leaf] = make_float4(tb.mx.x, tb.mx.y, tb.mx.z, 0.0f);
+    float4 w0, w1, w2;
+    woop_rows(v0, v1, v2, w0, w1, w2);
+    a.woop[3 * (int64_t)p + 0] = w0;
+    a.woop[3 * (int64_t)p + 1] = w1;
+    a.woop[3 * (int64_t)p + 2] = w2;
+    cost = 0.0f;
+    if (a.adaptive) {
+        Box3 bb = box_empty();
+        box_grow(bb, tb.mn);
+        box_grow(bb, tb.mx);
+        cost = a.ci * box_area(bb.mn, bb.mx);
+        a.cost[leaf] = cost;
+    }
+    return tb;
 }
 
 // GENERAL: the tree is not a Karras tree (PLOC, ATRBVH), so an internal
@@ -406,42 +449,9 @@ __global__ __launch_bounds__(kClimbThreads) void k_climb_local(ClimbArgs a, int3
         const int p = base + k * kClimbThreads + threadIdx.x;
         if (p > last_leaf) break;
         leaf_parent[k] = a.parent[n - 1 + p];
-        const int id = a.sorted_tri[p];
-        const int tri = a.ref_tri ? a.ref_tri[id] : id;
-        float3 v0, v1, v2;
-        if (a.tri_pack) {
-            const float4 q0 = a.tri_pack[3 * (int64_t)tri], q1 = a.tri_pack[3 * (int64_t)tri + 1];
-            const float4 q2 = a.tri_pack[3 * (int64_t)tri + 2];
-            v0 = make_float3(q0.x, q0.y, q0.z);
-            v1 = make_float3(q0.w, q1.x, q1.y);
-            v2 = make_float3(q1.z, q1.w, q2.x);
-        } else {
-            tri_vertices(a.verts, a.tris, tri, v0, v1, v2);
-        }
-        Box3 tb;
-        if (a.ref_min) {
-            const float4 rmn = a.ref_min[id], rmx = a.ref_max[id];
-            tb.mn = make_float3(rmn.x, rmn.y, rmn.z);
-            tb.mx = make_float3(rmx.x, rmx.y, rmx.z);
-        } else {
-            tb = tri_box(v0, v1, v2);
-        }
+        float cost;
+        const Box3 tb = climb_leaf(a, p, cost);
         const int leaf = n - 1 + p;
-        a.nmin[leaf] = make_float4(tb.mn.x, tb.mn.y, tb.mn.z, 0.0f);
-        a.nmax[leaf] = make_float4(tb.mx.x, tb.mx.y, tb.mx.z, 0.0f);
-        float4 w0, w1, w2;
-        woop_rows(v0, v1, v2, w0, w1, w2);
-        a.woop[3 * (int64_t)p + 0] = w0;
-        a.woop[3 * (int64_t)p + 1] = w1;
-        a.woop[3 * (int64_t)p + 2] = w2;
-        float cost = 0.0f;
-        if (a.adaptive) {
-            Box3 bb = box_empty();
-            box_grow(bb, tb.mn);
-            box_grow(bb, tb.mx);
-            cost = a.ci * box_area(bb.mn, bb.mx);
-            a.cost[leaf] = cost;
-        }
         const int sl = T + p - base;
         s_box[0][sl] = tb.mn.x; s_box[1][sl] = tb.mn.y; s_box[2][sl] = tb.mn.z;
         s_box[3][sl] = tb.mx.x; s_box[4][sl] = tb.mx.y; s_box[5][sl] = tb.mx.z;
@@ -504,7 +514,7 @@ __global__ __launch_bounds__(kClimbThreads) void k_climb_local(ClimbArgs a, int3
             } else {
                 s_state[li] = c.state;
             }
-            if (!a.refit_only && c.state == 0 && !seg_top(a.mesh_of, a.mesh_off, node, first, last))
+            if (!a.refit_only && c.state == 0 && !seg_top(a.mesh_of, a.mesh_off, n, first, last))
                 atomicMax(&s_end[first - base], last);
             child = node;
             node = GENERAL ? a.parent[node] : s_par[li];
@@ -556,6 +566,16 @@ HX_DEV void st_agent(float4* p, float4 v) {
 }
 HX_DEV float ld_agent(float* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 HX_DEV void st_agent(float* p, float v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+HX_DEV int ld_agent(int32_t* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+HX_DEV void st_agent(int32_t* p, int v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+
+// Stores issued so far complete before the next memory operation (the
+// compiler fences keep them above the wait).
+HX_DEV void complete_stores() {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
 
 template <bool ATR>
 __global__ __launch_bounds__(kBlock) void k_climb_top(ClimbArgs a, const int32_t* __restrict__ frontier,
@@ -590,7 +610,7 @@ __global__ __launch_bounds__(kBlock) void k_climb_top(ClimbArgs a, const int32_t
             if (!a.refit_only) {
                 if (a.adaptive) st_agent(a.cost + node, c.cost);
                 a.state[node] = c.state;
-                if (c.state == 0 && !seg_top(a.mesh_of, a.mesh_off, node, first, last))
+                if (c.state == 0 && !seg_top(a.mesh_of, a.mesh_off, (int)a.n, first, last))
                     atomicMax(&a.end_at[first], last);
             }
             // Complete this node's stores before the parent's counter can be
@@ -599,6 +619,205 @@ __global__ __launch_bounds__(kBlock) void k_climb_top(ClimbArgs a, const int32_t
             __builtin_amdgcn_s_waitcnt(0);
             __atomic_signal_fence(__ATOMIC_SEQ_CST);
             node = a.parent[node];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// LBVH hierarchy built during the climb (Apetrei's bottom-up formulation of
+// the Karras binary radix tree; the reference builds it top-down in
+// construct, LBVHBuilderKernels.cu:152-252, then climbs).  With
+// delta(q) = common-prefix length of sorted keys q and q+1 (index tie-break
+// and mesh bits as in kdelta; -1 outside [0, n-2]), node [l, r] is the left
+// child of the node split at r when delta(r) > delta(l-1), and the right
+// child of the node split at l-1 otherwise (the two never tie for distinct
+// augmented keys).  The tree is the Karras tree; internal nodes are numbered
+// by split position s (left child ends at s), which lies inside their range,
+// so the two-level scheme of the climb applies unchanged:
+//   k_lbvh_local: one workgroup per 512 leaves; deltas of its range in LDS;
+//     each arrival at an in-block parent leaves its id and range end in LDS,
+//     the second arrival combines.  A child whose parent straddles the block
+//     (split outside the block, or only one child arrived) joins the frontier.
+//   k_lbvh_top: the frontier climbs with the same rule on global memory, each
+//     arrival writing its side (child id, range end) with sc1 stores that
+//     complete before the parent's counter is bumped.
+// ---------------------------------------------------------------------------
+
+template <bool K64>
+__global__ __launch_bounds__(kClimbThreads) void k_lbvh_local(ClimbArgs a) {
+    constexpr int T = kClimbLeaves;
+    // Slots: internal node s -> s - base, leaf p -> T + p - base.
+    __shared__ float s_box[6][2 * T];
+    __shared__ float s_cost[2 * T];
+    __shared__ int s_par[2 * T];
+    __shared__ int s_cnt[T];
+    __shared__ int s_first[T], s_last[T], s_left[T], s_right[T], s_state[T];
+    __shared__ int s_delta[T + 1];  // delta(base - 1 + k)
+    __shared__ int s_front[T];
+    __shared__ int s_end[T];
+    __shared__ int s_nfront, s_fbase;
+    const int n = (int)a.n;
+    const int base = blockIdx.x * T;
+    const int last_leaf = min(base + T, n) - 1;
+    if (threadIdx.x == 0) s_nfront = 0;
+    for (int i = threadIdx.x; i < T; i += kClimbThreads) {
+        s_cnt[i] = 0;
+        s_end[i] = -1;
+        s_left[i] = -1;
+        s_right[i] = -1;
+        s_par[i] = -1;
+        s_par[T + i] = -1;
+    }
+    for (int k = threadIdx.x; k <= T; k += kClimbThreads) {
+        const int q = base - 1 + k;
+        s_delta[k] = (q < 0 || q >= n - 1) ? -1 : kdelta<K64>(q, q + 1, n, a.keys, a.mesh_of, a.mesh_off);
+    }
+#pragma unroll
+    for (int k = 0; k < T / kClimbThreads; ++k) {
+        const int p = base + k * kClimbThreads + threadIdx.x;
+        if (p > last_leaf) break;
+        float cost;
+        const Box3 tb = climb_leaf(a, p, cost);
+        const int sl = T + p - base;
+        s_box[0][sl] = tb.mn.x; s_box[1][sl] = tb.mn.y; s_box[2][sl] = tb.mn.z;
+        s_box[3][sl] = tb.mx.x; s_box[4][sl] = tb.mx.y; s_box[5][sl] = tb.mx.z;
+        s_cost[sl] = cost;
+    }
+    __syncthreads();
+
+    auto slot = [&](int node) { return node >= n - 1 ? T + (node - (n - 1)) - base : node - base; };
+#pragma unroll
+    for (int k = 0; k < T / kClimbThreads; ++k) {
+        const int p = base + k * kClimbThreads + threadIdx.x;
+        if (p > last_leaf) break;
+        int node = n - 1 + p, ns = T + p - base, l = p, r = p;
+        while (!(l == 0 && r == n - 1)) {
+            const bool as_left = s_delta[r - base + 1] > s_delta[l - base];
+            const int sp = as_left ? r : l - 1;
+            s_par[ns] = sp;
+            if (sp < base || sp >= last_leaf) {
+                s_front[atomicAdd(&s_nfront, 1)] = node;
+                break;
+            }
+            const int li = sp - base;
+            if (as_left) {
+                s_left[li] = node;
+                s_first[li] = l;
+            } else {
+                s_right[li] = node;
+                s_last[li] = r;
+            }
+            if (__hip_atomic_fetch_add(&s_cnt[li], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) == 0) break;
+            int ls, rs;
+            if (as_left) {
+                r = s_last[li];
+                ls = ns;
+                rs = slot(s_right[li]);
+            } else {
+                l = s_first[li];
+                ls = slot(s_left[li]);
+                rs = ns;
+            }
+            const Combined c = combine(
+                a, l, r, make_float3(s_box[0][ls], s_box[1][ls], s_box[2][ls]),
+                make_float3(s_box[3][ls], s_box[4][ls], s_box[5][ls]), s_cost[ls],
+                make_float3(s_box[0][rs], s_box[1][rs], s_box[2][rs]),
+                make_float3(s_box[3][rs], s_box[4][rs], s_box[5][rs]), s_cost[rs]);
+            s_box[0][li] = c.mn.x; s_box[1][li] = c.mn.y; s_box[2][li] = c.mn.z;
+            s_box[3][li] = c.mx.x; s_box[4][li] = c.mx.y; s_box[5][li] = c.mx.z;
+            s_cost[li] = c.cost;
+            s_state[li] = c.state;
+            if (c.state == 0 && !seg_top(a.mesh_of, a.mesh_off, n, l, r)) atomicMax(&s_end[l - base], r);
+            node = sp;
+            ns = li;
+        }
+    }
+    __syncthreads();
+    // In-block parents with one arrival: the other child lies (partly)
+    // outside the block, so the arrived child continues at the top level.
+    for (int i = threadIdx.x; i < T; i += kClimbThreads)
+        if (s_cnt[i] == 1) s_front[atomicAdd(&s_nfront, 1)] = s_left[i] >= 0 ? s_left[i] : s_right[i];
+    __syncthreads();
+    if (threadIdx.x == 0) s_fbase = atomicAdd(a.frontier_count, s_nfront);
+    __syncthreads();
+    for (int i = threadIdx.x; i < s_nfront; i += kClimbThreads) a.frontier[s_fbase + i] = s_front[i];
+    for (int i = threadIdx.x; i < T && base + i <= last_leaf; i += kClimbThreads) {
+        a.end_at[base + i] = s_end[i];
+        a.parent[n - 1 + base + i] = s_par[T + i];
+    }
+    // Internal nodes of the block's split range: the finished ones written
+    // out coalesced, every counter cleared for the top level.
+    for (int i = threadIdx.x; i < T; i += kClimbThreads) {
+        const int node = base + i;
+        if (node >= n - 1) break;
+        a.counters[node] = 0;
+        if (s_cnt[i] != 2) continue;
+        a.first[node] = s_first[i];
+        a.last[node] = s_last[i];
+        a.left[node] = s_left[i];
+        a.right[node] = s_right[i];
+        a.parent[node] = s_par[i];
+        a.nmin[node] = make_float4(s_box[0][i], s_box[1][i], s_box[2][i], 0.0f);
+        a.nmax[node] = make_float4(s_box[3][i], s_box[4][i], s_box[5][i], 0.0f);
+        if (a.adaptive) a.cost[node] = s_cost[i];
+        a.state[node] = s_state[i];
+    }
+}
+
+template <bool K64>
+__global__ __launch_bounds__(kBlock) void k_lbvh_top(ClimbArgs a) {
+    const int count = *a.frontier_count;
+    const int n = (int)a.n;
+    for (int e = blockIdx.x * kBlock + threadIdx.x; e < count; e += gridDim.x * kBlock) {
+        int node = a.frontier[e];
+        int l, r;
+        if (node >= n - 1) {
+            l = r = node - (n - 1);
+        } else {
+            l = a.first[node];
+            r = a.last[node];
+        }
+        while (true) {
+            if (l == 0 && r == n - 1) {
+                a.parent[node] = -1;
+                break;
+            }
+            const int dl = l == 0 ? -1 : kdelta<K64>(l - 1, l, n, a.keys, a.mesh_of, a.mesh_off);
+            const int dr = r == n - 1 ? -1 : kdelta<K64>(r, r + 1, n, a.keys, a.mesh_of, a.mesh_off);
+            const bool as_left = dr > dl;
+            const int sp = as_left ? r : l - 1;
+            a.parent[node] = sp;
+            if (as_left) {
+                st_agent(a.left + sp, node);
+                st_agent(a.first + sp, l);
+            } else {
+                st_agent(a.right + sp, node);
+                st_agent(a.last + sp, r);
+            }
+            complete_stores();
+            if (__hip_atomic_fetch_add(&a.counters[sp], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) break;
+            int lc, rc;
+            if (as_left) {
+                lc = node;
+                rc = ld_agent(a.right + sp);
+                r = ld_agent(a.last + sp);
+            } else {
+                l = ld_agent(a.first + sp);
+                lc = ld_agent(a.left + sp);
+                rc = node;
+            }
+            const float4 lmn = ld_agent(a.nmin + lc), lmx = ld_agent(a.nmax + lc);
+            const float4 rmn = ld_agent(a.nmin + rc), rmx = ld_agent(a.nmax + rc);
+            const float lcost = a.adaptive ? ld_agent(a.cost + lc) : 0.0f;
+            const float rcost = a.adaptive ? ld_agent(a.cost + rc) : 0.0f;
+            const Combined c = combine(a, l, r, xyz(lmn), xyz(lmx), lcost, xyz(rmn), xyz(rmx), rcost);
+            st_agent(a.nmin + sp, make_float4(c.mn.x, c.mn.y, c.mn.z, 0.0f));
+            st_agent(a.nmax + sp, make_float4(c.mx.x, c.mx.y, c.mx.z, 0.0f));
+            if (a.adaptive) st_agent(a.cost + sp, c.cost);
+            a.state[sp] = c.state;
+            if (c.state == 0 && !seg_top(a.mesh_of, a.mesh_off, n, l, r)) atomicMax(&a.end_at[l], r);
+            complete_stores();
+            node = sp;
         }
     }
 }
@@ -706,7 +925,7 @@ HX_DEV int interior_flag(const FlagCtx& f, int i) {
     if (i >= f.n - 1) return 0;
     const int fi = f.first[i], la = f.last[i];
     if (!f.mesh_of) {
-        if (i == 0) return 1;
+        if (fi == 0 && la == f.n - 1) return 1;  // root
     } else {
         const int m = f.mesh_of[fi];
         if (f.mesh_of[la] != m) return 0;                       // separator
@@ -826,7 +1045,7 @@ __global__ __launch_bounds__(kBlock) void k_emit(EmitArgs a, FlagCtx f) {
         if (c < n - 1 && a.state[c] == 1) {
             cidx[k] = ms.interior(a.int_slot[c]);
         } else {
-            const int cf = a.first[c], cl = a.last[c];
+            const int cf = c >= n - 1 ? c - (n - 1) : a.first[c], cl = c >= n - 1 ? cf : a.last[c];
             const int L = ms.leaf(a.leaf_num[cf]);
             cidx[k] = ~L;
             write_node(a.nodes, ms.tab.x + L, cmn[k], cmx[k], cmn[k], cmx[k], cf - ms.toff, cl + 1 - ms.toff,
@@ -844,7 +1063,7 @@ __global__ __launch_bounds__(kBlock) void k_mesh_roots(EmitArgs a, FlagCtx f) {
     if (i >= f.n - 1) return;
     const int fi = f.first[i], la = f.last[i];
     if (!f.mesh_of) {
-        if (i == 0) a.root_slot[0] = a.int_slot[0];
+        if (fi == 0 && la == f.n - 1) a.root_slot[0] = a.int_slot[i];
         return;
     }
     const int m = f.mesh_of[fi];
@@ -1030,6 +1249,24 @@ int launch_climb(Context* c, const ClimbArgs& a) {
         hipLaunchKernelGGL(k_climb_top<false>, dim3(top), dim3(kBlock), 0, c->stream, a, a.frontier,
                            a.frontier_count);
     return check_launch("climb_top");
+}
+
+int launch_lbvh_climb(Context* c, const ClimbArgs& a) {
+    LaunchScope ls(c, "climb");
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, a.frontier_count, 0, (int64_t)1);
+    HX_TRY(check_launch("frontier_reset"));
+    const unsigned blocks = (unsigned)((a.n + kClimbLeaves - 1) / kClimbLeaves);
+    if (a.keys64)
+        hipLaunchKernelGGL(k_lbvh_local<true>, dim3(blocks), dim3(kClimbThreads), 0, c->stream, a);
+    else
+        hipLaunchKernelGGL(k_lbvh_local<false>, dim3(blocks), dim3(kClimbThreads), 0, c->stream, a);
+    HX_TRY(check_launch("lbvh_local"));
+    const unsigned top = (unsigned)std::max(1, std::min<int>(c->cu_count * 4, (int)((a.n + kBlock - 1) / kBlock)));
+    if (a.keys64)
+        hipLaunchKernelGGL(k_lbvh_top<true>, dim3(top), dim3(kBlock), 0, c->stream, a);
+    else
+        hipLaunchKernelGGL(k_lbvh_top<false>, dim3(top), dim3(kBlock), 0, c->stream, a);
+    return check_launch("lbvh_top");
 }
 
 int launch_collapse_emit(Context* c, const EmitArgs& a) {

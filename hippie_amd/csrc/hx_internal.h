@@ -125,12 +125,15 @@ struct ClimbArgs {
     const int32_t* sorted_tri;
     const float* verts;
     const int32_t* tris;
-    const int32_t* left;
-    const int32_t* right;
-    const int32_t* parent;
-    const int32_t* first;     // 2n-1: range of sorted (or DFS-ordered) leaves under each node
-    const int32_t* last;
+    // Written by launch_lbvh_climb, read by launch_climb.
+    int32_t* left;
+    int32_t* right;
+    int32_t* parent;
+    int32_t* first;           // 2n-1: range of sorted (or DFS-ordered) leaves under each node
+    int32_t* last;
     const int32_t* split;     // n-1, or NULL for a Karras tree: last leaf of each node's left child
+    const void* keys;         // launch_lbvh_climb: the sorted Morton keys
+    int keys64;
     int32_t* counters;
     float4* nmin;
     float4* nmax;
@@ -160,6 +163,10 @@ struct ClimbArgs {
     float ct, ci;
 };
 int launch_climb(Context* c, const ClimbArgs& a);
+// Binary radix tree of the sorted keys built bottom-up together with the
+// climb (boxes, Woop rows, costs, collapse states); internal nodes are
+// numbered by split position.  frontier_count must be zero on entry.
+int launch_lbvh_climb(Context* c, const ClimbArgs& a);
 struct EmitArgs {
     int64_t n;
     const int32_t* left;

@@ -115,9 +115,12 @@ typedef struct hx_bvh_debug_view {
     void* sorted_keys;       /* uint64 or uint32 per sorted reference          */
     int32_t keys_are_64bit;
     int32_t* sorted_tri;     /* triangle id per sorted position                */
-    int32_t* left;           /* 2n-1, Karras indexing (leaves n-1..2n-2)        */
-    int32_t* right;
-    int32_t* parent;
+    int32_t* left;           /* children of the n-1 internal nodes: leaves are  */
+    int32_t* right;          /* n-1+p; LBVH builds number internal nodes by     */
+                             /* split position (the Karras node whose left      */
+                             /* child index is s), the other builders' binary   */
+                             /* start trees by Karras index                     */
+    int32_t* parent;         /* 2n-1 (leaves at n-1..2n-2)                      */
     float* node_min;         /* float4[2n-1]                                   */
     float* node_max;
     int32_t* state;          /* n-1 collapse states of internal nodes (0 leaf) */

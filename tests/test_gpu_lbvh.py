@@ -49,11 +49,18 @@ def _check_build(oracle, ctx, v, t, bits, adaptive=True, max_leaf=8, check_binar
     assert np.array_equal(dbg["sorted_keys"], ref["sorted_keys"]), "sorted Morton keys"
     assert np.array_equal(dbg["sorted_tri"], ref["sorted_tri"]), "sorted order"
     if check_binary and n > 1:
-        assert np.array_equal(dbg["left"], ref["left"]), "Karras left"
-        assert np.array_equal(dbg["right"], ref["right"]), "Karras right"
-        assert np.array_equal(dbg["parent"], ref["parent"]), "Karras parent"
-        assert np.array_equal(dbg["node_min"][:, :3].view(np.uint32), ref["nmin"][:, :3].view(np.uint32)), "box min"
-        assert np.array_equal(dbg["node_max"][:, :3].view(np.uint32), ref["nmax"][:, :3].view(np.uint32)), "box max"
+        # The GPU numbers internal nodes by split position (the last leaf of
+        # the left child); a Karras node's left child index is that position.
+        L, R, P = ref["left"][: n - 1], ref["right"][: n - 1], ref["parent"]
+        split = np.where(L < n - 1, L, L - (n - 1))
+        assert np.array_equal(np.sort(split), np.arange(n - 1)), "split positions"
+        node = np.concatenate([split, np.arange(n - 1, 2 * n - 1)])  # Karras index -> GPU index
+        assert np.array_equal(dbg["left"][split], node[L]), "Karras left"
+        assert np.array_equal(dbg["right"][split], node[R]), "Karras right"
+        assert np.array_equal(dbg["parent"][node], np.where(P < 0, -1, node[np.maximum(P, 0)])), "Karras parent"
+        for k in ("min", "max"):
+            got = dbg["node_" + k][node][:, :3].view(np.uint32)
+            assert np.array_equal(got, ref["n" + k][:, :3].view(np.uint32)), "box " + k
     nodes = bvh.nodes_host()
     tri = bvh.tri_indices_host()
     assert bvh.n_interior == ref["n_interior"] and bvh.n_leaves == ref["n_leaves"], "node counts"
