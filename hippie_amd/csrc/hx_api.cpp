@@ -142,6 +142,7 @@ struct hx_bvh {
     // clustering workspace (cluster ids, boxes and neighbours, ping-pong).
     DevBuf split, ploc_work, atr_work, hlbvh_work, ins_work;
     DevBuf node_at;  // general trees: internal node at each split position
+    DevBuf link;     // LBVH builds: arrival words of the climb's top level
     int ploc_rounds = 0;
     int64_t atr_accepted = 0;
     int hlbvh_clusters = 0;
@@ -298,6 +299,7 @@ int reserve_bvh(hx_bvh* b, int64_t n) {
         HX_TRY(b->keys_orig.reserve(8 * (size_t)n));
         HX_TRY(b->tri_local.reserve(4 * (size_t)n));
     }
+    if (b->params.method == HX_METHOD_LBVH) HX_TRY(b->link.reserve(8 * (size_t)n));
     if (b->params.method == HX_METHOD_PLOC) {
         HX_TRY(b->split.reserve(4 * (size_t)n));
         HX_TRY(b->node_at.reserve(4 * (size_t)n));
@@ -356,6 +358,16 @@ hx::ClimbArgs climb_args(hx_bvh* b, bool refit_only) {
     ca.max_leaf_size = p.max_leaf_size;
     ca.ct = p.ct;
     ca.ci = p.ci;
+    return ca;
+}
+
+// The LBVH build's climb, which also builds the hierarchy from the keys.
+hx::ClimbArgs lbvh_climb_args(hx_bvh* b, const void* keys, bool keys64) {
+    hx::ClimbArgs ca = climb_args(b, false);
+    ca.keys = keys;
+    ca.keys64 = keys64 ? 1 : 0;
+    ca.link = b->link.as<uint64_t>();
+    ca.root_id = b->counts.as<int32_t>() + 3;
     return ca;
 }
 
@@ -751,9 +763,7 @@ int run_batch(hx_bvh* b) {
     } else {
         HX_TRY(hx::launch_gather_keys(c, b->tri_idx.as<int32_t>(), korig, n, static_cast<uint64_t*>(keys[0])));
         b->keys_index = 0;
-        hx::ClimbArgs ca = climb_args(b, false);
-        ca.keys = keys[0];
-        ca.keys64 = 1;
+        hx::ClimbArgs ca = lbvh_climb_args(b, keys[0], true);
         HX_TRY(hx::launch_lbvh_climb(c, ca));
         HX_TRY(hx::launch_collapse_emit(c, emit_args(b)));
     }
@@ -860,11 +870,10 @@ int run_build(hx_bvh* b) {
     } else if (p.method == HX_METHOD_INSERTION) {
         HX_TRY(run_insertion(b));
     } else {
-        hx::ClimbArgs ca = climb_args(b, false);
-        ca.keys = b->keys[b->keys_index].ptr;
-        ca.keys64 = b->keys64 ? 1 : 0;
+        const hx::ClimbArgs ca = lbvh_climb_args(b, b->keys[b->keys_index].ptr, b->keys64);
         HX_TRY(hx::launch_lbvh_climb(c, ca));
-        const hx::EmitArgs ea = emit_args(b);
+        hx::EmitArgs ea = emit_args(b);
+        ea.root_id = ca.root_id;
         HX_TRY(hx::launch_collapse_emit(c, ea));
     }
     if (b->refs_active) {

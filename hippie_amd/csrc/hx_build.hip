@@ -691,7 +691,11 @@ __global__ __launch_bounds__(kClimbThreads) void k_lbvh_local(ClimbArgs a) {
         const int p = base + k * kClimbThreads + threadIdx.x;
         if (p > last_leaf) break;
         int node = n - 1 + p, ns = T + p - base, l = p, r = p;
-        while (!(l == 0 && r == n - 1)) {
+        while (true) {
+            if (l == 0 && r == n - 1) {
+                *a.root_id = node;
+                break;
+            }
             const bool as_left = s_delta[r - base + 1] > s_delta[l - base];
             const int sp = as_left ? r : l - 1;
             s_par[ns] = sp;
@@ -746,12 +750,15 @@ __global__ __launch_bounds__(kClimbThreads) void k_lbvh_local(ClimbArgs a) {
         a.parent[n - 1 + base + i] = s_par[T + i];
     }
     // Internal nodes of the block's split range: the finished ones written
-    // out coalesced, every counter cleared for the top level.
+    // out coalesced; the others are completed by the top level, whose
+    // arrival words start cleared.
     for (int i = threadIdx.x; i < T; i += kClimbThreads) {
         const int node = base + i;
         if (node >= n - 1) break;
-        a.counters[node] = 0;
-        if (s_cnt[i] != 2) continue;
+        if (s_cnt[i] != 2) {
+            a.link[node] = 0;
+            continue;
+        }
         a.first[node] = s_first[i];
         a.last[node] = s_last[i];
         a.left[node] = s_left[i];
@@ -764,6 +771,12 @@ __global__ __launch_bounds__(kClimbThreads) void k_lbvh_local(ClimbArgs a) {
     }
 }
 
+// Each arrival at a parent ORs one 64-bit word: presence (bit 63), its node
+// id (bits 31-61) and its far range end (bits 0-30: first of the parent if
+// it is the left child, last if the right), so the second arrival learns
+// the sibling and the parent's range from the atomic's result.  A node's box
+// travels in registers up the chain and through nmin/nmax (sc1 stores that
+// complete before the arrival) to the sibling.
 template <bool K64>
 __global__ __launch_bounds__(kBlock) void k_lbvh_top(ClimbArgs a) {
     const int count = *a.frontier_count;
@@ -777,46 +790,52 @@ __global__ __launch_bounds__(kBlock) void k_lbvh_top(ClimbArgs a) {
             l = a.first[node];
             r = a.last[node];
         }
+        int dl = l == 0 ? -1 : kdelta<K64>(l - 1, l, n, a.keys, a.mesh_of, a.mesh_off);
+        int dr = r == n - 1 ? -1 : kdelta<K64>(r, r + 1, n, a.keys, a.mesh_of, a.mesh_off);
+        float3 mn = xyz(a.nmin[node]), mx = xyz(a.nmax[node]);
+        float cost = a.adaptive ? a.cost[node] : 0.0f;
         while (true) {
             if (l == 0 && r == n - 1) {
                 a.parent[node] = -1;
+                *a.root_id = node;
                 break;
             }
-            const int dl = l == 0 ? -1 : kdelta<K64>(l - 1, l, n, a.keys, a.mesh_of, a.mesh_off);
-            const int dr = r == n - 1 ? -1 : kdelta<K64>(r, r + 1, n, a.keys, a.mesh_of, a.mesh_off);
             const bool as_left = dr > dl;
             const int sp = as_left ? r : l - 1;
+            complete_stores();
+            const uint64_t mine = (1ull << 63) | ((uint64_t)(uint32_t)node << 31) | (uint32_t)(as_left ? l : r);
+            const uint64_t old = __hip_atomic_fetch_or(&a.link[sp], mine, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             a.parent[node] = sp;
             if (as_left) {
-                st_agent(a.left + sp, node);
-                st_agent(a.first + sp, l);
+                a.left[sp] = node;
+                a.first[sp] = l;
             } else {
-                st_agent(a.right + sp, node);
-                st_agent(a.last + sp, r);
+                a.right[sp] = node;
+                a.last[sp] = r;
             }
-            complete_stores();
-            if (__hip_atomic_fetch_add(&a.counters[sp], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) break;
-            int lc, rc;
+            if (old == 0) break;
+            const int sib = (int)((old >> 31) & 0x7fffffffu);
+            const int far = (int)(old & 0x7fffffffu);
+            const float4 smn = ld_agent(a.nmin + sib), smx = ld_agent(a.nmax + sib);
+            const float scost = a.adaptive ? ld_agent(a.cost + sib) : 0.0f;
+            Combined c;
             if (as_left) {
-                lc = node;
-                rc = ld_agent(a.right + sp);
-                r = ld_agent(a.last + sp);
+                r = far;
+                dr = r == n - 1 ? -1 : kdelta<K64>(r, r + 1, n, a.keys, a.mesh_of, a.mesh_off);
+                c = combine(a, l, r, mn, mx, cost, xyz(smn), xyz(smx), scost);
             } else {
-                l = ld_agent(a.first + sp);
-                lc = ld_agent(a.left + sp);
-                rc = node;
+                l = far;
+                dl = l == 0 ? -1 : kdelta<K64>(l - 1, l, n, a.keys, a.mesh_of, a.mesh_off);
+                c = combine(a, l, r, xyz(smn), xyz(smx), scost, mn, mx, cost);
             }
-            const float4 lmn = ld_agent(a.nmin + lc), lmx = ld_agent(a.nmax + lc);
-            const float4 rmn = ld_agent(a.nmin + rc), rmx = ld_agent(a.nmax + rc);
-            const float lcost = a.adaptive ? ld_agent(a.cost + lc) : 0.0f;
-            const float rcost = a.adaptive ? ld_agent(a.cost + rc) : 0.0f;
-            const Combined c = combine(a, l, r, xyz(lmn), xyz(lmx), lcost, xyz(rmn), xyz(rmx), rcost);
-            st_agent(a.nmin + sp, make_float4(c.mn.x, c.mn.y, c.mn.z, 0.0f));
-            st_agent(a.nmax + sp, make_float4(c.mx.x, c.mx.y, c.mx.z, 0.0f));
-            if (a.adaptive) st_agent(a.cost + sp, c.cost);
+            mn = c.mn;
+            mx = c.mx;
+            cost = c.cost;
+            st_agent(a.nmin + sp, make_float4(mn.x, mn.y, mn.z, 0.0f));
+            st_agent(a.nmax + sp, make_float4(mx.x, mx.y, mx.z, 0.0f));
+            if (a.adaptive) st_agent(a.cost + sp, cost);
             a.state[sp] = c.state;
             if (c.state == 0 && !seg_top(a.mesh_of, a.mesh_off, n, l, r)) atomicMax(&a.end_at[l], r);
-            complete_stores();
             node = sp;
         }
     }
@@ -1056,16 +1075,13 @@ __global__ __launch_bounds__(kBlock) void k_emit(EmitArgs a, FlagCtx f) {
     write_node(a.nodes, ms.tab.x + ls, cmn[0], cmx[0], cmn[1], cmx[1], cidx[0], cidx[1], last - first + 1, par);
 }
 
-// Mesh tables from the slot scans: the root's interior slot, then per mesh
-// the node base, first interior slot, first leaf number and interior count.
+// Mesh tables from the slot scans: each mesh root's interior slot (batched
+// builds), then per mesh the node base, first interior slot, first leaf
+// number and interior count.
 __global__ __launch_bounds__(kBlock) void k_mesh_roots(EmitArgs a, FlagCtx f) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= f.n - 1) return;
     const int fi = f.first[i], la = f.last[i];
-    if (!f.mesh_of) {
-        if (fi == 0 && la == f.n - 1) a.root_slot[0] = a.int_slot[i];
-        return;
-    }
     const int m = f.mesh_of[fi];
     if (f.mesh_of[la] == m && fi == f.off[m] && la == f.off[m + 1] - 1) a.root_slot[m] = a.int_slot[i];
 }
@@ -1076,6 +1092,7 @@ __global__ __launch_bounds__(kBlock) void k_mesh_tab(EmitArgs a, FlagCtx f) {
     // Interior slots before position p (position n-1 holds no internal node).
     auto slots_before = [&](int p) { return p < f.n - 1 ? a.int_slot[node_of_pos(f, p)] : a.counts[0]; };
     int ib = 0, lb = 0, ie = a.counts[0];
+    if (!f.mesh_of) a.root_slot[0] = a.int_slot[a.root_id ? *a.root_id : 0];
     if (f.mesh_of) {
         ib = slots_before(f.off[m]);
         lb = a.leaf_num[f.off[m]];
@@ -1303,8 +1320,10 @@ int launch_collapse_emit(Context* c, const EmitArgs& a) {
     }
     {
         LaunchScope ls(c, "collapse_emit");
-        hipLaunchKernelGGL(k_mesh_roots, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, a, f);
-        HX_TRY(check_launch("mesh_roots"));
+        if (a.mesh_of) {
+            hipLaunchKernelGGL(k_mesh_roots, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, a, f);
+            HX_TRY(check_launch("mesh_roots"));
+        }
         hipLaunchKernelGGL(k_mesh_tab, dim3(grid_for(a.n_meshes)), dim3(kBlock), 0, c->stream, a, f);
         HX_TRY(check_launch("mesh_tab"));
         hipLaunchKernelGGL(k_emit, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, a, f);

@@ -132,8 +132,10 @@ struct ClimbArgs {
     int32_t* first;           // 2n-1: range of sorted (or DFS-ordered) leaves under each node
     int32_t* last;
     const int32_t* split;     // n-1, or NULL for a Karras tree: last leaf of each node's left child
-    const void* keys;         // launch_lbvh_climb: the sorted Morton keys
+    const void* keys;         // launch_lbvh_climb: the sorted Morton keys,
     int keys64;
+    uint64_t* link;           // n-1 arrival words of the top level,
+    int32_t* root_id;         // and the root's node id (out)
     int32_t* counters;
     float4* nmin;
     float4* nmax;
@@ -193,6 +195,7 @@ struct EmitArgs {
     // a Karras tree, whose node ids already lie in their ranges.
     const int32_t* split;
     int32_t* node_at;
+    const int32_t* root_id;   // single-mesh root's node id (NULL: node 0)
 };
 int launch_collapse_emit(Context* c, const EmitArgs& a);
 // Refit: re-zero the climb counters and frontier count, then (after the
