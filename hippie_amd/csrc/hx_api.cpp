@@ -368,6 +368,7 @@ hx::ClimbArgs lbvh_climb_args(hx_bvh* b, const void* keys, bool keys64) {
     ca.keys64 = keys64 ? 1 : 0;
     ca.link = b->link.as<uint64_t>();
     ca.root_id = b->counts.as<int32_t>() + 3;
+    ca.all_boxes = b->params.keep_debug ? 1 : 0;
     return ca;
 }
 

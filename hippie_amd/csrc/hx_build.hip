@@ -357,7 +357,9 @@ HX_DEV float atr_area(float3 mn, float3 mx) {
 }
 
 // Leaf p of the sorted order: its box (the reference box when presplit),
-// Woop rows and SAH leaf cost, written to the leaf's node entries.
+// Woop rows and SAH leaf cost; the Woop rows are stored, and with STORE the
+// box and cost to the leaf's node entries.
+template <bool STORE = true>
 HX_DEV Box3 climb_leaf(const ClimbArgs& a, int p, float& cost) {
     const int n = (int)a.n;
     const int id = a.sorted_tri[p];
@@ -381,8 +383,10 @@ HX_DEV Box3 climb_leaf(const ClimbArgs& a, int p, float& cost) {
         tb = tri_box(v0, v1, v2);
     }
     const int leaf = n - 1 + p;
-    a.nmin[leaf] = make_float4(tb.mn.x, tb.mn.y, tb.mn.z, 0.0f);
-    a.nmax[leaf] = make_float4(tb.mx.x, tb.mx.y, tb.mx.z, 0.0f);
+    if (STORE) {
+        a.nmin[leaf] = make_float4(tb.mn.x, tb.mn.y, tb.mn.z, 0.0f);
+        a.nmax[leaf] = make_float4(tb.mx.x, tb.mx.y, tb.mx.z, 0.0f);
+    }
     float4 w0, w1, w2;
     woop_rows(v0, v1, v2, w0, w1, w2);
     a.woop[3 * (int64_t)p + 0] = w0;
@@ -394,7 +398,7 @@ HX_DEV Box3 climb_leaf(const ClimbArgs& a, int p, float& cost) {
         box_grow(bb, tb.mn);
         box_grow(bb, tb.mx);
         cost = a.ci * box_area(bb.mn, bb.mx);
-        a.cost[leaf] = cost;
+        if (STORE) a.cost[leaf] = cost;
     }
     return tb;
 }
@@ -677,7 +681,7 @@ __global__ __launch_bounds__(kClimbThreads) void k_lbvh_local(ClimbArgs a) {
         const int p = base + k * kClimbThreads + threadIdx.x;
         if (p > last_leaf) break;
         float cost;
-        const Box3 tb = climb_leaf(a, p, cost);
+        const Box3 tb = climb_leaf<false>(a, p, cost);
         const int sl = T + p - base;
         s_box[0][sl] = tb.mn.x; s_box[1][sl] = tb.mn.y; s_box[2][sl] = tb.mn.z;
         s_box[3][sl] = tb.mx.x; s_box[4][sl] = tb.mx.y; s_box[5][sl] = tb.mx.z;
@@ -745,9 +749,24 @@ __global__ __launch_bounds__(kClimbThreads) void k_lbvh_local(ClimbArgs a) {
     if (threadIdx.x == 0) s_fbase = atomicAdd(a.frontier_count, s_nfront);
     __syncthreads();
     for (int i = threadIdx.x; i < s_nfront; i += kClimbThreads) a.frontier[s_fbase + i] = s_front[i];
+    // Boxes and costs go out only where a later stage reads them: the top
+    // level reads both for nodes whose parent it completes, the emit reads
+    // the boxes of the children of final interiors (state-1 parents and the
+    // roots, which stay interiors whatever their state).
+    auto store_box = [&](int node, int sl) {
+        const int pp = s_par[sl];
+        const bool parent_here = pp >= base && pp < last_leaf && s_cnt[pp - base] == 2;
+        if (a.all_boxes || !parent_here || s_state[pp - base] == 1 ||
+            seg_top(a.mesh_of, a.mesh_off, n, s_first[pp - base], s_last[pp - base])) {
+            a.nmin[node] = make_float4(s_box[0][sl], s_box[1][sl], s_box[2][sl], 0.0f);
+            a.nmax[node] = make_float4(s_box[3][sl], s_box[4][sl], s_box[5][sl], 0.0f);
+        }
+        if (a.adaptive && (a.all_boxes || !parent_here)) a.cost[node] = s_cost[sl];
+    };
     for (int i = threadIdx.x; i < T && base + i <= last_leaf; i += kClimbThreads) {
         a.end_at[base + i] = s_end[i];
         a.parent[n - 1 + base + i] = s_par[T + i];
+        store_box(n - 1 + base + i, T + i);
     }
     // Internal nodes of the block's split range: the finished ones written
     // out coalesced; the others are completed by the top level, whose
@@ -764,9 +783,7 @@ __global__ __launch_bounds__(kClimbThreads) void k_lbvh_local(ClimbArgs a) {
         a.left[node] = s_left[i];
         a.right[node] = s_right[i];
         a.parent[node] = s_par[i];
-        a.nmin[node] = make_float4(s_box[0][i], s_box[1][i], s_box[2][i], 0.0f);
-        a.nmax[node] = make_float4(s_box[3][i], s_box[4][i], s_box[5][i], 0.0f);
-        if (a.adaptive) a.cost[node] = s_cost[i];
+        store_box(node, i);
         a.state[node] = s_state[i];
     }
 }
@@ -852,10 +869,12 @@ __global__ __launch_bounds__(kBlock) void k_lbvh_top(ClimbArgs a) {
 //   - a final leaf starts at p iff p == 0 or cover[p-1] < p;
 //   - internal node i is a final interior iff i is the root or
 //     (state[i] == 1 and cover[first_i] < last_i).
-// Interior slots number the final interiors in Karras index order (root = 0);
-// leaf slots follow in start order, and a leaf's triangles are the contiguous
-// sorted range [first, last], so the triangle index array is the sorted
-// triangle ids themselves.
+// Interior slots number the final interiors in slot-position order (the node
+// id in Karras and split-numbered trees, the split position in general
+// trees), each mesh's root rotated to its first slot; leaf slots follow in
+// start order, and a leaf's triangles are the contiguous sorted range
+// [first, last], so the triangle index array is the sorted triangle ids
+// themselves.
 // ---------------------------------------------------------------------------
 
 constexpr int kScanItems = 16;
@@ -868,6 +887,90 @@ struct Sum2Op {
     HX_DEV int2 operator()(int2 a, int2 b) const { return make_int2(a.x + b.x, a.y + b.y); }
 };
 
+struct FlagCtx {
+    int n;
+    const int32_t* cover;
+    const int32_t* first;
+    const int32_t* last;
+    const int32_t* state;
+    const int32_t* mesh_of;  // batched builds only
+    const int32_t* off;
+    const int32_t* node_at;  // general trees: node at each split position (else NULL)
+};
+
+// The internal node whose slot position is p (the node at split position p
+// in a general tree; node p itself in Karras and split-numbered trees).
+HX_DEV int node_of_pos(const FlagCtx& f, int p) { return f.node_at ? f.node_at[p] : p; }
+
+HX_DEV int leaf_start(const FlagCtx& f, int p) {
+    return (p < f.n) && (p == 0 || f.cover[p - 1] < p);
+}
+
+HX_DEV int interior_flag(const FlagCtx& f, int i) {
+    if (i >= f.n - 1) return 0;
+    const int fi = f.first[i], la = f.last[i];
+    if (!f.mesh_of) {
+        if (fi == 0 && la == f.n - 1) return 1;  // root
+    } else {
+        const int m = f.mesh_of[fi];
+        if (f.mesh_of[la] != m) return 0;                       // separator
+        if (fi == f.off[m] && la == f.off[m + 1] - 1) return 1;  // mesh root
+    }
+    if (f.state[i] != 1) return 0;
+    return f.cover[fi] < la;
+}
+
+// Single-pass versions of the two scans (decoupled look-back over tiles;
+// each workgroup takes the next tile number from a counter, so every tile it
+// waits on has started and publishes without waiting).  Status words: flag
+// in the top two bits (aggregate or inclusive prefix), value below.
+constexpr uint64_t kLb2Aggregate = 1ull << 62;
+constexpr uint64_t kLb2Prefix = 2ull << 62;
+constexpr uint64_t kLb2Half = (1ull << 31) - 1ull;
+
+// Look-back by one whole wave: lane k reads the status of tile j - k, so 64
+// predecessors arrive per round trip; the lanes up to the nearest published
+// prefix are combined, or, if a tile among them has published nothing yet,
+// the lanes before it, and the walk resumes there.  Tiles before 0 read as
+// an identity prefix.  P: status word type, decoding and combine.
+template <class P>
+HX_DEV typename P::V lookback_wave(const typename P::W* status, int tile) {
+    using V = typename P::V;
+    using W = typename P::W;
+    const int lane = lane_id();
+    V acc = P::identity();
+    int j = tile - 1;
+    while (true) {
+        const int q = j - lane;
+        const W w = q >= 0 ? P::load(status + q) : P::kPrefixIdentity;
+        const int flag = P::flag(w);
+        const uint64_t pre = ballot64(flag == 2);
+        const uint64_t none = ballot64(flag == 0);
+        const int stop = pre ? __ffsll((unsigned long long)pre) - 1 : kWave;
+        const uint64_t upto = stop < kWave ? (2ull << stop) - 1ull : ~0ull;
+        const uint64_t waiting = none & upto;
+        const int take = waiting ? __ffsll((unsigned long long)waiting) - 1 : (stop < kWave ? stop + 1 : kWave);
+        acc = P::combine(acc, P::wave_reduce(lane < take ? P::value(w) : P::identity()));
+        if (!waiting && stop < kWave) return acc;
+        j -= take;
+        if (waiting) __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+struct SlotsLb {  // (interior count, leaf count), 31 bits each
+    using V = int2;
+    using W = uint64_t;
+    static constexpr W kPrefixIdentity = kLb2Prefix;
+    HX_DEV static W load(const W* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+    HX_DEV static int flag(W w) { return (int)(w >> 62); }
+    HX_DEV static V value(W w) { return make_int2((int)((w >> 31) & kLb2Half), (int)(w & kLb2Half)); }
+    HX_DEV static V identity() { return make_int2(0, 0); }
+    HX_DEV static V combine(V a, V b) { return make_int2(a.x + b.x, a.y + b.y); }
+    HX_DEV static V wave_reduce(V v) { return make_int2(wave_sum_i32(v.x), wave_sum_i32(v.y)); }
+};
+
+// cover[p] = max over a <= p of end_at[a], in place: tile maxima, their
+// exclusive scan, then the tile scans.
 __global__ __launch_bounds__(kBlock) void k_tile_max(const int32_t* __restrict__ in, int64_t n, int32_t* __restrict__ partial) {
     const int64_t base = blockIdx.x * (int64_t)kScanTile;
     int m = -1;
@@ -887,7 +990,6 @@ __global__ __launch_bounds__(kBlock) void k_tile_max(const int32_t* __restrict__
     }
 }
 
-// Single block: exclusive prefix max of the tile partials, in place.
 __global__ __launch_bounds__(kBlock) void k_scan_partials_max(int32_t* __restrict__ partial, int64_t ntiles) {
     __shared__ int wt[kWavesPerBlock];
     __shared__ int incs[kBlock];
@@ -921,93 +1023,63 @@ __global__ __launch_bounds__(kBlock) void k_tile_cover(int32_t* __restrict__ cov
     }
 }
 
-struct FlagCtx {
-    int n;
-    const int32_t* cover;
-    const int32_t* first;
-    const int32_t* last;
-    const int32_t* state;
-    const int32_t* mesh_of;  // batched builds only
-    const int32_t* off;
-    const int32_t* node_at;  // general trees: node at each split position (else NULL)
-};
-
-// The internal node whose slot position is p (its split position in a
-// general tree, its id in a Karras tree).
-HX_DEV int node_of_pos(const FlagCtx& f, int p) { return f.node_at ? f.node_at[p] : p; }
-
-HX_DEV int leaf_start(const FlagCtx& f, int p) {
-    return (p < f.n) && (p == 0 || f.cover[p - 1] < p);
-}
-
-HX_DEV int interior_flag(const FlagCtx& f, int i) {
-    if (i >= f.n - 1) return 0;
-    const int fi = f.first[i], la = f.last[i];
-    if (!f.mesh_of) {
-        if (fi == 0 && la == f.n - 1) return 1;  // root
-    } else {
-        const int m = f.mesh_of[fi];
-        if (f.mesh_of[la] != m) return 0;                       // separator
-        if (fi == f.off[m] && la == f.off[m + 1] - 1) return 1;  // mesh root
-    }
-    if (f.state[i] != 1) return 0;
-    return f.cover[fi] < la;
-}
-
-__global__ __launch_bounds__(kBlock) void k_tile_flag_counts(FlagCtx f, int2* __restrict__ partial) {
-    const int64_t base = blockIdx.x * (int64_t)kScanTile;
-    int cl = 0, ci = 0;
+// Interior slots and leaf numbers in one pass: int_slot[node] is the number
+// of final interiors at earlier positions, complemented (~) when the node is
+// not a final interior; leaf_num[p] the number of leaves starting before p.
+// The last tile writes the totals to counts[0..1].
+__global__ __launch_bounds__(kBlock) void k_slots_lb(FlagCtx f, int32_t* __restrict__ int_slot,
+                                                     int32_t* __restrict__ leaf_num, int32_t* __restrict__ counts,
+                                                     uint64_t* __restrict__ status, uint32_t* __restrict__ counter) {
+    __shared__ int2 wt[kWavesPerBlock];
+    __shared__ int s_tile;
+    __shared__ int2 s_excl;
+    if (threadIdx.x == 0) s_tile = (int)atomicAdd(counter, 1u);
+    __syncthreads();
+    const int tile = s_tile;
+    const int64_t base = tile * (int64_t)kScanTile;
+    const int ntiles = (f.n + kScanTile - 1) / kScanTile;
+    uint32_t bits = 0;  // bit 2k: interior flag, bit 2k+1: leaf start of item k
+    int ci = 0, cl = 0;
+#pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
         const int64_t p = base + k * kBlock + threadIdx.x;
         if (p < f.n) {
-            ci += p < f.n - 1 ? interior_flag(f, node_of_pos(f, (int)p)) : 0;
-            cl += leaf_start(f, (int)p);
+            const int fi = p < f.n - 1 ? interior_flag(f, node_of_pos(f, (int)p)) : 0;
+            const int ls = leaf_start(f, (int)p);
+            bits |= (uint32_t)fi << (2 * k) | (uint32_t)ls << (2 * k + 1);
+            ci += fi;
+            cl += ls;
         }
     }
-    ci = wave_sum_i32(ci);
-    cl = wave_sum_i32(cl);
-    __shared__ int2 ws[kWavesPerBlock];
-    if (lane_id() == 0) ws[threadIdx.x / kWave] = make_int2(ci, cl);
+    int2 agg;
+    block_inclusive_scan<kWavesPerBlock>(make_int2(ci, cl), Sum2Op(), wt, agg);
+    if (threadIdx.x < kWave) {
+        const uint64_t aw = ((uint64_t)agg.x << 31) | (uint64_t)agg.y;
+        if (threadIdx.x == 0)
+            __hip_atomic_store(status + tile, (tile == 0 ? kLb2Prefix : kLb2Aggregate) | aw, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        const int2 excl = tile == 0 ? make_int2(0, 0) : lookback_wave<SlotsLb>(status, tile);
+        if (threadIdx.x == 0) {
+            const uint64_t iw = ((uint64_t)(excl.x + agg.x) << 31) | (uint64_t)(excl.y + agg.y);
+            if (tile != 0) __hip_atomic_store(status + tile, kLb2Prefix | iw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (tile == ntiles - 1) {
+                counts[0] = excl.x + agg.x;
+                counts[1] = excl.y + agg.y;
+            }
+            s_excl = excl;
+        }
+    }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        int2 r = ws[0];
-        for (int i = 1; i < kWavesPerBlock; ++i) r = make_int2(r.x + ws[i].x, r.y + ws[i].y);
-        partial[blockIdx.x] = r;
-    }
-}
-
-__global__ __launch_bounds__(kBlock) void k_scan_partials_sum2(int2* __restrict__ partial, int64_t ntiles,
-                                                               int32_t* __restrict__ counts) {
-    __shared__ int2 wt[kWavesPerBlock];
-    int2 carry = make_int2(0, 0);
-    for (int64_t base = 0; base < ntiles; base += kBlock) {
-        const int64_t t = base + threadIdx.x;
-        const int2 v = t < ntiles ? partial[t] : make_int2(0, 0);
-        int2 tot;
-        const int2 inc = block_inclusive_scan<kWavesPerBlock>(v, Sum2Op(), wt, tot);
-        if (t < ntiles) partial[t] = make_int2(carry.x + inc.x - v.x, carry.y + inc.y - v.y);
-        carry = make_int2(carry.x + tot.x, carry.y + tot.y);
-    }
-    if (threadIdx.x == 0) {
-        counts[0] = carry.x;  // interiors
-        counts[1] = carry.y;  // leaves
-    }
-}
-
-__global__ __launch_bounds__(kBlock) void k_tile_slots(FlagCtx f, const int2* __restrict__ partial,
-                                                       int32_t* __restrict__ int_slot, int32_t* __restrict__ leaf_num) {
-    __shared__ int2 wt[kWavesPerBlock];
-    const int64_t base = blockIdx.x * (int64_t)kScanTile;
-    int2 carry = partial[blockIdx.x];
+    int2 carry = s_excl;
+#pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
         const int64_t p = base + k * kBlock + threadIdx.x;
-        int2 v = make_int2(0, 0);
-        const int node = p < f.n - 1 ? node_of_pos(f, (int)p) : 0;
-        if (p < f.n) v = make_int2(p < f.n - 1 ? interior_flag(f, node) : 0, leaf_start(f, (int)p));
+        const int2 v = make_int2((bits >> (2 * k)) & 1, (bits >> (2 * k + 1)) & 1);
         int2 tot;
         const int2 inc = block_inclusive_scan<kWavesPerBlock>(v, Sum2Op(), wt, tot);
         if (p < f.n) {
-            if (p < f.n - 1) int_slot[node] = carry.x + inc.x - v.x;
+            const int slot = carry.x + inc.x - v.x;
+            if (p < f.n - 1) int_slot[node_of_pos(f, (int)p)] = v.x ? slot : ~slot;
             leaf_num[p] = carry.y + inc.y - v.y;
         }
         carry = make_int2(carry.x + tot.x, carry.y + tot.y);
@@ -1048,10 +1120,10 @@ __global__ __launch_bounds__(kBlock) void k_emit(EmitArgs a, FlagCtx f) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     const int n = (int)a.n;
     if (i >= n - 1) return;
-    if (!interior_flag(f, i)) return;
+    const int s = a.int_slot[i];
+    if (s < 0) return;  // not a final interior
     const int first = a.first[i], last = a.last[i];
     const MeshSlots ms = mesh_slots(a, f, first);
-    const int s = a.int_slot[i];
     const int ls = ms.interior(s);
     const int ch[2] = {a.left[i], a.right[i]};
     int cidx[2];
@@ -1090,7 +1162,11 @@ __global__ __launch_bounds__(kBlock) void k_mesh_tab(EmitArgs a, FlagCtx f) {
     const int m = blockIdx.x * kBlock + threadIdx.x;
     if (m >= a.n_meshes) return;
     // Interior slots before position p (position n-1 holds no internal node).
-    auto slots_before = [&](int p) { return p < f.n - 1 ? a.int_slot[node_of_pos(f, p)] : a.counts[0]; };
+    auto slots_before = [&](int p) {
+        if (p >= f.n - 1) return a.counts[0];
+        const int v = a.int_slot[node_of_pos(f, p)];
+        return v >= 0 ? v : ~v;
+    };
     int ib = 0, lb = 0, ie = a.counts[0];
     if (!f.mesh_of) a.root_slot[0] = a.int_slot[a.root_id ? *a.root_id : 0];
     if (f.mesh_of) {
@@ -1289,7 +1365,7 @@ int launch_lbvh_climb(Context* c, const ClimbArgs& a) {
 int launch_collapse_emit(Context* c, const EmitArgs& a) {
     const int64_t n = a.n;
     const int64_t ntiles = (n + kScanTile - 1) / kScanTile;
-    HX_TRY(c->scan_partials.reserve(sizeof(int2) * (size_t)ntiles + 64));
+    HX_TRY(c->scan_partials.reserve(16 * (size_t)ntiles + 64));
     const bool general = a.split != nullptr;
     if (general) {
         hipLaunchKernelGGL(k_node_at, dim3(grid_for(n - 1)), dim3(kBlock), 0, c->stream, a.split, (int)(n - 1),
@@ -1297,9 +1373,13 @@ int launch_collapse_emit(Context* c, const EmitArgs& a) {
         HX_TRY(check_launch("node_at"));
     }
     FlagCtx f{(int)n, a.cover, a.first, a.last, a.state, a.mesh_of, a.mesh_off, general ? a.node_at : nullptr};
+    // The slots' look-back status words (8 B per tile) and tile counter,
+    // after the cover's tile maxima.
+    int32_t* pm = c->scan_partials.as<int32_t>();
+    uint64_t* st_slots = reinterpret_cast<uint64_t*>(pm + ((ntiles + 1) & ~(int64_t)1));
+    uint32_t* tile_ctr = reinterpret_cast<uint32_t*>(st_slots + ntiles);
     {
         LaunchScope ls(c, "collapse_cover");
-        int32_t* pm = c->scan_partials.as<int32_t>();
         hipLaunchKernelGGL(k_tile_max, dim3((unsigned)ntiles), dim3(kBlock), 0, c->stream, a.cover, n, pm);
         HX_TRY(check_launch("tile_max"));
         hipLaunchKernelGGL(k_scan_partials_max, dim3(1), dim3(kBlock), 0, c->stream, pm, ntiles);
@@ -1309,14 +1389,13 @@ int launch_collapse_emit(Context* c, const EmitArgs& a) {
     }
     {
         LaunchScope ls(c, "collapse_slots");
-        int2* p2 = c->scan_partials.as<int2>();
-        hipLaunchKernelGGL(k_tile_flag_counts, dim3((unsigned)ntiles), dim3(kBlock), 0, c->stream, f, p2);
-        HX_TRY(check_launch("tile_flag_counts"));
-        hipLaunchKernelGGL(k_scan_partials_sum2, dim3(1), dim3(kBlock), 0, c->stream, p2, ntiles, a.counts);
-        HX_TRY(check_launch("scan_partials_sum2"));
-        hipLaunchKernelGGL(k_tile_slots, dim3((unsigned)ntiles), dim3(kBlock), 0, c->stream, f, p2, a.int_slot,
-                           a.leaf_num);
-        HX_TRY(check_launch("tile_slots"));
+        const int64_t words = 2 * ntiles + 1;
+        hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(words)), dim3(kBlock), 0, c->stream,
+                           reinterpret_cast<int32_t*>(st_slots), 0, words);
+        HX_TRY(check_launch("slots_status"));
+        hipLaunchKernelGGL(k_slots_lb, dim3((unsigned)ntiles), dim3(kBlock), 0, c->stream, f, a.int_slot, a.leaf_num,
+                           a.counts, st_slots, tile_ctr);
+        HX_TRY(check_launch("slots_lb"));
     }
     {
         LaunchScope ls(c, "collapse_emit");

@@ -136,6 +136,7 @@ struct ClimbArgs {
     int keys64;
     uint64_t* link;           // n-1 arrival words of the top level,
     int32_t* root_id;         // and the root's node id (out)
+    int all_boxes;            // 1: store every node's box (else only those a later stage reads)
     int32_t* counters;
     float4* nmin;
     float4* nmax;
