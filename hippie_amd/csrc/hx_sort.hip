@@ -25,7 +25,6 @@ namespace {
 
 constexpr int kSortBlock = 256;
 constexpr int kSortWaves = kSortBlock / kWave;
-constexpr int kSortItems = 16;  // keys per thread in the histogram grid sizing
 constexpr int kMaxPasses = 8;    // 64-bit keys at 8-bit digits; 7 at 10-bit
 
 constexpr uint32_t kFlagAggregate = 1u << 30;
@@ -50,43 +49,59 @@ HX_DEV uint32_t digit_of(K key, int shift, uint32_t mask) {
     return (uint32_t)(key >> shift) & mask;
 }
 
-// Histograms of every digit position in one read of the keys.
+// Histograms of every digit position in one read of the keys.  One
+// 1024-thread workgroup per CU, four keys in flight per thread: the LDS
+// histograms are merged into the global one with device-scope atomics,
+// which are expensive, so few workgroups merge.
+constexpr int kHistBlock = 1024;
+constexpr int kHistUnroll = 4;
+
 template <class K, int RB>
-__global__ __launch_bounds__(kSortBlock) void k_sort_hist(const K* __restrict__ keys, int64_t n, int passes,
+__global__ __launch_bounds__(kHistBlock) void k_sort_hist(const K* __restrict__ keys, int64_t n, int passes,
                                                           int end_bit, uint32_t* __restrict__ hist) {
     constexpr int R = 1 << RB;
     constexpr int P = (8 * (int)sizeof(K) + RB - 1) / RB;
     __shared__ uint32_t h[P][R];
-    for (int i = threadIdx.x; i < P * R; i += kSortBlock) (&h[0][0])[i] = 0u;
+    for (int i = threadIdx.x; i < P * R; i += kHistBlock) (&h[0][0])[i] = 0u;
     __syncthreads();
     // Morton keys in input order share their high digits across a wave, and
     // 64 lanes adding to one LDS word serialise: a pass whose digit is the
     // same in every live lane adds the lane count once; otherwise every lane
     // adds its own.
     const int lane = lane_id();
-    const int64_t stride = (int64_t)gridDim.x * kSortBlock;
-    const int64_t n_round = (n + kWave - 1) / kWave * kWave;
-    for (int64_t i = blockIdx.x * (int64_t)kSortBlock + threadIdx.x; i < n_round; i += stride) {
-        const bool valid = i < n;
-        const K key = valid ? keys[i] : (K)0;
-        for (int p = 0; p < passes; ++p) {
-            const int shift = RB * p;
-            const int nb = min(RB, end_bit - shift);
-            const uint32_t d = digit_of(key, shift, (1u << nb) - 1u);
+    const int64_t stride = (int64_t)gridDim.x * kHistBlock;
+    const int64_t n_round = (n + kWave - 1) / kWave * kWave;  // whole waves agree on i < n_round
+    for (int64_t i0 = blockIdx.x * (int64_t)kHistBlock + threadIdx.x; i0 < n_round; i0 += kHistUnroll * stride) {
+        K key[kHistUnroll];
+#pragma unroll
+        for (int u = 0; u < kHistUnroll; ++u) {
+            const int64_t i = i0 + u * stride;
+            key[u] = i < n ? keys[i] : (K)0;
+        }
+#pragma unroll
+        for (int u = 0; u < kHistUnroll; ++u) {
+            const int64_t i = i0 + u * stride;
+            if (i >= n_round) break;
+            const bool valid = i < n;
             const uint64_t act = ballot64(valid);
             if (!act) continue;
             const int src = __builtin_ctzll(act);
-            const uint32_t v = (uint32_t)__builtin_amdgcn_readlane((int)d, src);
-            const uint64_t same = ballot64(d == v) & act;
-            if (same == act) {
-                if (lane == src) atomicAdd(&h[p][v], (uint32_t)__popcll(act));
-            } else if (valid) {
-                atomicAdd(&h[p][d], 1u);
+            for (int p = 0; p < passes; ++p) {
+                const int shift = RB * p;
+                const int nb = min(RB, end_bit - shift);
+                const uint32_t d = digit_of(key[u], shift, (1u << nb) - 1u);
+                const uint32_t v = (uint32_t)__builtin_amdgcn_readlane((int)d, src);
+                const uint64_t same = ballot64(d == v) & act;
+                if (same == act) {
+                    if (lane == src) atomicAdd(&h[p][v], (uint32_t)__popcll(act));
+                } else if (valid) {
+                    atomicAdd(&h[p][d], 1u);
+                }
             }
         }
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < passes * R; i += kSortBlock) {
+    for (int i = threadIdx.x; i < passes * R; i += kHistBlock) {
         const uint32_t c = (&h[0][0])[i];
         if (c) atomicAdd(&hist[i], c);
     }
@@ -336,8 +351,9 @@ int sort_impl(Context* c, void* keys[2], int32_t* vals[2], int64_t n, int end_bi
         const unsigned zb = (unsigned)std::min<int64_t>((words + 255) / 256, 4096);
         hipLaunchKernelGGL(k_zero_u32, dim3(zb), dim3(256), 0, c->stream, status, words);
         HX_TRY(check_launch("sort_zero"));
-        const unsigned hb = (unsigned)std::min<int64_t>((n + kSortBlock * kSortItems - 1) / (kSortBlock * kSortItems), 2048);
-        hipLaunchKernelGGL((k_sort_hist<K, RB>), dim3(hb), dim3(kSortBlock), 0, c->stream,
+        const unsigned hb = (unsigned)std::max<int64_t>(
+            1, std::min<int64_t>((n + kHistBlock * kHistUnroll - 1) / (kHistBlock * kHistUnroll), c->cu_count));
+        hipLaunchKernelGGL((k_sort_hist<K, RB>), dim3(hb), dim3(kHistBlock), 0, c->stream,
                            static_cast<const K*>(keys[0]), n, passes, end_bit, hist);
         HX_TRY(check_launch("sort_hist"));
         hipLaunchKernelGGL(k_sort_scan_hist<RB>, dim3(passes), dim3(kSortBlock), 0, c->stream, hist);
