@@ -143,6 +143,7 @@ struct hx_bvh {
     DevBuf split, ploc_work, atr_work, hlbvh_work, ins_work;
     DevBuf node_at;  // general trees: internal node at each split position
     DevBuf link;     // LBVH builds: arrival words of the climb's top level
+    DevBuf proto;    // LBVH builds: Bin rows 0-2 of each internal node from the climb
     int ploc_rounds = 0;
     int64_t atr_accepted = 0;
     int hlbvh_clusters = 0;
@@ -299,7 +300,10 @@ int reserve_bvh(hx_bvh* b, int64_t n) {
         HX_TRY(b->keys_orig.reserve(8 * (size_t)n));
         HX_TRY(b->tri_local.reserve(4 * (size_t)n));
     }
-    if (b->params.method == HX_METHOD_LBVH) HX_TRY(b->link.reserve(8 * (size_t)n));
+    if (b->params.method == HX_METHOD_LBVH) {
+        HX_TRY(b->link.reserve(8 * (size_t)n));
+        HX_TRY(b->proto.reserve(48 * (size_t)n));
+    }
     if (b->params.method == HX_METHOD_PLOC) {
         HX_TRY(b->split.reserve(4 * (size_t)n));
         HX_TRY(b->node_at.reserve(4 * (size_t)n));
@@ -369,6 +373,7 @@ hx::ClimbArgs lbvh_climb_args(hx_bvh* b, const void* keys, bool keys64) {
     ca.link = b->link.as<uint64_t>();
     ca.root_id = b->counts.as<int32_t>() + 3;
     ca.all_boxes = b->params.keep_debug ? 1 : 0;
+    ca.proto = b->proto.as<float4>();
     return ca;
 }
 
@@ -766,7 +771,9 @@ int run_batch(hx_bvh* b) {
         b->keys_index = 0;
         hx::ClimbArgs ca = lbvh_climb_args(b, keys[0], true);
         HX_TRY(hx::launch_lbvh_climb(c, ca));
-        HX_TRY(hx::launch_collapse_emit(c, emit_args(b)));
+        hx::EmitArgs ea = emit_args(b);
+        ea.proto = ca.proto;
+        HX_TRY(hx::launch_collapse_emit(c, ea));
     }
     if (b->refs_active) {
         HX_TRY(hx::launch_refs_to_tris(c, b->tri_idx.as<int32_t>(), b->ref_tri.as<int32_t>(), n));
@@ -875,6 +882,7 @@ int run_build(hx_bvh* b) {
         HX_TRY(hx::launch_lbvh_climb(c, ca));
         hx::EmitArgs ea = emit_args(b);
         ea.root_id = ca.root_id;
+        ea.proto = ca.proto;
         HX_TRY(hx::launch_collapse_emit(c, ea));
     }
     if (b->refs_active) {

@@ -356,6 +356,14 @@ HX_DEV float atr_area(float3 mn, float3 mx) {
     return 2.0f * (x * y + x * z + y * z);
 }
 
+// Bin rows 0-2 of a node from its two child boxes (HipBVHKernels.cu layout:
+// x and y extents of each child, then both z extents).
+HX_DEV void bin_rows(float4 amn, float4 amx, float4 bmn, float4 bmx, int4& r0, int4& r1, int4& r2) {
+    r0 = make_int4(__float_as_int(amn.x), __float_as_int(amx.x), __float_as_int(amn.y), __float_as_int(amx.y));
+    r1 = make_int4(__float_as_int(bmn.x), __float_as_int(bmx.x), __float_as_int(bmn.y), __float_as_int(bmx.y));
+    r2 = make_int4(__float_as_int(amn.z), __float_as_int(amx.z), __float_as_int(bmn.z), __float_as_int(bmx.z));
+}
+
 // Leaf p of the sorted order: its box (the reference box when presplit),
 // Woop rows and SAH leaf cost; the Woop rows are stored, and with STORE the
 // box and cost to the leaf's node entries.
@@ -749,19 +757,21 @@ __global__ __launch_bounds__(kClimbThreads) void k_lbvh_local(ClimbArgs a) {
     if (threadIdx.x == 0) s_fbase = atomicAdd(a.frontier_count, s_nfront);
     __syncthreads();
     for (int i = threadIdx.x; i < s_nfront; i += kClimbThreads) a.frontier[s_fbase + i] = s_front[i];
-    // Boxes and costs go out only where a later stage reads them: the top
-    // level reads both for nodes whose parent it completes, the emit reads
-    // the boxes of the children of final interiors (state-1 parents and the
-    // roots, which stay interiors whatever their state).
+    // Boxes and costs go out only where the top level reads them (nodes whose
+    // parent it completes); the emit reads the Bin rows each state-1 or root
+    // node gets from its children's boxes here.
     auto store_box = [&](int node, int sl) {
         const int pp = s_par[sl];
         const bool parent_here = pp >= base && pp < last_leaf && s_cnt[pp - base] == 2;
-        if (a.all_boxes || !parent_here || s_state[pp - base] == 1 ||
-            seg_top(a.mesh_of, a.mesh_off, n, s_first[pp - base], s_last[pp - base])) {
+        if (a.all_boxes || !parent_here) {
             a.nmin[node] = make_float4(s_box[0][sl], s_box[1][sl], s_box[2][sl], 0.0f);
             a.nmax[node] = make_float4(s_box[3][sl], s_box[4][sl], s_box[5][sl], 0.0f);
+            if (a.adaptive) a.cost[node] = s_cost[sl];
         }
-        if (a.adaptive && (a.all_boxes || !parent_here)) a.cost[node] = s_cost[sl];
+    };
+    auto box_at = [&](int sl, float4& mn, float4& mx) {
+        mn = make_float4(s_box[0][sl], s_box[1][sl], s_box[2][sl], 0.0f);
+        mx = make_float4(s_box[3][sl], s_box[4][sl], s_box[5][sl], 0.0f);
     };
     for (int i = threadIdx.x; i < T && base + i <= last_leaf; i += kClimbThreads) {
         a.end_at[base + i] = s_end[i];
@@ -785,6 +795,17 @@ __global__ __launch_bounds__(kClimbThreads) void k_lbvh_local(ClimbArgs a) {
         a.parent[node] = s_par[i];
         store_box(node, i);
         a.state[node] = s_state[i];
+        if (s_state[i] == 1 || seg_top(a.mesh_of, a.mesh_off, n, s_first[i], s_last[i])) {
+            float4 amn, amx, bmn, bmx;
+            box_at(slot(s_left[i]), amn, amx);
+            box_at(slot(s_right[i]), bmn, bmx);
+            int4 r0, r1, r2;
+            bin_rows(amn, amx, bmn, bmx, r0, r1, r2);
+            int4* pr = reinterpret_cast<int4*>(a.proto + 3 * (int64_t)node);
+            pr[0] = r0;
+            pr[1] = r1;
+            pr[2] = r2;
+        }
     }
 }
 
@@ -845,14 +866,27 @@ __global__ __launch_bounds__(kBlock) void k_lbvh_top(ClimbArgs a) {
                 dl = l == 0 ? -1 : kdelta<K64>(l - 1, l, n, a.keys, a.mesh_of, a.mesh_off);
                 c = combine(a, l, r, xyz(smn), xyz(smx), scost, mn, mx, cost);
             }
+            st_agent(a.nmin + sp, make_float4(c.mn.x, c.mn.y, c.mn.z, 0.0f));
+            st_agent(a.nmax + sp, make_float4(c.mx.x, c.mx.y, c.mx.z, 0.0f));
+            if (a.adaptive) st_agent(a.cost + sp, c.cost);
+            a.state[sp] = c.state;
+            const bool top = seg_top(a.mesh_of, a.mesh_off, n, l, r);
+            if (c.state == 0 && !top) atomicMax(&a.end_at[l], r);
+            if (c.state == 1 || top) {
+                const float4 omn = make_float4(mn.x, mn.y, mn.z, 0.0f), omx = make_float4(mx.x, mx.y, mx.z, 0.0f);
+                int4 r0, r1, r2;
+                if (as_left)
+                    bin_rows(omn, omx, smn, smx, r0, r1, r2);
+                else
+                    bin_rows(smn, smx, omn, omx, r0, r1, r2);
+                int4* pr = reinterpret_cast<int4*>(a.proto + 3 * (int64_t)sp);
+                pr[0] = r0;
+                pr[1] = r1;
+                pr[2] = r2;
+            }
             mn = c.mn;
             mx = c.mx;
             cost = c.cost;
-            st_agent(a.nmin + sp, make_float4(mn.x, mn.y, mn.z, 0.0f));
-            st_agent(a.nmax + sp, make_float4(mx.x, mx.y, mx.z, 0.0f));
-            if (a.adaptive) st_agent(a.cost + sp, cost);
-            a.state[sp] = c.state;
-            if (c.state == 0 && !seg_top(a.mesh_of, a.mesh_off, n, l, r)) atomicMax(&a.end_at[l], r);
             node = sp;
         }
     }
@@ -1116,6 +1150,9 @@ HX_DEV MeshSlots mesh_slots(const EmitArgs& a, const FlagCtx& f, int first) {
     return ms;
 }
 
+// PROTO: the rows come from the climb's per-node record (one coalesced read)
+// instead of the children's boxes.
+template <bool PROTO>
 __global__ __launch_bounds__(kBlock) void k_emit(EmitArgs a, FlagCtx f) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     const int n = (int)a.n;
@@ -1126,13 +1163,29 @@ __global__ __launch_bounds__(kBlock) void k_emit(EmitArgs a, FlagCtx f) {
     const MeshSlots ms = mesh_slots(a, f, first);
     const int ls = ms.interior(s);
     const int ch[2] = {a.left[i], a.right[i]};
-    int cidx[2];
+    int4 r0, r1, r2;
     float4 cmn[2], cmx[2];
+    if (PROTO) {
+        const int4* pr = reinterpret_cast<const int4*>(a.proto + 3 * (int64_t)i);
+        r0 = pr[0];
+        r1 = pr[1];
+        r2 = pr[2];
+        cmn[0] = make_float4(__int_as_float(r0.x), __int_as_float(r0.z), __int_as_float(r2.x), 0.0f);
+        cmx[0] = make_float4(__int_as_float(r0.y), __int_as_float(r0.w), __int_as_float(r2.y), 0.0f);
+        cmn[1] = make_float4(__int_as_float(r1.x), __int_as_float(r1.z), __int_as_float(r2.z), 0.0f);
+        cmx[1] = make_float4(__int_as_float(r1.y), __int_as_float(r1.w), __int_as_float(r2.w), 0.0f);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            cmn[k] = a.nmin[ch[k]];
+            cmx[k] = a.nmax[ch[k]];
+        }
+        bin_rows(cmn[0], cmx[0], cmn[1], cmx[1], r0, r1, r2);
+    }
+    int cidx[2];
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
         const int c = ch[k];
-        cmn[k] = a.nmin[c];
-        cmx[k] = a.nmax[c];
         if (c < n - 1 && a.state[c] == 1) {
             cidx[k] = ms.interior(a.int_slot[c]);
         } else {
@@ -1144,7 +1197,11 @@ __global__ __launch_bounds__(kBlock) void k_emit(EmitArgs a, FlagCtx f) {
         }
     }
     const int par = (s == ms.root) ? -1 : ms.interior(a.int_slot[a.parent[i]]);
-    write_node(a.nodes, ms.tab.x + ls, cmn[0], cmx[0], cmn[1], cmx[1], cidx[0], cidx[1], last - first + 1, par);
+    int4* o = a.nodes + 4 * (int64_t)(ms.tab.x + ls);
+    o[0] = r0;
+    o[1] = r1;
+    o[2] = r2;
+    o[3] = make_int4(cidx[0], cidx[1], last - first + 1, par);
 }
 
 // Mesh tables from the slot scans: each mesh root's interior slot (batched
@@ -1405,7 +1462,10 @@ int launch_collapse_emit(Context* c, const EmitArgs& a) {
         }
         hipLaunchKernelGGL(k_mesh_tab, dim3(grid_for(a.n_meshes)), dim3(kBlock), 0, c->stream, a, f);
         HX_TRY(check_launch("mesh_tab"));
-        hipLaunchKernelGGL(k_emit, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, a, f);
+        if (a.proto)
+            hipLaunchKernelGGL(k_emit<true>, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, a, f);
+        else
+            hipLaunchKernelGGL(k_emit<false>, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, a, f);
     }
     return check_launch("emit");
 }
@@ -1420,7 +1480,10 @@ int launch_refit_reset(Context* c, int32_t* counters, int64_t n_internal, int32_
 int launch_emit_only(Context* c, const EmitArgs& a) {
     FlagCtx f{(int)a.n, a.cover, a.first, a.last, a.state, a.mesh_of, a.mesh_off, a.split ? a.node_at : nullptr};
     LaunchScope ls(c, "collapse_emit");
-    hipLaunchKernelGGL(k_emit, dim3(grid_for(a.n)), dim3(kBlock), 0, c->stream, a, f);
+    if (a.proto)
+        hipLaunchKernelGGL(k_emit<true>, dim3(grid_for(a.n)), dim3(kBlock), 0, c->stream, a, f);
+    else
+        hipLaunchKernelGGL(k_emit<false>, dim3(grid_for(a.n)), dim3(kBlock), 0, c->stream, a, f);
     return check_launch("emit");
 }
 

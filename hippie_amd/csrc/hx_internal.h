@@ -137,6 +137,7 @@ struct ClimbArgs {
     uint64_t* link;           // n-1 arrival words of the top level,
     int32_t* root_id;         // and the root's node id (out)
     int all_boxes;            // 1: store every node's box (else only those a later stage reads)
+    float4* proto;            // n-1 x 3: Bin rows 0-2 (both child boxes) of every state-1 or root node
     int32_t* counters;
     float4* nmin;
     float4* nmax;
@@ -197,6 +198,7 @@ struct EmitArgs {
     const int32_t* split;
     int32_t* node_at;
     const int32_t* root_id;   // single-mesh root's node id (NULL: node 0)
+    const float4* proto;      // Bin rows 0-2 per internal node from the climb (NULL: from nmin/nmax)
 };
 int launch_collapse_emit(Context* c, const EmitArgs& a);
 // Refit: re-zero the climb counters and frontier count, then (after the
