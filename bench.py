@@ -31,6 +31,27 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "lbvh60_16M_hbm_traffic.md")
+
+
+# The kernels behind each timed build stage, as the PMC summary names them.
+PMC_KERNELS = {"scene_box": ("k_scene_box_partial", "k_scene_box_final"), "morton": ("k_morton",),
+               "sort_hist": ("k_sort_hist",), "climb": ("k_lbvh_local", "k_lbvh_top"),
+               "collapse_emit": ("k_emit",)}
+
+
+def measured_traffic(kernel):
+    """HBM bytes per dispatch of `kernel` (calibrated FETCH_SIZE + WRITE_SIZE
+    read + written MB) from the committed rocprofv3 PMC summary, or None."""
+    try:
+        with open(TRAFFIC_PROFILE) as fh:
+            for line in fh:
+                cells = [c.strip() for c in line.strip().strip("|").split("|")]
+                if len(cells) == 4 and cells[0] == kernel:
+                    return int(round((float(cells[2]) + float(cells[3])) * 1e6))
+    except (OSError, ValueError):
+        return None
+    return None
 
 
 def log(msg):
@@ -441,8 +462,10 @@ def main():
             # one histogram per sort: the build's keys and the two ray sets
             per_launch = (8 * n + 8 * (prim.shape[0] + sec.shape[0])) / (cnt / args.steps)
         gbs = per_launch / (tot / cnt * 1e-3) / 1e9
+        traffic = [measured_traffic(name) for name in PMC_KERNELS.get(k, ())]
         build_kernels[k] = {"avg_ms": round(tot / cnt, 5), "bytes_per_launch": int(per_launch),
-                            "achieved_gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+                            "achieved_gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
+                            "traffic": sum(traffic) if traffic and None not in traffic else None}
 
     builders = bench_builders(ctx, scene, n, args) if args.builders else {}
     if builders:
@@ -488,7 +511,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            # PMC-measured for the 16,777,216-key passes this benchmark runs
+            # (the committed profile's command); None if it is missing.
+            "traffic": measured_traffic("k_onesweep"),
             "bytes_per_launch": int(bytes_per_launch),
             "avg_launch_ms": round(avg_launch_ms, 5),
         },
