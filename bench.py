@@ -442,7 +442,8 @@ def main():
     # Every LBVH build kernel against the same peak, from its algorithmic
     # bytes per launch (n triangles; SAH-collapsed node count for the emit):
     #   scene_box  12 B index + 36 B vertices per triangle           48n
-    #   morton     the same reads + an 8 B key                          56n
+    #   morton     the same reads + an 8 B key + the 48 B packed
+    #              triangle the climb reads                             104n
     #   sort_hist  one read of the 8 B keys                              8 per key
     #   climb      hierarchy built during the climb: 8 B key + 4 B id +
     #              48 B packed triangle read; leaf and interior boxes
@@ -450,7 +451,7 @@ def main():
     #              (16 B), parents (8 B), state, cover, counter written  216n
     #   emit       ~100 B read per internal node, 64 B per Bin node     100n + 64 nodes
     nodes_out = bvh.n_interior + bvh.n_leaves
-    alg = {"scene_box": 48 * n, "morton": 56 * n, "sort_hist": 8 * n, "climb": 216 * n,
+    alg = {"scene_box": 48 * n, "morton": 104 * n, "sort_hist": 8 * n, "climb": 216 * n,
            "collapse_emit": 100 * n + 64 * nodes_out}
     build_kernels = {}
     for k, b in alg.items():
