@@ -446,12 +446,15 @@ def main():
     #              triangle the climb reads                             104n
     #   sort_hist  one read of the 8 B keys                              8 per key
     #   climb      hierarchy built during the climb: 8 B key + 4 B id +
-    #              48 B packed triangle read; leaf and interior boxes
-    #              (64 B), Woop (48 B), costs (8 B), first/last/left/right
-    #              (16 B), parents (8 B), state, cover, counter written  216n
-    #   emit       ~100 B read per internal node, 64 B per Bin node     100n + 64 nodes
+    #              48 B packed triangle read; Woop (48 B), first/last/
+    #              left/right (16 B), parents (8 B), state, cover (8 B)
+    #              and the 48 B Bin rows of the state-1 nodes (about half
+    #              the internal nodes: 24 B per triangle) written       164n
+    #   emit       int_slot of every internal node, then per final
+    #              interior its 48 B Bin rows, range, children, parent
+    #              and their slots (~100 B); 64 B per Bin node          100n + 64 nodes
     nodes_out = bvh.n_interior + bvh.n_leaves
-    alg = {"scene_box": 48 * n, "morton": 104 * n, "sort_hist": 8 * n, "climb": 216 * n,
+    alg = {"scene_box": 48 * n, "morton": 104 * n, "sort_hist": 8 * n, "climb": 164 * n,
            "collapse_emit": 100 * n + 64 * nodes_out}
     build_kernels = {}
     for k, b in alg.items():
