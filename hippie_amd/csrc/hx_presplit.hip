@@ -135,23 +135,46 @@ HX_DEV Box3 tri_bounds(const float3 v[3]) {
 HX_DEV int mesh_at(const PsArgs& a, int t) { return a.mesh_of ? a.mesh_of[t] : 0; }
 HX_DEV const float* frame_box(const PsArgs& a, int m) { return a.mesh_of ? a.mesh_box + 6 * (int64_t)m : a.box; }
 
-// Adds s to acc[2 * m + k]: once per wave when the wave's live lanes share a
-// mesh (all but the waves at mesh boundaries), else per lane.
-HX_DEV void mesh_add_u64(unsigned long long* acc, int k, bool live, int m, unsigned long long s) {
-    const int m0 = __shfl(live ? m : -1, __builtin_ctzll(ballot64(live) | (1ull << 63)), kWave);
-    const bool uniform = __all(!live || m == m0);
-    if (uniform) {
-#pragma unroll
-        for (int sh = 1; sh < kWave; sh <<= 1) {
-            const uint32_t lo = __shfl_xor((uint32_t)s, sh, kWave);
-            const uint32_t hi = __shfl_xor((uint32_t)(s >> 32), sh, kWave);
-            s += ((unsigned long long)hi << 32) | lo;
-        }
-        if (lane_id() == 0 && m0 >= 0 && s) atomicAdd(acc + 2 * (int64_t)m0 + k, s);
-    } else if (live && s) {
-        atomicAdd(acc + 2 * (int64_t)m + k, s);
+// Per-mesh sums acc[2 * m + k] accumulated by a wave over its iterations:
+// while the wave's live lanes share the mesh of the running sum it grows in
+// registers, and it goes out with one device-scope atomic when the mesh
+// changes or the wave ends (device atomics on one word serialise, so a
+// per-wave atomic over 16M references would take milliseconds).  Waves
+// whose lanes span meshes add per lane.
+struct MeshSum {
+    unsigned long long* acc;
+    int k;
+    int m = -1;
+    unsigned long long s = 0;
+    HX_DEV void flush() {
+        if (m >= 0 && s && lane_id() == 0) atomicAdd(acc + 2 * (int64_t)m + k, s);
+        s = 0;
     }
-}
+    HX_DEV void add(bool live, int mesh, unsigned long long v) {
+        const uint64_t act = ballot64(live);
+        if (!act) return;
+        const int m0 = __shfl(mesh, __builtin_ctzll(act), kWave);
+        if (__all(!live || mesh == m0)) {
+            if (!live) v = 0;
+#pragma unroll
+            for (int sh = 1; sh < kWave; sh <<= 1) {
+                const uint32_t lo = __shfl_xor((uint32_t)v, sh, kWave);
+                const uint32_t hi = __shfl_xor((uint32_t)(v >> 32), sh, kWave);
+                v += ((unsigned long long)hi << 32) | lo;
+            }
+            if (m0 != m) {
+                flush();
+                m = m0;
+            }
+            s += v;
+        } else if (live && v) {
+            atomicAdd(acc + 2 * (int64_t)mesh + k, v);
+        }
+    }
+};
+
+// Each workgroup takes kSumChunk consecutive references.
+constexpr int kSumChunk = 64 * kBlock;
 
 // computePriorities (:11-71).
 HX_DEV float priority(const float3 v[3], const SceneFrame& S) {
@@ -173,33 +196,43 @@ HX_DEV float priority(const float3 v[3], const SceneFrame& S) {
 
 // Priorities and their exact fixed-point sums per mesh (macc[2m]).
 __global__ __launch_bounds__(kBlock) void k_ps_priority(PsArgs a) {
-    const int64_t t = blockIdx.x * (int64_t)kBlock + threadIdx.x;
-    const bool live = t < a.n;
-    unsigned long long s = 0;
-    int m = 0;
-    if (live) {
-        m = mesh_at(a, (int)t);
-        const SceneFrame S = frame_of(frame_box(a, m));
-        float3 v[3];
-        tri_verts(a, (int)t, v);
-        const float p = priority(v, S);
-        a.prio[t] = p;
-        if (p >= 0.0f && p <= 1.0e9f) s = (unsigned long long)((double)p * 4294967296.0);
+    MeshSum sum{a.macc, 0};
+    const int64_t end = min((int64_t)(blockIdx.x + 1) * kSumChunk, a.n);
+    for (int64_t t = blockIdx.x * (int64_t)kSumChunk + threadIdx.x; t < (int64_t)(blockIdx.x + 1) * kSumChunk;
+         t += kBlock) {
+        const bool live = t < end;
+        unsigned long long s = 0;
+        int m = 0;
+        if (live) {
+            m = mesh_at(a, (int)t);
+            const SceneFrame S = frame_of(frame_box(a, m));
+            float3 v[3];
+            tri_verts(a, (int)t, v);
+            const float p = priority(v, S);
+            a.prio[t] = p;
+            if (p >= 0.0f && p <= 1.0e9f) s = (unsigned long long)((double)p * 4294967296.0);
+        }
+        sum.add(live, m, s);
     }
-    mesh_add_u64(a.macc, 0, live, m, s);
+    sum.flush();
 }
 
 // sumPrioritiesRound (:116-148): per-mesh sums of floor(D_m p) (macc[2m+1]).
 __global__ __launch_bounds__(kBlock) void k_ps_round_sum(PsArgs a) {
-    const int64_t t = blockIdx.x * (int64_t)kBlock + threadIdx.x;
-    const bool live = t < a.n;
-    unsigned long long s = 0;
-    int m = 0;
-    if (live) {
-        m = mesh_at(a, (int)t);
-        s = (unsigned long long)(long long)f2i_sat(a.Dm[m] * a.prio[t]);
+    MeshSum sum{a.macc, 1};
+    const int64_t end = min((int64_t)(blockIdx.x + 1) * kSumChunk, a.n);
+    for (int64_t t = blockIdx.x * (int64_t)kSumChunk + threadIdx.x; t < (int64_t)(blockIdx.x + 1) * kSumChunk;
+         t += kBlock) {
+        const bool live = t < end;
+        unsigned long long s = 0;
+        int m = 0;
+        if (live) {
+            m = mesh_at(a, (int)t);
+            s = (unsigned long long)(long long)f2i_sat(a.Dm[m] * a.prio[t]);
+        }
+        sum.add(live, m, s);
     }
-    mesh_add_u64(a.macc, 1, live, m, s);
+    sum.flush();
 }
 
 // Per-tile sums of floor(D p) + 1.
@@ -379,12 +412,14 @@ __global__ void k_ps_zero_both(int32_t* qcount) {
 }
 
 // Per-mesh round sums of floor(D_m p) for the multipliers `D` (M entries).
+unsigned sum_blocks(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + kSumChunk - 1) / kSumChunk); }
+
 int round_sums(Context* c, const PsArgs& a, int M, const std::vector<float>& D, std::vector<int64_t>& out) {
     HX_HIP(hipMemcpyAsync(a.Dm, D.data(), sizeof(float) * (size_t)M, hipMemcpyHostToDevice, c->stream));
     hipLaunchKernelGGL(k_ps_zero_u64, dim3(std::min<unsigned>(grid_for(2 * (int64_t)M), 1024u)), dim3(kBlock), 0,
                        c->stream, a.macc, 2 * (int64_t)M);
     HX_TRY(check_launch("ps_zero"));
-    hipLaunchKernelGGL(k_ps_round_sum, dim3(grid_for(a.n)), dim3(kBlock), 0, c->stream, a);
+    hipLaunchKernelGGL(k_ps_round_sum, dim3(sum_blocks(a.n)), dim3(kBlock), 0, c->stream, a);
     HX_TRY(check_launch("ps_round_sum"));
     std::vector<unsigned long long> h(2 * (size_t)M);
     HX_HIP(hipMemcpyAsync(h.data(), a.macc, sizeof(unsigned long long) * h.size(), hipMemcpyDeviceToHost,
@@ -414,7 +449,7 @@ int launch_presplit_count(Context* c, const PsArgs& a, float beta, int M, const 
     hipLaunchKernelGGL(k_ps_zero_u64, dim3(std::min<unsigned>(grid_for(2 * (int64_t)M), 1024u)), dim3(kBlock), 0,
                        c->stream, a.macc, 2 * (int64_t)M);
     HX_TRY(check_launch("ps_zero"));
-    hipLaunchKernelGGL(k_ps_priority, dim3(grid_for(a.n)), dim3(kBlock), 0, c->stream, a);
+    hipLaunchKernelGGL(k_ps_priority, dim3(sum_blocks(a.n)), dim3(kBlock), 0, c->stream, a);
     HX_TRY(check_launch("ps_priority"));
     std::vector<unsigned long long> h(2 * (size_t)M);
     HX_HIP(hipMemcpyAsync(h.data(), a.macc, sizeof(unsigned long long) * h.size(), hipMemcpyDeviceToHost,
