@@ -40,6 +40,7 @@ constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / kWave;
 constexpr int kItems = 16;
 constexpr int kTile = kBlock * kItems;
+constexpr int kPlocBatch = 16;  // rounds launched between host reads
 constexpr int kMaxLdsRadius = 1024;   // window of 256 + 2r boxes, 6 floats each, in LDS
 
 inline unsigned grid_for(int64_t n, int block = kBlock) { return (unsigned)((n + block - 1) / block); }
@@ -98,14 +99,26 @@ HX_DEV Box3 cluster_box(float4 a, float2 b) {
 // (a seventh row of mesh ids in batched builds).  With cmesh, a cluster only
 // looks at clusters of its own mesh: the meshes' cluster runs are contiguous,
 // so this is the single-mesh window clipped to the mesh.
+// The round's cluster count m is read from its record (rounds run ahead of the
+// host); a finished build (m <= 1) makes the round a no-op.  Batched builds
+// pass the cluster mesh ids: once every mesh is one cluster (m <= n_meshes)
+// the round pairs clusters (2k, 2k+1) instead (see launch_ploc).
 template <bool LDS>
 __global__ __launch_bounds__(kBlock) void k_ploc_neighbours(const float4* __restrict__ ca,
-                                                            const float2* __restrict__ cb, int m, int r,
+                                                            const float2* __restrict__ cb,
+                                                            const int4* __restrict__ round, int r,
                                                             int32_t* __restrict__ nb,
-                                                            const int32_t* __restrict__ cmesh) {
+                                                            const int32_t* __restrict__ cmesh_in, int n_meshes) {
     extern __shared__ float s_win[];
+    const int m = round->x;
     const int b0 = blockIdx.x * kBlock;
     const int i = b0 + threadIdx.x;
+    if (m <= 1 || b0 >= m) return;
+    if (cmesh_in && m <= n_meshes) {
+        if (i < m) nb[i] = (i ^ 1) < m ? (i ^ 1) : -1;
+        return;
+    }
+    const int32_t* cmesh = cmesh_in;
     const int lo = max(0, b0 - r);
     if (LDS) {
         const int hi = min(m, b0 + kBlock + r);
@@ -165,14 +178,6 @@ __global__ __launch_bounds__(kBlock) void k_ploc_neighbours(const float4* __rest
     }
 }
 
-// Separator rounds of a batched build (every mesh already one cluster): pair
-// clusters (2k, 2k+1) so the joins keep the meshes in order, whatever their
-// boxes; an odd last cluster waits for the next round.
-__global__ __launch_bounds__(kBlock) void k_ploc_pairs(int m, int32_t* __restrict__ nb) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i < m) nb[i] = (i ^ 1) < m ? (i ^ 1) : -1;
-}
-
 // (merging, surviving) of cluster i (PLOCBuilderKernels.cu:232-240): -- a mutual pair merges
 // into the lower index; the higher index drops out of the compacted list.
 HX_DEV int2 ploc_flags(const int32_t* __restrict__ nb, int m, int64_t i) {
@@ -182,9 +187,12 @@ HX_DEV int2 ploc_flags(const int32_t* __restrict__ nb, int m, int64_t i) {
     return make_int2(mutual && i < j ? 1 : 0, (mutual && i > j) ? 0 : 1);
 }
 
-__global__ __launch_bounds__(kBlock) void k_ploc_tile_counts(const int32_t* __restrict__ nb, int m,
+__global__ __launch_bounds__(kBlock) void k_ploc_tile_counts(const int32_t* __restrict__ nb,
+                                                             const int4* __restrict__ round,
                                                              int2* __restrict__ partial) {
+    const int m = round->x;
     const int64_t base = blockIdx.x * (int64_t)kTile;
+    if (m <= 1 || base >= m) return;
     int2 c = make_int2(0, 0);
     for (int k = 0; k < kItems; ++k) {
         const int2 f = ploc_flags(nb, m, base + k * kBlock + threadIdx.x);
@@ -203,10 +211,21 @@ __global__ __launch_bounds__(kBlock) void k_ploc_tile_counts(const int32_t* __re
     }
 }
 
-// Single block: exclusive scan of the tile counts in place; totals to *total.
-__global__ __launch_bounds__(kBlock) void k_ploc_scan(int2* __restrict__ partial, int ntiles,
-                                                      int2* __restrict__ total) {
+// Single block: exclusive scan of the tile counts in place; the round's
+// (merged, surviving) totals go to its record and the survivors become the
+// next round's cluster count.
+__global__ __launch_bounds__(kBlock) void k_ploc_scan(int2* __restrict__ partial, int4* __restrict__ round) {
     __shared__ int2 wt[kWavesPerBlock];
+    const int m = round->x;
+    if (m <= 1) {
+        if (threadIdx.x == 0) {
+            round->y = 0;
+            round->z = m;
+            round[1] = make_int4(m, 0, 0, 0);
+        }
+        return;
+    }
+    const int ntiles = (m + kTile - 1) / kTile;
     int2 carry = make_int2(0, 0);
     for (int base = 0; base < ntiles; base += kBlock) {
         const int t = base + threadIdx.x;
@@ -216,16 +235,23 @@ __global__ __launch_bounds__(kBlock) void k_ploc_scan(int2* __restrict__ partial
         if (t < ntiles) partial[t] = make_int2(carry.x + inc.x - v.x, carry.y + inc.y - v.y);
         carry = make_int2(carry.x + tot.x, carry.y + tot.y);
     }
-    if (threadIdx.x == 0) *total = carry;
+    if (threadIdx.x == 0) {
+        round->y = carry.x;
+        round->z = carry.y;
+        round[1] = make_int4(carry.y, 0, 0, 0);
+    }
 }
 
 // merge (PLOCBuilderKernels.cu:207-286) + compact (:288-389) in one pass.
 // The new node's box is the select-based union of the two cluster boxes
 // (the reference uses fminf/fmaxf, which differ from selects only in the sign
 // of a zero); its subtree size goes to last[node] for the relabelling.
-__global__ __launch_bounds__(kBlock) void k_ploc_merge(PlocArgs a, int src, int m, int node_offset,
+__global__ __launch_bounds__(kBlock) void k_ploc_merge(PlocArgs a, int src, const int4* __restrict__ round,
                                                        const int2* __restrict__ partial) {
     __shared__ int2 wt[kWavesPerBlock];
+    const int m = round->x;
+    if (m <= 1 || blockIdx.x * (int64_t)kTile >= m) return;
+    const int node_offset = m - 2;
     const int n = (int)a.n;
     const int32_t* __restrict__ cl = a.cl[src];
     const float4* __restrict__ ca = a.ca[src];
@@ -270,6 +296,10 @@ __global__ __launch_bounds__(kBlock) void k_ploc_merge(PlocArgs a, int src, int 
         }
         carry = make_int2(carry.x + tot.x, carry.y + tot.y);
     }
+}
+
+__global__ void k_ploc_round_start(int4* round, int m) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *round = make_int4(m, 0, 0, 0);
 }
 
 // Depth-first placement of the children of the nodes list[0, count) (or, if
@@ -334,57 +364,69 @@ int launch_ploc(Context* c, const PlocArgs& a, int* rounds_out) {
         HX_TRY(check_launch("ploc_leaves"));
     }
     const int64_t max_tiles = (n + kTile - 1) / kTile;
-    HX_TRY(c->scan_partials.reserve(sizeof(int2) * (size_t)(max_tiles + 1) + 64));
+    HX_TRY(c->scan_partials.reserve(sizeof(int2) * (size_t)max_tiles + sizeof(int4) * (kPlocBatch + 1) + 64));
     int2* partial = c->scan_partials.as<int2>();
-    int2* total = partial + max_tiles;
+    int4* rounds = reinterpret_cast<int4*>(partial + ((max_tiles + 1) & ~(int64_t)1));
     std::vector<int2> ranges;   // [lo, hi] of the nodes each round created
-    int m = n, src = 0, r = a.radius;
     // Batched builds merge inside each mesh until every mesh is one cluster
     // (m == n_meshes), then join the mesh roots pairwise in mesh order
     // (separators, never emitted), so mesh m's leaves keep positions
     // [off[m], off[m+1]) after the depth-first relabelling.
     const bool batched = a.cm[0] != nullptr && a.n_meshes > 1;
-    bool per_mesh = batched;
+    // Rounds run on the device in batches of kPlocBatch: each reads its cluster
+    // count from the previous round's record, and the host reads the records
+    // once per batch (grids are sized by the count known at the batch start).
+    int m = n, src = 0, r = a.radius;
+    std::vector<int4> h_rounds(kPlocBatch + 1);
+    hipLaunchKernelGGL(k_ploc_round_start, dim3(1), dim3(64), 0, c->stream, rounds, m);
+    HX_TRY(check_launch("ploc_round_start"));
     while (m > 1) {
-        if (per_mesh && m == a.n_meshes) per_mesh = false;
-        const int32_t* cmesh = per_mesh ? a.cm[src] : nullptr;
-        if (a.adaptive) r = std::min(r + 1, a.max_radius);
         const int ntiles = (m + kTile - 1) / kTile;
-        {
-            LaunchScope ls(c, "ploc_neighbours");
-            const size_t lds = sizeof(float) * (cmesh ? 7 : 6) * (size_t)(kBlock + 2 * r);
-            if (batched && !per_mesh) {
-                hipLaunchKernelGGL(k_ploc_pairs, dim3(grid_for(m)), dim3(kBlock), 0, c->stream, m, a.nb);
-            } else if (r <= kMaxLdsRadius && lds <= 65536) {
-                hipLaunchKernelGGL(k_ploc_neighbours<true>, dim3(grid_for(m)), dim3(kBlock), lds, c->stream, a.ca[src],
-                                   a.cb[src], m, r, a.nb, cmesh);
-            } else {
-                hipLaunchKernelGGL(k_ploc_neighbours<false>, dim3(grid_for(m)), dim3(kBlock), 0, c->stream,
-                                   a.ca[src], a.cb[src], m, r, a.nb, cmesh);
+        for (int b = 0; b < kPlocBatch; ++b) {
+            if (a.adaptive) r = std::min(r + 1, a.max_radius);
+            {
+                LaunchScope ls(c, "ploc_neighbours");
+                const size_t lds = sizeof(float) * (batched ? 7 : 6) * (size_t)(kBlock + 2 * r);
+                const int32_t* cmesh = batched ? a.cm[src] : nullptr;
+                if (r <= kMaxLdsRadius && lds <= 65536) {
+                    hipLaunchKernelGGL(k_ploc_neighbours<true>, dim3(grid_for(m)), dim3(kBlock), lds, c->stream,
+                                       a.ca[src], a.cb[src], rounds + b, r, a.nb, cmesh, a.n_meshes);
+                } else {
+                    hipLaunchKernelGGL(k_ploc_neighbours<false>, dim3(grid_for(m)), dim3(kBlock), 0, c->stream,
+                                       a.ca[src], a.cb[src], rounds + b, r, a.nb, cmesh, a.n_meshes);
+                }
+                HX_TRY(check_launch("ploc_neighbours"));
             }
-            HX_TRY(check_launch("ploc_neighbours"));
+            {
+                LaunchScope ls(c, "ploc_merge");
+                hipLaunchKernelGGL(k_ploc_tile_counts, dim3(ntiles), dim3(kBlock), 0, c->stream, a.nb, rounds + b,
+                                   partial);
+                HX_TRY(check_launch("ploc_tile_counts"));
+                hipLaunchKernelGGL(k_ploc_scan, dim3(1), dim3(kBlock), 0, c->stream, partial, rounds + b);
+                HX_TRY(check_launch("ploc_scan"));
+                hipLaunchKernelGGL(k_ploc_merge, dim3(ntiles), dim3(kBlock), 0, c->stream, a, src, rounds + b,
+                                   partial);
+                HX_TRY(check_launch("ploc_merge"));
+            }
+            src ^= 1;
         }
-        const int node_offset = m - 2;
-        {
-            LaunchScope ls(c, "ploc_merge");
-            hipLaunchKernelGGL(k_ploc_tile_counts, dim3(ntiles), dim3(kBlock), 0, c->stream, a.nb, m, partial);
-            HX_TRY(check_launch("ploc_tile_counts"));
-            hipLaunchKernelGGL(k_ploc_scan, dim3(1), dim3(kBlock), 0, c->stream, partial, ntiles, total);
-            HX_TRY(check_launch("ploc_scan"));
-            hipLaunchKernelGGL(k_ploc_merge, dim3(ntiles), dim3(kBlock), 0, c->stream, a, src, m, node_offset,
-                               partial);
-            HX_TRY(check_launch("ploc_merge"));
-        }
-        int2 t = make_int2(0, 0);
-        HX_HIP(hipMemcpyAsync(&t, total, sizeof(t), hipMemcpyDeviceToHost, c->stream));
+        HX_HIP(hipMemcpyAsync(h_rounds.data(), rounds, sizeof(int4) * h_rounds.size(), hipMemcpyDeviceToHost,
+                              c->stream));
         HX_HIP(hipStreamSynchronize(c->stream));
-        if (t.x <= 0 || t.x + t.y != m) {
-            set_error("PLOC round with %d clusters merged %d pairs (non-finite boxes?)", m, t.x);
-            return HX_ERR_INVALID_ARG;
+        for (int b = 0; b < kPlocBatch; ++b) {
+            const int4 t = h_rounds[b];
+            if (t.x <= 1) break;
+            if (t.y <= 0 || t.y + t.z != t.x) {
+                set_error("PLOC round with %d clusters merged %d pairs (non-finite boxes?)", t.x, t.y);
+                return HX_ERR_INVALID_ARG;
+            }
+            ranges.push_back(make_int2(t.x - 2 - t.y + 1, t.x - 2));
         }
-        ranges.push_back(make_int2(node_offset - t.x + 1, node_offset));
-        m = t.y;
-        src ^= 1;
+        m = h_rounds[kPlocBatch].x;
+        if (m > 1) {
+            hipLaunchKernelGGL(k_ploc_round_start, dim3(1), dim3(64), 0, c->stream, rounds, m);
+            HX_TRY(check_launch("ploc_round_start"));
+        }
     }
     {
         LaunchScope ls(c, "ploc_relabel");
