@@ -507,7 +507,7 @@ int run_front(hx_bvh* b) {
                              rmax, pack));
     b->pack_valid = pack != nullptr;
     // The sorted triangle ids must end in tri_idx: pass p reads vals[p&1].
-    const int passes = hx::sort_pass_count(c, p.morton_bits);
+    const int passes = hx::sort_pass_count(c, p.morton_bits, n, b->keys64);
     int32_t* vals[2];
     vals[passes & 1] = b->tri_idx.as<int32_t>();
     vals[(passes & 1) ^ 1] = b->vals_tmp.as<int32_t>();
@@ -1065,6 +1065,9 @@ int hx_context_set_tuning(hx_context* ctx, const char* name, int32_t value) {
     } else if (!std::strcmp(name, "sort_items")) {
         HX_CHECK_ARG(value == 16 || value == 32, "sort_items must be 16 or 32");
         t.sort_items = value;
+    } else if (!std::strcmp(name, "sort_local_run")) {
+        HX_CHECK_ARG(value >= 0 && value <= 2048, "sort_local_run must be in [0, 2048]");
+        t.sort_local_run = value;
     } else {
         hx::set_error("unknown tuning parameter '%s'", name);
         return HX_ERR_INVALID_ARG;
@@ -1081,6 +1084,7 @@ int hx_context_get_tuning(hx_context* ctx, const char* name, int32_t* value) {
     else if (!std::strcmp(name, "sort_digit_bits")) *value = t.sort_digit_bits;
     else if (!std::strcmp(name, "sort_items")) *value = t.sort_items;
     else if (!std::strcmp(name, "sort_lookback")) *value = t.sort_lookback;
+    else if (!std::strcmp(name, "sort_local_run")) *value = t.sort_local_run;
     else {
         hx::set_error("unknown tuning parameter '%s'", name);
         return HX_ERR_INVALID_ARG;
@@ -1486,7 +1490,7 @@ int hx_trace_sorted(hx_context* ctx, hx_bvh* bvh, const float* d_rays, int64_t n
     void* keys[2] = {c->ray_keys[0].ptr, c->ray_keys[1].ptr};
     int32_t* vals[2] = {c->ray_order[0].as<int32_t>(), c->ray_order[1].as<int32_t>()};
     int ri = 0;
-    HX_TRY(hx::launch_sort_pairs(c, keys, vals, n_rays, key_bits, true, true, &ri));
+    HX_TRY(hx::launch_sort_pairs(c, keys, vals, n_rays, key_bits, true, true, &ri, false));
     const int32_t* order = vals[ri];
     HX_TRY(trace_common(ctx, bvh, d_rays, n_rays, any_hit, d_results, d_stats, order));
     if (d_order)

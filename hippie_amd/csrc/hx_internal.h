@@ -72,6 +72,7 @@ struct Tuning {
     int sort_digit_bits = 8;     // radix digit width: 8 or 10
     int sort_items = 16;         // keys per thread in a sort pass: 16 or 32
     int sort_lookback = 4;       // predecessor tiles read per look-back round trip: 4 or 8
+    int sort_local_run = 64;     // longest run of equal high key bits the local pass ranks (0: full LSD)
 };
 
 struct Context {
@@ -404,14 +405,19 @@ int launch_tri_local(Context* c, const int32_t* sorted_tri, const int32_t* mesh_
                      int32_t* tri_local);
 
 // hx_sort.hip
-// Number of digit passes a sort over bits [0, end_bit) takes (digit width is
-// the "sort_digit_bits" tunable); the result lands in buffer (passes & 1).
-int sort_pass_count(const Context* c, int end_bit);
+// Number of passes a sort of n keys over bits [0, end_bit) takes (digit width
+// is the "sort_digit_bits" tunable; a split sort counts its four high passes
+// and the local pass); the result lands in buffer (passes & 1) unless a split
+// sort falls back to the low passes.
+int sort_pass_count(const Context* c, int end_bit, int64_t n, bool keys64);
 // Stable LSD sort of (key, value) over bits [0, end_bit).  keys/vals ping-pong
 // between [0] and [1]; returns in *result_index which buffer holds the output.
-// vals_identity: the input values are 0..n-1 and are not read.
+// vals_identity: the input values are 0..n-1 and are not read.  allow_split:
+// long keys may take the split sort (sort_local_run), which pays off when runs
+// of equal high bits are short (Morton codes of geometry), not for ray keys
+// whose origins coincide (a pinhole camera's).
 int launch_sort_pairs(Context* c, void* keys[2], int32_t* vals[2], int64_t n, int end_bit, bool keys64,
-                      bool vals_identity, int* result_index);
+                      bool vals_identity, int* result_index, bool allow_split = true);
 
 // hx_trace.hip
 struct TraceArgs {

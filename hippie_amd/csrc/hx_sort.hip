@@ -57,8 +57,8 @@ constexpr int kHistBlock = 1024;
 constexpr int kHistUnroll = 4;
 
 template <class K, int RB>
-__global__ __launch_bounds__(kHistBlock) void k_sort_hist(const K* __restrict__ keys, int64_t n, int passes,
-                                                          int end_bit, uint32_t* __restrict__ hist) {
+__global__ __launch_bounds__(kHistBlock) void k_sort_hist(const K* __restrict__ keys, int64_t n, int lo_bit,
+                                                          int passes, int end_bit, uint32_t* __restrict__ hist) {
     constexpr int R = 1 << RB;
     constexpr int P = (8 * (int)sizeof(K) + RB - 1) / RB;
     __shared__ uint32_t h[P][R];
@@ -87,7 +87,7 @@ __global__ __launch_bounds__(kHistBlock) void k_sort_hist(const K* __restrict__ 
             if (!act) continue;
             const int src = __builtin_ctzll(act);
             for (int p = 0; p < passes; ++p) {
-                const int shift = RB * p;
+                const int shift = lo_bit + RB * p;
                 const int nb = min(RB, end_bit - shift);
                 const uint32_t d = digit_of(key[u], shift, (1u << nb) - 1u);
                 const uint32_t v = (uint32_t)__builtin_amdgcn_readlane((int)d, src);
@@ -327,10 +327,74 @@ __global__ __launch_bounds__(kSortBlock) __attribute__((amdgpu_waves_per_eu(IT <
     }
 }
 
+// The low-bit pass of a sort whose high bits [lo_bit, end) are already sorted
+// stably: each key is ranked inside its run of equal high bits by its low
+// bits, ties by position (so the order stays stable), and written to run
+// start + rank.  A run longer than `limit` keys sets *overflow and nothing of
+// it is written; the caller then runs the remaining digit passes instead.
+// Each workgroup stages its kLocalTile keys and `limit` keys either side in
+// LDS, which covers every run it ranks.
+constexpr int kLocalTile = 4 * kSortBlock;
+constexpr int kLocalMaxRun = 2048;
+
+__global__ __launch_bounds__(kSortBlock) void k_sort_local(const uint64_t* __restrict__ keys_in,
+                                                           const int32_t* __restrict__ vals_in,
+                                                           uint64_t* __restrict__ keys_out,
+                                                           int32_t* __restrict__ vals_out, int64_t n, int lo_bit,
+                                                           int limit, int32_t* __restrict__ overflow) {
+    extern __shared__ uint64_t s_keys[];  // [base - limit, base + kLocalTile + limit)
+    const int64_t base = blockIdx.x * (int64_t)kLocalTile;
+    const int64_t lo = base - limit;
+    const int span = kLocalTile + 2 * limit;
+    for (int k = threadIdx.x; k < span; k += kSortBlock) {
+        const int64_t q = lo + k;
+        s_keys[k] = (q >= 0 && q < n) ? keys_in[q] : 0ull;
+    }
+    __syncthreads();
+    const uint64_t low_mask = (1ull << lo_bit) - 1ull;
+#pragma unroll
+    for (int u = 0; u < kLocalTile / kSortBlock; ++u) {
+        const int64_t p = base + u * kSortBlock + threadIdx.x;
+        if (p >= n) break;
+        const int lp = (int)(p - lo);
+        const uint64_t k = s_keys[lp];
+        const uint64_t hi = k >> lo_bit;
+        // A run of more than limit keys starting at p reaches p + limit: one
+        // probe finds every such run at its first key without walking it.
+        if (p + limit < n && (s_keys[lp + limit] >> lo_bit) == hi) {
+            atomicOr(overflow, 1);
+            continue;
+        }
+        int ls = lp, le = lp + 1;
+        while (lo + ls > 0 && ls > 0 && (s_keys[ls - 1] >> lo_bit) == hi) --ls;
+        while (lo + le < n && le < span && (s_keys[le] >> lo_bit) == hi) ++le;
+        if (le - ls > limit || (ls == 0 && lo > 0) || (le == span && lo + le < n)) {
+            atomicOr(overflow, 1);  // longer than limit (or reaching past the staged keys)
+            continue;
+        }
+        const uint64_t lk = k & low_mask;
+        int r = 0;
+        for (int q = ls; q < le; ++q) {
+            const uint64_t kq = s_keys[q] & low_mask;
+            r += (kq < lk) || (kq == lk && q < lp);
+        }
+        const int64_t dst = lo + ls + r;
+        keys_out[dst] = k;
+        vals_out[dst] = vals_in[p];
+    }
+}
+
+__global__ void k_sort_flag_clear(int32_t* flag) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *flag = 0;
+}
+
+// LSD passes over key bits [lo_bit, end_bit) from buffers 0; *result = the
+// buffer index holding the output.
 template <class K, int RB, int IT, int LB>
-int sort_impl(Context* c, void* keys[2], int32_t* vals[2], int64_t n, int end_bit, bool vals_identity, int* result) {
+int sort_impl(Context* c, void* keys[2], int32_t* vals[2], int64_t n, int lo_bit, int end_bit, bool vals_identity,
+              int* result) {
     constexpr int R = 1 << RB;
-    const int passes = sort_passes(end_bit, RB);
+    const int passes = sort_passes(end_bit - lo_bit, RB);
     *result = 0;
     if (n <= 1 || passes == 0) {
         if (vals_identity && n > 0) {
@@ -354,14 +418,14 @@ int sort_impl(Context* c, void* keys[2], int32_t* vals[2], int64_t n, int end_bi
         const unsigned hb = (unsigned)std::max<int64_t>(
             1, std::min<int64_t>((n + kHistBlock * kHistUnroll - 1) / (kHistBlock * kHistUnroll), c->cu_count));
         hipLaunchKernelGGL((k_sort_hist<K, RB>), dim3(hb), dim3(kHistBlock), 0, c->stream,
-                           static_cast<const K*>(keys[0]), n, passes, end_bit, hist);
+                           static_cast<const K*>(keys[0]), n, lo_bit, passes, end_bit, hist);
         HX_TRY(check_launch("sort_hist"));
         hipLaunchKernelGGL(k_sort_scan_hist<RB>, dim3(passes), dim3(kSortBlock), 0, c->stream, hist);
         HX_TRY(check_launch("sort_scan_hist"));
     }
     for (int p = 0; p < passes; ++p) {
         const int in = p & 1, out = in ^ 1;
-        const int shift = RB * p;
+        const int shift = lo_bit + RB * p;
         const int nb = std::min(RB, end_bit - shift);
         LaunchScope ls(c, "sort_onesweep");
         if (p == 0 && vals_identity)
@@ -378,21 +442,72 @@ int sort_impl(Context* c, void* keys[2], int32_t* vals[2], int64_t n, int end_bi
     return HX_OK;
 }
 
+constexpr int64_t kLocalMinKeys = 1 << 18;
+
+// High passes, the local pass into buffers 1, then (on overflow) the low
+// passes over the high passes' output instead.  A stable LSD sort of any
+// order in which equal keys keep their input order gives the same result, so
+// the fallback may start from the partly sorted buffers.
+template <int RB, int IT, int LB>
+int sort_split(Context* c, void* keys[2], int32_t* vals[2], int64_t n, int lo_bit, int end_bit, bool vals_identity,
+               int* result) {
+    int hi_res = 0;
+    HX_TRY((sort_impl<uint64_t, RB, IT, LB>(c, keys, vals, n, lo_bit, end_bit, vals_identity, &hi_res)));
+    void* k2[2] = {keys[hi_res], keys[hi_res ^ 1]};
+    int32_t* v2[2] = {vals[hi_res], vals[hi_res ^ 1]};
+    HX_TRY(c->sort_status.reserve(sizeof(uint32_t) * 64));
+    int32_t* flag = reinterpret_cast<int32_t*>(c->sort_status.as<uint32_t>());
+    {
+        LaunchScope ls(c, "sort_local");
+        hipLaunchKernelGGL(k_sort_flag_clear, dim3(1), dim3(64), 0, c->stream, flag);
+        HX_TRY(check_launch("sort_flag_clear"));
+        const int limit = std::min(c->tune.sort_local_run, kLocalMaxRun);
+        hipLaunchKernelGGL(k_sort_local, dim3((unsigned)((n + kLocalTile - 1) / kLocalTile)), dim3(kSortBlock),
+                           sizeof(uint64_t) * (size_t)(kLocalTile + 2 * limit), c->stream,
+                           static_cast<const uint64_t*>(k2[0]), v2[0], static_cast<uint64_t*>(k2[1]), v2[1], n,
+                           lo_bit, limit, flag);
+        HX_TRY(check_launch("sort_local"));
+    }
+    int32_t h_flag = 0;
+    HX_HIP(hipMemcpyAsync(&h_flag, flag, sizeof(h_flag), hipMemcpyDeviceToHost, c->stream));
+    HX_HIP(hipStreamSynchronize(c->stream));
+    if (!h_flag) {
+        *result = hi_res ^ 1;
+        return HX_OK;
+    }
+    int lo_res = 0;
+    HX_TRY((sort_impl<uint64_t, RB, IT, LB>(c, k2, v2, n, 0, end_bit, false, &lo_res)));
+    *result = lo_res == 0 ? hi_res : hi_res ^ 1;
+    return HX_OK;
+}
+
 }  // namespace
 
-int sort_pass_count(const Context* c, int end_bit) { return sort_passes(end_bit, c->tune.sort_digit_bits); }
+int sort_pass_count(const Context* c, int end_bit, int64_t n, bool keys64) {
+    const int bits = c->tune.sort_digit_bits;
+    const int lo_bit = end_bit - 4 * bits;
+    if (keys64 && c->tune.sort_local_run > 0 && n >= kLocalMinKeys && lo_bit >= bits) return 5;
+    return sort_passes(end_bit, bits);
+}
 
 int launch_sort_pairs(Context* c, void* keys[2], int32_t* vals[2], int64_t n, int end_bit, bool keys64,
-                      bool vals_identity, int* result_index) {
+                      bool vals_identity, int* result_index, bool allow_split) {
     if (n >= (int64_t)1 << 30) {
         set_error("radix sort: n = %lld exceeds 2^30 - 1", (long long)n);
         return HX_ERR_INVALID_ARG;
     }
     const int bits = c->tune.sort_digit_bits, items = c->tune.sort_items, lb = c->tune.sort_lookback;
-#define HX_SORT_CASE(RB, IT, LB)                                                                              \
-    if (bits == RB && items == IT && lb == LB) {                                                             \
-        if (keys64) return sort_impl<uint64_t, RB, IT, LB>(c, keys, vals, n, end_bit, vals_identity, result_index); \
-        return sort_impl<uint32_t, RB, IT, LB>(c, keys, vals, n, end_bit, vals_identity, result_index);           \
+    // Long keys, many of them: four digit passes over the high bits leave runs
+    // of equal high bits that are short for Morton codes (at most 19 keys for
+    // the 16.7M-triangle benchmark scene at 32 high bits), which one pass
+    // ranks in place; a long run falls back to the remaining passes.
+    const int lo_bit = end_bit - 4 * bits;
+    const bool split = allow_split && keys64 && c->tune.sort_local_run > 0 && n >= kLocalMinKeys && lo_bit >= bits;
+#define HX_SORT_CASE(RB, IT, LB)                                                                                   \
+    if (bits == RB && items == IT && lb == LB) {                                                                  \
+        if (split) return sort_split<RB, IT, LB>(c, keys, vals, n, lo_bit, end_bit, vals_identity, result_index); \
+        if (keys64) return sort_impl<uint64_t, RB, IT, LB>(c, keys, vals, n, 0, end_bit, vals_identity, result_index); \
+        return sort_impl<uint32_t, RB, IT, LB>(c, keys, vals, n, 0, end_bit, vals_identity, result_index);           \
     }
     HX_SORT_CASE(8, 16, 4)
     HX_SORT_CASE(8, 16, 8)

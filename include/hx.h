@@ -150,7 +150,13 @@ int hx_context_profile_reset(hx_context* ctx);
  *   "trace_blocks_per_cu"  persistent trace workgroups per CU (0 = occupancy)
  *   "sort_digit_bits"      radix digit width of every sort (8 or 10)
  *   "sort_items"           keys per thread in a sort pass (16 or 32)
- *   "sort_lookback"        tiles read per sort look-back round trip (4, 8) */
+ *   "sort_lookback"        tiles read per sort look-back round trip (4, 8)
+ *   "sort_local_run"       the builds' Morton sorts and hx_sort_pairs_u64 of
+ *                          > 40-bit keys with at least 2^18 keys run four
+ *                          digit passes over the high bits, then rank each run
+ *                          of equal high bits in place when no run is longer
+ *                          than this (else the remaining passes run); 0 =
+ *                          every pass (ray sorts always).  Same result. */
 int hx_context_set_tuning(hx_context* ctx, const char* name, int32_t value);
 int hx_context_get_tuning(hx_context* ctx, const char* name, int32_t* value);
 

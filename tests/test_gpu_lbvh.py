@@ -180,6 +180,45 @@ def test_sort_pairs(ctx, n, kind):
     assert np.array_equal(vt.cpu().numpy(), vals[order])
 
 
+@pytest.mark.parametrize("run,limit", [(1, 64), (7, 64), (64, 64), (65, 64), (9, 8), (2, 1), (40, 2048), (1500, 2048)])
+def test_sort_split_runs(ctx, run, limit):
+    """Sorts of 60-bit keys with at least 2^18 keys rank each run of equal
+    high 32 bits in place when no run exceeds sort_local_run keys, and run the
+    low digit passes otherwise: both give the stable order, ties included."""
+    n = 300007
+    rng = np.random.default_rng(run * 1000 + limit)
+    high = np.repeat(rng.integers(0, 1 << 32, n // run + 1, dtype=np.uint64), run)[:n]
+    low = rng.integers(0, 4, n, dtype=np.uint64) * np.uint64(1 << 26) + rng.integers(0, 3, n, dtype=np.uint64)
+    k = (high << np.uint64(28)) | low
+    perm = rng.permutation(n)
+    k = k[perm]
+    vals = rng.permutation(n).astype(np.int32)
+    old = ctx.get_tuning("sort_local_run")
+    ctx.set_tuning("sort_local_run", limit)
+    try:
+        kt = torch.from_numpy(k.view(np.int64).copy()).cuda()
+        vt = torch.from_numpy(vals.copy()).cuda()
+        hx.sort_pairs(ctx, kt, vt, 60)
+    finally:
+        ctx.set_tuning("sort_local_run", old)
+    order = np.argsort(k, kind="stable")
+    assert np.array_equal(kt.cpu().numpy().view(np.uint64), k[order])
+    assert np.array_equal(vt.cpu().numpy(), vals[order])
+
+
+def test_lbvh_split_sort_off(oracle, ctx):
+    """The LBVH build with every digit pass (sort_local_run 0) and with the
+    split sort gives the same tree."""
+    v, t = scenes.config2_scene(n_mixture=200000, grid_cells=256, seed=9)
+    old = ctx.get_tuning("sort_local_run")
+    try:
+        ctx.set_tuning("sort_local_run", 0)
+        _check_build(oracle, ctx, v, t, 60)
+    finally:
+        ctx.set_tuning("sort_local_run", old)
+    _check_build(oracle, ctx, v, t, 60)
+
+
 @pytest.mark.parametrize("end_bit", [4, 13, 30, 33, 64])
 def test_sort_pairs_bit_ranges(ctx, end_bit):
     n = 50000
