@@ -37,7 +37,7 @@ TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "prof
 # The kernels behind each timed build stage, as the PMC summary names them.
 PMC_KERNELS = {"scene_box": ("k_scene_box_partial", "k_scene_box_final"), "morton": ("k_morton",),
                "sort_hist": ("k_sort_hist",), "climb": ("k_lbvh_local", "k_lbvh_top"),
-               "collapse_emit": ("k_emit",)}
+               "sort_local": ("k_sort_local",), "collapse_emit": ("k_emit",)}
 
 
 def measured_traffic(kernel):
@@ -431,7 +431,11 @@ def main():
     sort_cnt, sort_ms = prof.get("sort_onesweep", (0, 0.0))
     digit_bits = ctx.get_tuning("sort_digit_bits")
     passes = (60 + digit_bits - 1) // digit_bits
-    bytes_build = n * (20 + 24 * (passes - 1))
+    # The build's Morton sort runs four passes over the high bits and the
+    # local pass when the split sort is on (it is for n >= 2^18 keys); the ray
+    # sorts run every pass.
+    build_passes = 4 if ctx.get_tuning("sort_local_run") > 0 and n >= 1 << 18 else passes
+    bytes_build = n * (20 + 24 * (build_passes - 1))
     bytes_rays = prim.shape[0] * (20 + 24 * (passes - 1)) + sec.shape[0] * (20 + 24 * (passes - 1))
     sort_bytes_per_step = bytes_build + bytes_rays
     launches_per_step = sort_cnt / args.steps
@@ -445,6 +449,8 @@ def main():
     #   morton     the same reads + an 8 B key + the 48 B packed
     #              triangle the climb reads                             104n
     #   sort_hist  one read of the 8 B keys                              8 per key
+    #   sort_local the split sort's local pass: 12 B key + value read
+    #              and written                                          24n
     #   climb      hierarchy built during the climb: 8 B key + 4 B id +
     #              48 B packed triangle read; Woop (48 B), first/last/
     #              left/right (16 B), parents (8 B), state, cover (8 B)
@@ -454,7 +460,7 @@ def main():
     #              interior its 48 B Bin rows, range, children, parent
     #              and their slots (~100 B); 64 B per Bin node          100n + 64 nodes
     nodes_out = bvh.n_interior + bvh.n_leaves
-    alg = {"scene_box": 48 * n, "morton": 104 * n, "sort_hist": 8 * n, "climb": 164 * n,
+    alg = {"scene_box": 48 * n, "morton": 104 * n, "sort_hist": 8 * n, "sort_local": 24 * n, "climb": 164 * n,
            "collapse_emit": 100 * n + 64 * nodes_out}
     build_kernels = {}
     for k, b in alg.items():
