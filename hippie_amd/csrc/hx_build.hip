@@ -655,8 +655,12 @@ __global__ __launch_bounds__(kBlock) void k_climb_top(ClimbArgs a, const int32_t
 //     complete before the parent's counter is bumped.
 // ---------------------------------------------------------------------------
 
+// One leaf per thread: the LDS footprint is the block's, so more threads
+// per block means more waves per CU for the same 512 leaves.
+constexpr int kLbvhThreads = 512;
+
 template <bool K64>
-__global__ __launch_bounds__(kClimbThreads) void k_lbvh_local(ClimbArgs a) {
+__global__ __launch_bounds__(kLbvhThreads) void k_lbvh_local(ClimbArgs a) {
     constexpr int T = kClimbLeaves;
     // Slots: internal node s -> s - base, leaf p -> T + p - base.
     __shared__ float s_box[6][2 * T];
@@ -672,7 +676,7 @@ __global__ __launch_bounds__(kClimbThreads) void k_lbvh_local(ClimbArgs a) {
     const int base = blockIdx.x * T;
     const int last_leaf = min(base + T, n) - 1;
     if (threadIdx.x == 0) s_nfront = 0;
-    for (int i = threadIdx.x; i < T; i += kClimbThreads) {
+    for (int i = threadIdx.x; i < T; i += kLbvhThreads) {
         s_cnt[i] = 0;
         s_end[i] = -1;
         s_left[i] = -1;
@@ -680,13 +684,13 @@ __global__ __launch_bounds__(kClimbThreads) void k_lbvh_local(ClimbArgs a) {
         s_par[i] = -1;
         s_par[T + i] = -1;
     }
-    for (int k = threadIdx.x; k <= T; k += kClimbThreads) {
+    for (int k = threadIdx.x; k <= T; k += kLbvhThreads) {
         const int q = base - 1 + k;
         s_delta[k] = (q < 0 || q >= n - 1) ? -1 : kdelta<K64>(q, q + 1, n, a.keys, a.mesh_of, a.mesh_off);
     }
 #pragma unroll
-    for (int k = 0; k < T / kClimbThreads; ++k) {
-        const int p = base + k * kClimbThreads + threadIdx.x;
+    for (int k = 0; k < T / kLbvhThreads; ++k) {
+        const int p = base + k * kLbvhThreads + threadIdx.x;
         if (p > last_leaf) break;
         float cost;
         const Box3 tb = climb_leaf<false>(a, p, cost);
@@ -699,8 +703,8 @@ __global__ __launch_bounds__(kClimbThreads) void k_lbvh_local(ClimbArgs a) {
 
     auto slot = [&](int node) { return node >= n - 1 ? T + (node - (n - 1)) - base : node - base; };
 #pragma unroll
-    for (int k = 0; k < T / kClimbThreads; ++k) {
-        const int p = base + k * kClimbThreads + threadIdx.x;
+    for (int k = 0; k < T / kLbvhThreads; ++k) {
+        const int p = base + k * kLbvhThreads + threadIdx.x;
         if (p > last_leaf) break;
         int node = n - 1 + p, ns = T + p - base, l = p, r = p;
         while (true) {
@@ -751,12 +755,12 @@ __global__ __launch_bounds__(kClimbThreads) void k_lbvh_local(ClimbArgs a) {
     __syncthreads();
     // In-block parents with one arrival: the other child lies (partly)
     // outside the block, so the arrived child continues at the top level.
-    for (int i = threadIdx.x; i < T; i += kClimbThreads)
+    for (int i = threadIdx.x; i < T; i += kLbvhThreads)
         if (s_cnt[i] == 1) s_front[atomicAdd(&s_nfront, 1)] = s_left[i] >= 0 ? s_left[i] : s_right[i];
     __syncthreads();
     if (threadIdx.x == 0) s_fbase = atomicAdd(a.frontier_count, s_nfront);
     __syncthreads();
-    for (int i = threadIdx.x; i < s_nfront; i += kClimbThreads) a.frontier[s_fbase + i] = s_front[i];
+    for (int i = threadIdx.x; i < s_nfront; i += kLbvhThreads) a.frontier[s_fbase + i] = s_front[i];
     // Boxes and costs go out only where the top level reads them (nodes whose
     // parent it completes); the emit reads the Bin rows each state-1 or root
     // node gets from its children's boxes here.
@@ -773,7 +777,7 @@ __global__ __launch_bounds__(kClimbThreads) void k_lbvh_local(ClimbArgs a) {
         mn = make_float4(s_box[0][sl], s_box[1][sl], s_box[2][sl], 0.0f);
         mx = make_float4(s_box[3][sl], s_box[4][sl], s_box[5][sl], 0.0f);
     };
-    for (int i = threadIdx.x; i < T && base + i <= last_leaf; i += kClimbThreads) {
+    for (int i = threadIdx.x; i < T && base + i <= last_leaf; i += kLbvhThreads) {
         a.end_at[base + i] = s_end[i];
         a.parent[n - 1 + base + i] = s_par[T + i];
         store_box(n - 1 + base + i, T + i);
@@ -781,7 +785,7 @@ __global__ __launch_bounds__(kClimbThreads) void k_lbvh_local(ClimbArgs a) {
     // Internal nodes of the block's split range: the finished ones written
     // out coalesced; the others are completed by the top level, whose
     // arrival words start cleared.
-    for (int i = threadIdx.x; i < T; i += kClimbThreads) {
+    for (int i = threadIdx.x; i < T; i += kLbvhThreads) {
         const int node = base + i;
         if (node >= n - 1) break;
         if (s_cnt[i] != 2) {
@@ -1407,9 +1411,9 @@ int launch_lbvh_climb(Context* c, const ClimbArgs& a) {
     HX_TRY(check_launch("frontier_reset"));
     const unsigned blocks = (unsigned)((a.n + kClimbLeaves - 1) / kClimbLeaves);
     if (a.keys64)
-        hipLaunchKernelGGL(k_lbvh_local<true>, dim3(blocks), dim3(kClimbThreads), 0, c->stream, a);
+        hipLaunchKernelGGL(k_lbvh_local<true>, dim3(blocks), dim3(kLbvhThreads), 0, c->stream, a);
     else
-        hipLaunchKernelGGL(k_lbvh_local<false>, dim3(blocks), dim3(kClimbThreads), 0, c->stream, a);
+        hipLaunchKernelGGL(k_lbvh_local<false>, dim3(blocks), dim3(kLbvhThreads), 0, c->stream, a);
     HX_TRY(check_launch("lbvh_local"));
     const unsigned top = (unsigned)std::max(1, std::min<int>(c->cu_count * 4, (int)((a.n + kBlock - 1) / kBlock)));
     if (a.keys64)
