@@ -1187,13 +1187,26 @@ __global__ __launch_bounds__(kBlock) void k_emit(EmitArgs a, FlagCtx f) {
         bin_rows(cmn[0], cmx[0], cmn[1], cmx[1], r0, r1, r2);
     }
     int cidx[2];
+    // With PROTO the tree is split-numbered: node i's children cover
+    // [first, i] and [i+1, last], and a child is a final interior exactly when
+    // its int_slot is non-negative, so no child range or state is gathered.
+    int cslot[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) cslot[k] = PROTO && ch[k] < n - 1 ? a.int_slot[ch[k]] : -1;
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
         const int c = ch[k];
-        if (c < n - 1 && a.state[c] == 1) {
-            cidx[k] = ms.interior(a.int_slot[c]);
+        if (PROTO ? cslot[k] >= 0 : (c < n - 1 && a.state[c] == 1)) {
+            cidx[k] = ms.interior(PROTO ? cslot[k] : a.int_slot[c]);
         } else {
-            const int cf = c >= n - 1 ? c - (n - 1) : a.first[c], cl = c >= n - 1 ? cf : a.last[c];
+            int cf, cl;
+            if (PROTO) {
+                cf = k == 0 ? first : i + 1;
+                cl = k == 0 ? i : last;
+            } else {
+                cf = c >= n - 1 ? c - (n - 1) : a.first[c];
+                cl = c >= n - 1 ? cf : a.last[c];
+            }
             const int L = ms.leaf(a.leaf_num[cf]);
             cidx[k] = ~L;
             write_node(a.nodes, ms.tab.x + L, cmn[k], cmx[k], cmn[k], cmx[k], cf - ms.toff, cl + 1 - ms.toff,
