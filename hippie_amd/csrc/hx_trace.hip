@@ -36,23 +36,31 @@ HX_DEV float span_end(float a0, float a1, float b0, float b1, float c0, float c1
 // Traversal stack: entries [0, LDS_DEPTH) in LDS, one column per lane (entry e
 // of lane t at e * kTraceBlock + t, so a wave's accesses hit distinct banks);
 // deeper entries in private (scratch) memory, touched only by deep traversals.
+// The LDS column is typed as LDS (address space 3) so that the compiler keeps
+// LDS and private accesses apart: through a generic pointer it merges the two
+// into one flat load, which waits on both memory counters.
+typedef __attribute__((address_space(3))) int32_t lds_i32;
+
 template <int LDS_DEPTH>
 struct TraversalStack {
     static constexpr int kDeep = kStackSize - LDS_DEPTH > 0 ? kStackSize - LDS_DEPTH : 1;
-    int32_t* col;
+    lds_i32* col;
     int32_t deep[kDeep];
     HX_DEV void put(int e, int32_t v) {
         if (e < LDS_DEPTH) col[e * kTraceBlock] = v;
         else deep[e - LDS_DEPTH] = v;
     }
-    HX_DEV int32_t get(int e) const { return e < LDS_DEPTH ? col[e * kTraceBlock] : deep[e - LDS_DEPTH]; }
+    HX_DEV int32_t get(int e) const {
+        if (e < LDS_DEPTH) return col[e * kTraceBlock];
+        return deep[e - LDS_DEPTH];
+    }
 };
 
 template <bool STATS, bool ORDERED, int LDS_DEPTH>
 __global__ __launch_bounds__(kTraceBlock) void k_trace(TraceArgs a) {
     __shared__ int32_t s_stack[LDS_DEPTH > 0 ? LDS_DEPTH * kTraceBlock : 1];
     TraversalStack<LDS_DEPTH> stack;
-    stack.col = s_stack + threadIdx.x;
+    stack.col = (lds_i32*)(s_stack + threadIdx.x);
     int sp = 0;
     float ox = 0.f, oy = 0.f, oz = 0.f, tmin = 0.f, dx = 0.f, dy = 0.f, dz = 0.f;
     float idx = 0.f, idy = 0.f, idz = 0.f, oodx = 0.f, oody = 0.f, oodz = 0.f;
@@ -116,6 +124,9 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(TraceArgs a) {
                 const float4 n1xy = np[1];
                 const float4 nz = np[2];
                 const int4 cn = a.nodes[4 * (nodeBase + nodeAddr) + 3];
+                // The child links share the node's cache line: keep their load
+                // with the box loads instead of after the box test.
+                asm volatile("" ::"v"(cn.x), "v"(cn.y));
                 if (STATS) ++nNodes;
                 const float c0lox = fmaf(n0xy.x, idx, -oodx);
                 const float c0hix = fmaf(n0xy.y, idx, -oodx);
