@@ -377,12 +377,13 @@ int launch_relabel_by_height(Context* c, const AtrArgs& a, const PlocArgs& relab
                           c->stream));
     std::vector<int> offs(kBuckets + 1, 0);
     for (int b = 0; b < kBuckets; ++b) offs[b + 1] = offs[b] + counts[b];
-    // Both buckets of a height are contiguous: one launch per height.
+    // Both buckets of a height are contiguous: one step per height.
+    std::vector<int2> segs;
     for (int h = kAtrMaxLevels - 1; h >= 0; --h) {
         const int cnt = counts[2 * h] + counts[2 * h + 1];
-        if (cnt > 0) HX_TRY(launch_tree_down(c, relabel, a.list + offs[2 * h], cnt));
+        if (cnt > 0) segs.push_back(make_int2(offs[2 * h], cnt));
     }
-    return HX_OK;
+    return launch_tree_down_steps(c, relabel, a.list, segs);
 }
 
 int launch_atr(Context* c, const AtrArgs& a, const PlocArgs& relabel, int64_t* accepted_out) {

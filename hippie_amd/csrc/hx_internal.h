@@ -85,6 +85,8 @@ struct Context {
     DevBuf sort_status, sort_hist, scan_partials, control;
     DevBuf ray_keys[2], ray_order[2];  // sorted trace
     DevBuf tmp_keys, tmp_vals;         // hx_sort_pairs_u64 ping-pong
+    DevBuf segments;                   // relabelling: node ranges for the single-workgroup pass
+    std::vector<int2> segments_host;   // their host copy, alive until the next relabelling
 };
 
 // RAII-free launch scope: records events around one launch when profiling.
@@ -262,6 +264,10 @@ int launch_ploc(Context* c, const PlocArgs& a, int* rounds_out);
 // right, parent, first, last (sizes in, last ranks out), split, sorted_tri and
 // tri_out of a); the parents of every listed node must be placed already.
 int launch_tree_down(Context* c, const PlocArgs& a, const int32_t* list, int count);
+// The whole top-down relabelling: segs[k] = (start, count) of the nodes placed
+// in step k (ids start..start+count-1, or list[start..] if list is given),
+// parents' steps first.  Leading small steps run in one workgroup.
+int launch_tree_down_steps(Context* c, const PlocArgs& a, const int32_t* list, const std::vector<int2>& segs);
 
 // hx_atr.hip: ATRBVH treelet restructuring of a Karras tree prepared by the
 // climb (ClimbArgs::atr_*), then the depth-first relabelling.  n >= 2.

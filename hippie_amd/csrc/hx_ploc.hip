@@ -306,11 +306,7 @@ __global__ void k_ploc_round_start(int4* round, int m) {
 // list is NULL, of lo + [0, count)), all independent: every parent was placed
 // by an earlier launch.  last[v] holds v's subtree size on entry and its last
 // leaf rank on exit; first[v] was written by v's parent (the root starts at 0).
-__global__ __launch_bounds__(kBlock) void k_tree_down(PlocArgs a, const int32_t* __restrict__ list, int lo,
-                                                      int count) {
-    const int k = blockIdx.x * kBlock + threadIdx.x;
-    if (k >= count) return;
-    const int v = list ? list[k] : lo + k;
+HX_DEV void tree_down_node(const PlocArgs& a, int v) {
     const int n = (int)a.n;
     const int off = v == 0 ? 0 : a.first[v];
     const int size = a.last[v];
@@ -348,7 +344,52 @@ __global__ __launch_bounds__(kBlock) void k_tree_down(PlocArgs a, const int32_t*
     }
 }
 
+__global__ __launch_bounds__(kBlock) void k_tree_down(PlocArgs a, const int32_t* __restrict__ list, int lo,
+                                                      int count) {
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k >= count) return;
+    tree_down_node(a, list ? list[k] : lo + k);
+}
+
+// Several placement steps in one workgroup, a barrier between steps (the
+// barrier orders the global writes of a step before the next step's reads
+// within the workgroup).
+constexpr int kStepBlock = 1024;
+constexpr int kSmallStep = 4 * kStepBlock;  // steps up to this many nodes share the workgroup
+
+__global__ __launch_bounds__(kStepBlock) void k_tree_down_steps(PlocArgs a, const int32_t* __restrict__ list,
+                                                                const int2* __restrict__ segs, int nsegs) {
+    for (int k = 0; k < nsegs; ++k) {
+        const int2 sg = segs[k];
+        for (int j = threadIdx.x; j < sg.y; j += kStepBlock) tree_down_node(a, list ? list[sg.x + j] : sg.x + j);
+        __syncthreads();
+    }
+}
+
 }  // namespace
+
+int launch_tree_down_steps(Context* c, const PlocArgs& a, const int32_t* list, const std::vector<int2>& segs) {
+    size_t small = 0;
+    while (small < segs.size() && segs[small].y <= kSmallStep) ++small;
+    if (small > 1) {
+        HX_TRY(c->segments.reserve(sizeof(int2) * small));
+        c->segments_host.assign(segs.begin(), segs.begin() + small);
+        HX_HIP(hipMemcpyAsync(c->segments.ptr, c->segments_host.data(), sizeof(int2) * small, hipMemcpyHostToDevice,
+                              c->stream));
+        hipLaunchKernelGGL(k_tree_down_steps, dim3(1), dim3(kStepBlock), 0, c->stream, a, list,
+                           c->segments.as<int2>(), (int)small);
+        HX_TRY(check_launch("tree_down_steps"));
+    } else {
+        small = 0;
+    }
+    for (size_t k = small; k < segs.size(); ++k) {
+        if (segs[k].y <= 0) continue;
+        hipLaunchKernelGGL(k_tree_down, dim3(grid_for(segs[k].y)), dim3(kBlock), 0, c->stream, a,
+                           list ? list + segs[k].x : nullptr, list ? 0 : segs[k].x, segs[k].y);
+        HX_TRY(check_launch("tree_down"));
+    }
+    return HX_OK;
+}
 
 int launch_tree_down(Context* c, const PlocArgs& a, const int32_t* list, int count) {
     if (count <= 0) return HX_OK;
@@ -430,12 +471,9 @@ int launch_ploc(Context* c, const PlocArgs& a, int* rounds_out) {
     }
     {
         LaunchScope ls(c, "ploc_relabel");
-        for (size_t k = ranges.size(); k-- > 0;) {
-            const int lo = ranges[k].x, hi = ranges[k].y;
-            hipLaunchKernelGGL(k_tree_down, dim3(grid_for(hi - lo + 1)), dim3(kBlock), 0, c->stream, a,
-                               (const int32_t*)nullptr, lo, hi - lo + 1);
-            HX_TRY(check_launch("tree_down"));
-        }
+        std::vector<int2> segs;  // rounds in reverse order: parents first
+        for (size_t k = ranges.size(); k-- > 0;) segs.push_back(make_int2(ranges[k].x, ranges[k].y - ranges[k].x + 1));
+        HX_TRY(launch_tree_down_steps(c, a, nullptr, segs));
     }
     if (rounds_out) *rounds_out = (int)ranges.size();
     return HX_OK;
