@@ -95,6 +95,29 @@ HX_DEV Box3 cluster_box(float4 a, float2 b) {
     return r;
 }
 
+// Nearest cluster of i within [i - r, i + r] (same mesh if cmesh), the
+// lowest index on ties, from global memory.
+HX_DEV int nearest_cluster(const float4* __restrict__ ca, const float2* __restrict__ cb, int m, int r, int i,
+                           const int32_t* __restrict__ cmesh) {
+    const Box3 own = cluster_box(ca[i], cb[i]);
+    int best = -1;
+    float bestD = kMaxFloat;
+    const int jlo = max(0, i - r), jhi = min(m - 1, i + r);
+    const int mine = cmesh ? cmesh[i] : 0;
+    for (int j = jlo; j <= jhi; ++j) {
+        if (j == i) continue;
+        if (cmesh && cmesh[j] != mine) continue;
+        const float4 x = ca[j];
+        const float2 y = cb[j];
+        const float d = union_area(x.x, x.y, x.z, x.w, y.x, y.y, own);
+        if (bestD > d) {
+            bestD = d;
+            best = j;
+        }
+    }
+    return best;
+}
+
 // LDS window: boxes of clusters [lo, lo + W) as six rows of stride `stride`
 // (a seventh row of mesh ids in batched builds).  With cmesh, a cluster only
 // looks at clusters of its own mesh: the meshes' cluster runs are contiguous,
@@ -158,23 +181,7 @@ __global__ __launch_bounds__(kBlock) void k_ploc_neighbours(const float4* __rest
         nb[i] = best;
     } else {
         if (i >= m) return;
-        const Box3 own = cluster_box(ca[i], cb[i]);
-        int best = -1;
-        float bestD = kMaxFloat;
-        const int jlo = max(0, i - r), jhi = min(m - 1, i + r);
-        const int mine = cmesh ? cmesh[i] : 0;
-        for (int j = jlo; j <= jhi; ++j) {
-            if (j == i) continue;
-            if (cmesh && cmesh[j] != mine) continue;
-            const float4 x = ca[j];
-            const float2 y = cb[j];
-            const float d = union_area(x.x, x.y, x.z, x.w, y.x, y.y, own);
-            if (bestD > d) {
-                bestD = d;
-                best = j;
-            }
-        }
-        nb[i] = best;
+        nb[i] = nearest_cluster(ca, cb, m, r, i, cmesh);
     }
 }
 
@@ -242,6 +249,44 @@ __global__ __launch_bounds__(kBlock) void k_ploc_scan(int2* __restrict__ partial
     }
 }
 
+// Cluster i of a round with flags f (merging, surviving) and ranks among the
+// merging and surviving clusters: a merging cluster creates node
+// node_offset - mrank from itself and its neighbour; survivors move to slot
+// srank of the other cluster buffer.
+HX_DEV void merge_cluster(const PlocArgs& a, int src, int i, int2 f, int mrank, int srank, int node_offset) {
+    const int n = (int)a.n;
+    const int32_t* __restrict__ cl = a.cl[src];
+    const float4* __restrict__ ca = a.ca[src];
+    const float2* __restrict__ cb = a.cb[src];
+    int32_t* __restrict__ cl_out = a.cl[src ^ 1];
+    float4* __restrict__ ca_out = a.ca[src ^ 1];
+    float2* __restrict__ cb_out = a.cb[src ^ 1];
+    if (f.x) {
+        const int j = a.nb[i];
+        const int ln = cl[i], rn = cl[j];
+        const int node = node_offset - mrank;
+        const Box3 lb = cluster_box(ca[i], cb[i]);
+        const Box3 rb = cluster_box(ca[j], cb[j]);
+        a.parent[ln] = node;
+        a.parent[rn] = node;
+        a.left[node] = ln;
+        a.right[node] = rn;
+        const int sl = ln >= n - 1 ? 1 : a.last[ln];
+        const int sr = rn >= n - 1 ? 1 : a.last[rn];
+        a.last[node] = sl + sr;
+        cl_out[srank] = node;
+        if (a.cm[0]) a.cm[src ^ 1][srank] = a.cm[src][i];
+        ca_out[srank] = make_float4(mins(lb.mn.x, rb.mn.x), mins(lb.mn.y, rb.mn.y), mins(lb.mn.z, rb.mn.z),
+                                    maxs(lb.mx.x, rb.mx.x));
+        cb_out[srank] = make_float2(maxs(lb.mx.y, rb.mx.y), maxs(lb.mx.z, rb.mx.z));
+    } else if (f.y) {
+        cl_out[srank] = cl[i];
+        if (a.cm[0]) a.cm[src ^ 1][srank] = a.cm[src][i];
+        ca_out[srank] = ca[i];
+        cb_out[srank] = cb[i];
+    }
+}
+
 // merge (PLOCBuilderKernels.cu:207-286) + compact (:288-389) in one pass.
 // The new node's box is the select-based union of the two cluster boxes
 // (the reference uses fminf/fmaxf, which differ from selects only in the sign
@@ -252,13 +297,6 @@ __global__ __launch_bounds__(kBlock) void k_ploc_merge(PlocArgs a, int src, cons
     const int m = round->x;
     if (m <= 1 || blockIdx.x * (int64_t)kTile >= m) return;
     const int node_offset = m - 2;
-    const int n = (int)a.n;
-    const int32_t* __restrict__ cl = a.cl[src];
-    const float4* __restrict__ ca = a.ca[src];
-    const float2* __restrict__ cb = a.cb[src];
-    int32_t* __restrict__ cl_out = a.cl[src ^ 1];
-    float4* __restrict__ ca_out = a.ca[src ^ 1];
-    float2* __restrict__ cb_out = a.cb[src ^ 1];
     const int64_t base = blockIdx.x * (int64_t)kTile;
     int2 carry = partial[blockIdx.x];
     for (int k = 0; k < kItems; ++k) {
@@ -266,40 +304,61 @@ __global__ __launch_bounds__(kBlock) void k_ploc_merge(PlocArgs a, int src, cons
         const int2 f = ploc_flags(a.nb, m, i);
         int2 tot;
         const int2 inc = block_inclusive_scan<kWavesPerBlock>(f, Sum2(), wt, tot);
-        if (i < m) {
-            const int mrank = carry.x + inc.x - f.x;
-            const int srank = carry.y + inc.y - f.y;
-            if (f.x) {
-                const int j = a.nb[i];
-                const int ln = cl[i], rn = cl[j];
-                const int node = node_offset - mrank;
-                const Box3 lb = cluster_box(ca[i], cb[i]);
-                const Box3 rb = cluster_box(ca[j], cb[j]);
-                a.parent[ln] = node;
-                a.parent[rn] = node;
-                a.left[node] = ln;
-                a.right[node] = rn;
-                const int sl = ln >= n - 1 ? 1 : a.last[ln];
-                const int sr = rn >= n - 1 ? 1 : a.last[rn];
-                a.last[node] = sl + sr;
-                cl_out[srank] = node;
-                if (a.cm[0]) a.cm[src ^ 1][srank] = a.cm[src][i];
-                ca_out[srank] = make_float4(mins(lb.mn.x, rb.mn.x), mins(lb.mn.y, rb.mn.y), mins(lb.mn.z, rb.mn.z),
-                                            maxs(lb.mx.x, rb.mx.x));
-                cb_out[srank] = make_float2(maxs(lb.mx.y, rb.mx.y), maxs(lb.mx.z, rb.mx.z));
-            } else if (f.y) {
-                cl_out[srank] = cl[i];
-                if (a.cm[0]) a.cm[src ^ 1][srank] = a.cm[src][i];
-                ca_out[srank] = ca[i];
-                cb_out[srank] = cb[i];
-            }
-        }
+        if (i < m) merge_cluster(a, src, (int)i, f, carry.x + inc.x - f.x, carry.y + inc.y - f.y, node_offset);
         carry = make_int2(carry.x + tot.x, carry.y + tot.y);
     }
 }
 
 __global__ void k_ploc_round_start(int4* round, int m) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *round = make_int4(m, 0, 0, 0);
+}
+
+// The last rounds (at most kTailClusters clusters) in one workgroup: the same
+// neighbour search, flags, ranks and merge as the grid-wide kernels, with a
+// barrier between the phases instead of a launch each.  Each thread ranks a
+// contiguous chunk of clusters.  rec[k] = (clusters, merged, surviving) of
+// tail round k; *nrec = rounds run (a round that merges nothing ends it).
+constexpr int kTailBlock = 1024;
+constexpr int kTailClusters = 4096;
+
+__global__ __launch_bounds__(kTailBlock) void k_ploc_tail(PlocArgs a, int src, int m, int r, int4* __restrict__ rec,
+                                                          int* __restrict__ nrec) {
+    __shared__ int2 wt[kTailBlock / kWave];
+    const bool batched = a.cm[0] != nullptr && a.n_meshes > 1;
+    int k = 0;
+    while (m > 1) {
+        if (a.adaptive) r = min(r + 1, a.max_radius);
+        const bool pairs = batched && m <= a.n_meshes;
+        const int32_t* cmesh = batched && !pairs ? a.cm[src] : nullptr;
+        for (int i = threadIdx.x; i < m; i += kTailBlock)
+            a.nb[i] = pairs ? ((i ^ 1) < m ? (i ^ 1) : -1) : nearest_cluster(a.ca[src], a.cb[src], m, r, i, cmesh);
+        __syncthreads();
+        const int chunk = (m + kTailBlock - 1) / kTailBlock;
+        const int lo = threadIdx.x * chunk, hi = min(m, lo + chunk);
+        int2 cnt = make_int2(0, 0);
+        for (int i = lo; i < hi; ++i) {
+            const int2 f = ploc_flags(a.nb, m, i);
+            cnt = make_int2(cnt.x + f.x, cnt.y + f.y);
+        }
+        int2 tot;
+        const int2 inc = block_inclusive_scan<kTailBlock / kWave>(cnt, Sum2(), wt, tot);
+        int mrank = inc.x - cnt.x, srank = inc.y - cnt.y;
+        if (tot.x > 0) {
+            for (int i = lo; i < hi; ++i) {
+                const int2 f = ploc_flags(a.nb, m, i);
+                merge_cluster(a, src, i, f, mrank, srank, m - 2);
+                mrank += f.x;
+                srank += f.y;
+            }
+        }
+        if (threadIdx.x == 0) rec[k] = make_int4(m, tot.x, tot.y, 0);
+        ++k;
+        __syncthreads();
+        if (tot.x == 0) break;
+        m = tot.y;
+        src ^= 1;
+    }
+    if (threadIdx.x == 0) *nrec = k;
 }
 
 // Depth-first placement of the children of the nodes list[0, count) (or, if
@@ -405,9 +464,12 @@ int launch_ploc(Context* c, const PlocArgs& a, int* rounds_out) {
         HX_TRY(check_launch("ploc_leaves"));
     }
     const int64_t max_tiles = (n + kTile - 1) / kTile;
-    HX_TRY(c->scan_partials.reserve(sizeof(int2) * (size_t)max_tiles + sizeof(int4) * (kPlocBatch + 1) + 64));
+    HX_TRY(c->scan_partials.reserve(sizeof(int2) * (size_t)max_tiles + sizeof(int4) * (kPlocBatch + kTailClusters + 2) +
+                                    64));
     int2* partial = c->scan_partials.as<int2>();
     int4* rounds = reinterpret_cast<int4*>(partial + ((max_tiles + 1) & ~(int64_t)1));
+    int4* tail_rec = rounds + kPlocBatch + 1;
+    int* tail_count = reinterpret_cast<int*>(tail_rec + kTailClusters);
     std::vector<int2> ranges;   // [lo, hi] of the nodes each round created
     // Batched builds merge inside each mesh until every mesh is one cluster
     // (m == n_meshes), then join the mesh roots pairwise in mesh order
@@ -422,6 +484,29 @@ int launch_ploc(Context* c, const PlocArgs& a, int* rounds_out) {
     hipLaunchKernelGGL(k_ploc_round_start, dim3(1), dim3(64), 0, c->stream, rounds, m);
     HX_TRY(check_launch("ploc_round_start"));
     while (m > 1) {
+        if (m <= kTailClusters) {
+            LaunchScope ls(c, "ploc_tail");
+            hipLaunchKernelGGL(k_ploc_tail, dim3(1), dim3(kTailBlock), 0, c->stream, a, src, m, r, tail_rec,
+                               tail_count);
+            HX_TRY(check_launch("ploc_tail"));
+            int cnt = 0;
+            HX_HIP(hipMemcpyAsync(&cnt, tail_count, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+            HX_HIP(hipStreamSynchronize(c->stream));
+            std::vector<int4> rec((size_t)std::max(cnt, 0));
+            if (cnt > 0) {
+                HX_HIP(hipMemcpyAsync(rec.data(), tail_rec, sizeof(int4) * rec.size(), hipMemcpyDeviceToHost,
+                                      c->stream));
+                HX_HIP(hipStreamSynchronize(c->stream));
+            }
+            for (const int4& t : rec) {
+                if (t.y <= 0 || t.y + t.z != t.x) {
+                    set_error("PLOC round with %d clusters merged %d pairs (non-finite boxes?)", t.x, t.y);
+                    return HX_ERR_INVALID_ARG;
+                }
+                ranges.push_back(make_int2(t.x - 2 - t.y + 1, t.x - 2));
+            }
+            break;
+        }
         const int ntiles = (m + kTile - 1) / kTile;
         for (int b = 0; b < kPlocBatch; ++b) {
             if (a.adaptive) r = std::min(r + 1, a.max_radius);
