@@ -300,7 +300,7 @@ HX_DEV void woop_rows(float3 v0, float3 v1, float3 v2, float4& w0, float4& w1, f
 //     writeback or invalidate is needed.
 // ---------------------------------------------------------------------------
 
-constexpr int kClimbThreads = 256;
+constexpr int kClimbThreads = 512;  // one leaf per thread
 constexpr int kClimbLeaves = 512;
 
 struct Combined {
