@@ -20,10 +20,11 @@ constexpr int kTraceBlock = 256;
 constexpr int kStackSize = 64;                 // HipTracerBinKernels.cu:5
 constexpr int32_t kSentinel = 0x76543210;      // HipTracerKernels.h:11
 
-// fminf/fmaxf written as selects, and the integer min/max on float bits of
-// HipTracerKernels.h:144-195 (spanBeginKepler / spanEndKepler).
-HX_DEV float fmin_s(float a, float b) { return (b < a) ? b : a; }
-HX_DEV float fmax_s(float a, float b) { return (b > a) ? b : a; }
+// fminf/fmaxf (v_min_f32 / v_max_f32: a NaN operand yields the other, -0
+// orders below +0, as the reference's fminf/fmaxf do) and the integer min/max
+// on float bits of HipTracerKernels.h:144-195 (spanBeginKepler / spanEndKepler).
+HX_DEV float fmin_s(float a, float b) { return fminf(a, b); }
+HX_DEV float fmax_s(float a, float b) { return fmaxf(a, b); }
 HX_DEV float span_begin(float a0, float a1, float b0, float b1, float c0, float c1, float d) {
     const int c = max(min(__float_as_int(c0), __float_as_int(c1)), __float_as_int(d));
     return __int_as_float(max(max(__float_as_int(fmin_s(a0, a1)), __float_as_int(fmin_s(b0, b1))), c));

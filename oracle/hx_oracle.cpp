@@ -194,10 +194,21 @@ void set_child_box(int32_t* node, int i, const V3& mn, const V3& mx) {
     node[2 * i + 9] = f2w(mx.z);
 }
 
-/* Trace helpers: rt/tracer/HipTracerKernels.h:124-195 with the float
- * fminf/fmaxf written as selects and the integer min/max on float bits. */
-inline float fmin_s(float a, float b) { return (b < a) ? b : a; }
-inline float fmax_s(float a, float b) { return (b > a) ? b : a; }
+/* Trace helpers: rt/tracer/HipTracerKernels.h:124-195 -- fminf/fmaxf with
+ * the semantics of the GPU min/max instructions (a NaN operand yields the
+ * other operand; -0 orders below +0), then the integer min/max on float bits. */
+inline float fmin_s(float a, float b) {
+    if (a != a) return b;
+    if (b != b) return a;
+    if (a == b) return std::signbit(a) ? a : b;
+    return a < b ? a : b;
+}
+inline float fmax_s(float a, float b) {
+    if (a != a) return b;
+    if (b != b) return a;
+    if (a == b) return std::signbit(a) ? b : a;
+    return a > b ? a : b;
+}
 inline int32_t fb(float f) { return f2w(f); }
 inline float bf(int32_t i) { return w2f(i); }
 inline float i_min_max(float a, float b, float c) {  /* max(min(a,b),c) on bits */
