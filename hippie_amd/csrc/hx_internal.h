@@ -260,13 +260,11 @@ struct PlocArgs {
     int radius, max_radius, adaptive;
 };
 int launch_ploc(Context* c, const PlocArgs& a, int* rounds_out);
-// Depth-first placement of the children of list[0, count) (uses n, left,
-// right, parent, first, last (sizes in, last ranks out), split, sorted_tri and
-// tri_out of a); the parents of every listed node must be placed already.
-int launch_tree_down(Context* c, const PlocArgs& a, const int32_t* list, int count);
-// The whole top-down relabelling: segs[k] = (start, count) of the nodes placed
-// in step k (ids start..start+count-1, or list[start..] if list is given),
-// parents' steps first.  Leading small steps run in one workgroup.
+// The whole top-down (depth-first) relabelling: segs[k] = (start, count) of
+// the nodes whose children step k places (ids start..start+count-1, or
+// list[start..] if list is given), parents' steps first; uses n, left, right,
+// parent, first, last (sizes in, last ranks out), split, sorted_tri and tri_out
+// of a.  Leading small steps run in one workgroup.
 int launch_tree_down_steps(Context* c, const PlocArgs& a, const int32_t* list, const std::vector<int2>& segs);
 
 // hx_atr.hip: ATRBVH treelet restructuring of a Karras tree prepared by the

@@ -450,12 +450,6 @@ int launch_tree_down_steps(Context* c, const PlocArgs& a, const int32_t* list, c
     return HX_OK;
 }
 
-int launch_tree_down(Context* c, const PlocArgs& a, const int32_t* list, int count) {
-    if (count <= 0) return HX_OK;
-    hipLaunchKernelGGL(k_tree_down, dim3(grid_for(count)), dim3(kBlock), 0, c->stream, a, list, 0, count);
-    return check_launch("tree_down");
-}
-
 int launch_ploc(Context* c, const PlocArgs& a, int* rounds_out) {
     const int n = (int)a.n;
     {
