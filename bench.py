@@ -76,6 +76,7 @@ def parse():
     ap.add_argument("--batch-rays-per-1k", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--no-autotune", action="store_true", help="keep libhx's default launch shapes")
+    ap.add_argument("--no-diff-single", action="store_true", help="skip the one-mesh differentiable step")
     ap.add_argument("--tune", action="append", default=[], metavar="NAME=VALUE",
                     help="libhx launch tunable (hx_context_set_tuning), e.g. trace_lds_stack=16")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -232,6 +233,56 @@ def batch_workload(args, device):
     roffs = np.concatenate([[0], np.cumsum(rc)])
     return dict(V=V, T=T, offs=np.array(offs, np.int64), vbase=np.array(vbase, np.int64), counts=counts,
                 tris_local=tris_local, rays=rays, owner=owner, roffs=roffs)
+
+
+def bench_diff_single(ctx, args):
+    """Differentiable refit -> trace -> loss -> adjoint step on one mesh: a
+    displaced level-8 icosphere (1,310,720 triangles) and 16 orbit cameras of
+    512x512 rays (4,194,304 rays), loss sum (t - t_target)^2 over rays that hit
+    in both, t_target from a second displacement seed; beside it the plain
+    refit + trace of the same step."""
+    import numpy as np
+    import torch
+    import hippie_amd as hx
+    from hippie_amd import scenes
+    from hippie_amd.diff import DifferentiableTracer
+    dev = torch.device("cuda", 0)
+    t0 = time.perf_counter()
+    v0, f = scenes.displaced_icosphere(8, seed=1)
+    v1, _ = scenes.displaced_icosphere(8, seed=2)
+    rays = torch.as_tensor(scenes.orbit_rays(16, 512), device=dev)
+    tris = torch.as_tensor(f, device=dev)
+    verts0 = torch.as_tensor(v0, device=dev)
+    verts1 = torch.as_tensor(v1, device=dev)
+    log(f"diff workload: {f.shape[0]} triangles, {rays.shape[0]} rays ({time.perf_counter() - t0:.1f} s)")
+    with torch.no_grad():
+        t_tgt, _, _, ids_t = DifferentiableTracer(verts1, tris, ctx)(verts1, rays)
+    dt = DifferentiableTracer(verts0, tris, ctx)
+    vp = verts0.clone().requires_grad_()
+    zero = torch.zeros((), device=dev)
+
+    def step():
+        t, _, _, ids = dt(vp, rays)
+        loss = torch.where((ids >= 0) & (ids_t >= 0), (t - t_tgt) ** 2, zero).sum()
+        vp.grad = None
+        loss.backward()
+
+    step_ms = timed(step, args.batch_steps, 1)
+    tracer = dt.tracer()
+    rb = hx.RayBuffer(rays)
+    scene = hx.Scene(verts0, tris)
+
+    def plain():
+        dt.bvh.update(scene)
+        tracer.trace(rb)
+
+    plain_ms = timed(plain, args.batch_steps, 1)
+    n_rays = int(rays.shape[0])
+    log(f"diff step (one mesh) {step_ms:.3f} ms ({n_rays / (step_ms * 1e-3) / 1e6:.1f} Mrays/s), "
+        f"plain refit + trace {plain_ms:.3f} ms")
+    return {"workload": f"displaced icosphere level 8 ({f.shape[0]} triangles), 16 cameras x 512^2 rays",
+            "triangles": int(f.shape[0]), "rays": n_rays, "step_ms": round(step_ms, 3),
+            "step_mrays_s": round(n_rays / (step_ms * 1e-3) / 1e6, 1), "plain_refit_trace_ms": round(plain_ms, 3)}
 
 
 def bench_batch(ctx, args):
@@ -481,6 +532,7 @@ def main():
     if builders:
         builders["lbvh"] = {"ms": round(build_ms, 3), "sah": round(bvh.get_cost(), 4)}
     batched = bench_batch(ctx, args) if args.batch_meshes > 0 else None
+    diff_single = bench_diff_single(ctx, args) if not args.no_diff_single else None
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -532,6 +584,7 @@ def main():
         "tuning": tuning,
         "builders": builders,
         "batched": batched,
+        "diff_single": diff_single,
     }
     print(json.dumps(out), flush=True)
 
