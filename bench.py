@@ -77,6 +77,8 @@ def parse():
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--no-autotune", action="store_true", help="keep libhx's default launch shapes")
     ap.add_argument("--no-diff-single", action="store_true", help="skip the one-mesh differentiable step")
+    ap.add_argument("--c5-meshes", type=int, default=512, help="meshes of the presplit PLOC differentiable cycle (0 = skip)")
+    ap.add_argument("--c5-rays", type=int, default=64 * 1024 * 1024, help="rays per step of that cycle")
     ap.add_argument("--tune", action="append", default=[], metavar="NAME=VALUE",
                     help="libhx launch tunable (hx_context_set_tuning), e.g. trace_lds_stack=16")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -283,6 +285,130 @@ def bench_diff_single(ctx, args):
     return {"workload": f"displaced icosphere level 8 ({f.shape[0]} triangles), 16 cameras x 512^2 rays",
             "triangles": int(f.shape[0]), "rays": n_rays, "step_ms": round(step_ms, 3),
             "step_mrays_s": round(n_rays / (step_ms * 1e-3) / 1e6, 1), "plain_refit_trace_ms": round(plain_ms, 3)}
+
+
+def bench_config5(ctx, args):
+    """Batched differentiable cycle at scale: 512 meshes of log-uniform 1k..200k
+    triangles, 30% of them long thin slivers (aspect up to 1:200), presplit
+    (beta 0.3) PLOC (r = 25) rebuild once per 10 steps, and per step refit ->
+    trace -> loss -> adjoint (hipGraph-captured) of 64M rays spread over the
+    meshes in proportion to their triangles; against the per-mesh loop of the
+    same cycle.  The vertices are refit in place each step (same work as a
+    deformation)."""
+    import math
+    import numpy as np
+    import torch
+    import hippie_amd as hx
+    from hippie_amd import scenes
+    from hippie_amd.diff import backward as single_backward
+    dev = torch.device("cuda", 0)
+    t0 = time.perf_counter()
+    rng = np.random.default_rng(5)
+    M, R = args.c5_meshes, args.c5_rays
+    counts = np.rint(np.exp(rng.uniform(math.log(1000), math.log(200000), M))).astype(np.int64)
+    meshes = []
+    for c in counts:
+        if rng.random() < 0.3:
+            meshes.append(scenes.sliver_soup(int(c), rng))
+        else:
+            meshes.append(scenes.batch_mesh(int(c), rng, bool(rng.random() < 0.5)))
+    mb = hx.MeshBatch.from_meshes(meshes)
+    N = int(counts.sum())
+    per = np.maximum(1, np.rint(counts * (R / N))).astype(np.int64)
+    R = int(per.sum())
+    owner = torch.repeat_interleave(torch.arange(M, device=dev, dtype=torch.int32),
+                                    torch.as_tensor(per, device=dev))
+    g = torch.Generator(device=dev)
+    g.manual_seed(5)
+    d = torch.randn((R, 3), generator=g, device=dev)
+    o = 0.5 + 1.5 * d / d.norm(dim=1, keepdim=True)
+    tgt = torch.rand((R, 3), generator=g, device=dev)
+    dirn = tgt - o
+    dirn = dirn / dirn.norm(dim=1, keepdim=True)
+    rays = torch.zeros((R, 8), dtype=torch.float32, device=dev)
+    rays[:, 0:3] = o
+    rays[:, 4:7] = dirn
+    rays[:, 7] = 4.0
+    torch.cuda.synchronize()
+    log(f"config 5 workload: {M} meshes, {N} triangles, {R} rays ({time.perf_counter() - t0:.1f} s)")
+    params = dict(method="ploc", ploc_radius=25, presplitting=1, presplitter_beta=0.3)
+    hb = hx.HipBatch(ctx, mb, **params)
+    steps = args.batch_steps
+    rebuild_ms = timed(hb.rebuild, steps, 1)
+    res = hb.trace(rays, owner)
+    t_ref = res[:, 1].view(torch.float32).clone() * 0.999
+    grad = torch.zeros_like(mb.scene.vertices)
+    dl_dt = torch.zeros(R, dtype=torch.float32, device=dev)
+
+    def batched_step():
+        hb.refit()
+        hb.trace(rays, owner, results=res)
+        hit = res[:, 0] >= 0
+        t = res[:, 1].view(torch.float32)
+        torch.where(hit, 2.0 * (t - t_ref), torch.zeros_like(t), out=dl_dt)
+        grad.zero_()
+        hb.backward(rays, owner, res, dl_dt=dl_dt, out=grad)
+
+    e_ms = timed(batched_step, steps, 1)
+    graph_ms = None
+    try:
+        st = torch.cuda.Stream()
+        st.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(st):
+            ctx.set_stream(st)
+            batched_step()
+        torch.cuda.current_stream().wait_stream(st)
+        torch.cuda.synchronize()
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            ctx.set_stream(torch.cuda.current_stream())
+            batched_step()
+        ctx.set_stream(torch.cuda.current_stream())
+        graph_ms = timed(graph.replay, steps, 1)
+    except Exception as e:  # capture is optional; the eager step is the measured fallback
+        log(f"graph capture failed: {e}")
+        ctx.set_stream(torch.cuda.current_stream())
+    step_ms = graph_ms if graph_ms is not None else e_ms
+    cycle_ms = rebuild_ms + 10 * step_ms
+
+    # Per-mesh loop of the same cycle.
+    builder = hx.PLOCBuilder(ctx, **{k: v for k, v in params.items() if k != "method"})
+    scenes_m = [hx.Scene(v, t) for v, t in meshes]
+    bvhs = [builder.build(sc) for sc in scenes_m]
+    roffs = np.concatenate([[0], np.cumsum(per)])
+    res_m = [res[int(roffs[m]):int(roffs[m + 1])] for m in range(M)]
+    rays_m = [rays[int(roffs[m]):int(roffs[m + 1])] for m in range(M)]
+    dl_m = [dl_dt[int(roffs[m]):int(roffs[m + 1])] for m in range(M)]
+    grads_m = [torch.zeros_like(sc.vertices) for sc in scenes_m]
+    tracer = hx.HipTracer(ctx)
+
+    def loop_rebuild():
+        for sc, bv in zip(scenes_m, bvhs):
+            builder.rebuild(bv, sc)
+
+    def loop_step():
+        for m in range(M):
+            bv, sc = bvhs[m], scenes_m[m]
+            bv.update(sc)
+            tracer.set_bvh(bv)
+            rb = hx.RayBuffer.__new__(hx.RayBuffer)
+            rb.rays, rb.results, rb.closest_hit = rays_m[m], res_m[m], True
+            tracer.trace(rb)
+            single_backward(ctx, sc.vertices, sc.triangles, rays_m[m], res_m[m], dl_dt=dl_m[m], out=grads_m[m])
+
+    l_rebuild_ms = timed(loop_rebuild, 1, 0)
+    l_step_ms = timed(loop_step, 1, 1)
+    l_cycle_ms = l_rebuild_ms + 10 * l_step_ms
+    log(f"config 5: batched rebuild {rebuild_ms:.2f} ms, step {e_ms:.2f} ms (graph {graph_ms}), cycle "
+        f"{cycle_ms:.1f} ms; per-mesh loop rebuild {l_rebuild_ms:.1f} ms, step {l_step_ms:.1f} ms, cycle "
+        f"{l_cycle_ms:.1f} ms ({l_cycle_ms / cycle_ms:.1f}x)")
+    return {"workload": f"{M} meshes, {N} triangles (log-uniform 1000..200000, 30% sliver meshes), {R} rays; "
+                        "presplit (0.3) PLOC r=25 rebuild + 10 refit/trace/adjoint steps",
+            "meshes": M, "triangles": N, "references": int(hb.info("references")),
+            "rays": R, "rebuild_ms": round(rebuild_ms, 3), "step_ms": round(e_ms, 3),
+            "step_graph_ms": None if graph_ms is None else round(graph_ms, 3), "cycle_ms": round(cycle_ms, 3),
+            "loop_rebuild_ms": round(l_rebuild_ms, 3), "loop_step_ms": round(l_step_ms, 3),
+            "loop_cycle_ms": round(l_cycle_ms, 3), "cycle_speedup": round(l_cycle_ms / cycle_ms, 2)}
 
 
 def bench_batch(ctx, args):
@@ -533,6 +659,7 @@ def main():
         builders["lbvh"] = {"ms": round(build_ms, 3), "sah": round(bvh.get_cost(), 4)}
     batched = bench_batch(ctx, args) if args.batch_meshes > 0 else None
     diff_single = bench_diff_single(ctx, args) if not args.no_diff_single else None
+    config5 = bench_config5(ctx, args) if args.c5_meshes > 0 else None
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -585,6 +712,7 @@ def main():
         "builders": builders,
         "batched": batched,
         "diff_single": diff_single,
+        "batched_diff_presplit_ploc": config5,
     }
     print(json.dumps(out), flush=True)
 

@@ -533,6 +533,12 @@ class HipBatch:
         s = self.batch.scene
         call("hx_batch_rebuild", self.ctx.handle, self.handle, _ptr(s.vertices), _ptr(s.triangles))
 
+    def info(self, key):
+        """Build statistic of the whole batch (hx_bvh_info keys, e.g. "references")."""
+        v = ctypes.c_int64()
+        call("hx_bvh_info", self.handle, key.encode(), ctypes.byref(v))
+        return v.value
+
     def refit(self, vertices=None):
         s = self.batch.scene
         if vertices is not None:
