@@ -314,24 +314,60 @@ __global__ void k_ploc_round_start(int4* round, int m) {
 }
 
 // The last rounds (at most kTailClusters clusters) in one workgroup: the same
-// neighbour search, flags, ranks and merge as the grid-wide kernels, with a
-// barrier between the phases instead of a launch each.  Each thread ranks a
+// neighbour search (over boxes staged in LDS), flags, ranks and merge as the
+// grid-wide kernels, with a barrier between the phases instead of a launch
+// each.  Each thread ranks a
 // contiguous chunk of clusters.  rec[k] = (clusters, merged, surviving) of
 // tail round k; *nrec = rounds run (a round that merges nothing ends it).
 constexpr int kTailBlock = 1024;
-constexpr int kTailClusters = 4096;
+constexpr int kTailClusters = 2048;
 
 __global__ __launch_bounds__(kTailBlock) void k_ploc_tail(PlocArgs a, int src, int m, int r, int4* __restrict__ rec,
                                                           int* __restrict__ nrec) {
     __shared__ int2 wt[kTailBlock / kWave];
+    // The round's cluster boxes (six rows) and mesh ids, staged once per round.
+    __shared__ float s_box[6][kTailClusters];
+    __shared__ int s_mesh[kTailClusters];
     const bool batched = a.cm[0] != nullptr && a.n_meshes > 1;
     int k = 0;
     while (m > 1) {
         if (a.adaptive) r = min(r + 1, a.max_radius);
         const bool pairs = batched && m <= a.n_meshes;
-        const int32_t* cmesh = batched && !pairs ? a.cm[src] : nullptr;
-        for (int i = threadIdx.x; i < m; i += kTailBlock)
-            a.nb[i] = pairs ? ((i ^ 1) < m ? (i ^ 1) : -1) : nearest_cluster(a.ca[src], a.cb[src], m, r, i, cmesh);
+        const bool by_mesh = batched && !pairs;
+        for (int i = threadIdx.x; i < m; i += kTailBlock) {
+            const float4 x = a.ca[src][i];
+            const float2 y = a.cb[src][i];
+            s_box[0][i] = x.x;
+            s_box[1][i] = x.y;
+            s_box[2][i] = x.z;
+            s_box[3][i] = x.w;
+            s_box[4][i] = y.x;
+            s_box[5][i] = y.y;
+            if (by_mesh) s_mesh[i] = a.cm[src][i];
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < m; i += kTailBlock) {
+            if (pairs) {
+                a.nb[i] = (i ^ 1) < m ? (i ^ 1) : -1;
+                continue;
+            }
+            Box3 own;
+            own.mn = make_float3(s_box[0][i], s_box[1][i], s_box[2][i]);
+            own.mx = make_float3(s_box[3][i], s_box[4][i], s_box[5][i]);
+            int best = -1;
+            float bestD = kMaxFloat;
+            const int jlo = max(0, i - r), jhi = min(m - 1, i + r);
+            for (int j = jlo; j <= jhi; ++j) {
+                if (j == i || (by_mesh && s_mesh[j] != s_mesh[i])) continue;
+                const float d = union_area(s_box[0][j], s_box[1][j], s_box[2][j], s_box[3][j], s_box[4][j],
+                                           s_box[5][j], own);
+                if (bestD > d) {
+                    bestD = d;
+                    best = j;
+                }
+            }
+            a.nb[i] = best;
+        }
         __syncthreads();
         const int chunk = (m + kTailBlock - 1) / kTailBlock;
         const int lo = threadIdx.x * chunk, hi = min(m, lo + chunk);
