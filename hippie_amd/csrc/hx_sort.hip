@@ -346,6 +346,14 @@ __global__ __launch_bounds__(kSortBlock) void k_sort_local(const uint64_t* __res
     const int64_t base = blockIdx.x * (int64_t)kLocalTile;
     const int64_t lo = base - limit;
     const int span = kLocalTile + 2 * limit;
+    // The values of this thread's keys are loaded with the staging so their
+    // latency is not paid after each rank.
+    int32_t val[kLocalTile / kSortBlock];
+#pragma unroll
+    for (int u = 0; u < kLocalTile / kSortBlock; ++u) {
+        const int64_t p = base + u * kSortBlock + threadIdx.x;
+        val[u] = p < n ? vals_in[p] : 0;
+    }
     for (int k = threadIdx.x; k < span; k += kSortBlock) {
         const int64_t q = lo + k;
         s_keys[k] = (q >= 0 && q < n) ? keys_in[q] : 0ull;
@@ -380,7 +388,7 @@ __global__ __launch_bounds__(kSortBlock) void k_sort_local(const uint64_t* __res
         }
         const int64_t dst = lo + ls + r;
         keys_out[dst] = k;
-        vals_out[dst] = vals_in[p];
+        vals_out[dst] = val[u];
     }
 }
 
