@@ -438,6 +438,16 @@ class HipTracer:
         return rays.results
 
 
+def normalize_vertices(ctx, vertices):
+    """SceneLoader's load-time normalisation, in place on a (V,3) float32 CUDA
+    tensor: v = (v - mn) / largest extent of the vertex box; returns the box."""
+    if vertices.dtype != torch.float32 or not vertices.is_cuda or not vertices.is_contiguous():
+        raise ValueError("vertices must be a contiguous float32 CUDA tensor")
+    box = torch.zeros(6, dtype=torch.float32, device=vertices.device)
+    call("hx_normalize_vertices", ctx.handle, _ptr(vertices), int(vertices.shape[0]), _ptr(box))
+    return box
+
+
 def scene_box(ctx, scene):
     out = torch.zeros(6, dtype=torch.float32, device=scene.vertices.device)
     call("hx_scene_box", ctx.handle, _ptr(scene.vertices), _ptr(scene.triangles), scene.n_triangles, _ptr(out))

@@ -1430,6 +1430,17 @@ int hx_bvh_cost(hx_bvh* bvh, double* out) {
     return HX_OK;
 }
 
+int hx_normalize_vertices(hx_context* ctx, float* d_vertices, int64_t n_vertices, float* d_box6) {
+    HX_CHECK_ARG(ctx && d_vertices && n_vertices > 0, "invalid argument");
+    Context* c = &ctx->c;
+    float* box = d_box6;
+    if (!box) {
+        HX_TRY(c->control.reserve(64));
+        box = c->control.as<float>() + 8;
+    }
+    return hx::launch_normalize(c, d_vertices, n_vertices, box);
+}
+
 int hx_scene_box(hx_context* ctx, const float* d_vertices, const int32_t* d_triangles, int64_t n, float* d_box6) {
     HX_CHECK_ARG(ctx && d_vertices && d_triangles && d_box6 && n > 0, "invalid argument");
     return hx::launch_scene_box(&ctx->c, d_vertices, d_triangles, n, d_box6);

@@ -45,7 +45,11 @@ __global__ __launch_bounds__(kBlock) void k_scene_box_partial(const float* __res
     float r[6] = {kMaxFloat, kMaxFloat, kMaxFloat, -kMaxFloat, -kMaxFloat, -kMaxFloat};
     for (int64_t t = blockIdx.x * (int64_t)kBlock + threadIdx.x; t < n; t += (int64_t)gridDim.x * kBlock) {
         float3 a, b, c;
-        tri_vertices(verts, tris, t, a, b, c);
+        if (tris) {
+            tri_vertices(verts, tris, t, a, b, c);
+        } else {  // the vertex box: element t is vertex t
+            a = b = c = ld3(verts + 3 * t);
+        }
         r[0] = fminf(r[0], fminf(fminf(a.x, b.x), c.x));
         r[1] = fminf(r[1], fminf(fminf(a.y, b.y), c.y));
         r[2] = fminf(r[2], fminf(fminf(a.z, b.z), c.z));
@@ -100,6 +104,23 @@ __global__ __launch_bounds__(kBlock) void k_scene_box_final(const float* __restr
         float v = s[0][k];
         for (int w = 1; w < kWavesPerBlock; ++w) v = (k < 3) ? fminf(v, s[w][k]) : fmaxf(v, s[w][k]);
         box[k] = v;
+    }
+}
+
+// Scene normalisation (SceneLoader.cpp:134-141): v = (v - mn) * scale with
+// scale = 1 / the largest extent of the vertex box (AssimpImporter.cpp:99-113
+// grows it over every vertex), subtraction and product rounded separately.
+__global__ __launch_bounds__(kBlock) void k_normalize(float* __restrict__ verts, int64_t nv,
+                                                     const float* __restrict__ box) {
+    const int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    if (i >= nv) return;
+    const float dx = box[3] - box[0], dy = box[4] - box[1], dz = box[5] - box[2];
+    const float dxy = dx < dy ? dy : dx;  // qMax(a, b) = a < b ? b : a
+    const float scale = 1.0f / (dxy < dz ? dz : dxy);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float d = verts[3 * i + k] - box[k];
+        verts[3 * i + k] = d * scale;
     }
 }
 
@@ -1366,6 +1387,13 @@ int launch_scene_box(Context* c, const float* verts, const int32_t* tris, int64_
                            d_box6);
     }
     return check_launch("scene_box_final");
+}
+
+int launch_normalize(Context* c, float* verts, int64_t nv, float* d_box6) {
+    HX_TRY(launch_scene_box(c, verts, nullptr, nv, d_box6));
+    LaunchScope ls(c, "normalize");
+    hipLaunchKernelGGL(k_normalize, dim3(grid_for(nv)), dim3(kBlock), 0, c->stream, verts, nv, d_box6);
+    return check_launch("normalize");
 }
 
 int launch_morton(Context* c, const float* verts, const int32_t* tris, int64_t n, const float* d_box6, int bits,

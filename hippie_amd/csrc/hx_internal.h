@@ -110,6 +110,9 @@ int check_launch(const char* name);
 
 // hx_build.hip
 int launch_scene_box(Context* c, const float* verts, const int32_t* tris, int64_t n, float* d_box6);
+// Scene normalisation in place: the vertex box (tris = NULL in
+// launch_scene_box) to d_box6, then v = (v - mn) / largest extent.
+int launch_normalize(Context* c, float* verts, int64_t nv, float* d_box6);
 // mesh_of / off (batched builds, else NULL): mesh of each triangle (sorted
 // position for construct) and the M+1 triangle offsets; d_box6 then holds
 // one box per mesh.

@@ -183,6 +183,11 @@ int hx_bvh_info(hx_bvh* bvh, const char* key, int64_t* value);
 int hx_bvh_cost(hx_bvh* bvh, double* out);
 
 /* ---- stages (the reference's individually launched kernels) --------------- */
+/* Scene normalisation at load (SceneLoader::loadStaticScene,
+ * SceneLoader.cpp:134-141): in place, v = (v - mn) * (1 / max extent) over the
+ * box of all n_vertices vertices (AssimpImporter.cpp:99-113); d_box6 (device,
+ * may be NULL) receives that box. */
+int hx_normalize_vertices(hx_context* ctx, float* d_vertices, int64_t n_vertices, float* d_box6);
 /* Scene box: exact min/max over the triangles' vertices (SceneLoader.cpp:143-153).
  * d_box6 receives mn.xyz, mx.xyz on the device. */
 int hx_scene_box(hx_context* ctx, const float* d_vertices, const int32_t* d_triangles, int64_t n,

@@ -277,6 +277,25 @@ void hxo_set_threads(int threads) {
 
 int hxo_get_threads(void) { return omp_get_max_threads(); }
 
+/* SceneLoader.cpp:134-141 after AssimpImporter.cpp:99-113: the box of every
+ * vertex, scale = 1 / qMax(qMax(d.x, d.y), d.z) (qMax(a, b) = a < b ? b : a),
+ * then v -= mn and v *= scale per component; box6 receives the vertex box. */
+void hxo_normalize(float* verts, int64_t nv, float box6[6]) {
+    Box b;
+    for (int64_t i = 0; i < nv; ++i) b.grow(V3{verts[3 * i], verts[3 * i + 1], verts[3 * i + 2]});
+    const float dx = b.mx.x - b.mn.x, dy = b.mx.y - b.mn.y, dz = b.mx.z - b.mn.z;
+    const float dxy = dx < dy ? dy : dx;
+    const float scale = 1.0f / (dxy < dz ? dz : dxy);
+    const float mn[3] = {b.mn.x, b.mn.y, b.mn.z};
+    for (int64_t i = 0; i < nv; ++i)
+        for (int k = 0; k < 3; ++k) {
+            const float d = verts[3 * i + k] - mn[k];
+            verts[3 * i + k] = d * scale;
+        }
+    box6[0] = b.mn.x; box6[1] = b.mn.y; box6[2] = b.mn.z;
+    box6[3] = b.mx.x; box6[4] = b.mx.y; box6[5] = b.mx.z;
+}
+
 void hxo_scene_box(const float* verts, const int32_t* tris, int64_t n, float out[6]) {
     Box scene;
     for (int64_t t = 0; t < n; ++t) {

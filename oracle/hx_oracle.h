@@ -27,6 +27,7 @@ int hxo_get_threads(void);
 
 /* Scene box = union of triangle boxes, triangles in order.
  * rt/scene/SceneLoader.cpp:143-153. out = {mn.x,mn.y,mn.z,mx.x,mx.y,mx.z}. */
+void hxo_normalize(float* verts, int64_t nv, float box6[6]);
 void hxo_scene_box(const float* verts, const int32_t* tris, int64_t n, float out[6]);
 
 /* Per-triangle boxes: rt/bvh/LBVHBuilderKernels.cu:65-98 (setupBoxes).

@@ -34,6 +34,7 @@ def lib():
             "hxo_set_threads": (None, [_i32]),
             "hxo_get_threads": (_i32, []),
             "hxo_scene_box": (None, [_p, _p, _i64, _p]),
+            "hxo_normalize": (None, [_p, _i64, _p]),
             "hxo_setup_boxes": (None, [_p, _p, _i64, _p, _p]),
             "hxo_morton_codes": (None, [_p, _p, _i64, _p, _i32, _p]),
             "hxo_sort_pairs": (None, [_p, _p, _i64, _i32]),
@@ -92,6 +93,14 @@ def set_threads(n):
 
 def get_threads():
     return lib().hxo_get_threads()
+
+
+def normalize(verts):
+    """Load-time normalisation: (normalised copy of verts, vertex box)."""
+    v = np.array(verts, dtype=np.float32, order="C", copy=True)
+    box = np.zeros(6, np.float32)
+    lib().hxo_normalize(_ptr(v), v.shape[0], _ptr(box))
+    return v, box
 
 
 def scene_box(verts, tris):

@@ -300,3 +300,14 @@ def test_lbvh_sort_32_keys_per_thread(oracle, ctx):
         _check_build(oracle, ctx, v, t, 60)
     finally:
         ctx.set_tuning("sort_items", old)
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_normalize_vertices_matches_oracle(oracle, ctx, seed):
+    rng = np.random.default_rng(seed)
+    v = (rng.normal(size=(100003, 3)) * np.array([0.3, 4.0, 1.5]) + 7.0).astype(np.float32)
+    ref, rbox = oracle.normalize(v)
+    vt = torch.from_numpy(v.copy()).cuda()
+    box = hx.normalize_vertices(ctx, vt).cpu().numpy()
+    assert np.array_equal(box, rbox)
+    assert np.array_equal(vt.cpu().numpy().view(np.uint32), ref.view(np.uint32))

@@ -357,3 +357,19 @@ def test_validate_catches_planted_mistakes(oracle):
     bad = nodes.copy()
     bad[1, 15] = 7                                                # wrong parent
     assert oracle.validate_bin(bad, ni, nl, tri, v, t) != 0
+
+
+@pytest.mark.parametrize("seed,scale", [(1, (3.0, 1.0, 0.5)), (2, (0.2, 5.0, 1.0)), (3, (1.0, 1.0, 7.0))])
+def test_normalize_closed_form(oracle, seed, scale):
+    """SceneLoader.cpp:134-141: the vertex box, then (v - mn) * (1 / largest
+    extent) with each operation rounded to float32, whichever axis is longest."""
+    rng = np.random.default_rng(seed)
+    v = (rng.normal(size=(5000, 3)) * np.array(scale) - 2.5).astype(np.float32)
+    got, box = oracle.normalize(v)
+    mn, mx = v.min(0), v.max(0)
+    assert np.array_equal(box, np.concatenate([mn, mx]))
+    d = mx - mn
+    s = np.float32(1.0) / np.float32(d.max())
+    exp = (v - mn).astype(np.float32) * s
+    assert np.array_equal(got.view(np.uint32), exp.astype(np.float32).view(np.uint32))
+    assert got.min() == 0.0 and np.isclose(got.max(), 1.0, rtol=1e-6)
