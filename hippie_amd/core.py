@@ -250,6 +250,22 @@ class HipBVH:
         )
         return out
 
+    def leaf_stats(self):
+        """(counts of leaves of size 0..32, sum of leaf sizes) -- the inputs of
+        HipBVH::getLeafSizeHistogram / getAvgLeafSize."""
+        h = np.zeros(33, np.int32)
+        s = ctypes.c_int64()
+        call("hx_bvh_leaf_stats", self.handle, h.ctypes.data_as(ctypes.c_void_p), ctypes.byref(s))
+        return h, s.value
+
+    def avg_leaf_size(self):
+        """HipBVH::getAvgLeafSize (HipBVH.cpp:380-404)."""
+        return self.leaf_stats()[1] / float(self.n_leaves)
+
+    def leaf_size_histogram(self):
+        """HipBVH::getLeafSizeHistogram (HipBVH.cpp:406-442): fraction of leaves of each size 0..32."""
+        return self.leaf_stats()[0] / float(self.n_leaves)
+
     def get_cost(self):
         c = ctypes.c_double()
         call("hx_bvh_cost", self.handle, ctypes.byref(c))

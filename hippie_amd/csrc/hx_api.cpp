@@ -144,6 +144,7 @@ struct hx_bvh {
     DevBuf node_at;  // general trees: internal node at each split position
     DevBuf link;     // LBVH builds: arrival words of the climb's top level
     DevBuf proto;    // LBVH builds: Bin rows 0-2 of each internal node from the climb
+    DevBuf leaf_stats;  // hx_bvh_leaf_stats: size counts and their sum
     int ploc_rounds = 0;
     int64_t atr_accepted = 0;
     int hlbvh_clusters = 0;
@@ -1427,6 +1428,21 @@ int hx_bvh_cost(hx_bvh* bvh, double* out) {
                            bvh->scene_box.as<float>(), bvh->params.ct, bvh->params.ci, bvh->cost_out.as<double>()));
     HX_HIP(hipMemcpyAsync(out, bvh->cost_out.ptr, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HX_HIP(hipStreamSynchronize(c->stream));
+    return HX_OK;
+}
+
+int hx_bvh_leaf_stats(hx_bvh* bvh, int32_t* histogram33, int64_t* sum_of_leaf_sizes) {
+    HX_CHECK_ARG(bvh && histogram33 && sum_of_leaf_sizes, "NULL argument");
+    HX_CHECK_ARG(!bvh->batched, "leaf statistics are per BVH; a batch has one BVH per mesh");
+    Context* c = &bvh->ctx->c;
+    HX_TRY(bvh->leaf_stats.reserve(sizeof(int32_t) * (hx::kMaxLeafSizeStat + 2)));
+    HX_TRY(hx::launch_leaf_stats(c, bvh->nodes.as<int4>(), bvh->counts.as<int32_t>(),
+                                 std::max<int64_t>(bvh->n_refs, 1), bvh->leaf_stats.as<int32_t>()));
+    int32_t h[hx::kMaxLeafSizeStat + 2];
+    HX_HIP(hipMemcpyAsync(h, bvh->leaf_stats.ptr, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HX_HIP(hipStreamSynchronize(c->stream));
+    for (int k = 0; k <= hx::kMaxLeafSizeStat; ++k) histogram33[k] = h[k];
+    *sum_of_leaf_sizes = (uint32_t)h[hx::kMaxLeafSizeStat + 1];
     return HX_OK;
 }
 

@@ -1345,6 +1345,28 @@ __global__ __launch_bounds__(kBlock) void k_cost_final(const double* __restrict_
     if (threadIdx.x == 0) out[0] = s[0];
 }
 
+// Leaf statistics (computeSumOfLeafSizes / computeLeafSizeHistogram,
+// HipBVHKernels.cu:124-182): per leaf node its size (~word 14); out[0..32]
+// counts sizes 0..32, out[33] sums all sizes.  Block counts in LDS, then one
+// atomic per non-zero entry.
+__global__ __launch_bounds__(kBlock) void k_leaf_stats(const int4* __restrict__ nodes,
+                                                      const int32_t* __restrict__ counts,
+                                                      int32_t* __restrict__ out) {
+    __shared__ int s_hist[kMaxLeafSizeStat + 2];
+    for (int k = threadIdx.x; k < kMaxLeafSizeStat + 2; k += kBlock) s_hist[k] = 0;
+    __syncthreads();
+    const int ni = counts[0], nl = counts[1];
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < nl) {
+        const int size = ~nodes[4 * (int64_t)(ni + i) + 3].z;
+        if (size >= 0 && size <= kMaxLeafSizeStat) atomicAdd(&s_hist[size], 1);
+        atomicAdd(&s_hist[kMaxLeafSizeStat + 1], size);
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < kMaxLeafSizeStat + 2; k += kBlock)
+        if (s_hist[k]) atomicAdd(&out[k], s_hist[k]);
+}
+
 __global__ __launch_bounds__(kBlock) void k_node_at(const int32_t* __restrict__ split, int m,
                                                    int32_t* __restrict__ node_at) {
     const int v = blockIdx.x * kBlock + threadIdx.x;
@@ -1538,6 +1560,15 @@ int launch_single_ref(Context* c, const float* verts, const int32_t* tris, const
     hipLaunchKernelGGL(k_single_ref, dim3(1), dim3(64), 0, c->stream, verts, tris, sorted_tri, nmin, nmax, woop, nodes,
                        counts);
     return check_launch("single_ref");
+}
+
+int launch_leaf_stats(Context* c, const int4* nodes, const int32_t* d_counts, int64_t max_leaves, int32_t* d_out) {
+    LaunchScope ls(c, "leaf_stats");
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, d_out, 0, (int64_t)(kMaxLeafSizeStat + 2));
+    HX_TRY(check_launch("leaf_stats_clear"));
+    hipLaunchKernelGGL(k_leaf_stats, dim3(grid_for(std::max<int64_t>(max_leaves, 1))), dim3(kBlock), 0, c->stream,
+                       nodes, d_counts, d_out);
+    return check_launch("leaf_stats");
 }
 
 int launch_cost(Context* c, const int4* nodes, const int32_t* d_counts, int64_t max_nodes, const float* d_box6,

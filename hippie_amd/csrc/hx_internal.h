@@ -231,6 +231,10 @@ int launch_single_ref(Context* c, const float* verts, const int32_t* tris, const
                       float4* nmin, float4* nmax, float4* woop, int4* nodes, int32_t* counts);
 int launch_cost(Context* c, const int4* nodes, const int32_t* d_counts, int64_t max_nodes,
                 const float* d_box6, float ct, float ci, double* d_out);
+// Leaf-size histogram (sizes 0..kMaxLeafSizeStat) and sum of leaf sizes into
+// d_out[0..kMaxLeafSizeStat+1] (HipBVHKernels.cu:124-182).
+constexpr int kMaxLeafSizeStat = 32;  // MAX_LEAF_SIZE, HipBVHKernels.h:13
+int launch_leaf_stats(Context* c, const int4* nodes, const int32_t* d_counts, int64_t max_leaves, int32_t* d_out);
 int launch_fill_i32(Context* c, int32_t* p, int32_t v, int64_t n);
 
 // hx_ploc.hip: PLOC clustering over the sorted leaves, then depth-first

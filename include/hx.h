@@ -181,6 +181,10 @@ int hx_bvh_debug(hx_bvh* bvh, hx_bvh_debug_view* out);
 int hx_bvh_info(hx_bvh* bvh, const char* key, int64_t* value);
 /* HipBVH::getCost (HipBVH.cpp:346-370): SAH cost, FP64 deterministic sum. */
 int hx_bvh_cost(hx_bvh* bvh, double* out);
+/* HipBVH::getAvgLeafSize / getLeafSizeHistogram (HipBVH.cpp:380-442): counts
+ * of leaves of each size 0..32 (host int32[33]) and the sum of all leaf sizes;
+ * divide by the leaf count for the reference's fractions.  Synchronises. */
+int hx_bvh_leaf_stats(hx_bvh* bvh, int32_t* histogram33, int64_t* sum_of_leaf_sizes);
 
 /* ---- stages (the reference's individually launched kernels) --------------- */
 /* Scene normalisation at load (SceneLoader::loadStaticScene,

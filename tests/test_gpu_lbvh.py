@@ -311,3 +311,16 @@ def test_normalize_vertices_matches_oracle(oracle, ctx, seed):
     box = hx.normalize_vertices(ctx, vt).cpu().numpy()
     assert np.array_equal(box, rbox)
     assert np.array_equal(vt.cpu().numpy().view(np.uint32), ref.view(np.uint32))
+
+
+@pytest.mark.parametrize("adaptive,max_leaf", [(True, 8), (False, 4), (False, 40)])
+def test_leaf_stats(oracle, ctx, adaptive, max_leaf):
+    """HipBVH::getAvgLeafSize / getLeafSizeHistogram inputs from the Bin leaves."""
+    v, t = scenes.gaussian_mixture_soup(50000, seed=21)
+    scene, bvh = _gpu_build(ctx, v, t, 60, adaptive, max_leaf)
+    nodes = bvh.nodes_host()
+    sizes = ~nodes[bvh.n_interior:bvh.n_interior + bvh.n_leaves, 14]
+    hist, total = bvh.leaf_stats()
+    assert total == sizes.sum() == t.shape[0]
+    assert np.array_equal(hist, np.bincount(sizes[sizes <= 32], minlength=33)[:33])
+    assert abs(bvh.avg_leaf_size() - t.shape[0] / bvh.n_leaves) < 1e-9
