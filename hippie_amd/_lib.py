@@ -109,6 +109,7 @@ SIGNATURES = {
     "hx_scene_box": [_p, _p, _p, _i64, _p],
     "hx_normalize_vertices": [_p, _p, _i64, _p],
     "hx_bvh_leaf_stats": [_p, _p, ctypes.POINTER(_i64)],
+    "hx_bvh_validate": [_p, ctypes.POINTER(_i32)],
     "hx_morton_codes": [_p, _p, _p, _i64, _p, _i32, _p],
     "hx_sort_pairs_u64": [_p, _p, _p, _i64, _i32],
     "hx_trace": [_p, _p, _p, _i64, _i32, _p, _p],

@@ -250,6 +250,13 @@ class HipBVH:
         )
         return out
 
+    def validate(self):
+        """HipBVH::validate on the device: 0 if the BVH is valid for the geometry
+        of its last build or refit, else the OR of the failed checks (hx.h)."""
+        f = ctypes.c_int32()
+        call("hx_bvh_validate", self.handle, ctypes.byref(f))
+        return f.value
+
     def leaf_stats(self):
         """(counts of leaves of size 0..32, sum of leaf sizes) -- the inputs of
         HipBVH::getLeafSizeHistogram / getAvgLeafSize."""
@@ -281,6 +288,7 @@ class HipBVH:
         """HipBVH::update -- refit + re-woopify with unchanged topology."""
         s = scene or self.scene
         call("hx_refit", self.ctx.handle, self.handle, _ptr(s.vertices), _ptr(s.triangles))
+        self.scene = s  # the BVH keeps the refit geometry's device pointers
 
     def __del__(self):
         h = getattr(self, "handle", None)

@@ -1431,6 +1431,22 @@ int hx_bvh_cost(hx_bvh* bvh, double* out) {
     return HX_OK;
 }
 
+int hx_bvh_validate(hx_bvh* bvh, int32_t* failed) {
+    HX_CHECK_ARG(bvh && failed, "NULL argument");
+    HX_CHECK_ARG(!bvh->batched, "validation is per BVH; a batch has one BVH per mesh");
+    HX_CHECK_ARG(bvh->built, "the BVH has not been built");
+    Context* c = &bvh->ctx->c;
+    const int64_t max_nodes = std::max<int64_t>(2 * bvh->n_refs - 1, 2);
+    HX_TRY(bvh->leaf_stats.reserve(sizeof(int32_t) * (size_t)(max_nodes + bvh->n_tri + 64)));
+    int32_t* work = bvh->leaf_stats.as<int32_t>() + 64;
+    int32_t* flag = bvh->leaf_stats.as<int32_t>();
+    HX_TRY(hx::launch_validate(c, bvh->nodes.as<int4>(), bvh->counts.as<int32_t>(), max_nodes,
+                               bvh->tri_idx.as<int32_t>(), bvh->n_refs, bvh->verts, bvh->tris, bvh->n_tri, work, flag));
+    HX_HIP(hipMemcpyAsync(failed, flag, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    HX_HIP(hipStreamSynchronize(c->stream));
+    return HX_OK;
+}
+
 int hx_bvh_leaf_stats(hx_bvh* bvh, int32_t* histogram33, int64_t* sum_of_leaf_sizes) {
     HX_CHECK_ARG(bvh && histogram33 && sum_of_leaf_sizes, "NULL argument");
     HX_CHECK_ARG(!bvh->batched, "leaf statistics are per BVH; a batch has one BVH per mesh");

@@ -1367,6 +1367,83 @@ __global__ __launch_bounds__(kBlock) void k_leaf_stats(const int4* __restrict__ 
         if (s_hist[k]) atomicAdd(&out[k], s_hist[k]);
 }
 
+// HipBVH::validate (HipBVH.cpp:157-310) on the device, one thread per Bin
+// node: an interior's children must name it as parent and their sizes sum to
+// its size; a leaf's range must be non-empty and inside the references and
+// its box must overlap each of its triangles' boxes.  Every child reference
+// and every leaf triangle is counted (node_hits, tri_hits); k_validate_counts
+// then requires one reference per node but the root and one or more per
+// triangle.  Every index is range-checked before use.  Failures OR bits into
+// *flags: 1 child index, 2 parent, 4 size, 8 leaf range, 16 leaf box,
+// 32 triangle index, 64 node references, 128 triangle references.
+__global__ __launch_bounds__(kBlock) void k_validate_nodes(const int4* __restrict__ nodes, const int32_t* __restrict__ counts,
+                                                          const int32_t* __restrict__ tri_idx, int64_t n_refs,
+                                                          const float* __restrict__ verts, const int32_t* __restrict__ tris,
+                                                          int64_t n_tri, int32_t* __restrict__ node_hits,
+                                                          int32_t* __restrict__ tri_hits, int32_t* __restrict__ flags) {
+    const int ni = counts[0], nn = counts[0] + counts[1];
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= nn) return;
+    const int4* nd = nodes + 4 * (int64_t)i;
+    const int4 q3 = nd[3];
+    int f = 0;
+    if (i < ni) {
+        int sum = 0;
+        const int ch[2] = {q3.x, q3.y};
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            // A single reference is one leaf named by both of the root's links.
+            if (k == 1 && ch[0] == ch[1] && n_refs == 1) continue;
+            const int c = ch[k] < 0 ? ~ch[k] : ch[k];
+            if (c <= 0 || c >= nn) {
+                f |= 1;
+                continue;
+            }
+            const int4 cq = nodes[4 * (int64_t)c + 3];
+            if (cq.w != i) f |= 2;
+            sum += cq.z < 0 ? ~cq.z : cq.z;
+            atomicAdd(&node_hits[c], 1);
+        }
+        if (sum != q3.z) f |= 4;
+        if (i == 0 && q3.w != -1) f |= 2;
+    } else {
+        const int b = q3.x, e = q3.y;
+        if (!(b < e) || b < 0 || e > n_refs || ~q3.z != e - b) {
+            f |= 8;
+        } else {
+            const float4 r0 = reinterpret_cast<const float4*>(nd)[0], r2 = reinterpret_cast<const float4*>(nd)[2];
+            const float mnx = r0.x, mxx = r0.y, mny = r0.z, mxy = r0.w, mnz = r2.x, mxz = r2.y;
+            for (int p = b; p < e; ++p) {
+                const int t = tri_idx[p];
+                if (t < 0 || t >= n_tri) {
+                    f |= 32;
+                    continue;
+                }
+                atomicAdd(&tri_hits[t], 1);
+                float3 v0, v1, v2;
+                tri_vertices(verts, tris, t, v0, v1, v2);
+                const Box3 tb = tri_box(v0, v1, v2);
+                if (tb.mx.x < mnx || tb.mn.x > mxx || tb.mx.y < mny || tb.mn.y > mxy || tb.mx.z < mnz ||
+                    tb.mn.z > mxz)
+                    f |= 16;
+            }
+        }
+    }
+    if (f) atomicOr(flags, f);
+}
+
+__global__ __launch_bounds__(kBlock) void k_validate_counts(const int32_t* __restrict__ counts,
+                                                           const int32_t* __restrict__ node_hits,
+                                                           const int32_t* __restrict__ tri_hits, int64_t n_tri,
+                                                           int32_t* __restrict__ flags) {
+    const int nn = counts[0] + counts[1];
+    const int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    int f = 0;
+    if (i < nn && node_hits[i] != (i == 0 ? 0 : 1)) f |= 64;
+    if (i < n_tri && tri_hits[i] < 1) f |= 128;
+    if (f) atomicOr(flags, f);
+}
+
 __global__ __launch_bounds__(kBlock) void k_node_at(const int32_t* __restrict__ split, int m,
                                                    int32_t* __restrict__ node_at) {
     const int v = blockIdx.x * kBlock + threadIdx.x;
@@ -1560,6 +1637,25 @@ int launch_single_ref(Context* c, const float* verts, const int32_t* tris, const
     hipLaunchKernelGGL(k_single_ref, dim3(1), dim3(64), 0, c->stream, verts, tris, sorted_tri, nmin, nmax, woop, nodes,
                        counts);
     return check_launch("single_ref");
+}
+
+int launch_validate(Context* c, const int4* nodes, const int32_t* d_counts, int64_t max_nodes, const int32_t* tri_idx,
+                    int64_t n_refs, const float* verts, const int32_t* tris, int64_t n_tri, int32_t* work,
+                    int32_t* d_flags) {
+    LaunchScope ls(c, "validate");
+    int32_t* node_hits = work;
+    int32_t* tri_hits = work + max_nodes;
+    hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(max_nodes + n_tri)), dim3(kBlock), 0, c->stream, work, 0,
+                       max_nodes + n_tri);
+    HX_TRY(check_launch("validate_clear"));
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, d_flags, 0, (int64_t)1);
+    HX_TRY(check_launch("validate_clear_flags"));
+    hipLaunchKernelGGL(k_validate_nodes, dim3(grid_for(max_nodes)), dim3(kBlock), 0, c->stream, nodes, d_counts,
+                       tri_idx, n_refs, verts, tris, n_tri, node_hits, tri_hits, d_flags);
+    HX_TRY(check_launch("validate_nodes"));
+    hipLaunchKernelGGL(k_validate_counts, dim3(grid_for(std::max(max_nodes, n_tri))), dim3(kBlock), 0, c->stream,
+                       d_counts, node_hits, tri_hits, n_tri, d_flags);
+    return check_launch("validate_counts");
 }
 
 int launch_leaf_stats(Context* c, const int4* nodes, const int32_t* d_counts, int64_t max_leaves, int32_t* d_out) {

@@ -235,6 +235,11 @@ int launch_cost(Context* c, const int4* nodes, const int32_t* d_counts, int64_t 
 // d_out[0..kMaxLeafSizeStat+1] (HipBVHKernels.cu:124-182).
 constexpr int kMaxLeafSizeStat = 32;  // MAX_LEAF_SIZE, HipBVHKernels.h:13
 int launch_leaf_stats(Context* c, const int4* nodes, const int32_t* d_counts, int64_t max_leaves, int32_t* d_out);
+// HipBVH::validate checks on the device; work holds max_nodes + n_tri ints;
+// *d_flags receives the OR of the failed checks (0 = valid).
+int launch_validate(Context* c, const int4* nodes, const int32_t* d_counts, int64_t max_nodes, const int32_t* tri_idx,
+                    int64_t n_refs, const float* verts, const int32_t* tris, int64_t n_tri, int32_t* work,
+                    int32_t* d_flags);
 int launch_fill_i32(Context* c, int32_t* p, int32_t v, int64_t n);
 
 // hx_ploc.hip: PLOC clustering over the sorted leaves, then depth-first

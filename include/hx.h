@@ -185,6 +185,12 @@ int hx_bvh_cost(hx_bvh* bvh, double* out);
  * of leaves of each size 0..32 (host int32[33]) and the sum of all leaf sizes;
  * divide by the leaf count for the reference's fractions.  Synchronises. */
 int hx_bvh_leaf_stats(hx_bvh* bvh, int32_t* histogram33, int64_t* sum_of_leaf_sizes);
+/* HipBVH::validate (HipBVH.cpp:157-310) on the device against the geometry of
+ * the last build or refit: *failed = 0 if valid, else the OR of 1 child index,
+ * 2 parent, 4 size, 8 leaf range, 16 leaf box misses a triangle, 32 triangle
+ * index, 64 a node referenced other than once, 128 an unreferenced triangle.
+ * Synchronises. */
+int hx_bvh_validate(hx_bvh* bvh, int32_t* failed);
 
 /* ---- stages (the reference's individually launched kernels) --------------- */
 /* Scene normalisation at load (SceneLoader::loadStaticScene,

@@ -324,3 +324,35 @@ def test_leaf_stats(oracle, ctx, adaptive, max_leaf):
     assert total == sizes.sum() == t.shape[0]
     assert np.array_equal(hist, np.bincount(sizes[sizes <= 32], minlength=33)[:33])
     assert abs(bvh.avg_leaf_size() - t.shape[0] / bvh.n_leaves) < 1e-9
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("method,presplit", [("lbvh", 0), ("ploc", 0), ("atr", 0), ("hlbvh", 0), ("insertion", 0),
+                                             ("lbvh", 1)])
+def test_validate(oracle, ctx, method, presplit):
+    """HipBVH::validate on the device agrees with the oracle's validation of the same nodes."""
+    v, t = scenes.merge(scenes.gaussian_mixture_soup(4000, seed=5), scenes.ground_grid(3))
+    scene = hx.Scene(v, t)
+    p = hx.BuildParams.from_env({"Bvh.method": method})
+    p.presplitting, p.presplitter_beta = presplit, 0.3
+    bvh = hx.BVHBuilder(ctx, params=p).build(scene)
+    assert bvh.validate() == 0
+    nodes, tri_idx = bvh.nodes_host(), bvh.tri_indices_host()
+    assert oracle.validate_bin_refs(nodes, bvh.n_interior, bvh.n_leaves, tri_idx, v, t) == 0
+    # Geometry moved after the build without a refit: the leaf boxes miss their
+    # triangles and nothing else fails; a refit makes the BVH valid again.
+    moved = hx.Scene(v + np.float32(100.0), t)
+    bvh.update(moved)
+    assert bvh.validate() == 0
+    moved.vertices += 50.0
+    assert bvh.validate() == 16
+    bvh.update()
+    assert bvh.validate() == 0
+
+
+@pytest.mark.gpu
+def test_validate_single_triangle(ctx):
+    v = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+    t = np.array([[0, 1, 2]], np.int32)
+    bvh = hx.LBVHBuilder(ctx).build(hx.Scene(v, t))
+    assert bvh.validate() == 0
