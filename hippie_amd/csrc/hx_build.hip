@@ -385,6 +385,16 @@ HX_DEV void bin_rows(float4 amn, float4 amx, float4 bmn, float4 bmx, int4& r0, i
     r2 = make_int4(__float_as_int(amn.z), __float_as_int(amx.z), __float_as_int(bmn.z), __float_as_int(bmx.z));
 }
 
+// SAH cost of a one-reference leaf with box [mn, mx] (0 without adaptive
+// collapse).
+HX_DEV float leaf_sah_cost(const ClimbArgs& a, float3 mn, float3 mx) {
+    if (!a.adaptive) return 0.0f;
+    Box3 bb = box_empty();
+    box_grow(bb, mn);
+    box_grow(bb, mx);
+    return a.ci * box_area(bb.mn, bb.mx);
+}
+
 // Leaf p of the sorted order: its box (the reference box when presplit),
 // Woop rows and SAH leaf cost; the Woop rows are stored, and with STORE the
 // box and cost to the leaf's node entries.
@@ -421,14 +431,8 @@ HX_DEV Box3 climb_leaf(const ClimbArgs& a, int p, float& cost) {
     a.woop[3 * (int64_t)p + 0] = w0;
     a.woop[3 * (int64_t)p + 1] = w1;
     a.woop[3 * (int64_t)p + 2] = w2;
-    cost = 0.0f;
-    if (a.adaptive) {
-        Box3 bb = box_empty();
-        box_grow(bb, tb.mn);
-        box_grow(bb, tb.mx);
-        cost = a.ci * box_area(bb.mn, bb.mx);
-        if (STORE) a.cost[leaf] = cost;
-    }
+    cost = leaf_sah_cost(a, tb.mn, tb.mx);
+    if (STORE && a.adaptive) a.cost[leaf] = cost;
     return tb;
 }
 
@@ -683,14 +687,19 @@ constexpr int kLbvhThreads = 512;
 template <bool K64>
 __global__ __launch_bounds__(kLbvhThreads) void k_lbvh_local(ClimbArgs a) {
     constexpr int T = kClimbLeaves;
-    // Slots: internal node s -> s - base, leaf p -> T + p - base.
+    // Slots: internal node s -> s - base, leaf p -> T + p - base.  Every link
+    // kept in LDS stays inside the block -- range ends, child slots, parent
+    // splits, frontier slots -- so it is a 16-bit offset from base; deltas
+    // (-1..159, stored plus one) and states fit 8 bits; leaf costs are
+    // recomputed from the staged box: 38 KB per block, four blocks per CU.
     __shared__ float s_box[6][2 * T];
-    __shared__ float s_cost[2 * T];
-    __shared__ int s_par[2 * T];
+    __shared__ float s_cost[T];         // internal slots
+    __shared__ int16_t s_par[2 * T];    // parent split - base, -1 if outside [base, last_leaf)
     __shared__ int s_cnt[T];
-    __shared__ int s_first[T], s_last[T], s_left[T], s_right[T], s_state[T];
-    __shared__ int s_delta[T + 1];  // delta(base - 1 + k)
-    __shared__ int s_front[T];
+    __shared__ int16_t s_first[T], s_last[T], s_left[T], s_right[T];  // offsets; children as slots
+    __shared__ int8_t s_state[T];
+    __shared__ uint8_t s_delta[T + 1];  // delta(base - 1 + k) + 1
+    __shared__ int16_t s_front[T];
     __shared__ int s_end[T];
     __shared__ int s_nfront, s_fbase;
     const int n = (int)a.n;
@@ -707,69 +716,66 @@ __global__ __launch_bounds__(kLbvhThreads) void k_lbvh_local(ClimbArgs a) {
     }
     for (int k = threadIdx.x; k <= T; k += kLbvhThreads) {
         const int q = base - 1 + k;
-        s_delta[k] = (q < 0 || q >= n - 1) ? -1 : kdelta<K64>(q, q + 1, n, a.keys, a.mesh_of, a.mesh_off);
+        s_delta[k] = (uint8_t)((q < 0 || q >= n - 1) ? 0 : 1 + kdelta<K64>(q, q + 1, n, a.keys, a.mesh_of, a.mesh_off));
     }
 #pragma unroll
     for (int k = 0; k < T / kLbvhThreads; ++k) {
         const int p = base + k * kLbvhThreads + threadIdx.x;
         if (p > last_leaf) break;
-        float cost;
-        const Box3 tb = climb_leaf<false>(a, p, cost);
+        float unused;
+        const Box3 tb = climb_leaf<false>(a, p, unused);
         const int sl = T + p - base;
         s_box[0][sl] = tb.mn.x; s_box[1][sl] = tb.mn.y; s_box[2][sl] = tb.mn.z;
         s_box[3][sl] = tb.mx.x; s_box[4][sl] = tb.mx.y; s_box[5][sl] = tb.mx.z;
-        s_cost[sl] = cost;
     }
     __syncthreads();
 
-    auto slot = [&](int node) { return node >= n - 1 ? T + (node - (n - 1)) - base : node - base; };
+    auto node_of = [&](int sl) { return sl < T ? base + sl : n - 1 + base + (sl - T); };
+    auto box_mn = [&](int sl) { return make_float3(s_box[0][sl], s_box[1][sl], s_box[2][sl]); };
+    auto box_mx = [&](int sl) { return make_float3(s_box[3][sl], s_box[4][sl], s_box[5][sl]); };
+    auto cost_at = [&](int sl) { return sl < T ? s_cost[sl] : leaf_sah_cost(a, box_mn(sl), box_mx(sl)); };
 #pragma unroll
     for (int k = 0; k < T / kLbvhThreads; ++k) {
         const int p = base + k * kLbvhThreads + threadIdx.x;
         if (p > last_leaf) break;
-        int node = n - 1 + p, ns = T + p - base, l = p, r = p;
+        int ns = T + p - base, l = p, r = p;
         while (true) {
             if (l == 0 && r == n - 1) {
-                *a.root_id = node;
+                *a.root_id = node_of(ns);
                 break;
             }
             const bool as_left = s_delta[r - base + 1] > s_delta[l - base];
             const int sp = as_left ? r : l - 1;
-            s_par[ns] = sp;
             if (sp < base || sp >= last_leaf) {
-                s_front[atomicAdd(&s_nfront, 1)] = node;
+                s_front[atomicAdd(&s_nfront, 1)] = (int16_t)ns;
                 break;
             }
             const int li = sp - base;
+            s_par[ns] = (int16_t)li;
             if (as_left) {
-                s_left[li] = node;
-                s_first[li] = l;
+                s_left[li] = (int16_t)ns;
+                s_first[li] = (int16_t)(l - base);
             } else {
-                s_right[li] = node;
-                s_last[li] = r;
+                s_right[li] = (int16_t)ns;
+                s_last[li] = (int16_t)(r - base);
             }
             if (__hip_atomic_fetch_add(&s_cnt[li], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) == 0) break;
             int ls, rs;
             if (as_left) {
-                r = s_last[li];
+                r = base + s_last[li];
                 ls = ns;
-                rs = slot(s_right[li]);
+                rs = s_right[li];
             } else {
-                l = s_first[li];
-                ls = slot(s_left[li]);
+                l = base + s_first[li];
+                ls = s_left[li];
                 rs = ns;
             }
-            const Combined c = combine(
-                a, l, r, make_float3(s_box[0][ls], s_box[1][ls], s_box[2][ls]),
-                make_float3(s_box[3][ls], s_box[4][ls], s_box[5][ls]), s_cost[ls],
-                make_float3(s_box[0][rs], s_box[1][rs], s_box[2][rs]),
-                make_float3(s_box[3][rs], s_box[4][rs], s_box[5][rs]), s_cost[rs]);
+            const Combined c = combine(a, l, r, box_mn(ls), box_mx(ls), cost_at(ls), box_mn(rs), box_mx(rs), cost_at(rs));
             s_box[0][li] = c.mn.x; s_box[1][li] = c.mn.y; s_box[2][li] = c.mn.z;
             s_box[3][li] = c.mx.x; s_box[4][li] = c.mx.y; s_box[5][li] = c.mx.z;
             s_cost[li] = c.cost;
-            s_state[li] = c.state;
+            s_state[li] = (int8_t)c.state;
             if (c.state == 0 && !seg_top(a.mesh_of, a.mesh_off, n, l, r)) atomicMax(&s_end[l - base], r);
-            node = sp;
             ns = li;
         }
     }
@@ -781,26 +787,29 @@ __global__ __launch_bounds__(kLbvhThreads) void k_lbvh_local(ClimbArgs a) {
     __syncthreads();
     if (threadIdx.x == 0) s_fbase = atomicAdd(a.frontier_count, s_nfront);
     __syncthreads();
-    for (int i = threadIdx.x; i < s_nfront; i += kLbvhThreads) a.frontier[s_fbase + i] = s_front[i];
+    for (int i = threadIdx.x; i < s_nfront; i += kLbvhThreads) a.frontier[s_fbase + i] = node_of(s_front[i]);
     // Boxes and costs go out only where the top level reads them (nodes whose
     // parent it completes); the emit reads the Bin rows each state-1 or root
-    // node gets from its children's boxes here.
+    // node gets from its children's boxes here.  Parents outside the block are
+    // written -1 here and set by the top level, which climbs from every
+    // frontier node.
     auto store_box = [&](int node, int sl) {
-        const int pp = s_par[sl];
-        const bool parent_here = pp >= base && pp < last_leaf && s_cnt[pp - base] == 2;
+        const int po = s_par[sl];
+        const bool parent_here = po >= 0 && s_cnt[po] == 2;
         if (a.all_boxes || !parent_here) {
             a.nmin[node] = make_float4(s_box[0][sl], s_box[1][sl], s_box[2][sl], 0.0f);
             a.nmax[node] = make_float4(s_box[3][sl], s_box[4][sl], s_box[5][sl], 0.0f);
-            if (a.adaptive) a.cost[node] = s_cost[sl];
+            if (a.adaptive) a.cost[node] = cost_at(sl);
         }
     };
+    auto parent_of = [&](int sl) { return s_par[sl] >= 0 ? base + s_par[sl] : -1; };
     auto box_at = [&](int sl, float4& mn, float4& mx) {
         mn = make_float4(s_box[0][sl], s_box[1][sl], s_box[2][sl], 0.0f);
         mx = make_float4(s_box[3][sl], s_box[4][sl], s_box[5][sl], 0.0f);
     };
     for (int i = threadIdx.x; i < T && base + i <= last_leaf; i += kLbvhThreads) {
         a.end_at[base + i] = s_end[i];
-        a.parent[n - 1 + base + i] = s_par[T + i];
+        a.parent[n - 1 + base + i] = parent_of(T + i);
         store_box(n - 1 + base + i, T + i);
     }
     // Internal nodes of the block's split range: the finished ones written
@@ -813,17 +822,18 @@ __global__ __launch_bounds__(kLbvhThreads) void k_lbvh_local(ClimbArgs a) {
             a.link[node] = 0;
             continue;
         }
-        a.first[node] = s_first[i];
-        a.last[node] = s_last[i];
-        a.left[node] = s_left[i];
-        a.right[node] = s_right[i];
-        a.parent[node] = s_par[i];
+        const int first = base + s_first[i], last = base + s_last[i];
+        a.first[node] = first;
+        a.last[node] = last;
+        a.left[node] = node_of(s_left[i]);
+        a.right[node] = node_of(s_right[i]);
+        a.parent[node] = parent_of(i);
         store_box(node, i);
         a.state[node] = s_state[i];
-        if (s_state[i] == 1 || seg_top(a.mesh_of, a.mesh_off, n, s_first[i], s_last[i])) {
+        if (s_state[i] == 1 || seg_top(a.mesh_of, a.mesh_off, n, first, last)) {
             float4 amn, amx, bmn, bmx;
-            box_at(slot(s_left[i]), amn, amx);
-            box_at(slot(s_right[i]), bmn, bmx);
+            box_at(s_left[i], amn, amx);
+            box_at(s_right[i], bmn, bmx);
             int4 r0, r1, r2;
             bin_rows(amn, amx, bmn, bmx, r0, r1, r2);
             int4* pr = reinterpret_cast<int4*>(a.proto + 3 * (int64_t)node);
