@@ -1548,7 +1548,9 @@ int launch_climb(Context* c, const ClimbArgs& a) {
         hipLaunchKernelGGL((k_climb_local<false, false>), dim3(blocks), dim3(kClimbThreads), 0, c->stream, a,
                            a.frontier, a.frontier_count);
     HX_TRY(check_launch("climb_local"));
-    const unsigned top = (unsigned)std::max(1, std::min<int>(c->cu_count * 4, (int)((a.n + kBlock - 1) / kBlock)));
+    // One frontier chain per thread: the chains are latency-bound walks, so
+    // the grid fills every wave slot (8 blocks of 256 per CU).
+    const unsigned top = (unsigned)std::max(1, std::min<int>(c->cu_count * 8, (int)((a.n + kBlock - 1) / kBlock)));
     if (atr)
         hipLaunchKernelGGL(k_climb_top<true>, dim3(top), dim3(kBlock), 0, c->stream, a, a.frontier, a.frontier_count);
     else
@@ -1567,7 +1569,9 @@ int launch_lbvh_climb(Context* c, const ClimbArgs& a) {
     else
         hipLaunchKernelGGL(k_lbvh_local<false>, dim3(blocks), dim3(kLbvhThreads), 0, c->stream, a);
     HX_TRY(check_launch("lbvh_local"));
-    const unsigned top = (unsigned)std::max(1, std::min<int>(c->cu_count * 4, (int)((a.n + kBlock - 1) / kBlock)));
+    // Every wave slot, as for the general climb (measured 1.483 -> 1.455 ms
+    // for the two climb kernels at 16.7M triangles against 4 blocks per CU).
+    const unsigned top = (unsigned)std::max(1, std::min<int>(c->cu_count * 8, (int)((a.n + kBlock - 1) / kBlock)));
     if (a.keys64)
         hipLaunchKernelGGL(k_lbvh_top<true>, dim3(top), dim3(kBlock), 0, c->stream, a);
     else
