@@ -811,6 +811,7 @@ __global__ __launch_bounds__(kLbvhThreads) void k_lbvh_local(ClimbArgs a) {
     };
     for (int i = threadIdx.x; i < T && base + i <= last_leaf; i += kLbvhThreads) {
         a.end_at[base + i] = s_end[i];
+        // Every leaf's parent: the refit climb (k_climb_local) starts from it.
         a.parent[n - 1 + base + i] = parent_of(T + i);
         store_box(n - 1 + base + i, T + i);
     }
