@@ -164,77 +164,10 @@ def bench_builders(ctx, scene, n, args):
 
 
 def batch_workload(args, device):
-    """Config 3 geometry and rays generated on the device: meshes of
-    round(exp(U[ln lo, ln hi])) triangles, half displaced icospheres (largest
-    level that fits, topped up with soup) and half soups, each in the unit box;
-    256 rays per 1k triangles from a sphere around the box aimed inside it."""
-    import math
-    import numpy as np
-    import torch
-    from hippie_amd import scenes
-    rng = np.random.default_rng(3)
-    M = args.batch_meshes
-    counts = np.rint(np.exp(rng.uniform(math.log(args.batch_lo), math.log(args.batch_hi), M))).astype(np.int64)
-    sphere = rng.random(M) < 0.5
-    g = torch.Generator(device=device)
-    g.manual_seed(3)
-    ico = {}
-    verts, tris_local, offs, vbase = [], [], [0], [0]
-
-    def soup(c):
-        cen = torch.rand((c, 1, 3), generator=g, device=device)
-        sc = 0.001 + 0.019 * torch.rand((c, 1, 1), generator=g, device=device)
-        v = cen + sc * (torch.rand((c, 3, 3), generator=g, device=device) - 0.5)
-        return v.reshape(-1, 3), torch.arange(3 * c, device=device, dtype=torch.int32).reshape(-1, 3)
-
-    for m in range(M):
-        c = int(counts[m])
-        if sphere[m] and c >= 20:
-            level = 0
-            while 20 * 4 ** (level + 1) <= c and level < 7:
-                level += 1
-            if level not in ico:
-                v0, f0 = scenes.icosphere(level)
-                ico[level] = (torch.as_tensor(v0, dtype=torch.float32, device=device),
-                              torch.as_tensor(f0, dtype=torch.int32, device=device))
-            v0, f0 = ico[level]
-            k = torch.randint(1, 6, (3,), generator=g, device=device).float()
-            ph = 6.283 * torch.rand(3, generator=g, device=device)
-            r = 1.0 + 0.05 * torch.prod(torch.sin(k * v0 + ph), dim=1)
-            v = 0.5 + 0.45 * v0 * r[:, None]
-            f = f0
-            extra = c - f0.shape[0]
-            if extra > 0:
-                sv, sf = soup(extra)
-                f = torch.cat([f0, sf + v.shape[0]])
-                v = torch.cat([v, sv])
-        else:
-            v, f = soup(c)
-        verts.append(v)
-        tris_local.append(f)
-        offs.append(offs[-1] + c)
-        vbase.append(vbase[-1] + v.shape[0])
-    V = torch.cat(verts).contiguous()
-    T = torch.cat([f + vb for f, vb in zip(tris_local, vbase[:-1])]).contiguous()
-    # Rays: per mesh round(per_1k * c / 1000) (at least 1), grouped by mesh.
-    rc = np.maximum(1, np.rint(args.batch_rays_per_1k * counts / 1000.0)).astype(np.int64)
-    owner = torch.as_tensor(np.repeat(np.arange(M, dtype=np.int32), rc), device=device)
-    R = int(rc.sum())
-    u = torch.randn((R, 3), generator=g, device=device)
-    u = u / u.norm(dim=1, keepdim=True)
-    rad = 0.75 * math.sqrt(3.0) + 1e-3
-    o = 0.5 + rad * u
-    tgt = torch.rand((R, 3), generator=g, device=device)
-    d = tgt - o
-    d = d / d.norm(dim=1, keepdim=True)
-    rays = torch.empty((R, 8), dtype=torch.float32, device=device)
-    rays[:, 0:3] = o
-    rays[:, 3] = 0.0
-    rays[:, 4:7] = d
-    rays[:, 7] = 3.0 * rad
-    roffs = np.concatenate([[0], np.cumsum(rc)])
-    return dict(V=V, T=T, offs=np.array(offs, np.int64), vbase=np.array(vbase, np.int64), counts=counts,
-                tris_local=tris_local, rays=rays, owner=owner, roffs=roffs)
+    """Config 3 geometry and rays, generated on the device (hippie_amd.workloads)."""
+    from hippie_amd import workloads
+    return workloads.batched_build_trace(device, args.batch_meshes, args.batch_lo, args.batch_hi,
+                                         args.batch_rays_per_1k)
 
 
 def bench_diff_single(ctx, args):
@@ -295,40 +228,17 @@ def bench_config5(ctx, args):
     meshes in proportion to their triangles; against the per-mesh loop of the
     same cycle.  The vertices are refit in place each step (same work as a
     deformation)."""
-    import math
-    import numpy as np
     import torch
     import hippie_amd as hx
-    from hippie_amd import scenes
+    from hippie_amd import workloads
     from hippie_amd.diff import backward as single_backward
     dev = torch.device("cuda", 0)
     t0 = time.perf_counter()
-    rng = np.random.default_rng(5)
-    M, R = args.c5_meshes, args.c5_rays
-    counts = np.rint(np.exp(rng.uniform(math.log(1000), math.log(200000), M))).astype(np.int64)
-    meshes = []
-    for c in counts:
-        if rng.random() < 0.3:
-            meshes.append(scenes.sliver_soup(int(c), rng))
-        else:
-            meshes.append(scenes.batch_mesh(int(c), rng, bool(rng.random() < 0.5)))
+    w = workloads.batched_presplit_diff(dev, args.c5_meshes, args.c5_rays)
+    M, meshes, counts, rays, owner = args.c5_meshes, w["meshes"], w["counts"], w["rays"], w["owner"]
     mb = hx.MeshBatch.from_meshes(meshes)
     N = int(counts.sum())
-    per = np.maximum(1, np.rint(counts * (R / N))).astype(np.int64)
-    R = int(per.sum())
-    owner = torch.repeat_interleave(torch.arange(M, device=dev, dtype=torch.int32),
-                                    torch.as_tensor(per, device=dev))
-    g = torch.Generator(device=dev)
-    g.manual_seed(5)
-    d = torch.randn((R, 3), generator=g, device=dev)
-    o = 0.5 + 1.5 * d / d.norm(dim=1, keepdim=True)
-    tgt = torch.rand((R, 3), generator=g, device=dev)
-    dirn = tgt - o
-    dirn = dirn / dirn.norm(dim=1, keepdim=True)
-    rays = torch.zeros((R, 8), dtype=torch.float32, device=dev)
-    rays[:, 0:3] = o
-    rays[:, 4:7] = dirn
-    rays[:, 7] = 4.0
+    R = int(rays.shape[0])
     torch.cuda.synchronize()
     log(f"config 5 workload: {M} meshes, {N} triangles, {R} rays ({time.perf_counter() - t0:.1f} s)")
     params = dict(method="ploc", ploc_radius=25, presplitting=1, presplitter_beta=0.3)
@@ -375,7 +285,7 @@ def bench_config5(ctx, args):
     builder = hx.PLOCBuilder(ctx, **{k: v for k, v in params.items() if k != "method"})
     scenes_m = [hx.Scene(v, t) for v, t in meshes]
     bvhs = [builder.build(sc) for sc in scenes_m]
-    roffs = np.concatenate([[0], np.cumsum(per)])
+    roffs = w["roffs"]
     res_m = [res[int(roffs[m]):int(roffs[m + 1])] for m in range(M)]
     rays_m = [rays[int(roffs[m]):int(roffs[m + 1])] for m in range(M)]
     dl_m = [dl_dt[int(roffs[m]):int(roffs[m + 1])] for m in range(M)]

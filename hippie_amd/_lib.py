@@ -99,8 +99,8 @@ SIGNATURES = {
     "hx_context_get_tuning": [_p, ctypes.c_char_p, ctypes.POINTER(_i32)],
     "hx_build_params_default": [ctypes.POINTER(HxBuildParams)],
     "hx_build": [_p, ctypes.POINTER(HxBuildParams), _p, _i64, _p, _i64, _pp],
-    "hx_rebuild": [_p, _p, _p, _p],
-    "hx_refit": [_p, _p, _p, _p],
+    "hx_rebuild": [_p, _p, _p, _i64, _p, _i64],
+    "hx_refit": [_p, _p, _p, _i64, _p, _i64],
     "hx_bvh_destroy": [_p],
     "hx_bvh_get_view": [_p, ctypes.POINTER(HxBvhView)],
     "hx_bvh_debug": [_p, ctypes.POINTER(HxBvhDebugView)],
@@ -114,12 +114,15 @@ SIGNATURES = {
     "hx_sort_pairs_u64": [_p, _p, _p, _i64, _i32],
     "hx_trace": [_p, _p, _p, _i64, _i32, _p, _p],
     "hx_trace_sorted": [_p, _p, _p, _i64, _i32, _i32, _i32, _f32, _p, _p, _p],
+    "hx_trace_sorted_timed": [_p, _p, _p, _i64, _i32, _i32, _i32, _f32, _p, _p, _p, ctypes.POINTER(_f32),
+                              ctypes.POINTER(_f32)],
+    "hx_trace_view": [_p, _p, _i64, _p, _p, _i64, _p, _i64, _i32, _p, _p, _p],
     "hx_ray_keys": [_p, _p, _i64, _i32, _i32, _f32, _p],
     "hx_memcpy": [_p, _p, _p, _i64],
     "hx_backward": [_p, _p, _p, _p, _p, _i64, _p, _p, _p, _p],
     "hx_build_batch": [_p, _p, _p, _i64, _p, _p, _i32, _p],
-    "hx_batch_refit": [_p, _p, _p, _p],
-    "hx_batch_rebuild": [_p, _p, _p, _p],
+    "hx_batch_refit": [_p, _p, _p, _i64, _p, _i64],
+    "hx_batch_rebuild": [_p, _p, _p, _i64, _p, _i64],
     "hx_batch_destroy": [_p],
     "hx_batch_mesh": [_p, _i32, _p],
     "hx_batch_trace": [_p, _p, _p, _p, _i64, _i32, _p, _p],

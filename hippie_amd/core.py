@@ -127,6 +127,13 @@ class Scene:
         return int(self.vertices.shape[0])
 
 
+def _same_geometry(old, new):
+    """A BVH's buffers are sized for the geometry it was built from."""
+    if new.n_triangles != old.n_triangles or tuple(new.vertices.shape) != tuple(old.vertices.shape):
+        raise ValueError(f"geometry of {new.n_triangles} triangles / {new.n_vertices} vertices does not match the "
+                         f"BVH's {old.n_triangles} triangles / {old.n_vertices} vertices")
+
+
 class BuildParams:
     """hx_build_params with the reference's Bvh.* defaults (AppEnvironment.cpp:52-75)."""
 
@@ -285,9 +292,12 @@ class HipBVH:
         return v.value
 
     def update(self, scene=None):
-        """HipBVH::update -- refit + re-woopify with unchanged topology."""
+        """HipBVH::update -- refit + re-woopify with unchanged topology.  The
+        scene must have the triangle and vertex counts of the build."""
         s = scene or self.scene
-        call("hx_refit", self.ctx.handle, self.handle, _ptr(s.vertices), _ptr(s.triangles))
+        _same_geometry(self.scene, s)
+        call("hx_refit", self.ctx.handle, self.handle, _ptr(s.vertices), s.n_vertices, _ptr(s.triangles),
+             s.n_triangles)
         self.scene = s  # the BVH keeps the refit geometry's device pointers
 
     def __del__(self):
@@ -328,8 +338,12 @@ class BVHBuilder:
         return HipBVH(self.ctx, h, scene, self.params)
 
     def rebuild(self, bvh, scene=None):
+        """BVHBuilder::rebuild: the reference rebuilds from the BVH's own scene;
+        a new scene must have the same triangle and vertex counts."""
         s = scene or bvh.scene
-        call("hx_rebuild", self.ctx.handle, bvh.handle, _ptr(s.vertices), _ptr(s.triangles))
+        _same_geometry(bvh.scene, s)
+        call("hx_rebuild", self.ctx.handle, bvh.handle, _ptr(s.vertices), s.n_vertices, _ptr(s.triangles),
+             s.n_triangles)
         bvh.scene = s
 
 
@@ -461,6 +475,33 @@ class HipTracer:
              _ptr(rays.results), None, _ptr(rays.indices))
         return rays.results
 
+    def trace_sort_timed(self, rays, method="aila", bits=60, ray_length=0.25):
+        """HipTracer::traceSort(rays, sortTime, traceTime): (sort seconds, trace
+        seconds), from events around the ray keys + sort and around the trace."""
+        self._need_bvh()
+        if rays.indices is None:
+            rays.indices = torch.zeros(rays.size, dtype=torch.int32, device=rays.rays.device)
+        ts, tt = ctypes.c_float(), ctypes.c_float()
+        call("hx_trace_sorted_timed", self.ctx.handle, self.bvh.handle, _ptr(rays.rays), rays.size,
+             int(not rays.closest_hit), RAY_KEY_METHODS[method], int(bits), float(ray_length),
+             _ptr(rays.results), None, _ptr(rays.indices), ctypes.byref(ts), ctypes.byref(tt))
+        return ts.value, tt.value
+
+
+def trace_view(ctx, nodes, woop, tri_indices, rays, any_hit=False, stats=None, indices=None, results=None):
+    """TRACE_FUNC over caller-held Bin BVH tensors (int32 (N,16) nodes, float32
+    (R,12) Woop rows, int32 triangle indices); rays (n,8) float32 on the device.
+    indices: optional int32 ray order (slot i traces ray indices[i])."""
+    for name, t in (("nodes", nodes), ("woop", woop), ("tri_indices", tri_indices), ("rays", rays)):
+        if not t.is_cuda or not t.is_contiguous():
+            raise ValueError(f"{name} must be a contiguous CUDA tensor")
+    n = int(rays.shape[0])
+    if results is None:
+        results = torch.zeros((n, 4), dtype=torch.int32, device=rays.device)
+    call("hx_trace_view", ctx.handle, _ptr(nodes), int(nodes.shape[0]), _ptr(woop), _ptr(tri_indices),
+         int(tri_indices.shape[0]), _ptr(rays), n, int(bool(any_hit)), _ptr(results), _ptr(stats), _ptr(indices))
+    return results
+
 
 def normalize_vertices(ctx, vertices):
     """SceneLoader's load-time normalisation, in place on a (V,3) float32 CUDA
@@ -565,7 +606,8 @@ class HipBatch:
 
     def rebuild(self):
         s = self.batch.scene
-        call("hx_batch_rebuild", self.ctx.handle, self.handle, _ptr(s.vertices), _ptr(s.triangles))
+        call("hx_batch_rebuild", self.ctx.handle, self.handle, _ptr(s.vertices), s.n_vertices, _ptr(s.triangles),
+             s.n_triangles)
 
     def info(self, key):
         """Build statistic of the whole batch (hx_bvh_info keys, e.g. "references")."""
@@ -574,10 +616,16 @@ class HipBatch:
         return v.value
 
     def refit(self, vertices=None):
+        """Refit every mesh to new vertex positions of the batch's shape."""
         s = self.batch.scene
         if vertices is not None:
-            s.vertices = torch.as_tensor(vertices).to(device=s.vertices.device, dtype=torch.float32).contiguous()
-        call("hx_batch_refit", self.ctx.handle, self.handle, _ptr(s.vertices), _ptr(s.triangles))
+            v = torch.as_tensor(vertices).to(device=s.vertices.device, dtype=torch.float32).contiguous()
+            if tuple(v.shape) != tuple(s.vertices.shape):
+                raise ValueError(f"vertices of shape {tuple(v.shape)} do not match the batch's "
+                                 f"{tuple(s.vertices.shape)}")
+            s.vertices = v
+        call("hx_batch_refit", self.ctx.handle, self.handle, _ptr(s.vertices), s.n_vertices, _ptr(s.triangles),
+             s.n_triangles)
 
     def trace(self, rays, ray_mesh, any_hit=False, results=None):
         """rays: RayBuffer or (R,8) tensor on the device; ray_mesh: int32 (R,) on the device."""

@@ -933,27 +933,102 @@ int read_box(hx_bvh* b) {
     return HX_OK;
 }
 
-int trace_common(hx_context* ctx, hx_bvh* bvh, const float* d_rays, int64_t n_rays, int32_t any_hit,
-                 int32_t* d_results, int32_t* d_stats, const int32_t* d_order) {
-    HX_CHECK_ARG(ctx && bvh, "context or bvh is NULL");
-    HX_CHECK_ARG(n_rays >= 0, "n_rays must be >= 0");
-    if (n_rays == 0) return HX_OK;
-    HX_CHECK_ARG(d_rays && d_results, "rays and results must be device pointers");
-    Context* c = &ctx->c;
-    HX_TRY(c->control.reserve(64));
+// One persistent trace over Bin BVH arrays (TRACE_FUNC's arguments); the ray
+// counter and overflow count live in `control` (2 int32).
+int trace_arrays(Context* c, const void* nodes, const void* woop, const int32_t* tri_idx, const float* d_rays,
+                 int64_t n_rays, int32_t any_hit, int32_t* d_results, int32_t* d_stats, const int32_t* d_order,
+                 int32_t* control) {
     hx::TraceArgs a{};
     a.n_rays = n_rays;
     a.any_hit = any_hit ? 1 : 0;
     a.rays = reinterpret_cast<const float4*>(d_rays);
     a.results = reinterpret_cast<int4*>(d_results);
-    a.nodes = bvh->nodes.as<int4>();
-    a.woop = bvh->woop.as<float4>();
-    a.tri_idx = bvh->tri_idx.as<int32_t>();
+    a.nodes = static_cast<const int4*>(nodes);
+    a.woop = static_cast<const float4*>(woop);
+    a.tri_idx = tri_idx;
     a.stats = reinterpret_cast<int2*>(d_stats);
     a.order = d_order;
-    a.control = c->control.as<int32_t>();
+    a.control = control;
     a.fetch_threshold = c->tune.trace_fetch;
     return hx::launch_trace(c, a);
+}
+
+int trace_common(hx_context* ctx, hx_bvh* bvh, const float* d_rays, int64_t n_rays, int32_t any_hit,
+                 int32_t* d_results, int32_t* d_stats, const int32_t* d_order) {
+    HX_CHECK_ARG(ctx && bvh, "context or bvh is NULL");
+    HX_CHECK_ARG(n_rays >= 0 && n_rays < ((int64_t)1 << 30), "n_rays must be in [0, 2^30)");
+    if (n_rays == 0) return HX_OK;
+    HX_CHECK_ARG(d_rays && d_results, "rays and results must be device pointers");
+    Context* c = &ctx->c;
+    HX_TRY(c->control.reserve(64));
+    return trace_arrays(c, bvh->nodes.ptr, bvh->woop.ptr, bvh->tri_idx.as<int32_t>(), d_rays, n_rays, any_hit,
+                        d_results, d_stats, d_order, c->control.as<int32_t>());
+}
+
+// Ray keys and their sort into `order` (the sorted ray indices), from the
+// context's buffers or, with ws, from caller memory (keys[2], order[2]).
+int ray_sort(Context* c, const float* d_rays, int64_t n_rays, int32_t key_method, int32_t key_bits, float ray_length,
+             void* keys[2], int32_t* order[2], const hx::SortWorkspace* ws, const int32_t** sorted) {
+    HX_TRY(hx::launch_ray_keys(c, reinterpret_cast<const float4*>(d_rays), n_rays, key_method, key_bits, ray_length,
+                               static_cast<uint64_t*>(keys[0])));
+    int ri = 0;
+    HX_TRY(hx::launch_sort_pairs(c, keys, order, n_rays, key_bits, true, true, &ri, false, ws));
+    *sorted = order[ri];
+    return HX_OK;
+}
+
+int check_ray_keys(int32_t key_method, int32_t key_bits) {
+    HX_CHECK_ARG(key_method == HX_RAYKEY_AILA || key_method == HX_RAYKEY_TWOPOINT, "unknown ray key method %d",
+                 key_method);
+    HX_CHECK_ARG(key_bits >= 1 && key_bits <= 64, "key_bits must be in [1,64]");
+    return HX_OK;
+}
+
+// hx_trace_sorted with optional event timing of its two halves.
+int trace_sorted_common(hx_context* ctx, hx_bvh* bvh, const float* d_rays, int64_t n_rays, int32_t any_hit,
+                        int32_t key_method, int32_t key_bits, float ray_length, int32_t* d_results, int32_t* d_stats,
+                        int32_t* d_order, float* sort_seconds, float* trace_seconds) {
+    HX_CHECK_ARG(ctx && bvh, "context or bvh is NULL");
+    HX_CHECK_ARG(n_rays >= 0 && n_rays < ((int64_t)1 << 30), "n_rays must be in [0, 2^30)");
+    HX_TRY(check_ray_keys(key_method, key_bits));
+    if (sort_seconds) *sort_seconds = 0.0f;
+    if (trace_seconds) *trace_seconds = 0.0f;
+    if (n_rays == 0) return HX_OK;
+    HX_CHECK_ARG(d_rays && d_results, "rays and results must be device pointers");
+    Context* c = &ctx->c;
+    for (int k = 0; k < 2; ++k) {
+        HX_TRY(c->ray_keys[k].reserve(8 * (size_t)n_rays));
+        HX_TRY(c->ray_order[k].reserve(4 * (size_t)n_rays));
+    }
+    const bool timed = sort_seconds || trace_seconds;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    if (timed) {
+        for (hipEvent_t& e : ev) HX_HIP(hipEventCreate(&e));
+        HX_HIP(hipEventRecord(ev[0], c->stream));
+    }
+    void* keys[2] = {c->ray_keys[0].ptr, c->ray_keys[1].ptr};
+    int32_t* order[2] = {c->ray_order[0].as<int32_t>(), c->ray_order[1].as<int32_t>()};
+    const int32_t* sorted = nullptr;
+    int st = ray_sort(c, d_rays, n_rays, key_method, key_bits, ray_length, keys, order, nullptr, &sorted);
+    if (st == HX_OK && timed) st = hipEventRecord(ev[1], c->stream) == hipSuccess ? HX_OK : HX_ERR_HIP;
+    if (st == HX_OK) st = trace_common(ctx, bvh, d_rays, n_rays, any_hit, d_results, d_stats, sorted);
+    if (st == HX_OK && timed) st = hipEventRecord(ev[2], c->stream) == hipSuccess ? HX_OK : HX_ERR_HIP;
+    if (st == HX_OK && d_order &&
+        hipMemcpyAsync(d_order, sorted, 4 * (size_t)n_rays, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
+        st = HX_ERR_HIP;
+    if (st == HX_OK && timed) {
+        float ms_sort = 0.0f, ms_trace = 0.0f;
+        if (hipEventSynchronize(ev[2]) != hipSuccess || hipEventElapsedTime(&ms_sort, ev[0], ev[1]) != hipSuccess ||
+            hipEventElapsedTime(&ms_trace, ev[1], ev[2]) != hipSuccess) {
+            hx::set_error("trace timing: event query failed");
+            st = HX_ERR_HIP;
+        }
+        if (sort_seconds) *sort_seconds = 1e-3f * ms_sort;
+        if (trace_seconds) *trace_seconds = 1e-3f * ms_trace;
+    }
+    for (hipEvent_t e : ev)
+        if (e) (void)hipEventDestroy(e);
+    return st;
 }
 
 }  // namespace
@@ -1149,18 +1224,32 @@ int hx_build(hx_context* ctx, const hx_build_params* params, const float* d_vert
     return HX_OK;
 }
 
-int hx_rebuild(hx_context* ctx, hx_bvh* bvh, const float* d_vertices, const int32_t* d_triangles) {
+// The BVH's buffers are sized for the geometry it was built from: new
+// geometry must have the same triangle and vertex counts.
+int check_same_geometry(const hx_bvh* b, int64_t n_vertices, int64_t n_triangles) {
+    HX_CHECK_ARG(n_triangles == b->n_tri, "geometry has %lld triangles, the BVH was built for %lld",
+                 (long long)n_triangles, (long long)b->n_tri);
+    HX_CHECK_ARG(n_vertices == b->n_verts, "geometry has %lld vertices, the BVH was built for %lld",
+                 (long long)n_vertices, (long long)b->n_verts);
+    return HX_OK;
+}
+
+int hx_rebuild(hx_context* ctx, hx_bvh* bvh, const float* d_vertices, int64_t n_vertices, const int32_t* d_triangles,
+               int64_t n_triangles) {
     HX_CHECK_ARG(ctx && bvh, "context or bvh is NULL");
     HX_CHECK_ARG(d_vertices && d_triangles, "geometry pointers invalid");
+    HX_TRY(check_same_geometry(bvh, n_vertices, n_triangles));
     bvh->ctx = ctx;
     bvh->verts = d_vertices;
     bvh->tris = d_triangles;
     return run_build(bvh);
 }
 
-int hx_refit(hx_context* ctx, hx_bvh* bvh, const float* d_vertices, const int32_t* d_triangles) {
+int hx_refit(hx_context* ctx, hx_bvh* bvh, const float* d_vertices, int64_t n_vertices, const int32_t* d_triangles,
+             int64_t n_triangles) {
     HX_CHECK_ARG(ctx && bvh && d_vertices && d_triangles, "NULL argument");
     HX_CHECK_ARG(bvh->built, "the BVH has not been built");
+    HX_TRY(check_same_geometry(bvh, n_vertices, n_triangles));
     bvh->ctx = ctx;
     bvh->verts = d_vertices;
     bvh->tris = d_triangles;
@@ -1241,18 +1330,22 @@ int hx_build_batch(hx_context* ctx, const hx_build_params* params, const float* 
     return HX_OK;
 }
 
-int hx_batch_refit(hx_context* ctx, hx_batch* batch, const float* d_vertices, const int32_t* d_triangles) {
+int hx_batch_refit(hx_context* ctx, hx_batch* batch, const float* d_vertices, int64_t n_vertices,
+                   const int32_t* d_triangles, int64_t n_triangles) {
     HX_CHECK_ARG(ctx && batch && d_vertices && d_triangles, "NULL argument");
     hx_bvh* b = &batch->bvh;
+    HX_TRY(check_same_geometry(b, n_vertices, n_triangles));
     b->ctx = ctx;
     b->verts = d_vertices;
     b->tris = d_triangles;
     return run_refit(b);
 }
 
-int hx_batch_rebuild(hx_context* ctx, hx_batch* batch, const float* d_vertices, const int32_t* d_triangles) {
+int hx_batch_rebuild(hx_context* ctx, hx_batch* batch, const float* d_vertices, int64_t n_vertices,
+                     const int32_t* d_triangles, int64_t n_triangles) {
     HX_CHECK_ARG(ctx && batch && d_vertices && d_triangles, "NULL argument");
     hx_bvh* b = &batch->bvh;
+    HX_TRY(check_same_geometry(b, n_vertices, n_triangles));
     b->ctx = ctx;
     b->verts = d_vertices;
     b->tris = d_triangles;
@@ -1291,7 +1384,7 @@ int hx_batch_mesh(hx_batch* batch, int32_t mesh, hx_batch_mesh_view* out) {
 int hx_batch_trace(hx_context* ctx, hx_batch* batch, const float* d_rays, const int32_t* d_ray_mesh,
                    int64_t n_rays, int32_t any_hit, int32_t* d_results, int32_t* d_stats) {
     HX_CHECK_ARG(ctx && batch && d_ray_mesh, "NULL argument");
-    HX_CHECK_ARG(n_rays >= 0, "n_rays must be >= 0");
+    HX_CHECK_ARG(n_rays >= 0 && n_rays < ((int64_t)1 << 30), "n_rays must be in [0, 2^30)");
     if (n_rays == 0) return HX_OK;
     HX_CHECK_ARG(d_rays && d_results, "rays and results must be device pointers");
     hx_bvh* b = &batch->bvh;
@@ -1511,9 +1604,7 @@ int hx_trace(hx_context* ctx, hx_bvh* bvh, const float* d_rays, int64_t n_rays, 
 int hx_ray_keys(hx_context* ctx, const float* d_rays, int64_t n_rays, int32_t key_method, int32_t key_bits,
                 float ray_length, uint64_t* d_keys) {
     HX_CHECK_ARG(ctx && d_rays && d_keys && n_rays >= 0, "invalid argument");
-    HX_CHECK_ARG(key_method == HX_RAYKEY_AILA || key_method == HX_RAYKEY_TWOPOINT, "unknown ray key method %d",
-                 key_method);
-    HX_CHECK_ARG(key_bits >= 1 && key_bits <= 64, "key_bits must be in [1,64]");
+    HX_TRY(check_ray_keys(key_method, key_bits));
     return hx::launch_ray_keys(&ctx->c, reinterpret_cast<const float4*>(d_rays), n_rays, key_method, key_bits,
                                ray_length, d_keys);
 }
@@ -1521,24 +1612,33 @@ int hx_ray_keys(hx_context* ctx, const float* d_rays, int64_t n_rays, int32_t ke
 int hx_trace_sorted(hx_context* ctx, hx_bvh* bvh, const float* d_rays, int64_t n_rays, int32_t any_hit,
                     int32_t key_method, int32_t key_bits, float ray_length, int32_t* d_results, int32_t* d_stats,
                     int32_t* d_order) {
-    HX_CHECK_ARG(ctx && bvh, "context or bvh is NULL");
+    return trace_sorted_common(ctx, bvh, d_rays, n_rays, any_hit, key_method, key_bits, ray_length, d_results,
+                               d_stats, d_order, nullptr, nullptr);
+}
+
+int hx_trace_sorted_timed(hx_context* ctx, hx_bvh* bvh, const float* d_rays, int64_t n_rays, int32_t any_hit,
+                          int32_t key_method, int32_t key_bits, float ray_length, int32_t* d_results,
+                          int32_t* d_stats, int32_t* d_order, float* sort_seconds, float* trace_seconds) {
+    HX_CHECK_ARG(sort_seconds && trace_seconds, "sort_seconds and trace_seconds must be host pointers");
+    return trace_sorted_common(ctx, bvh, d_rays, n_rays, any_hit, key_method, key_bits, ray_length, d_results,
+                               d_stats, d_order, sort_seconds, trace_seconds);
+}
+
+int hx_trace_view(hx_context* ctx, const void* d_nodes, int64_t n_nodes, const void* d_woop,
+                  const int32_t* d_tri_indices, int64_t n_references, const float* d_rays, int64_t n_rays,
+                  int32_t any_hit, int32_t* d_results, int32_t* d_stats, const int32_t* d_indices) {
+    HX_CHECK_ARG(ctx, "context is NULL");
     HX_CHECK_ARG(n_rays >= 0 && n_rays < ((int64_t)1 << 30), "n_rays must be in [0, 2^30)");
+    HX_CHECK_ARG(d_nodes && d_woop && d_tri_indices, "nodes, woop triangles and triangle indices must be device "
+                                                     "pointers");
+    HX_CHECK_ARG(n_nodes >= 2 && n_references >= 1, "a Bin BVH has at least 2 nodes and 1 reference (got %lld, %lld)",
+                 (long long)n_nodes, (long long)n_references);
     if (n_rays == 0) return HX_OK;
+    HX_CHECK_ARG(d_rays && d_results, "rays and results must be device pointers");
     Context* c = &ctx->c;
-    for (int k = 0; k < 2; ++k) {
-        HX_TRY(c->ray_keys[k].reserve(8 * (size_t)n_rays));
-        HX_TRY(c->ray_order[k].reserve(4 * (size_t)n_rays));
-    }
-    HX_TRY(hx_ray_keys(ctx, d_rays, n_rays, key_method, key_bits, ray_length, c->ray_keys[0].as<uint64_t>()));
-    void* keys[2] = {c->ray_keys[0].ptr, c->ray_keys[1].ptr};
-    int32_t* vals[2] = {c->ray_order[0].as<int32_t>(), c->ray_order[1].as<int32_t>()};
-    int ri = 0;
-    HX_TRY(hx::launch_sort_pairs(c, keys, vals, n_rays, key_bits, true, true, &ri, false));
-    const int32_t* order = vals[ri];
-    HX_TRY(trace_common(ctx, bvh, d_rays, n_rays, any_hit, d_results, d_stats, order));
-    if (d_order)
-        HX_HIP(hipMemcpyAsync(d_order, order, 4 * (size_t)n_rays, hipMemcpyDeviceToDevice, c->stream));
-    return HX_OK;
+    HX_TRY(c->control.reserve(64));
+    return trace_arrays(c, d_nodes, d_woop, d_tri_indices, d_rays, n_rays, any_hit, d_results, d_stats, d_indices,
+                        c->control.as<int32_t>());
 }
 
 int hx_memcpy(hx_context* ctx, void* dst, const void* src, int64_t bytes) {

@@ -83,6 +83,7 @@ struct Context {
     Profiler prof;
     // Scratch reused across calls (grow-only).
     DevBuf sort_status, sort_hist, scan_partials, control;
+    DevBuf sort_flag;                  // the split sort's overflow flag (gates its fallback passes)
     DevBuf ray_keys[2], ray_order[2];  // sorted trace
     DevBuf tmp_keys, tmp_vals;         // hx_sort_pairs_u64 ping-pong
     DevBuf segments;                   // relabelling: node ranges for the single-workgroup pass
@@ -423,17 +424,28 @@ int launch_tri_local(Context* c, const int32_t* sorted_tri, const int32_t* mesh_
 // hx_sort.hip
 // Number of passes a sort of n keys over bits [0, end_bit) takes (digit width
 // is the "sort_digit_bits" tunable; a split sort counts its four high passes
-// and the local pass); the result lands in buffer (passes & 1) unless a split
-// sort falls back to the low passes.
+// and the local pass, and its result lands in buffer 1 whether or not its
+// gated fallback passes run); the result lands in buffer (passes & 1).
 int sort_pass_count(const Context* c, int end_bit, int64_t n, bool keys64);
+// Caller-provided sort status memory (look-back words, histograms, tile
+// counters) instead of the context's grow-only buffer; no split sort.
+struct SortWorkspace {
+    uint32_t* status;
+    int64_t status_words;
+};
+// Status words a sort of n keys over [0, end_bit) needs with the context's
+// digit width and keys per thread (no split sort).
+int sort_status_words_max(const Context* c, int64_t n, int end_bit);
 // Stable LSD sort of (key, value) over bits [0, end_bit).  keys/vals ping-pong
 // between [0] and [1]; returns in *result_index which buffer holds the output.
 // vals_identity: the input values are 0..n-1 and are not read.  allow_split:
 // long keys may take the split sort (sort_local_run), which pays off when runs
 // of equal high bits are short (Morton codes of geometry), not for ray keys
-// whose origins coincide (a pinhole camera's).
+// whose origins coincide (a pinhole camera's).  ws: status memory of the
+// caller (nothing is allocated).
 int launch_sort_pairs(Context* c, void* keys[2], int32_t* vals[2], int64_t n, int end_bit, bool keys64,
-                      bool vals_identity, int* result_index, bool allow_split = true);
+                      bool vals_identity, int* result_index, bool allow_split = true,
+                      const SortWorkspace* ws = nullptr);
 
 // hx_trace.hip
 struct TraceArgs {

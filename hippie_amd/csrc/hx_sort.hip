@@ -34,7 +34,13 @@ constexpr uint32_t kValueMask = (1u << 30) - 1u;
 inline int sort_passes(int end_bit, int rb) { return (end_bit + rb - 1) / rb; }
 inline int64_t sort_tiles(int64_t n, int items) { return (n + kSortBlock * items - 1) / (kSortBlock * items); }
 
-__global__ void k_zero_u32(uint32_t* p, int64_t n) {
+// `gate` (may be NULL) switches a launch off on the device: the kernels of a
+// sort's fallback passes are always enqueued and return at once unless the
+// local pass set *gate, so the host never waits to choose between them.
+HX_DEV bool gated_off(const int32_t* gate) { return gate != nullptr && *gate == 0; }
+
+__global__ void k_zero_u32(uint32_t* p, int64_t n, const int32_t* gate) {
+    if (gated_off(gate)) return;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         p[i] = 0u;
 }
@@ -58,7 +64,9 @@ constexpr int kHistUnroll = 4;
 
 template <class K, int RB>
 __global__ __launch_bounds__(kHistBlock) void k_sort_hist(const K* __restrict__ keys, int64_t n, int lo_bit,
-                                                          int passes, int end_bit, uint32_t* __restrict__ hist) {
+                                                          int passes, int end_bit, uint32_t* __restrict__ hist,
+                                                          const int32_t* gate) {
+    if (gated_off(gate)) return;
     constexpr int R = 1 << RB;
     constexpr int P = (8 * (int)sizeof(K) + RB - 1) / RB;
     __shared__ uint32_t h[P][R];
@@ -110,7 +118,8 @@ __global__ __launch_bounds__(kHistBlock) void k_sort_hist(const K* __restrict__ 
 // Exclusive scan of each pass's R digit counts, in place; each thread owns
 // R / 256 consecutive digits.
 template <int RB>
-__global__ __launch_bounds__(kSortBlock) void k_sort_scan_hist(uint32_t* __restrict__ hist) {
+__global__ __launch_bounds__(kSortBlock) void k_sort_scan_hist(uint32_t* __restrict__ hist, const int32_t* gate) {
+    if (gated_off(gate)) return;
     constexpr int R = 1 << RB;
     constexpr int DPT = R / kSortBlock;
     __shared__ uint32_t wt[kSortWaves];
@@ -136,7 +145,9 @@ __global__ __launch_bounds__(kSortBlock) __attribute__((amdgpu_waves_per_eu(IT <
                                                          int32_t* __restrict__ vals_out, int64_t n, int shift,
                                                          int nbits, const uint32_t* __restrict__ digit_offsets,
                                                          uint32_t* __restrict__ status,
-                                                         uint32_t* __restrict__ tile_counter) {
+                                                         uint32_t* __restrict__ tile_counter,
+                                                         const int32_t* gate) {
+    if (gated_off(gate)) return;
     constexpr int R = 1 << RB;
     constexpr int DPT = R / kSortBlock;  // digits per thread in the per-digit phase
     constexpr int kTileKeys = kSortBlock * IT;
@@ -327,10 +338,11 @@ __global__ __launch_bounds__(kSortBlock) __attribute__((amdgpu_waves_per_eu(IT <
     }
 }
 
-// The low-bit pass of a sort whose high bits [lo_bit, end) are already sorted
-// stably: each key is ranked inside its run of equal high bits by its low
-// bits, ties by position (so the order stays stable), and written to run
-// start + rank.  A run longer than `limit` keys sets *overflow and nothing of
+// The low-bit pass of a sort whose high bits [lo_bit, end_bit) are already
+// sorted stably: each key is ranked inside its run of equal high bits by its
+// low bits, ties by position (so the order stays stable), and written to run
+// start + rank.  Key bits at and above end_bit are not part of the sort key
+// and are ignored, as the digit passes ignore them.  A run longer than `limit` keys sets *overflow and nothing of
 // it is written; the caller then runs the remaining digit passes instead.
 // Each workgroup stages its kLocalTile keys and `limit` keys either side in
 // LDS, which covers every run it ranks.
@@ -341,7 +353,7 @@ __global__ __launch_bounds__(kSortBlock) void k_sort_local(const uint64_t* __res
                                                            const int32_t* __restrict__ vals_in,
                                                            uint64_t* __restrict__ keys_out,
                                                            int32_t* __restrict__ vals_out, int64_t n, int lo_bit,
-                                                           int limit, int32_t* __restrict__ overflow) {
+                                                           int end_bit, int limit, int32_t* __restrict__ overflow) {
     extern __shared__ uint64_t s_keys[];  // [base - limit, base + kLocalTile + limit)
     const int64_t base = blockIdx.x * (int64_t)kLocalTile;
     const int64_t lo = base - limit;
@@ -360,22 +372,24 @@ __global__ __launch_bounds__(kSortBlock) void k_sort_local(const uint64_t* __res
     }
     __syncthreads();
     const uint64_t low_mask = (1ull << lo_bit) - 1ull;
+    const uint64_t key_mask = end_bit >= 64 ? ~0ull : (1ull << end_bit) - 1ull;
+    auto high = [&](uint64_t k) { return (k & key_mask) >> lo_bit; };
 #pragma unroll
     for (int u = 0; u < kLocalTile / kSortBlock; ++u) {
         const int64_t p = base + u * kSortBlock + threadIdx.x;
         if (p >= n) break;
         const int lp = (int)(p - lo);
         const uint64_t k = s_keys[lp];
-        const uint64_t hi = k >> lo_bit;
+        const uint64_t hi = high(k);
         // A run of more than limit keys starting at p reaches p + limit: one
         // probe finds every such run at its first key without walking it.
-        if (p + limit < n && (s_keys[lp + limit] >> lo_bit) == hi) {
+        if (p + limit < n && high(s_keys[lp + limit]) == hi) {
             atomicOr(overflow, 1);
             continue;
         }
         int ls = lp, le = lp + 1;
-        while (lo + ls > 0 && ls > 0 && (s_keys[ls - 1] >> lo_bit) == hi) --ls;
-        while (lo + le < n && le < span && (s_keys[le] >> lo_bit) == hi) ++le;
+        while (lo + ls > 0 && ls > 0 && high(s_keys[ls - 1]) == hi) --ls;
+        while (lo + le < n && le < span && high(s_keys[le]) == hi) ++le;
         if (le - ls > limit || (ls == 0 && lo > 0) || (le == span && lo + le < n)) {
             atomicOr(overflow, 1);  // longer than limit (or reaching past the staged keys)
             continue;
@@ -396,16 +410,37 @@ __global__ void k_sort_flag_clear(int32_t* flag) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *flag = 0;
 }
 
+// Gated copy of a fallback sort's output into the buffers the local pass
+// writes, so that both paths leave the result in the same place.
+__global__ void k_sort_copy_gated(const uint64_t* __restrict__ ks, const int32_t* __restrict__ vs,
+                                  uint64_t* __restrict__ kd, int32_t* __restrict__ vd, int64_t n,
+                                  const int32_t* gate) {
+    if (gated_off(gate)) return;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        kd[i] = ks[i];
+        vd[i] = vs[i];
+    }
+}
+
+// Status words (look-back flags, digit histograms, tile counters) of an LSD
+// sort of n keys over `bits` key bits.
+template <int RB, int IT>
+int64_t sort_status_words(int64_t n, int bits) {
+    const int passes = sort_passes(bits, RB);
+    return (int64_t)passes * sort_tiles(n, IT) * (1 << RB) + (int64_t)passes * (1 << RB) + kMaxPasses;
+}
+
 // LSD passes over key bits [lo_bit, end_bit) from buffers 0; *result = the
-// buffer index holding the output.
+// buffer index holding the output.  With a gate every launch is enqueued and
+// does its work only if *gate is set (status must then be reserved already).
 template <class K, int RB, int IT, int LB>
 int sort_impl(Context* c, void* keys[2], int32_t* vals[2], int64_t n, int lo_bit, int end_bit, bool vals_identity,
-              int* result) {
+              int* result, const int32_t* gate = nullptr, const SortWorkspace* ws = nullptr) {
     constexpr int R = 1 << RB;
     const int passes = sort_passes(end_bit - lo_bit, RB);
     *result = 0;
     if (n <= 1 || passes == 0) {
-        if (vals_identity && n > 0) {
+        if (vals_identity && n > 0 && gate == nullptr) {
             hipLaunchKernelGGL(k_iota, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, vals[0], n);
             HX_TRY(check_launch("iota"));
         }
@@ -414,21 +449,35 @@ int sort_impl(Context* c, void* keys[2], int32_t* vals[2], int64_t n, int lo_bit
     const int64_t ntiles = sort_tiles(n, IT);
     const int64_t status_words = (int64_t)passes * ntiles * R;
     const int64_t words = status_words + (int64_t)passes * R + kMaxPasses;
-    HX_TRY(c->sort_status.reserve(sizeof(uint32_t) * (size_t)words));
-    uint32_t* status = c->sort_status.as<uint32_t>();
+    uint32_t* status = nullptr;
+    if (ws != nullptr) {
+        if (ws->status_words < words) {
+            set_error("radix sort: %lld keys need %lld status words, the workspace holds %lld", (long long)n,
+                      (long long)words, (long long)ws->status_words);
+            return HX_ERR_INVALID_ARG;
+        }
+        status = ws->status;
+    } else {
+        if (gate != nullptr && c->sort_status.bytes < sizeof(uint32_t) * (size_t)words) {
+            set_error("radix sort: gated passes need their status words reserved in advance");
+            return HX_ERR_INVALID_ARG;
+        }
+        HX_TRY(c->sort_status.reserve(sizeof(uint32_t) * (size_t)words));
+        status = c->sort_status.as<uint32_t>();
+    }
     uint32_t* hist = status + status_words;
     uint32_t* counters = hist + passes * R;
     {
         LaunchScope ls(c, "sort_hist");
         const unsigned zb = (unsigned)std::min<int64_t>((words + 255) / 256, 4096);
-        hipLaunchKernelGGL(k_zero_u32, dim3(zb), dim3(256), 0, c->stream, status, words);
+        hipLaunchKernelGGL(k_zero_u32, dim3(zb), dim3(256), 0, c->stream, status, words, gate);
         HX_TRY(check_launch("sort_zero"));
         const unsigned hb = (unsigned)std::max<int64_t>(
             1, std::min<int64_t>((n + kHistBlock * kHistUnroll - 1) / (kHistBlock * kHistUnroll), c->cu_count));
         hipLaunchKernelGGL((k_sort_hist<K, RB>), dim3(hb), dim3(kHistBlock), 0, c->stream,
-                           static_cast<const K*>(keys[0]), n, lo_bit, passes, end_bit, hist);
+                           static_cast<const K*>(keys[0]), n, lo_bit, passes, end_bit, hist, gate);
         HX_TRY(check_launch("sort_hist"));
-        hipLaunchKernelGGL(k_sort_scan_hist<RB>, dim3(passes), dim3(kSortBlock), 0, c->stream, hist);
+        hipLaunchKernelGGL(k_sort_scan_hist<RB>, dim3(passes), dim3(kSortBlock), 0, c->stream, hist, gate);
         HX_TRY(check_launch("sort_scan_hist"));
     }
     for (int p = 0; p < passes; ++p) {
@@ -439,11 +488,12 @@ int sort_impl(Context* c, void* keys[2], int32_t* vals[2], int64_t n, int lo_bit
         if (p == 0 && vals_identity)
             hipLaunchKernelGGL((k_onesweep<K, true, RB, IT, LB>), dim3((unsigned)ntiles), dim3(kSortBlock), 0, c->stream,
                                static_cast<const K*>(keys[in]), static_cast<K*>(keys[out]), (const int32_t*)nullptr,
-                               vals[out], n, shift, nb, hist + p * R, status + (int64_t)p * ntiles * R, counters + p);
+                               vals[out], n, shift, nb, hist + p * R, status + (int64_t)p * ntiles * R, counters + p,
+                               gate);
         else
             hipLaunchKernelGGL((k_onesweep<K, false, RB, IT, LB>), dim3((unsigned)ntiles), dim3(kSortBlock), 0, c->stream,
                                static_cast<const K*>(keys[in]), static_cast<K*>(keys[out]), vals[in], vals[out], n,
-                               shift, nb, hist + p * R, status + (int64_t)p * ntiles * R, counters + p);
+                               shift, nb, hist + p * R, status + (int64_t)p * ntiles * R, counters + p, gate);
         HX_TRY(check_launch("sort_onesweep"));
     }
     *result = passes & 1;
@@ -452,19 +502,26 @@ int sort_impl(Context* c, void* keys[2], int32_t* vals[2], int64_t n, int lo_bit
 
 constexpr int64_t kLocalMinKeys = 1 << 18;
 
-// High passes, the local pass into buffers 1, then (on overflow) the low
-// passes over the high passes' output instead.  A stable LSD sort of any
-// order in which equal keys keep their input order gives the same result, so
-// the fallback may start from the partly sorted buffers.
+// High passes, then the local pass into buffers 1.  A run longer than the
+// local pass ranks sets a flag on the device; the low passes over the high
+// passes' output are always enqueued behind it and run only when the flag is
+// set (a stable LSD sort of any order in which equal keys keep their input
+// order gives the same result, so they may start from the partly sorted
+// buffers), followed by a gated copy that puts their output where the local
+// pass writes.  The host never reads the flag: no synchronisation, and a
+// build stays capturable in a graph.
 template <int RB, int IT, int LB>
 int sort_split(Context* c, void* keys[2], int32_t* vals[2], int64_t n, int lo_bit, int end_bit, bool vals_identity,
                int* result) {
+    // Every buffer the gated passes touch is reserved before anything runs.
+    const int64_t words = std::max(sort_status_words<RB, IT>(n, end_bit), sort_status_words<RB, IT>(n, end_bit - lo_bit));
+    HX_TRY(c->sort_status.reserve(sizeof(uint32_t) * (size_t)words));
+    HX_TRY(c->sort_flag.reserve(64));
+    int32_t* flag = c->sort_flag.as<int32_t>();
     int hi_res = 0;
     HX_TRY((sort_impl<uint64_t, RB, IT, LB>(c, keys, vals, n, lo_bit, end_bit, vals_identity, &hi_res)));
     void* k2[2] = {keys[hi_res], keys[hi_res ^ 1]};
     int32_t* v2[2] = {vals[hi_res], vals[hi_res ^ 1]};
-    HX_TRY(c->sort_status.reserve(sizeof(uint32_t) * 64));
-    int32_t* flag = reinterpret_cast<int32_t*>(c->sort_status.as<uint32_t>());
     {
         LaunchScope ls(c, "sort_local");
         hipLaunchKernelGGL(k_sort_flag_clear, dim3(1), dim3(64), 0, c->stream, flag);
@@ -473,19 +530,18 @@ int sort_split(Context* c, void* keys[2], int32_t* vals[2], int64_t n, int lo_bi
         hipLaunchKernelGGL(k_sort_local, dim3((unsigned)((n + kLocalTile - 1) / kLocalTile)), dim3(kSortBlock),
                            sizeof(uint64_t) * (size_t)(kLocalTile + 2 * limit), c->stream,
                            static_cast<const uint64_t*>(k2[0]), v2[0], static_cast<uint64_t*>(k2[1]), v2[1], n,
-                           lo_bit, limit, flag);
+                           lo_bit, end_bit, limit, flag);
         HX_TRY(check_launch("sort_local"));
     }
-    int32_t h_flag = 0;
-    HX_HIP(hipMemcpyAsync(&h_flag, flag, sizeof(h_flag), hipMemcpyDeviceToHost, c->stream));
-    HX_HIP(hipStreamSynchronize(c->stream));
-    if (!h_flag) {
-        *result = hi_res ^ 1;
-        return HX_OK;
-    }
     int lo_res = 0;
-    HX_TRY((sort_impl<uint64_t, RB, IT, LB>(c, k2, v2, n, 0, end_bit, false, &lo_res)));
-    *result = lo_res == 0 ? hi_res : hi_res ^ 1;
+    HX_TRY((sort_impl<uint64_t, RB, IT, LB>(c, k2, v2, n, 0, end_bit, false, &lo_res, flag)));
+    if (lo_res == 0) {
+        const unsigned g = (unsigned)std::min<int64_t>((n + 255) / 256, 4096);
+        hipLaunchKernelGGL(k_sort_copy_gated, dim3(g), dim3(256), 0, c->stream, static_cast<const uint64_t*>(k2[0]),
+                           v2[0], static_cast<uint64_t*>(k2[1]), v2[1], n, (const int32_t*)flag);
+        HX_TRY(check_launch("sort_copy_gated"));
+    }
+    *result = hi_res ^ 1;
     return HX_OK;
 }
 
@@ -498,8 +554,14 @@ int sort_pass_count(const Context* c, int end_bit, int64_t n, bool keys64) {
     return sort_passes(end_bit, bits);
 }
 
+int sort_status_words_max(const Context* c, int64_t n, int end_bit) {
+    const int rb = c->tune.sort_digit_bits, it = c->tune.sort_items;
+    const int passes = sort_passes(end_bit, rb);
+    return (int)((int64_t)passes * sort_tiles(n, it) * (1 << rb) + (int64_t)passes * (1 << rb) + kMaxPasses);
+}
+
 int launch_sort_pairs(Context* c, void* keys[2], int32_t* vals[2], int64_t n, int end_bit, bool keys64,
-                      bool vals_identity, int* result_index, bool allow_split) {
+                      bool vals_identity, int* result_index, bool allow_split, const SortWorkspace* ws) {
     if (n >= (int64_t)1 << 30) {
         set_error("radix sort: n = %lld exceeds 2^30 - 1", (long long)n);
         return HX_ERR_INVALID_ARG;
@@ -510,12 +572,16 @@ int launch_sort_pairs(Context* c, void* keys[2], int32_t* vals[2], int64_t n, in
     // the 16.7M-triangle benchmark scene at 32 high bits), which one pass
     // ranks in place; a long run falls back to the remaining passes.
     const int lo_bit = end_bit - 4 * bits;
-    const bool split = allow_split && keys64 && c->tune.sort_local_run > 0 && n >= kLocalMinKeys && lo_bit >= bits;
+    const bool split = allow_split && ws == nullptr && keys64 && c->tune.sort_local_run > 0 && n >= kLocalMinKeys &&
+                       lo_bit >= bits;
 #define HX_SORT_CASE(RB, IT, LB)                                                                                   \
     if (bits == RB && items == IT && lb == LB) {                                                                  \
         if (split) return sort_split<RB, IT, LB>(c, keys, vals, n, lo_bit, end_bit, vals_identity, result_index); \
-        if (keys64) return sort_impl<uint64_t, RB, IT, LB>(c, keys, vals, n, 0, end_bit, vals_identity, result_index); \
-        return sort_impl<uint32_t, RB, IT, LB>(c, keys, vals, n, 0, end_bit, vals_identity, result_index);           \
+        if (keys64)                                                                                                \
+            return sort_impl<uint64_t, RB, IT, LB>(c, keys, vals, n, 0, end_bit, vals_identity, result_index,     \
+                                                   nullptr, ws);                                                  \
+        return sort_impl<uint32_t, RB, IT, LB>(c, keys, vals, n, 0, end_bit, vals_identity, result_index, nullptr, \
+                                               ws);                                                               \
     }
     HX_SORT_CASE(8, 16, 4)
     HX_SORT_CASE(8, 16, 8)

@@ -35,7 +35,11 @@ class _ClosestHit(torch.autograd.Function):
     def forward(fctx, vertices, tracer, rays):
         tr = tracer
         v = vertices.detach().contiguous()
-        call("hx_refit", tr.ctx.handle, tr.bvh.handle, _ptr(v), _ptr(tr.triangles))
+        if tuple(v.shape) != tuple(tr.scene.vertices.shape):
+            raise ValueError(f"vertices of shape {tuple(v.shape)} do not match the traced mesh's "
+                             f"{tuple(tr.scene.vertices.shape)}")
+        call("hx_refit", tr.ctx.handle, tr.bvh.handle, _ptr(v), int(v.shape[0]), _ptr(tr.triangles),
+             int(tr.triangles.shape[0]))
         call("hx_trace", tr.ctx.handle, tr.bvh.handle, _ptr(rays.rays), rays.size, 0, _ptr(rays.results), None)
         res = rays.results.clone()
         n = rays.size

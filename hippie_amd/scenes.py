@@ -161,6 +161,23 @@ def secondary_rays(verts, tris, rays, results, seed=3, offset=1e-4, tmax=3.0):
     return out
 
 
+def ray_key_edge_rays(n, seed):
+    """Random rays plus the cases the ray keys' slab test special-cases:
+    origins outside the unit box, rays leaving or missing it, axis-parallel
+    directions (zero components), origins on box faces, negative and zero tmax."""
+    rng = np.random.default_rng(seed)
+    r = random_rays(n, seed=seed).copy()
+    k = n // 6
+    r[:k, 0:3] = rng.uniform(-1.0, 2.0, (k, 3))
+    ax = rng.integers(0, 3, 2 * k)
+    r[k:3 * k, 4:7] = 0.0
+    r[np.arange(k, 3 * k), 4 + ax] = rng.choice([-1.0, 1.0], 2 * k)
+    r[3 * k:4 * k, rng.integers(0, 3)] = rng.choice([0.0, 1.0], k)
+    r[4 * k:5 * k, 7] = rng.uniform(-1.0, 0.3, k)
+    r[5 * k:5 * k + 16, 7] = 0.0
+    return r.astype(np.float32)
+
+
 def icosphere(level):
     """Unit icosphere: 20 * 4^level triangles, 10 * 4^level + 2 shared vertices."""
     p = (1.0 + 5 ** 0.5) / 2.0

@@ -165,12 +165,16 @@ int hx_build_params_default(hx_build_params* p);
 /* BVHBuilder::build: allocates a new BVH owned by the caller (hx_bvh_destroy). */
 int hx_build(hx_context* ctx, const hx_build_params* params, const float* d_vertices, int64_t n_vertices,
              const int32_t* d_triangles, int64_t n_triangles, hx_bvh** out);
-/* BVHBuilder::rebuild: rebuild in place from (possibly moved) geometry. */
-int hx_rebuild(hx_context* ctx, hx_bvh* bvh, const float* d_vertices, const int32_t* d_triangles);
+/* BVHBuilder::rebuild: rebuild in place from (possibly moved) geometry with
+ * the vertex and triangle counts of the first build (else
+ * HX_ERR_INVALID_ARG: the BVH's buffers are sized for them). */
+int hx_rebuild(hx_context* ctx, hx_bvh* bvh, const float* d_vertices, int64_t n_vertices, const int32_t* d_triangles,
+               int64_t n_triangles);
 /* HipBVH::update (HipBVH.cpp:472-477; refitLeaves/refitInteriors,
  * HipBVHKernels.cu:240-326): refit the final layout and re-woopify from new
- * vertex positions, topology unchanged. */
-int hx_refit(hx_context* ctx, hx_bvh* bvh, const float* d_vertices, const int32_t* d_triangles);
+ * vertex positions, topology unchanged; the counts must equal the build's. */
+int hx_refit(hx_context* ctx, hx_bvh* bvh, const float* d_vertices, int64_t n_vertices, const int32_t* d_triangles,
+             int64_t n_triangles);
 int hx_bvh_destroy(hx_bvh* bvh);
 /* Synchronises on first call after a build to read the node counts. */
 int hx_bvh_get_view(hx_bvh* bvh, hx_bvh_view* out);
@@ -219,6 +223,24 @@ int hx_trace(hx_context* ctx, hx_bvh* bvh, const float* d_rays, int64_t n_rays, 
 int hx_trace_sorted(hx_context* ctx, hx_bvh* bvh, const float* d_rays, int64_t n_rays, int32_t any_hit,
                     int32_t key_method, int32_t key_bits, float ray_length, int32_t* d_results,
                     int32_t* d_stats, int32_t* d_order);
+/* HipTracer::traceSort(rays, sortTime, traceTime) (HipTracer.h:41,
+ * HipTracer.cpp:114-121): hx_trace_sorted, then waits for it and returns the
+ * seconds spent in the ray keys + sort and in the trace, from events recorded
+ * on the context's stream around each half. */
+int hx_trace_sorted_timed(hx_context* ctx, hx_bvh* bvh, const float* d_rays, int64_t n_rays, int32_t any_hit,
+                          int32_t key_method, int32_t key_bits, float ray_length, int32_t* d_results,
+                          int32_t* d_stats, int32_t* d_order, float* sort_seconds, float* trace_seconds);
+/* The kernel ABI TRACE_FUNC (HipTracerKernels.h:29-39: numRays, anyHit, rays,
+ * results, nodes, tris, stats, indices, triIndices) over caller-held Bin BVH
+ * arrays, e.g. a HipBVH's buffers built by other code: d_nodes (n_nodes Bin
+ * nodes, root at 0), d_woop (48 B per reference slot), d_tri_indices (slot ->
+ * triangle id, n_references entries).  The arrays must form a valid Bin BVH
+ * (hx_bvh_validate's checks); they are not checked on the device.  d_indices
+ * (optional, int32[n_rays]) is traceSort's ray order: slot i traces ray
+ * d_indices[i] and writes its result (and stats) at that index. */
+int hx_trace_view(hx_context* ctx, const void* d_nodes, int64_t n_nodes, const void* d_woop,
+                  const int32_t* d_tri_indices, int64_t n_references, const float* d_rays, int64_t n_rays,
+                  int32_t any_hit, int32_t* d_results, int32_t* d_stats, const int32_t* d_indices);
 /* RayBuffer::computeMortonCodes (RayBuffer.cpp:31-85) for the [0,1]^3 ray box. */
 int hx_ray_keys(hx_context* ctx, const float* d_rays, int64_t n_rays, int32_t key_method, int32_t key_bits,
                 float ray_length, uint64_t* d_keys);
@@ -268,10 +290,12 @@ typedef struct hx_batch_mesh_view {
 } hx_batch_mesh_view;
 int hx_build_batch(hx_context* ctx, const hx_build_params* params, const float* d_vertices, int64_t n_vertices,
                    const int32_t* d_triangles, const int64_t* h_mesh_offsets, int32_t n_meshes, hx_batch** out);
-/* Rebuild every mesh from new vertices/triangles (same offsets), reusing the
- * batch's memory. */
-int hx_batch_rebuild(hx_context* ctx, hx_batch* batch, const float* d_vertices, const int32_t* d_triangles);
-int hx_batch_refit(hx_context* ctx, hx_batch* batch, const float* d_vertices, const int32_t* d_triangles);
+/* Rebuild every mesh from new vertices/triangles (same offsets and counts),
+ * reusing the batch's memory. */
+int hx_batch_rebuild(hx_context* ctx, hx_batch* batch, const float* d_vertices, int64_t n_vertices,
+                     const int32_t* d_triangles, int64_t n_triangles);
+int hx_batch_refit(hx_context* ctx, hx_batch* batch, const float* d_vertices, int64_t n_vertices,
+                   const int32_t* d_triangles, int64_t n_triangles);
 int hx_batch_destroy(hx_batch* batch);
 /* Synchronises on first call after a build to read the mesh tables. */
 int hx_batch_mesh(hx_batch* batch, int32_t mesh, hx_batch_mesh_view* out);

@@ -5,10 +5,13 @@ Gaussian-mixture soup + 2048x2048 ground grid).  The device build must give
 the oracle's tree (LBVHBuilder.cpp:183-263 through BVHCollapser.cpp:521-675) in
 canonical form, pass the device-side HipBVH::validate, and trace a sample of
 primary and diffuse rays (HipTracerKernels) to the oracle's hits, bit for bit;
-the Morton-sorted trace of the same rays returns the same hits.  The other
-builders at this size (where the oracle's builds take minutes) are checked
-through size-independent properties: a valid tree and the same closest hits as
-the LBVH, since the closest hit and its tie rule do not depend on the tree.
+the Morton-sorted trace of the same rays returns the same hits.  The
+benchmark's PLOC (radius 25) and ATRBVH (treelet 9, 2 iterations) builds of
+the same scene must equal the oracle's full-size builds in canonical form,
+with the SAH cost within 0.1 % (the contract; the FP64 sums agree far
+closer).  Every builder at this size is also checked through size-independent
+properties: a valid tree and the same closest hits as the LBVH, since the
+closest hit and its tie rule do not depend on the tree.
 """
 import numpy as np
 import pytest
@@ -83,3 +86,34 @@ def test_builders_full_size_valid_and_same_hits(ctx, full_scene, method, prespli
     if presplit:
         assert bvh.view().n_references > t.shape[0]
     assert _same_hits(_trace(ctx, bvh, rays), want)
+
+
+def _scene_area(bvh):
+    sb = bvh.scene_box()
+    d = sb[3:6] - sb[0:3]
+    return float(np.float32(2.0) * (d[0] * d[1] + d[1] * d[2] + d[2] * d[0]))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("method", ["ploc", "atr"])
+def test_config2_ploc_atr_full_size_match_oracle(oracle, ctx, full_scene, method):
+    """Config 2's PLOC r = 25 and ATRBVH t = 9 x 2 at 16,777,216 triangles."""
+    v, t = full_scene
+    print(f"full-size {method} vs the oracle's full-size build", flush=True)
+    if method == "ploc":
+        bvh = hx.PLOCBuilder(ctx, morton_bits=60, ploc_radius=25).build(hx.Scene(v, t))
+        ref = oracle.build_ploc(v, t, bits=60, radius=25)
+    else:
+        bvh = hx.ATRBuilder(ctx, morton_bits=60, atr_treelet_size=9, atr_iterations=2).build(hx.Scene(v, t))
+        ref = oracle.build_atr(v, t, bits=60, treelet_size=9, iterations=2)
+    assert bvh.validate() == 0
+    c_ref = oracle.cost(ref["nodes"], _scene_area(bvh))
+    c_gpu = bvh.get_cost()
+    print(f"{method}: SAH {c_gpu:.6f} (oracle {c_ref:.6f}, rel {abs(c_gpu - c_ref) / c_ref:.2e})", flush=True)
+    assert abs(c_gpu - c_ref) <= 1e-3 * abs(c_ref), "SAH within 0.1 %"
+    assert (bvh.n_interior, bvh.n_leaves) == (ref["n_interior"], ref["n_leaves"])
+    nodes, tri = bvh.nodes_host(), bvh.tri_indices_host()
+    cg = oracle.bin_canonical(nodes, bvh.n_interior, bvh.n_leaves, tri)
+    co = oracle.bin_canonical(ref["nodes"], ref["n_interior"], ref["n_leaves"], ref["tri_idx"])
+    assert np.array_equal(cg, co), "canonical Bin tree"
+    assert np.array_equal(bvh.woop_host().view(np.uint32), oracle.woopify(v, t, tri).view(np.uint32)), "Woop rows"

@@ -206,6 +206,26 @@ def test_sort_split_runs(ctx, run, limit):
     assert np.array_equal(vt.cpu().numpy(), vals[order])
 
 
+@pytest.mark.parametrize("end_bit", [40, 52, 60, 63])
+def test_sort_split_ignores_bits_above_end_bit(ctx, end_bit):
+    """Key bits at and above end_bit are not part of the sort key: junk there
+    must not split a run of equal [lo_bit, end_bit) bits in the local pass."""
+    n = 300007
+    rng = np.random.default_rng(end_bit)
+    mask = np.uint64((1 << end_bit) - 1)
+    k = rng.integers(0, 1 << 20, n, dtype=np.uint64) << np.uint64(end_bit - 20)
+    k |= rng.integers(0, 4, n, dtype=np.uint64)
+    junk = rng.integers(0, 1 << (64 - end_bit), n, dtype=np.uint64) << np.uint64(end_bit)
+    kj = k | junk
+    vals = rng.permutation(n).astype(np.int32)
+    kt = torch.from_numpy(kj.view(np.int64).copy()).cuda()
+    vt = torch.from_numpy(vals.copy()).cuda()
+    hx.sort_pairs(ctx, kt, vt, end_bit)
+    order = np.argsort(kj & mask, kind="stable")
+    assert np.array_equal(vt.cpu().numpy(), vals[order])
+    assert np.array_equal(kt.cpu().numpy().view(np.uint64), kj[order])
+
+
 def test_lbvh_split_sort_off(oracle, ctx):
     """The LBVH build with every digit pass (sort_local_run 0) and with the
     split sort gives the same tree."""
