@@ -22,6 +22,8 @@ HX_METHOD_PLOC = 2
 HX_METHOD_ATRBVH = 3
 HX_METHOD_INSERTION = 4
 HX_LAYOUT_BIN = 2
+HX_LAYOUT_QUAD = 4
+HX_LAYOUT_OCT = 8
 HX_RAYKEY_TWOPOINT = 0
 HX_RAYKEY_AILA = 1
 

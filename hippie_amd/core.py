@@ -26,6 +26,7 @@ METHODS = {
     "atr": _lib.HX_METHOD_ATRBVH,
     "insertion": _lib.HX_METHOD_INSERTION,
 }
+LAYOUTS = {"bin": _lib.HX_LAYOUT_BIN, "quad": _lib.HX_LAYOUT_QUAD, "oct": _lib.HX_LAYOUT_OCT}
 RAY_KEY_METHODS = {"twopoint": _lib.HX_RAYKEY_TWOPOINT, "aila": _lib.HX_RAYKEY_AILA}
 
 
@@ -175,8 +176,18 @@ class BuildParams:
 
     @classmethod
     def from_env(cls, env):
-        """Map Bvh.* key/value pairs of a reference .env block onto parameters."""
+        """Map the Bvh.* options of a reference .env file onto parameters.
+        ``env`` is a formats.Environment (typed values and the reference's
+        defaults) or a dict of Bvh.* key/value pairs."""
+        from .formats import Environment
+        if isinstance(env, Environment):
+            env = env.as_dict("Bvh.")
+        layout = str(env.get("Bvh.layout", "bin")).lower()
         p = cls()
+        if layout != "bin":
+            if layout not in LAYOUTS:
+                raise ValueError(f"unknown Bvh.layout '{layout}'")
+            p.layout = LAYOUTS[layout]
         method = str(env.get("Bvh.method", "lbvh"))
         p.method = method
         if method in cls.BITS_KEYS and cls.BITS_KEYS[method] in env:

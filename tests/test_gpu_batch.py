@@ -14,6 +14,8 @@ import hippie_amd as hx
 from hippie_amd import scenes
 from hippie_amd.diff import backward as single_backward
 
+from adjoint_ref import adjoint_terms_fp64, assert_scatter_within_bound
+
 pytestmark = pytest.mark.gpu
 
 
@@ -84,15 +86,19 @@ def test_batch_trace_refit_backward(oracle, ctx):
     # Adjoint: batched scatter equals the single-mesh adjoint mesh by mesh.
     res_t = hb.trace(d_rays, d_owner)
     dl_dt = torch.rand(len(rays), device="cuda", dtype=torch.float32)
-    grad = hb.backward(d_rays, d_owner, res_t, dl_dt=dl_dt).cpu().numpy()
+    grad = hb.backward(d_rays, d_owner, res_t, dl_dt=dl_dt)
+    g64, S, K = adjoint_terms_fp64(mb.scene.vertices, mb.scene.triangles, d_rays, res_t, dl_dt, owner=d_owner,
+                                   tri_offsets=mb.offsets[:-1])
+    assert_scatter_within_bound(grad, g64, S, K, "batched adjoint")
     base = 0
     for m, (v, t) in enumerate(meshes):
         sel = torch.as_tensor(owner == m, device="cuda")
         vm = torch.as_tensor(v2[base:base + len(v)], device="cuda")
         tm = torch.as_tensor(t, device="cuda")
         gm = single_backward(ctx, vm, tm, d_rays[sel].contiguous(), res_t[sel].contiguous(),
-                             dl_dt=dl_dt[sel].contiguous()).cpu().numpy()
-        np.testing.assert_allclose(grad[base:base + len(v)], gm, rtol=1e-5, atol=1e-6)
+                             dl_dt=dl_dt[sel].contiguous())
+        sl = slice(3 * base, 3 * (base + len(v)))
+        assert_scatter_within_bound(gm, g64[sl], S[sl], K[sl], f"mesh {m} adjoint")
         base += len(v)
 
 
@@ -151,7 +157,11 @@ def test_batch_diff_step_graph_replays_match_eager(ctx):
         g.replay()
         torch.cuda.synchronize()
         assert torch.equal(res, res_e), f"replay {k}: results"
-        torch.testing.assert_close(grad, grad_e, rtol=1e-5, atol=1e-6)
+        # Atomic float32 scatter: both within the summation bound of the FP64 sum.
+        g64, S, K = adjoint_terms_fp64(mb.scene.vertices, mb.scene.triangles, d_rays, res, dl_dt, owner=d_owner,
+                                       tri_offsets=mb.offsets[:-1])
+        assert_scatter_within_bound(grad_e, g64, S, K, f"step {k} eager")
+        assert_scatter_within_bound(grad, g64, S, K, f"step {k} replay")
 
 
 def _check_meshes_presplit(oracle, hb, meshes, bits, beta):
@@ -222,15 +232,19 @@ def test_batch_presplit_trace_refit_backward(oracle, ctx):
     # Adjoint: batched scatter equals the single-mesh adjoint mesh by mesh.
     res_t = hb.trace(d_rays, d_owner)
     dl_dt = torch.rand(len(rays), device="cuda", dtype=torch.float32)
-    grad = hb.backward(d_rays, d_owner, res_t, dl_dt=dl_dt).cpu().numpy()
+    grad = hb.backward(d_rays, d_owner, res_t, dl_dt=dl_dt)
+    g64, S, K = adjoint_terms_fp64(mb.scene.vertices, mb.scene.triangles, d_rays, res_t, dl_dt, owner=d_owner,
+                                   tri_offsets=mb.offsets[:-1])
+    assert_scatter_within_bound(grad, g64, S, K, "batched adjoint")
     base = 0
     for m, (v, t) in enumerate(meshes):
         sel = torch.as_tensor(owner == m, device="cuda")
         vm = torch.as_tensor(v2[base:base + len(v)], device="cuda")
         tm = torch.as_tensor(t, device="cuda")
         gm = single_backward(ctx, vm, tm, d_rays[sel].contiguous(), res_t[sel].contiguous(),
-                             dl_dt=dl_dt[sel].contiguous()).cpu().numpy()
-        np.testing.assert_allclose(grad[base:base + len(v)], gm, rtol=1e-5, atol=1e-6)
+                             dl_dt=dl_dt[sel].contiguous())
+        sl = slice(3 * base, 3 * (base + len(v)))
+        assert_scatter_within_bound(gm, g64[sl], S[sl], K[sl], f"mesh {m} adjoint")
         base += len(v)
 
 
