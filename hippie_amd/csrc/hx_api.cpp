@@ -1566,6 +1566,95 @@ int hx_normalize_vertices(hx_context* ctx, float* d_vertices, int64_t n_vertices
     return hx::launch_normalize(c, d_vertices, n_vertices, box);
 }
 
+int hx_init_seeds(hx_context* ctx, int64_t n, int32_t frame_index, uint32_t* d_seeds) {
+    HX_CHECK_ARG(ctx && n >= 0 && n < ((int64_t)1 << 31), "n must be in [0, 2^31)");
+    if (n == 0) return HX_OK;
+    HX_CHECK_ARG(d_seeds, "seeds must be a device pointer");
+    return hx::launch_init_seeds(&ctx->c, n, (uint32_t)frame_index, d_seeds);
+}
+
+int hx_primary_rays(hx_context* ctx, int32_t sample_index, const float* origin3, const float* screen_to_world16,
+                    int32_t width, int32_t height, float max_dist, const int32_t* d_index_to_pixel, float* d_rays) {
+    HX_CHECK_ARG(ctx && origin3 && screen_to_world16, "context, origin and matrix must not be NULL");
+    HX_CHECK_ARG(width > 0 && height > 0 && (int64_t)width * height < ((int64_t)1 << 31),
+                 "image size %d x %d is empty or too large", width, height);
+    HX_CHECK_ARG(sample_index >= 0, "sample_index must be >= 0");
+    HX_CHECK_ARG(d_rays, "rays must be a device pointer");
+    return hx::launch_primary_rays(&ctx->c, sample_index, origin3, screen_to_world16, width, height, max_dist,
+                                   d_index_to_pixel, d_rays);
+}
+
+int hx_shadow_rays(hx_context* ctx, int64_t batch_begin, int64_t batch_size, int32_t n_samples,
+                   const uint32_t* d_seeds, float light_radius, const float* light3, const float* d_in_rays,
+                   const int32_t* d_in_results, float* d_out_rays) {
+    HX_CHECK_ARG(ctx && light3, "context and light must not be NULL");
+    HX_CHECK_ARG(batch_begin >= 0 && batch_size >= 0 && n_samples >= 1, "invalid batch or sample count");
+    HX_CHECK_ARG(batch_size * (int64_t)n_samples < ((int64_t)1 << 31), "too many output rays");
+    if (batch_size == 0) return HX_OK;
+    HX_CHECK_ARG(d_seeds && d_in_rays && d_in_results && d_out_rays, "buffers must be device pointers");
+    return hx::launch_secondary_rays(&ctx->c, false, batch_begin, batch_size, n_samples, d_seeds, light_radius, light3,
+                                     0.0f, d_in_rays, d_in_results, nullptr, nullptr, d_out_rays);
+}
+
+int hx_ao_rays(hx_context* ctx, int64_t batch_begin, int64_t batch_size, int32_t n_samples, const uint32_t* d_seeds,
+               float max_dist, const float* d_in_rays, const int32_t* d_in_results, const int32_t* d_triangles,
+               const float* d_vertices, float* d_out_rays) {
+    HX_CHECK_ARG(ctx, "context is NULL");
+    HX_CHECK_ARG(batch_begin >= 0 && batch_size >= 0 && n_samples >= 1, "invalid batch or sample count");
+    HX_CHECK_ARG(batch_size * (int64_t)n_samples < ((int64_t)1 << 31), "too many output rays");
+    if (batch_size == 0) return HX_OK;
+    HX_CHECK_ARG(d_seeds && d_in_rays && d_in_results && d_triangles && d_vertices && d_out_rays,
+                 "buffers must be device pointers");
+    return hx::launch_secondary_rays(&ctx->c, true, batch_begin, batch_size, n_samples, d_seeds, 0.0f, nullptr,
+                                     max_dist, d_in_rays, d_in_results, d_triangles, d_vertices, d_out_rays);
+}
+
+int hx_path_rays(hx_context* ctx, int32_t russian_roulette, int64_t n_in, uint32_t* d_seeds,
+                 const int32_t* d_in_index_to_pixel, const int32_t* d_mat_indices, const int32_t* d_triangles,
+                 const float* d_normals, const void* d_materials, const float* d_in_rays, const int32_t* d_in_results,
+                 float* d_out_rays, int32_t* d_out_index_to_pixel, float* d_decreases, int32_t* d_n_out) {
+    HX_CHECK_ARG(ctx && n_in >= 0 && n_in < ((int64_t)1 << 31), "n_in must be in [0, 2^31)");
+    HX_CHECK_ARG(d_n_out, "n_out must be a device pointer");
+    if (n_in > 0)
+        HX_CHECK_ARG(d_seeds && d_in_index_to_pixel && d_mat_indices && d_triangles && d_normals && d_materials &&
+                         d_in_rays && d_in_results && d_out_rays && d_out_index_to_pixel && d_decreases,
+                     "buffers must be device pointers");
+    hx::PathRayArgs a{};
+    a.n = n_in;
+    a.russian_roulette = russian_roulette ? 1 : 0;
+    a.seeds = d_seeds;
+    a.in_pixel = d_in_index_to_pixel;
+    a.mat_indices = d_mat_indices;
+    a.tris = d_triangles;
+    a.normals = d_normals;
+    a.materials = d_materials;
+    a.in_rays = d_in_rays;
+    a.in_results = d_in_results;
+    a.out_rays = d_out_rays;
+    a.out_pixel = d_out_index_to_pixel;
+    a.decreases = d_decreases;
+    a.n_out = d_n_out;
+    return hx::launch_path_rays(&ctx->c, a);
+}
+
+int hx_reconstruct_occlusion(hx_context* ctx, int64_t batch_begin, int64_t batch_size, int32_t n_samples,
+                             int32_t replace, const int32_t* d_results, const int32_t* d_index_to_pixel,
+                             const int32_t* d_index_to_slot, const float* d_in_pixels, float* d_out_pixels) {
+    HX_CHECK_ARG(ctx && batch_begin >= 0 && batch_size >= 0 && n_samples >= 1, "invalid batch or sample count");
+    if (batch_size == 0) return HX_OK;
+    HX_CHECK_ARG(d_results && d_index_to_pixel && d_in_pixels && d_out_pixels, "buffers must be device pointers");
+    return hx::launch_reconstruct_occlusion(&ctx->c, batch_begin, batch_size, n_samples, replace != 0, d_results,
+                                            d_index_to_pixel, d_index_to_slot, d_in_pixels, d_out_pixels);
+}
+
+int hx_interpolate_vertices(hx_context* ctx, int64_t n_static, int64_t n_dynamic, float t, const float* d_frame0,
+                            const float* d_frame1, float* d_out, int32_t normalize) {
+    HX_CHECK_ARG(ctx && n_static >= 0 && n_dynamic >= 0, "invalid argument");
+    if (n_dynamic == 0) return HX_OK;
+    HX_CHECK_ARG(d_frame0 && d_frame1 && d_out, "buffers must be device pointers");
+    return hx::launch_interpolate(&ctx->c, n_static, n_dynamic, t, d_frame0, d_frame1, d_out, normalize != 0);
+}
+
 int hx_scene_box(hx_context* ctx, const float* d_vertices, const int32_t* d_triangles, int64_t n, float* d_box6) {
     HX_CHECK_ARG(ctx && d_vertices && d_triangles && d_box6 && n > 0, "invalid argument");
     return hx::launch_scene_box(&ctx->c, d_vertices, d_triangles, n, d_box6);

@@ -447,6 +447,37 @@ int launch_sort_pairs(Context* c, void* keys[2], int32_t* vals[2], int64_t n, in
                       bool vals_identity, int* result_index, bool allow_split = true,
                       const SortWorkspace* ws = nullptr);
 
+// hx_raygen.hip: ray generation, occlusion reconstruction, interpolation.
+int launch_init_seeds(Context* c, int64_t n, uint32_t frame, uint32_t* seeds);
+int launch_primary_rays(Context* c, int sample_index, const float origin[3], const float m[16], int width, int height,
+                        float max_dist, const int32_t* index_to_pixel, float* rays);
+// ao: AO rays (tris/verts used, light unused), else shadow rays.
+int launch_secondary_rays(Context* c, bool ao, int64_t begin, int64_t size, int n_samples, const uint32_t* seeds,
+                          float light_radius, const float light[3], float max_dist, const float* in_rays,
+                          const int32_t* in_results, const int32_t* tris, const float* verts, float* out_rays);
+struct PathRayArgs {
+    int64_t n;
+    int russian_roulette;
+    uint32_t* seeds;
+    const int32_t* in_pixel;
+    const int32_t* mat_indices;
+    const int32_t* tris;
+    const float* normals;
+    const void* materials;
+    const float* in_rays;
+    const int32_t* in_results;
+    float* out_rays;
+    int32_t* out_pixel;
+    float* decreases;
+    int32_t* n_out;
+};
+int launch_path_rays(Context* c, const PathRayArgs& p);
+int launch_reconstruct_occlusion(Context* c, int64_t begin, int64_t size, int n_samples, bool replace,
+                                 const int32_t* results, const int32_t* index_to_pixel, const int32_t* index_to_slot,
+                                 const float* in_pixels, float* out_pixels);
+int launch_interpolate(Context* c, int64_t n_static, int64_t n, float t, const float* a, const float* b, float* out,
+                       bool normalize);
+
 // hx_trace.hip
 struct TraceArgs {
     int64_t n_rays;

@@ -202,6 +202,57 @@ int hx_bvh_validate(hx_bvh* bvh, int32_t* failed);
  * box of all n_vertices vertices (AssimpImporter.cpp:99-113); d_box6 (device,
  * may be NULL) receives that box. */
 int hx_normalize_vertices(hx_context* ctx, float* d_vertices, int64_t n_vertices, float* d_box6);
+/* ---- the steps either side of a trace ----------------------------------------
+ * Rays are float4 (origin, tmin) + float4 (direction, tmax); results int4
+ * (id, t, u, v); pixels float4.  Every buffer is caller-owned device memory. */
+/* initSeeds (RayGenKernels.cu:520-535): seeds[i] = tea<16>(i, frame_index). */
+int hx_init_seeds(hx_context* ctx, int64_t n, int32_t frame_index, uint32_t* d_seeds);
+/* generatePrimaryRays (RayGenKernels.cu:106-153; RayGen::primary, RayGen.cpp:35-63):
+ * pixel p = index_to_pixel[i] (NULL: p = i) of a width x height image, offset
+ * by halton2D(sample_index), through screen_to_world (host, row-major 4x4:
+ * element (r, c) at 4r + c, the inverse of the camera's projection * view);
+ * origin (host, 3 floats), tmin 0, tmax max_dist.  Writes width*height rays. */
+int hx_primary_rays(hx_context* ctx, int32_t sample_index, const float* origin3, const float* screen_to_world16,
+                    int32_t width, int32_t height, float max_dist, const int32_t* d_index_to_pixel, float* d_rays);
+/* generateShadowRays (RayGenKernels.cu:155-229): n_samples rays per input ray
+ * batch_begin + i (i < batch_size) towards a light of radius light_radius
+ * around light3 (host), Sobol/Hammersley points with a Cranley-Patterson shift
+ * hashed from seeds[i]; written at n_samples * i + s.  A missed input gives
+ * rays with tmax -1. */
+int hx_shadow_rays(hx_context* ctx, int64_t batch_begin, int64_t batch_size, int32_t n_samples,
+                   const uint32_t* d_seeds, float light_radius, const float* light3, const float* d_in_rays,
+                   const int32_t* d_in_results, float* d_out_rays);
+/* generateAORays (RayGenKernels.cu:232-349): n_samples cosine-weighted Halton
+ * directions about the hit triangle's normal (flipped to face the ray), rotated
+ * by an angle hashed from seeds[i], from 1e-4 before the hit, tmax max_dist
+ * (-1 for a missed input). */
+int hx_ao_rays(hx_context* ctx, int64_t batch_begin, int64_t batch_size, int32_t n_samples, const uint32_t* d_seeds,
+               float max_dist, const float* d_in_rays, const int32_t* d_in_results, const int32_t* d_triangles,
+               const float* d_vertices, float* d_out_rays);
+/* generatePathRays (RayGenKernels.cu:354-519): one continuation per hit path
+ * (glossy or cosine lobe by the material's 32-byte record {int tex_index;
+ * float shininess; float diffuse[3]; float specular[3]}, smooth normals,
+ * LCG draws from seeds, which advance), written in input order (the reference
+ * appends in warp-atomic order); decreases (float3 per pixel) *= the path
+ * weight; *d_n_out = number of output rays.  Texture lookups are not part of
+ * this library: a material's diffuse colour is used as is. */
+int hx_path_rays(hx_context* ctx, int32_t russian_roulette, int64_t n_in, uint32_t* d_seeds,
+                 const int32_t* d_in_index_to_pixel, const int32_t* d_mat_indices, const int32_t* d_triangles,
+                 const float* d_normals, const void* d_materials, const float* d_in_rays, const int32_t* d_in_results,
+                 float* d_out_rays, int32_t* d_out_index_to_pixel, float* d_decreases, int32_t* d_n_out);
+/* reconstructAO / reconstructShadow (RendererKernels.cu:302-372): pixel
+ * index_to_pixel[batch_begin + i] gets in_pixels' colour times the fraction of
+ * its n_samples results (at index_to_slot[n_samples * i + s], NULL: identity)
+ * that miss; replace != 0 stores it, else adds it (w included). */
+int hx_reconstruct_occlusion(hx_context* ctx, int64_t batch_begin, int64_t batch_size, int32_t n_samples,
+                             int32_t replace, const int32_t* d_results, const int32_t* d_index_to_pixel,
+                             const int32_t* d_index_to_slot, const float* d_in_pixels, float* d_out_pixels);
+/* interpolateVertices / interpolateNormals (InterpolatorKernels.cu:10-46;
+ * Interpolator::updateAdaptive, Interpolator.cpp:21-95): out[n_static + i] =
+ * frame0[i] (1 - t) + frame1[i] t for i < n_dynamic (float3), normalised when
+ * normalize != 0; the refit input of an animated scene. */
+int hx_interpolate_vertices(hx_context* ctx, int64_t n_static, int64_t n_dynamic, float t, const float* d_frame0,
+                            const float* d_frame1, float* d_out, int32_t normalize);
 /* Scene box: exact min/max over the triangles' vertices (SceneLoader.cpp:143-153).
  * d_box6 receives mn.xyz, mx.xyz on the device. */
 int hx_scene_box(hx_context* ctx, const float* d_vertices, const int32_t* d_triangles, int64_t n,

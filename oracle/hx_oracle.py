@@ -70,6 +70,17 @@ def lib():
                                            _p, _p, _p, _p, _p, _p, _f32]),
             "hxo_ray_keys_aila64": (None, [_p, _i64, _i32, _p]),
             "hxo_ray_keys_twopoint64": (None, [_p, _i64, _i32, _f32, _p]),
+            "hxo_tea16": (None, [_p, _p, _i64, _p]),
+            "hxo_jenkins_mix": (None, [_p, _i64]),
+            "hxo_halton": (None, [_p, _i64, _i32, _p]),
+            "hxo_sobol2d": (None, [_p, _i64, _p]),
+            "hxo_init_seeds": (None, [_i64, _i32, _p]),
+            "hxo_primary_rays": (None, [_i32, _p, _p, _i32, _i32, _f32, _p, _p]),
+            "hxo_shadow_rays": (None, [_i64, _i64, _i32, _p, _f32, _p, _p, _p, _p]),
+            "hxo_ao_rays": (None, [_i64, _i64, _i32, _p, _f32, _p, _p, _p, _p, _p]),
+            "hxo_path_rays": (_i64, [_i32, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p]),
+            "hxo_reconstruct_occlusion": (None, [_i64, _i64, _i32, _i32, _p, _p, _p, _p, _p]),
+            "hxo_interpolate": (None, [_i64, _i64, _f32, _p, _p, _p, _i32]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -428,3 +439,105 @@ def ray_keys_twopoint64(rays, bits=60, ray_length=0.25):
     keys = np.zeros(r.shape[0], np.uint64)
     lib().hxo_ray_keys_twopoint64(_ptr(r), r.shape[0], int(bits), float(ray_length), _ptr(keys))
     return keys
+
+
+# --- ray generation, reconstruction, interpolation (hx_oracle_raygen.cpp) -----
+
+MATERIAL_DTYPE = np.dtype([("tex_index", "<i4"), ("shininess", "<f4"), ("diffuse", "<f4", (3,)),
+                           ("specular", "<f4", (3,))])
+
+
+def tea16(v0, v1):
+    """(v0, v1) after 16 rounds, shape (n, 2); the seed is column 0."""
+    v0, v1 = _c(v0, np.uint32), _c(v1, np.uint32)
+    out = np.empty((v0.size, 2), np.uint32)
+    lib().hxo_tea16(_ptr(v0), _ptr(v1), v0.size, _ptr(out))
+    return out
+
+
+def jenkins_mix(abc):
+    a = _c(abc, np.uint32).copy()
+    lib().hxo_jenkins_mix(_ptr(a), a.shape[0])
+    return a
+
+
+def halton(idx, base):
+    i = _c(idx, np.int32)
+    out = np.empty(i.shape, np.float32)
+    lib().hxo_halton(_ptr(i), i.size, int(base), _ptr(out))
+    return out
+
+
+def sobol2d(idx):
+    i = _c(idx, np.int32)
+    out = np.empty((i.size, 2), np.float32)
+    lib().hxo_sobol2d(_ptr(i), i.size, _ptr(out))
+    return out
+
+
+def init_seeds(n, frame):
+    out = np.empty(n, np.uint32)
+    lib().hxo_init_seeds(int(n), int(frame), _ptr(out))
+    return out
+
+
+def primary_rays(sample_index, origin, m, width, height, max_dist, index_to_pixel=None):
+    o = _c(origin, np.float32)
+    mm = _c(m, np.float32).reshape(16)
+    itp = None if index_to_pixel is None else _c(index_to_pixel, np.int32)
+    out = np.empty((width * height, 8), np.float32)
+    lib().hxo_primary_rays(int(sample_index), _ptr(o), _ptr(mm), int(width), int(height), float(max_dist),
+                           _ptr(itp), _ptr(out))
+    return out
+
+
+def shadow_rays(begin, size, n_samples, seeds, light_radius, light, in_rays, in_results):
+    s, L = _c(seeds, np.uint32), _c(light, np.float32)
+    r, res = _c(in_rays, np.float32), _c(in_results, np.int32)
+    out = np.empty((size * n_samples, 8), np.float32)
+    lib().hxo_shadow_rays(int(begin), int(size), int(n_samples), _ptr(s), float(light_radius), _ptr(L), _ptr(r),
+                          _ptr(res), _ptr(out))
+    return out
+
+
+def ao_rays(begin, size, n_samples, seeds, max_dist, in_rays, in_results, verts, tris):
+    s, r, res = _c(seeds, np.uint32), _c(in_rays, np.float32), _c(in_results, np.int32)
+    t, v = _c(tris, np.int32), _c(verts, np.float32)
+    out = np.empty((size * n_samples, 8), np.float32)
+    lib().hxo_ao_rays(int(begin), int(size), int(n_samples), _ptr(s), float(max_dist), _ptr(r), _ptr(res), _ptr(t),
+                      _ptr(v), _ptr(out))
+    return out
+
+
+def path_rays(russian_roulette, seeds, in_pixel, mat_indices, tris, normals, materials, in_rays, in_results,
+              decreases):
+    """Returns (out_rays, out_pixel, seeds after, decreases after)."""
+    seeds = _c(seeds, np.uint32).copy()
+    dec = _c(decreases, np.float32).copy()
+    n = seeds.shape[0]
+    mats = np.ascontiguousarray(materials, MATERIAL_DTYPE)
+    ip, mi, t, nn = _c(in_pixel, np.int32), _c(mat_indices, np.int32), _c(tris, np.int32), _c(normals, np.float32)
+    r, res = _c(in_rays, np.float32), _c(in_results, np.int32)
+    out = np.empty((n, 8), np.float32)
+    op = np.empty(n, np.int32)
+    k = lib().hxo_path_rays(int(bool(russian_roulette)), n, _ptr(seeds), _ptr(ip), _ptr(mi), _ptr(t), _ptr(nn),
+                            mats.ctypes.data_as(ctypes.c_void_p), _ptr(r), _ptr(res), _ptr(out), _ptr(op), _ptr(dec))
+    return out[:k], op[:k], seeds, dec
+
+
+def reconstruct_occlusion(begin, size, n_samples, replace, results, index_to_pixel, in_pixels, out_pixels,
+                          index_to_slot=None):
+    res, itp = _c(results, np.int32), _c(index_to_pixel, np.int32)
+    its = None if index_to_slot is None else _c(index_to_slot, np.int32)
+    inp = _c(in_pixels, np.float32)
+    out = _c(out_pixels, np.float32).copy()
+    lib().hxo_reconstruct_occlusion(int(begin), int(size), int(n_samples), int(bool(replace)), _ptr(res), _ptr(itp),
+                                    _ptr(its), _ptr(inp), _ptr(out))
+    return out
+
+
+def interpolate(n_static, frame0, frame1, t, out, normalize=False):
+    a, b = _c(frame0, np.float32), _c(frame1, np.float32)
+    o = _c(out, np.float32).copy()
+    lib().hxo_interpolate(int(n_static), a.shape[0], float(t), _ptr(a), _ptr(b), _ptr(o), int(bool(normalize)))
+    return o

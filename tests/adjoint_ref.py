@@ -3,13 +3,24 @@
 For each hit ray with upstream dL/dt, the adjoint adds u*lambda, v*lambda and
 (1-u-v)*lambda to the gradients of the triangle's three vertices, where
 lambda = -(e1 x e2) * dL/dt / det and det = e1 . (e2 x -d) (the solve of
-o + t d = v2 + u e1 + v e2).  This module sums those terms in FP64 with torch,
-and also returns, per vertex coordinate, the sum of the terms' magnitudes S and
-their count K.  A float32 scatter-add of K terms (each rounded to float32 once)
+o + t d = v2 + u e1 + v e2).  This module computes each term with the adjoint
+kernel's FP64 operations in the kernel's order (so a term equals the kernel's
+before its rounding to float32, even where the solve is ill-conditioned, as on
+grazing rays over slivers), sums them in FP64 with torch, and also returns,
+per vertex coordinate, the sum of the terms' magnitudes S and their count K.  A float32 scatter-add of K terms (each rounded to float32 once)
 in any order lies within K * 2^-24 * S of the exact sum, which is the bound
 the GPU's atomically accumulated gradient is held to.
 """
 import torch
+
+
+def _cross(a, b):
+    return torch.stack([a[:, 1] * b[:, 2] - a[:, 2] * b[:, 1], a[:, 2] * b[:, 0] - a[:, 0] * b[:, 2],
+                        a[:, 0] * b[:, 1] - a[:, 1] * b[:, 0]], 1)
+
+
+def _dot(a, b):
+    return (a[:, 0] * b[:, 0] + a[:, 1] * b[:, 1]) + a[:, 2] * b[:, 2]
 
 
 def adjoint_terms_fp64(vertices, triangles, rays, results, dl_dt, owner=None, tri_offsets=None, chunk=1 << 22):
@@ -39,17 +50,20 @@ def adjoint_terms_fp64(vertices, triangles, rays, results, dl_dt, owner=None, tr
         v0, v1, v2 = (vertices[vid[:, k]].double() for k in range(3))
         o = rays[r0:r1][idx, 0:3].double()
         d = rays[r0:r1][idx, 4:7].double()
+        # The kernel's FP64 operations in its order (left-to-right dot
+        # products, no fused multiply-adds), so each term equals its term.
         e1, e2, c, b = v0 - v2, v1 - v2, -d, o - v2
-        bc = torch.cross(e2, c, dim=1)
-        ca = torch.cross(c, e1, dim=1)
-        ab = torch.cross(e1, e2, dim=1)
-        det = (e1 * bc).sum(1)
+        bc, ca, ab = _cross(e2, c), _cross(c, e1), _cross(e1, e2)
+        det = _dot(e1, bc)
         ok = det != 0
-        inv = torch.where(ok, 1.0 / torch.where(ok, det, torch.ones_like(det)), torch.zeros_like(det))
-        u = (bc * b).sum(1) * inv
-        v = (ca * b).sum(1) * inv
-        lam = -(ab * (gt[idx] * inv)[:, None])
-        for k, beta in enumerate((u, v, 1.0 - u - v)):
+        inv = 1.0 / torch.where(ok, det, torch.ones_like(det))
+        u = _dot(bc, b) * inv
+        v = _dot(ca, b) * inv
+        w = (1.0 - u) - v
+        gtd = gt[idx]
+        zero = torch.zeros_like(gtd)
+        lam = torch.stack([-((bc[:, j] * zero + ca[:, j] * zero) + ab[:, j] * gtd) * inv for j in range(3)], 1)
+        for k, beta in enumerate((u, v, w)):
             term = beta[:, None] * lam
             term = torch.where(ok[:, None], term, torch.zeros_like(term))
             for j in range(3):
