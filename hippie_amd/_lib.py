@@ -125,6 +125,7 @@ SIGNATURES = {
     "hx_path_rays": [_p, _i32, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p],
     "hx_reconstruct_occlusion": [_p, _i64, _i64, _i32, _i32, _p, _p, _p, _p, _p],
     "hx_interpolate_vertices": [_p, _i64, _i64, _f32, _p, _p, _p, _i32],
+    "hx_trace_sorted_noalloc": [_p, _p, _p, _i64, _i32, _i32, _i32, _f32, _p, _p, _p],
     "hx_trace_view": [_p, _p, _i64, _p, _p, _i64, _p, _i64, _i32, _p, _p, _p],
     "hx_ray_keys": [_p, _p, _i64, _i32, _i32, _f32, _p],
     "hx_memcpy": [_p, _p, _p, _i64],

@@ -486,6 +486,18 @@ class HipTracer:
              _ptr(rays.results), None, _ptr(rays.indices))
         return rays.results
 
+    def trace_sort_noalloc(self, rays, method="aila", bits=60, ray_length=0.25):
+        """trace_sort through hx_trace_sorted_noalloc: no device allocation by
+        the library (at most HX_NOALLOC_MAX_RAYS rays; calls overlapping on one
+        GPU race on its fixed scratch)."""
+        self._need_bvh()
+        if rays.indices is None:
+            rays.indices = torch.zeros(rays.size, dtype=torch.int32, device=rays.rays.device)
+        call("hx_trace_sorted_noalloc", self.ctx.handle, self.bvh.handle, _ptr(rays.rays), rays.size,
+             int(not rays.closest_hit), RAY_KEY_METHODS[method], int(bits), float(ray_length),
+             _ptr(rays.results), None, _ptr(rays.indices))
+        return rays.results
+
     def trace_sort_timed(self, rays, method="aila", bits=60, ray_length=0.25):
         """HipTracer::traceSort(rays, sortTime, traceTime): (sort seconds, trace
         seconds), from events around the ray keys + sort and around the trace."""

@@ -984,10 +984,13 @@ int check_ray_keys(int32_t key_method, int32_t key_bits) {
     return HX_OK;
 }
 
-// hx_trace_sorted with optional event timing of its two halves.
+// hx_trace_sorted with optional event timing of its two halves.  With scr,
+// the ray keys, order, sort status and trace counters come from that fixed
+// scratch and nothing is allocated; else from the context's buffers.
 int trace_sorted_common(hx_context* ctx, hx_bvh* bvh, const float* d_rays, int64_t n_rays, int32_t any_hit,
                         int32_t key_method, int32_t key_bits, float ray_length, int32_t* d_results, int32_t* d_stats,
-                        int32_t* d_order, float* sort_seconds, float* trace_seconds) {
+                        int32_t* d_order, float* sort_seconds, float* trace_seconds,
+                        const hx::TraceScratch* scr = nullptr) {
     HX_CHECK_ARG(ctx && bvh, "context or bvh is NULL");
     HX_CHECK_ARG(n_rays >= 0 && n_rays < ((int64_t)1 << 30), "n_rays must be in [0, 2^30)");
     HX_TRY(check_ray_keys(key_method, key_bits));
@@ -996,9 +999,26 @@ int trace_sorted_common(hx_context* ctx, hx_bvh* bvh, const float* d_rays, int64
     if (n_rays == 0) return HX_OK;
     HX_CHECK_ARG(d_rays && d_results, "rays and results must be device pointers");
     Context* c = &ctx->c;
-    for (int k = 0; k < 2; ++k) {
-        HX_TRY(c->ray_keys[k].reserve(8 * (size_t)n_rays));
-        HX_TRY(c->ray_order[k].reserve(4 * (size_t)n_rays));
+    void* keys[2];
+    int32_t* order[2];
+    int32_t* control = nullptr;
+    const hx::SortWorkspace* ws = nullptr;
+    if (scr) {
+        for (int k = 0; k < 2; ++k) {
+            keys[k] = scr->keys[k];
+            order[k] = scr->order[k];
+        }
+        control = scr->control;
+        ws = &scr->sort;
+    } else {
+        for (int k = 0; k < 2; ++k) {
+            HX_TRY(c->ray_keys[k].reserve(8 * (size_t)n_rays));
+            HX_TRY(c->ray_order[k].reserve(4 * (size_t)n_rays));
+            keys[k] = c->ray_keys[k].ptr;
+            order[k] = c->ray_order[k].as<int32_t>();
+        }
+        HX_TRY(c->control.reserve(64));
+        control = c->control.as<int32_t>();
     }
     const bool timed = sort_seconds || trace_seconds;
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
@@ -1006,12 +1026,12 @@ int trace_sorted_common(hx_context* ctx, hx_bvh* bvh, const float* d_rays, int64
         for (hipEvent_t& e : ev) HX_HIP(hipEventCreate(&e));
         HX_HIP(hipEventRecord(ev[0], c->stream));
     }
-    void* keys[2] = {c->ray_keys[0].ptr, c->ray_keys[1].ptr};
-    int32_t* order[2] = {c->ray_order[0].as<int32_t>(), c->ray_order[1].as<int32_t>()};
     const int32_t* sorted = nullptr;
-    int st = ray_sort(c, d_rays, n_rays, key_method, key_bits, ray_length, keys, order, nullptr, &sorted);
+    int st = ray_sort(c, d_rays, n_rays, key_method, key_bits, ray_length, keys, order, ws, &sorted);
     if (st == HX_OK && timed) st = hipEventRecord(ev[1], c->stream) == hipSuccess ? HX_OK : HX_ERR_HIP;
-    if (st == HX_OK) st = trace_common(ctx, bvh, d_rays, n_rays, any_hit, d_results, d_stats, sorted);
+    if (st == HX_OK)
+        st = trace_arrays(c, bvh->nodes.ptr, bvh->woop.ptr, bvh->tri_idx.as<int32_t>(), d_rays, n_rays, any_hit,
+                          d_results, d_stats, sorted, control);
     if (st == HX_OK && timed) st = hipEventRecord(ev[2], c->stream) == hipSuccess ? HX_OK : HX_ERR_HIP;
     if (st == HX_OK && d_order &&
         hipMemcpyAsync(d_order, sorted, 4 * (size_t)n_rays, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
@@ -1066,6 +1086,11 @@ int hx_context_create(int device, hx_context** out) {
         return hx::hip_fail(e, "hipGetDeviceProperties", __FILE__, __LINE__);
     }
     ctx->c.cu_count = prop.multiProcessorCount;
+    const int st = hx::noalloc_setup(device);
+    if (st != HX_OK) {
+        delete ctx;
+        return st;
+    }
     *out = ctx;
     return HX_OK;
 }
@@ -1711,6 +1736,17 @@ int hx_trace_sorted_timed(hx_context* ctx, hx_bvh* bvh, const float* d_rays, int
     HX_CHECK_ARG(sort_seconds && trace_seconds, "sort_seconds and trace_seconds must be host pointers");
     return trace_sorted_common(ctx, bvh, d_rays, n_rays, any_hit, key_method, key_bits, ray_length, d_results,
                                d_stats, d_order, sort_seconds, trace_seconds);
+}
+
+int hx_trace_sorted_noalloc(hx_context* ctx, hx_bvh* bvh, const float* d_rays, int64_t n_rays, int32_t any_hit,
+                            int32_t key_method, int32_t key_bits, float ray_length, int32_t* d_results,
+                            int32_t* d_stats, int32_t* d_order) {
+    HX_CHECK_ARG(ctx && bvh, "context or bvh is NULL");
+    HX_CHECK_ARG(n_rays >= 0, "n_rays must be >= 0");
+    hx::TraceScratch scr{};
+    HX_TRY(hx::noalloc_trace_scratch(ctx->c.device, n_rays, &scr));
+    return trace_sorted_common(ctx, bvh, d_rays, n_rays, any_hit, key_method, key_bits, ray_length, d_results,
+                               d_stats, d_order, nullptr, nullptr, &scr);
 }
 
 int hx_trace_view(hx_context* ctx, const void* d_nodes, int64_t n_nodes, const void* d_woop,

@@ -478,6 +478,20 @@ int launch_reconstruct_occlusion(Context* c, int64_t begin, int64_t size, int n_
 int launch_interpolate(Context* c, int64_t n_static, int64_t n, float t, const float* a, const float* b, float* out,
                        bool normalize);
 
+// hx_noalloc.hip: the fixed per-device scratch of hx_trace_sorted_noalloc.
+struct TraceScratch {
+    void* keys[2];          // u64 ray keys, ping-pong
+    int32_t* order[2];      // sorted ray order, ping-pong
+    int32_t* control;       // trace counters (2 int32)
+    SortWorkspace sort;     // radix-sort status words
+};
+// Looks up the scratch array's address on `device` (the current device) once;
+// hx_context_create calls it.
+int noalloc_setup(int device);
+// The scratch of `device` carved for n_rays rays; HX_ERR_INVALID_ARG beyond
+// its fixed maximum or before noalloc_setup ran on the device.
+int noalloc_trace_scratch(int device, int64_t n_rays, TraceScratch* s);
+
 // hx_trace.hip
 struct TraceArgs {
     int64_t n_rays;

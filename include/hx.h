@@ -292,6 +292,19 @@ int hx_trace_sorted_timed(hx_context* ctx, hx_bvh* bvh, const float* d_rays, int
 int hx_trace_view(hx_context* ctx, const void* d_nodes, int64_t n_nodes, const void* d_woop,
                   const int32_t* d_tri_indices, int64_t n_references, const float* d_rays, int64_t n_rays,
                   int32_t any_hit, int32_t* d_results, int32_t* d_stats, const int32_t* d_indices);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under hipGraph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): hx_trace_sorted with its ray keys, sorted order,
+ * radix-sort status and trace counters in a fixed 32 MiB + 64 KiB array per
+ * GPU (held from the first context on that GPU until the process ends), so no
+ * call allocates.  At most HX_NOALLOC_MAX_RAYS rays per call
+ * (HX_ERR_INVALID_ARG beyond).  Every call on one GPU shares that GPU's array:
+ * calls that overlap on one GPU (several streams or host threads) race.  Its
+ * stack-overflow count is not reported by hx_trace_overflow_count. */
+#define HX_NOALLOC_MAX_RAYS (1 << 20)
+int hx_trace_sorted_noalloc(hx_context* ctx, hx_bvh* bvh, const float* d_rays, int64_t n_rays, int32_t any_hit,
+                            int32_t key_method, int32_t key_bits, float ray_length, int32_t* d_results,
+                            int32_t* d_stats, int32_t* d_order);
 /* RayBuffer::computeMortonCodes (RayBuffer.cpp:31-85) for the [0,1]^3 ray box. */
 int hx_ray_keys(hx_context* ctx, const float* d_rays, int64_t n_rays, int32_t key_method, int32_t key_bits,
                 float ray_length, uint64_t* d_keys);
