@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libhx.so")
 ARCH = os.environ.get("HX_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["hx_build.hip", "hx_ploc.hip", "hx_atr.hip", "hx_hlbvh.hip", "hx_insertion.hip", "hx_presplit.hip", "hx_batch.hip", "hx_sort.hip", "hx_trace.hip", "hx_diff.hip", "hx_noalloc.hip", "hx_raygen.hip", "hx_api.cpp"]
+SOURCES = ["hx_build.hip", "hx_ploc.hip", "hx_atr.hip", "hx_hlbvh.hip", "hx_insertion.hip", "hx_presplit.hip", "hx_batch.hip", "hx_sort.hip", "hx_trace.hip", "hx_diff.hip", "hx_noalloc.hip", "hx_raygen.hip", "hx_wide.hip", "hx_api.cpp"]
 
 HIP_FLAGS = [
     f"--offload-arch={ARCH}",

@@ -61,6 +61,7 @@ class HxBvhView(ctypes.Structure):
         ("n_leaves", ctypes.c_int64),
         ("n_references", ctypes.c_int64),
         ("scene_box", ctypes.c_float * 6),
+        ("layout", ctypes.c_int32),
     ]
 
 

@@ -162,6 +162,8 @@ class BuildParams:
             return
         if name == "method" and isinstance(value, str):
             value = METHODS[value]
+        if name == "layout" and isinstance(value, str):
+            value = LAYOUTS[value]
         if name == "scene_box":
             arr = (ctypes.c_float * 6)(*[float(x) for x in value])
             self.c.scene_box = arr
@@ -236,7 +238,12 @@ class HipBVH:
 
     def nodes_host(self):
         v = self.view()
-        return self._copy(v.nodes, (v.n_interior + v.n_leaves, 16), np.int32)
+        # 8N words per node: 16 (Bin), 32 (Quad), 64 (Oct)
+        return self._copy(v.nodes, (v.n_interior + v.n_leaves, 8 * v.layout), np.int32)
+
+    @property
+    def layout(self):
+        return self.view().layout
 
     def woop_host(self):
         v = self.view()

@@ -177,6 +177,12 @@ struct hx_bvh {
     bool built = false;
     bool counts_known = false;
     int64_t n_interior = 0, n_leaves = 0;
+    // Quad / Oct layouts: children per node (0: Bin), the breadth-first entries
+    // of the wide nodes (kept for refits) and the collapse workspace.
+    int wide = 0;
+    bool wide_lbvh_root = false;
+    int64_t wide_entries = 0;
+    DevBuf wide_cost, wide_cnt, e_b, e_budget, e_leaf, e_parent, e_rank, e_first, e_nchild, wide_scan, wide_partial;
     float host_box[6] = {0, 0, 0, 0, 0, 0};
     bool box_known = false;
 };
@@ -202,7 +208,8 @@ int validate_params(const hx_build_params* p) {
     HX_CHECK_ARG(p->max_leaf_size > 0 && p->max_leaf_size <= 1024, "max_leaf_size must be in (0,1024] (got %d)",
                  p->max_leaf_size);
     HX_CHECK_ARG(p->ct > 0.0f && p->ci > 0.0f, "ct and ci must be positive");
-    HX_CHECK_ARG(p->layout == HX_LAYOUT_BIN, "only the Bin layout is supported (got %d)", p->layout);
+    HX_CHECK_ARG(p->layout == HX_LAYOUT_BIN || p->layout == HX_LAYOUT_QUAD || p->layout == HX_LAYOUT_OCT,
+                 "layout must be HX_LAYOUT_BIN, HX_LAYOUT_QUAD or HX_LAYOUT_OCT (got %d)", p->layout);
     if (p->method != HX_METHOD_LBVH && p->method != HX_METHOD_PLOC && p->method != HX_METHOD_ATRBVH &&
         p->method != HX_METHOD_HLBVH && p->method != HX_METHOD_INSERTION) {
         hx::set_error("builder method %d is not available in this build", p->method);
@@ -267,7 +274,8 @@ size_t ins_work_bytes(int64_t n) {
 int reserve_bvh(hx_bvh* b, int64_t n) {
     const int64_t N = std::max<int64_t>(2 * n - 1, 2);
     const size_t ks = b->keys64 ? 8 : 4;
-    HX_TRY(b->nodes.reserve(64 * (size_t)N));
+    const int wide = b->params.layout == HX_LAYOUT_BIN ? 0 : b->params.layout;
+    HX_TRY(b->nodes.reserve((wide ? 32 * (size_t)wide : 64) * (size_t)std::max<int64_t>(2 * n, 2)));
     HX_TRY(b->woop.reserve(48 * (size_t)n));
     HX_TRY(b->tri_idx.reserve(4 * (size_t)n));
     HX_TRY(b->keys[0].reserve(ks * (size_t)n));
@@ -373,7 +381,7 @@ hx::ClimbArgs lbvh_climb_args(hx_bvh* b, const void* keys, bool keys64) {
     ca.keys64 = keys64 ? 1 : 0;
     ca.link = b->link.as<uint64_t>();
     ca.root_id = b->counts.as<int32_t>() + 3;
-    ca.all_boxes = b->params.keep_debug ? 1 : 0;
+    ca.all_boxes = b->params.keep_debug || b->params.layout != HX_LAYOUT_BIN ? 1 : 0;
     ca.proto = b->proto.as<float4>();
     return ca;
 }
@@ -859,6 +867,63 @@ int run_hlbvh(hx_bvh* b) {
     return hx::launch_collapse_emit(c, emit_args(b));
 }
 
+// The Quad / Oct layout from the build's binary tree (BVHCollapser::
+// collapseAdaptive / collapse for layouts other than Bin, BVHCollapser.cpp
+// :573-595), on the device (hx_wide.hip).  lbvh_root: the root id is the split
+// position the LBVH climb recorded (else node 0).
+hx::WideArgs wide_args(hx_bvh* b, bool lbvh_root) {
+    const hx_build_params& p = b->params;
+    hx::WideArgs a{};
+    a.n = b->n_refs;
+    a.N = p.layout;
+    a.left = b->left.as<int32_t>();
+    a.right = b->right.as<int32_t>();
+    a.parent = b->parent.as<int32_t>();
+    a.first = b->first.as<int32_t>();
+    a.last = b->last.as<int32_t>();
+    a.nmin = b->nmin.as<float4>();
+    a.nmax = b->nmax.as<float4>();
+    a.root_id = lbvh_root ? b->counts.as<int32_t>() + 3 : nullptr;
+    a.ct = p.ct;
+    a.ci = p.ci;
+    a.max_leaf = p.adaptive_leaf_size ? INT32_MAX : p.max_leaf_size;
+    a.cost = b->wide_cost.as<float>();
+    a.cnt = b->wide_cnt.as<uint16_t>();
+    a.counters = b->counters.as<int32_t>();
+    a.e_b = b->e_b.as<int32_t>();
+    a.e_budget = b->e_budget.as<int8_t>();
+    a.e_leaf = b->e_leaf.as<uint8_t>();
+    a.e_parent = b->e_parent.as<int32_t>();
+    a.e_rank = b->e_rank.as<int32_t>();
+    a.e_first = b->e_first.as<int32_t>();
+    a.e_nchild = b->e_nchild.as<int32_t>();
+    a.scan = b->wide_scan.as<int2>();
+    a.nodes = b->nodes.as<int4>();
+    return a;
+}
+
+int run_wide(hx_bvh* b, bool lbvh_root) {
+    Context* c = &b->ctx->c;
+    const int64_t n = b->n_refs, T = std::max<int64_t>(2 * n - 1, 1);
+    const int N = b->params.layout;
+    HX_TRY(b->wide_cost.reserve(4 * (size_t)(N - 1) * (size_t)T));
+    HX_TRY(b->wide_cnt.reserve(2 * (size_t)(N - 1) * (size_t)T));
+    for (DevBuf* d : {&b->e_b, &b->e_parent, &b->e_rank, &b->e_first, &b->e_nchild})
+        HX_TRY(d->reserve(4 * (size_t)(2 * n)));
+    HX_TRY(b->e_budget.reserve((size_t)(2 * n)));
+    HX_TRY(b->e_leaf.reserve((size_t)(2 * n)));
+    HX_TRY(b->wide_scan.reserve(8 * (size_t)(2 * n + 1)));
+    int64_t ni = 0, nl = 0, ne = 0;
+    HX_TRY(hx::launch_wide_build(c, wide_args(b, lbvh_root), &ni, &nl, &ne));
+    b->wide = N;
+    b->wide_lbvh_root = lbvh_root;
+    b->wide_entries = ne;
+    b->n_interior = ni;
+    b->n_leaves = nl;
+    b->counts_known = true;
+    return HX_OK;
+}
+
 int run_build(hx_bvh* b) {
     Context* c = &b->ctx->c;
     const hx_build_params& p = b->params;
@@ -891,6 +956,8 @@ int run_build(hx_bvh* b) {
         b->refs_active = false;
     }
     b->counts_known = false;
+    b->wide = 0;
+    if (p.layout != HX_LAYOUT_BIN) HX_TRY(run_wide(b, p.method == HX_METHOD_LBVH && n > 1));
     b->built = true;
     return HX_OK;
 }
@@ -904,11 +971,15 @@ int run_refit(hx_bvh* b) {
     b->pack_valid = false;  // the packed vertices are the build's
     HX_HIP(hipSetDevice(c->device));
     const int32_t* sorted_tri = b->tri_idx.as<int32_t>();
-    if (n == 1)
-        return hx::launch_single_ref(c, b->verts, b->tris, sorted_tri, b->nmin.as<float4>(), b->nmax.as<float4>(),
-                                     b->woop.as<float4>(), b->nodes.as<int4>(), b->counts.as<int32_t>());
+    if (n == 1) {
+        HX_TRY(hx::launch_single_ref(c, b->verts, b->tris, sorted_tri, b->nmin.as<float4>(), b->nmax.as<float4>(),
+                                     b->woop.as<float4>(), b->nodes.as<int4>(), b->counts.as<int32_t>()));
+        if (b->wide) return hx::launch_wide_emit(c, wide_args(b, false), b->wide_entries, b->n_interior);
+        return HX_OK;
+    }
     HX_TRY(hx::launch_refit_reset(c, b->counters.as<int32_t>(), n - 1, b->counts.as<int32_t>() + 2));
     HX_TRY(hx::launch_climb(c, climb_args(b, true)));
+    if (b->wide) return hx::launch_wide_emit(c, wide_args(b, b->wide_lbvh_root), b->wide_entries, b->n_interior);
     return hx::launch_emit_only(c, emit_args(b));
 }
 
@@ -937,7 +1008,7 @@ int read_box(hx_bvh* b) {
 // counter and overflow count live in `control` (2 int32).
 int trace_arrays(Context* c, const void* nodes, const void* woop, const int32_t* tri_idx, const float* d_rays,
                  int64_t n_rays, int32_t any_hit, int32_t* d_results, int32_t* d_stats, const int32_t* d_order,
-                 int32_t* control) {
+                 int32_t* control, int wide = 0) {
     hx::TraceArgs a{};
     a.n_rays = n_rays;
     a.any_hit = any_hit ? 1 : 0;
@@ -950,6 +1021,7 @@ int trace_arrays(Context* c, const void* nodes, const void* woop, const int32_t*
     a.order = d_order;
     a.control = control;
     a.fetch_threshold = c->tune.trace_fetch;
+    if (wide) return hx::launch_trace_wide(c, wide, a);
     return hx::launch_trace(c, a);
 }
 
@@ -962,7 +1034,7 @@ int trace_common(hx_context* ctx, hx_bvh* bvh, const float* d_rays, int64_t n_ra
     Context* c = &ctx->c;
     HX_TRY(c->control.reserve(64));
     return trace_arrays(c, bvh->nodes.ptr, bvh->woop.ptr, bvh->tri_idx.as<int32_t>(), d_rays, n_rays, any_hit,
-                        d_results, d_stats, d_order, c->control.as<int32_t>());
+                        d_results, d_stats, d_order, c->control.as<int32_t>(), bvh->wide);
 }
 
 // Ray keys and their sort into `order` (the sorted ray indices), from the
@@ -1031,7 +1103,7 @@ int trace_sorted_common(hx_context* ctx, hx_bvh* bvh, const float* d_rays, int64
     if (st == HX_OK && timed) st = hipEventRecord(ev[1], c->stream) == hipSuccess ? HX_OK : HX_ERR_HIP;
     if (st == HX_OK)
         st = trace_arrays(c, bvh->nodes.ptr, bvh->woop.ptr, bvh->tri_idx.as<int32_t>(), d_rays, n_rays, any_hit,
-                          d_results, d_stats, sorted, control);
+                          d_results, d_stats, sorted, control, bvh->wide);
     if (st == HX_OK && timed) st = hipEventRecord(ev[2], c->stream) == hipSuccess ? HX_OK : HX_ERR_HIP;
     if (st == HX_OK && d_order &&
         hipMemcpyAsync(d_order, sorted, 4 * (size_t)n_rays, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
@@ -1323,6 +1395,7 @@ int hx_build_batch(hx_context* ctx, const hx_build_params* params, const float* 
     HX_TRY(validate_params(params));
     HX_CHECK_ARG(params->method == HX_METHOD_LBVH || params->method == HX_METHOD_PLOC,
                  "batched builds use the LBVH or PLOC method (got %d)", params->method);
+    HX_CHECK_ARG(params->layout == HX_LAYOUT_BIN, "batched builds use the Bin layout (got %d)", params->layout);
 
     HX_CHECK_ARG(n_meshes >= 1, "n_meshes must be positive (got %d)", n_meshes);
     HX_CHECK_ARG(h_mesh_offsets[0] == 0, "mesh offsets must start at 0");
@@ -1494,6 +1567,7 @@ int hx_bvh_get_view(hx_bvh* bvh, hx_bvh_view* out) {
     out->n_leaves = bvh->n_leaves;
     out->n_references = bvh->n_refs;
     std::memcpy(out->scene_box, bvh->host_box, sizeof(out->scene_box));
+    out->layout = bvh->wide ? bvh->wide : HX_LAYOUT_BIN;
     return HX_OK;
 }
 
@@ -1541,6 +1615,22 @@ int hx_bvh_info(hx_bvh* bvh, const char* key, int64_t* value) {
 int hx_bvh_cost(hx_bvh* bvh, double* out) {
     HX_CHECK_ARG(bvh && out, "NULL argument");
     Context* c = &bvh->ctx->c;
+    if (bvh->wide) {
+        // computeCost of HipBVHNodeQuad / Oct (HipBVHKernels.cu:184-238) over
+        // the scene box's area (HipBVH::getCost, HipBVH.cpp:346-370).
+        HX_TRY(read_box(bvh));
+        const float* bx = bvh->host_box;
+        const float dx = bx[3] - bx[0], dy = bx[4] - bx[1], dz = bx[5] - bx[2];
+        const bool valid = bx[0] <= bx[3] && bx[1] <= bx[4] && bx[2] <= bx[5];
+        const float area = valid ? 2.0f * (dx * dy + dy * dz + dz * dx) : 0.0f;
+        const int64_t nn = bvh->n_interior + bvh->n_leaves;
+        HX_TRY(bvh->wide_partial.reserve(8 * (size_t)hx::wide_cost_partials(nn)));
+        HX_TRY(hx::launch_wide_cost(c, bvh->wide, bvh->nodes.as<int4>(), nn, area, bvh->params.ct, bvh->params.ci,
+                                    bvh->wide_partial.as<double>(), bvh->cost_out.as<double>()));
+        HX_HIP(hipMemcpyAsync(out, bvh->cost_out.ptr, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HX_HIP(hipStreamSynchronize(c->stream));
+        return HX_OK;
+    }
     const int64_t max_nodes = std::max<int64_t>(2 * bvh->n_refs - 1, 2);
     HX_TRY(hx::launch_cost(c, bvh->nodes.as<int4>(), bvh->counts.as<int32_t>(), max_nodes,
                            bvh->scene_box.as<float>(), bvh->params.ct, bvh->params.ci, bvh->cost_out.as<double>()));
@@ -1552,6 +1642,7 @@ int hx_bvh_cost(hx_bvh* bvh, double* out) {
 int hx_bvh_validate(hx_bvh* bvh, int32_t* failed) {
     HX_CHECK_ARG(bvh && failed, "NULL argument");
     HX_CHECK_ARG(!bvh->batched, "validation is per BVH; a batch has one BVH per mesh");
+    HX_CHECK_ARG(!bvh->wide, "device validation covers the Bin layout");
     HX_CHECK_ARG(bvh->built, "the BVH has not been built");
     Context* c = &bvh->ctx->c;
     const int64_t max_nodes = std::max<int64_t>(2 * bvh->n_refs - 1, 2);
@@ -1568,6 +1659,7 @@ int hx_bvh_validate(hx_bvh* bvh, int32_t* failed) {
 int hx_bvh_leaf_stats(hx_bvh* bvh, int32_t* histogram33, int64_t* sum_of_leaf_sizes) {
     HX_CHECK_ARG(bvh && histogram33 && sum_of_leaf_sizes, "NULL argument");
     HX_CHECK_ARG(!bvh->batched, "leaf statistics are per BVH; a batch has one BVH per mesh");
+    HX_CHECK_ARG(!bvh->wide, "leaf statistics cover the Bin layout");
     Context* c = &bvh->ctx->c;
     HX_TRY(bvh->leaf_stats.reserve(sizeof(int32_t) * (hx::kMaxLeafSizeStat + 2)));
     HX_TRY(hx::launch_leaf_stats(c, bvh->nodes.as<int4>(), bvh->counts.as<int32_t>(),

@@ -447,6 +447,45 @@ int launch_sort_pairs(Context* c, void* keys[2], int32_t* vals[2], int64_t n, in
                       bool vals_identity, int* result_index, bool allow_split = true,
                       const SortWorkspace* ws = nullptr);
 
+// hx_wide.hip: Quad / Oct layouts from a built binary tree (internal nodes
+// 0..n-2 with left/right/parent and inclusive leaf ranges first/last, leaves
+// n-1+p, boxes of all 2n-1 nodes) and their traversal.
+struct WideArgs {
+    int64_t n;
+    int N;                    // children per node: 4 or 8
+    const int32_t* left;
+    const int32_t* right;
+    const int32_t* parent;
+    const int32_t* first;
+    const int32_t* last;
+    const float4* nmin;
+    const float4* nmax;
+    const int32_t* root_id;   // device; NULL: node 0
+    float ct, ci;
+    int max_leaf;             // references a collapsed leaf may hold (INT_MAX: adaptive)
+    // Workspace: costs and slot splits (N-1 per binary node), arrival counters.
+    float* cost;
+    uint16_t* cnt;
+    int32_t* counters;
+    // Wide nodes in breadth-first order (<= 2n entries): binary node, slots,
+    // leaf flag, parent entry, rank among interiors / leaves, first child
+    // entry and child count.
+    int32_t* e_b;
+    int8_t* e_budget;
+    uint8_t* e_leaf;
+    int32_t* e_parent;
+    int32_t* e_rank;
+    int32_t* e_first;
+    int32_t* e_nchild;
+    int2* scan;               // per-level counts, 2n + 1
+    int4* nodes;              // out: 2N int4 per node
+};
+int launch_wide_build(Context* c, const WideArgs& a, int64_t* n_interior, int64_t* n_leaves, int64_t* n_entries);
+int launch_wide_emit(Context* c, const WideArgs& a, int64_t n_entries, int64_t n_interior);
+int wide_cost_partials(int64_t n_nodes);
+int launch_wide_cost(Context* c, int N, const int4* nodes, int64_t n_nodes, float scene_area, float ct, float ci,
+                     double* partial, double* out);
+
 // hx_raygen.hip: ray generation, occlusion reconstruction, interpolation.
 int launch_init_seeds(Context* c, int64_t n, uint32_t frame, uint32_t* seeds);
 int launch_primary_rays(Context* c, int sample_index, const float origin[3], const float m[16], int width, int height,
@@ -512,6 +551,7 @@ struct TraceArgs {
     int n_meshes;
 };
 int launch_trace(Context* c, const TraceArgs& a);
+int launch_trace_wide(Context* c, int N, const TraceArgs& a);
 int launch_ray_keys(Context* c, const float4* rays, int64_t n, int method, int bits, float ray_length,
                     uint64_t* keys);
 

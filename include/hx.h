@@ -65,6 +65,8 @@ extern "C" {
 #define HX_METHOD_INSERTION 4
 
 #define HX_LAYOUT_BIN 2
+#define HX_LAYOUT_QUAD 4   /* HipBVHNodeQuad, 128 B (HipBVHNode.h:13-114), collapsed as BVH::collapseWide */
+#define HX_LAYOUT_OCT 8    /* HipBVHNodeOct, 256 B */
 
 /* Ray key methods: RayBuffer::MortonCodeMethod (rt/ray/RayBuffer.h). */
 #define HX_RAYKEY_TWOPOINT 0
@@ -92,7 +94,7 @@ typedef struct hx_build_params {
     int32_t atr_treelet_size;    /* ATRBuilder::setTreeletSize [4,30]         */
     int32_t hlbvh_sah_bits;      /* Bvh.hlbvhMortonCodeSAHBits                */
     int32_t insertion_mod;       /* Bvh.insertionMod                          */
-    int32_t layout;              /* Bvh.layout (HX_LAYOUT_BIN)                */
+    int32_t layout;              /* Bvh.layout: HX_LAYOUT_BIN / _QUAD / _OCT  */
     int32_t use_scene_box;       /* 1: scene_box below is the Morton frame    */
     float scene_box[6];          /* mn.xyz, mx.xyz                            */
     int32_t keep_debug;          /* 1: keep keys, binary arrays for hx_bvh_debug */
@@ -100,13 +102,14 @@ typedef struct hx_build_params {
 
 /* Device pointers and counts of a built BVH (HipBVH accessors). */
 typedef struct hx_bvh_view {
-    void* nodes;             /* Bin nodes, 64 B each                          */
+    void* nodes;             /* Bin nodes, 64 B each (Quad 128 B, Oct 256 B)  */
     void* woop;              /* 48 B per reference slot                       */
     int32_t* tri_indices;    /* reference slot -> triangle id                 */
     int64_t n_interior;
     int64_t n_leaves;
     int64_t n_references;
     float scene_box[6];
+    int32_t layout;          /* HX_LAYOUT_BIN, HX_LAYOUT_QUAD or HX_LAYOUT_OCT */
 } hx_bvh_view;
 
 /* Intermediate arrays of the last build (requires keep_debug), for parity

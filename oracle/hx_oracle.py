@@ -81,6 +81,8 @@ def lib():
             "hxo_path_rays": (_i64, [_i32, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p]),
             "hxo_reconstruct_occlusion": (None, [_i64, _i64, _i32, _i32, _p, _p, _p, _p, _p]),
             "hxo_interpolate": (None, [_i64, _i64, _f32, _p, _p, _p, _i32]),
+            "hxo_collapse_wide": (_i64, [_i32, _i64, _i32, _p, _p, _p, _p, _f32, _f32, _i32, _p, _p]),
+            "hxo_cost_wide": (ctypes.c_double, [_i32, _p, _i64, _f32, _f32, _f32]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -541,3 +543,22 @@ def interpolate(n_static, frame0, frame1, t, out, normalize=False):
     o = _c(out, np.float32).copy()
     lib().hxo_interpolate(int(n_static), a.shape[0], float(t), _ptr(a), _ptr(b), _ptr(o), int(bool(normalize)))
     return o
+
+
+# --- wide (Quad / Oct) layouts (hx_oracle_wide.cpp) ---------------------------
+
+def collapse_wide(N, n, root, left, right, nmin, nmax, ct=3.0, ci=2.0, max_leaf_size=0):
+    """Wide nodes (int32 (nodes, 8N)) and the interior count of a binary tree
+    (internal 0..n-2, leaves n-1+p, float4 boxes); max_leaf_size <= 0: adaptive."""
+    l, r = _c(left, np.int32), _c(right, np.int32)
+    mn, mx = _c(nmin, np.float32), _c(nmax, np.float32)
+    out = np.zeros((2 * n, 8 * N), np.int32)
+    ni = ctypes.c_int64()
+    k = lib().hxo_collapse_wide(int(N), int(n), int(root), _ptr(l), _ptr(r), _ptr(mn), _ptr(mx), float(ct),
+                                float(ci), int(max_leaf_size), _ptr(out), ctypes.byref(ni))
+    return out[:k].copy(), int(ni.value)
+
+
+def cost_wide(N, nodes, scene_area, ct=3.0, ci=2.0):
+    w = _c(nodes, np.int32)
+    return lib().hxo_cost_wide(int(N), _ptr(w), w.shape[0], float(scene_area), float(ct), float(ci))
