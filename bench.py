@@ -81,13 +81,26 @@ def parse():
     ap.add_argument("--c5-rays", type=int, default=64 * 1024 * 1024, help="rays per step of that cycle")
     ap.add_argument("--tune", action="append", default=[], metavar="NAME=VALUE",
                     help="libhx launch tunable (hx_context_set_tuning), e.g. trace_lds_stack=16")
+    ap.add_argument("--layouts", default="quad,oct", help="wide layouts to time on the same scene and rays ('' = none)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run one plain device copy of this many MiB first (calibrates PMC byte counters)")
     return ap.parse_args()
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(verts, tris, rays, args):
-    """Oracle (C++ restatement, all host threads) on a bounded sample of the same workload."""
+    """Oracle (C++ restatement, all host threads) on a bounded sample of the same workload:
+    one warm-up build, then the median of 5 timed builds."""
     import numpy as np
     from oracle import hx_oracle as O
     rng = np.random.default_rng(11)
@@ -97,22 +110,82 @@ def cpu_baseline(verts, tris, rays, args):
     t_s = tris[sel]
     O.lib()
     threads = O.get_threads()
+    b = O.build_lbvh(verts, t_s, bits=60)
     times = []
-    for _ in range(2):
+    for _ in range(5):
         t0 = time.perf_counter()
         b = O.build_lbvh(verts, t_s, bits=60)
         times.append(time.perf_counter() - t0)
-    build_s = min(times)
+    build_s = float(np.median(times))
     what = "all" if m == n else f"{m} of the"
     r = rays[: min(args.cpu_rays, rays.shape[0])]
     t0 = time.perf_counter()
     O.trace(b["nodes"], b["woop"], b["tri_idx"], r)
     trace_s = time.perf_counter() - t0
-    return dict(value=round(m / build_s / 1e6, 3), unit="Mtris/s", cores=threads, kind="port",
-                sample=f"LBVH-60 build of {what} {n} triangles (best of 2): "
+    return dict(value=round(m / build_s / 1e6, 3), unit="Mtris/s", cores=threads, kind="port", cpu=cpu_model(),
+                sample=f"LBVH-60 build of {what} {n} triangles (median of 5 after a warm-up): "
                        f"{build_s:.2f} s; closest-hit trace of {r.shape[0]} primary rays on that BVH: "
                        f"{r.shape[0] / trace_s / 1e6:.3f} Mrays/s",
                 trace_mrays_s=round(r.shape[0] / trace_s / 1e6, 3))
+
+
+# FP32 VALU issue rate of one MI355X without packed math: 256 CUs x 64 lanes x
+# 2.4 GHz lane-operations per second.
+VALU_LANE_OPS = 256 * 64 * 2.4e9
+
+
+def traversal_model(ctx, bvh, rb, hx, valu=True):
+    """Per-ray nodes visited and triangles tested (the stats kernel) and, for
+    the Bin layout, the VALU model of a ray's cost, 40 ops per node + 26 per
+    triangle test, with the Mrays/s that VALU issue would allow at that cost."""
+    import numpy as np
+    r = hx.RayBuffer(rb.rays)
+    hx.HipTracer(ctx, bvh).trace_stats(r)
+    st = r.stats.cpu().numpy().astype(np.float64)
+    nodes, tris = float(st[:, 0].mean()), float(st[:, 1].mean())
+    if not valu:
+        return {"nodes_per_ray": round(nodes, 2), "tris_per_ray": round(tris, 2)}
+    ops = 40.0 * nodes + 26.0 * tris
+    return {"nodes_per_ray": round(nodes, 2), "tris_per_ray": round(tris, 2), "valu_ops_per_ray": round(ops, 1),
+            "valu_bound_mrays_s": round(VALU_LANE_OPS / ops / 1e6, 1)}
+
+
+def sort_and_trace_ms(tracer, rbs, reps=3):
+    """(ray keys + sort ms, trace ms) per step over the ray sets, from
+    HipTracer::traceSort(rays, sortTime, traceTime)."""
+    s = t = 0.0
+    for _ in range(reps):
+        for rb in rbs:
+            a, b = tracer.trace_sort_timed(rb, method="aila", bits=60)
+            s += a
+            t += b
+    return 1e3 * s / reps, 1e3 * t / reps
+
+
+def bench_layouts(ctx, scene, n, rbs, n_rays, args):
+    """Quad and Oct layouts of the same LBVH-60 (Bvh.layout quad / oct): build,
+    Aila-sorted trace of the same rays, trace-only rate, ray-sort time, SAH."""
+    import hippie_amd as hx
+    out = {}
+    for name in [x for x in args.layouts.split(",") if x]:
+        log(f"layout {name}")
+        b = hx.LBVHBuilder(ctx, morton_bits=60, layout=name)
+        bv = b.build(scene)
+        ms = timed(lambda: b.rebuild(bv, scene), 3, 1)
+        tr = hx.HipTracer(ctx, bv)
+        for rb in rbs:
+            tr.trace_sort(rb, method="aila", bits=60)
+        full = timed(lambda: [tr.trace_sort(rb, method="aila", bits=60) for rb in rbs], 3, 0)
+        sort_ms, trace_ms = sort_and_trace_ms(tr, rbs)
+        out[name] = {"build_ms": round(ms, 3), "build_mtris_s": round(n / (ms * 1e-3) / 1e6, 2),
+                     "nodes": bv.n_interior + bv.n_leaves, "sah": round(bv.get_cost(), 4),
+                     "sorted_trace_ms": round(full, 3), "sorted_trace_mrays_s": round(n_rays / (full * 1e-3) / 1e6, 2),
+                     "ray_sort_ms": round(sort_ms, 3), "trace_only_ms": round(trace_ms, 3),
+                     "trace_only_mrays_s": round(n_rays / (trace_ms * 1e-3) / 1e6, 2),
+                     "work_primary": traversal_model(ctx, bv, rbs[0], hx, valu=False),
+                     "work_diffuse": traversal_model(ctx, bv, rbs[1], hx, valu=False)}
+        del tr, bv
+    return out
 
 
 def timed(fn, steps, warmup):
@@ -564,6 +637,9 @@ def main():
                             "achieved_gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
                             "traffic": sum(traffic) if traffic and None not in traffic else None}
 
+    ray_sort_ms, trace_only_ms = sort_and_trace_ms(tracer, [prim_rb, sec_rb])
+    model = {"primary": traversal_model(ctx, bvh, prim_rb, hx), "diffuse": traversal_model(ctx, bvh, sec_rb, hx)}
+    layouts = bench_layouts(ctx, scene, n, [prim_rb, sec_rb], n_rays, args) if args.layouts else {}
     builders = bench_builders(ctx, scene, n, args) if args.builders else {}
     if builders:
         builders["lbvh"] = {"ms": round(build_ms, 3), "sah": round(bvh.get_cost(), 4)}
@@ -602,6 +678,11 @@ def main():
         "build_mtris_s": round(n / (build_ms * 1e-3) / 1e6, 2),
         "trace_ms": round(trace_ms, 3),
         "trace_mrays_s": round(n_rays / (trace_ms * 1e-3) / 1e6, 2),
+        "ray_sort_ms": round(ray_sort_ms, 3),
+        "trace_only_ms": round(trace_only_ms, 3),
+        "trace_only_mrays_s": round(n_rays / (trace_only_ms * 1e-3) / 1e6, 2),
+        "traversal_model": model,
+        "layouts": layouts,
         "kernels_ms_per_step": {k: round(v[1] / args.steps, 4) for k, v in prof.items()},
         "roofline": {
             "bound": "hbm",

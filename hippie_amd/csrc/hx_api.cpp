@@ -898,6 +898,7 @@ hx::WideArgs wide_args(hx_bvh* b, bool lbvh_root) {
     a.e_first = b->e_first.as<int32_t>();
     a.e_nchild = b->e_nchild.as<int32_t>();
     a.scan = b->wide_scan.as<int2>();
+    a.scan_part = b->wide_scan.as<int2>() + (2 * b->n_refs + 1);
     a.nodes = b->nodes.as<int4>();
     return a;
 }
@@ -912,7 +913,7 @@ int run_wide(hx_bvh* b, bool lbvh_root) {
         HX_TRY(d->reserve(4 * (size_t)(2 * n)));
     HX_TRY(b->e_budget.reserve((size_t)(2 * n)));
     HX_TRY(b->e_leaf.reserve((size_t)(2 * n)));
-    HX_TRY(b->wide_scan.reserve(8 * (size_t)(2 * n + 1)));
+    HX_TRY(b->wide_scan.reserve(8 * (size_t)(2 * n + 1 + (2 * n) / 2048 + 2)));
     int64_t ni = 0, nl = 0, ne = 0;
     HX_TRY(hx::launch_wide_build(c, wide_args(b, lbvh_root), &ni, &nl, &ne));
     b->wide = N;

@@ -478,6 +478,7 @@ struct WideArgs {
     int32_t* e_first;
     int32_t* e_nchild;
     int2* scan;               // per-level counts, 2n + 1
+    int2* scan_part;          // their per-block sums, 2n / 2048 + 2
     int4* nodes;              // out: 2N int4 per node
 };
 int launch_wide_build(Context* c, const WideArgs& a, int64_t* n_interior, int64_t* n_leaves, int64_t* n_entries);

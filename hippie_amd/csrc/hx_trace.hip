@@ -42,9 +42,9 @@ HX_DEV float span_end(float a0, float a1, float b0, float b1, float c0, float c1
 // into one flat load, which waits on both memory counters.
 typedef __attribute__((address_space(3))) int32_t lds_i32;
 
-template <int LDS_DEPTH>
+template <int LDS_DEPTH, int SIZE = kStackSize>
 struct TraversalStack {
-    static constexpr int kDeep = kStackSize - LDS_DEPTH > 0 ? kStackSize - LDS_DEPTH : 1;
+    static constexpr int kDeep = SIZE - LDS_DEPTH > 0 ? SIZE - LDS_DEPTH : 1;
     lds_i32* col;
     int32_t deep[kDeep];
     HX_DEV void put(int e, int32_t v) {
@@ -230,6 +230,206 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(TraceArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Quad / Oct traversal (HipTracerQuadKernels.cu, HipTracerOctKernels.cu; node
+// words of HipBVHNode.h:13-114).  One ray per lane, as the Bin kernel: a node
+// visit tests its N child boxes (child c: float4 c = min/max x, min/max y;
+// float4 N + c = min/max z, link), sorts the hits by entry distance with a
+// compare-exchange network in registers (ties by link), continues into the
+// nearest and pushes the others farthest first.  The reference gives each
+// ray N lanes; on 64-lane waves that leaves 64/N rays per wave to hide each
+// node fetch, so here every lane keeps its own ray and the waves stay full.
+// Leaves (link ~index) hold [begin, end) at words 4N+2, 4N+3; the triangle
+// test, closest-hit tie rule, postponed leaf and wave refill are the Bin
+// kernel's.
+// ---------------------------------------------------------------------------
+
+template <int N>
+struct WideNet;
+template <>
+struct WideNet<4> {
+    static constexpr int kPairs = 5;
+    HX_DEV static int2 pair(int i) {
+        constexpr int a[5] = {0, 2, 0, 1, 1}, b[5] = {1, 3, 2, 3, 2};
+        return make_int2(a[i], b[i]);
+    }
+};
+template <>
+struct WideNet<8> {
+    // Batcher's odd-even merge sort for 8 inputs.
+    static constexpr int kPairs = 19;
+    HX_DEV static int2 pair(int i) {
+        constexpr int a[19] = {0, 2, 4, 6, 0, 1, 4, 5, 1, 5, 0, 1, 2, 3, 2, 3, 1, 3, 5};
+        constexpr int b[19] = {1, 3, 5, 7, 2, 3, 6, 7, 2, 6, 4, 5, 6, 7, 4, 5, 2, 4, 6};
+        return make_int2(a[i], b[i]);
+    }
+};
+
+template <int N>
+struct WideStackSize {
+    static constexpr int kSize = N == 4 ? 96 : 160;
+};
+
+template <int N, bool STATS, bool ORDERED, int LDS_DEPTH>
+__global__ __launch_bounds__(kTraceBlock) void k_trace_wide(TraceArgs a) {
+    constexpr int kSize = WideStackSize<N>::kSize;
+    __shared__ int32_t s_stack[LDS_DEPTH > 0 ? LDS_DEPTH * kTraceBlock : 1];
+    TraversalStack<LDS_DEPTH, kSize> stack;
+    stack.col = (lds_i32*)(s_stack + threadIdx.x);
+    int sp = 0;
+    float ox = 0.f, oy = 0.f, oz = 0.f, tmin = 0.f, dx = 0.f, dy = 0.f, dz = 0.f;
+    float idx = 0.f, idy = 0.f, idz = 0.f, oodx = 0.f, oody = 0.f, oodz = 0.f;
+    float hitT = 0.f, hitU = 0.f, hitV = 0.f;
+    int32_t nodeAddr = kSentinel, leafAddr = 0, hitIndex = -1, hitId = -1;
+    int64_t rayidx = 0;
+    int32_t nNodes = 0, nTris = 0;
+    int32_t* rayCounter = a.control;
+    const float4* nodes4 = reinterpret_cast<const float4*>(a.nodes);
+
+    while (true) {
+        const bool terminated = nodeAddr == kSentinel;
+        const uint64_t tmask = ballot64(terminated);
+        const int ntm = __popcll(tmask);
+        const int below = count_below(tmask);
+        const int leader = __ffsll((unsigned long long)tmask) - 1;
+        int base = 0;
+        if (terminated && below == 0) base = atomicAdd(rayCounter, ntm);
+        base = __shfl(base, leader, kWave);
+        if (terminated) {
+            rayidx = (int64_t)base + below;
+            if (rayidx >= a.n_rays) break;
+            if (ORDERED) rayidx = a.order[rayidx];
+            const float4 o = a.rays[2 * rayidx + 0];
+            const float4 d = a.rays[2 * rayidx + 1];
+            ox = o.x; oy = o.y; oz = o.z; tmin = o.w;
+            dx = d.x; dy = d.y; dz = d.z; hitT = d.w;
+            const float ooeps = 8.2718061255302767e-25f;  // exp2f(-80)
+            idx = 1.0f / (fabsf(dx) > ooeps ? dx : copysignf(ooeps, dx));
+            idy = 1.0f / (fabsf(dy) > ooeps ? dy : copysignf(ooeps, dy));
+            idz = 1.0f / (fabsf(dz) > ooeps ? dz : copysignf(ooeps, dz));
+            oodx = ox * idx;
+            oody = oy * idy;
+            oodz = oz * idz;
+            stack.put(0, kSentinel);
+            sp = 0;
+            leafAddr = 0;
+            nodeAddr = 0;
+            hitIndex = -1;
+            hitId = -1;
+            if (STATS) { nNodes = 0; nTris = 0; }
+        }
+        while (nodeAddr != kSentinel) {
+            while ((uint32_t)nodeAddr < (uint32_t)kSentinel) {
+                const float4* np = nodes4 + (int64_t)nodeAddr * 2 * N;
+                if (STATS) ++nNodes;
+                float dist[N];
+                int32_t link[N];
+#pragma unroll
+                for (int c = 0; c < N; ++c) {
+                    const float4 xy = np[c];
+                    const float4 zl = np[N + c];
+                    const float lox = fmaf(xy.x, idx, -oodx);
+                    const float hix = fmaf(xy.y, idx, -oodx);
+                    const float loy = fmaf(xy.z, idy, -oody);
+                    const float hiy = fmaf(xy.w, idy, -oody);
+                    const float loz = fmaf(zl.x, idz, -oodz);
+                    const float hiz = fmaf(zl.y, idz, -oodz);
+                    const float cmin = span_begin(lox, hix, loy, hiy, loz, hiz, tmin);
+                    const float cmax = span_end(lox, hix, loy, hiy, loz, hiz, hitT);
+                    const bool hit = cmax >= cmin;
+                    dist[c] = hit ? cmin : __int_as_float(0x7f800000);
+                    link[c] = hit ? __float_as_int(zl.z) : kSentinel;
+                }
+#pragma unroll
+                for (int q = 0; q < WideNet<N>::kPairs; ++q) {
+                    const int2 pr = WideNet<N>::pair(q);
+                    const float da = dist[pr.x], db = dist[pr.y];
+                    const int32_t la = link[pr.x], lb = link[pr.y];
+                    const bool sw = db < da || (db == da && lb < la);
+                    dist[pr.x] = sw ? db : da;
+                    dist[pr.y] = sw ? da : db;
+                    link[pr.x] = sw ? lb : la;
+                    link[pr.y] = sw ? la : lb;
+                }
+                int cnt = 0;
+#pragma unroll
+                for (int c = 0; c < N; ++c) cnt += link[c] != kSentinel ? 1 : 0;
+                if (cnt == 0) {
+                    nodeAddr = stack.get(sp--);
+                } else {
+                    if (sp + cnt >= kSize) {
+                        atomicAdd(a.control + 1, 1);
+                        nodeAddr = kSentinel;
+                        leafAddr = 0;
+                        break;
+                    }
+#pragma unroll
+                    for (int c = N - 1; c >= 1; --c)
+                        if (c < cnt) stack.put(++sp, link[c]);
+                    nodeAddr = link[0];
+                }
+                if (nodeAddr < 0 && leafAddr >= 0) {
+                    leafAddr = nodeAddr;
+                    nodeAddr = stack.get(sp--);
+                }
+                if (!__any(leafAddr >= 0)) break;
+            }
+
+            while (leafAddr < 0) {
+                const float4 lb = nodes4[(int64_t)(~leafAddr) * 2 * N + N];
+                const int end = __float_as_int(lb.w);
+                for (int tri = __float_as_int(lb.z); tri < end; ++tri) {
+                    if (STATS) ++nTris;
+                    const float4 v00 = a.woop[3 * (int64_t)tri + 0];
+                    const float Oz = fmaf(-oz, v00.z, fmaf(-oy, v00.y, fmaf(-ox, v00.x, v00.w)));
+                    const float den = fmaf(dz, v00.z, fmaf(dy, v00.y, dx * v00.x));
+                    const float t = Oz * (1.0f / den);
+                    if (t > tmin && t <= hitT) {
+                        const float4 v11 = a.woop[3 * (int64_t)tri + 1];
+                        const float Ox = fmaf(oz, v11.z, fmaf(oy, v11.y, fmaf(ox, v11.x, v11.w)));
+                        const float Dx = fmaf(dz, v11.z, fmaf(dy, v11.y, dx * v11.x));
+                        const float u = fmaf(t, Dx, Ox);
+                        if (u >= 0.0f) {
+                            const float4 v22 = a.woop[3 * (int64_t)tri + 2];
+                            const float Oy = fmaf(oz, v22.z, fmaf(oy, v22.y, fmaf(ox, v22.x, v22.w)));
+                            const float Dy = fmaf(dz, v22.z, fmaf(dy, v22.y, dx * v22.x));
+                            const float v = fmaf(t, Dy, Oy);
+                            if (v >= 0.0f && u + v <= 1.0f) {
+                                const int32_t id = a.tri_idx[tri];
+                                // Closest hit; equal t resolves to the smaller triangle id.
+                                if (t < hitT || id < hitId) {
+                                    hitT = t;
+                                    hitIndex = tri;
+                                    hitId = id;
+                                    hitU = u;
+                                    hitV = v;
+                                    if (a.any_hit) {
+                                        nodeAddr = kSentinel;
+                                        break;
+                                    }
+                                }
+                            }
+                        }
+                    }
+                }
+                leafAddr = nodeAddr;
+                if (nodeAddr < 0) nodeAddr = stack.get(sp--);
+            }
+
+            if (__popcll(ballot64(true)) < a.fetch_threshold) break;
+        }
+
+        if (nodeAddr == kSentinel) {
+            if (hitIndex == -1)
+                reinterpret_cast<int2*>(a.results)[2 * rayidx] = make_int2(-1, __float_as_int(hitT));
+            else
+                a.results[rayidx] =
+                    make_int4(hitId, __float_as_int(hitT), __float_as_int(hitU), __float_as_int(hitV));
+            if (STATS) a.stats[rayidx] = make_int2(nNodes, nTris);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Ray Morton keys for the [0,1]^3 ray box (RayBuffer.cpp:63-66).
 // ---------------------------------------------------------------------------
 
@@ -354,7 +554,52 @@ int launch_trace_depth(Context* c, const TraceArgs& a) {
     }
 }
 
+template <int N, bool STATS, bool ORDERED, int LDS_DEPTH>
+int launch_trace_wide_t(Context* c, const TraceArgs& a) {
+    static int occ = 0;
+    if (occ == 0) {
+        int q = 0;
+        HX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&q, k_trace_wide<N, STATS, ORDERED, LDS_DEPTH>, kTraceBlock,
+                                                             0));
+        occ = std::max(q, 1);
+    }
+    const int per_cu = c->tune.trace_blocks_per_cu > 0 ? std::min(c->tune.trace_blocks_per_cu, occ) : occ;
+    const int64_t want = (a.n_rays + kTraceBlock - 1) / kTraceBlock;
+    const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>((int64_t)c->cu_count * per_cu, want));
+    hipLaunchKernelGGL((k_trace_wide<N, STATS, ORDERED, LDS_DEPTH>), dim3(blocks), dim3(kTraceBlock), 0, c->stream, a);
+    return check_launch("trace_wide");
+}
+
+template <int N, bool STATS, bool ORDERED>
+int launch_trace_wide_depth(Context* c, const TraceArgs& a) {
+    switch (c->tune.trace_lds_stack) {
+        case 0: return launch_trace_wide_t<N, STATS, ORDERED, 0>(c, a);
+        case 16: return launch_trace_wide_t<N, STATS, ORDERED, 16>(c, a);
+        case 24: return launch_trace_wide_t<N, STATS, ORDERED, 24>(c, a);
+        case 32: return launch_trace_wide_t<N, STATS, ORDERED, 32>(c, a);
+        default: return launch_trace_wide_t<N, STATS, ORDERED, 64>(c, a);
+    }
+}
+
+template <int N>
+int launch_trace_wide_n(Context* c, const TraceArgs& a) {
+    if (a.stats)
+        return a.order ? launch_trace_wide_depth<N, true, true>(c, a) : launch_trace_wide_depth<N, true, false>(c, a);
+    return a.order ? launch_trace_wide_depth<N, false, true>(c, a) : launch_trace_wide_depth<N, false, false>(c, a);
+}
+
 }  // namespace
+
+int launch_trace_wide(Context* c, int N, const TraceArgs& a) {
+    if (a.n_rays <= 0) return HX_OK;
+    hipLaunchKernelGGL(k_zero_control, dim3(1), dim3(64), 0, c->stream, a.control);
+    HX_TRY(check_launch("zero_control"));
+    LaunchScope ls(c, a.order ? "trace_wide_sorted" : "trace_wide");
+    if (N == 4) return launch_trace_wide_n<4>(c, a);
+    if (N == 8) return launch_trace_wide_n<8>(c, a);
+    set_error("wide trace: %d children per node is not supported (4 or 8)", N);
+    return HX_ERR_INVALID_ARG;
+}
 
 int launch_trace(Context* c, const TraceArgs& a) {
     if (a.n_rays <= 0) return HX_OK;

@@ -20,12 +20,7 @@
 //  3. k_wide_emit writes the node records from the binary boxes; a refit
 //     reruns it after the binary climb (same entries, new boxes).
 //
-// Traversal: N consecutive lanes trace one ray (64/N rays per wave); lane k
-// tests child k, the hits are sorted by entry distance across the N lanes
-// (a bitonic network on DPP quad permutes and ds_swizzle), pushed on the
-// group's LDS stack nearest-last, and leaves split their references over the
-// N lanes with a (t, id) lexicographic reduction -- the closest-hit and tie
-// rule of the Bin kernel.
+// The traversal of these layouts is k_trace_wide in hx_trace.hip.
 #include "hx_device.h"
 #include "hx_internal.h"
 
@@ -43,6 +38,8 @@ HX_DEV int cnt_x(uint16_t c) { return c == kLeafCnt ? -1 : (int)(c & 0xFF); }
 HX_DEV int cnt_y(uint16_t c) { return c == kLeafCnt ? -1 : (int)(c >> 8); }
 
 HX_DEV int root_of(const WideArgs& a) { return a.root_id ? *a.root_id : 0; }
+HX_DEV float ld_agent(float* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+HX_DEV void st_agent(float* p, float v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 
 // Cost of a subtree (binary node b) with j slots, j = 1..N-1, at cost[(j-1) T + b]
 // and the split of those slots at cnt[(j-1) T + b]; the areas are divided by the
@@ -55,22 +52,24 @@ __global__ __launch_bounds__(kWideBlock) void k_wide_dp(WideArgs a) {
     const int S = N - 1;
     const int root = root_of(a);
     const float Ar = area4(a.nmin[root], a.nmax[root]);
-    // The leaf: one reference, ceil(1 / N) * N slots of leaf cost.
+    // The leaf: one reference, ceil(1 / N) * N slots of leaf cost.  Costs
+    // pass between threads (and XCDs) through agent-coherent relaxed stores
+    // and loads; a thread's stores complete before it bumps the parent's
+    // arrival counter, so the second arrival reads both children's costs.
     {
         const int64_t b = a.n - 1 + p;
         const float A = area4(a.nmin[b], a.nmax[b]);
         const float c = a.ci * A / Ar * (float)div_ceil_i(1, N) * (float)N;
         for (int j = 0; j < S; ++j) {
-            a.cost[j * T + b] = c;
+            st_agent(a.cost + j * T + b, c);
             a.cnt[j * T + b] = kLeafCnt;
         }
     }
     if (a.n == 1) return;
     int node = a.parent[a.n - 1 + p];
     while (node >= 0) {
-        // Release this subtree's costs, acquire the sibling's: the second
-        // arrival (any XCD) combines them.
-        if (__hip_atomic_fetch_add(a.counters + node, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == 0) return;
+        __builtin_amdgcn_s_waitcnt(0);
+        if (__hip_atomic_fetch_add(a.counters + node, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) return;
         const int l = a.left[node], r = a.right[node];
         const float A = area4(a.nmin[node], a.nmax[node]);
         const int size = a.last[node] - a.first[node] + 1;
@@ -78,8 +77,8 @@ __global__ __launch_bounds__(kWideBlock) void k_wide_dp(WideArgs a) {
         float cl[N - 1], cr[N - 1];
 #pragma unroll
         for (int j = 0; j < S; ++j) {
-            cl[j] = a.cost[j * T + l];
-            cr[j] = a.cost[j * T + r];
+            cl[j] = ld_agent(a.cost + j * T + l);
+            cr[j] = ld_agent(a.cost + j * T + r);
         }
         float subtree = a.ct * A / Ar;
         float cd = kMaxFloat;
@@ -102,7 +101,7 @@ __global__ __launch_bounds__(kWideBlock) void k_wide_dp(WideArgs a) {
             prev = leaf_cost;
             prevc = kLeafCnt;
         }
-        a.cost[node] = prev;
+        st_agent(a.cost + node, prev);
         a.cnt[node] = prevc;
 #pragma unroll
         for (int j = 2; j < N; ++j) {
@@ -119,7 +118,7 @@ __global__ __launch_bounds__(kWideBlock) void k_wide_dp(WideArgs a) {
                 prev = cd;
                 prevc = bc;
             }
-            a.cost[(j - 1) * T + node] = prev;
+            st_agent(a.cost + (j - 1) * T + node, prev);
             a.cnt[(j - 1) * T + node] = prevc;
         }
         node = a.parent[node];
@@ -202,25 +201,61 @@ __global__ __launch_bounds__(kWideBlock) void k_wide_count(WideArgs a, int64_t l
     out[e - lo] = r;
 }
 
-// Exclusive scan of int2 counts (one workgroup), total at out[m].
-__global__ __launch_bounds__(1024) void k_wide_scan(int2* __restrict__ v, int64_t m) {
+// Exclusive scan of a level's int2 counts in place, total at v[m]: per-block
+// sums (kScanPer items per block), one workgroup scans the sums, then each
+// block scans its items from its offset.
+constexpr int kScanItems = 8;
+constexpr int kScanPer = kWideBlock * kScanItems;
+
+HX_DEV int2 add2(int2 p, int2 q) { return make_int2(p.x + q.x, p.y + q.y); }
+
+__global__ __launch_bounds__(kWideBlock) void k_scan_reduce(const int2* __restrict__ v, int64_t m, int2* __restrict__ part) {
+    __shared__ int2 wt[kWideBlock / kWave];
+    const int64_t base = blockIdx.x * (int64_t)kScanPer;
+    int2 s = make_int2(0, 0);
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const int64_t i = base + k * kWideBlock + threadIdx.x;
+        if (i < m) s = add2(s, v[i]);
+    }
+    int2 total;
+    block_inclusive_scan<kWideBlock / kWave>(s, add2, wt, total);
+    if (threadIdx.x == 0) part[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_parts(int2* __restrict__ part, int64_t nb) {
     __shared__ int2 wt[1024 / kWave];
     __shared__ int2 carry;
     if (threadIdx.x == 0) carry = make_int2(0, 0);
     __syncthreads();
-    for (int64_t base = 0; base < m; base += 1024) {
+    for (int64_t base = 0; base < nb; base += 1024) {
         const int64_t i = base + threadIdx.x;
-        const int2 x = i < m ? v[i] : make_int2(0, 0);
+        const int2 x = i < nb ? part[i] : make_int2(0, 0);
         int2 total;
-        const int2 inc = block_inclusive_scan<1024 / kWave>(
-            x, [](int2 p, int2 q) { return make_int2(p.x + q.x, p.y + q.y); }, wt, total);
+        const int2 inc = block_inclusive_scan<1024 / kWave>(x, add2, wt, total);
         const int2 c0 = carry;
-        if (i < m) v[i] = make_int2(c0.x + inc.x - x.x, c0.y + inc.y - x.y);
+        if (i < nb) part[i] = make_int2(c0.x + inc.x - x.x, c0.y + inc.y - x.y);
         __syncthreads();
-        if (threadIdx.x == 0) carry = make_int2(c0.x + total.x, c0.y + total.y);
+        if (threadIdx.x == 0) carry = add2(c0, total);
         __syncthreads();
     }
-    if (threadIdx.x == 0) v[m] = carry;
+    if (threadIdx.x == 0) part[nb] = carry;
+}
+
+__global__ __launch_bounds__(kWideBlock) void k_scan_apply(int2* __restrict__ v, int64_t m, const int2* __restrict__ part) {
+    __shared__ int2 wt[kWideBlock / kWave];
+    const int64_t base = blockIdx.x * (int64_t)kScanPer;
+    int2 carry = part[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const int64_t i = base + k * kWideBlock + threadIdx.x;
+        const int2 x = i < m ? v[i] : make_int2(0, 0);
+        int2 total;
+        const int2 inc = block_inclusive_scan<kWideBlock / kWave>(x, add2, wt, total);
+        if (i < m) v[i] = make_int2(carry.x + inc.x - x.x, carry.y + inc.y - x.y);
+        carry = add2(carry, total);
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) v[m] = carry;
 }
 
 // Children of the level's interiors written as the next level, breadth-first:
@@ -381,257 +416,6 @@ __global__ __launch_bounds__(256) void k_wide_cost_sum(const double* __restrict_
     if (threadIdx.x == 0) *out = s[0];
 }
 
-// --- traversal ---------------------------------------------------------------------
-
-constexpr int32_t kNoHit = -1;
-
-// Lane exchange across a group of N lanes: DPP quad permutes for lane^1 and
-// lane^2, ds_swizzle (xor 4) inside 32-lane halves for lane^4.
-template <int MASK>
-HX_DEV int xor_lane(int v) {
-    if (MASK == 1) return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false);   // quad_perm [1,0,3,2]
-    if (MASK == 2) return __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false);   // quad_perm [2,3,0,1]
-    return __builtin_amdgcn_ds_swizzle(v, 0x101F);                               // and 0x1F, xor 4
-}
-template <int MASK>
-HX_DEV float xor_lane_f(float v) { return __int_as_float(xor_lane<MASK>(__float_as_int(v))); }
-
-// One bitonic compare-exchange between lanes k and k^MASK: the lower lane of
-// the pair keeps the smaller (dist, link) when `up`.
-template <int MASK>
-HX_DEV void cx(float& d, int& l, int k, bool up) {
-    const float od = xor_lane_f<MASK>(d);
-    const int ol = xor_lane<MASK>(l);
-    const bool lower = (k & MASK) == 0;
-    const bool take_min = lower == up;
-    const bool other_less = od < d || (od == d && ol < l);
-    if (take_min ? other_less : !other_less && (od != d || ol != l)) {
-        d = od;
-        l = ol;
-    }
-}
-
-template <int N>
-HX_DEV void sort_group(float& d, int& l, int k) {
-    // size 2
-    cx<1>(d, l, k, (k & 2) == 0);
-    // size 4
-    cx<2>(d, l, k, (k & 4) == 0);
-    cx<1>(d, l, k, (k & 4) == 0);
-    if (N == 8) {
-        cx<4>(d, l, k, true);
-        cx<2>(d, l, k, true);
-        cx<1>(d, l, k, true);
-    }
-}
-
-template <int MASK>
-HX_DEV void reduce_hit_step(float& t, int& id, int& idx, float& u, float& v) {
-    const float ot = xor_lane_f<MASK>(t);
-    const int oid = xor_lane<MASK>(id);
-    const int oidx = xor_lane<MASK>(idx);
-    const float ou = xor_lane_f<MASK>(u);
-    const float ov = xor_lane_f<MASK>(v);
-    // (t, id) lexicographic minimum: every lane starts from the group's
-    // (hitT, hitId), which any accepted hit lies below.
-    const bool take = ot < t || (ot == t && oid < id);
-    if (take) {
-        t = ot;
-        id = oid;
-        idx = oidx;
-        u = ou;
-        v = ov;
-    }
-}
-
-HX_DEV float fmin_s(float a, float b) { return fminf(a, b); }
-HX_DEV float fmax_s(float a, float b) { return fmaxf(a, b); }
-HX_DEV float span_begin(float a0, float a1, float b0, float b1, float c0, float c1, float d) {
-    const int c = max(min(__float_as_int(c0), __float_as_int(c1)), __float_as_int(d));
-    return __int_as_float(max(max(__float_as_int(fmin_s(a0, a1)), __float_as_int(fmin_s(b0, b1))), c));
-}
-HX_DEV float span_end(float a0, float a1, float b0, float b1, float c0, float c1, float d) {
-    const int c = min(max(__float_as_int(c0), __float_as_int(c1)), __float_as_int(d));
-    return __int_as_float(min(min(__float_as_int(fmax_s(a0, a1)), __float_as_int(fmax_s(b0, b1))), c));
-}
-
-typedef __attribute__((address_space(3))) int32_t lds_i32;
-
-template <int N>
-struct WideStack {
-    static constexpr int kDepth = N == 4 ? 64 : 96;
-};
-
-template <int N, bool STATS, bool ORDERED>
-__global__ __launch_bounds__(kWideBlock) void k_trace_wide(TraceArgs a) {
-    constexpr int kGroups = kWideBlock / N;
-    constexpr int kDepth = WideStack<N>::kDepth;
-    __shared__ int32_t s_stack[kGroups * kDepth];
-    const int k = threadIdx.x & (N - 1);
-    const int g = threadIdx.x / N;
-    lds_i32* stack = (lds_i32*)(s_stack + g * kDepth);
-    const int lane = lane_id();
-    const int lane0 = lane & ~(N - 1);
-    int sp = -1;
-    bool live = false;
-    float ox = 0.f, oy = 0.f, oz = 0.f, tmin = 0.f, dx = 0.f, dy = 0.f, dz = 0.f;
-    float idx_ = 0.f, idy = 0.f, idz = 0.f, oodx = 0.f, oody = 0.f, oodz = 0.f;
-    float hitT = 0.f, hitU = 0.f, hitV = 0.f;
-    int hitId = kNoHit, hitIndex = -1;
-    int64_t rayidx = 0;
-    int nNodes = 0, nTris = 0;
-    int32_t* counter = a.control;
-
-    while (true) {
-        // Rays for the groups whose traversal ended: one atomic per wave.
-        const bool need = sp < 0;
-        const uint64_t lead = ballot64(need && k == 0);
-        if (need) {
-            const int first = __ffsll((unsigned long long)lead) - 1;
-            int base = 0;
-            if (lane == first) base = atomicAdd(counter, __popcll(lead));
-            base = __shfl(base, first, kWave);
-            const int rank = __popcll(lead & ((1ull << lane0) - 1));
-            rayidx = (int64_t)base + rank;
-            if (rayidx >= a.n_rays) break;
-            if (ORDERED) rayidx = a.order[rayidx];
-            const float4 o = a.rays[2 * rayidx + 0];
-            const float4 d = a.rays[2 * rayidx + 1];
-            ox = o.x; oy = o.y; oz = o.z; tmin = o.w;
-            dx = d.x; dy = d.y; dz = d.z; hitT = d.w;
-            const float ooeps = 8.2718061255302767e-25f;  // exp2f(-80)
-            idx_ = 1.0f / (fabsf(dx) > ooeps ? dx : copysignf(ooeps, dx));
-            idy = 1.0f / (fabsf(dy) > ooeps ? dy : copysignf(ooeps, dy));
-            idz = 1.0f / (fabsf(dz) > ooeps ? dz : copysignf(ooeps, dz));
-            oodx = ox * idx_;
-            oody = oy * idy;
-            oodz = oz * idz;
-            if (k == 0) stack[0] = 0;
-            sp = 0;
-            hitId = kNoHit;
-            hitIndex = -1;
-            hitU = hitV = 0.0f;
-            live = true;
-            if (STATS) nNodes = nTris = 0;
-        }
-        while (sp >= 0) {
-            const int curr = stack[sp];
-            --sp;
-            if (curr >= 0) {
-                const float4* np = reinterpret_cast<const float4*>(a.nodes) + (int64_t)curr * 2 * N;
-                const float4 xy = np[k];
-                const float4 zi = np[N + k];
-                if (STATS) ++nNodes;
-                const float lox = fmaf(xy.x, idx_, -oodx);
-                const float hix = fmaf(xy.y, idx_, -oodx);
-                const float loy = fmaf(xy.z, idy, -oody);
-                const float hiy = fmaf(xy.w, idy, -oody);
-                const float loz = fmaf(zi.x, idz, -oodz);
-                const float hiz = fmaf(zi.y, idz, -oodz);
-                const float cmin = span_begin(lox, hix, loy, hiy, loz, hiz, tmin);
-                const float cmax = span_end(lox, hix, loy, hiy, loz, hiz, hitT);
-                const bool hit = cmax >= cmin;
-                const uint64_t hm = (ballot64(hit) >> lane0) & ((1ull << N) - 1);
-                const int cnt = __popcll(hm);
-                if (cnt == 0) continue;
-                float dist = hit ? cmin : __int_as_float(0x7f800000);
-                int link = __float_as_int(zi.z);
-                sort_group<N>(dist, link, k);
-                if (sp + cnt >= kDepth) {   // stack overflow: the ray's result is not valid
-                    if (k == 0) atomicAdd(a.control + 1, 1);
-                    sp = -1;
-                    break;
-                }
-                // Nearest on top: sorted lane r (r < cnt) goes to sp + cnt - r.
-                if (k < cnt) stack[sp + cnt - k] = link;
-                __builtin_amdgcn_wave_barrier();   // the group's next pop reads another lane's push
-                sp += cnt;
-            } else {
-                const float4 lb = reinterpret_cast<const float4*>(a.nodes)[(int64_t)(~curr) * 2 * N + N];
-                const int begin = __float_as_int(lb.z), end = __float_as_int(lb.w);
-                if (STATS) nTris += end - begin;
-                float bt = hitT, bu = hitU, bv = hitV;
-                int bid = hitId, bidx = hitIndex;
-                for (int tri = begin + k; tri < end; tri += N) {
-                    const float4 v00 = a.woop[3 * (int64_t)tri + 0];
-                    const float Oz = fmaf(-oz, v00.z, fmaf(-oy, v00.y, fmaf(-ox, v00.x, v00.w)));
-                    const float den = fmaf(dz, v00.z, fmaf(dy, v00.y, dx * v00.x));
-                    const float t = Oz * (1.0f / den);
-                    if (t > tmin && t <= hitT) {
-                        const float4 v11 = a.woop[3 * (int64_t)tri + 1];
-                        const float Ox = fmaf(oz, v11.z, fmaf(oy, v11.y, fmaf(ox, v11.x, v11.w)));
-                        const float Dx = fmaf(dz, v11.z, fmaf(dy, v11.y, dx * v11.x));
-                        const float u = fmaf(t, Dx, Ox);
-                        if (u >= 0.0f) {
-                            const float4 v22 = a.woop[3 * (int64_t)tri + 2];
-                            const float Oy = fmaf(oz, v22.z, fmaf(oy, v22.y, fmaf(ox, v22.x, v22.w)));
-                            const float Dy = fmaf(dz, v22.z, fmaf(dy, v22.y, dx * v22.x));
-                            const float v = fmaf(t, Dy, Oy);
-                            if (v >= 0.0f && u + v <= 1.0f) {
-                                const int id = a.tri_idx[tri];
-                                // Closest hit; equal t resolves to the smaller triangle id.
-                                if (t < bt || (t == bt && id < bid)) {
-                                    bt = t;
-                                    bid = id;
-                                    bidx = tri;
-                                    bu = u;
-                                    bv = v;
-                                }
-                            }
-                        }
-                    }
-                }
-                reduce_hit_step<1>(bt, bid, bidx, bu, bv);
-                reduce_hit_step<2>(bt, bid, bidx, bu, bv);
-                if (N == 8) reduce_hit_step<4>(bt, bid, bidx, bu, bv);
-                const bool found = bid != hitId || bidx != hitIndex;
-                hitT = bt;
-                hitId = bid;
-                hitIndex = bidx;
-                hitU = bu;
-                hitV = bv;
-                if (a.any_hit && found && hitId != kNoHit) sp = -1;
-            }
-            if (__popcll(ballot64(sp >= 0)) < a.fetch_threshold) break;
-        }
-        if (sp < 0 && live) {
-            if (k == 0) {
-                if (hitId == kNoHit)
-                    reinterpret_cast<int2*>(a.results)[2 * rayidx] = make_int2(-1, __float_as_int(hitT));
-                else
-                    a.results[rayidx] = make_int4(hitId, __float_as_int(hitT), __float_as_int(hitU), __float_as_int(hitV));
-                if (STATS) a.stats[rayidx] = make_int2(nNodes, nTris);
-            }
-            live = false;
-        }
-    }
-}
-
-__global__ void k_wide_zero_control(int32_t* control) {
-    if (threadIdx.x < 2) control[threadIdx.x] = 0;
-}
-
-template <int N, bool STATS, bool ORDERED>
-int launch_trace_wide_t(Context* c, const TraceArgs& a) {
-    static int occ = 0;
-    if (occ == 0) {
-        int q = 0;
-        HX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&q, k_trace_wide<N, STATS, ORDERED>, kWideBlock, 0));
-        occ = std::max(q, 1);
-    }
-    const int per_cu = c->tune.trace_blocks_per_cu > 0 ? std::min(c->tune.trace_blocks_per_cu, occ) : occ;
-    const int64_t want = (a.n_rays * N + kWideBlock - 1) / kWideBlock;
-    const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>((int64_t)c->cu_count * per_cu, want));
-    hipLaunchKernelGGL((k_trace_wide<N, STATS, ORDERED>), dim3(blocks), dim3(kWideBlock), 0, c->stream, a);
-    return check_launch("trace_wide");
-}
-
-template <int N>
-int launch_trace_wide_n(Context* c, const TraceArgs& a) {
-    if (a.stats) return a.order ? launch_trace_wide_t<N, true, true>(c, a) : launch_trace_wide_t<N, true, false>(c, a);
-    return a.order ? launch_trace_wide_t<N, false, true>(c, a) : launch_trace_wide_t<N, false, false>(c, a);
-}
-
 unsigned grid_of(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + kWideBlock - 1) / kWideBlock); }
 
 template <int N>
@@ -657,8 +441,13 @@ int build_wide_n(Context* c, WideArgs a, int64_t* n_interior, int64_t* n_leaves,
         const int64_t m = hi - lo;
         hipLaunchKernelGGL(k_wide_count<N>, dim3(grid_of(m)), dim3(kWideBlock), 0, c->stream, a, lo, hi, a.scan);
         HX_TRY(check_launch("wide_count"));
-        hipLaunchKernelGGL(k_wide_scan, dim3(1), dim3(1024), 0, c->stream, a.scan, m);
-        HX_TRY(check_launch("wide_scan"));
+        const int64_t nb = (m + kScanPer - 1) / kScanPer;
+        hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(kWideBlock), 0, c->stream, a.scan, m, a.scan_part);
+        HX_TRY(check_launch("wide_scan_reduce"));
+        hipLaunchKernelGGL(k_scan_parts, dim3(1), dim3(1024), 0, c->stream, a.scan_part, nb);
+        HX_TRY(check_launch("wide_scan_parts"));
+        hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(kWideBlock), 0, c->stream, a.scan, m, a.scan_part);
+        HX_TRY(check_launch("wide_scan_apply"));
         HX_HIP(hipMemcpyAsync(&tot, a.scan + m, sizeof(int2), hipMemcpyDeviceToHost, c->stream));
         HX_HIP(hipStreamSynchronize(c->stream));
         if (tot.x == 0) break;
@@ -713,16 +502,5 @@ int launch_wide_cost(Context* c, int N, const int4* nodes, int64_t n_nodes, floa
 }
 
 int wide_cost_partials(int64_t n_nodes) { return (int)grid_of(n_nodes); }
-
-int launch_trace_wide(Context* c, int N, const TraceArgs& a) {
-    if (a.n_rays <= 0) return HX_OK;
-    hipLaunchKernelGGL(k_wide_zero_control, dim3(1), dim3(64), 0, c->stream, a.control);
-    HX_TRY(check_launch("zero_control"));
-    LaunchScope ls(c, a.order ? "trace_wide_sorted" : "trace_wide");
-    if (N == 4) return launch_trace_wide_n<4>(c, a);
-    if (N == 8) return launch_trace_wide_n<8>(c, a);
-    set_error("wide trace: %d children per node is not supported", N);
-    return HX_ERR_INVALID_ARG;
-}
 
 }  // namespace hx
