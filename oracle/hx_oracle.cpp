@@ -694,9 +694,229 @@ bool atr_treelet(AtrTree& T, int32_t X, int t, float ct) {
     return true;
 }
 
+/* TR (TRBuilder.cpp:140-347, TRBuilderKernels.cu:11-476): the treelet of 7
+ * leaves formed as ATRBVH's, then the optimal binary tree over it by a DP over
+ * its 128 leaf subsets: C(S) = min(ct A(S) + min_P [C(P) + C(S\P)],
+ * ci A(S) R) with R the treelet root's reference count (the reference's leaf
+ * option counts every reference of the treelet).  Subsets of 2..5 leaves take
+ * the first strict minimum in processSchedule's partition order; the 6-leaf
+ * subsets take it over 4 lanes and the whole treelet over 32 lanes, with the
+ * reference's shuffle-down trees deciding ties.  The kept structure is laid
+ * out as updateTreelet does (internal nodes handed out left sides first in
+ * stack order, then right sides).  Every enumerated partition leaves out the
+ * subset's lowest leaf, so that leaf always goes right.  The costs written
+ * matter: later treelets of the same pass that reach into this one read them.
+ * A new right child's cost is the reference's costs[partition] -- the left
+ * side's -- and the root keeps its climb cost. */
+constexpr int kTrT = 7;
+
+float tr_area(const V3& mn, const V3& mx) { return atr_area(mn, mx); }
+
+bool tr_treelet(AtrTree& T, int32_t X, float ct, float ci) {
+    const int t = kTrT;
+    int internal[kTrT], leaves[kTrT];
+    float areas[kTrT];
+    internal[0] = X;
+    leaves[0] = T.left[X];
+    leaves[1] = T.right[X];
+    areas[0] = T.area[leaves[0]];
+    areas[1] = T.area[leaves[1]];
+    for (int it = 0; it < t - 2; ++it) {
+        int idx[64];
+        float val[64];
+        const int k = 2 + it;
+        for (int i = 0; i < k; ++i) {
+            idx[i] = i;
+            val[i] = T.is_leaf(leaves[i]) ? -kMaxFloat : areas[i];
+        }
+        lane_tree(k, idx, val, [](float o, float m) { return o > m; });
+        const int e = idx[0];
+        const int32_t rep = leaves[e];
+        internal[it + 1] = rep;
+        leaves[e] = T.left[rep];
+        leaves[it + 2] = T.right[rep];
+        areas[e] = T.area[leaves[e]];
+        areas[it + 2] = T.area[leaves[it + 2]];
+    }
+    /* calculateSubsetSurfaceAreas (:22-81). */
+    const int S = 1 << t;
+    std::vector<V3> smn(S), smx(S);
+    std::vector<float> cost(S);
+    std::vector<int> mask(S, 0);
+    for (int s = 0; s < S; ++s) {
+        V3 mn{kMaxFloat, kMaxFloat, kMaxFloat}, mx{-kMaxFloat, -kMaxFloat, -kMaxFloat};
+        for (int i = 0; i < t; ++i)
+            if (s & (1 << i)) {
+                const V3 a = T.bmin(leaves[i]), b = T.bmax(leaves[i]);
+                mn = V3{std::fmin(a.x, mn.x), std::fmin(a.y, mn.y), std::fmin(a.z, mn.z)};
+                mx = V3{std::fmax(b.x, mx.x), std::fmax(b.y, mx.y), std::fmax(b.z, mx.z)};
+            }
+        smn[s] = mn;
+        smx[s] = mx;
+        cost[s] = tr_area(mn, mx);
+    }
+    for (int i = 0; i < t; ++i) cost[1 << i] = T.cost[leaves[i]];
+    const float R = (float)T.refs[X];
+    auto finish = [&](int s, float opt, int part) {
+        cost[s] = std::fmin(ct * cost[s] + opt, ci * cost[s] * R);
+        mask[s] = part;
+    };
+    /* processSchedule (:84-114): subsets of 2..t-2 leaves, smaller first. */
+    for (int size = 2; size <= t - 2; ++size)
+        for (int s = 1; s < S; ++s) {
+            if (__builtin_popcount(s) != size) continue;
+            float opt = kMaxFloat;
+            int best = 0;
+            const int delta = (s - 1) & s;
+            int p = (-delta) & s;
+            while (p != 0) {
+                const float c = cost[p] + cost[p ^ s];
+                if (c < opt) {
+                    opt = c;
+                    best = p;
+                }
+                p = (p - delta) & s;
+            }
+            finish(s, opt, best);
+        }
+    /* processSubsets (:116-196): the t subsets of t-1 leaves over 4 lanes each. */
+    for (int g = 0; g < t; ++g) {
+        const int s = (S - 1) & ~(1 << g);
+        int idx[64];
+        float val[64];
+        for (int q = 0; q < 4; ++q) {
+            const int lane = 4 * g + q;
+            const int leftMask = -(1 << std::max(lane / 4 - 1, 0));
+            int base = (lane & 3) + 1;
+            float opt = kMaxFloat;
+            int best = 0;
+            const int parts = (1 << (t - 2)) - 1;
+            int p = (((base & leftMask) << 1) | (base & ~leftMask)) << 1;
+            for (int j = (lane & 3); j < parts; j += 4) {
+                const float c = cost[p] + cost[p ^ s];
+                if (c < opt) {
+                    opt = c;
+                    best = p;
+                }
+                base += 4;
+                p = (((base & leftMask) << 1) | (base & ~leftMask)) << 1;
+            }
+            idx[q] = best;
+            val[q] = opt;
+        }
+        lane_tree(4, idx, val, [](float o, float m) { return o < m; });
+        finish(s, val[0], idx[0]);
+    }
+    {
+        const int s = S - 1;
+        int idx[64];
+        float val[64];
+        for (int lane = 0; lane < 32; ++lane) {
+            float opt = kMaxFloat;
+            int best = 0;
+            int p = (lane + 1) * 2;
+            for (int j = lane; j < (1 << (t - 1)) - 1; j += 32) {
+                const float c = cost[p] + cost[p ^ s];
+                if (c < opt) {
+                    opt = c;
+                    best = p;
+                }
+                p += 64;
+            }
+            idx[lane] = best;
+            val[lane] = opt;
+        }
+        lane_tree(32, idx, val, [](float o, float m) { return o < m; });
+        finish(s, val[0], idx[0]);
+    }
+    /* updateTreelet (:198-324). */
+    if (!(cost[S - 1] < T.cost[X])) return false;
+    int st_node[kTrT], st_mask[kTrT], nst = 1, next = 1;
+    st_node[0] = X;
+    st_mask[0] = S - 1;
+    while (nst > 0) {
+        int nn[kTrT], nm[kTrT], k = 0;
+        auto place = [&](int root, int part, bool left) {
+            int32_t child;
+            if (__builtin_popcount(part) > 1) {
+                child = internal[next++];
+                store4(T.nmin + 4 * (int64_t)child, smn[part]);
+                store4(T.nmax + 4 * (int64_t)child, smx[part]);
+                T.area[child] = tr_area(smn[part], smx[part]);
+                nn[k] = child;
+                nm[k] = part;
+                ++k;
+            } else {
+                child = leaves[__builtin_ctz(part)];
+            }
+            if (left) T.left[root] = child;
+            else T.right[root] = child;
+            T.parent[child] = root;
+            return child;
+        };
+        std::vector<std::pair<int32_t, int>> lefts, rights;
+        for (int i = 0; i < nst; ++i) {
+            const int part = mask[st_mask[i]];
+            const int32_t c = place(st_node[i], part, true);
+            if (__builtin_popcount(part) > 1) T.cost[c] = cost[part];
+        }
+        for (int i = 0; i < nst; ++i) {
+            const int part = mask[st_mask[i]];
+            const int comp = part ^ st_mask[i];
+            const int32_t c = place(st_node[i], comp, false);
+            if (__builtin_popcount(comp) > 1) T.cost[c] = cost[part];   /* the reference's costs[partition] */
+        }
+        for (int i = 0; i < k; ++i) {
+            st_node[i] = nn[i];
+            st_mask[i] = nm[i];
+        }
+        nst = k;
+    }
+    return true;
+}
+
 }  // namespace
 
 extern "C" {
+
+/* TRBuilder::optimize (TRBuilder.cpp:140-191): gamma = 7, doubling; post-order
+ * climb per iteration as the ATRBVH restatement's. */
+int64_t hxo_tr(int64_t n, int iterations, float ct, float ci, int32_t* left, int32_t* right, int32_t* parent,
+               float* nmin, float* nmax) {
+    if (n < 2) return 0;
+    const int64_t N = 2 * n - 1;
+    AtrTree T{n, left, right, parent, nmin, nmax, std::vector<float>(N), std::vector<float>(N),
+              std::vector<int32_t>(N)};
+    for (int64_t v = 0; v < N; ++v) T.area[v] = atr_area(T.bmin((int32_t)v), T.bmax((int32_t)v));
+    int64_t accepted = 0;
+    int64_t gamma = kTrT;
+    std::vector<int32_t> stack;
+    for (int i = 0; i < iterations; ++i) {
+        for (int64_t p = 0; p < n; ++p) {
+            T.refs[n - 1 + p] = 1;
+            T.cost[n - 1 + p] = ci * T.area[n - 1 + p];
+        }
+        stack.assign(1, 0);
+        std::vector<uint8_t> seen(n - 1, 0);
+        while (!stack.empty()) {
+            const int32_t v = stack.back();
+            if (!seen[v]) {
+                seen[v] = 1;
+                if (!T.is_leaf(T.right[v])) stack.push_back(T.right[v]);
+                if (!T.is_leaf(T.left[v])) stack.push_back(T.left[v]);
+                continue;
+            }
+            stack.pop_back();
+            const int32_t l = T.left[v], r = T.right[v];
+            T.refs[v] = T.refs[l] + T.refs[r];
+            T.cost[v] = ct * T.area[v] + T.cost[l] + T.cost[r];
+            if (T.refs[v] >= gamma && tr_treelet(T, v, ct, ci)) ++accepted;
+        }
+        gamma *= 2;
+        if (gamma > n) break;
+    }
+    return accepted;
+}
 
 int64_t hxo_atr(int64_t n, int treelet_size, int iterations, float ct, float ci, int32_t* left,
                            int32_t* right, int32_t* parent, float* nmin, float* nmax) {
@@ -2212,6 +2432,37 @@ int64_t hxo_build_atr(const float* verts, const int32_t* tris, int64_t n_tri, co
               nmax.data());
     const int64_t acc = hxo_atr(n, treelet_size, iterations, ct, ci, left.data(), right.data(), parent.data(),
                                 nmin.data(), nmax.data());
+    if (accepted) *accepted = acc;
+    int64_t nn = hxo_collapse(n, vals.data(), left.data(), right.data(), parent.data(), nmin.data(), nmax.data(),
+                              adaptive, max_leaf_size, ct, ci, nodes, tri_out, n_interior, n_leaves);
+    refs_to_tris(R, tri_out);
+    hxo_woopify(verts, tris, tri_out, n, woop);
+    return nn;
+}
+
+/* TRBuilder::build (TRBuilder.cpp:193-245): LBVH front half and hierarchy,
+ * refit, TR optimisation, collapse, woopify. */
+int64_t hxo_build_tr(const float* verts, const int32_t* tris, int64_t n_tri, const float* scene_box, int bits,
+                     int iterations, int adaptive, int max_leaf_size, float ct, float ci, int32_t* nodes,
+                     int32_t* tri_out, float* woop, int64_t* n_interior, int64_t* n_leaves, int64_t* accepted) {
+    float sb[6];
+    if (scene_box) std::memcpy(sb, scene_box, sizeof(sb));
+    else hxo_scene_box(verts, tris, n_tri, sb);
+    RefSet R = setup_refs(verts, tris, n_tri, sb, -1.0f);
+    const int64_t n = R.n;
+    std::vector<uint64_t> keys(n);
+    std::vector<int32_t> vals(n);
+    hxo_morton_codes(R.mn.data(), R.mx.data(), n, sb, bits, keys.data());
+    for (int64_t i = 0; i < n; ++i) vals[i] = (int32_t)i;
+    hxo_sort_pairs(keys.data(), vals.data(), n, bits);
+    const int64_t N = std::max<int64_t>(2 * n - 1, 1);
+    std::vector<int32_t> left(N), right(N), parent(N);
+    std::vector<float> nmin(4 * N), nmax(4 * N);
+    if (n > 1) hxo_construct(keys.data(), n, left.data(), right.data(), parent.data());
+    hxo_refit(vals.data(), R.mn.data(), R.mx.data(), n, left.data(), right.data(), parent.data(), nmin.data(),
+              nmax.data());
+    const int64_t acc = hxo_tr(n, iterations, ct, ci, left.data(), right.data(), parent.data(), nmin.data(),
+                               nmax.data());
     if (accepted) *accepted = acc;
     int64_t nn = hxo_collapse(n, vals.data(), left.data(), right.data(), parent.data(), nmin.data(), nmax.data(),
                               adaptive, max_leaf_size, ct, ci, nodes, tri_out, n_interior, n_leaves);

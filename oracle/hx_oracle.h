@@ -162,6 +162,18 @@ int64_t hxo_build_atr(const float* verts, const int32_t* tris, int64_t n_tri, co
                       int32_t* nodes, int32_t* tri_out, float* woop, int64_t* n_interior, int64_t* n_leaves,
                       int64_t* accepted, float presplit_beta);
 
+/* TR optimisation (TRBuilder.cpp:140-191, TRBuilderKernels.cu:11-476) of the
+ * binary tree in place with treelets of 7 leaves: the optimal restructuring of
+ * each treelet by the subset DP, the reference's warp-32 tie rules emulated.
+ * Returns the number of treelets whose restructuring was kept. */
+int64_t hxo_tr(int64_t n, int iterations, float ct, float ci, int32_t* left, int32_t* right, int32_t* parent,
+               float* nmin, float* nmax);
+
+/* TRBuilder::build (TRBuilder.cpp:193-245). */
+int64_t hxo_build_tr(const float* verts, const int32_t* tris, int64_t n_tri, const float* scene_box, int bits,
+                     int iterations, int adaptive, int max_leaf_size, float ct, float ci, int32_t* nodes,
+                     int32_t* tri_out, float* woop, int64_t* n_interior, int64_t* n_leaves, int64_t* accepted);
+
 /* HLBVH upper levels over a Karras tree (left/right/parent, boxes 2n-1) of the
  * sorted keys, in place; returns the number of clusters (1: unchanged). */
 int64_t hxo_hlbvh(int64_t n, int bits, int sah_bits, const uint64_t* sorted_keys, int32_t* left, int32_t* right,

@@ -59,6 +59,9 @@ def lib():
             "hxo_atr": (_i64, [_i64, _i32, _i32, _f32, _f32, _p, _p, _p, _p, _p]),
             "hxo_build_atr": (_i64, [_p, _p, _i64, _p, _i32, _i32, _i32, _i32, _i32, _f32, _f32,
                                      _p, _p, _p, _p, _p, _p, _f32]),
+            "hxo_tr": (_i64, [_i64, _i32, _f32, _f32, _p, _p, _p, _p, _p]),
+            "hxo_build_tr": (_i64, [_p, _p, _i64, _p, _i32, _i32, _i32, _i32, _f32, _f32,
+                                    _p, _p, _p, _p, _p, _p]),
             "hxo_hlbvh": (_i64, [_i64, _i32, _i32, _p, _p, _p, _p, _p, _p]),
             "hxo_build_hlbvh": (_i64, [_p, _p, _i64, _p, _i32, _i32, _i32, _i32, _f32, _f32,
                                        _p, _p, _p, _p, _p, _p, _f32]),
@@ -357,6 +360,31 @@ def build_atr(verts, tris, bits=60, treelet_size=9, iterations=2, scene_box=None
     nn = lib().hxo_build_atr(_ptr(v), _ptr(t), t.shape[0], _ptr(sb), int(bits), int(treelet_size), int(iterations),
                              int(bool(adaptive)), int(max_leaf_size), float(ct), float(ci), _ptr(nodes), _ptr(tri),
                              _ptr(woop), ctypes.byref(ni), ctypes.byref(nl), ctypes.byref(acc), beta)
+    return dict(nodes=nodes[:nn].copy(), tri_idx=tri, woop=woop, n_interior=ni.value, n_leaves=nl.value,
+                accepted=acc.value)
+
+
+def tr(left, right, parent, nmin, nmax, iterations=2, ct=3.0, ci=2.0):
+    """TR optimisation of a binary tree (copies); returns left, right, parent, nmin, nmax, accepted."""
+    l, r, p = (_c(a, np.int32).copy() for a in (left, right, parent))
+    mn, mx = _c(nmin, np.float32).copy(), _c(nmax, np.float32).copy()
+    n = (l.shape[0] + 1) // 2
+    acc = lib().hxo_tr(n, int(iterations), float(ct), float(ci), _ptr(l), _ptr(r), _ptr(p), _ptr(mn), _ptr(mx))
+    return l, r, p, mn, mx, acc
+
+
+def build_tr(verts, tris, bits=60, iterations=2, scene_box=None, adaptive=True, max_leaf_size=8, ct=3.0, ci=2.0):
+    v, t = _c(verts, np.float32), _c(tris, np.int32)
+    sb = None if scene_box is None else _c(scene_box, np.float32)
+    n = t.shape[0]
+    N = max(2 * n - 1, 2)
+    nodes = np.zeros((N, 16), np.int32)
+    tri = np.zeros(n, np.int32)
+    woop = np.zeros((n, 12), np.float32)
+    ni, nl, acc = ctypes.c_int64(0), ctypes.c_int64(0), ctypes.c_int64(0)
+    nn = lib().hxo_build_tr(_ptr(v), _ptr(t), n, _ptr(sb), int(bits), int(iterations), int(bool(adaptive)),
+                            int(max_leaf_size), float(ct), float(ci), _ptr(nodes), _ptr(tri), _ptr(woop),
+                            ctypes.byref(ni), ctypes.byref(nl), ctypes.byref(acc))
     return dict(nodes=nodes[:nn].copy(), tri_idx=tri, woop=woop, n_interior=ni.value, n_leaves=nl.value,
                 accepted=acc.value)
 
