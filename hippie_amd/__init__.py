@@ -6,12 +6,12 @@ this package is the host-side mirror of hippie's builder/tracer interface.
 from . import _lib  # noqa: F401
 from . import formats, raygen  # noqa: F401
 from .core import (ATRBuilder, BuildParams, BVHBuilder, Context, HipBatch, HipBVH, HipTracer, HLBVHBuilder,  # noqa: F401
-                   InsertionBuilder, LBVHBuilder, MeshBatch, PLOCBuilder, RayBuffer, Scene, morton_codes, normalize_vertices, ray_keys, scene_box, sort_pairs,
-                   trace_view)
+                   InsertionBuilder, LBVHBuilder, MeshBatch, PLOCBuilder, RayBuffer, Scene, TRBuilder, morton_codes,
+                   normalize_vertices, ray_keys, scene_box, sort_pairs, trace_view)
 
 __all__ = [
     "ATRBuilder", "BuildParams", "BVHBuilder", "Context", "HipBatch", "HipBVH", "HipTracer", "HLBVHBuilder", "InsertionBuilder",
     "LBVHBuilder",
-    "MeshBatch", "PLOCBuilder", "RayBuffer", "Scene", "morton_codes", "normalize_vertices", "ray_keys", "scene_box", "sort_pairs",
+    "MeshBatch", "PLOCBuilder", "RayBuffer", "Scene", "TRBuilder", "morton_codes", "normalize_vertices", "ray_keys", "scene_box", "sort_pairs",
     "trace_view",
 ]

@@ -21,6 +21,7 @@ HX_METHOD_HLBVH = 1
 HX_METHOD_PLOC = 2
 HX_METHOD_ATRBVH = 3
 HX_METHOD_INSERTION = 4
+HX_METHOD_TR = 5
 HX_LAYOUT_BIN = 2
 HX_LAYOUT_QUAD = 4
 HX_LAYOUT_OCT = 8
@@ -43,6 +44,7 @@ class HxBuildParams(ctypes.Structure):
         ("ploc_adaptive", ctypes.c_int32),
         ("atr_iterations", ctypes.c_int32),
         ("atr_treelet_size", ctypes.c_int32),
+        ("tr_iterations", ctypes.c_int32),
         ("hlbvh_sah_bits", ctypes.c_int32),
         ("insertion_mod", ctypes.c_int32),
         ("layout", ctypes.c_int32),

@@ -25,6 +25,7 @@ METHODS = {
     "ploc": _lib.HX_METHOD_PLOC,
     "atr": _lib.HX_METHOD_ATRBVH,
     "insertion": _lib.HX_METHOD_INSERTION,
+    "tr": _lib.HX_METHOD_TR,
 }
 LAYOUTS = {"bin": _lib.HX_LAYOUT_BIN, "quad": _lib.HX_LAYOUT_QUAD, "oct": _lib.HX_LAYOUT_OCT}
 RAY_KEY_METHODS = {"twopoint": _lib.HX_RAYKEY_TWOPOINT, "aila": _lib.HX_RAYKEY_AILA}
@@ -144,10 +145,11 @@ class BuildParams:
         "Bvh.presplitting": "presplitting", "Bvh.presplitterBeta": "presplitter_beta",
         "Bvh.plocRadius": "ploc_radius", "Bvh.plocMaxRadius": "ploc_max_radius",
         "Bvh.plocAdaptive": "ploc_adaptive", "Bvh.atrIterations": "atr_iterations",
+        "Bvh.trIterations": "tr_iterations",
         "Bvh.hlbvhMortonCodeSAHBits": "hlbvh_sah_bits", "Bvh.insertionMod": "insertion_mod",
     }
     BITS_KEYS = {"lbvh": "Bvh.lbvhMortonCodeBits", "ploc": "Bvh.plocMortonCodeBits",
-                 "atr": "Bvh.atrMortonCodeBits", "hlbvh": "Bvh.hlbvhMortonCodeBits",
+                 "atr": "Bvh.atrMortonCodeBits", "tr": "Bvh.trMortonCodeBits", "hlbvh": "Bvh.hlbvhMortonCodeBits",
                  "insertion": "Bvh.insertionMortonCodeBits"}
 
     def __init__(self, **kw):
@@ -418,6 +420,23 @@ class ATRBuilder(BVHBuilder):
         if not (0 <= k <= 25):
             raise ValueError("iterations must be in range of [0,25]")
         self.params.atr_iterations = k
+
+
+class TRBuilder(BVHBuilder):
+    """TRBuilder (rt/bvh/TRBuilder.h, TRBuilder.cpp:281-332): LBVH plus optimal
+    restructuring of treelets of 7 leaves (Karras and Aila 2013).  The treelet
+    size is fixed at 7, as TRBuilder::setTreeletSize accepts only 7."""
+
+    method = "tr"
+
+    def set_treelet_size(self, t):
+        if t != 7:
+            raise ValueError("treelet size must be in range of [7,7]")
+
+    def set_iterations(self, k):
+        if not (0 <= k <= 25):
+            raise ValueError("iterations must be in range of [0,25]")
+        self.params.tr_iterations = k
 
 
 class InsertionBuilder(BVHBuilder):

@@ -211,7 +211,7 @@ int validate_params(const hx_build_params* p) {
     HX_CHECK_ARG(p->layout == HX_LAYOUT_BIN || p->layout == HX_LAYOUT_QUAD || p->layout == HX_LAYOUT_OCT,
                  "layout must be HX_LAYOUT_BIN, HX_LAYOUT_QUAD or HX_LAYOUT_OCT (got %d)", p->layout);
     if (p->method != HX_METHOD_LBVH && p->method != HX_METHOD_PLOC && p->method != HX_METHOD_ATRBVH &&
-        p->method != HX_METHOD_HLBVH && p->method != HX_METHOD_INSERTION) {
+        p->method != HX_METHOD_HLBVH && p->method != HX_METHOD_INSERTION && p->method != HX_METHOD_TR) {
         hx::set_error("builder method %d is not available in this build", p->method);
         return HX_ERR_UNSUPPORTED;
     }
@@ -238,6 +238,11 @@ int validate_params(const hx_build_params* p) {
                      "atr_treelet_size must be in [4,30] (got %d)", p->atr_treelet_size);
         HX_CHECK_ARG(p->atr_iterations >= 0 && p->atr_iterations <= 25, "atr_iterations must be in [0,25] (got %d)",
                      p->atr_iterations);
+    }
+    if (p->method == HX_METHOD_TR) {
+        // TRBuilder::setIterations (TRBuilder.cpp:329-332).
+        HX_CHECK_ARG(p->tr_iterations >= 0 && p->tr_iterations <= 25, "tr_iterations must be in [0,25] (got %d)",
+                     p->tr_iterations);
     }
     if (p->presplitting) {
         // Presplitter::setBeta (Presplitter.cpp:328-331).
@@ -318,7 +323,7 @@ int reserve_bvh(hx_bvh* b, int64_t n) {
         HX_TRY(b->node_at.reserve(4 * (size_t)n));
         HX_TRY(b->ploc_work.reserve(ploc_work_bytes(n)));
     }
-    if (b->params.method == HX_METHOD_ATRBVH) {
+    if (b->params.method == HX_METHOD_ATRBVH || b->params.method == HX_METHOD_TR) {
         HX_TRY(b->split.reserve(4 * (size_t)n));
         HX_TRY(b->node_at.reserve(4 * (size_t)n));
         HX_TRY(b->atr_work.reserve(atr_work_bytes(n)));
@@ -604,6 +609,7 @@ hx::PlocArgs relabel_args(hx_bvh* b) {
 // relabelling, the general climb and the collapse.
 int run_atr(hx_bvh* b) {
     Context* c = &b->ctx->c;
+    const bool tr = b->params.method == HX_METHOD_TR;
     const hx_build_params& p = b->params;
     const int64_t n = b->n_refs;
     HX_TRY(hx::launch_construct(c, b->keys[b->keys_index].ptr, b->keys64, n, b->left.as<int32_t>(),
@@ -618,8 +624,9 @@ int run_atr(hx_bvh* b) {
     };
     hx::AtrArgs a{};
     a.n = n;
-    a.treelet_size = p.atr_treelet_size;
-    a.iterations = p.atr_iterations;
+    // TRBuilder: treelets of 7 leaves, Bvh.trIterations (TRBuilder.cpp:281-332).
+    a.treelet_size = tr ? 7 : p.atr_treelet_size;
+    a.iterations = tr ? p.tr_iterations : p.atr_iterations;
     a.ct = p.ct;
     a.ci = p.ci;
     a.left = b->left.as<int32_t>();
@@ -643,7 +650,8 @@ int run_atr(hx_bvh* b) {
     ca.atr_height = a.height;
     HX_TRY(hx::launch_climb(c, ca));
     const hx::PlocArgs rl = relabel_args(b);
-    HX_TRY(hx::launch_atr(c, a, rl, &b->atr_accepted));
+    if (tr) HX_TRY(hx::launch_tr(c, a, rl, &b->atr_accepted));
+    else HX_TRY(hx::launch_atr(c, a, rl, &b->atr_accepted));
     b->general = true;
     std::swap(b->tri_idx.ptr, b->vals_tmp.ptr);
     std::swap(b->tri_idx.bytes, b->vals_tmp.bytes);
@@ -938,7 +946,7 @@ int run_build(hx_bvh* b) {
                                      b->woop.as<float4>(), b->nodes.as<int4>(), b->counts.as<int32_t>()));
     } else if (p.method == HX_METHOD_PLOC) {
         HX_TRY(run_ploc(b));
-    } else if (p.method == HX_METHOD_ATRBVH) {
+    } else if (p.method == HX_METHOD_ATRBVH || p.method == HX_METHOD_TR) {
         HX_TRY(run_atr(b));
     } else if (p.method == HX_METHOD_HLBVH) {
         HX_TRY(run_hlbvh(b));
@@ -1288,6 +1296,7 @@ int hx_build_params_default(hx_build_params* p) {
     p->ploc_adaptive = 0;
     p->atr_iterations = 2;
     p->atr_treelet_size = 20;
+    p->tr_iterations = 2;
     p->hlbvh_sah_bits = 15;
     p->insertion_mod = 1;
     p->layout = HX_LAYOUT_BIN;

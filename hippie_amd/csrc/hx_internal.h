@@ -304,6 +304,9 @@ struct AtrArgs {
 constexpr int kAtrMaxLevels = 1024;
 // relabel: fields of a PLOC argument block used by launch_tree_down.
 int launch_atr(Context* c, const AtrArgs& a, const PlocArgs& relabel, int64_t* accepted_out);
+// TR (TRBuilder.cpp:140-191): the same level driver with the optimal 7-leaf
+// treelet restructuring; a.treelet_size must be 7.
+int launch_tr(Context* c, const AtrArgs& a, const PlocArgs& relabel, int64_t* accepted_out);
 // Depth-first relabelling of any binary tree whose internal nodes carry leaf
 // counts (a.refs) and heights (a.height): nodes are bucketed by height and
 // placed top down.  Uses a.n, refs, height, list, hist, cursor.

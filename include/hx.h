@@ -63,6 +63,7 @@ extern "C" {
 #define HX_METHOD_PLOC 2
 #define HX_METHOD_ATRBVH 3
 #define HX_METHOD_INSERTION 4
+#define HX_METHOD_TR 5         /* "tr": TRBuilder, treelets of 7 (TRBuilder.cpp:140-347) */
 
 #define HX_LAYOUT_BIN 2
 #define HX_LAYOUT_QUAD 4   /* HipBVHNodeQuad, 128 B (HipBVHNode.h:13-114), collapsed as BVH::collapseWide */
@@ -80,7 +81,7 @@ typedef struct hx_bvh hx_bvh;
  * equal the registered defaults there. */
 typedef struct hx_build_params {
     int32_t method;              /* Bvh.method                                */
-    int32_t morton_bits;         /* Bvh.{lbvh,ploc,atr,...}MortonCodeBits [6,60] */
+    int32_t morton_bits;         /* Bvh.{lbvh,ploc,atr,tr,...}MortonCodeBits [6,60] */
     int32_t max_leaf_size;       /* Bvh.maxLeafSize (0,1024]                  */
     int32_t adaptive_leaf_size;  /* Bvh.adaptiveLeafSize                      */
     float ct;                    /* Bvh.ct                                    */
@@ -92,6 +93,7 @@ typedef struct hx_build_params {
     int32_t ploc_adaptive;       /* Bvh.plocAdaptive                          */
     int32_t atr_iterations;      /* Bvh.atrIterations [0,25]                  */
     int32_t atr_treelet_size;    /* ATRBuilder::setTreeletSize [4,30]         */
+    int32_t tr_iterations;       /* Bvh.trIterations [0,25]                   */
     int32_t hlbvh_sah_bits;      /* Bvh.hlbvhMortonCodeSAHBits                */
     int32_t insertion_mod;       /* Bvh.insertionMod                          */
     int32_t layout;              /* Bvh.layout: HX_LAYOUT_BIN / _QUAD / _OCT  */
