@@ -6,8 +6,8 @@ depth (HipTracerBinKernels.cu:5-15), radix digit width (RadixSortKernels.h
 measurement on the caller's own workload:
 
   sort_digit_bits      radix digit width of every sort       8, 10
-  sort_items           keys per thread in a sort pass        16, 32
-  sort_lookback        tiles read per look-back round trip   4, 8
+  sort_items           keys per thread in a sort pass        8, 16, 32
+  sort_lookback        tiles read per look-back round trip   4, 8, 16
   trace_lds_stack      traversal-stack entries kept in LDS   0, 16, 24, 32, 64
   trace_fetch          wave refill threshold (busy lanes)    16 .. 64
   trace_blocks_per_cu  persistent trace workgroups per CU    0 (occupancy), 1, 2, 4
@@ -30,8 +30,8 @@ import torch
 from . import core
 
 SORT_DIGITS = (8, 10)
-SORT_ITEMS = (16, 32)
-SORT_LOOKBACK = (4, 8)
+SORT_ITEMS = (8, 16, 32)
+SORT_LOOKBACK = (4, 8, 16)
 LDS_STACK = (0, 16, 24, 32, 64)
 FETCH = (16, 24, 32, 40, 48, 56, 64)
 BLOCKS_PER_CU = (0, 1, 2, 4)

@@ -1242,10 +1242,10 @@ int hx_context_set_tuning(hx_context* ctx, const char* name, int32_t value) {
         HX_CHECK_ARG(value == 8 || value == 10, "sort_digit_bits must be 8 or 10");
         t.sort_digit_bits = value;
     } else if (!std::strcmp(name, "sort_lookback")) {
-        HX_CHECK_ARG(value == 4 || value == 8, "sort_lookback must be 4 or 8");
+        HX_CHECK_ARG(value == 4 || value == 8 || value == 16, "sort_lookback must be 4, 8 or 16");
         t.sort_lookback = value;
     } else if (!std::strcmp(name, "sort_items")) {
-        HX_CHECK_ARG(value == 16 || value == 32, "sort_items must be 16 or 32");
+        HX_CHECK_ARG(value == 8 || value == 16 || value == 32, "sort_items must be 8, 16 or 32");
         t.sort_items = value;
     } else if (!std::strcmp(name, "sort_local_run")) {
         HX_CHECK_ARG(value >= 0 && value <= 2048, "sort_local_run must be in [0, 2048]");

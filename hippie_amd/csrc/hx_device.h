@@ -108,28 +108,44 @@ HX_DEV int wave_sum_i32(int v) {
     return v;
 }
 
-// Shuffle of any 4-byte-multiple type, word by word.
-template <class T>
-HX_DEV T shfl_up_any(T v, int d) {
+// DPP move of every 32-bit word of v (GFX9 controls: row_shr:n = 0x110 + n,
+// row_bcast:15 = 0x142, row_bcast:31 = 0x143).  Lanes of rows outside
+// ROW_MASK, or whose source lies outside the pattern, keep their own value;
+// callers combine only where the source is valid.
+constexpr int kDppRowShr1 = 0x111, kDppRowShr2 = 0x112, kDppRowShr4 = 0x114, kDppRowShr8 = 0x118;
+constexpr int kDppRowBcast15 = 0x142, kDppRowBcast31 = 0x143;
+template <int CTRL, int ROW_MASK, class T>
+HX_DEV T dpp_move(T v) {
     static_assert(sizeof(T) % 4 == 0, "word-sized types only");
     int w[sizeof(T) / 4];
     __builtin_memcpy(w, &v, sizeof(T));
 #pragma unroll
-    for (int k = 0; k < (int)(sizeof(T) / 4); ++k) w[k] = __shfl_up(w[k], d, kWave);
+    for (int k = 0; k < (int)(sizeof(T) / 4); ++k) w[k] = __builtin_amdgcn_update_dpp(w[k], w[k], CTRL, ROW_MASK, 0xF, false);
     T r;
     __builtin_memcpy(&r, w, sizeof(T));
     return r;
 }
 
-// Inclusive scan over the 64 lanes of a wave (Hillis-Steele on shuffles).
+// Inclusive scan over the 64 lanes of a wave, all in DPP moves (no LDS
+// traffic): Hillis-Steele within each row of 16 lanes, then the row totals
+// carried by row_bcast:15 (rows 1 and 3 take lanes 15 and 47) and
+// row_bcast:31 (rows 2 and 3 take lane 31).  Op must be associative.
 template <class T, class Op>
 HX_DEV T wave_inclusive_scan(T v, Op op) {
     const int lane = lane_id();
-#pragma unroll
-    for (int d = 1; d < kWave; d <<= 1) {
-        T o = shfl_up_any(v, d);
-        if (lane >= d) v = op(o, v);
-    }
+    const int r = lane & 15;
+    T o = dpp_move<kDppRowShr1, 0xF>(v);
+    if (r >= 1) v = op(o, v);
+    o = dpp_move<kDppRowShr2, 0xF>(v);
+    if (r >= 2) v = op(o, v);
+    o = dpp_move<kDppRowShr4, 0xF>(v);
+    if (r >= 4) v = op(o, v);
+    o = dpp_move<kDppRowShr8, 0xF>(v);
+    if (r >= 8) v = op(o, v);
+    o = dpp_move<kDppRowBcast15, 0xA>(v);
+    if (lane & 16) v = op(o, v);
+    o = dpp_move<kDppRowBcast31, 0xC>(v);
+    if (lane >= 32) v = op(o, v);
     return v;
 }
 

@@ -14,7 +14,8 @@
 // M^-1 has rows (e2 x c, c x e1, e1 x e2) / det with c = -d, det = e1.(e2 x c).
 // The solve runs in FP64 (cheap on MI355X, and it keeps the adjoint exact to
 // FP32 rounding of the result even for nearly grazing rays); the vertex
-// gradients are scatter-added with hardware FP32 atomics.
+// gradients are merged per workgroup in LDS and scatter-added with hardware
+// FP32 atomics.
 #include "hx_device.h"
 #include "hx_internal.h"
 
@@ -69,28 +70,119 @@ HX_DEV int hit_triangle(const BackwardArgs& a, int64_t r, int id) {
     return a.ray_mesh ? a.mesh_off[a.ray_mesh[r]] + id : id;
 }
 
-__global__ __launch_bounds__(kDiffBlock) void k_backward(BackwardArgs a) {
-    const int64_t r = blockIdx.x * (int64_t)kDiffBlock + threadIdx.x;
-    if (r >= a.n_rays) return;
-    const int id = a.results[r].x;
-    if (id < 0) return;
-    const double gt = a.dl_dt ? (double)a.dl_dt[r] : 0.0;
-    const double gu = a.dl_du ? (double)a.dl_du[r] : 0.0;
-    const double gv = a.dl_dv ? (double)a.dl_dv[r] : 0.0;
-    if (gt == 0.0 && gu == 0.0 && gv == 0.0) return;
-    Hit h;
-    if (!solve_hit(a, r, hit_triangle(a, r, id), h)) return;
-    const double w = 1.0 - h.u - h.v;
-    const D3 lam{-(h.bc.x * gu + h.ca.x * gv + h.ab.x * gt) * h.inv, -(h.bc.y * gu + h.ca.y * gv + h.ab.y * gt) * h.inv,
-                 -(h.bc.z * gu + h.ca.z * gv + h.ab.z * gt) * h.inv};
-    float* g = a.dl_dverts;
-    const double beta[3] = {h.u, h.v, w};
-    const int vid[3] = {h.i0, h.i1, h.i2};
+// Merge of the vertex contributions before the global atomics, in two levels:
+// neighbouring lanes that hit the same triangle are summed in registers (a
+// segmented DPP scan), then the rays of one workgroup that hit the same or
+// adjacent triangles (coherent camera rays hit each triangle several times)
+// add into one LDS slot per vertex, and each distinct vertex costs three
+// global atomics instead of three per ray.  Open addressing over kAggSlots slots; an insert that finds no slot
+// within kAggProbes probes adds straight to global memory, so incoherent rays
+// cost a few LDS probes more and nothing else.
+constexpr int kAggSlots = 1024;
+
+// The three vertex contributions of one ray (or of a run of rays).
+struct Contrib {
+    float g[9];
+    HX_DEV void add(const Contrib& o) {
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        atomicAdd(g + 3 * (int64_t)vid[k] + 0, (float)(beta[k] * lam.x));
-        atomicAdd(g + 3 * (int64_t)vid[k] + 1, (float)(beta[k] * lam.y));
-        atomicAdd(g + 3 * (int64_t)vid[k] + 2, (float)(beta[k] * lam.z));
+        for (int k = 0; k < 9; ++k) g[k] += o.g[k];
+    }
+};
+constexpr int kAggProbes = 8;
+constexpr int32_t kAggEmpty = -1;
+
+HX_DEV void add_vertex(int32_t* keys, float* vx, float* vy, float* vz, float* g, int vid, float x, float y, float z) {
+    uint32_t h = ((uint32_t)vid * 2654435761u) >> (32 - 10);
+#pragma unroll 1
+    for (int probe = 0; probe < kAggProbes; ++probe) {
+        const int32_t old = atomicCAS(&keys[h], kAggEmpty, vid);
+        if (old == kAggEmpty || old == vid) {
+            atomicAdd(&vx[h], x);
+            atomicAdd(&vy[h], y);
+            atomicAdd(&vz[h], z);
+            return;
+        }
+        h = (h + 1) & (kAggSlots - 1);
+    }
+    atomicAdd(g + 3 * (int64_t)vid + 0, x);
+    atomicAdd(g + 3 * (int64_t)vid + 1, y);
+    atomicAdd(g + 3 * (int64_t)vid + 2, z);
+}
+
+__global__ __launch_bounds__(kDiffBlock) void k_backward(BackwardArgs a) {
+    static_assert(kAggSlots == 1 << 10, "the hash takes 10 bits");
+    __shared__ int32_t s_key[kAggSlots];
+    __shared__ float s_x[kAggSlots], s_y[kAggSlots], s_z[kAggSlots];
+    for (int i = threadIdx.x; i < kAggSlots; i += kDiffBlock) {
+        s_key[i] = kAggEmpty;
+        s_x[i] = 0.0f;
+        s_y[i] = 0.0f;
+        s_z[i] = 0.0f;
+    }
+    __syncthreads();
+    float* g = a.dl_dverts;
+    const int64_t r = blockIdx.x * (int64_t)kDiffBlock + threadIdx.x;
+    const int id = r < a.n_rays ? a.results[r].x : -1;
+    double gt = 0.0, gu = 0.0, gv = 0.0;
+    if (id >= 0) {
+        gt = a.dl_dt ? (double)a.dl_dt[r] : 0.0;
+        gu = a.dl_du ? (double)a.dl_du[r] : 0.0;
+        gv = a.dl_dv ? (double)a.dl_dv[r] : 0.0;
+    }
+    // Per ray: its triangle (-1 without a contribution) and the three vertex
+    // contributions beta_k * lambda.
+    Hit h;
+    int tri = -1;
+    Contrib cv{};
+    if ((gt != 0.0 || gu != 0.0 || gv != 0.0)) {
+        tri = hit_triangle(a, r, id);
+        if (solve_hit(a, r, tri, h)) {
+            const double w = 1.0 - h.u - h.v;
+            const D3 lam{-(h.bc.x * gu + h.ca.x * gv + h.ab.x * gt) * h.inv,
+                         -(h.bc.y * gu + h.ca.y * gv + h.ab.y * gt) * h.inv,
+                         -(h.bc.z * gu + h.ca.z * gv + h.ab.z * gt) * h.inv};
+            const double beta[3] = {h.u, h.v, w};
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                cv.g[3 * k + 0] = (float)(beta[k] * lam.x);
+                cv.g[3 * k + 1] = (float)(beta[k] * lam.y);
+                cv.g[3 * k + 2] = (float)(beta[k] * lam.z);
+            }
+        } else {
+            tri = -1;
+        }
+    }
+    // Neighbouring lanes that hit the same triangle (runs inside each row of
+    // 16 lanes) are summed by a segmented DPP scan; the run's last lane holds
+    // the sum and adds it for all of them.
+    const int lane = lane_id();
+    const int prev = __shfl_up(tri, 1, kWave);
+    const uint64_t heads = ballot64((lane & 15) == 0 || tri != prev);
+    const uint64_t upto = lane == kWave - 1 ? ~0ull : ((2ull << lane) - 1ull);
+    const int dist = lane - (63 - __clzll(heads & upto));
+    Contrib o = dpp_move<kDppRowShr1, 0xF>(cv);
+    if (dist >= 1) cv.add(o);
+    o = dpp_move<kDppRowShr2, 0xF>(cv);
+    if (dist >= 2) cv.add(o);
+    o = dpp_move<kDppRowShr4, 0xF>(cv);
+    if (dist >= 4) cv.add(o);
+    o = dpp_move<kDppRowShr8, 0xF>(cv);
+    if (dist >= 8) cv.add(o);
+    const bool last = lane == kWave - 1 || ((heads >> (lane + 1)) & 1ull);
+    if (last && tri >= 0) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const int vid = k == 0 ? h.i0 : k == 1 ? h.i1 : h.i2;
+            add_vertex(s_key, s_x, s_y, s_z, g, vid, cv.g[3 * k], cv.g[3 * k + 1], cv.g[3 * k + 2]);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kAggSlots; i += kDiffBlock) {
+        const int vid = s_key[i];
+        if (vid == kAggEmpty) continue;
+        atomicAdd(g + 3 * (int64_t)vid + 0, s_x[i]);
+        atomicAdd(g + 3 * (int64_t)vid + 1, s_y[i]);
+        atomicAdd(g + 3 * (int64_t)vid + 2, s_z[i]);
     }
 }
 

@@ -140,7 +140,7 @@ __global__ __launch_bounds__(kSortBlock) void k_sort_scan_hist(uint32_t* __restr
 }
 
 template <class K, bool IDENT, int RB, int IT, int LB>
-__global__ __launch_bounds__(kSortBlock) __attribute__((amdgpu_waves_per_eu(IT <= 16 ? 4 : 2))) void k_onesweep(const K* __restrict__ keys_in, K* __restrict__ keys_out,
+__global__ __launch_bounds__(kSortBlock) __attribute__((amdgpu_waves_per_eu(IT <= 8 ? 6 : IT <= 16 ? 4 : 2))) void k_onesweep(const K* __restrict__ keys_in, K* __restrict__ keys_out,
                                                          const int32_t* __restrict__ vals_in,
                                                          int32_t* __restrict__ vals_out, int64_t n, int shift,
                                                          int nbits, const uint32_t* __restrict__ digit_offsets,
@@ -583,14 +583,24 @@ int launch_sort_pairs(Context* c, void* keys[2], int32_t* vals[2], int64_t n, in
         return sort_impl<uint32_t, RB, IT, LB>(c, keys, vals, n, 0, end_bit, vals_identity, result_index, nullptr, \
                                                ws);                                                               \
     }
+    HX_SORT_CASE(8, 8, 4)
+    HX_SORT_CASE(8, 8, 8)
+    HX_SORT_CASE(8, 8, 16)
     HX_SORT_CASE(8, 16, 4)
     HX_SORT_CASE(8, 16, 8)
+    HX_SORT_CASE(8, 16, 16)
     HX_SORT_CASE(8, 32, 4)
     HX_SORT_CASE(8, 32, 8)
+    HX_SORT_CASE(8, 32, 16)
+    HX_SORT_CASE(10, 8, 4)
+    HX_SORT_CASE(10, 8, 8)
+    HX_SORT_CASE(10, 8, 16)
     HX_SORT_CASE(10, 16, 4)
     HX_SORT_CASE(10, 16, 8)
+    HX_SORT_CASE(10, 16, 16)
     HX_SORT_CASE(10, 32, 4)
     HX_SORT_CASE(10, 32, 8)
+    HX_SORT_CASE(10, 32, 16)
 #undef HX_SORT_CASE
     set_error("radix sort: no kernel for %d-bit digits, %d keys per thread, look-back %d", bits, items, lb);
     return HX_ERR_INVALID_ARG;

@@ -154,8 +154,8 @@ int hx_context_profile_reset(hx_context* ctx);
  *   "trace_fetch"          refill a wave when fewer of its 64 lanes are busy
  *   "trace_blocks_per_cu"  persistent trace workgroups per CU (0 = occupancy)
  *   "sort_digit_bits"      radix digit width of every sort (8 or 10)
- *   "sort_items"           keys per thread in a sort pass (16 or 32)
- *   "sort_lookback"        tiles read per sort look-back round trip (4, 8)
+ *   "sort_items"           keys per thread in a sort pass (8, 16 or 32)
+ *   "sort_lookback"        tiles read per sort look-back round trip (4, 8, 16)
  *   "sort_local_run"       the builds' Morton sorts and hx_sort_pairs_u64 of
  *                          > 40-bit keys with at least 2^18 keys run four
  *                          digit passes over the high bits, then rank each run
