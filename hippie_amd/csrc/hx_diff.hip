@@ -79,6 +79,7 @@ HX_DEV int hit_triangle(const BackwardArgs& a, int64_t r, int id) {
 // within kAggProbes probes adds straight to global memory, so incoherent rays
 // cost a few LDS probes more and nothing else.
 constexpr int kAggSlots = 1024;
+constexpr int kDiffRaysPerThread = 4;
 
 // The three vertex contributions of one ray (or of a run of rays).
 struct Contrib {
@@ -121,59 +122,62 @@ __global__ __launch_bounds__(kDiffBlock) void k_backward(BackwardArgs a) {
     }
     __syncthreads();
     float* g = a.dl_dverts;
-    const int64_t r = blockIdx.x * (int64_t)kDiffBlock + threadIdx.x;
-    const int id = r < a.n_rays ? a.results[r].x : -1;
-    double gt = 0.0, gu = 0.0, gv = 0.0;
-    if (id >= 0) {
-        gt = a.dl_dt ? (double)a.dl_dt[r] : 0.0;
-        gu = a.dl_du ? (double)a.dl_du[r] : 0.0;
-        gv = a.dl_dv ? (double)a.dl_dv[r] : 0.0;
-    }
-    // Per ray: its triangle (-1 without a contribution) and the three vertex
-    // contributions beta_k * lambda.
-    Hit h;
-    int tri = -1;
-    Contrib cv{};
-    if ((gt != 0.0 || gu != 0.0 || gv != 0.0)) {
-        tri = hit_triangle(a, r, id);
-        if (solve_hit(a, r, tri, h)) {
-            const double w = 1.0 - h.u - h.v;
-            const D3 lam{-(h.bc.x * gu + h.ca.x * gv + h.ab.x * gt) * h.inv,
-                         -(h.bc.y * gu + h.ca.y * gv + h.ab.y * gt) * h.inv,
-                         -(h.bc.z * gu + h.ca.z * gv + h.ab.z * gt) * h.inv};
-            const double beta[3] = {h.u, h.v, w};
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                cv.g[3 * k + 0] = (float)(beta[k] * lam.x);
-                cv.g[3 * k + 1] = (float)(beta[k] * lam.y);
-                cv.g[3 * k + 2] = (float)(beta[k] * lam.z);
-            }
-        } else {
-            tri = -1;
-        }
-    }
-    // Neighbouring lanes that hit the same triangle (runs inside each row of
-    // 16 lanes) are summed by a segmented DPP scan; the run's last lane holds
-    // the sum and adds it for all of them.
     const int lane = lane_id();
-    const int prev = __shfl_up(tri, 1, kWave);
-    const uint64_t heads = ballot64((lane & 15) == 0 || tri != prev);
-    const uint64_t upto = lane == kWave - 1 ? ~0ull : ((2ull << lane) - 1ull);
-    const int dist = lane - (63 - __clzll(heads & upto));
-    Contrib o = dpp_move<kDppRowShr1, 0xF>(cv);
-    if (dist >= 1) cv.add(o);
-    o = dpp_move<kDppRowShr2, 0xF>(cv);
-    if (dist >= 2) cv.add(o);
-    o = dpp_move<kDppRowShr4, 0xF>(cv);
-    if (dist >= 4) cv.add(o);
-    o = dpp_move<kDppRowShr8, 0xF>(cv);
-    if (dist >= 8) cv.add(o);
-    const bool last = lane == kWave - 1 || ((heads >> (lane + 1)) & 1ull);
-    if (last && tri >= 0) {
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            const int vid = k == 0 ? h.i0 : k == 1 ? h.i1 : h.i2;
-            add_vertex(s_key, s_x, s_y, s_z, g, vid, cv.g[3 * k], cv.g[3 * k + 1], cv.g[3 * k + 2]);
+    // Consecutive rays of kDiffRaysPerThread block-wide slices share the table.
+    for (int slice = 0; slice < kDiffRaysPerThread; ++slice) {
+        const int64_t r = (blockIdx.x * (int64_t)kDiffRaysPerThread + slice) * kDiffBlock + threadIdx.x;
+        const int id = r < a.n_rays ? a.results[r].x : -1;
+        double gt = 0.0, gu = 0.0, gv = 0.0;
+        if (id >= 0) {
+            gt = a.dl_dt ? (double)a.dl_dt[r] : 0.0;
+            gu = a.dl_du ? (double)a.dl_du[r] : 0.0;
+            gv = a.dl_dv ? (double)a.dl_dv[r] : 0.0;
+        }
+        // Per ray: its triangle (-1 without a contribution) and the three vertex
+        // contributions beta_k * lambda.
+        Hit h;
+        int tri = -1;
+        Contrib cv{};
+        if ((gt != 0.0 || gu != 0.0 || gv != 0.0)) {
+            tri = hit_triangle(a, r, id);
+            if (solve_hit(a, r, tri, h)) {
+                const double w = 1.0 - h.u - h.v;
+                const D3 lam{-(h.bc.x * gu + h.ca.x * gv + h.ab.x * gt) * h.inv,
+                             -(h.bc.y * gu + h.ca.y * gv + h.ab.y * gt) * h.inv,
+                             -(h.bc.z * gu + h.ca.z * gv + h.ab.z * gt) * h.inv};
+                const double beta[3] = {h.u, h.v, w};
+    #pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    cv.g[3 * k + 0] = (float)(beta[k] * lam.x);
+                    cv.g[3 * k + 1] = (float)(beta[k] * lam.y);
+                    cv.g[3 * k + 2] = (float)(beta[k] * lam.z);
+                }
+            } else {
+                tri = -1;
+            }
+        }
+        // Neighbouring lanes that hit the same triangle (runs inside each row of
+        // 16 lanes) are summed by a segmented DPP scan; the run's last lane holds
+        // the sum and adds it for all of them.
+        const int prev = __shfl_up(tri, 1, kWave);
+        const uint64_t heads = ballot64((lane & 15) == 0 || tri != prev);
+        const uint64_t upto = lane == kWave - 1 ? ~0ull : ((2ull << lane) - 1ull);
+        const int dist = lane - (63 - __clzll(heads & upto));
+        Contrib o = dpp_move<kDppRowShr1, 0xF>(cv);
+        if (dist >= 1) cv.add(o);
+        o = dpp_move<kDppRowShr2, 0xF>(cv);
+        if (dist >= 2) cv.add(o);
+        o = dpp_move<kDppRowShr4, 0xF>(cv);
+        if (dist >= 4) cv.add(o);
+        o = dpp_move<kDppRowShr8, 0xF>(cv);
+        if (dist >= 8) cv.add(o);
+        const bool last = lane == kWave - 1 || ((heads >> (lane + 1)) & 1ull);
+        if (last && tri >= 0) {
+    #pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const int vid = k == 0 ? h.i0 : k == 1 ? h.i1 : h.i2;
+                add_vertex(s_key, s_x, s_y, s_z, g, vid, cv.g[3 * k], cv.g[3 * k + 1], cv.g[3 * k + 2]);
+            }
         }
     }
     __syncthreads();
@@ -219,7 +223,8 @@ int launch_recompute_hits(Context* c, const BackwardArgs& a, float* t_out, float
 int launch_backward(Context* c, const BackwardArgs& a) {
     if (a.n_rays <= 0) return HX_OK;
     LaunchScope ls(c, "backward");
-    hipLaunchKernelGGL(k_backward, dim3((unsigned)((a.n_rays + kDiffBlock - 1) / kDiffBlock)), dim3(kDiffBlock), 0,
+    const int64_t per_block = (int64_t)kDiffBlock * kDiffRaysPerThread;
+    hipLaunchKernelGGL(k_backward, dim3((unsigned)((a.n_rays + per_block - 1) / per_block)), dim3(kDiffBlock), 0,
                        c->stream, a);
     return check_launch("backward");
 }
