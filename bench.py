@@ -374,9 +374,7 @@ def bench_config5(ctx, args):
             bv, sc = bvhs[m], scenes_m[m]
             bv.update(sc)
             tracer.set_bvh(bv)
-            rb = hx.RayBuffer.__new__(hx.RayBuffer)
-            rb.rays, rb.results, rb.closest_hit = rays_m[m], res_m[m], True
-            tracer.trace(rb)
+            tracer.trace(hx.RayBuffer.wrap(rays_m[m], res_m[m]))
             single_backward(ctx, sc.vertices, sc.triangles, rays_m[m], res_m[m], dl_dt=dl_m[m], out=grads_m[m])
 
     l_rebuild_ms = timed(loop_rebuild, 1, 0)
@@ -477,9 +475,7 @@ def bench_batch(ctx, args):
             bv, sc = bvhs[m], scenes_m[m]
             bv.update(sc)
             tracer.set_bvh(bv)
-            rb = hx.RayBuffer.__new__(hx.RayBuffer)
-            rb.rays, rb.results, rb.closest_hit = rays_m[m], res_m[m], True
-            tracer.trace(rb)
+            tracer.trace(hx.RayBuffer.wrap(rays_m[m], res_m[m]))
             single_backward(ctx, sc.vertices, sc.triangles, rays_m[m], res_m[m], dl_dt=dl_m[m], out=gm[m])
     ld_ms = timed(loop_step, steps, 1)
     best = graph_ms if graph_ms is not None else e_ms
@@ -546,7 +542,9 @@ def main():
         from hippie_amd.autotune import Autotuner
         at = Autotuner(ctx, repeats=2, log=log)
         tuning = dict(at.tune_sort(n, bits=60))
-        tuning.update(at.tune_trace(bvh, sec_rb, sort_method="aila"))
+        # each ray set keeps its own trace launch shape (RayBuffer.launch)
+        tuning["trace_primary"] = at.tune_trace(bvh, prim_rb, sort_method="aila", label="primary")
+        tuning["trace_diffuse"] = at.tune_trace(bvh, sec_rb, sort_method="aila", label="diffuse")
         log(f"autotuned: {tuning}")
     torch.cuda.synchronize()
     log(f"setup {time.perf_counter() - t_setup:.1f} s")

@@ -9,7 +9,7 @@ measurement on the caller's own workload:
   sort_items           keys per thread in a sort pass        8, 16, 32
   sort_lookback        tiles read per look-back round trip   4, 8, 16
   trace_lds_stack      traversal-stack entries kept in LDS   0, 16, 24, 32, 64
-  trace_fetch          wave refill threshold (busy lanes)    16 .. 64
+  trace_fetch          wave refill threshold (busy lanes)    8 .. 64
   trace_blocks_per_cu  persistent trace workgroups per CU    0 (occupancy), 1, 2, 4
   atr_treelet_size     ATRBVH treelet (quality vs time)      5 .. 16
 
@@ -33,7 +33,7 @@ SORT_DIGITS = (8, 10)
 SORT_ITEMS = (8, 16, 32)
 SORT_LOOKBACK = (4, 8, 16)
 LDS_STACK = (0, 16, 24, 32, 64)
-FETCH = (16, 24, 32, 40, 48, 56, 64)
+FETCH = (8, 16, 24, 32, 40, 48, 56, 64)
 BLOCKS_PER_CU = (0, 1, 2, 4)
 TREELETS = (5, 7, 9, 12, 16)
 
@@ -123,21 +123,32 @@ class Autotuner:
         self._save()
         return best
 
-    def tune_trace(self, bvh, rays, sort_method=None):
-        """Trace launch shape for this BVH and ray set (RayBuffer)."""
-        key = self._key(f"trace-{sort_method or 'unsorted'}", rays.size)
+    def tune_trace(self, bvh, rays, sort_method=None, label=""):
+        """Trace launch shape for this BVH and ray set (RayBuffer), kept in
+        rays.launch so that HipTracer applies it to this ray set's traces only
+        (coherent primaries and incoherent secondaries differ); the context's
+        own values are left as they were.  label: tells apart ray sets of one
+        size in the cache."""
+        key = self._key(f"trace-{sort_method or 'unsorted'}{'-' + label if label else ''}", rays.size)
         if key in self.cache:
-            self.apply(self.cache[key])
+            rays.launch = dict(self.cache[key])
             return self.cache[key]
         tracer = core.HipTracer(self.ctx, bvh)
+        knobs = [("trace_lds_stack", LDS_STACK), ("trace_fetch", FETCH), ("trace_blocks_per_cu", BLOCKS_PER_CU)]
+        saved = {k: self.ctx.get_tuning(k) for k, _ in knobs}
+        launch, rays.launch = rays.launch, {}
 
         def run():
             if sort_method:
                 tracer.trace_sort(rays, method=sort_method, bits=60)
             else:
                 tracer.trace(rays)
-        best = self._descend([("trace_lds_stack", LDS_STACK), ("trace_fetch", FETCH),
-                              ("trace_blocks_per_cu", BLOCKS_PER_CU)], run)
+        try:
+            best = self._descend(knobs, run)
+        finally:
+            self.apply(saved)
+            rays.launch = launch
+        rays.launch = dict(best)
         self.cache[key] = best
         self._save()
         return best

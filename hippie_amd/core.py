@@ -11,6 +11,7 @@ Class and method names follow the reference (src/hippie/):
 Invalid parameters raise ValueError instead of the reference's log-and-ignore.
 PyTorch only provides device memory and the stream; all work runs in libhx.
 """
+import contextlib
 import ctypes
 
 import numpy as np
@@ -457,7 +458,13 @@ class InsertionBuilder(BVHBuilder):
 
 
 class RayBuffer:
-    """Rays (float4 origin/tmin, float4 direction/tmax), results int4, stats int2."""
+    """Rays (float4 origin/tmin, float4 direction/tmax), results int4, stats int2.
+
+    ``launch``: trace tunables ("trace_fetch", "trace_lds_stack",
+    "trace_blocks_per_cu") that HipTracer applies to the context around every
+    trace of this ray set, e.g. from Autotuner.tune_trace: coherent primary
+    rays and incoherent secondaries favour different refill thresholds.  The
+    results do not depend on them."""
 
     def __init__(self, rays, device=0, closest_hit=True):
         r = torch.as_tensor(rays)
@@ -469,6 +476,16 @@ class RayBuffer:
         self.stats = None
         self.indices = None
         self.closest_hit = closest_hit
+        self.launch = {}
+
+    @classmethod
+    def wrap(cls, rays, results, closest_hit=True):
+        """A ray set over caller-held device tensors (rays (R,8) float32,
+        results (R,4) int32), no copy: e.g. one mesh's slice of a batch."""
+        rb = cls.__new__(cls)
+        rb.rays, rb.results, rb.closest_hit = rays, results, closest_hit
+        rb.stats, rb.indices, rb.launch = None, None, {}
+        return rb
 
     @property
     def size(self):
@@ -489,27 +506,45 @@ class HipTracer:
         if self.bvh is None:
             raise RuntimeError("HipTracer: no BVH set")
 
+    @contextlib.contextmanager
+    def _launch(self, rays):
+        """The ray set's trace tunables on the context for one call."""
+        if not rays.launch:
+            yield
+            return
+        old = {k: self.ctx.get_tuning(k) for k in rays.launch}
+        for k, v in rays.launch.items():
+            self.ctx.set_tuning(k, int(v))
+        try:
+            yield
+        finally:
+            for k, v in old.items():
+                self.ctx.set_tuning(k, v)
+
     def trace(self, rays):
         self._need_bvh()
-        call("hx_trace", self.ctx.handle, self.bvh.handle, _ptr(rays.rays), rays.size, int(not rays.closest_hit),
-             _ptr(rays.results), None)
+        with self._launch(rays):
+            call("hx_trace", self.ctx.handle, self.bvh.handle, _ptr(rays.rays), rays.size, int(not rays.closest_hit),
+                 _ptr(rays.results), None)
         return rays.results
 
     def trace_stats(self, rays):
         self._need_bvh()
         if rays.stats is None:
             rays.stats = torch.zeros((rays.size, 2), dtype=torch.int32, device=rays.rays.device)
-        call("hx_trace", self.ctx.handle, self.bvh.handle, _ptr(rays.rays), rays.size, int(not rays.closest_hit),
-             _ptr(rays.results), _ptr(rays.stats))
+        with self._launch(rays):
+            call("hx_trace", self.ctx.handle, self.bvh.handle, _ptr(rays.rays), rays.size, int(not rays.closest_hit),
+                 _ptr(rays.results), _ptr(rays.stats))
         return rays.results, rays.stats
 
     def trace_sort(self, rays, method="aila", bits=60, ray_length=0.25):
         self._need_bvh()
         if rays.indices is None:
             rays.indices = torch.zeros(rays.size, dtype=torch.int32, device=rays.rays.device)
-        call("hx_trace_sorted", self.ctx.handle, self.bvh.handle, _ptr(rays.rays), rays.size,
-             int(not rays.closest_hit), RAY_KEY_METHODS[method], int(bits), float(ray_length),
-             _ptr(rays.results), None, _ptr(rays.indices))
+        with self._launch(rays):
+            call("hx_trace_sorted", self.ctx.handle, self.bvh.handle, _ptr(rays.rays), rays.size,
+                 int(not rays.closest_hit), RAY_KEY_METHODS[method], int(bits), float(ray_length),
+                 _ptr(rays.results), None, _ptr(rays.indices))
         return rays.results
 
     def trace_sort_noalloc(self, rays, method="aila", bits=60, ray_length=0.25):
@@ -519,9 +554,10 @@ class HipTracer:
         self._need_bvh()
         if rays.indices is None:
             rays.indices = torch.zeros(rays.size, dtype=torch.int32, device=rays.rays.device)
-        call("hx_trace_sorted_noalloc", self.ctx.handle, self.bvh.handle, _ptr(rays.rays), rays.size,
-             int(not rays.closest_hit), RAY_KEY_METHODS[method], int(bits), float(ray_length),
-             _ptr(rays.results), None, _ptr(rays.indices))
+        with self._launch(rays):
+            call("hx_trace_sorted_noalloc", self.ctx.handle, self.bvh.handle, _ptr(rays.rays), rays.size,
+                 int(not rays.closest_hit), RAY_KEY_METHODS[method], int(bits), float(ray_length),
+                 _ptr(rays.results), None, _ptr(rays.indices))
         return rays.results
 
     def trace_sort_timed(self, rays, method="aila", bits=60, ray_length=0.25):
@@ -531,9 +567,10 @@ class HipTracer:
         if rays.indices is None:
             rays.indices = torch.zeros(rays.size, dtype=torch.int32, device=rays.rays.device)
         ts, tt = ctypes.c_float(), ctypes.c_float()
-        call("hx_trace_sorted_timed", self.ctx.handle, self.bvh.handle, _ptr(rays.rays), rays.size,
-             int(not rays.closest_hit), RAY_KEY_METHODS[method], int(bits), float(ray_length),
-             _ptr(rays.results), None, _ptr(rays.indices), ctypes.byref(ts), ctypes.byref(tt))
+        with self._launch(rays):
+            call("hx_trace_sorted_timed", self.ctx.handle, self.bvh.handle, _ptr(rays.rays), rays.size,
+                 int(not rays.closest_hit), RAY_KEY_METHODS[method], int(bits), float(ray_length),
+                 _ptr(rays.results), None, _ptr(rays.indices), ctypes.byref(ts), ctypes.byref(tt))
         return ts.value, tt.value
 
 

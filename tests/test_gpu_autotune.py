@@ -24,16 +24,21 @@ def test_autotune_sort_trace_and_cache(oracle, ctx, tmp_path):
         scene = hx.Scene(v, t)
         bvh = hx.LBVHBuilder(ctx).build(scene)
         rays = hx.RayBuffer(scenes.random_rays(60000, seed=3))
+        before = {k: ctx.get_tuning(k) for k in ("trace_lds_stack", "trace_fetch", "trace_blocks_per_cu")}
         tr = at.tune_trace(bvh, rays)
         assert tr["trace_lds_stack"] in LDS_STACK and tr["trace_fetch"] in FETCH
         assert tr["trace_blocks_per_cu"] in BLOCKS_PER_CU
+        # the choice belongs to this ray set; the context keeps its values
+        assert rays.launch == tr
+        assert {k: ctx.get_tuning(k) for k in before} == before
         tl = at.tune_treelet(scene, rays, sizes=(5, 9))
         assert tl["atr_treelet_size"] in (5, 9)
         stored = json.loads(cache.read_text())
         assert len(stored) == 3
         # A second tuner reads the cache instead of timing again.
         at2 = Autotuner(ctx, cache_path=str(cache))
-        assert at2.tune_trace(bvh, rays) == tr and not at2.history
+        rays2 = hx.RayBuffer(scenes.random_rays(60000, seed=3))
+        assert at2.tune_trace(bvh, rays2) == tr and not at2.history and rays2.launch == tr
         # Tuned context: the build and the trace still match the oracle.
         bvh2 = hx.LBVHBuilder(ctx).build(scene)
         ref = oracle.build_lbvh(v, t)

@@ -96,9 +96,7 @@ def _check_oracle_mesh(oracle, got, ref, v, t, rays_m, res_m, tag):
 
 
 def _trace_into(ctx, bvh, rays, out):
-    rb = hx.RayBuffer.__new__(hx.RayBuffer)
-    rb.rays, rb.results, rb.closest_hit, rb.stats, rb.indices = rays, out, True, None, None
-    hx.HipTracer(ctx, bvh).trace(rb)
+    hx.HipTracer(ctx, bvh).trace(hx.RayBuffer.wrap(rays, out))
     return out
 
 

@@ -194,6 +194,22 @@ def test_trace_tunables_keep_results(oracle, ctx, name, value):
         ctx.set_tuning(name, old)
 
 
+def test_ray_set_launch_applies_per_call(oracle, ctx):
+    """RayBuffer.launch holds for that ray set's traces only; results are the same."""
+    v, t = scenes.gaussian_mixture_soup(20000, seed=32)
+    bvh = hx.LBVHBuilder(ctx).build(hx.Scene(v, t))
+    rays = scenes.random_rays(20000, seed=9)
+    plain = hx.RayBuffer(rays)
+    tuned = hx.RayBuffer(rays)
+    tuned.launch = {"trace_fetch": 8, "trace_lds_stack": 32}
+    before = (ctx.get_tuning("trace_fetch"), ctx.get_tuning("trace_lds_stack"))
+    tr = hx.HipTracer(ctx, bvh)
+    a = tr.trace(plain).cpu().numpy()
+    b = tr.trace_sort(tuned, method="aila", bits=60).cpu().numpy()
+    assert (ctx.get_tuning("trace_fetch"), ctx.get_tuning("trace_lds_stack")) == before
+    assert np.array_equal(a, b)
+
+
 def test_trace_view_on_external_buffers(oracle, ctx):
     """TRACE_FUNC's argument list over Bin BVH arrays the caller holds (copies
     of a build's buffers in fresh tensors): the same hits as hx_trace, and with
