@@ -69,7 +69,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=0, help="triangles in the CPU baseline sample (0 = all)")
     ap.add_argument("--cpu-rays", type=int, default=4194304, help="rays in the CPU trace sample")
-    ap.add_argument("--builders", default="ploc,atr,hlbvh,insertion,presplit", help="other builders to time on the same scene ('' = none)")
+    ap.add_argument("--builders", default="ploc,atr,tr,hlbvh,insertion,presplit",
+                    help="other builders to time on the same scene ('' = none)")
     ap.add_argument("--batch-meshes", type=int, default=4096, help="meshes in the batched workload (0 = skip)")
     ap.add_argument("--batch-lo", type=int, default=64)
     ap.add_argument("--batch-hi", type=int, default=262144)
@@ -204,12 +205,14 @@ def timed(fn, steps, warmup):
 
 
 def bench_builders(ctx, scene, n, args):
-    """PLOC r=25, ATRBVH t=9 x2, HLBVH and insertion rebuilds of the benchmark scene, with SAH costs."""
+    """PLOC r=25, ATRBVH t=9 x2, TR x2, HLBVH, insertion and presplit LBVH rebuilds of the benchmark scene,
+    with SAH costs."""
     import hippie_amd as hx
     out = {}
     kinds = [k for k in args.builders.split(",") if k]
     specs = {"ploc": ("ploc_r25", hx.PLOCBuilder, dict(ploc_radius=25)),
              "atr": ("atr_t9_i2", hx.ATRBuilder, dict(atr_treelet_size=9, atr_iterations=2)),
+             "tr": ("tr_i2", hx.TRBuilder, dict(tr_iterations=2)),
              "hlbvh": ("hlbvh_s15", hx.HLBVHBuilder, dict(hlbvh_sah_bits=15)),
              "insertion": ("insertion_m1", hx.InsertionBuilder, dict(insertion_mod=1)),
              "presplit": ("presplit_b03_lbvh", hx.LBVHBuilder, dict(presplitting=1, presplitter_beta=0.3))}
@@ -220,7 +223,7 @@ def bench_builders(ctx, scene, n, args):
         reps = 1 if k == "insertion" else max(1, args.steps // 2)
         ms = timed(lambda: b.rebuild(bvh, scene), reps, 0 if k == "insertion" else 1)
         out[name] = {"ms": round(ms, 3), "mtris_s": round(n / (ms * 1e-3) / 1e6, 2), "sah": round(bvh.get_cost(), 4)}
-        if k == "atr":
+        if k in ("atr", "tr"):
             out[name]["treelets_kept"] = bvh.info("atr_accepted")
         if k == "ploc":
             out[name]["rounds"] = bvh.info("ploc_rounds")
