@@ -383,6 +383,34 @@ def bench_config5(ctx, args):
     l_rebuild_ms = timed(loop_rebuild, 1, 0)
     l_step_ms = timed(loop_step, 1, 1)
     l_cycle_ms = l_rebuild_ms + 10 * l_step_ms
+
+    # Where a step's time goes: each stage of the batched step on its own, the
+    # loop's traces on their own, and the traversal work per ray (stats trace)
+    # on sliver and on other meshes.
+    b_refit_ms = timed(hb.refit, steps, 1)
+    b_trace_ms = timed(lambda: hb.trace(rays, owner, results=res), steps, 1)
+    b_back_ms = timed(lambda: hb.backward(rays, owner, res, dl_dt=dl_dt, out=grad), steps, 1)
+
+    def loop_trace():
+        for m in range(M):
+            tracer.set_bvh(bvhs[m])
+            tracer.trace(hx.RayBuffer.wrap(rays_m[m], res_m[m]))
+    l_trace_ms = timed(loop_trace, 1, 1)
+    stats = torch.zeros((R, 2), dtype=torch.int32, device=dev)
+    hb.trace(rays, owner, results=torch.zeros_like(res), stats=stats)
+    sl = torch.as_tensor(w["sliver"], device=dev)[owner.long()]
+    st = stats.double()
+
+    def per_ray(mask):
+        k = max(int(mask.sum()), 1)
+        return {"rays": int(mask.sum()), "nodes_per_ray": round(float(st[mask, 0].sum()) / k, 2),
+                "tris_per_ray": round(float(st[mask, 1].sum()) / k, 2)}
+    breakdown = {"refit_ms": round(b_refit_ms, 3), "trace_ms": round(b_trace_ms, 3),
+                 "backward_ms": round(b_back_ms, 3), "loop_trace_ms": round(l_trace_ms, 3),
+                 "sliver_meshes": per_ray(sl), "other_meshes": per_ray(~sl),
+                 # the cycle with free refit, loss and adjoint: a bound on the speedup
+                 "speedup_bound_trace_only": round(l_cycle_ms / (rebuild_ms + 10 * b_trace_ms), 2)}
+    log(f"config 5 breakdown: {breakdown}")
     log(f"config 5: batched rebuild {rebuild_ms:.2f} ms, step {e_ms:.2f} ms (graph {graph_ms}), cycle "
         f"{cycle_ms:.1f} ms; per-mesh loop rebuild {l_rebuild_ms:.1f} ms, step {l_step_ms:.1f} ms, cycle "
         f"{l_cycle_ms:.1f} ms ({l_cycle_ms / cycle_ms:.1f}x)")
@@ -392,7 +420,8 @@ def bench_config5(ctx, args):
             "rays": R, "rebuild_ms": round(rebuild_ms, 3), "step_ms": round(e_ms, 3),
             "step_graph_ms": None if graph_ms is None else round(graph_ms, 3), "cycle_ms": round(cycle_ms, 3),
             "loop_rebuild_ms": round(l_rebuild_ms, 3), "loop_step_ms": round(l_step_ms, 3),
-            "loop_cycle_ms": round(l_cycle_ms, 3), "cycle_speedup": round(l_cycle_ms / cycle_ms, 2)}
+            "loop_cycle_ms": round(l_cycle_ms, 3), "cycle_speedup": round(l_cycle_ms / cycle_ms, 2),
+            "breakdown": breakdown}
 
 
 def bench_batch(ctx, args):

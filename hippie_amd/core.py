@@ -713,14 +713,15 @@ class HipBatch:
         call("hx_batch_refit", self.ctx.handle, self.handle, _ptr(s.vertices), s.n_vertices, _ptr(s.triangles),
              s.n_triangles)
 
-    def trace(self, rays, ray_mesh, any_hit=False, results=None):
-        """rays: RayBuffer or (R,8) tensor on the device; ray_mesh: int32 (R,) on the device."""
+    def trace(self, rays, ray_mesh, any_hit=False, results=None, stats=None):
+        """rays: RayBuffer or (R,8) tensor on the device; ray_mesh: int32 (R,) on the device;
+        stats: optional int32 (R,2) for (nodes visited, triangles tested) per ray."""
         r = rays.rays if isinstance(rays, RayBuffer) else rays
         n = int(r.shape[0])
         if results is None:
             results = torch.zeros((n, 4), dtype=torch.int32, device=r.device)
         call("hx_batch_trace", self.ctx.handle, self.handle, _ptr(r), _ptr(ray_mesh), n, int(bool(any_hit)),
-             _ptr(results), None)
+             _ptr(results), _ptr(stats))
         return results
 
     def backward(self, rays, ray_mesh, results, dl_dt=None, dl_du=None, dl_dv=None, out=None):

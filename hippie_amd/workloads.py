@@ -96,15 +96,17 @@ def batched_presplit_diff(device, meshes=512, rays=1 << 26, lo=1000, hi=200000, 
     (scenes.batch_mesh); `rays` rays spread over the meshes in proportion to
     their triangles (at least one each), from a radius-1.5 sphere around the
     unit box aimed at uniform points inside it, grouped by mesh.  Returns a
-    dict: host `meshes` [(vertices, triangles)], `counts`, rays (R, 8) and
-    `owner` on the device, and ray offsets `roffs`."""
+    dict: host `meshes` [(vertices, triangles)], `counts`, `sliver` (which
+    meshes are slivers), rays (R, 8) and `owner` on the device, and ray
+    offsets `roffs`."""
     rng = np.random.default_rng(seed)
     M = meshes
     counts = np.rint(np.exp(rng.uniform(math.log(lo), math.log(hi), M))).astype(np.int64)
-    out = []
-    for c in counts:
+    out, sliver = [], np.zeros(M, dtype=bool)
+    for m, c in enumerate(counts):
         if rng.random() < sliver_fraction:
             out.append(scenes.sliver_soup(int(c), rng))
+            sliver[m] = True
         else:
             out.append(scenes.batch_mesh(int(c), rng, bool(rng.random() < 0.5)))
     N = int(counts.sum())
@@ -123,4 +125,5 @@ def batched_presplit_diff(device, meshes=512, rays=1 << 26, lo=1000, hi=200000, 
     r[:, 0:3] = o
     r[:, 4:7] = dirn
     r[:, 7] = 4.0
-    return dict(meshes=out, counts=counts, rays=r, owner=owner, roffs=np.concatenate([[0], np.cumsum(per)]))
+    return dict(meshes=out, counts=counts, sliver=sliver, rays=r, owner=owner,
+                roffs=np.concatenate([[0], np.cumsum(per)]))
