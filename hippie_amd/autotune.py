@@ -11,6 +11,7 @@ measurement on the caller's own workload:
   trace_lds_stack      traversal-stack entries kept in LDS   0, 16, 24, 32, 64
   trace_fetch          wave refill threshold (busy lanes)    8 .. 64
   trace_blocks_per_cu  persistent trace workgroups per CU    0 (occupancy), 1, 2, 4
+  trace_postpone       leaves held back per lane             1 .. 4
   atr_treelet_size     ATRBVH treelet (quality vs time)      5 .. 16
 
 The search is coordinate descent from the current values: one knob at a time,
@@ -35,6 +36,7 @@ SORT_LOOKBACK = (4, 8, 16)
 LDS_STACK = (0, 16, 24, 32, 64)
 FETCH = (8, 16, 24, 32, 40, 48, 56, 64)
 BLOCKS_PER_CU = (0, 1, 2, 4)
+POSTPONE = (1, 2, 3, 4)
 TREELETS = (5, 7, 9, 12, 16)
 
 
@@ -134,7 +136,8 @@ class Autotuner:
             rays.launch = dict(self.cache[key])
             return self.cache[key]
         tracer = core.HipTracer(self.ctx, bvh)
-        knobs = [("trace_lds_stack", LDS_STACK), ("trace_fetch", FETCH), ("trace_blocks_per_cu", BLOCKS_PER_CU)]
+        knobs = [("trace_lds_stack", LDS_STACK), ("trace_fetch", FETCH), ("trace_blocks_per_cu", BLOCKS_PER_CU),
+                 ("trace_postpone", POSTPONE)]
         saved = {k: self.ctx.get_tuning(k) for k, _ in knobs}
         launch, rays.launch = rays.launch, {}
 

@@ -69,9 +69,10 @@ struct Tuning {
     int trace_lds_stack = 16;    // traversal-stack entries kept in LDS: 0, 16, 24, 32, 64
     int trace_fetch = 40;        // refill a wave when fewer lanes than this are still busy
     int trace_blocks_per_cu = 0; // persistent trace workgroups per CU (0 = occupancy limit)
+    int trace_postpone = 3;      // leaves a lane holds back while its wave traverses (1 to 4)
     int sort_digit_bits = 8;     // radix digit width: 8 or 10
-    int sort_items = 16;         // keys per thread in a sort pass: 16 or 32
-    int sort_lookback = 4;       // predecessor tiles read per look-back round trip: 4 or 8
+    int sort_items = 16;         // keys per thread in a sort pass: 8, 16 or 32
+    int sort_lookback = 4;       // predecessor tiles read per look-back round trip: 4, 8 or 16
     int sort_local_run = 64;     // longest run of equal high key bits the local pass ranks (0: full LSD)
 };
 

@@ -57,7 +57,12 @@ struct TraversalStack {
     }
 };
 
-template <bool STATS, bool ORDERED, int LDS_DEPTH>
+// POSTPONE: leaves a lane may hold back while its wave still traverses (1, as
+// HipTracerBinKernels.cu:169-177; 2 is the reference's commented variant,
+// :171-173, 255; up to 4 here).  Held leaves wait in leafAddr (newest) ..
+// leafAddr4; a lane whose slots are full stops until its wave has a leaf in
+// every lane.  The closest hit does not depend on the order leaves are tested.
+template <bool STATS, bool ORDERED, int LDS_DEPTH, int POSTPONE>
 __global__ __launch_bounds__(kTraceBlock) void k_trace(TraceArgs a) {
     __shared__ int32_t s_stack[LDS_DEPTH > 0 ? LDS_DEPTH * kTraceBlock : 1];
     TraversalStack<LDS_DEPTH> stack;
@@ -66,7 +71,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(TraceArgs a) {
     float ox = 0.f, oy = 0.f, oz = 0.f, tmin = 0.f, dx = 0.f, dy = 0.f, dz = 0.f;
     float idx = 0.f, idy = 0.f, idz = 0.f, oodx = 0.f, oody = 0.f, oodz = 0.f;
     float hitT = 0.f, hitU = 0.f, hitV = 0.f;
-    int32_t nodeAddr = kSentinel, leafAddr = 0, hitIndex = -1, hitId = -1;
+    int32_t nodeAddr = kSentinel, leafAddr = 0, leafAddr2 = 0, leafAddr3 = 0, leafAddr4 = 0, hitIndex = -1,
+            hitId = -1;
     int64_t rayidx = 0;
     int32_t nNodes = 0, nTris = 0;
     // Batched trace: this lane's mesh (node base, first triangle position).
@@ -112,6 +118,9 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(TraceArgs a) {
             stack.put(0, kSentinel);
             sp = 0;
             leafAddr = 0;
+            leafAddr2 = 0;
+            leafAddr3 = 0;
+            leafAddr4 = 0;
             nodeAddr = 0;
             hitIndex = -1;
             hitId = -1;
@@ -162,13 +171,18 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(TraceArgs a) {
                         if (sp + 1 >= kStackSize) {
                             atomicAdd(a.control + 1, 1);
                             nodeAddr = kSentinel;
-                            leafAddr = 0;
+                            leafAddr = leafAddr2 = leafAddr3 = leafAddr4 = 0;
                             break;
                         }
                         stack.put(++sp, cy);
                     }
                 }
-                if (nodeAddr < 0 && leafAddr >= 0) {
+                const int32_t lastSlot = POSTPONE == 1 ? leafAddr : POSTPONE == 2 ? leafAddr2
+                                         : POSTPONE == 3 ? leafAddr3 : leafAddr4;
+                if (nodeAddr < 0 && lastSlot >= 0) {
+                    if (POSTPONE >= 4) leafAddr4 = leafAddr3;
+                    if (POSTPONE >= 3) leafAddr3 = leafAddr2;
+                    if (POSTPONE >= 2) leafAddr2 = leafAddr;
                     leafAddr = nodeAddr;
                     nodeAddr = stack.get(sp--);
                 }
@@ -204,6 +218,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(TraceArgs a) {
                                     hitV = v;
                                     if (a.any_hit) {
                                         nodeAddr = kSentinel;
+                                        leafAddr2 = leafAddr3 = leafAddr4 = 0;
                                         break;
                                     }
                                 }
@@ -211,8 +226,15 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(TraceArgs a) {
                         }
                     }
                 }
-                leafAddr = nodeAddr;
-                if (nodeAddr < 0) nodeAddr = stack.get(sp--);
+                if (POSTPONE >= 2 && leafAddr2 < 0) {  // the other held leaves next
+                    leafAddr = leafAddr2;
+                    leafAddr2 = POSTPONE >= 3 ? leafAddr3 : 0;
+                    leafAddr3 = POSTPONE >= 4 ? leafAddr4 : 0;
+                    leafAddr4 = 0;
+                } else {
+                    leafAddr = nodeAddr;
+                    if (nodeAddr < 0) nodeAddr = stack.get(sp--);
+                }
             }
 
             if (__popcll(ballot64(true)) < a.fetch_threshold) break;
@@ -527,30 +549,42 @@ __global__ void k_zero_control(int32_t* control) {
     if (threadIdx.x < 2) control[threadIdx.x] = 0;
 }
 
-template <bool STATS, bool ORDERED, int LDS_DEPTH>
+template <bool STATS, bool ORDERED, int LDS_DEPTH, int POSTPONE>
 int launch_trace_t(Context* c, const TraceArgs& a) {
     // Occupancy limit of this instantiation, queried once per process.
     static int occ = 0;
     if (occ == 0) {
         int q = 0;
-        HX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&q, k_trace<STATS, ORDERED, LDS_DEPTH>, kTraceBlock, 0));
+        HX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&q, k_trace<STATS, ORDERED, LDS_DEPTH, POSTPONE>,
+                                                             kTraceBlock, 0));
         occ = std::max(q, 1);
     }
     const int per_cu = c->tune.trace_blocks_per_cu > 0 ? std::min(c->tune.trace_blocks_per_cu, occ) : occ;
     const int64_t want = (a.n_rays + kTraceBlock - 1) / kTraceBlock;
     const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>((int64_t)c->cu_count * per_cu, want));
-    hipLaunchKernelGGL((k_trace<STATS, ORDERED, LDS_DEPTH>), dim3(blocks), dim3(kTraceBlock), 0, c->stream, a);
+    hipLaunchKernelGGL((k_trace<STATS, ORDERED, LDS_DEPTH, POSTPONE>), dim3(blocks), dim3(kTraceBlock), 0, c->stream,
+                       a);
     return check_launch("trace");
 }
 
-template <bool STATS, bool ORDERED>
+template <bool STATS, bool ORDERED, int POSTPONE>
 int launch_trace_depth(Context* c, const TraceArgs& a) {
     switch (c->tune.trace_lds_stack) {
-        case 0: return launch_trace_t<STATS, ORDERED, 0>(c, a);
-        case 16: return launch_trace_t<STATS, ORDERED, 16>(c, a);
-        case 24: return launch_trace_t<STATS, ORDERED, 24>(c, a);
-        case 32: return launch_trace_t<STATS, ORDERED, 32>(c, a);
-        default: return launch_trace_t<STATS, ORDERED, 64>(c, a);
+        case 0: return launch_trace_t<STATS, ORDERED, 0, POSTPONE>(c, a);
+        case 16: return launch_trace_t<STATS, ORDERED, 16, POSTPONE>(c, a);
+        case 24: return launch_trace_t<STATS, ORDERED, 24, POSTPONE>(c, a);
+        case 32: return launch_trace_t<STATS, ORDERED, 32, POSTPONE>(c, a);
+        default: return launch_trace_t<STATS, ORDERED, 64, POSTPONE>(c, a);
+    }
+}
+
+template <bool STATS, bool ORDERED>
+int launch_trace_postpone(Context* c, const TraceArgs& a) {
+    switch (c->tune.trace_postpone) {
+        case 2: return launch_trace_depth<STATS, ORDERED, 2>(c, a);
+        case 3: return launch_trace_depth<STATS, ORDERED, 3>(c, a);
+        case 4: return launch_trace_depth<STATS, ORDERED, 4>(c, a);
+        default: return launch_trace_depth<STATS, ORDERED, 1>(c, a);
     }
 }
 
@@ -607,8 +641,8 @@ int launch_trace(Context* c, const TraceArgs& a) {
     HX_TRY(check_launch("zero_control"));
     LaunchScope ls(c, a.order ? "trace_sorted" : "trace");
     if (a.stats)
-        return a.order ? launch_trace_depth<true, true>(c, a) : launch_trace_depth<true, false>(c, a);
-    return a.order ? launch_trace_depth<false, true>(c, a) : launch_trace_depth<false, false>(c, a);
+        return a.order ? launch_trace_postpone<true, true>(c, a) : launch_trace_postpone<true, false>(c, a);
+    return a.order ? launch_trace_postpone<false, true>(c, a) : launch_trace_postpone<false, false>(c, a);
 }
 
 int launch_ray_keys(Context* c, const float4* rays, int64_t n, int method, int bits, float ray_length,
