@@ -321,6 +321,10 @@ def bench_config5(ctx, args):
     hb = hx.HipBatch(ctx, mb, **params)
     steps = args.batch_steps
     rebuild_ms = timed(hb.rebuild, steps, 1)
+    tuned = None
+    if not args.no_autotune:
+        from hippie_amd.autotune import Autotuner
+        tuned = Autotuner(ctx, repeats=2, log=log).tune_batch_trace(hb, rays, owner, label="config5")
     res = hb.trace(rays, owner)
     t_ref = res[:, 1].view(torch.float32).clone() * 0.999
     grad = torch.zeros_like(mb.scene.vertices)
@@ -367,6 +371,10 @@ def bench_config5(ctx, args):
     dl_m = [dl_dt[int(roffs[m]):int(roffs[m + 1])] for m in range(M)]
     grads_m = [torch.zeros_like(sc.vertices) for sc in scenes_m]
     tracer = hx.HipTracer(ctx)
+    # the loop's traces get the batched trace's launch shape too
+    rbs_m = [hx.RayBuffer.wrap(rays_m[m], res_m[m]) for m in range(M)]
+    for rb in rbs_m:
+        rb.launch = dict(hb.launch)
 
     def loop_rebuild():
         for sc, bv in zip(scenes_m, bvhs):
@@ -377,7 +385,7 @@ def bench_config5(ctx, args):
             bv, sc = bvhs[m], scenes_m[m]
             bv.update(sc)
             tracer.set_bvh(bv)
-            tracer.trace(hx.RayBuffer.wrap(rays_m[m], res_m[m]))
+            tracer.trace(rbs_m[m])
             single_backward(ctx, sc.vertices, sc.triangles, rays_m[m], res_m[m], dl_dt=dl_m[m], out=grads_m[m])
 
     l_rebuild_ms = timed(loop_rebuild, 1, 0)
@@ -394,7 +402,7 @@ def bench_config5(ctx, args):
     def loop_trace():
         for m in range(M):
             tracer.set_bvh(bvhs[m])
-            tracer.trace(hx.RayBuffer.wrap(rays_m[m], res_m[m]))
+            tracer.trace(rbs_m[m])
     l_trace_ms = timed(loop_trace, 1, 1)
     stats = torch.zeros((R, 2), dtype=torch.int32, device=dev)
     hb.trace(rays, owner, results=torch.zeros_like(res), stats=stats)
@@ -421,7 +429,7 @@ def bench_config5(ctx, args):
             "step_graph_ms": None if graph_ms is None else round(graph_ms, 3), "cycle_ms": round(cycle_ms, 3),
             "loop_rebuild_ms": round(l_rebuild_ms, 3), "loop_step_ms": round(l_step_ms, 3),
             "loop_cycle_ms": round(l_cycle_ms, 3), "cycle_speedup": round(l_cycle_ms / cycle_ms, 2),
-            "breakdown": breakdown}
+            "breakdown": breakdown, "trace_launch": tuned}
 
 
 def bench_batch(ctx, args):
@@ -461,6 +469,10 @@ def bench_batch(ctx, args):
 
     # Differentiable step: refit -> trace -> loss (t - t_ref)^2 over hits -> adjoint.
     rays, owner = w["rays"], w["owner"]
+    tuned = None
+    if not args.no_autotune:
+        from hippie_amd.autotune import Autotuner
+        tuned = Autotuner(ctx, repeats=2, log=log).tune_batch_trace(hb, rays, owner, label="config3")
     res = hb.trace(rays, owner)
     t_ref = res[:, 1].view(torch.float32).clone() * 0.999
     grad = torch.zeros_like(mb.scene.vertices)
@@ -502,12 +514,17 @@ def bench_batch(ctx, args):
     gm = [grad[int(w["vbase"][m]):int(w["vbase"][m + 1])] for m in range(M)]
     tracer = hx.HipTracer(ctx)
 
+    # the loop's traces get the batched trace's launch shape too
+    rbs_m = [hx.RayBuffer.wrap(rays_m[m], res_m[m]) for m in range(M)]
+    for rb in rbs_m:
+        rb.launch = dict(hb.launch)
+
     def loop_step():
         for m in range(M):
             bv, sc = bvhs[m], scenes_m[m]
             bv.update(sc)
             tracer.set_bvh(bv)
-            tracer.trace(hx.RayBuffer.wrap(rays_m[m], res_m[m]))
+            tracer.trace(rbs_m[m])
             single_backward(ctx, sc.vertices, sc.triangles, rays_m[m], res_m[m], dl_dt=dl_m[m], out=gm[m])
     ld_ms = timed(loop_step, steps, 1)
     best = graph_ms if graph_ms is not None else e_ms
@@ -521,6 +538,7 @@ def bench_batch(ctx, args):
         "diff_step_ms": round(e_ms, 3),
         "diff_step_graph_ms": None if graph_ms is None else round(graph_ms, 3),
         "loop_diff_step_ms": round(ld_ms, 3), "diff_step_speedup": round(ld_ms / best, 2),
+        "trace_launch": tuned,
     }
     for bv in bvhs:
         del bv
@@ -577,6 +595,9 @@ def main():
         # each ray set keeps its own trace launch shape (RayBuffer.launch)
         tuning["trace_primary"] = at.tune_trace(bvh, prim_rb, sort_method="aila", label="primary")
         tuning["trace_diffuse"] = at.tune_trace(bvh, sec_rb, sort_method="aila", label="diffuse")
+        # ATRBVH treelet size scored by build + trace time of the diffuse rays (reported; the
+        # builders table keeps the configured t = 9)
+        tuning["atr_treelet"] = at.tune_treelet(scene, sec_rb, iterations=2)["atr_treelet_size"]
         log(f"autotuned: {tuning}")
     torch.cuda.synchronize()
     log(f"setup {time.perf_counter() - t_setup:.1f} s")

@@ -156,6 +156,29 @@ class Autotuner:
         self._save()
         return best
 
+    def tune_batch_trace(self, batch, rays, ray_mesh, label=""):
+        """Trace launch shape of a batched trace (HipBatch) over these rays,
+        kept in batch.launch; the context's own values are left as they were."""
+        n = int(rays.shape[0])
+        key = self._key(f"batch-trace{'-' + label if label else ''}", n)
+        if key in self.cache:
+            batch.launch = dict(self.cache[key])
+            return self.cache[key]
+        knobs = [("trace_lds_stack", LDS_STACK), ("trace_fetch", FETCH), ("trace_blocks_per_cu", BLOCKS_PER_CU),
+                 ("trace_postpone", POSTPONE)]
+        saved = {k: self.ctx.get_tuning(k) for k, _ in knobs}
+        launch, batch.launch = batch.launch, {}
+        results = torch.zeros((n, 4), dtype=torch.int32, device=rays.device)
+        try:
+            best = self._descend(knobs, lambda: batch.trace(rays, ray_mesh, results=results))
+        finally:
+            self.apply(saved)
+            batch.launch = launch
+        batch.launch = dict(best)
+        self.cache[key] = best
+        self._save()
+        return best
+
     def tune_treelet(self, scene, rays, sizes=TREELETS, iterations=2):
         """ATRBVH treelet size scored by build time + closest-hit trace time."""
         key = self._key(f"atr-i{iterations}", scene.n_triangles)

@@ -492,6 +492,22 @@ class RayBuffer:
         return int(self.rays.shape[0])
 
 
+@contextlib.contextmanager
+def _launch_scope(ctx, launch):
+    """Trace tunables `launch` on the context for one call, then the old values."""
+    if not launch:
+        yield
+        return
+    old = {k: ctx.get_tuning(k) for k in launch}
+    for k, v in launch.items():
+        ctx.set_tuning(k, int(v))
+    try:
+        yield
+    finally:
+        for k, v in old.items():
+            ctx.set_tuning(k, v)
+
+
 class HipTracer:
     """HipTracer::setBVH / trace / traceStats / traceSort (rt/tracer/HipTracer.h:16-44)."""
 
@@ -506,20 +522,8 @@ class HipTracer:
         if self.bvh is None:
             raise RuntimeError("HipTracer: no BVH set")
 
-    @contextlib.contextmanager
     def _launch(self, rays):
-        """The ray set's trace tunables on the context for one call."""
-        if not rays.launch:
-            yield
-            return
-        old = {k: self.ctx.get_tuning(k) for k in rays.launch}
-        for k, v in rays.launch.items():
-            self.ctx.set_tuning(k, int(v))
-        try:
-            yield
-        finally:
-            for k, v in old.items():
-                self.ctx.set_tuning(k, v)
+        return _launch_scope(self.ctx, rays.launch)
 
     def trace(self, rays):
         self._need_bvh()
@@ -659,7 +663,8 @@ class MeshBatch:
 
 class HipBatch:
     """Batched LBVH (hx_build_batch): one Bin BVH per mesh, built, refit and
-    traced together."""
+    traced together.  ``launch``: trace tunables applied around this batch's
+    traces (Autotuner.tune_batch_trace), as RayBuffer.launch for one BVH."""
 
     def __init__(self, ctx, batch, params=None, **kw):
         self.ctx = ctx or Context.default()
@@ -673,6 +678,7 @@ class HipBatch:
              _ptr(s.triangles), batch.offsets.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), batch.n_meshes,
              ctypes.byref(h))
         self.handle = h
+        self.launch = {}
 
     def mesh_view(self, m):
         v = HxBatchMeshView()
@@ -720,8 +726,9 @@ class HipBatch:
         n = int(r.shape[0])
         if results is None:
             results = torch.zeros((n, 4), dtype=torch.int32, device=r.device)
-        call("hx_batch_trace", self.ctx.handle, self.handle, _ptr(r), _ptr(ray_mesh), n, int(bool(any_hit)),
-             _ptr(results), _ptr(stats))
+        with _launch_scope(self.ctx, self.launch):
+            call("hx_batch_trace", self.ctx.handle, self.handle, _ptr(r), _ptr(ray_mesh), n, int(bool(any_hit)),
+                 _ptr(results), _ptr(stats))
         return results
 
     def backward(self, rays, ray_mesh, results, dl_dt=None, dl_du=None, dl_dv=None, out=None):

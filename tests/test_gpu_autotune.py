@@ -52,3 +52,19 @@ def test_autotune_sort_trace_and_cache(oracle, ctx, tmp_path):
     finally:
         for k, val in saved.items():
             ctx.set_tuning(k, val)
+
+
+def test_autotune_batch_trace_keeps_hits(ctx):
+    """The batched trace's launch shape belongs to the batch; hits do not change."""
+    meshes = [scenes.uniform_soup(n, seed=n) for n in (500, 3000, 12000)]
+    mb = hx.MeshBatch.from_meshes(meshes)
+    hb = hx.HipBatch(ctx, mb, morton_bits=60)
+    import torch
+    rays = torch.from_numpy(np.concatenate([scenes.random_rays(4000, seed=s) for s in range(3)])).cuda()
+    owner = torch.repeat_interleave(torch.arange(3, dtype=torch.int32), 4000).cuda()
+    before = hb.trace(rays, owner).clone()
+    saved = {k: ctx.get_tuning(k) for k in ("trace_lds_stack", "trace_fetch", "trace_blocks_per_cu", "trace_postpone")}
+    best = Autotuner(ctx, repeats=1).tune_batch_trace(hb, rays, owner)
+    assert hb.launch == best
+    assert {k: ctx.get_tuning(k) for k in saved} == saved
+    assert torch.equal(hb.trace(rays, owner), before)
