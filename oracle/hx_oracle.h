@@ -174,6 +174,15 @@ int64_t hxo_build_tr(const float* verts, const int32_t* tris, int64_t n_tri, con
                      int iterations, int adaptive, int max_leaf_size, float ct, float ci, int32_t* nodes,
                      int32_t* tri_out, float* woop, int64_t* n_interior, int64_t* n_leaves, int64_t* accepted);
 
+/* SBVHBuilder (SBVHBuilder.cpp:14-401) + BVHConverter::convertAdaptive: the
+ * spatial-split BVH of n_tri triangles as Bin nodes (interiors first, depth
+ * first), reference slots (triangle ids, duplicates from spatial splits) and
+ * Woop rows.  Returns the node count, -1 when node_cap or ref_cap is too
+ * small, -2 for fewer than two triangles. */
+int64_t hxo_build_sbvh(const float* verts, const int32_t* tris, int64_t n_tri, float alpha, int max_leaf_size, float ct,
+                       float ci, int32_t* nodes_out, int64_t node_cap, int32_t* tri_out, int64_t ref_cap, float* woop,
+                       int64_t* n_interior, int64_t* n_leaves, int64_t* n_refs);
+
 /* HLBVH upper levels over a Karras tree (left/right/parent, boxes 2n-1) of the
  * sorted keys, in place; returns the number of clusters (1: unchanged). */
 int64_t hxo_hlbvh(int64_t n, int bits, int sah_bits, const uint64_t* sorted_keys, int32_t* left, int32_t* right,

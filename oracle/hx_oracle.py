@@ -59,6 +59,7 @@ def lib():
             "hxo_atr": (_i64, [_i64, _i32, _i32, _f32, _f32, _p, _p, _p, _p, _p]),
             "hxo_build_atr": (_i64, [_p, _p, _i64, _p, _i32, _i32, _i32, _i32, _i32, _f32, _f32,
                                      _p, _p, _p, _p, _p, _p, _f32]),
+            "hxo_build_sbvh": (_i64, [_p, _p, _i64, _f32, _i32, _f32, _f32, _p, _i64, _p, _i64, _p, _p, _p, _p]),
             "hxo_tr": (_i64, [_i64, _i32, _f32, _f32, _p, _p, _p, _p, _p]),
             "hxo_build_tr": (_i64, [_p, _p, _i64, _p, _i32, _i32, _i32, _i32, _f32, _f32,
                                     _p, _p, _p, _p, _p, _p]),
@@ -387,6 +388,30 @@ def build_tr(verts, tris, bits=60, iterations=2, scene_box=None, adaptive=True, 
                             ctypes.byref(ni), ctypes.byref(nl), ctypes.byref(acc))
     return dict(nodes=nodes[:nn].copy(), tri_idx=tri, woop=woop, n_interior=ni.value, n_leaves=nl.value,
                 accepted=acc.value)
+
+
+def build_sbvh(verts, tris, alpha=1e-5, max_leaf_size=8, ct=3.0, ci=2.0):
+    """CPU SBVH (the GPU builders' quality reference) as Bin nodes; returns dict
+    with nodes, tri_idx (reference slots), woop and counts."""
+    v, t = _c(verts, np.float32), _c(tris, np.int32)
+    n = t.shape[0]
+    cap = 4 * n + 64
+    while True:
+        nodes = np.zeros((2 * cap, 16), np.int32)
+        tri = np.zeros(cap, np.int32)
+        woop = np.zeros((cap, 12), np.float32)
+        ni, nl, nr = ctypes.c_int64(0), ctypes.c_int64(0), ctypes.c_int64(0)
+        nn = lib().hxo_build_sbvh(_ptr(v), _ptr(t), n, float(alpha), int(max_leaf_size), float(ct), float(ci),
+                                  _ptr(nodes), 2 * cap, _ptr(tri), cap, _ptr(woop), ctypes.byref(ni), ctypes.byref(nl),
+                                  ctypes.byref(nr))
+        if nn != -1:
+            break
+        cap *= 2
+    if nn < 0:
+        raise ValueError("the SBVH needs at least two triangles")
+    r = nr.value
+    return dict(nodes=nodes[:nn].copy(), tri_idx=tri[:r].copy(), woop=woop[:r].copy(), n_interior=ni.value,
+                n_leaves=nl.value, n_refs=r)
 
 
 def hlbvh(sorted_keys, left, right, parent, nmin, nmax, bits=60, sah_bits=15):
