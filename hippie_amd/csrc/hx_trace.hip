@@ -580,6 +580,9 @@ int launch_trace_depth(Context* c, const TraceArgs& a) {
 
 template <bool STATS, bool ORDERED>
 int launch_trace_postpone(Context* c, const TraceArgs& a) {
+    // An any-hit trace reports the first hit in traversal order, which holding
+    // leaves back changes: it keeps the reference's one postponed leaf.
+    if (a.any_hit) return launch_trace_depth<STATS, ORDERED, 1>(c, a);
     switch (c->tune.trace_postpone) {
         case 2: return launch_trace_depth<STATS, ORDERED, 2>(c, a);
         case 3: return launch_trace_depth<STATS, ORDERED, 3>(c, a);

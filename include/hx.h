@@ -153,7 +153,8 @@ int hx_context_profile_reset(hx_context* ctx);
  *   "trace_lds_stack"      traversal-stack entries held in LDS (0,16,24,32,64)
  *   "trace_fetch"          refill a wave when fewer of its 64 lanes are busy
  *   "trace_blocks_per_cu"  persistent trace workgroups per CU (0 = occupancy)
- *   "trace_postpone"       leaves a lane holds back while its wave traverses (1 to 4; Bin layout)
+ *   "trace_postpone"       leaves a lane holds back while its wave traverses (1 to 4; Bin layout,
+ *                          closest hit; any-hit traces keep 1)
  *   "sort_digit_bits"      radix digit width of every sort (8 or 10)
  *   "sort_items"           keys per thread in a sort pass (8, 16 or 32)
  *   "sort_lookback"        tiles read per sort look-back round trip (4, 8, 16)
