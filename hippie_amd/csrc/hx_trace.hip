@@ -34,6 +34,42 @@ HX_DEV float span_end(float a0, float a1, float b0, float b1, float c0, float c1
     return __int_as_float(min(min(__float_as_int(fmax_s(a0, a1)), __float_as_int(fmax_s(b0, b1))), c));
 }
 
+// Bin node reads.  A vector load of a node costs the CU's texture address and
+// data units four 16-byte-per-lane instructions however many lanes share the
+// node, and on coherent (sorted primary) rays those units, not the ALUs, set
+// the pace.  When every active lane of the wave visits the same node -- the
+// common case near the root for coherent rays -- the node is read with scalar
+// loads instead (constant address space: the trace never writes the nodes), so
+// no vector-memory instruction is issued for it.
+HX_DEV int64_t first_lane_i64(int64_t v) {
+    return (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(v >> 32)) << 32) |
+                     (uint32_t)__builtin_amdgcn_readfirstlane((int)v));
+}
+
+HX_DEV void load_bin_node(const int4* nodes, int64_t node, float4& n0xy, float4& n1xy, float4& nz, int2& cn) {
+    const int64_t n0 = first_lane_i64(node);
+    if (__all(node == n0)) {
+        typedef const __attribute__((address_space(4))) float const_f32;
+        typedef const __attribute__((address_space(4))) int32_t const_i32;
+        const_f32* f = (const_f32*)(nodes + 4 * n0);
+        const_i32* w = (const_i32*)(nodes + 4 * n0);
+        n0xy = make_float4(f[0], f[1], f[2], f[3]);
+        n1xy = make_float4(f[4], f[5], f[6], f[7]);
+        nz = make_float4(f[8], f[9], f[10], f[11]);
+        cn = make_int2(w[12], w[13]);
+    } else {
+        const float4* np = reinterpret_cast<const float4*>(nodes + 4 * node);
+        n0xy = np[0];
+        n1xy = np[1];
+        nz = np[2];
+        const int4 links = nodes[4 * node + 3];
+        // The child links share the node's cache line: keep their load with
+        // the box loads instead of after the box test.
+        asm volatile("" ::"v"(links.x), "v"(links.y));
+        cn = make_int2(links.x, links.y);
+    }
+}
+
 // Traversal stack: entries [0, LDS_DEPTH) in LDS, one column per lane (entry e
 // of lane t at e * kTraceBlock + t, so a wave's accesses hit distinct banks);
 // deeper entries in private (scratch) memory, touched only by deep traversals.
@@ -129,14 +165,9 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(TraceArgs a) {
 
         while (nodeAddr != kSentinel) {
             while ((uint32_t)nodeAddr < (uint32_t)kSentinel) {
-                const float4* np = reinterpret_cast<const float4*>(a.nodes + 4 * (nodeBase + nodeAddr));
-                const float4 n0xy = np[0];
-                const float4 n1xy = np[1];
-                const float4 nz = np[2];
-                const int4 cn = a.nodes[4 * (nodeBase + nodeAddr) + 3];
-                // The child links share the node's cache line: keep their load
-                // with the box loads instead of after the box test.
-                asm volatile("" ::"v"(cn.x), "v"(cn.y));
+                float4 n0xy, n1xy, nz;
+                int2 cn;
+                load_bin_node(a.nodes, nodeBase + nodeAddr, n0xy, n1xy, nz, cn);
                 if (STATS) ++nNodes;
                 const float c0lox = fmaf(n0xy.x, idx, -oodx);
                 const float c0hix = fmaf(n0xy.y, idx, -oodx);
@@ -291,6 +322,29 @@ struct WideStackSize {
     static constexpr int kSize = N == 4 ? 96 : 160;
 };
 
+// A wide node's 2N float4 rows, with scalar loads when the whole wave visits
+// the node (as load_bin_node).
+template <int N>
+HX_DEV void load_wide_node(const float4* nodes4, int32_t node, float4 (&xy)[N], float4 (&zl)[N]) {
+    const int32_t n0 = __builtin_amdgcn_readfirstlane(node);
+    if (__all(node == n0)) {
+        typedef const __attribute__((address_space(4))) float const_f32;
+        const_f32* f = (const_f32*)(nodes4 + (int64_t)n0 * 2 * N);
+#pragma unroll
+        for (int c = 0; c < N; ++c) {
+            xy[c] = make_float4(f[4 * c], f[4 * c + 1], f[4 * c + 2], f[4 * c + 3]);
+            zl[c] = make_float4(f[4 * (N + c)], f[4 * (N + c) + 1], f[4 * (N + c) + 2], f[4 * (N + c) + 3]);
+        }
+    } else {
+        const float4* np = nodes4 + (int64_t)node * 2 * N;
+#pragma unroll
+        for (int c = 0; c < N; ++c) {
+            xy[c] = np[c];
+            zl[c] = np[N + c];
+        }
+    }
+}
+
 template <int N, bool STATS, bool ORDERED, int LDS_DEPTH>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_wide(TraceArgs a) {
     constexpr int kSize = WideStackSize<N>::kSize;
@@ -341,14 +395,15 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_wide(TraceArgs a) {
         }
         while (nodeAddr != kSentinel) {
             while ((uint32_t)nodeAddr < (uint32_t)kSentinel) {
-                const float4* np = nodes4 + (int64_t)nodeAddr * 2 * N;
                 if (STATS) ++nNodes;
+                float4 xys[N], zls[N];
+                load_wide_node<N>(nodes4, nodeAddr, xys, zls);
                 float dist[N];
                 int32_t link[N];
 #pragma unroll
                 for (int c = 0; c < N; ++c) {
-                    const float4 xy = np[c];
-                    const float4 zl = np[N + c];
+                    const float4 xy = xys[c];
+                    const float4 zl = zls[c];
                     const float lox = fmaf(xy.x, idx, -oodx);
                     const float hix = fmaf(xy.y, idx, -oodx);
                     const float loy = fmaf(xy.z, idy, -oody);
