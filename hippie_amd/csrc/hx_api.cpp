@@ -269,11 +269,12 @@ size_t hlbvh_work_bytes(int64_t n) {
     return (size_t)n * 8 + K * (4 * 8 + 16 * 4) + 8192;
 }
 
-// Insertion workspace: out, red, locks and the best tree (2n-1 entries
-// each), refs, height and the height list (n each), bucket counts, counters.
+// Insertion workspace: out, red, locks, the best tree and the packed walk
+// records (2n-1 entries each), refs, height and the height list (n each),
+// bucket counts, counters.
 size_t ins_work_bytes(int64_t n) {
     const size_t N = (size_t)(2 * n);
-    return N * (4 + 4 + 8 + 12) + (size_t)n * 12 + 2 * sizeof(int32_t) * (2 * hx::kAtrMaxLevels + 1) + 8192;
+    return N * (4 + 4 + 8 + 12 + 64) + (size_t)n * 12 + 2 * sizeof(int32_t) * (2 * hx::kAtrMaxLevels + 1) + 8192;
 }
 
 int reserve_bvh(hx_bvh* b, int64_t n) {
@@ -703,6 +704,7 @@ int run_insertion(hx_bvh* b) {
     a.acc = reinterpret_cast<unsigned long long*>(carve(32));
     a.refs = reinterpret_cast<int32_t*>(carve(4 * (size_t)n));
     a.height = reinterpret_cast<int32_t*>(carve(4 * (size_t)n));
+    a.rec = reinterpret_cast<int4*>(carve(64 * N));
     HX_TRY(hx::launch_insertion(c, a, &b->ins_steps, &b->ins_reinserted));
     hx::AtrArgs rel{};
     rel.n = n;

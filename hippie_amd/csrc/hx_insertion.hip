@@ -60,6 +60,43 @@ HX_DEV float area_of(const Box3& b) { return box_area(b.mn, b.mx); }
 
 HX_DEV int sibling_of(const InsArgs& a, int v, int p) { return a.left[p] == v ? a.right[p] : a.left[p]; }
 
+// The walk's view of the tree, packed per node into one 64-byte record so
+// that a step issues two vector loads instead of four or five (boxes, parent
+// and child links otherwise live in five arrays, each its own cache line):
+//   [0] min corner, parent    [1] max corner, left child
+//   [2] left child, right child (internal nodes; -1 for leaves)    [3] unused
+// Written before every find from the arrays the other kernels update.
+__global__ __launch_bounds__(kBlock) void k_ins_pack(InsArgs a) {
+    const int N = (int)(2 * a.n - 1);
+    const int nl = (int)a.n - 1;
+    const int v = blockIdx.x * kBlock + threadIdx.x;
+    if (v >= N) return;
+    const float4 mn = a.nmin[v], mx = a.nmax[v];
+    const int l = v < nl ? a.left[v] : -1, r = v < nl ? a.right[v] : -1;
+    float4* rec = reinterpret_cast<float4*>(a.rec) + 4 * (int64_t)v;
+    rec[0] = make_float4(mn.x, mn.y, mn.z, __int_as_float(a.parent[v]));
+    rec[1] = make_float4(mx.x, mx.y, mx.z, __int_as_float(l));
+    reinterpret_cast<int4*>(rec)[2] = make_int4(l, r, 0, 0);
+}
+
+struct WalkNode {
+    Box3 box;
+    int parent, left;
+};
+
+HX_DEV WalkNode walk_node(const InsArgs& a, int v) {
+    const float4* rec = reinterpret_cast<const float4*>(a.rec) + 4 * (int64_t)v;
+    const float4 r0 = rec[0], r1 = rec[1];
+    WalkNode w;
+    w.box.mn = make_float3(r0.x, r0.y, r0.z);
+    w.box.mx = make_float3(r1.x, r1.y, r1.z);
+    w.parent = __float_as_int(r0.w);
+    w.left = __float_as_int(r1.w);
+    return w;
+}
+
+HX_DEV int2 walk_links(const InsArgs& a, int v) { return *reinterpret_cast<const int2*>(a.rec + 4 * (int64_t)v + 2); }
+
 // findBestNode (InsertionBuilderKernels.cu:11-185).  The walk visits the
 // sibling's subtree (the sibling itself is the current position and is not a
 // candidate), then climbs: at each ancestor it visits the other child's
@@ -75,12 +112,14 @@ __global__ __launch_bounds__(kBlock) void k_ins_find(InsArgs a, int mod, int rem
     const int in = mod * k + rem;
     if (in <= 0 || in >= N) return;
     const int nl = (int)a.n - 1;
-    const int p = a.parent[in];
-    const Box3 inBox = node_box(a, in);
-    const float pArea = area_of(node_box(a, p));
+    const WalkNode inNode = walk_node(a, in);
+    const int p = inNode.parent;
+    const Box3 inBox = inNode.box;
+    const float pArea = area_of(walk_node(a, p).box);
     const float bound = pArea - area_of(inBox);
     Box3 highest = box_empty();
-    const int sib = sibling_of(a, in, p);
+    const int2 pl = walk_links(a, p);
+    const int sib = pl.x == in ? pl.y : pl.x;
     int out = sib, highestIdx = p;
     bool down = true;
     int best = -1;
@@ -88,7 +127,8 @@ __global__ __launch_bounds__(kBlock) void k_ins_find(InsArgs a, int mod, int rem
     const int64_t limit = 4 * (int64_t)N + 64;
     int64_t it = 0;
     for (; it < limit; ++it) {
-        Box3 outBox = node_box(a, out);
+        WalkNode on = walk_node(a, out);
+        Box3 outBox = on.box;
         Box3 merged = box_union(inBox, outBox);
         if (down) {
             if (out != sib) {
@@ -100,14 +140,15 @@ __global__ __launch_bounds__(kBlock) void k_ins_find(InsArgs a, int mod, int rem
             }
             red += area_of(outBox) - area_of(merged);
             if (out >= nl || red + bound <= bestRed) down = false;
-            else out = a.left[out];
+            else out = on.left;
         } else {
-            int op = a.parent[out];
+            int op = on.parent;
             red -= area_of(outBox) - area_of(merged);
             if (op == highestIdx) {
                 box_grow(highest, outBox.mn);
                 box_grow(highest, outBox.mx);
                 out = op;
+                const WalkNode up = walk_node(a, out);
                 if (out != p && out != 0) {
                     merged = box_union(inBox, highest);
                     const float direct = pArea - area_of(merged);
@@ -115,19 +156,23 @@ __global__ __launch_bounds__(kBlock) void k_ins_find(InsArgs a, int mod, int rem
                         bestRed = direct + red;
                         best = out;
                     }
-                    outBox = node_box(a, out);
+                    outBox = up.box;
                     red += area_of(outBox) - area_of(highest);
                 }
-                op = a.parent[out];
+                op = up.parent;
                 if (op < 0) break;
                 highestIdx = op;
                 down = true;
-                out = sibling_of(a, out, highestIdx);
-            } else if (a.left[op] == out) {
-                down = true;
-                out = a.right[op];
+                const int2 hl = walk_links(a, highestIdx);
+                out = hl.x == out ? hl.y : hl.x;
             } else {
-                out = op;
+                const int2 ol = walk_links(a, op);
+                if (ol.x == out) {
+                    down = true;
+                    out = ol.y;
+                } else {
+                    out = op;
+                }
             }
         }
     }
@@ -385,6 +430,8 @@ int launch_insertion(Context* c, const InsArgs& a, int64_t* steps_out, int64_t* 
         const unsigned g = grid_for(count);
         hipLaunchKernelGGL(k_ins_zero, dim3(1), dim3(64), 0, c->stream, a.acc, 4);
         HX_TRY(check_launch("ins_zero"));
+        hipLaunchKernelGGL(k_ins_pack, dim3(grid_for(N)), dim3(kBlock), 0, c->stream, a);
+        HX_TRY(check_launch("ins_pack"));
         hipLaunchKernelGGL(k_ins_find, dim3(g), dim3(kBlock), 0, c->stream, a, mod_cur, rem, count);
         HX_TRY(check_launch("ins_find"));
         HX_TRY(launch_fill_i32(c, reinterpret_cast<int32_t*>(a.locks), 0, 2 * (int64_t)N));

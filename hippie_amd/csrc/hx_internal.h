@@ -376,6 +376,7 @@ struct InsArgs {
     unsigned long long* acc;  // [0] cost (2^-32 units), [1] moves found, [2] moves applied
     int32_t* refs;            // n-1 leaf counts (filled by the final refit)
     int32_t* height;          // n-1 heights
+    int4* rec;                // 2n-1 packed walk records, 4 int4 each
 };
 int launch_insertion(Context* c, const InsArgs& a, int64_t* steps_out, int64_t* reinserted_out);
 int launch_ins_refit(Context* c, const InsArgs& a, bool sizes);
