@@ -132,19 +132,25 @@ def test_trace_stats_and_sorted(oracle, ctx):
     res_s, stats, _ = _trace(ctx, bvh, rays, stats=True)
     assert np.array_equal(plain, res_s)
     _, ost, _ = oracle.trace(bvh.nodes_host(), bvh.woop_host(), bvh.tri_indices_host(), rays, want_stats=True)
-    # A lane may traverse further before its postponed leaf is processed than
-    # the one-lane oracle does, so node counts agree only in aggregate.  Until
-    # a ray's first triangle test its hit distance is tmax on both sides, so
-    # both visit the same nodes up to its first leaf: whether a ray tests any
-    # triangle at all, and a ray that misses the root box (one node, no
-    # triangle), agree ray for ray.
+    # A lane may traverse further before its held-back leaves are tested than
+    # the one-lane oracle does (which tests each leaf as soon as it is found),
+    # so counts differ ray for ray, but one way only: both walk the same
+    # near-first order, and when the lane reaches a node it has tested a
+    # subset of the leaves the oracle had by then, so its hit distance is no
+    # smaller and every box the oracle enters it enters too.  Each ray's
+    # node and triangle counts are therefore at least the oracle's.  Until a
+    # ray's first triangle test its hit distance is tmax on both sides, so
+    # whether a ray tests any triangle at all, and a ray that misses the root
+    # box (one node, no triangle), agree ray for ray.
     live = rays[:, 7] > 0
     assert np.all(stats[live, 0] >= 1)
+    assert np.all(stats[:, 0] >= ost[:, 0])
+    assert np.all(stats[:, 1] >= ost[:, 1])
     ratio = stats[:, 0].sum() / max(ost[:, 0].sum(), 1)
     tri_ratio = stats[:, 1].sum() / max(ost[:, 1].sum(), 1)
     print(f"traceStats vs one-lane oracle: nodes x{ratio:.4f}, triangles x{tri_ratio:.4f}")
-    assert 0.95 <= ratio <= 1.5, ratio
-    assert 0.9 <= tri_ratio <= 1.5, tri_ratio
+    assert 1.0 <= ratio <= 1.5, ratio
+    assert 1.0 <= tri_ratio <= 1.5, tri_ratio
     assert np.array_equal(stats[:, 1] > 0, ost[:, 1] > 0)
     assert np.array_equal(stats[:, 0] == 1, ost[:, 0] == 1)
     for method in ("aila", "twopoint"):
