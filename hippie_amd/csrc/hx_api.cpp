@@ -733,9 +733,17 @@ int run_batch(hx_bvh* b) {
     const int64_t n_tri = b->n_tri;
     HX_HIP(hipSetDevice(c->device));
     HX_TRY(b->tri_off.reserve(4 * (size_t)(M + 1)));
-    HX_TRY(b->tri_mesh_of.reserve(4 * (size_t)n_tri));
     int32_t* toff = b->tri_off.as<int32_t>();
-    int32_t* tmesh = b->tri_mesh_of.as<int32_t>();
+    int32_t* tmesh;
+    if (p.presplitting) {
+        HX_TRY(b->tri_mesh_of.reserve(4 * (size_t)n_tri));
+        tmesh = b->tri_mesh_of.as<int32_t>();
+    } else {
+        // Without presplitting the references are the triangles: one mesh map
+        // serves the boxes and the build.
+        HX_TRY(b->mesh_of.reserve(4 * (size_t)n_tri));
+        tmesh = b->mesh_of.as<int32_t>();
+    }
     HX_HIP(hipMemcpyAsync(toff, b->h_off.data(), 4 * (size_t)(M + 1), hipMemcpyHostToDevice, c->stream));
     HX_TRY(hx::launch_mesh_of(c, toff, M, n_tri, tmesh));
     HX_TRY(hx::launch_mesh_boxes(c, b->verts, b->tris, n_tri, tmesh, M, b->mesh_box_bits.as<uint32_t>(),
@@ -751,7 +759,7 @@ int run_batch(hx_bvh* b) {
     int32_t* off = b->mesh_off.as<int32_t>();
     int32_t* mesh_of = b->mesh_of.as<int32_t>();
     HX_HIP(hipMemcpyAsync(off, b->h_roff.data(), 4 * (size_t)(M + 1), hipMemcpyHostToDevice, c->stream));
-    HX_TRY(hx::launch_mesh_of(c, off, M, n, mesh_of));
+    if (p.presplitting) HX_TRY(hx::launch_mesh_of(c, off, M, n, mesh_of));
     const float4* rmin = b->refs_active ? b->ref_min.as<float4>() : nullptr;
     const float4* rmax = b->refs_active ? b->ref_max.as<float4>() : nullptr;
     uint64_t* korig = b->keys_orig.as<uint64_t>();

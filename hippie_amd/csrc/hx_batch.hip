@@ -25,17 +25,13 @@ HX_DEV uint32_t ord(float f) {
 }
 HX_DEV float unord(uint32_t k) { return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k); }
 
-__global__ __launch_bounds__(kBlock) void k_mesh_of(const int32_t* __restrict__ off, int M, int n,
-                                                    int32_t* __restrict__ mesh_of) {
-    const int t = blockIdx.x * kBlock + threadIdx.x;
-    if (t >= n) return;
-    int lo = 0, hi = M - 1;  // last m with off[m] <= t
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (off[mid] <= t) lo = mid;
-        else hi = mid - 1;
-    }
-    mesh_of[t] = lo;
+// mesh_of[t] = m for t in [off[m], off[m+1]): one workgroup per mesh writes
+// its own range (offsets start at 0 and increase strictly, so the ranges tile
+// [0, n)).
+__global__ __launch_bounds__(kBlock) void k_mesh_of(const int32_t* __restrict__ off, int32_t* __restrict__ mesh_of) {
+    const int m = blockIdx.x;
+    const int end = off[m + 1];
+    for (int t = off[m] + (int)threadIdx.x; t < end; t += kBlock) mesh_of[t] = m;
 }
 
 __global__ __launch_bounds__(kBlock) void k_mesh_box_init(uint32_t* __restrict__ bits, int M) {
@@ -164,7 +160,8 @@ __global__ __launch_bounds__(kBlock) void k_tri_local(const int32_t* __restrict_
 
 int launch_mesh_of(Context* c, const int32_t* off, int M, int64_t n, int32_t* mesh_of) {
     LaunchScope ls(c, "batch_mesh_of");
-    hipLaunchKernelGGL(k_mesh_of, dim3(grid_for(n)), dim3(kBlock), 0, c->stream, off, M, (int)n, mesh_of);
+    if (M <= 0 || n <= 0) return HX_OK;
+    hipLaunchKernelGGL(k_mesh_of, dim3((unsigned)M), dim3(kBlock), 0, c->stream, off, mesh_of);
     return check_launch("mesh_of");
 }
 
