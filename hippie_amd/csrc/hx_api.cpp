@@ -780,7 +780,7 @@ int run_batch(hx_bvh* b) {
     if (M > 1) {
         int mbits = 0;
         while ((1 << mbits) < M) ++mbits;
-        HX_TRY(hx::launch_gather_mesh_keys(c, sorted, mesh_of, n, static_cast<uint32_t*>(keys[0])));
+        HX_TRY(hx::launch_gather_mesh_keys(c, sorted, off, M, n, static_cast<uint32_t*>(keys[0])));
         int32_t* v2[2] = {sorted, vals[ri ^ 1]};
         int r2 = 0;
         HX_TRY(hx::launch_sort_pairs(c, keys, v2, n, mbits, false, false, &r2));

@@ -421,7 +421,7 @@ int launch_hlbvh(Context* c, const HlbvhArgs& a, bool keys64, int* clusters_out)
 int launch_mesh_of(Context* c, const int32_t* off, int M, int64_t n, int32_t* mesh_of);
 int launch_mesh_boxes(Context* c, const float* verts, const int32_t* tris, int64_t n, const int32_t* mesh_of, int M,
                       uint32_t* bits, float* boxes);
-int launch_gather_mesh_keys(Context* c, const int32_t* vals, const int32_t* mesh_of, int64_t n, uint32_t* keys);
+int launch_gather_mesh_keys(Context* c, const int32_t* vals, const int32_t* off, int M, int64_t n, uint32_t* keys);
 int launch_gather_keys(Context* c, const int32_t* vals, const uint64_t* src, int64_t n, uint64_t* dst);
 int launch_tri_local(Context* c, const int32_t* sorted_tri, const int32_t* mesh_of, const int32_t* off, int64_t n,
                      int32_t* tri_local);
