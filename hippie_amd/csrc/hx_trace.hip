@@ -222,19 +222,24 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(TraceArgs a) {
 
             while (leafAddr < 0) {
                 const int4 lb = a.nodes[4 * (nodeBase + ~leafAddr) + 3];
-                for (int tri = lb.x; tri < lb.y; ++tri) {
+                // The next triangle's first Woop row is read while this one is
+                // tested (leaves of long thin triangles test hundreds per ray).
+                const float4* wrow = a.woop + 3 * (triBase + lb.x);
+                float4 vnext = lb.x < lb.y ? wrow[0] : make_float4(0.f, 0.f, 0.f, 0.f);
+                for (int tri = lb.x; tri < lb.y; ++tri, wrow += 3) {
                     if (STATS) ++nTris;
-                    const float4 v00 = a.woop[3 * (triBase + tri) + 0];
+                    const float4 v00 = vnext;
+                    if (tri + 1 < lb.y) vnext = wrow[3];
                     const float Oz = fmaf(-oz, v00.z, fmaf(-oy, v00.y, fmaf(-ox, v00.x, v00.w)));
                     const float den = fmaf(dz, v00.z, fmaf(dy, v00.y, dx * v00.x));
                     const float t = Oz * (1.0f / den);
                     if (t > tmin && t <= hitT) {
-                        const float4 v11 = a.woop[3 * (triBase + tri) + 1];
+                        const float4 v11 = wrow[1];
                         const float Ox = fmaf(oz, v11.z, fmaf(oy, v11.y, fmaf(ox, v11.x, v11.w)));
                         const float Dx = fmaf(dz, v11.z, fmaf(dy, v11.y, dx * v11.x));
                         const float u = fmaf(t, Dx, Ox);
                         if (u >= 0.0f) {
-                            const float4 v22 = a.woop[3 * (triBase + tri) + 2];
+                            const float4 v22 = wrow[2];
                             const float Oy = fmaf(oz, v22.z, fmaf(oy, v22.y, fmaf(ox, v22.x, v22.w)));
                             const float Dy = fmaf(dz, v22.z, fmaf(dy, v22.y, dx * v22.x));
                             const float v = fmaf(t, Dy, Oy);
