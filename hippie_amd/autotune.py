@@ -12,6 +12,8 @@ measurement on the caller's own workload:
   trace_fetch          wave refill threshold (busy lanes)    8 .. 64
   trace_blocks_per_cu  persistent trace workgroups per CU    0 (occupancy), 1, 2, 4
   trace_postpone       leaves held back per lane             1 .. 4
+  trace_woop_ahead     Woop rows of the next triangle read   1, 2
+                       during a triangle test
   atr_treelet_size     ATRBVH treelet (quality vs time)      5 .. 16
 
 The search is coordinate descent from the current values: one knob at a time,
@@ -37,6 +39,7 @@ LDS_STACK = (0, 16, 24, 32, 64)
 FETCH = (8, 16, 24, 32, 40, 48, 56, 64)
 BLOCKS_PER_CU = (0, 1, 2, 4)
 POSTPONE = (1, 2, 3, 4)
+WOOP_AHEAD = (1, 2)
 TREELETS = (5, 7, 9, 12, 16)
 
 
@@ -137,7 +140,7 @@ class Autotuner:
             return self.cache[key]
         tracer = core.HipTracer(self.ctx, bvh)
         knobs = [("trace_lds_stack", LDS_STACK), ("trace_fetch", FETCH), ("trace_blocks_per_cu", BLOCKS_PER_CU),
-                 ("trace_postpone", POSTPONE)]
+                 ("trace_postpone", POSTPONE), ("trace_woop_ahead", WOOP_AHEAD)]
         saved = {k: self.ctx.get_tuning(k) for k, _ in knobs}
         launch, rays.launch = rays.launch, {}
 
@@ -165,7 +168,7 @@ class Autotuner:
             batch.launch = dict(self.cache[key])
             return self.cache[key]
         knobs = [("trace_lds_stack", LDS_STACK), ("trace_fetch", FETCH), ("trace_blocks_per_cu", BLOCKS_PER_CU),
-                 ("trace_postpone", POSTPONE)]
+                 ("trace_postpone", POSTPONE), ("trace_woop_ahead", WOOP_AHEAD)]
         saved = {k: self.ctx.get_tuning(k) for k, _ in knobs}
         launch, batch.launch = batch.launch, {}
         results = torch.zeros((n, 4), dtype=torch.int32, device=rays.device)

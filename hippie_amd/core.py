@@ -461,7 +461,7 @@ class RayBuffer:
     """Rays (float4 origin/tmin, float4 direction/tmax), results int4, stats int2.
 
     ``launch``: trace tunables ("trace_fetch", "trace_lds_stack",
-    "trace_blocks_per_cu", "trace_postpone") that HipTracer applies to the context around every
+    "trace_blocks_per_cu", "trace_postpone", "trace_woop_ahead") that HipTracer applies to the context around every
     trace of this ray set, e.g. from Autotuner.tune_trace: coherent primary
     rays and incoherent secondaries favour different refill thresholds.  The
     results do not depend on them."""

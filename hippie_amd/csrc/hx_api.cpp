@@ -1251,6 +1251,9 @@ int hx_context_set_tuning(hx_context* ctx, const char* name, int32_t value) {
     } else if (!std::strcmp(name, "trace_postpone")) {
         HX_CHECK_ARG(value >= 1 && value <= 4, "trace_postpone must be in [1, 4]");
         t.trace_postpone = value;
+    } else if (!std::strcmp(name, "trace_woop_ahead")) {
+        HX_CHECK_ARG(value == 1 || value == 2, "trace_woop_ahead must be 1 or 2");
+        t.trace_woop_ahead = value;
     } else if (!std::strcmp(name, "sort_digit_bits")) {
         HX_CHECK_ARG(value == 8 || value == 10, "sort_digit_bits must be 8 or 10");
         t.sort_digit_bits = value;
@@ -1277,6 +1280,7 @@ int hx_context_get_tuning(hx_context* ctx, const char* name, int32_t* value) {
     else if (!std::strcmp(name, "trace_fetch")) *value = t.trace_fetch;
     else if (!std::strcmp(name, "trace_blocks_per_cu")) *value = t.trace_blocks_per_cu;
     else if (!std::strcmp(name, "trace_postpone")) *value = t.trace_postpone;
+    else if (!std::strcmp(name, "trace_woop_ahead")) *value = t.trace_woop_ahead;
     else if (!std::strcmp(name, "sort_digit_bits")) *value = t.sort_digit_bits;
     else if (!std::strcmp(name, "sort_items")) *value = t.sort_items;
     else if (!std::strcmp(name, "sort_lookback")) *value = t.sort_lookback;

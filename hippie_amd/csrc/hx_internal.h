@@ -70,6 +70,7 @@ struct Tuning {
     int trace_fetch = 40;        // refill a wave when fewer lanes than this are still busy
     int trace_blocks_per_cu = 0; // persistent trace workgroups per CU (0 = occupancy limit)
     int trace_postpone = 3;      // leaves a lane holds back while its wave traverses (1 to 4)
+    int trace_woop_ahead = 1;    // Woop rows of the next triangle read during a test (1 or 2)
     int sort_digit_bits = 8;     // radix digit width: 8 or 10
     int sort_items = 16;         // keys per thread in a sort pass: 8, 16 or 32
     int sort_lookback = 4;       // predecessor tiles read per look-back round trip: 4, 8 or 16

@@ -98,7 +98,7 @@ struct TraversalStack {
 // :171-173, 255; up to 4 here).  Held leaves wait in leafAddr (newest) ..
 // leafAddr4; a lane whose slots are full stops until its wave has a leaf in
 // every lane.  The closest hit does not depend on the order leaves are tested.
-template <bool STATS, bool ORDERED, int LDS_DEPTH, int POSTPONE>
+template <bool STATS, bool ORDERED, int LDS_DEPTH, int POSTPONE, int AHEAD>
 __global__ __launch_bounds__(kTraceBlock) void k_trace(TraceArgs a) {
     __shared__ int32_t s_stack[LDS_DEPTH > 0 ? LDS_DEPTH * kTraceBlock : 1];
     TraversalStack<LDS_DEPTH> stack;
@@ -222,19 +222,26 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(TraceArgs a) {
 
             while (leafAddr < 0) {
                 const int4 lb = a.nodes[4 * (nodeBase + ~leafAddr) + 3];
-                // The next triangle's first Woop row is read while this one is
-                // tested (leaves of long thin triangles test hundreds per ray).
+                // The next triangle's first Woop row (AHEAD = 1), or its first
+                // two (AHEAD = 2), are read while this one is tested: leaves of
+                // long thin triangles test hundreds per ray, and most of those
+                // tests pass the plane test and need the second row.
                 const float4* wrow = a.woop + 3 * (triBase + lb.x);
                 float4 vnext = lb.x < lb.y ? wrow[0] : make_float4(0.f, 0.f, 0.f, 0.f);
+                float4 vnext1 = AHEAD >= 2 && lb.x < lb.y ? wrow[1] : make_float4(0.f, 0.f, 0.f, 0.f);
                 for (int tri = lb.x; tri < lb.y; ++tri, wrow += 3) {
                     if (STATS) ++nTris;
                     const float4 v00 = vnext;
-                    if (tri + 1 < lb.y) vnext = wrow[3];
+                    const float4 v11n = vnext1;
+                    if (tri + 1 < lb.y) {
+                        vnext = wrow[3];
+                        if (AHEAD >= 2) vnext1 = wrow[4];
+                    }
                     const float Oz = fmaf(-oz, v00.z, fmaf(-oy, v00.y, fmaf(-ox, v00.x, v00.w)));
                     const float den = fmaf(dz, v00.z, fmaf(dy, v00.y, dx * v00.x));
                     const float t = Oz * (1.0f / den);
                     if (t > tmin && t <= hitT) {
-                        const float4 v11 = wrow[1];
+                        const float4 v11 = AHEAD >= 2 ? v11n : wrow[1];
                         const float Ox = fmaf(oz, v11.z, fmaf(oy, v11.y, fmaf(ox, v11.x, v11.w)));
                         const float Dx = fmaf(dz, v11.z, fmaf(dy, v11.y, dx * v11.x));
                         const float u = fmaf(t, Dx, Ox);
@@ -609,45 +616,51 @@ __global__ void k_zero_control(int32_t* control) {
     if (threadIdx.x < 2) control[threadIdx.x] = 0;
 }
 
-template <bool STATS, bool ORDERED, int LDS_DEPTH, int POSTPONE>
+template <bool STATS, bool ORDERED, int LDS_DEPTH, int POSTPONE, int AHEAD>
 int launch_trace_t(Context* c, const TraceArgs& a) {
     // Occupancy limit of this instantiation, queried once per process.
     static int occ = 0;
     if (occ == 0) {
         int q = 0;
-        HX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&q, k_trace<STATS, ORDERED, LDS_DEPTH, POSTPONE>,
+        HX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&q, k_trace<STATS, ORDERED, LDS_DEPTH, POSTPONE, AHEAD>,
                                                              kTraceBlock, 0));
         occ = std::max(q, 1);
     }
     const int per_cu = c->tune.trace_blocks_per_cu > 0 ? std::min(c->tune.trace_blocks_per_cu, occ) : occ;
     const int64_t want = (a.n_rays + kTraceBlock - 1) / kTraceBlock;
     const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>((int64_t)c->cu_count * per_cu, want));
-    hipLaunchKernelGGL((k_trace<STATS, ORDERED, LDS_DEPTH, POSTPONE>), dim3(blocks), dim3(kTraceBlock), 0, c->stream,
-                       a);
+    hipLaunchKernelGGL((k_trace<STATS, ORDERED, LDS_DEPTH, POSTPONE, AHEAD>), dim3(blocks), dim3(kTraceBlock), 0,
+                       c->stream, a);
     return check_launch("trace");
 }
 
-template <bool STATS, bool ORDERED, int POSTPONE>
+template <bool STATS, bool ORDERED, int POSTPONE, int AHEAD>
 int launch_trace_depth(Context* c, const TraceArgs& a) {
     switch (c->tune.trace_lds_stack) {
-        case 0: return launch_trace_t<STATS, ORDERED, 0, POSTPONE>(c, a);
-        case 16: return launch_trace_t<STATS, ORDERED, 16, POSTPONE>(c, a);
-        case 24: return launch_trace_t<STATS, ORDERED, 24, POSTPONE>(c, a);
-        case 32: return launch_trace_t<STATS, ORDERED, 32, POSTPONE>(c, a);
-        default: return launch_trace_t<STATS, ORDERED, 64, POSTPONE>(c, a);
+        case 0: return launch_trace_t<STATS, ORDERED, 0, POSTPONE, AHEAD>(c, a);
+        case 16: return launch_trace_t<STATS, ORDERED, 16, POSTPONE, AHEAD>(c, a);
+        case 24: return launch_trace_t<STATS, ORDERED, 24, POSTPONE, AHEAD>(c, a);
+        case 32: return launch_trace_t<STATS, ORDERED, 32, POSTPONE, AHEAD>(c, a);
+        default: return launch_trace_t<STATS, ORDERED, 64, POSTPONE, AHEAD>(c, a);
     }
+}
+
+template <bool STATS, bool ORDERED, int POSTPONE>
+int launch_trace_ahead(Context* c, const TraceArgs& a) {
+    if (c->tune.trace_woop_ahead >= 2) return launch_trace_depth<STATS, ORDERED, POSTPONE, 2>(c, a);
+    return launch_trace_depth<STATS, ORDERED, POSTPONE, 1>(c, a);
 }
 
 template <bool STATS, bool ORDERED>
 int launch_trace_postpone(Context* c, const TraceArgs& a) {
     // An any-hit trace reports the first hit in traversal order, which holding
     // leaves back changes: it keeps the reference's one postponed leaf.
-    if (a.any_hit) return launch_trace_depth<STATS, ORDERED, 1>(c, a);
+    if (a.any_hit) return launch_trace_ahead<STATS, ORDERED, 1>(c, a);
     switch (c->tune.trace_postpone) {
-        case 2: return launch_trace_depth<STATS, ORDERED, 2>(c, a);
-        case 3: return launch_trace_depth<STATS, ORDERED, 3>(c, a);
-        case 4: return launch_trace_depth<STATS, ORDERED, 4>(c, a);
-        default: return launch_trace_depth<STATS, ORDERED, 1>(c, a);
+        case 2: return launch_trace_ahead<STATS, ORDERED, 2>(c, a);
+        case 3: return launch_trace_ahead<STATS, ORDERED, 3>(c, a);
+        case 4: return launch_trace_ahead<STATS, ORDERED, 4>(c, a);
+        default: return launch_trace_ahead<STATS, ORDERED, 1>(c, a);
     }
 }
 

@@ -155,6 +155,8 @@ int hx_context_profile_reset(hx_context* ctx);
  *   "trace_blocks_per_cu"  persistent trace workgroups per CU (0 = occupancy)
  *   "trace_postpone"       leaves a lane holds back while its wave traverses (1 to 4; Bin layout,
  *                          closest hit; any-hit traces keep 1)
+ *   "trace_woop_ahead"     Woop rows of a leaf's next triangle read while the current one is
+ *                          tested (1 or 2; Bin layout)
  *   "sort_digit_bits"      radix digit width of every sort (8 or 10)
  *   "sort_items"           keys per thread in a sort pass (8, 16 or 32)
  *   "sort_lookback"        tiles read per sort look-back round trip (4, 8, 16)

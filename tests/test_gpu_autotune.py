@@ -63,7 +63,8 @@ def test_autotune_batch_trace_keeps_hits(ctx):
     rays = torch.from_numpy(np.concatenate([scenes.random_rays(4000, seed=s) for s in range(3)])).cuda()
     owner = torch.repeat_interleave(torch.arange(3, dtype=torch.int32), 4000).cuda()
     before = hb.trace(rays, owner).clone()
-    saved = {k: ctx.get_tuning(k) for k in ("trace_lds_stack", "trace_fetch", "trace_blocks_per_cu", "trace_postpone")}
+    saved = {k: ctx.get_tuning(k) for k in ("trace_lds_stack", "trace_fetch", "trace_blocks_per_cu", "trace_postpone",
+                                         "trace_woop_ahead")}
     best = Autotuner(ctx, repeats=1).tune_batch_trace(hb, rays, owner)
     assert hb.launch == best
     assert {k: ctx.get_tuning(k) for k in saved} == saved

@@ -188,7 +188,8 @@ def test_trace_single_triangle_and_dead_rays(oracle, ctx):
 
 @pytest.mark.parametrize("name,value", [("trace_lds_stack", 0), ("trace_lds_stack", 32), ("trace_lds_stack", 64),
                                         ("trace_fetch", 1), ("trace_fetch", 65), ("trace_blocks_per_cu", 1),
-                                        ("trace_blocks_per_cu", 3), ("trace_postpone", 2), ("trace_postpone", 4)])
+                                        ("trace_blocks_per_cu", 3), ("trace_postpone", 2), ("trace_postpone", 4),
+                                        ("trace_woop_ahead", 2)])
 def test_trace_tunables_keep_results(oracle, ctx, name, value):
     """Every launch-shape choice the autotuner can make gives the same hits."""
     old = ctx.get_tuning(name)
