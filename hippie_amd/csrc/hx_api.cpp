@@ -174,6 +174,11 @@ struct hx_bvh {
     DevBuf tri_pack;
     bool pack_valid = false;
     std::vector<int32_t> h_roff;
+    // Bin layout, Karras trees: the Bin slot of every binary node, recorded by
+    // the first refit after a build (a build leaves it stale); later refits
+    // write the Bin rows from the climb.
+    DevBuf bin_of;
+    bool bin_of_valid = false;
     bool built = false;
     bool counts_known = false;
     int64_t n_interior = 0, n_leaves = 0;
@@ -284,6 +289,7 @@ int reserve_bvh(hx_bvh* b, int64_t n) {
     HX_TRY(b->nodes.reserve((wide ? 32 * (size_t)wide : 64) * (size_t)std::max<int64_t>(2 * n, 2)));
     HX_TRY(b->woop.reserve(48 * (size_t)n));
     HX_TRY(b->tri_idx.reserve(4 * (size_t)n));
+    if (!wide) HX_TRY(b->bin_of.reserve(sizeof(int32_t) * (size_t)std::max<int64_t>(2 * n, 2)));
     HX_TRY(b->keys[0].reserve(ks * (size_t)n));
     HX_TRY(b->keys[1].reserve(ks * (size_t)n));
     HX_TRY(b->vals_tmp.reserve(4 * (size_t)n));
@@ -809,6 +815,7 @@ int run_batch(hx_bvh* b) {
     HX_TRY(hx::launch_tri_local(c, b->tri_idx.as<int32_t>(), mesh_of, toff, n, b->tri_local.as<int32_t>()));
     b->counts_known = false;
     b->tab_known = false;
+    b->bin_of_valid = false;
     b->built = true;
     return HX_OK;
 }
@@ -975,6 +982,7 @@ int run_build(hx_bvh* b) {
         b->refs_active = false;
     }
     b->counts_known = false;
+    b->bin_of_valid = false;
     b->wide = 0;
     if (p.layout != HX_LAYOUT_BIN) HX_TRY(run_wide(b, p.method == HX_METHOD_LBVH && n > 1));
     b->built = true;
@@ -997,9 +1005,32 @@ int run_refit(hx_bvh* b) {
         return HX_OK;
     }
     HX_TRY(hx::launch_refit_reset(c, b->counters.as<int32_t>(), n - 1, b->counts.as<int32_t>() + 2));
-    HX_TRY(hx::launch_climb(c, climb_args(b, true)));
+    // Karras trees (LBVH builds) keep their Bin slots, links and sizes in a
+    // refit: once the first refit after a build has recorded every binary
+    // node's Bin slot, the climb writes the Bin rows itself and the emit is
+    // skipped.  General trees rerun the emit.
+    hx::ClimbArgs ca = climb_args(b, true);
+    const bool fused = !b->wide && !b->general && b->bin_of_valid;
+    if (fused) {
+        ca.bin_of = b->bin_of.as<int32_t>();
+        ca.bin_nodes = b->nodes.as<int4>();
+    }
+    HX_TRY(hx::launch_climb(c, ca));
     if (b->wide) return hx::launch_wide_emit(c, wide_args(b, b->wide_lbvh_root), b->wide_entries, b->n_interior);
-    return hx::launch_emit_only(c, emit_args(b));
+    if (fused) return HX_OK;
+    hx::EmitArgs ea = emit_args(b);
+    if (!b->general) {
+        const int64_t slots = std::max<int64_t>(2 * n, 2);
+        HX_TRY(hx::launch_fill_i32(c, b->bin_of.as<int32_t>(), -1, slots));
+        ea.bin_of = b->bin_of.as<int32_t>();
+    }
+    HX_TRY(hx::launch_emit_only(c, ea));
+    // Under stream capture the recording emit only runs when the graph is
+    // replayed: the slots stay unrecorded until an uncaptured refit.
+    hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
+    HX_HIP(hipStreamIsCapturing(c->stream, &capture));
+    b->bin_of_valid = !b->general && capture == hipStreamCaptureStatusNone;
+    return HX_OK;
 }
 
 int read_counts(hx_bvh* b) {

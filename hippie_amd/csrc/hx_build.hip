@@ -385,6 +385,26 @@ HX_DEV void bin_rows(float4 amn, float4 amx, float4 bmn, float4 bmx, int4& r0, i
     r2 = make_int4(__float_as_int(amn.z), __float_as_int(amx.z), __float_as_int(bmn.z), __float_as_int(bmx.z));
 }
 
+// Refit of a Karras-tree Bin BVH: rows 0-2 of the Bin node of binary node
+// `node`, if it has one -- an interior's from its children's boxes, a leaf's
+// from its own box twice (the emit's formulas; row 3 does not change).
+HX_DEV void refit_bin_rows(const ClimbArgs& a, int node, float3 mn, float3 mx, float3 lmn, float3 lmx, float3 rmn,
+                           float3 rmx) {
+    const int b = a.bin_of[node];
+    if (b == -1) return;
+    int4 r0, r1, r2;
+    if (b >= 0)
+        bin_rows(make_float4(lmn.x, lmn.y, lmn.z, 0.0f), make_float4(lmx.x, lmx.y, lmx.z, 0.0f),
+                 make_float4(rmn.x, rmn.y, rmn.z, 0.0f), make_float4(rmx.x, rmx.y, rmx.z, 0.0f), r0, r1, r2);
+    else
+        bin_rows(make_float4(mn.x, mn.y, mn.z, 0.0f), make_float4(mx.x, mx.y, mx.z, 0.0f),
+                 make_float4(mn.x, mn.y, mn.z, 0.0f), make_float4(mx.x, mx.y, mx.z, 0.0f), r0, r1, r2);
+    int4* o = a.bin_nodes + 4 * (int64_t)(b >= 0 ? b : -2 - b);
+    o[0] = r0;
+    o[1] = r1;
+    o[2] = r2;
+}
+
 // SAH cost of a one-reference leaf with box [mn, mx] (0 without adaptive
 // collapse).
 HX_DEV float leaf_sah_cost(const ClimbArgs& a, float3 mn, float3 mx) {
@@ -566,6 +586,13 @@ __global__ __launch_bounds__(kClimbThreads) void k_climb_local(ClimbArgs a, int3
     if (GENERAL) return;
     // Karras trees: the nodes this block finished (both children arrived,
     // s_cnt == 2) are the block's index range; write them out coalesced.
+    if (a.bin_nodes)
+        for (int i = threadIdx.x; i < T && base + i <= last_leaf; i += kClimbThreads) {
+            const int sl = T + i;
+            const float3 mn = make_float3(s_box[0][sl], s_box[1][sl], s_box[2][sl]);
+            const float3 mx = make_float3(s_box[3][sl], s_box[4][sl], s_box[5][sl]);
+            refit_bin_rows(a, n - 1 + base + i, mn, mx, mn, mx, mn, mx);
+        }
     for (int i = threadIdx.x; i < T; i += kClimbThreads) {
         const int node = base + i;
         if (node >= n - 1 || s_cnt[i] != 2) continue;
@@ -573,6 +600,16 @@ __global__ __launch_bounds__(kClimbThreads) void k_climb_local(ClimbArgs a, int3
         const float3 mx = make_float3(s_box[3][i], s_box[4][i], s_box[5][i]);
         a.nmin[node] = make_float4(mn.x, mn.y, mn.z, 0.0f);
         a.nmax[node] = make_float4(mx.x, mx.y, mx.z, 0.0f);
+        if (a.bin_nodes) {
+            // Both children finished in this block: their boxes are in LDS.
+            const int l = s_left[i], r = s_right[i];
+            const int ls = l >= n - 1 ? T + (l - (n - 1) - base) : l - base;
+            const int rs = r >= n - 1 ? T + (r - (n - 1) - base) : r - base;
+            refit_bin_rows(a, node, mn, mx, make_float3(s_box[0][ls], s_box[1][ls], s_box[2][ls]),
+                           make_float3(s_box[3][ls], s_box[4][ls], s_box[5][ls]),
+                           make_float3(s_box[0][rs], s_box[1][rs], s_box[2][rs]),
+                           make_float3(s_box[3][rs], s_box[4][rs], s_box[5][rs]));
+        }
         if (ATR) {
             a.atr_area[node] = atr_area(mn, mx);
             a.atr_cost[node] = s_ac[i];
@@ -631,6 +668,7 @@ __global__ __launch_bounds__(kBlock) void k_climb_top(ClimbArgs a, const int32_t
             const Combined c = combine(a, first, last, xyz(lmn), xyz(lmx), lc, xyz(rmn), xyz(rmx), rc);
             st_agent(a.nmin + node, make_float4(c.mn.x, c.mn.y, c.mn.z, 0.0f));
             st_agent(a.nmax + node, make_float4(c.mx.x, c.mx.y, c.mx.z, 0.0f));
+            if (a.bin_nodes) refit_bin_rows(a, node, c.mn, c.mx, xyz(lmn), xyz(lmx), xyz(rmn), xyz(rmx));
             if (ATR) {
                 const int n = (int)a.n;
                 const float ar = atr_area(c.mn, c.mx);
@@ -1245,6 +1283,7 @@ __global__ __launch_bounds__(kBlock) void k_emit(EmitArgs a, FlagCtx f) {
             cidx[k] = ~L;
             write_node(a.nodes, ms.tab.x + L, cmn[k], cmx[k], cmn[k], cmx[k], cf - ms.toff, cl + 1 - ms.toff,
                        ~(cl - cf + 1), ls);
+            if (a.bin_of) a.bin_of[c] = -2 - (ms.tab.x + L);
         }
     }
     const int par = (s == ms.root) ? -1 : ms.interior(a.int_slot[a.parent[i]]);
@@ -1253,7 +1292,10 @@ __global__ __launch_bounds__(kBlock) void k_emit(EmitArgs a, FlagCtx f) {
     o[1] = r1;
     o[2] = r2;
     o[3] = make_int4(cidx[0], cidx[1], last - first + 1, par);
+    if (a.bin_of) a.bin_of[i] = ms.tab.x + ls;
 }
+
+
 
 // Mesh tables from the slot scans: each mesh root's interior slot (batched
 // builds), then per mesh the node base, first interior slot, first leaf

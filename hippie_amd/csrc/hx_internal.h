@@ -171,6 +171,11 @@ struct ClimbArgs {
     const int32_t* ref_tri;
     const float4* tri_pack;   // packed vertices of each triangle (3 float4), or NULL
     int refit_only;           // boxes and Woop only: keep states, costs and cover
+    // Refit of a Karras-tree Bin BVH (else NULL): bin_of[node] is the Bin slot
+    // of binary node `node` (>= 0 interior, -2 - slot leaf, -1 none); the
+    // climb writes rows 0-2 of those Bin nodes as it finishes each node.
+    const int32_t* bin_of;
+    int4* bin_nodes;
     int adaptive;
     int max_leaf_size;
     float ct, ci;
@@ -208,6 +213,7 @@ struct EmitArgs {
     int32_t* node_at;
     const int32_t* root_id;   // single-mesh root's node id (NULL: node 0)
     const float4* proto;      // Bin rows 0-2 per internal node from the climb (NULL: from nmin/nmax)
+    int32_t* bin_of;          // optional out: per binary node its Bin slot (see ClimbArgs::bin_of)
 };
 int launch_collapse_emit(Context* c, const EmitArgs& a);
 // Refit: re-zero the climb counters and frontier count, then (after the

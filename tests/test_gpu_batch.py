@@ -83,6 +83,19 @@ def test_batch_trace_refit_backward(oracle, ctx):
         g = hb.mesh_host(m)
         exp = oracle.refit_final(before[m]["nodes"], g["n_interior"], g["n_leaves"], g["tri_idx"], vm, t)
         assert np.array_equal(g["nodes"], exp), f"mesh {m} refit"
+    # Later refits rewrite only the box rows recorded by the first: a second
+    # and a third refit give the oracle's nodes too.
+    for k, sigma in enumerate((0.002, 0.001)):
+        vk = (v_all + np.random.default_rng(20 + k).normal(0.0, sigma, v_all.shape)).astype(np.float32)
+        hb.refit(vk)
+        base = 0
+        for m, (v, t) in enumerate(meshes):
+            vm = vk[base:base + len(v)]
+            base += len(v)
+            g = hb.mesh_host(m)
+            exp = oracle.refit_final(before[m]["nodes"], g["n_interior"], g["n_leaves"], g["tri_idx"], vm, t)
+            assert np.array_equal(g["nodes"], exp), f"mesh {m} refit {k + 2}"
+    hb.refit(v2)
     # Adjoint: batched scatter equals the single-mesh adjoint mesh by mesh.
     res_t = hb.trace(d_rays, d_owner)
     dl_dt = torch.rand(len(rays), device="cuda", dtype=torch.float32)

@@ -40,6 +40,33 @@ def test_refit_same_vertices_reproduces_build(oracle, ctx):
     assert np.array_equal(a, bvh.nodes_host())
 
 
+@pytest.mark.parametrize("method", ["lbvh", "ploc"])
+def test_repeated_refits_and_rebuild(oracle, ctx, method):
+    """The first refit after a build runs the full emit and records where each
+    node's box rows come from; later refits rewrite only those rows.  Every
+    refit must equal a first refit of a fresh build, and a rebuild (new
+    topology) must drop the record."""
+    cls = hx.LBVHBuilder if method == "lbvh" else hx.PLOCBuilder
+    v, t = scenes.gaussian_mixture_soup(30000, seed=64)
+    rng = np.random.default_rng(65)
+    vs = [(v + rng.normal(0, 0.004 * (k + 1), v.shape)).astype(np.float32) for k in range(3)]
+    builder = cls(ctx, morton_bits=60)
+    bvh = builder.build(hx.Scene(v, t))
+    nodes0, tri = bvh.nodes_host(), bvh.tri_indices_host()
+    for vk in vs + [vs[0]]:
+        bvh.update(hx.Scene(vk, t))
+        fresh = cls(ctx, morton_bits=60).build(hx.Scene(v, t))
+        fresh.update(hx.Scene(vk, t))
+        assert np.array_equal(bvh.nodes_host(), fresh.nodes_host())
+        ref = oracle.refit_final(nodes0, bvh.n_interior, bvh.n_leaves, tri, vk, t)
+        assert np.array_equal(bvh.nodes_host().view(np.float32)[:, :12], ref.view(np.float32)[:, :12])
+    builder.rebuild(bvh, hx.Scene(vs[2], t))
+    bvh.update(hx.Scene(vs[1], t))
+    fresh = cls(ctx, morton_bits=60).build(hx.Scene(vs[2], t))
+    fresh.update(hx.Scene(vs[1], t))
+    assert np.array_equal(bvh.nodes_host(), fresh.nodes_host())
+
+
 def _plane_t64(v, t, rays, ids):
     tri = t[ids]
     v0, v1, v2 = (v[tri[:, k]].astype(np.float64) for k in range(3))
