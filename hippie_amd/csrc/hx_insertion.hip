@@ -64,8 +64,11 @@ HX_DEV int sibling_of(const InsArgs& a, int v, int p) { return a.left[p] == v ? 
 // that a step issues two vector loads instead of four or five (boxes, parent
 // and child links otherwise live in five arrays, each its own cache line):
 //   [0] min corner, parent    [1] max corner, left child
-//   [2] left child, right child (internal nodes; -1 for leaves)    [3] unused
-// Written before every find from the arrays the other kernels update.
+//   [2] left child, right child (internal nodes; -1 for leaves), sibling,
+//       1 if the node is its parent's left child (the root: -1, 0)    [3] unused
+// With the sibling in the node's own record a step up the tree reads the
+// line it already holds instead of the parent's.  Written before every find
+// from the arrays the other kernels update.
 __global__ __launch_bounds__(kBlock) void k_ins_pack(InsArgs a) {
     const int N = (int)(2 * a.n - 1);
     const int nl = (int)a.n - 1;
@@ -73,10 +76,17 @@ __global__ __launch_bounds__(kBlock) void k_ins_pack(InsArgs a) {
     if (v >= N) return;
     const float4 mn = a.nmin[v], mx = a.nmax[v];
     const int l = v < nl ? a.left[v] : -1, r = v < nl ? a.right[v] : -1;
+    const int p = a.parent[v];
+    int sib = -1, is_left = 0;
+    if (p >= 0) {
+        const int pl = a.left[p];
+        is_left = pl == v ? 1 : 0;
+        sib = is_left ? a.right[p] : pl;
+    }
     float4* rec = reinterpret_cast<float4*>(a.rec) + 4 * (int64_t)v;
-    rec[0] = make_float4(mn.x, mn.y, mn.z, __int_as_float(a.parent[v]));
+    rec[0] = make_float4(mn.x, mn.y, mn.z, __int_as_float(p));
     rec[1] = make_float4(mx.x, mx.y, mx.z, __int_as_float(l));
-    reinterpret_cast<int4*>(rec)[2] = make_int4(l, r, 0, 0);
+    reinterpret_cast<int4*>(rec)[2] = make_int4(l, r, sib, is_left);
 }
 
 struct WalkNode {
@@ -95,7 +105,10 @@ HX_DEV WalkNode walk_node(const InsArgs& a, int v) {
     return w;
 }
 
-HX_DEV int2 walk_links(const InsArgs& a, int v) { return *reinterpret_cast<const int2*>(a.rec + 4 * (int64_t)v + 2); }
+// (sibling, 1 if v is its parent's left child)
+HX_DEV int2 walk_side(const InsArgs& a, int v) {
+    return *reinterpret_cast<const int2*>(reinterpret_cast<const int32_t*>(a.rec + 4 * (int64_t)v + 2) + 2);
+}
 
 // findBestNode (InsertionBuilderKernels.cu:11-185).  The walk visits the
 // sibling's subtree (the sibling itself is the current position and is not a
@@ -118,8 +131,7 @@ __global__ __launch_bounds__(kBlock) void k_ins_find(InsArgs a, int mod, int rem
     const float pArea = area_of(walk_node(a, p).box);
     const float bound = pArea - area_of(inBox);
     Box3 highest = box_empty();
-    const int2 pl = walk_links(a, p);
-    const int sib = pl.x == in ? pl.y : pl.x;
+    const int sib = walk_side(a, in).x;
     int out = sib, highestIdx = p;
     bool down = true;
     int best = -1;
@@ -163,13 +175,12 @@ __global__ __launch_bounds__(kBlock) void k_ins_find(InsArgs a, int mod, int rem
                 if (op < 0) break;
                 highestIdx = op;
                 down = true;
-                const int2 hl = walk_links(a, highestIdx);
-                out = hl.x == out ? hl.y : hl.x;
+                out = walk_side(a, out).x;  // out's parent is highestIdx
             } else {
-                const int2 ol = walk_links(a, op);
-                if (ol.x == out) {
+                const int2 side = walk_side(a, out);  // out's parent is op
+                if (side.y) {
                     down = true;
-                    out = ol.y;
+                    out = side.x;
                 } else {
                     out = op;
                 }
