@@ -34,40 +34,16 @@ HX_DEV float span_end(float a0, float a1, float b0, float b1, float c0, float c1
     return __int_as_float(min(min(__float_as_int(fmax_s(a0, a1)), __float_as_int(fmax_s(b0, b1))), c));
 }
 
-// Bin node reads.  A vector load of a node costs the CU's texture address and
-// data units four 16-byte-per-lane instructions however many lanes share the
-// node, and on coherent (sorted primary) rays those units, not the ALUs, set
-// the pace.  When every active lane of the wave visits the same node -- the
-// common case near the root for coherent rays -- the node is read with scalar
-// loads instead (constant address space: the trace never writes the nodes), so
-// no vector-memory instruction is issued for it.
+// Bin node reads (k_trace).  A vector load of a node costs the CU's texture
+// address and data units four 16-byte-per-lane instructions however many lanes
+// share the node, and on coherent (sorted primary) rays those units, not the
+// ALUs, set the pace.  When every active lane of the wave visits the same node
+// -- the common case near the root for coherent rays -- the node is read with
+// scalar loads instead (constant address space: the trace never writes the
+// nodes), so no vector-memory instruction is issued for it.
 HX_DEV int64_t first_lane_i64(int64_t v) {
     return (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(v >> 32)) << 32) |
                      (uint32_t)__builtin_amdgcn_readfirstlane((int)v));
-}
-
-HX_DEV void load_bin_node(const int4* nodes, int64_t node, float4& n0xy, float4& n1xy, float4& nz, int2& cn) {
-    const int64_t n0 = first_lane_i64(node);
-    if (__all(node == n0)) {
-        typedef const __attribute__((address_space(4))) float const_f32;
-        typedef const __attribute__((address_space(4))) int32_t const_i32;
-        const_f32* f = (const_f32*)(nodes + 4 * n0);
-        const_i32* w = (const_i32*)(nodes + 4 * n0);
-        n0xy = make_float4(f[0], f[1], f[2], f[3]);
-        n1xy = make_float4(f[4], f[5], f[6], f[7]);
-        nz = make_float4(f[8], f[9], f[10], f[11]);
-        cn = make_int2(w[12], w[13]);
-    } else {
-        const float4* np = reinterpret_cast<const float4*>(nodes + 4 * node);
-        n0xy = np[0];
-        n1xy = np[1];
-        nz = np[2];
-        const int4 links = nodes[4 * node + 3];
-        // The child links share the node's cache line: keep their load with
-        // the box loads instead of after the box test.
-        asm volatile("" ::"v"(links.x), "v"(links.y));
-        cn = make_int2(links.x, links.y);
-    }
 }
 
 // Traversal stack: entries [0, LDS_DEPTH) in LDS, one column per lane (entry e
@@ -165,26 +141,54 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(TraceArgs a) {
 
         while (nodeAddr != kSentinel) {
             while ((uint32_t)nodeAddr < (uint32_t)kSentinel) {
-                float4 n0xy, n1xy, nz;
-                int2 cn;
-                load_bin_node(a.nodes, nodeBase + nodeAddr, n0xy, n1xy, nz, cn);
                 if (STATS) ++nNodes;
-                const float c0lox = fmaf(n0xy.x, idx, -oodx);
-                const float c0hix = fmaf(n0xy.y, idx, -oodx);
-                const float c0loy = fmaf(n0xy.z, idy, -oody);
-                const float c0hiy = fmaf(n0xy.w, idy, -oody);
-                const float c0loz = fmaf(nz.x, idz, -oodz);
-                const float c0hiz = fmaf(nz.y, idz, -oodz);
-                const float c1loz = fmaf(nz.z, idz, -oodz);
-                const float c1hiz = fmaf(nz.w, idz, -oodz);
-                const float c0min = span_begin(c0lox, c0hix, c0loy, c0hiy, c0loz, c0hiz, tmin);
-                const float c0max = span_end(c0lox, c0hix, c0loy, c0hiy, c0loz, c0hiz, hitT);
-                const float c1lox = fmaf(n1xy.x, idx, -oodx);
-                const float c1hix = fmaf(n1xy.y, idx, -oodx);
-                const float c1loy = fmaf(n1xy.z, idy, -oody);
-                const float c1hiy = fmaf(n1xy.w, idy, -oody);
-                const float c1min = span_begin(c1lox, c1hix, c1loy, c1hiy, c1loz, c1hiz, tmin);
-                const float c1max = span_end(c1lox, c1hix, c1loy, c1hiy, c1loz, c1hiz, hitT);
+                // The box test is written out in both branches so that, for a
+                // whole-wave node, it reads the scalar registers directly.
+                auto box_test = [&](float4 n0xy, float4 n1xy, float4 nz, float& c0min, float& c0max, float& c1min,
+                                    float& c1max) {
+                    const float c0lox = fmaf(n0xy.x, idx, -oodx);
+                    const float c0hix = fmaf(n0xy.y, idx, -oodx);
+                    const float c0loy = fmaf(n0xy.z, idy, -oody);
+                    const float c0hiy = fmaf(n0xy.w, idy, -oody);
+                    const float c0loz = fmaf(nz.x, idz, -oodz);
+                    const float c0hiz = fmaf(nz.y, idz, -oodz);
+                    const float c1loz = fmaf(nz.z, idz, -oodz);
+                    const float c1hiz = fmaf(nz.w, idz, -oodz);
+                    c0min = span_begin(c0lox, c0hix, c0loy, c0hiy, c0loz, c0hiz, tmin);
+                    c0max = span_end(c0lox, c0hix, c0loy, c0hiy, c0loz, c0hiz, hitT);
+                    const float c1lox = fmaf(n1xy.x, idx, -oodx);
+                    const float c1hix = fmaf(n1xy.y, idx, -oodx);
+                    const float c1loy = fmaf(n1xy.z, idy, -oody);
+                    const float c1hiy = fmaf(n1xy.w, idy, -oody);
+                    c1min = span_begin(c1lox, c1hix, c1loy, c1hiy, c1loz, c1hiz, tmin);
+                    c1max = span_end(c1lox, c1hix, c1loy, c1hiy, c1loz, c1hiz, hitT);
+                };
+                float c0min, c0max, c1min, c1max;
+                int2 cn;
+                {
+                    const int64_t na = nodeBase + nodeAddr;
+                    const int64_t n0 = first_lane_i64(na);
+                    if (__all(na == n0)) {
+                        typedef const __attribute__((address_space(4))) float const_f32;
+                        typedef const __attribute__((address_space(4))) int32_t const_i32;
+                        const_f32* f = (const_f32*)(a.nodes + 4 * n0);
+                        const_i32* w = (const_i32*)(a.nodes + 4 * n0);
+                        box_test(make_float4(f[0], f[1], f[2], f[3]), make_float4(f[4], f[5], f[6], f[7]),
+                                 make_float4(f[8], f[9], f[10], f[11]), c0min, c0max, c1min, c1max);
+                        cn = make_int2(w[12], w[13]);
+                    } else {
+                        const float4* np = reinterpret_cast<const float4*>(a.nodes + 4 * na);
+                        const float4 n0xy = np[0];
+                        const float4 n1xy = np[1];
+                        const float4 nz = np[2];
+                        const int4 links = a.nodes[4 * na + 3];
+                        // The child links share the node's cache line: keep their
+                        // load with the box loads instead of after the box test.
+                        asm volatile("" ::"v"(links.x), "v"(links.y));
+                        cn = make_int2(links.x, links.y);
+                        box_test(n0xy, n1xy, nz, c0min, c0max, c1min, c1max);
+                    }
+                }
                 const bool swp = c1min < c0min;
                 const bool t0 = c0max >= c0min;
                 const bool t1 = c1max >= c1min;
@@ -335,7 +339,7 @@ struct WideStackSize {
 };
 
 // A wide node's 2N float4 rows, with scalar loads when the whole wave visits
-// the node (as load_bin_node).
+// the node (as the Bin kernel does).
 template <int N>
 HX_DEV void load_wide_node(const float4* nodes4, int32_t node, float4 (&xy)[N], float4 (&zl)[N]) {
     const int32_t n0 = __builtin_amdgcn_readfirstlane(node);
