@@ -338,29 +338,6 @@ struct WideStackSize {
     static constexpr int kSize = N == 4 ? 96 : 160;
 };
 
-// A wide node's 2N float4 rows, with scalar loads when the whole wave visits
-// the node (as the Bin kernel does).
-template <int N>
-HX_DEV void load_wide_node(const float4* nodes4, int32_t node, float4 (&xy)[N], float4 (&zl)[N]) {
-    const int32_t n0 = __builtin_amdgcn_readfirstlane(node);
-    if (__all(node == n0)) {
-        typedef const __attribute__((address_space(4))) float const_f32;
-        const_f32* f = (const_f32*)(nodes4 + (int64_t)n0 * 2 * N);
-#pragma unroll
-        for (int c = 0; c < N; ++c) {
-            xy[c] = make_float4(f[4 * c], f[4 * c + 1], f[4 * c + 2], f[4 * c + 3]);
-            zl[c] = make_float4(f[4 * (N + c)], f[4 * (N + c) + 1], f[4 * (N + c) + 2], f[4 * (N + c) + 3]);
-        }
-    } else {
-        const float4* np = nodes4 + (int64_t)node * 2 * N;
-#pragma unroll
-        for (int c = 0; c < N; ++c) {
-            xy[c] = np[c];
-            zl[c] = np[N + c];
-        }
-    }
-}
-
 template <int N, bool STATS, bool ORDERED, int LDS_DEPTH>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_wide(TraceArgs a) {
     constexpr int kSize = WideStackSize<N>::kSize;
@@ -412,14 +389,11 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_wide(TraceArgs a) {
         while (nodeAddr != kSentinel) {
             while ((uint32_t)nodeAddr < (uint32_t)kSentinel) {
                 if (STATS) ++nNodes;
-                float4 xys[N], zls[N];
-                load_wide_node<N>(nodes4, nodeAddr, xys, zls);
                 float dist[N];
                 int32_t link[N];
-#pragma unroll
-                for (int c = 0; c < N; ++c) {
-                    const float4 xy = xys[c];
-                    const float4 zl = zls[c];
+                // Child c's test; written out in both node-read branches so that
+                // a whole-wave node is tested straight from scalar registers.
+                auto child_test = [&](int c, float4 xy, float4 zl) {
                     const float lox = fmaf(xy.x, idx, -oodx);
                     const float hix = fmaf(xy.y, idx, -oodx);
                     const float loy = fmaf(xy.z, idy, -oody);
@@ -431,6 +405,26 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_wide(TraceArgs a) {
                     const bool hit = cmax >= cmin;
                     dist[c] = hit ? cmin : __int_as_float(0x7f800000);
                     link[c] = hit ? __float_as_int(zl.z) : kSentinel;
+                };
+                const int32_t n0 = __builtin_amdgcn_readfirstlane(nodeAddr);
+                if (__all(nodeAddr == n0)) {
+                    typedef const __attribute__((address_space(4))) float const_f32;
+                    const_f32* f = (const_f32*)(nodes4 + (int64_t)n0 * 2 * N);
+#pragma unroll
+                    for (int c = 0; c < N; ++c)
+                        child_test(c, make_float4(f[4 * c], f[4 * c + 1], f[4 * c + 2], f[4 * c + 3]),
+                                   make_float4(f[4 * (N + c)], f[4 * (N + c) + 1], f[4 * (N + c) + 2],
+                                               f[4 * (N + c) + 3]));
+                } else {
+                    const float4* np = nodes4 + (int64_t)nodeAddr * 2 * N;
+                    float4 xys[N], zls[N];
+#pragma unroll
+                    for (int c = 0; c < N; ++c) {
+                        xys[c] = np[c];
+                        zls[c] = np[N + c];
+                    }
+#pragma unroll
+                    for (int c = 0; c < N; ++c) child_test(c, xys[c], zls[c]);
                 }
 #pragma unroll
                 for (int q = 0; q < WideNet<N>::kPairs; ++q) {
