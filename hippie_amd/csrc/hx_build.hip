@@ -1136,11 +1136,15 @@ __global__ __launch_bounds__(kBlock) void k_tile_cover(int32_t* __restrict__ cov
 // Interior slots and leaf numbers in one pass: int_slot[node] is the number
 // of final interiors at earlier positions, complemented (~) when the node is
 // not a final interior; leaf_num[p] the number of leaves starting before p.
-// The last tile writes the totals to counts[0..1].
+// The last tile writes the totals to counts[0..1].  Both flags are bits, so
+// each wave counts an item's flags with two ballots; the kScanItems x
+// kWavesPerBlock wave counts (one per wave of the block, in position order)
+// are scanned by the first wave, which also runs the look-back.
+static_assert(kScanItems * kWavesPerBlock == kWave, "one wave count per lane of the first wave");
 __global__ __launch_bounds__(kBlock) void k_slots_lb(FlagCtx f, int32_t* __restrict__ int_slot,
                                                      int32_t* __restrict__ leaf_num, int32_t* __restrict__ counts,
                                                      uint64_t* __restrict__ status, uint32_t* __restrict__ counter) {
-    __shared__ int2 wt[kWavesPerBlock];
+    __shared__ int2 s_pre[kScanItems * kWavesPerBlock];  // wave counts, then their exclusive prefixes
     __shared__ int s_tile;
     __shared__ int2 s_excl;
     if (threadIdx.x == 0) s_tile = (int)atomicAdd(counter, 1u);
@@ -1148,28 +1152,63 @@ __global__ __launch_bounds__(kBlock) void k_slots_lb(FlagCtx f, int32_t* __restr
     const int tile = s_tile;
     const int64_t base = tile * (int64_t)kScanTile;
     const int ntiles = (f.n + kScanTile - 1) / kScanTile;
+    const int w = threadIdx.x / kWave;
+    const int lane = lane_id();
     uint32_t bits = 0;  // bit 2k: interior flag, bit 2k+1: leaf start of item k
-    int ci = 0, cl = 0;
+    if (!f.mesh_of && !f.node_at) {
+        // One tree, node i at split position i: interior_flag and leaf_start
+        // with every item's loads issued before any is used (two dependent
+        // round trips per thread instead of three per item).
+        int fi[kScanItems], la[kScanItems], st[kScanItems], cp[kScanItems], cf[kScanItems];
 #pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        const int64_t p = base + k * kBlock + threadIdx.x;
-        if (p < f.n) {
-            const int fi = p < f.n - 1 ? interior_flag(f, node_of_pos(f, (int)p)) : 0;
-            const int ls = leaf_start(f, (int)p);
-            bits |= (uint32_t)fi << (2 * k) | (uint32_t)ls << (2 * k + 1);
-            ci += fi;
-            cl += ls;
+        for (int k = 0; k < kScanItems; ++k) {
+            const int64_t p = base + k * kBlock + threadIdx.x;
+            const bool in = p < f.n - 1;
+            fi[k] = in ? f.first[p] : 0;
+            la[k] = in ? f.last[p] : 0;
+            st[k] = in ? f.state[p] : 0;
+            cp[k] = p > 0 && p < f.n ? f.cover[p - 1] : 0;
+        }
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) cf[k] = st[k] == 1 ? f.cover[fi[k]] : 0;
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            const int64_t p = base + k * kBlock + threadIdx.x;
+            if (p < f.n) {
+                const bool root = fi[k] == 0 && la[k] == f.n - 1;
+                const int fl = p < f.n - 1 && (root || (st[k] == 1 && cf[k] < la[k]));
+                const int ls = p == 0 || cp[k] < p;
+                bits |= (uint32_t)fl << (2 * k) | (uint32_t)ls << (2 * k + 1);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            const int64_t p = base + k * kBlock + threadIdx.x;
+            if (p < f.n) {
+                const int fi = p < f.n - 1 ? interior_flag(f, node_of_pos(f, (int)p)) : 0;
+                const int ls = leaf_start(f, (int)p);
+                bits |= (uint32_t)fi << (2 * k) | (uint32_t)ls << (2 * k + 1);
+            }
         }
     }
-    int2 agg;
-    block_inclusive_scan<kWavesPerBlock>(make_int2(ci, cl), Sum2Op(), wt, agg);
-    if (threadIdx.x < kWave) {
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const uint64_t mi = ballot64((bits >> (2 * k)) & 1u), ml = ballot64((bits >> (2 * k + 1)) & 1u);
+        if (lane == 0) s_pre[k * kWavesPerBlock + w] = make_int2(__popcll(mi), __popcll(ml));
+    }
+    __syncthreads();
+    if (w == 0) {
+        const int2 c = s_pre[lane];
+        const int2 inc = wave_inclusive_scan(c, Sum2Op());
+        s_pre[lane] = make_int2(inc.x - c.x, inc.y - c.y);
+        const int2 agg = make_int2(__builtin_amdgcn_readlane(inc.x, kWave - 1), __builtin_amdgcn_readlane(inc.y, kWave - 1));
         const uint64_t aw = ((uint64_t)agg.x << 31) | (uint64_t)agg.y;
-        if (threadIdx.x == 0)
+        if (lane == 0)
             __hip_atomic_store(status + tile, (tile == 0 ? kLb2Prefix : kLb2Aggregate) | aw, __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
         const int2 excl = tile == 0 ? make_int2(0, 0) : lookback_wave<SlotsLb>(status, tile);
-        if (threadIdx.x == 0) {
+        if (lane == 0) {
             const uint64_t iw = ((uint64_t)(excl.x + agg.x) << 31) | (uint64_t)(excl.y + agg.y);
             if (tile != 0) __hip_atomic_store(status + tile, kLb2Prefix | iw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (tile == ntiles - 1) {
@@ -1180,19 +1219,18 @@ __global__ __launch_bounds__(kBlock) void k_slots_lb(FlagCtx f, int32_t* __restr
         }
     }
     __syncthreads();
-    int2 carry = s_excl;
+    const int2 excl = s_excl;
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
         const int64_t p = base + k * kBlock + threadIdx.x;
-        const int2 v = make_int2((bits >> (2 * k)) & 1, (bits >> (2 * k + 1)) & 1);
-        int2 tot;
-        const int2 inc = block_inclusive_scan<kWavesPerBlock>(v, Sum2Op(), wt, tot);
+        const uint32_t vi = (bits >> (2 * k)) & 1u, vl = (bits >> (2 * k + 1)) & 1u;
+        const uint64_t mi = ballot64(vi), ml = ballot64(vl);
+        const int2 pre = s_pre[k * kWavesPerBlock + w];
         if (p < f.n) {
-            const int slot = carry.x + inc.x - v.x;
-            if (p < f.n - 1) int_slot[node_of_pos(f, (int)p)] = v.x ? slot : ~slot;
-            leaf_num[p] = carry.y + inc.y - v.y;
+            const int slot = excl.x + pre.x + count_below(mi);
+            if (p < f.n - 1) int_slot[node_of_pos(f, (int)p)] = vi ? slot : ~slot;
+            leaf_num[p] = excl.y + pre.y + count_below(ml);
         }
-        carry = make_int2(carry.x + tot.x, carry.y + tot.y);
     }
 }
 
